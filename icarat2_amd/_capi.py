@@ -1,0 +1,312 @@
+"""ctypes binding of libicarat_hip.so (C ABI of include/icarat_hip.h).
+
+The library is the product: this module only marshals arguments.  Loading
+fails loudly when the built library is missing; there is no CPU fallback."""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libicarat_hip.so")
+
+OK, INVALID_ARGUMENT, HIP_ERROR, NOT_READY, OUT_OF_MEMORY, NOT_CONVERGED, NUMERICAL, INTERNAL = range(8)
+FP64, FP32, BF16 = 0, 1, 2
+PRECOND_JACOBI, PRECOND_LEARNED = 0, 1
+
+dp = C.POINTER(C.c_double)
+i32p = C.POINTER(C.c_int32)
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("lx", C.c_double), ("ly", C.c_double), ("lz", C.c_double),
+                ("young1", C.c_double), ("young2", C.c_double),
+                ("poisson", C.c_double), ("penalty", C.c_double)]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("maxit", C.c_int32), ("precision", C.c_int32),
+                ("precond", C.c_int32), ("inner_tol", C.c_double), ("max_refine", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iters", C.c_int32), ("refine_steps", C.c_int32), ("rec_relres", C.c_double),
+                ("true_relres", C.c_double), ("seconds", C.c_double), ("fallback", C.c_int32),
+                ("status", C.c_int32)]
+
+
+class SimpOpts(C.Structure):
+    _fields_ = [("n_iter", C.c_int32), ("optimizer", C.c_int32), ("use_projection", C.c_int32),
+                ("reserved", C.c_int32), ("xmin", C.c_double), ("xmax", C.c_double),
+                ("solve", SolveOpts)]
+
+
+class SimpStats(C.Structure):
+    _fields_ = [("seconds", C.c_double), ("solve_s", C.c_double), ("filter_s", C.c_double),
+                ("sens_s", C.c_double), ("update_s", C.c_double), ("pcg_iters", C.c_int64),
+                ("max_true_relres", C.c_double)]
+
+
+_SIGS = {
+    "icarat_hip_device_available": (C.c_int, []),
+    "icarat_hip_create": (C.c_int, [C.POINTER(GridDesc), C.POINTER(C.c_void_p)]),
+    "icarat_hip_destroy": (C.c_int, [C.c_void_p]),
+    "icarat_hip_last_error": (C.c_char_p, [C.c_void_p]),
+    "icarat_hip_counts": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                    C.POINTER(C.c_int64), i32p]),
+    "icarat_hip_add_dirichlet": (C.c_int, [C.c_void_p, i32p, dp, dp]),
+    "icarat_hip_make_dof": (C.c_int, [C.c_void_p, i32p]),
+    "icarat_hip_add_neumann": (C.c_int, [C.c_void_p, dp, dp]),
+    "icarat_hip_get_dof": (C.c_int, [C.c_void_p, i32p]),
+    "icarat_hip_get_fext": (C.c_int, [C.c_void_p, dp]),
+    "icarat_hip_set_fext": (C.c_int, [C.c_void_p, dp]),
+    "icarat_hip_get_ke0": (C.c_int, [C.c_void_p, dp]),
+    "icarat_hip_set_density": (C.c_int, [C.c_void_p, dp, C.c_int64]),
+    "icarat_hip_apply_K": (C.c_int, [C.c_void_p, dp, dp, C.c_int32]),
+    "icarat_hip_get_diag": (C.c_int, [C.c_void_p, dp]),
+    "icarat_hip_solve": (C.c_int, [C.c_void_p, C.POINTER(SolveOpts), dp, C.POINTER(SolveStats)]),
+    "icarat_hip_true_residual": (C.c_int, [C.c_void_p, dp, dp]),
+    "icarat_hip_compliance_sensitivity": (C.c_int, [C.c_void_p, C.c_double, dp, dp, dp]),
+    "icarat_hip_filter_setup": (C.c_int, [C.c_void_p, C.c_double]),
+    "icarat_hip_filter": (C.c_int, [C.c_void_p, dp, dp, C.c_int32]),
+    "icarat_hip_threshold": (C.c_int, [C.c_void_p, dp, C.c_double, C.c_double, dp, C.c_int32]),
+    "icarat_hip_oc_update": (C.c_int, [C.c_void_p, dp, dp, C.c_double, dp, C.c_double,
+                                       C.c_double, C.c_int32, C.c_double, i32p]),
+    "icarat_hip_mma_set_params": (C.c_int, [C.c_void_p] + [C.c_double] * 5),
+    "icarat_hip_mma_reset": (C.c_int, [C.c_void_p]),
+    "icarat_hip_mma_update": (C.c_int, [C.c_void_p, dp, dp, C.c_double, dp, C.c_double,
+                                        C.c_double, i32p]),
+    "icarat_hip_simp_run": (C.c_int, [C.c_void_p, C.POINTER(SimpOpts), dp, dp,
+                                      C.POINTER(SimpStats)]),
+    "icarat_hip_time_kernel": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, dp, dp, dp]),
+    "icarat_hip_gaussian_field": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                                            C.c_uint64, C.c_int32, C.c_double, C.c_double,
+                                            C.c_double, C.c_double, C.c_double, dp, dp]),
+}
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run `python -m icarat2_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def exported_symbols():
+    return sorted(_SIGS)
+
+
+class IcaratError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def _p(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(dp)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def device_available():
+    return bool(lib().icarat_hip_device_available())
+
+
+def gaussian_field(nx, ny, nz, seed, nblob=32, smin=4.0, smax=12.0, mean=0.3, lo=1e-3, hi=1.0,
+                   h=1.0):
+    out = np.zeros(nx * ny * nz)
+    m = C.c_double(0.0)
+    rc = lib().icarat_hip_gaussian_field(nx, ny, nz, h, seed, nblob, smin, smax, mean, lo, hi,
+                                         _p(out), C.byref(m))
+    if rc:
+        raise IcaratError(rc, "gaussian_field")
+    return out, m.value
+
+
+class Context:
+    """One voxel hexa8 SIMP problem on the GPU (mesh, BCs, K, filter, OC/MMA)."""
+
+    def __init__(self, nx, ny, nz, lx=None, ly=None, lz=None, young1=1e-9, young2=1.0,
+                 poisson=0.3, penalty=3.0):
+        self.L = lib()
+        d = GridDesc(nx, ny, nz, float(nx if lx is None else lx), float(ny if ly is None else ly),
+                     float(nz if lz is None else lz), young1, young2, poisson, penalty)
+        h = C.c_void_p()
+        rc = self.L.icarat_hip_create(C.byref(d), C.byref(h))
+        if rc:
+            raise IcaratError(rc, "icarat_hip_create failed")
+        self.h = h
+        self.nx, self.ny, self.nz = nx, ny, nz
+        a, b, c, e = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int32()
+        self.L.icarat_hip_counts(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(e))
+        self.numnp, self.nelx, self.neq, self.numeq = a.value, b.value, c.value, e.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.icarat_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc, what):
+        if rc:
+            raise IcaratError(rc, f"{what}: {self.L.icarat_hip_last_error(self.h).decode()}")
+
+    # --- boundary conditions
+    def add_dirichlet(self, flag, val, rng):
+        f = np.array(list(flag) + [0] * (6 - len(flag)), dtype=np.int32)
+        v = np.array(list(val) + [0.0] * (6 - len(val)), dtype=np.float64)
+        r = np.array(rng, dtype=np.float64).reshape(6)
+        self._chk(self.L.icarat_hip_add_dirichlet(self.h, f.ctypes.data_as(i32p), _p(v), _p(r)),
+                  "add_dirichlet")
+
+    def make_dof(self):
+        n = C.c_int32()
+        self._chk(self.L.icarat_hip_make_dof(self.h, C.byref(n)), "make_dof")
+        self.numeq = n.value
+        return n.value
+
+    def add_neumann(self, val, rng):
+        v = _f64(val)
+        r = np.array(rng, dtype=np.float64).reshape(6)
+        self._chk(self.L.icarat_hip_add_neumann(self.h, _p(v), _p(r)), "add_neumann")
+
+    def dof(self):
+        d = np.zeros(self.numnp * 3, dtype=np.int32)
+        self._chk(self.L.icarat_hip_get_dof(self.h, d.ctypes.data_as(i32p)), "get_dof")
+        return d.reshape(-1, 3)
+
+    def fext(self):
+        f = np.zeros(self.numeq)
+        self._chk(self.L.icarat_hip_get_fext(self.h, _p(f)), "get_fext")
+        return f
+
+    def set_fext(self, f):
+        self._chk(self.L.icarat_hip_set_fext(self.h, _p(_f64(f))), "set_fext")
+
+    def ke0(self):
+        k = np.zeros(576)
+        self._chk(self.L.icarat_hip_get_ke0(self.h, _p(k)), "get_ke0")
+        return k.reshape(24, 24)
+
+    # --- analysis
+    def set_density(self, x):
+        x = _f64(x)
+        self._chk(self.L.icarat_hip_set_density(self.h, _p(x), x.size), "set_density")
+
+    def apply_K(self, u, precision=FP64):
+        y = np.zeros(self.numeq)
+        self._chk(self.L.icarat_hip_apply_K(self.h, _p(_f64(u)), _p(y), precision), "apply_K")
+        return y
+
+    def diag(self):
+        d = np.zeros(self.numeq)
+        self._chk(self.L.icarat_hip_get_diag(self.h, _p(d)), "get_diag")
+        return d
+
+    def solve(self, tol=1e-8, maxit=10000, precision=FP64, precond=PRECOND_JACOBI,
+              inner_tol=1e-2, max_refine=60, want_u=True, allow_cap=False):
+        o = SolveOpts(tol, maxit, precision, precond, inner_tol, max_refine, 0)
+        st = SolveStats()
+        u = np.zeros(self.numeq) if want_u else None
+        rc = self.L.icarat_hip_solve(self.h, C.byref(o), None if u is None else _p(u), C.byref(st))
+        if rc and not (allow_cap and rc == NOT_CONVERGED):
+            self._chk(rc, "solve")
+        return u, st
+
+    def true_residual(self, u):
+        r = C.c_double()
+        self._chk(self.L.icarat_hip_true_residual(self.h, _p(_f64(u)), C.byref(r)), "true_residual")
+        return r.value
+
+    def compliance_sensitivity(self, vnorm=1.0):
+        c = C.c_double()
+        df = np.zeros(self.nelx)
+        dg = np.zeros(self.nelx)
+        self._chk(self.L.icarat_hip_compliance_sensitivity(self.h, vnorm, C.byref(c), _p(df), _p(dg)),
+                  "compliance_sensitivity")
+        return c.value, df, dg
+
+    # --- optimization
+    def filter_setup(self, radius):
+        self._chk(self.L.icarat_hip_filter_setup(self.h, radius), "filter_setup")
+
+    def density_filter(self, x, is_sens=False):
+        out = np.zeros(self.nelx)
+        self._chk(self.L.icarat_hip_filter(self.h, _p(_f64(x)), _p(out), 1 if is_sens else 0),
+                  "filter")
+        return out
+
+    def threshold(self, x, beta, T=0.5, deriv=False):
+        out = np.zeros(self.nelx)
+        self._chk(self.L.icarat_hip_threshold(self.h, _p(_f64(x)), beta, T, _p(out),
+                                              1 if deriv else 0), "threshold")
+        return out
+
+    def oc_update(self, x, dfdx, g, dgdx, xmin=1e-4, xmax=1.0, use_projection=False, beta=1.0):
+        x = np.array(x, dtype=np.float64)
+        steps = C.c_int32()
+        self._chk(self.L.icarat_hip_oc_update(self.h, _p(x), _p(_f64(dfdx)), g, _p(_f64(dgdx)),
+                                              xmin, xmax, 1 if use_projection else 0, beta,
+                                              C.byref(steps)), "oc_update")
+        return x, steps.value
+
+    def mma_set_params(self, move=0.15, albefa=0.2, asyminit=0.2, asymdec=0.7, asyminc=1.2):
+        self._chk(self.L.icarat_hip_mma_set_params(self.h, move, albefa, asyminit, asymdec,
+                                                   asyminc), "mma_set_params")
+
+    def mma_reset(self):
+        self._chk(self.L.icarat_hip_mma_reset(self.h), "mma_reset")
+
+    def mma_update(self, x, dfdx, g, dgdx, xmin=1e-4, xmax=1.0):
+        x = np.array(x, dtype=np.float64)
+        outer = C.c_int32()
+        self._chk(self.L.icarat_hip_mma_update(self.h, _p(x), _p(_f64(dfdx)), g, _p(_f64(dgdx)),
+                                               xmin, xmax, C.byref(outer)), "mma_update")
+        return x, outer.value
+
+    def time_kernel(self, kernel, reps=20):
+        ms, fl, by = C.c_double(), C.c_double(), C.c_double()
+        self._chk(self.L.icarat_hip_time_kernel(self.h, kernel, reps, C.byref(ms), C.byref(fl),
+                                                C.byref(by)), "time_kernel")
+        return ms.value, fl.value, by.value
+
+    def simp_run(self, x0, n_iter, optimizer=0, use_projection=False, xmin=1e-4, xmax=1.0,
+                 tol=1e-8, maxit=10000, precision=FP64, precond=PRECOND_JACOBI, inner_tol=1e-2,
+                 max_refine=60):
+        o = SimpOpts(n_iter, optimizer, 1 if use_projection else 0, 0, xmin, xmax,
+                     SolveOpts(tol, maxit, precision, precond, inner_tol, max_refine, 0))
+        x = np.array(x0, dtype=np.float64)
+        hist = np.zeros(4 * max(n_iter, 1))
+        st = SimpStats()
+        self._chk(self.L.icarat_hip_simp_run(self.h, C.byref(o), _p(x), _p(hist), C.byref(st)),
+                  "simp_run")
+        return x, hist[:4 * n_iter].reshape(n_iter, 4), st
+
+
+def cantilever(nx, ny, nz, **kw):
+    """The SIMP configs' cantilever: x = 0 clamped; unit total -z load on the
+    bottom edge (y = 0..ny, z = 0) of the free end x = nx as point loads
+    (-1/(2 ny) at the two ends, -1/ny inside)."""
+    c = Context(nx, ny, nz, **kw)
+    c.add_dirichlet([1, 1, 1, 0, 0, 0], [0.0] * 6, [[0, 0], [0, ny], [0, nz]])
+    c.make_dof()
+    for j in range(ny + 1):
+        v = -1.0 / (2 * ny) if j in (0, ny) else -1.0 / ny
+        c.add_neumann([0.0, 0.0, v], [[nx, nx], [j, j], [0, 0]])
+    return c

@@ -1,0 +1,122 @@
+// context.hpp -- device state of one SIMP problem and the host orchestration
+// of its kernels (solve, filter, OC, MMA, the design loop).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/icarat_hip.h"
+#include "hip_util.hpp"
+#include "kernels.hpp"
+#include "problem.hpp"
+
+namespace icarat_hip {
+
+class MixedSolver;
+class DeviceMMA;
+
+class Context {
+ public:
+  explicit Context(const GridSpec &g);
+  ~Context();
+
+  Problem &problem() { return prob_; }
+  const Lattice &lattice() const { return L_; }
+  hipStream_t stream() const { return stream_; }
+
+  int make_dof();                          // numbering + device upload of masks
+  void sync_fext();                        // host fext -> device lattice vector
+  void mark_fext_dirty() { fext_dirty_ = true; solved_ = false; }
+  void ensure_fext() { if (fext_dirty_) sync_fext(); }
+  void set_density(const double *xphys);   // host -> device densities, young, dinv
+  void set_density_device();               // d_dens already holds the densities
+
+  // y = K u on numeq vectors (host pointers)
+  void apply_k(const double *u, double *y, int precision);
+  void get_diag(double *d);
+  // FP64 / mixed solve of K u = fext; result kept on device (d_u, nodal with
+  // Dirichlet values); optional numeq copy to host.
+  void solve(const icarat_solve_opts &o, double *u_host, icarat_solve_stats *st);
+  double true_residual_device();          // ||f - K d_u|| / ||f||
+  double true_residual_host(const double *u);
+  void compliance_sensitivity(double vnorm, double *c, double *dfds, double *dgds);
+
+  void filter_setup(double radius);
+  void filter_device(const double *in, double *out, int is_sens);  // device pointers
+  void filter_host(const double *in, double *out, int is_sens);
+  void threshold_host(const double *in, double beta, double T, double *out, int deriv);
+  int oc_device(double *x, const double *dc, const double *dv, double g, double xmin, double xmax,
+                int use_projection, double beta);
+  int oc_host(double *x, const double *dfdx, double g, const double *dgdx, double xmin,
+              double xmax, int use_projection, double beta);
+
+  void simp_run(const icarat_simp_opts &o, double *x, double *hist, icarat_simp_stats *st);
+  void time_kernel(int kernel, int reps, double *avg_ms, double *flops, double *bytes);
+
+  // ---- device buffers (public for the solver components) ----
+  Lattice L_;
+  long long n3_ = 0;  // 3 * Np
+  DevBuf<double> d_ke;                // KE0 (576)
+  DevBuf<uint8_t> d_free;             // per node free mask
+  DevBuf<int> d_dof;                  // numnp*3 reference DOF numbers
+  DevBuf<double> d_dirval;            // numnp*3 Dirichlet values (AoS)
+  DevBuf<double> d_dirlat;            // Dirichlet values on the lattice (SoA, 0 at free DOFs)
+  DevBuf<double> d_f;                 // fext on the lattice (0 at constrained DOFs)
+  DevBuf<double> d_b;                 // right-hand side of the current solve
+  DevBuf<double> d_dens, d_young;     // physical densities, SIMP young
+  DevBuf<double> d_dinv;              // 1/diag K at free DOFs
+  DevBuf<double> d_x, d_r, d_q, d_p[2];  // PCG vectors
+  DevBuf<double> d_u;                 // nodal solution (free + Dirichlet values)
+  DevBuf<double> d_nv;                // flushed nodal values for sensitivities
+  DevBuf<double> d_y;                 // scratch lattice vector
+  DevBuf<double> d_numeq[2];          // numeq staging
+  DevBuf<double> d_partials;
+  DevBuf<unsigned> d_counter;
+  DevBuf<PcgScalars> d_pcg;
+  PinnedBuf<PcgScalars> h_pcg;
+  DevBuf<double> d_scal;              // small reduction results
+  PinnedBuf<double> h_scal;
+  DevBuf<double> d_e[8];              // element work vectors
+  DevBuf<OcState> d_oc;
+  PinnedBuf<OcState> h_oc;
+  DevBuf<int> d_flin;
+  DevBuf<double> d_fw;
+  FilterTable ftab_{0, 0, nullptr, nullptr};
+  bool filter_ready_ = false;
+  double radius_ = 0.0;
+  bool density_ready_ = false;
+  bool solved_ = false;
+  bool fext_dirty_ = false;
+  double vol_e_ = 1.0;
+  std::string last_error;
+
+  // PCG iteration batch captured as a graph
+  static constexpr int kBatch = 16;
+  hipGraphExec_t pcg_exec_ = nullptr;
+  hipGraph_t pcg_graph_ = nullptr;
+
+  std::unique_ptr<MixedSolver> mixed_;
+  std::unique_ptr<DeviceMMA> mma_;
+
+  // FP64 Jacobi-PCG on d_b -> d_x; returns iterations, status in S
+  PcgScalars pcg64(double tol, int maxit);
+  void assemble_nodal_solution();         // d_u = d_x + Dirichlet values
+  void sync() { HIPCHK(hipStreamSynchronize(stream_)); }
+  // Device buffers are allocated on first device use, so mesh, numbering,
+  // loads and KE0 work without a GPU.
+  void ensure_device();
+
+ private:
+  Problem prob_;
+  hipStream_t stream_ = nullptr;
+  bool dev_ready_ = false;
+  bool dof_uploaded_ = false;
+  void upload_dof();
+  void require_dof();
+  void require_density();
+  void build_pcg_graph();
+};
+
+}  // namespace icarat_hip

@@ -1,0 +1,117 @@
+// device_common.hpp -- shared device helpers: lattice geometry, the hexa8
+// local-node table, deterministic wave/block reductions and the
+// "last block finalises" pattern used by every reduction in the library.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace icarat_hip {
+
+// Node lattice of an nx*ny*nz voxel mesh.  Node n = i + NX*(j + NY*k) with
+// NX = nx+1 (x fastest), element e = i + nx*(j + ny*k); DOF vectors are
+// stored component-major: v[c*Np + n], Np = numnp padded to 256.
+struct Lattice {
+  int nx, ny, nz;   // elements
+  int NX, NY, NZ;   // nodes
+  long long numnp, nelx, Np;
+};
+
+// Local hexa8 node index of the element corner at offset (ox,oy,oz) in
+// {0,1}^3 from the element's minimum corner: bottom face counter-clockwise
+// 0..3, top face 4..7 (reference io/mesh_icarat.hpp:486-509).
+__host__ __device__ constexpr int lnode(int ox, int oy, int oz) {
+  return (ox == 0 ? (oy == 0 ? 0 : 3) : (oy == 0 ? 1 : 2)) + 4 * oz;
+}
+__host__ __device__ constexpr int lnode_x(int a) { return ((a & 3) == 1 || (a & 3) == 2) ? 1 : 0; }
+__host__ __device__ constexpr int lnode_y(int a) { return ((a & 3) >= 2) ? 1 : 0; }
+__host__ __device__ constexpr int lnode_z(int a) { return a >> 2; }
+
+// Scalars of one device-resident Jacobi-PCG solve.
+struct PcgScalars {
+  double rho;      // r.z of the current iterate
+  double pq;       // p.Ap
+  double rr;       // r.r
+  double bnorm2;   // b.b
+  double tol;      // relative tolerance on ||r|| / ||b||
+  double alpha, beta;
+  int iter, maxit;
+  int done;        // 1 once converged, capped or broken down
+  int status;      // 0 running/converged, 1 capped, 2 breakdown
+};
+
+// Scalars of the device-resident OC bisection.
+struct OcState {
+  double l0, l1, lam, lam_last, g, vol;
+  int counter, done;
+  int maxiter;
+  int pad;
+};
+
+// --- deterministic reductions --------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Sum over a block of NT threads; result valid on every thread.
+template <int NT, int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV], double *sh /* NV*NT/64 */) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double s = wave_sum(v[k]);
+    if (lane == 0) sh[k * (NT / 64) + w] = s;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < NT / 64; ++i) s += sh[k * (NT / 64) + i];
+    v[k] = s;
+  }
+  __syncthreads();
+}
+
+// Every block publishes its partial sums; the last block to finish sums the
+// partials in block order (deterministic for a fixed grid) and returns true
+// with the totals in tot[] on all its threads.  The counter is re-armed for
+// the next launch.
+template <int NT, int NV>
+__device__ bool last_block_sum(double (&v)[NV], double *partials, unsigned *counter,
+                               double (&tot)[NV], double *sh) {
+  __shared__ int s_last;
+  const unsigned G = gridDim.x * gridDim.y * gridDim.z;
+  const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  block_sum<NT, NV>(v, sh);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+      __hip_atomic_store(&partials[(size_t)k * G + bid], v[k], __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    unsigned prev = atomicAdd(counter, 1u);
+    s_last = (prev == G - 1);
+  }
+  __syncthreads();
+  if (!s_last) return false;
+  __threadfence();
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double s = 0.0;
+    for (unsigned i = threadIdx.x; i < G; i += NT)
+      s += __hip_atomic_load(&partials[(size_t)k * G + i], __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+    acc[k] = s;
+  }
+  block_sum<NT, NV>(acc, sh);
+#pragma unroll
+  for (int k = 0; k < NV; ++k) tot[k] = acc[k];
+  if (threadIdx.x == 0) *counter = 0u;
+  return true;
+}
+
+}  // namespace icarat_hip

@@ -1,0 +1,86 @@
+// kernels.hpp -- host-side launchers of the HIP kernels (one declaration per
+// kernel family; the definitions live next to their kernels in k_*.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+#include "hip_util.hpp"
+
+namespace icarat_hip {
+
+// ---- k_fp64.hip ----------------------------------------------------------
+// `ke` below is the 576-entry KE0 in device memory (row-major 24x24).
+// young[e] = SIMP(x[e]) (reference problem/elasticity.hpp:13-17)
+void launch_young(const double *x, double *young, long long nelx, double y1, double y2, double p,
+                  hipStream_t s);
+// dinv[c*Np+n] = 1 / diag K at free DOFs, 0 elsewhere
+void launch_diag_inv(const Lattice &L, const double *ke, const double *young,
+                     const uint8_t *freemask, double *dinv, double *diag, hipStream_t s);
+// y = K u on the lattice; y masked to free DOFs.  u is used as given
+// (Dirichlet values included).
+void launch_apply_k64(const Lattice &L, const double *ke, const double *young,
+                      const uint8_t *freemask, const double *u, double *y, hipStream_t s);
+// One fused Jacobi-PCG iteration, stencil half: p_new = dinv*r + beta*p_old;
+// q = K p_new; p.q reduced; alpha = rho / p.q.
+void launch_pcg_stencil64(const Lattice &L, const double *ke, const double *young,
+                          const uint8_t *freemask, const double *r, const double *dinv,
+                          const double *p_old, double *p_new, double *q, PcgScalars *S,
+                          double *partials, unsigned *counter, hipStream_t s);
+// Vector half: x += alpha p; r -= alpha q; rho' = r.dinv.r; r.r; beta; stop test.
+void launch_pcg_update64(long long n, double *x, double *r, const double *p, const double *q,
+                         const double *dinv, PcgScalars *S, double *partials, unsigned *counter,
+                         hipStream_t s);
+// x = 0, r = b, p = 0, rho = b.dinv.b, bnorm2 = b.b
+void launch_pcg_init64(long long n, const double *b, double *x, double *r, double *p0,
+                       const double *dinv, PcgScalars *S, double *partials, unsigned *counter,
+                       hipStream_t s);
+// res = b - y on free DOFs; sums ||res||^2 and ||b||^2 into out2[0..1]
+void launch_residual_norms(long long n, const double *b, const double *y, double *out2,
+                           double *partials, unsigned *counter, hipStream_t s);
+// out = a . b  (length n)
+void launch_dot(long long n, const double *a, const double *b, double *out, double *partials,
+                unsigned *counter, hipStream_t s);
+// nodal values with |v| < 1e-15 flushed (reference analysis/analysis_utility.hpp:26-28)
+void launch_flush_small(long long n, const double *u, double *v, hipStream_t s);
+// dfds[e] = -(E'/E) u_e^T (E KE0) u_e, dgds[e] = vol_e / vnorm
+void launch_sensitivity64(const Lattice &L, const double *ke, const double *dens,
+                          const double *nodeval, double y1, double y2, double p, double vol_e,
+                          double vnorm, double *dfds, double *dgds, hipStream_t s);
+// lattice <-> reference numeq ordering (dof[3*n+c] is the reference DOF number)
+void launch_numeq_to_lattice(const Lattice &L, const int *dof, int numeq, const double *v,
+                             const double *dirval, double *lat, hipStream_t s);
+void launch_lattice_to_numeq(const Lattice &L, const int *dof, int numeq, const double *lat,
+                             double *v, hipStream_t s);
+void launch_axpby(long long n, double a, const double *x, double b, double *y, hipStream_t s);
+
+// ---- k_filter.hip ---------------------------------------------------------
+// Filter neighbourhood in device memory: lin[t] is the LDS offset of entry t
+// for a halo of width R, w[t] = rmin - distance.
+struct FilterTable {
+  int count;          // number of offsets
+  int R;              // halo width in elements (1..3)
+  const int *lin;     // device
+  const double *w;    // device
+};
+// LDS linear offsets of (dx,dy,dz) for the tile shape used with halo R
+void filter_lds_offsets(const int *off, int count, int R, int *lin);
+// is_sens == 0: out = (sum w x)/(sum w), aa = sum w
+// is_sens == 1: out = sum w v / aa_j
+void launch_density_filter(const Lattice &L, const FilterTable &T, const double *in,
+                           double *aa, double *out, int is_sens, hipStream_t s);
+void launch_threshold(long long n, const double *x, double beta, double T, double *out,
+                      int deriv, hipStream_t s);
+void launch_mul(long long n, const double *a, double *b, hipStream_t s);
+// One OC bisection step (x' at lambda, filter, optional projection, volume
+// sum, bisection update of the device state).
+void launch_oc_step(const Lattice &L, const FilterTable &T, const double *x, const double *dc,
+                    const double *dv, double xmin, double xmax,
+                    int use_projection, double beta, double vol_e, OcState *S,
+                    double *partials, unsigned *counter, hipStream_t s);
+void launch_oc_final(long long n, double *x, const double *dc, const double *dv, double xmin,
+                     double xmax, const OcState *S, hipStream_t s);
+// sum of x (or of the projection of x) times vol_e, into *out
+void launch_volume(long long n, const double *x, double vol_e, double *out, double *partials,
+                   unsigned *counter, hipStream_t s);
+
+}  // namespace icarat_hip

@@ -1,0 +1,33 @@
+// mma.hpp -- MMA design update with one (volume) constraint on the device:
+// element passes on the GPU, the 1x1 dual Newton system on the host, all FP64.
+#pragma once
+#include "hip_util.hpp"
+#include "mma_kernels.hpp"
+
+namespace icarat_hip {
+
+class Context;
+
+class DeviceMMA {
+ public:
+  explicit DeviceMMA(Context &ctx);
+  void reset();  // iteration counter and history back to the start
+  void set_params(double move, double albefa, double asyminit, double asymdec, double asyminc);
+  // MMASolver::Update for m = 1 (reference optimization/opt-solver/MMASolver.cpp:66-82),
+  // a = 0, c = 1e5, d = 0.  x, dfdx, dgdx are device pointers; x updated in place.
+  // Returns the number of outer interior-point loops.
+  int update(double *x, const double *dfdx, double g, const double *dgdx, double xmin,
+             double xmax);
+  int last_newton_steps() const { return newton_steps_; }
+
+ private:
+  Context &ctx_;
+  long long n_;
+  DevBuf<double> buf_[10];
+  MmaArrays A_;
+  MmaParams P_;
+  double a_ = 0.0, c_ = 1.0e5, d_ = 0.0;
+  int newton_steps_ = 0;
+};
+
+}  // namespace icarat_hip

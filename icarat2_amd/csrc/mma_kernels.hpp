@@ -1,0 +1,31 @@
+// mma_kernels.hpp -- launchers of the element-parallel MMA passes (k_mma.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+#include "hip_util.hpp"
+
+namespace icarat_hip {
+
+// Parameters of MMASolver with the reference defaults
+// (optimization/opt-solver/MMASolver.cpp:41-56).
+struct MmaParams {
+  int iter;
+  double xmin, xmax;
+  double xmamieps = 1.0e-5, raa0 = 1.0e-5;
+  double move = 0.15, albefa = 0.2, asyminit = 0.2, asymdec = 0.7, asyminc = 1.2;
+};
+
+struct MmaArrays {
+  double *low, *upp, *alpha, *beta, *p0, *q0, *p1, *q1, *xold1, *xold2;
+};
+
+// Builds the subproblem at x; out[0] = sum p1/(U-x) + q1/(x-L).
+void launch_mma_gensub(long long n, const double *x, const double *dfdx, const double *dgdx,
+                       const MmaParams &P, const MmaArrays &A, double *partials,
+                       unsigned *counter, double *out, hipStream_t s);
+// x = x(lambda); out[0] = Sg, out[1] = Sh at that x.
+void launch_mma_dual(long long n, double lam, const MmaArrays &A, double *x, double *partials,
+                     unsigned *counter, double *out, hipStream_t s);
+
+}  // namespace icarat_hip

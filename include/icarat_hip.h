@@ -1,0 +1,215 @@
+/* icarat_hip.h -- C ABI of the MI355X (gfx950) implementation of ICARAT's
+ * linear-elastic SIMP compliance loop on a voxel hexa8 mesh.
+ *
+ * Conventions
+ *  - Every entry point returns an int status (ICARAT_HIP_OK == 0).  On a
+ *    non-zero status the text of the error is kept in the context and
+ *    icarat_hip_last_error() returns it.  No entry point aborts the process.
+ *  - Host pointers are borrowed for the duration of the call only.  Device
+ *    memory is owned by the context.
+ *  - Vectors on DOFs are in the reference's "numeq" order (free DOFs numbered
+ *    in node order, reference io/boundary_condition_base.hpp:35-95) and
+ *    element vectors in the reference's element order (x fastest, reference
+ *    io/mesh_icarat.hpp:480-511).
+ *  - A context is single-threaded: calls on one context must not overlap.
+ *    Distinct contexts are independent.
+ */
+#ifndef ICARAT_HIP_H
+#define ICARAT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  ICARAT_HIP_OK = 0,
+  ICARAT_HIP_INVALID_ARGUMENT = 1,
+  ICARAT_HIP_HIP_ERROR = 2,
+  ICARAT_HIP_NOT_READY = 3,
+  ICARAT_HIP_OUT_OF_MEMORY = 4,
+  ICARAT_HIP_NOT_CONVERGED = 5,
+  ICARAT_HIP_NUMERICAL = 6,
+  ICARAT_HIP_INTERNAL = 7
+};
+
+/* Precision of the element contraction used inside a solve. */
+enum {
+  ICARAT_HIP_FP64 = 0,         /* plain FP64 throughout */
+  ICARAT_HIP_FP32 = 1,         /* FP32 MFMA contraction inside FP64 refinement */
+  ICARAT_HIP_BF16 = 2          /* BF16 MFMA contraction (FP32 accumulate) inside FP64 refinement */
+};
+
+/* Preconditioner of the (inner) Krylov solve. */
+enum {
+  ICARAT_HIP_PRECOND_JACOBI = 0,   /* reference "Scaling_CG" (analysis/solver/CG.cpp:56-100) */
+  ICARAT_HIP_PRECOND_LEARNED = 1   /* U-Net correction in flexible PCG, Jacobi fallback */
+};
+
+typedef struct icarat_hip_ctx icarat_hip_ctx;
+
+/* Box mesh + material.  Replaces MeshIcarat::setParameter/meshBox for hexa8
+ * (io/mesh_icarat.hpp:72-139, 459-513) and Elastic::setParameter
+ * (problem/elasticity.cpp:19-33): young1/young2 are the reference's
+ * young = [E1, E2] (E(x) = (1 - x^p) E1 + x^p E2, problem/elasticity.hpp:13-17). */
+typedef struct icarat_grid_desc {
+  int32_t nx, ny, nz;     /* [mesh] division */
+  double lx, ly, lz;      /* [mesh] length */
+  double young1, young2;  /* [material] young */
+  double poisson;         /* [material] poisson */
+  double penalty;         /* [material] penalty */
+} icarat_grid_desc;
+
+typedef struct icarat_solve_opts {
+  double tol;             /* relative residual ||f - K u|| / ||f|| to reach */
+  int32_t maxit;          /* iteration cap of the (inner) Krylov method */
+  int32_t precision;      /* ICARAT_HIP_FP64 / FP32 / BF16 */
+  int32_t precond;        /* ICARAT_HIP_PRECOND_* */
+  double inner_tol;       /* mixed precision: relative reduction per refinement step */
+  int32_t max_refine;     /* mixed precision: cap on refinement steps */
+  int32_t reserved;
+} icarat_solve_opts;
+
+typedef struct icarat_solve_stats {
+  int32_t iters;          /* Krylov iterations (all refinement steps summed) */
+  int32_t refine_steps;   /* FP64 refinement steps (1 for FP64) */
+  double rec_relres;      /* recursive (Krylov) relative residual at exit */
+  double true_relres;     /* FP64 ||f - K u|| / ||f|| recomputed by the FP64 kernel */
+  double seconds;         /* device wall time of the solve */
+  int32_t fallback;       /* 1 if the learned preconditioner fell back to Jacobi */
+  int32_t status;         /* ICARAT_HIP_OK or ICARAT_HIP_NOT_CONVERGED */
+} icarat_solve_stats;
+
+/* ---- context ---------------------------------------------------------- */
+int icarat_hip_create(const icarat_grid_desc *desc, icarat_hip_ctx **out);
+int icarat_hip_destroy(icarat_hip_ctx *ctx);
+const char *icarat_hip_last_error(const icarat_hip_ctx *ctx);
+/* numnp, nelx, neq = 3*numnp (io/mesh_icarat.hpp:106-111), numeq (-1 before make_dof) */
+int icarat_hip_counts(const icarat_hip_ctx *ctx, int64_t *numnp, int64_t *nelx, int64_t *neq,
+                      int32_t *numeq);
+/* 1 if a HIP device is usable from this process, else 0 */
+int icarat_hip_device_available(void);
+
+/* ---- boundary conditions (BCIcarat) -------------------------------------- */
+/* Dirichlet box: flag[6]/val[6]/range {x0,x1,y0,y1,z0,z1}, tolerance 1e-7
+ * (io/boundary_condition_icarat.hpp:72-138, io/boundary_condition_base.hpp:19-33). */
+int icarat_hip_add_dirichlet(icarat_hip_ctx *ctx, const int32_t flag[6], const double val[6],
+                             const double range[6]);
+/* DOF numbering (BCIcarat::makeDOF -> countDOF, io/boundary_condition_icarat.hpp:140-148). */
+int icarat_hip_make_dof(icarat_hip_ctx *ctx, int32_t *numeq);
+/* Neumann box: point load if degenerate, else a face load integrated on every
+ * element face with 4 nodes in the box (BCIcarat::input_neumann /
+ * generateLoadMain, io/boundary_condition_icarat.hpp:212-410). */
+int icarat_hip_add_neumann(icarat_hip_ctx *ctx, const double val[3], const double range[6]);
+/* dof[3*node + c] = reference DOF number (Node::dof, problem/base.hpp:86) */
+int icarat_hip_get_dof(const icarat_hip_ctx *ctx, int32_t *dof);
+/* Force::fext_org (problem/base.hpp:131), numeq entries */
+int icarat_hip_get_fext(const icarat_hip_ctx *ctx, double *f);
+int icarat_hip_set_fext(icarat_hip_ctx *ctx, const double *f);
+/* Shared element stiffness at young = 1 (24x24, row-major, DOF order 3a+c):
+ * Ke of LinearElastic::makeKeFint (problem/elasticity.cpp:211-255) is young(x) * KE0. */
+int icarat_hip_get_ke0(const icarat_hip_ctx *ctx, double *ke);
+
+/* ---- analysis (LinearAnalysis / Solvers) --------------------------------- */
+/* Physical densities (Elastic::setDesignS for every element), nelx entries. */
+int icarat_hip_set_density(icarat_hip_ctx *ctx, const double *xphys, int64_t n);
+/* y = K(x) u on free DOFs (the assembled globalK of LinearAnalysis::assemblingK,
+ * analysis/linear.hpp:117-166, applied matrix-free).  precision selects the
+ * contraction kernel (ICARAT_HIP_FP64 / FP32 / BF16). */
+int icarat_hip_apply_K(icarat_hip_ctx *ctx, const double *u, double *y, int32_t precision);
+/* diag(K) on free DOFs (DiagonalPreconditioner, analysis/solver/CG.cpp:59) */
+int icarat_hip_get_diag(icarat_hip_ctx *ctx, double *d);
+/* Equilibrium solve K u = fext with x0 = 0 (LinearAnalysis::solve +
+ * Solvers::ScalingCG, analysis/linear.hpp:55-104, analysis/solver/CG.cpp:56-100).
+ * u: numeq solution.  Returns ICARAT_HIP_NOT_CONVERGED when the cap is hit
+ * (u then holds the last iterate). */
+int icarat_hip_solve(icarat_hip_ctx *ctx, const icarat_solve_opts *opts, double *u,
+                     icarat_solve_stats *stats);
+/* ||fext - K u|| / ||fext|| in FP64 (analysis/linear.hpp:93-103). */
+int icarat_hip_true_residual(icarat_hip_ctx *ctx, const double *u, double *relres);
+/* Compliance c = u . fext of the last solve (opt_macro_oc.cpp:110) and the
+ * element sensitivities of the driver's sensitivity() (opt_macro_oc.cpp:182-210):
+ * dfds = -(E'/E) u_e^T Ke u_e, dgds = volume / vnorm.  Either output may be NULL. */
+int icarat_hip_compliance_sensitivity(icarat_hip_ctx *ctx, double vnorm, double *c, double *dfds,
+                                      double *dgds);
+
+/* ---- optimization (Filter / OC / MMASolver) ------------------------------ */
+/* Filter<E,N>(fem, element, node, radius) (optimization/filter.hpp:63-70). */
+int icarat_hip_filter_setup(icarat_hip_ctx *ctx, double radius);
+/* Filter::densityFilter(in, is_sens) (optimization/filter.hpp:106-155). */
+int icarat_hip_filter(icarat_hip_ctx *ctx, const double *in, double *out, int32_t is_sens);
+/* Filter::threshold (deriv = 0) / thresholdDerivative (deriv = 1)
+ * (optimization/filter.hpp:216-248). */
+int icarat_hip_threshold(icarat_hip_ctx *ctx, const double *in, double beta, double T,
+                         double *out, int32_t deriv);
+/* OC::UpdateVariables with the drivers' volume callback (filter, optional
+ * projection, sum * volume) (optimization/opt-solver/OCSolver.hpp:20-79,
+ * opt_macro_oc.cpp:153-164).  x updated in place; *steps = bisection steps. */
+int icarat_hip_oc_update(icarat_hip_ctx *ctx, double *x, const double *dfdx, double g,
+                         const double *dgdx, double xmin, double xmax, int32_t use_projection,
+                         double beta, int32_t *steps);
+
+/* MMASolver(n = nelx, m = 1) (optimization/opt-solver/MMASolver.hpp:62-116) on the
+ * device: SetMoveLimit / SetOscillation / SetAsymptotes
+ * (MMASolver.hpp:72-75) and Reset. */
+int icarat_hip_mma_set_params(icarat_hip_ctx *ctx, double move, double albefa, double asyminit,
+                              double asymdec, double asyminc);
+int icarat_hip_mma_reset(icarat_hip_ctx *ctx);
+/* MMASolver::Update(x, dfdx, &g, dgdx, xmin, xmax) for the single volume
+ * constraint (optimization/opt-solver/MMASolver.cpp:66-82); x updated in place;
+ * *outer = interior-point outer loops (may be NULL). */
+int icarat_hip_mma_update(icarat_hip_ctx *ctx, double *x, const double *dfdx, double g,
+                          const double *dgdx, double xmin, double xmax, int32_t *outer);
+
+/* ---- SIMP driver loop ------------------------------------------------------ */
+typedef struct icarat_simp_opts {
+  int32_t n_iter;          /* design iterations */
+  int32_t optimizer;       /* 0 = OC, 1 = MMA (m = 1 volume constraint) */
+  int32_t use_projection;  /* 0 = plain method (no tanh projection) */
+  int32_t reserved;
+  double xmin, xmax;       /* Optimize(nelx, 1, xmin, xmax) (optimization/opt_base.hpp:20-24) */
+  icarat_solve_opts solve;
+} icarat_simp_opts;
+
+typedef struct icarat_simp_stats {
+  double seconds;          /* wall time of the whole loop */
+  double solve_s, filter_s, sens_s, update_s;
+  int64_t pcg_iters;       /* Krylov iterations, all design iterations */
+  double max_true_relres;  /* largest FP64 true residual over the loop */
+} icarat_simp_stats;
+
+/* The reference drivers' loop (sample/algorithm/opt-macro-oc/opt_macro_oc.cpp:91-178,
+ * opt-macro-mma-nondesign/opt_macro_mma_nondesign.cpp:90-178) with the cone
+ * filter set up by icarat_hip_filter_setup.  x: design variables in/out
+ * (nelx).  hist (may be NULL): per iteration {compliance, volume, iterations,
+ * true relative residual}. */
+int icarat_hip_simp_run(icarat_hip_ctx *ctx, const icarat_simp_opts *opts, double *x,
+                        double *hist, icarat_simp_stats *stats);
+
+/* ---- measurement ------------------------------------------------------------ */
+/* No reference counterpart: a timing hook for benchmarks.  Launches one kernel
+ * `reps` times back to back on the context's stream between two HIP events
+ * and returns the average duration and the algorithmic work of one launch
+ * (flops and compulsory HBM bytes as documented for each kernel):
+ *   kernel 0: FP64 PCG stencil half (p update + K.p + p.q), 1152 flop/element
+ *   kernel 1: FP64 PCG vector half (x, r update, r.z, r.r)
+ *   kernel 2: density filter (isSens = false) at the set-up radius
+ * Needs set_density (and filter_setup for kernel 2).  Overwrites the solver's
+ * work vectors. */
+int icarat_hip_time_kernel(icarat_hip_ctx *ctx, int32_t kernel, int32_t reps, double *avg_ms,
+                           double *flops, double *bytes);
+
+/* ---- inputs ---------------------------------------------------------------- */
+/* Seeded smooth density field (sum of nblob isotropic Gaussians, amplitude
+ * U[0.5,1.5], centre U(domain), sigma U[smin,smax], rescaled to `mean`,
+ * clipped to [lo,hi]).  Host only; returns the mean after clipping in *mean_out.
+ * No reference counterpart: the input generator of the benchmark configs. */
+int icarat_hip_gaussian_field(int32_t nx, int32_t ny, int32_t nz, double h, uint64_t seed,
+                              int32_t nblob, double smin, double smax, double mean, double lo,
+                              double hi, double *out, double *mean_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ICARAT_HIP_H */

@@ -1,0 +1,122 @@
+"""The product's host logic and C ABI without a GPU: the library loads, exports
+every declared entry point, numbers DOFs and builds loads and KE0 exactly as
+the oracle does, and reports errors as status codes."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for f in os.listdir(os.path.join(ROOT, "include")):
+        if f.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", f)).read()
+            names |= set(re.findall(r"\b(icarat_hip_\w+)\s*\(", txt))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol(capi):
+    L = capi.lib()
+    decl = declared_symbols()
+    assert len(decl) >= 25
+    for name in decl:
+        assert hasattr(L, name), name
+    # the binding covers the header
+    assert set(decl) == set(capi.exported_symbols())
+
+
+def test_library_is_gfx950_code_object():
+    import subprocess
+    path = os.path.join(ROOT, "icarat2_amd", "libicarat_hip.so")
+    out = subprocess.run(["/opt/rocm/llvm/bin/llvm-readobj", "--sections", path],
+                         capture_output=True, text=True).stdout
+    assert ".hip_fatbin" in out
+    blob = open(path, "rb").read()
+    assert b"gfx950" in blob
+
+
+@pytest.mark.parametrize("dims", [(60, 20, 20), (7, 5, 3)])
+def test_dof_and_loads_match_oracle(capi, oracle, dims):
+    nx, ny, nz = dims
+    c = capi.cantilever(nx, ny, nz)
+    o = oracle.cantilever(nx, ny, nz)
+    assert (c.numnp, c.nelx, c.numeq) == (o.numnp, o.nelx, o.numeq)
+    assert np.array_equal(c.dof(), o.dof())
+    assert np.array_equal(c.fext(), o.fext())  # point loads: bit-exact
+
+
+def test_partial_dirichlet_and_face_loads_match_oracle(capi, oracle):
+    args_d = [([1, 0, 1], [0.0, 0.0, 0.0], [[0, 0], [0, 3], [0, 2]]),
+              ([0, 1, 0], [0.0, 0.5, 0.0], [[4, 4], [0, 0], [0, 2]])]
+    args_n = [([0.0, 0.0, -2.0], [[4, 4], [0, 3], [0, 2]]),
+              ([1.0, 0.5, 0.0], [[0, 4], [0, 3], [2, 2]]),
+              ([0.0, 3.0, 0.0], [[0, 4], [3, 3], [0, 2]]),
+              ([0.0, 0.0, 0.7], [[2, 2], [1, 1], [1, 1]])]
+    c = capi.Context(4, 3, 2)
+    o = oracle.Problem(4, 3, 2)
+    for a in args_d:
+        c.add_dirichlet(*a)
+        o.add_dirichlet(*a)
+    assert c.make_dof() == o.make_dof()
+    for a in args_n:
+        c.add_neumann(*a)
+        o.add_neumann(*a)
+    assert np.array_equal(c.dof(), o.dof())
+    assert np.max(np.abs(c.fext() - o.fext())) < 1e-15
+
+
+def test_ke0_matches_oracle(capi, oracle):
+    for nu in (0.3, 0.25, 0.45):
+        c = capi.Context(2, 2, 2, poisson=nu)
+        o = oracle.Problem(2, 2, 2, poisson=nu)
+        assert np.max(np.abs(c.ke0() - o.ke0())) < 1e-15
+    # non-unit cubic voxels scale KE by h
+    c = capi.Context(4, 4, 4, 2.0, 2.0, 2.0)
+    o = oracle.Problem(4, 4, 4, 2.0, 2.0, 2.0)
+    assert np.max(np.abs(c.ke0() - o.ke0())) < 1e-15
+
+
+def test_gaussian_field_matches_oracle(capi, oracle):
+    for dims, seed in [((24, 12, 10), 2), ((16, 8, 8), 4)]:
+        a, ma = capi.gaussian_field(*dims, seed)
+        b, mb = oracle.gaussian_field(*dims, seed)
+        assert np.array_equal(a, b)
+        assert ma == mb
+
+
+def test_errors_are_status_codes(capi):
+    c = capi.Context(4, 2, 2)
+    # loads before numbering
+    with pytest.raises(capi.IcaratError) as e:
+        c.add_neumann([0, 0, -1.0], [[4, 4], [0, 0], [0, 0]])
+    assert e.value.code == capi.NOT_READY
+    c.add_dirichlet([1, 1, 1], [0, 0, 0], [[0, 0], [0, 2], [0, 2]])
+    c.make_dof()
+    # density of the wrong length
+    with pytest.raises(capi.IcaratError) as e:
+        c.set_density(np.ones(3))
+    assert e.value.code == capi.INVALID_ARGUMENT
+    # null pointers through the raw ABI
+    L = capi.lib()
+    assert L.icarat_hip_apply_K(c.h, None, None, 0) == capi.INVALID_ARGUMENT
+    assert L.icarat_hip_destroy(None) == capi.INVALID_ARGUMENT
+    # a bad grid
+    import ctypes as C
+    d = capi.GridDesc(0, 2, 2, 1.0, 1.0, 1.0, 1e-9, 1.0, 0.3, 3.0)
+    h = C.c_void_p()
+    assert L.icarat_hip_create(C.byref(d), C.byref(h)) == capi.INVALID_ARGUMENT
+    d = capi.GridDesc(2, 2, 2, 2.0, 2.0, 2.0, 1.0, 1e-9, 0.3, 3.0)  # young2 < young1
+    assert L.icarat_hip_create(C.byref(d), C.byref(h)) == capi.INVALID_ARGUMENT
+
+
+def test_device_calls_without_gpu_fail_loudly(capi):
+    if capi.device_available():
+        pytest.skip("a GPU is present")
+    c = capi.cantilever(4, 2, 2)
+    with pytest.raises(capi.IcaratError) as e:
+        c.set_density(np.full(c.nelx, 0.5))
+    assert e.value.code == capi.HIP_ERROR

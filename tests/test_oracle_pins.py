@@ -1,0 +1,436 @@
+"""Pins the CPU oracle against the reference's own known-answer tests, closed
+forms, invariants and independent formulations (no GPU)."""
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+
+# ---------------------------------------------------------------- mesh / DOF
+def test_counts_config1(oracle):
+    pr = oracle.cantilever(60, 20, 20)
+    # io/mesh_icarat.hpp:106-111 closed forms
+    assert pr.numnp == 61 * 21 * 21 == 26901
+    assert pr.nelx == 60 * 20 * 20 == 24000
+    assert pr.neq == 3 * 26901 == 80703
+    assert pr.numeq == 80703 - 3 * 21 * 21 == 79380
+
+
+def test_node_and_element_numbering(oracle):
+    nx, ny, nz = 3, 2, 2
+    pr = oracle.Problem(nx, ny, nz, 1.5, 2.0, 1.0)
+    X = pr.coords()
+    ids = np.arange(pr.numnp)
+    k, j, i = ids % (nx + 1), (ids // (nx + 1)) % (ny + 1), ids // ((nx + 1) * (ny + 1))
+    assert np.array_equal(X[:, 0], (1.5 / nx) * k)
+    assert np.array_equal(X[:, 1], (2.0 / ny) * j)
+    assert np.array_equal(X[:, 2], (1.0 / nz) * i)
+    conn = pr.connectivity()
+    e = np.arange(pr.nelx)
+    ex, ey, ez = e % nx, (e // nx) % ny, e // (nx * ny)
+    base = ex + (nx + 1) * ey + (nx + 1) * (ny + 1) * ez
+    sx, sxy = nx + 1, (nx + 1) * (ny + 1)
+    expect = np.stack([base, base + 1, base + 1 + sx, base + sx, base + sxy, base + 1 + sxy,
+                       base + 1 + sx + sxy, base + sx + sxy], axis=1)
+    assert np.array_equal(conn, expect)
+
+
+def test_dof_numbering_clamped_face(oracle):
+    nx, ny, nz = 5, 3, 2
+    pr = oracle.cantilever(nx, ny, nz)
+    dof = pr.dof()
+    n = np.arange(pr.numnp)
+    clamped = (n % (nx + 1)) == 0
+    nclamped_before = np.cumsum(clamped) - clamped
+    free = ~clamped
+    # free DOFs: 3*(n - #clamped nodes before n) + d
+    for d in range(3):
+        assert np.array_equal(dof[free, d], 3 * (n[free] - nclamped_before[free]) + d)
+    # constrained DOFs follow in node order
+    numeq = pr.numeq
+    assert np.array_equal(dof[clamped].ravel(), numeq + np.arange(3 * clamped.sum()))
+
+
+def test_dof_numbering_partial_flags_bruteforce(oracle):
+    """Partial Dirichlet flags: free first in (node, comp) order, then the
+    constrained in (node, comp) order (io/boundary_condition_base.hpp:35-95)."""
+    pr = oracle.Problem(4, 3, 2)
+    pr.add_dirichlet([1, 0, 1], [0.0, 0.0, 0.0], [[0, 0], [0, 3], [0, 2]])
+    pr.add_dirichlet([0, 1, 0], [0.0, 0.5, 0.0], [[4, 4], [0, 0], [0, 2]])
+    pr.add_dirichlet([0, 0, 0], [0.0, 0.0, 0.0], [[2, 2], [1, 1], [1, 1]])  # flags 0: all free
+    pr.make_dof()
+    X = pr.coords()
+    flags = np.zeros((pr.numnp, 3), dtype=int)
+    on0 = np.isclose(X[:, 0], 0)
+    flags[on0, 0] = 1
+    flags[on0, 2] = 1
+    on4 = np.isclose(X[:, 0], 4) & np.isclose(X[:, 1], 0)
+    flags[on4, 1] = 1
+    expect = -np.ones((pr.numnp, 3), dtype=int)
+    q = 0
+    for nd in range(pr.numnp):
+        for c in range(3):
+            if not flags[nd, c]:
+                expect[nd, c] = q
+                q += 1
+    numeq = q
+    for nd in range(pr.numnp):
+        for c in range(3):
+            if flags[nd, c]:
+                expect[nd, c] = q
+                q += 1
+    assert pr.numeq == numeq
+    assert np.array_equal(pr.dof(), expect)
+    dv = pr.dirichlet_values().reshape(-1, 3)
+    assert np.all(dv[on4, 1] == 0.5)
+
+
+# ------------------------------------------------------------------- loads
+def test_point_loads_config1(oracle):
+    pr = oracle.cantilever(60, 20, 20)
+    f = pr.fext()
+    dof = pr.dof()
+    nz_idx = np.nonzero(f)[0]
+    assert len(nz_idx) == 21
+    assert abs(f.sum() + 1.0) < 1e-15
+    for j in range(21):
+        n = 60 + 61 * j
+        expect = -1 / 40 if j in (0, 20) else -1 / 20
+        assert f[dof[n, 2]] == expect  # bit-exact
+        assert f[dof[n, 0]] == 0.0 and f[dof[n, 1]] == 0.0
+
+
+def test_face_load_unit_faces(oracle):
+    """Consistent nodal loads of a unit-voxel face load: val/4 per node per
+    face (corners 1/4, edges 1/2, interior 1 per unit traction)."""
+    nx, ny, nz = 3, 2, 2
+    pr = oracle.Problem(nx, ny, nz)
+    pr.add_dirichlet([1, 1, 1], [0, 0, 0], [[0, 0], [0, ny], [0, nz]])
+    pr.make_dof()
+    pr.add_neumann([0.0, 0.0, -2.0], [[nx, nx], [0, ny], [0, nz]])  # the x = nx face
+    f = pr.fext()
+    dof = pr.dof()
+    assert abs(f.sum() + 2.0 * ny * nz) < 1e-13
+    for j in range(ny + 1):
+        for k in range(nz + 1):
+            n = nx + (nx + 1) * (j + (ny + 1) * k)
+            nfaces = (1 + (0 < j < ny)) * (1 + (0 < k < nz))
+            assert abs(f[dof[n, 2]] - (-2.0) * nfaces / 4) < 1e-15
+    # the other faces through the same element sides
+    for side, rng, val in [("zmin", [[0, nx], [0, ny], [0, 0]], [1.0, 0, 0]),
+                           ("ymax", [[0, nx], [ny, ny], [0, nz]], [0, 3.0, 0]),
+                           ("zmax", [[0, nx], [0, ny], [nz, nz]], [0, 0, 1.0])]:
+        p2 = oracle.Problem(nx, ny, nz)
+        p2.make_dof()
+        p2.add_neumann(val, rng)
+        f2 = p2.fext()
+        area = {"zmin": nx * ny, "ymax": nx * nz, "zmax": nx * ny}[side]
+        assert abs(f2.sum() - sum(val) * area) < 1e-12, side
+
+
+# --------------------------------------------------------------------- KE0
+def _exact_h8_stiffness(nu):
+    """Exact stiffness of the unit trilinear hexahedron from products of 1-D
+    integrals (independent of any quadrature)."""
+    nu = Fraction(nu).limit_denominator(10**6)
+    lam = nu / ((1 + nu) * (1 - 2 * nu))
+    mu = Fraction(1, 2) / (1 + nu)
+    corners = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+    # 1-D integrals over [0,1] of phi_a^(da) phi_b^(db), phi_0 = 1 - t, phi_1 = t
+    def i1(a, b, da, db):
+        sa = -1 if a == 0 else 1
+        sb = -1 if b == 0 else 1
+        if da and db:
+            return Fraction(sa * sb)
+        if da:
+            return Fraction(sa, 2)
+        if db:
+            return Fraction(sb, 2)
+        return Fraction(1, 3) if a == b else Fraction(1, 6)
+
+    def g(a, b, p, q):  # integral of dN_a/dx_p dN_b/dx_q
+        v = Fraction(1)
+        for d in range(3):
+            v *= i1(corners[a][d], corners[b][d], d == p, d == q)
+        return v
+
+    K = np.zeros((24, 24))
+    for a in range(8):
+        for b in range(8):
+            for i in range(3):
+                for j in range(3):
+                    v = lam * g(a, b, i, j) + mu * g(a, b, j, i)
+                    if i == j:
+                        v += mu * sum(g(a, b, k, k) for k in range(3))
+                    K[3 * a + i, 3 * b + j] = float(v)
+    return K
+
+
+def test_ke0_equals_exact_integral(oracle):
+    pr = oracle.Problem(2, 2, 2, poisson=0.3)
+    K = pr.ke0()
+    Kx = _exact_h8_stiffness(0.3)
+    assert np.max(np.abs(K - Kx)) < 1e-15 * 10
+
+
+def test_ke0_invariants(oracle):
+    pr = oracle.Problem(2, 2, 2)
+    K = pr.ke0()
+    assert np.max(np.abs(K - K.T)) < 1e-16 * 10
+    w = np.linalg.eigvalsh(K)
+    assert np.sum(np.abs(w) < 1e-12) == 6      # six rigid modes
+    assert w.min() > -1e-14                     # positive semi-definite
+    X = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float)
+    for d in range(3):                          # translations
+        t = np.zeros(24)
+        t[d::3] = 1.0
+        assert np.max(np.abs(K @ t)) < 1e-15 * 10
+    for (a, b) in [(0, 1), (1, 2), (0, 2)]:     # infinitesimal rotations
+        r = np.zeros((8, 3))
+        r[:, a] = -X[:, b]
+        r[:, b] = X[:, a]
+        assert np.max(np.abs(K @ r.ravel())) < 1e-14
+    # reference KAT (test_bmatrix.cpp:123-146): unit hexa8 volume == 1
+    _, vol = pr.element_ke(0, 1.0)
+    assert abs(vol - 1.0) < 1e-10
+
+
+def test_element_ke_translation_and_scaling(oracle):
+    pr = oracle.Problem(60, 20, 20)
+    K0 = pr.ke0()
+    Ke, vol = pr.element_ke(57 + 60 * (13 + 20 * 9), 0.37)
+    assert abs(vol - 1.0) < 1e-12
+    assert np.max(np.abs(Ke - 0.37 * K0)) < 1e-14
+
+
+# ----------------------------------------------------------- K, CG, c, dc
+def test_csr_equals_elementwise(oracle):
+    rng = np.random.default_rng(1)
+    pr = oracle.cantilever(7, 4, 3)
+    x = rng.uniform(1e-3, 1.0, pr.nelx)
+    pr.set_density(x)
+    u = rng.standard_normal(pr.numeq)
+    y1 = pr.spmv(u)
+    y2 = pr.apply_k_elementwise(u)
+    assert np.max(np.abs(y1 - y2)) <= 1e-13 * np.max(np.abs(y1))
+    # symmetry of the assembled operator
+    v = rng.standard_normal(pr.numeq)
+    assert abs(v @ pr.spmv(u) - u @ pr.spmv(v)) < 1e-12 * abs(v @ y1)
+
+
+def _dense_K(pr):
+    n = pr.numeq
+    K = np.zeros((n, n))
+    for i in range(n):
+        e = np.zeros(n)
+        e[i] = 1.0
+        K[:, i] = pr.spmv(e)
+    return K
+
+
+def test_scaling_cg_against_dense_solve(oracle):
+    rng = np.random.default_rng(2)
+    pr = oracle.cantilever(6, 3, 3)
+    x = rng.uniform(0.05, 1.0, pr.nelx)
+    pr.set_density(x)
+    K = _dense_K(pr)
+    f = pr.fext()
+    ustar = np.linalg.solve(K, f)
+    u, it, rn = pr.scaling_cg(tol=1e-12, maxit=10000)
+    assert np.linalg.norm(u - ustar) < 1e-9 * np.linalg.norm(ustar)
+    assert rn < 1e-12 * np.linalg.norm(f)
+    assert np.allclose(pr.diag(), np.diag(K), rtol=0, atol=1e-15)
+    # a capped solve stops after maxit + 1 iterations (CG.cpp:90-91)
+    _, it2, _ = pr.scaling_cg(tol=1e-30, maxit=5)
+    assert it2 == 6
+    # true residual of the converged solution (linear.hpp:93-103)
+    assert pr.true_residual(u) < 1e-11
+
+
+def test_compliance_identity(oracle):
+    rng = np.random.default_rng(3)
+    pr = oracle.cantilever(8, 4, 4)
+    pr.set_density(rng.uniform(0.1, 1.0, pr.nelx))
+    u, _, _ = pr.scaling_cg(tol=1e-12)
+    c = pr.compliance(u)
+    assert abs(c - u @ pr.spmv(u)) < 1e-10 * c
+
+
+def test_sensitivity_finite_differences(oracle):
+    """The reference's own check (sample/algorithm/opt-mma-HelmholtzFilter/fdm.cpp):
+    the analytic dc/dx_e against a difference quotient of re-solved compliances
+    (central differences here for a tighter bound)."""
+    rng = np.random.default_rng(4)
+    pr = oracle.cantilever(5, 3, 2)
+    x = rng.uniform(0.2, 1.0, pr.nelx)
+    pr.set_density(x)
+    u, _, _ = pr.scaling_cg(tol=1e-14)
+    nv = pr.node_values(u)
+    df, dg = pr.sensitivity(nv)
+    assert np.all(np.abs(dg - 1.0) < 1e-14)  # element volume (sum of Gauss jac)
+
+    def comp(xx):
+        pr.set_density(xx)
+        uu, _, _ = pr.scaling_cg(tol=1e-14)
+        return pr.compliance(uu)
+
+    h = 1e-5
+    for e in rng.choice(pr.nelx, 6, replace=False):
+        xp, xm = x.copy(), x.copy()
+        xp[e] += h
+        xm[e] -= h
+        fd = (comp(xp) - comp(xm)) / (2 * h)
+        assert abs(fd - df[e]) < 1e-6 * abs(df[e]), (e, fd, df[e])
+
+
+# ---------------------------------------------------------------- filter
+def test_filter_reference_kat(oracle):
+    """test_filter.cpp:63-124 restated on a 2x2x1 hexa8 slab (the same four
+    centre distances as the reference's 2x2 quad4 mesh), radius 2."""
+    pr = oracle.Problem(2, 2, 1)
+    pr.make_dof()
+    F = oracle.Filter(pr, 2.0, brute=True)
+    s = np.array([1e-4, 1.0, 1e-4, 1.0])
+    answer = np.array([0.34587, 0.65423, 0.34587, 0.65423])
+    sf = F.density_filter(s, False)
+    assert np.sum((sf - answer) ** 2) < 1e-10
+    # closed form of the same numbers
+    w = np.array([2.0, 1.0, 1.0, 2.0 - np.sqrt(2.0)])
+    exact0 = (w[0] * 1e-4 + w[1] * 1.0 + w[2] * 1e-4 + w[3] * 1.0) / w.sum()
+    assert abs(sf[0] - exact0) < 1e-15
+    dfds = np.array([1e-4, 1.0, 1e-4, 1.0])
+    ds = F.density_filter(dfds, True)
+    assert np.sum((ds - answer) ** 2) < 1e-10
+    st = oracle.threshold(sf, 16.0, 0.5)
+    assert st[0] < 0.01 and st[1] > 0.99
+    d1 = oracle.threshold_deriv(sf, 1.0, 0.5)
+    assert np.sum((d1 - 1.0) ** 2) < 0.015
+
+
+@pytest.mark.parametrize("dims,h", [((7, 5, 4), 1.0), ((6, 4, 5), 0.5)])
+@pytest.mark.parametrize("radius", [1.2, 1.5, 2.0, 2.5])
+def test_structured_neighbours_equal_bruteforce(oracle, dims, h, radius):
+    nx, ny, nz = dims
+    pr = oracle.Problem(nx, ny, nz, nx * h, ny * h, nz * h)
+    pr.make_dof()
+    rad = radius * h
+    A = oracle.Filter(pr, rad, brute=True).lists()
+    B = oracle.Filter(pr, rad, brute=False).lists()
+    for a, b in zip(A, B):
+        assert np.array_equal(a, b)
+
+
+def test_filter_interior_neighbour_counts(oracle):
+    pr = oracle.Problem(9, 9, 9)
+    pr.make_dof()
+    centre = 4 + 9 * (4 + 9 * 4)
+    for r, cnt in [(1.2, 7), (1.5, 19), (2.0, 33), (2.5, 81)]:
+        rp, ids, dis = oracle.Filter(pr, r).lists()
+        assert rp[centre + 1] - rp[centre] == cnt
+
+
+def test_filter_chain_rule_is_transpose(oracle):
+    """densityFilter(., true) is the transpose of the linear map x -> x_tilde."""
+    rng = np.random.default_rng(5)
+    pr = oracle.Problem(6, 5, 4)
+    pr.make_dof()
+    F = oracle.Filter(pr, 1.5)
+    n = pr.nelx
+    M = np.zeros((n, n))
+    for i in range(n):
+        e = np.zeros(n)
+        e[i] = 1
+        M[:, i] = F.density_filter(e, False)
+    v = rng.standard_normal(n)
+    assert np.allclose(F.density_filter(v, True), M.T @ v, atol=1e-14)
+    assert np.allclose(M.sum(axis=1), 1.0, atol=1e-14)  # rows are averages
+
+
+# -------------------------------------------------------------- OC / MMA
+def _toy(s):
+    f = s[0] ** 2 + s[1] ** 2 + s[2] ** 2
+    df = 2 * s
+    g = np.array([(s[0] - 5) ** 2 + (s[1] - 2) ** 2 + (s[2] - 1) ** 2 - 9,
+                  (s[0] - 3) ** 2 + (s[1] - 4) ** 2 + (s[2] - 3) ** 2 - 9])
+    dg = np.array([2 * (s[0] - 5), 2 * (s[0] - 3), 2 * (s[1] - 2), 2 * (s[1] - 4),
+                   2 * (s[2] - 1), 2 * (s[2] - 3)])
+    return f, df, g, dg
+
+
+def _run_toy(oracle, reference):
+    m = oracle.MMA(3, 2, reference=reference)
+    m.set(0.5, 0.1, 0.5, 0.7, 1.2)
+    s = np.array([4.0, 3.0, 2.0])
+    traj = []
+    for _ in range(10):
+        f, df, g, dg = _toy(s)
+        m.update(s, df, g, dg, np.zeros(3), np.full(3, 5.0))
+        traj.append(s.copy())
+    f, df, g, dg = _toy(s)
+    return f, g, np.array(traj)
+
+
+def test_mma_reference_kat(oracle):
+    """test_opt_solver.cpp:17-73: f < 8.78, |g_j| < 1e-6 after 10 updates."""
+    f, g, _ = _run_toy(oracle, reference=False)
+    assert f < 8.78
+    assert np.all(np.abs(g) < 1e-6)
+
+
+def test_mma_matches_compiled_reference(oracle):
+    if oracle.ref_lib() is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    _, _, t1 = _run_toy(oracle, reference=False)
+    _, _, t2 = _run_toy(oracle, reference=True)
+    assert np.max(np.abs(t1 - t2)) < 1e-12
+    # a SIMP-like m = 1 problem
+    rng = np.random.default_rng(6)
+    n = 500
+    x0 = rng.uniform(0.1, 0.9, n)
+    xs = [x0.copy(), x0.copy()]
+    mm = [oracle.MMA(n, 1, reference=r) for r in (False, True)]
+    for it in range(6):
+        df = -rng.uniform(0.1, 3.0, n)
+        dg = np.full(n, 1.0 / n)
+        gval = np.array([xs[0].mean() - 0.4])
+        for k in range(2):
+            mm[k].update(xs[k], df, gval, dg, np.full(n, 1e-4), np.ones(n))
+        assert np.max(np.abs(xs[0] - xs[1])) < 1e-11
+
+
+def test_oc_matches_compiled_reference(oracle):
+    if oracle.ref_lib() is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    rng = np.random.default_rng(7)
+    pr = oracle.Problem(8, 6, 5)
+    pr.make_dof()
+    F = oracle.Filter(pr, 1.5)
+    n = pr.nelx
+    x = rng.uniform(0.2, 0.6, n)
+    dc = -rng.uniform(0.01, 2.0, n)
+    dv = np.ones(n)
+    g = 0.35 * n
+    x1, cnt, lam = F.oc_update(x, dc, g, dv)
+
+    def vol(ptr, nn, user):
+        arr = np.ctypeslib.as_array(ptr, shape=(nn,)).copy()
+        return float(F.density_filter(arr, False).sum())
+
+    cb = oracle.REF_VOLUME_FN(vol)
+    x2 = x.copy()
+    oracle.ref_lib().ref_oc_update(n, oracle.P(x2), oracle.P(dc.copy()), g, oracle.P(dv.copy()),
+                                   oracle.P(np.full(n, 1e-4)), oracle.P(np.ones(n)), cb, None)
+    assert np.max(np.abs(x1 - x2)) < 1e-14
+    assert 10 < cnt < 100
+    # volume constraint active at the converged multiplier
+    assert abs(F.density_filter(x1, False).sum() - g) < 1e-3 * g
+
+
+def test_gaussian_field_recipe(oracle):
+    x, m = oracle.gaussian_field(24, 12, 10, seed=2)
+    assert x.min() >= 1e-3 and x.max() <= 1.0
+    assert abs(m - x.mean()) < 1e-15
+    x2, _ = oracle.gaussian_field(24, 12, 10, seed=2)
+    assert np.array_equal(x, x2)
+    x3, _ = oracle.gaussian_field(24, 12, 10, seed=3)
+    assert not np.array_equal(x, x3)
