@@ -220,7 +220,7 @@ class Context:
         return d
 
     def solve(self, tol=1e-8, maxit=10000, precision=FP64, precond=PRECOND_JACOBI,
-              inner_tol=1e-2, max_refine=60, want_u=True, allow_cap=False):
+              inner_tol=0.1, max_refine=200, want_u=True, allow_cap=False):
         o = SolveOpts(tol, maxit, precision, precond, inner_tol, max_refine, 0)
         st = SolveStats()
         u = np.zeros(self.numeq) if want_u else None
@@ -287,8 +287,8 @@ class Context:
         return ms.value, fl.value, by.value
 
     def simp_run(self, x0, n_iter, optimizer=0, use_projection=False, xmin=1e-4, xmax=1.0,
-                 tol=1e-8, maxit=10000, precision=FP64, precond=PRECOND_JACOBI, inner_tol=1e-2,
-                 max_refine=60):
+                 tol=1e-8, maxit=10000, precision=FP64, precond=PRECOND_JACOBI, inner_tol=0.1,
+                 max_refine=200):
         o = SimpOpts(n_iter, optimizer, 1 if use_projection else 0, 0, xmin, xmax,
                      SolveOpts(tol, maxit, precision, precond, inner_tol, max_refine, 0))
         x = np.array(x0, dtype=np.float64)

@@ -79,8 +79,10 @@ void Context::ensure_device() {
 
 Context::~Context() {
   if (!dev_ready_) return;
-  if (pcg_exec_) (void)hipGraphExecDestroy(pcg_exec_);
-  if (pcg_graph_) (void)hipGraphDestroy(pcg_graph_);
+  for (int k = 0; k < 2; ++k) {
+    if (pcg_exec_[k]) (void)hipGraphExecDestroy(pcg_exec_[k]);
+    if (pcg_graph_[k]) (void)hipGraphDestroy(pcg_graph_[k]);
+  }
   mixed_.reset();
   mma_.reset();
   if (stream_) (void)hipStreamSynchronize(stream_);
@@ -189,41 +191,69 @@ void Context::get_diag(double *d) {
 }
 
 void Context::build_pcg_graph() {
-  if (pcg_exec_) return;
-  HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-  try {
-    for (int j = 0; j < kBatch; ++j) {
-      double *p_old = d_p[j & 1], *p_new = d_p[(j + 1) & 1];
-      launch_pcg_stencil64(L_, d_ke, d_young, d_free, d_r, d_dinv, p_old, p_new, d_q, d_pcg,
-                           d_partials, d_counter, stream_);
-      launch_pcg_update64(n3_, d_x, d_r, p_new, d_q, d_dinv, d_pcg, d_partials, d_counter, stream_);
+  if (pcg_exec_[0]) return;
+  for (int par = 0; par < 2; ++par) {
+    HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    try {
+      for (int j = 0; j < kBatch; ++j) {
+        double *p_old = d_p[(j + par) & 1], *p_new = d_p[(j + par + 1) & 1];
+        launch_pcg_stencil64(L_, d_ke, d_young, d_free, d_r, d_dinv, p_old, p_new, d_q, d_pcg,
+                             d_partials, d_counter, stream_);
+        launch_pcg_update64(n3_, d_x, d_r, p_new, d_q, d_dinv, d_pcg, d_partials, d_counter,
+                            stream_);
+      }
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(stream_, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
     }
-  } catch (...) {
-    hipGraph_t g = nullptr;
-    (void)hipStreamEndCapture(stream_, &g);
-    if (g) (void)hipGraphDestroy(g);
-    throw;
+    HIPCHK(hipStreamEndCapture(stream_, &pcg_graph_[par]));
+    HIPCHK(hipGraphInstantiate(&pcg_exec_[par], pcg_graph_[par], nullptr, nullptr, 0));
   }
-  HIPCHK(hipStreamEndCapture(stream_, &pcg_graph_));
-  HIPCHK(hipGraphInstantiate(&pcg_exec_, pcg_graph_, nullptr, nullptr, 0));
 }
 
-PcgScalars Context::pcg64(double tol, int maxit) {
+PcgScalars Context::pcg64(double tol, int maxit, bool warm) {
   build_pcg_graph();
   PcgScalars init{};
   init.tol = tol;
   init.maxit = maxit;
+  init.delta = 0.0;
   *h_pcg.p = init;
   h2d(d_pcg.p, h_pcg.p, 1, stream_);
+  if (warm) d2d(d_nv.p, d_x.p, n3_, stream_);
+  // x = 0, r = b, p = 0, bnorm2 = b.b
   launch_pcg_init64(n3_, d_b, d_x, d_r, d_p[0], d_dinv, d_pcg, d_partials, d_counter, stream_);
-  const long long max_batches = (long long)maxit / kBatch + 3;
-  for (long long b = 0; b < max_batches; ++b) {
-    HIPCHK(hipGraphLaunch(pcg_exec_, stream_));
+  if (warm) {
+    d2d(d_x.p, d_nv.p, n3_, stream_);
+    launch_apply_k64(L_, d_ke, d_young, d_free, d_x, d_y, stream_);
+    launch_replace64(n3_, d_b, d_y, d_r, d_dinv, d_pcg, d_partials, stream_);
+  }
+  d2h(h_pcg.p, d_pcg.p, 1, stream_);
+  sync();
+  replacements_ = 0;
+  while (!h_pcg.p->done || h_pcg.p->status == 0) {
+    // iterate until the recursive stop test, a cap or a breakdown
+    const long long max_batches = (long long)(maxit - h_pcg.p->iter) / kBatch + 3;
+    for (long long b = 0; b < max_batches && !h_pcg.p->done; ++b) {
+      HIPCHK(hipGraphLaunch(pcg_exec_[h_pcg.p->iter & 1], stream_));
+      d2h(h_pcg.p, d_pcg.p, 1, stream_);
+      sync();
+    }
+    if (!h_pcg.p->done) throw Error(kInternal, "PCG did not reach its stop test");
+    if (h_pcg.p->status != 0) break;
+    // verified stop: the FP64 true residual must meet the tolerance too;
+    // otherwise it replaces the recursive one and the iteration continues
+    launch_apply_k64(L_, d_ke, d_young, d_free, d_x, d_y, stream_);
+    launch_replace64(n3_, d_b, d_y, d_r, d_dinv, d_pcg, d_partials, stream_);
     d2h(h_pcg.p, d_pcg.p, 1, stream_);
     sync();
     if (h_pcg.p->done) break;
+    if (++replacements_ > 20 || h_pcg.p->iter > maxit) {
+      h_pcg.p->status = 1;
+      break;
+    }
   }
-  if (!h_pcg.p->done) throw Error(kInternal, "PCG did not reach its stop test");
   return *h_pcg.p;
 }
 
@@ -267,7 +297,7 @@ void Context::solve(const icarat_solve_opts &o, double *u_host, icarat_solve_sta
   if (o.precision == ICARAT_HIP_FP64 && o.precond == ICARAT_HIP_PRECOND_JACOBI) {
     PcgScalars r = pcg64(o.tol, o.maxit);
     S.iters = r.iter;
-    S.refine_steps = 1;
+    S.refine_steps = replacements_;
     S.rec_relres = r.bnorm2 > 0 ? std::sqrt(r.rr) / std::sqrt(r.bnorm2) : 0.0;
     if (r.status == 2) throw Error(kNumerical, "PCG breakdown (non-positive curvature or NaN)");
     if (r.status == 1) S.status = ICARAT_HIP_NOT_CONVERGED;
@@ -501,7 +531,7 @@ void Context::time_kernel(int kernel, int reps, double *avg_ms, double *flops, d
     h2d(d_pcg.p, h_pcg.p, 1, stream_);
     launch = [&]() {
       launch_pcg_stencil64(L_, d_ke, d_young, d_free, d_r, d_dinv, d_p[0], d_p[1], d_q, d_pcg,
-                           d_partials, d_counter, stream_);
+                           d_partials, d_counter, stream_, false);
     };
     F = 2.0 * 24 * 24 * E;                      // dense KE0 x U_e contraction
     B = 3.0 * N * 8 * 3 + 2.0 * 3 * N * 8 + E * 8 + N;  // r, dinv, p_old in; p, q out; young; mask
@@ -514,7 +544,8 @@ void Context::time_kernel(int kernel, int reps, double *avg_ms, double *flops, d
     *h_pcg.p = s0;
     h2d(d_pcg.p, h_pcg.p, 1, stream_);
     launch = [&]() {
-      launch_pcg_update64(n3_, d_x, d_r, d_p[1], d_q, d_dinv, d_pcg, d_partials, d_counter, stream_);
+      launch_pcg_update64(n3_, d_x, d_r, d_p[1], d_q, d_dinv, d_pcg, d_partials, d_counter,
+                          stream_, false);
     };
     F = 8.0 * 3 * N;
     B = (5.0 + 2.0) * 3 * N * 8;                // x, r, p, q, dinv in; x, r out

@@ -92,16 +92,18 @@ class Context {
   double vol_e_ = 1.0;
   std::string last_error;
 
-  // PCG iteration batch captured as a graph
+  // PCG iteration batch captured as a graph, one per parity of the p buffers
   static constexpr int kBatch = 16;
-  hipGraphExec_t pcg_exec_ = nullptr;
-  hipGraph_t pcg_graph_ = nullptr;
+  hipGraphExec_t pcg_exec_[2] = {nullptr, nullptr};
+  hipGraph_t pcg_graph_[2] = {nullptr, nullptr};
+  int replacements_ = 0;  // residual replacements of the last FP64 solve
 
   std::unique_ptr<MixedSolver> mixed_;
   std::unique_ptr<DeviceMMA> mma_;
 
-  // FP64 Jacobi-PCG on d_b -> d_x; returns iterations, status in S
-  PcgScalars pcg64(double tol, int maxit);
+  // FP64 Jacobi-PCG on d_b -> d_x (from x = 0, or from the current d_x when
+  // warm), stopped on the FP64 true residual; returns the final scalars.
+  PcgScalars pcg64(double tol, int maxit, bool warm = false);
   void assemble_nodal_solution();         // d_u = d_x + Dirichlet values
   void sync() { HIPCHK(hipStreamSynchronize(stream_)); }
   // Device buffers are allocated on first device use, so mesh, numbering,

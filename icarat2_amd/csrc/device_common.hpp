@@ -29,15 +29,17 @@ __host__ __device__ constexpr int lnode_z(int a) { return a >> 2; }
 
 // Scalars of one device-resident Jacobi-PCG solve.
 struct PcgScalars {
-  double rho;      // r.z of the current iterate
-  double pq;       // p.Ap
-  double rr;       // r.r
-  double bnorm2;   // b.b
-  double tol;      // relative tolerance on ||r|| / ||b||
+  double rho;        // r.z of the current iterate
+  double pq;         // p.Ap
+  double rr;         // r.r
+  double bnorm2;     // b.b
+  double tol;        // relative tolerance on ||r|| / ||b||
   double alpha, beta;
+  double delta;      // reliable updates: replace r once ||r|| < delta * ||r at last replacement||
+  double repl_norm;  // current replacement threshold (0: none)
   int iter, maxit;
-  int done;        // 1 once converged, capped or broken down
-  int status;      // 0 running/converged, 1 capped, 2 breakdown
+  int done;          // 1 once converged, capped, broken down or a replacement is due
+  int status;        // 0 running/converged, 1 capped, 2 breakdown, 3 replacement due
 };
 
 // Scalars of the device-resident OC bisection.
@@ -75,43 +77,45 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *sh /* NV*NT/6
   __syncthreads();
 }
 
-// Every block publishes its partial sums; the last block to finish sums the
-// partials in block order (deterministic for a fixed grid) and returns true
-// with the totals in tot[] on all its threads.  The counter is re-armed for
-// the next launch.
+// Every block stores its partial sums (plain stores, no fences); a separate
+// single-block finalize kernel sums them in block order after the kernel
+// boundary has made them visible.  Deterministic for a fixed grid.
 template <int NT, int NV>
-__device__ bool last_block_sum(double (&v)[NV], double *partials, unsigned *counter,
-                               double (&tot)[NV], double *sh) {
-  __shared__ int s_last;
+__device__ __forceinline__ void store_partials(double (&v)[NV], double *partials, double *sh) {
   const unsigned G = gridDim.x * gridDim.y * gridDim.z;
   const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
   block_sum<NT, NV>(v, sh);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int k = 0; k < NV; ++k)
-      __hip_atomic_store(&partials[(size_t)k * G + bid], v[k], __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    __threadfence();
-    unsigned prev = atomicAdd(counter, 1u);
-    s_last = (prev == G - 1);
+    for (int k = 0; k < NV; ++k) partials[(size_t)k * G + bid] = v[k];
   }
-  __syncthreads();
-  if (!s_last) return false;
-  __threadfence();
-  double acc[NV];
+}
+
+// Fixed-order sum of G partials per quantity (call from one block of NT threads).
+template <int NT, int NV>
+__device__ __forceinline__ void sum_partials(const double *partials, unsigned G, double (&tot)[NV],
+                                             double *sh) {
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
     double s = 0.0;
-    for (unsigned i = threadIdx.x; i < G; i += NT)
-      s += __hip_atomic_load(&partials[(size_t)k * G + i], __ATOMIC_RELAXED,
-                             __HIP_MEMORY_SCOPE_AGENT);
-    acc[k] = s;
+    for (unsigned i = threadIdx.x; i < G; i += NT) s += partials[(size_t)k * G + i];
+    tot[k] = s;
   }
-  block_sum<NT, NV>(acc, sh);
-#pragma unroll
-  for (int k = 0; k < NV; ++k) tot[k] = acc[k];
-  if (threadIdx.x == 0) *counter = 0u;
-  return true;
+  block_sum<NT, NV>(tot, sh);
 }
+
+// Scalar steps run by the finalize kernel after a reduction.
+enum FinalizeOp : int {
+  kFinOut = 0,        // out[k] = sum_k
+  kFinPcgInit = 1,    // rho = sum0, bnorm2 = rr = sum1, reset the iteration
+  kFinPcgAlpha = 2,   // pq = sum0, alpha = rho / pq, breakdown test
+  kFinPcgBeta = 3,    // rho' = sum0, rr = sum1, beta, iteration count, stop test
+  kFinOc = 4,         // OC bisection update from the volume sum0
+  kFinReplace = 5,    // rho = sum0, rr = sum1 of the replaced (true) residual; stop test
+};
+
+// Launches the single-block finalize of a reduction over G producer blocks.
+void launch_finalize(int op, int nv, const double *partials, unsigned G, PcgScalars *S,
+                     OcState *O, double *out, hipStream_t s);
 
 }  // namespace icarat_hip

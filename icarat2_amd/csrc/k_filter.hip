@@ -137,16 +137,8 @@ __global__ __launch_bounds__(NT) void k_oc_step(Lattice L, int count, const int 
     if (use_projection) xf = proj(xf, beta, 0.5);
     vol += xf * vol_e;
   }
-  double v[1] = {vol}, tot[1];
-  if (last_block_sum<NT, 1>(v, partials, counter, tot, sh) && threadIdx.x == 0) {
-    S->vol = tot[0];
-    if (tot[0] - S->g > 0.0) S->l0 = lam;
-    else S->l1 = lam;
-    S->counter += 1;
-    S->lam_last = lam;
-    const bool more = (S->l1 - S->l0) / (S->l1 + S->l0) > 1.0e-5 && S->maxiter > S->counter;
-    S->done = more ? 0 : 1;
-  }
+  double v[1] = {vol};
+  store_partials<NT, 1>(v, partials, sh);
 }
 
 __global__ void k_oc_final(long long n, double *__restrict__ x, const double *__restrict__ dc,
@@ -181,8 +173,8 @@ __global__ __launch_bounds__(NT) void k_volume(long long n, const double *__rest
   double s = 0.0;
   const long long stride = (long long)gridDim.x * NT;
   for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += stride) s += x[i] * vol_e;
-  double v[1] = {s}, tot[1];
-  if (last_block_sum<NT, 1>(v, partials, counter, tot, sh) && threadIdx.x == 0) out[0] = tot[0];
+  double v[1] = {s};
+  store_partials<NT, 1>(v, partials, sh);
 }
 
 inline unsigned blocks_for(long long n, int nt) { return (unsigned)((n + nt - 1) / nt); }
@@ -222,6 +214,8 @@ void launch_oc_step(const Lattice &L, const FilterTable &T, const double *x, con
                                           use_projection, beta,
                                           vol_e, S, partials, counter));
   HIPCHK(hipGetLastError());
+  const dim3 g = filter_grid(L);
+  launch_finalize(kFinOc, 1, partials, g.x * g.y * g.z, nullptr, S, nullptr, s);
 }
 
 void launch_oc_final(long long n, double *x, const double *dc, const double *dv, double xmin,
@@ -247,6 +241,7 @@ void launch_volume(long long n, const double *x, double vol_e, double *out, doub
                    unsigned *counter, hipStream_t s) {
   hipLaunchKernelGGL(k_volume, dim3(2048), dim3(NT), 0, s, n, x, vol_e, out, partials, counter);
   HIPCHK(hipGetLastError());
+  launch_finalize(kFinOut, 1, partials, 2048, nullptr, nullptr, out, s);
 }
 
 }  // namespace icarat_hip

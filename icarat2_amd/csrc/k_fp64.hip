@@ -138,17 +138,8 @@ __global__ __launch_bounds__(NT) void k_stencil64(
     }
   }
   if constexpr (PCG) {
-    double v[1] = {pq}, tot[1];
-    if (last_block_sum<NT, 1>(v, partials, counter, tot, sh) && threadIdx.x == 0) {
-      const double pqt = tot[0];
-      S->pq = pqt;
-      const double a = S->rho / pqt;
-      S->alpha = a;
-      if (!(pqt > 0.0) || !isfinite(a)) {
-        S->status = 2;
-        S->done = 1;
-      }
-    }
+    double v[1] = {pq};
+    store_partials<NT, 1>(v, partials, sh);
   }
 }
 
@@ -171,25 +162,8 @@ __global__ __launch_bounds__(NT) void k_pcg_update64(long long n, double *__rest
     rz += (dinv[i] * ri) * ri;
     rr += ri * ri;
   }
-  double v[2] = {rz, rr}, tot[2];
-  if (last_block_sum<NT, 2>(v, partials, counter, tot, sh) && threadIdx.x == 0) {
-    const double rho1 = tot[0];
-    S->beta = rho1 / S->rho;
-    S->rho = rho1;
-    S->rr = tot[1];
-    S->iter += 1;
-    const double rnorm = sqrt(tot[1]);
-    if (!isfinite(rnorm)) {
-      S->status = 2;
-      S->done = 1;
-    } else if (rnorm < S->tol * sqrt(S->bnorm2)) {
-      S->status = 0;
-      S->done = 1;
-    } else if (S->iter > S->maxit) {
-      S->status = 1;
-      S->done = 1;
-    }
-  }
+  double v[2] = {rz, rr};
+  store_partials<NT, 2>(v, partials, sh);
 }
 
 __global__ __launch_bounds__(NT) void k_pcg_init64(long long n, const double *__restrict__ b,
@@ -210,18 +184,27 @@ __global__ __launch_bounds__(NT) void k_pcg_init64(long long n, const double *__
     rz += (dinv[i] * bi) * bi;
     rr += bi * bi;
   }
-  double v[2] = {rz, rr}, tot[2];
-  if (last_block_sum<NT, 2>(v, partials, counter, tot, sh) && threadIdx.x == 0) {
-    S->rho = tot[0];
-    S->bnorm2 = tot[1];
-    S->rr = tot[1];
-    S->beta = 0.0;
-    S->alpha = 0.0;
-    S->pq = 0.0;
-    S->iter = 0;
-    S->status = 0;
-    S->done = tot[1] == 0.0 ? 1 : 0;
+  double v[2] = {rz, rr};
+  store_partials<NT, 2>(v, partials, sh);
+}
+
+// r = b - y (true residual after K x), rho = r.dinv.r, rr = r.r
+__global__ __launch_bounds__(NT) void k_replace64(long long n, const double *__restrict__ b,
+                                                  const double *__restrict__ y,
+                                                  double *__restrict__ r,
+                                                  const double *__restrict__ dinv,
+                                                  double *partials) {
+  __shared__ double sh[8];
+  double rz = 0.0, rr = 0.0;
+  const long long stride = (long long)gridDim.x * NT;
+  for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += stride) {
+    const double ri = b[i] - y[i];
+    r[i] = ri;
+    rz += (dinv[i] * ri) * ri;
+    rr += ri * ri;
   }
+  double v[2] = {rz, rr};
+  store_partials<NT, 2>(v, partials, sh);
 }
 
 __global__ __launch_bounds__(NT) void k_residual_norms(long long n, const double *__restrict__ b,
@@ -235,11 +218,8 @@ __global__ __launch_bounds__(NT) void k_residual_norms(long long n, const double
     a += d * d;
     c += b[i] * b[i];
   }
-  double v[2] = {a, c}, tot[2];
-  if (last_block_sum<NT, 2>(v, partials, counter, tot, sh) && threadIdx.x == 0) {
-    out2[0] = tot[0];
-    out2[1] = tot[1];
-  }
+  double v[2] = {a, c};
+  store_partials<NT, 2>(v, partials, sh);
 }
 
 __global__ __launch_bounds__(NT) void k_dot(long long n, const double *__restrict__ a,
@@ -249,8 +229,8 @@ __global__ __launch_bounds__(NT) void k_dot(long long n, const double *__restric
   double s = 0.0;
   const long long stride = (long long)gridDim.x * NT;
   for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += stride) s += a[i] * b[i];
-  double v[1] = {s}, tot[1];
-  if (last_block_sum<NT, 1>(v, partials, counter, tot, sh) && threadIdx.x == 0) out[0] = tot[0];
+  double v[1] = {s};
+  store_partials<NT, 1>(v, partials, sh);
 }
 
 __global__ void k_young(const double *__restrict__ x, double *__restrict__ young, long long n,
@@ -391,18 +371,21 @@ void launch_apply_k64(const Lattice &L, const double *ke, const double *young,
 void launch_pcg_stencil64(const Lattice &L, const double *ke, const double *young,
                           const uint8_t *freemask, const double *r, const double *dinv,
                           const double *p_old, double *p_new, double *q, PcgScalars *S,
-                          double *partials, unsigned *counter, hipStream_t s) {
-  hipLaunchKernelGGL(k_stencil64<true>, stencil_grid(L), dim3(NT), 0, s, L, ke, young, freemask, r,
+                          double *partials, unsigned *counter, hipStream_t s, bool finalize) {
+  const dim3 g = stencil_grid(L);
+  hipLaunchKernelGGL(k_stencil64<true>, g, dim3(NT), 0, s, L, ke, young, freemask, r,
                      dinv, p_old, p_new, q, S, partials, counter);
   HIPCHK(hipGetLastError());
+  if (finalize) launch_finalize(kFinPcgAlpha, 1, partials, g.x * g.y * g.z, S, nullptr, nullptr, s);
 }
 
 void launch_pcg_update64(long long n, double *x, double *r, const double *p, const double *q,
                          const double *dinv, PcgScalars *S, double *partials, unsigned *counter,
-                         hipStream_t s) {
+                         hipStream_t s, bool finalize) {
   hipLaunchKernelGGL(k_pcg_update64, dim3(VEC_BLOCKS), dim3(NT), 0, s, n, x, r, p, q, dinv, S,
                      partials, counter);
   HIPCHK(hipGetLastError());
+  if (finalize) launch_finalize(kFinPcgBeta, 2, partials, VEC_BLOCKS, S, nullptr, nullptr, s);
 }
 
 void launch_pcg_init64(long long n, const double *b, double *x, double *r, double *p0,
@@ -411,6 +394,14 @@ void launch_pcg_init64(long long n, const double *b, double *x, double *r, doubl
   hipLaunchKernelGGL(k_pcg_init64, dim3(VEC_BLOCKS), dim3(NT), 0, s, n, b, x, r, p0, dinv, S,
                      partials, counter);
   HIPCHK(hipGetLastError());
+  launch_finalize(kFinPcgInit, 2, partials, VEC_BLOCKS, S, nullptr, nullptr, s);
+}
+
+void launch_replace64(long long n, const double *b, const double *y, double *r,
+                      const double *dinv, PcgScalars *S, double *partials, hipStream_t s) {
+  hipLaunchKernelGGL(k_replace64, dim3(VEC_BLOCKS), dim3(NT), 0, s, n, b, y, r, dinv, partials);
+  HIPCHK(hipGetLastError());
+  launch_finalize(kFinReplace, 2, partials, VEC_BLOCKS, S, nullptr, nullptr, s);
 }
 
 void launch_residual_norms(long long n, const double *b, const double *y, double *out2,
@@ -418,12 +409,14 @@ void launch_residual_norms(long long n, const double *b, const double *y, double
   hipLaunchKernelGGL(k_residual_norms, dim3(VEC_BLOCKS), dim3(NT), 0, s, n, b, y, out2, partials,
                      counter);
   HIPCHK(hipGetLastError());
+  launch_finalize(kFinOut, 2, partials, VEC_BLOCKS, nullptr, nullptr, out2, s);
 }
 
 void launch_dot(long long n, const double *a, const double *b, double *out, double *partials,
                 unsigned *counter, hipStream_t s) {
   hipLaunchKernelGGL(k_dot, dim3(VEC_BLOCKS), dim3(NT), 0, s, n, a, b, out, partials, counter);
   HIPCHK(hipGetLastError());
+  launch_finalize(kFinOut, 1, partials, VEC_BLOCKS, nullptr, nullptr, out, s);
 }
 
 void launch_flush_small(long long n, const double *u, double *v, hipStream_t s) {

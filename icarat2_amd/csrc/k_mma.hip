@@ -75,8 +75,8 @@ __global__ __launch_bounds__(NT) void k_mma_gensub(long long n, const double *__
     A.xold1[i] = xi;
     bsum += p1 / ux + q1 / xl2;
   }
-  double v[1] = {bsum}, tot[1];
-  if (last_block_sum<NT, 1>(v, partials, counter, tot, sh) && threadIdx.x == 0) out[0] = tot[0];
+  double v[1] = {bsum};
+  store_partials<NT, 1>(v, partials, sh);
 }
 
 // x(lambda) (XYZofLAMBDA, reference MMASolver.cpp:359-398) and the two dual
@@ -103,11 +103,8 @@ __global__ __launch_bounds__(NT) void k_mma_dual(long long n, double lam, MmaArr
     if (xp < A.alpha[i] || xp > A.beta[i]) df2 = 0.0;
     shs += pq * df2 * pq;
   }
-  double v[2] = {sg, shs}, tot[2];
-  if (last_block_sum<NT, 2>(v, partials, counter, tot, sh) && threadIdx.x == 0) {
-    out[0] = tot[0];
-    out[1] = tot[1];
-  }
+  double v[2] = {sg, shs};
+  store_partials<NT, 2>(v, partials, sh);
 }
 
 }  // namespace
@@ -118,6 +115,7 @@ void launch_mma_gensub(long long n, const double *x, const double *dfdx, const d
   hipLaunchKernelGGL(k_mma_gensub, dim3(BLOCKS), dim3(NT), 0, s, n, x, dfdx, dgdx, P, A,
                      partials, counter, out);
   HIPCHK(hipGetLastError());
+  launch_finalize(kFinOut, 1, partials, BLOCKS, nullptr, nullptr, out, s);
 }
 
 void launch_mma_dual(long long n, double lam, const MmaArrays &A, double *x, double *partials,
@@ -125,6 +123,7 @@ void launch_mma_dual(long long n, double lam, const MmaArrays &A, double *x, dou
   hipLaunchKernelGGL(k_mma_dual, dim3(BLOCKS), dim3(NT), 0, s, n, lam, A, x, partials, counter,
                      out);
   HIPCHK(hipGetLastError());
+  launch_finalize(kFinOut, 2, partials, BLOCKS, nullptr, nullptr, out, s);
 }
 
 }  // namespace icarat_hip

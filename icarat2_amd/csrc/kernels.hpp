@@ -25,15 +25,20 @@ void launch_apply_k64(const Lattice &L, const double *ke, const double *young,
 void launch_pcg_stencil64(const Lattice &L, const double *ke, const double *young,
                           const uint8_t *freemask, const double *r, const double *dinv,
                           const double *p_old, double *p_new, double *q, PcgScalars *S,
-                          double *partials, unsigned *counter, hipStream_t s);
+                          double *partials, unsigned *counter, hipStream_t s,
+                          bool finalize = true);
 // Vector half: x += alpha p; r -= alpha q; rho' = r.dinv.r; r.r; beta; stop test.
 void launch_pcg_update64(long long n, double *x, double *r, const double *p, const double *q,
                          const double *dinv, PcgScalars *S, double *partials, unsigned *counter,
-                         hipStream_t s);
+                         hipStream_t s, bool finalize = true);
 // x = 0, r = b, p = 0, rho = b.dinv.b, bnorm2 = b.b
 void launch_pcg_init64(long long n, const double *b, double *x, double *r, double *p0,
                        const double *dinv, PcgScalars *S, double *partials, unsigned *counter,
                        hipStream_t s);
+// reliable update / verified stop: r = b - y with y = K x; rho and r.r of
+// the new residual through the kFinReplace finalize
+void launch_replace64(long long n, const double *b, const double *y, double *r,
+                      const double *dinv, PcgScalars *S, double *partials, hipStream_t s);
 // res = b - y on free DOFs; sums ||res||^2 and ||b||^2 into out2[0..1]
 void launch_residual_norms(long long n, const double *b, const double *y, double *out2,
                            double *partials, unsigned *counter, hipStream_t s);

@@ -66,14 +66,16 @@ typedef struct icarat_solve_opts {
   int32_t maxit;          /* iteration cap of the (inner) Krylov method */
   int32_t precision;      /* ICARAT_HIP_FP64 / FP32 / BF16 */
   int32_t precond;        /* ICARAT_HIP_PRECOND_* */
-  double inner_tol;       /* mixed precision: relative reduction per refinement step */
-  int32_t max_refine;     /* mixed precision: cap on refinement steps */
+  double inner_tol;       /* mixed precision: reliable-update ratio delta -- the FP64 true
+                             residual replaces the FP32 recursive one whenever the latter
+                             has dropped by delta since the last replacement (default 0.1) */
+  int32_t max_refine;     /* mixed precision: cap on residual replacements (default 200) */
   int32_t reserved;
 } icarat_solve_opts;
 
 typedef struct icarat_solve_stats {
   int32_t iters;          /* Krylov iterations (all refinement steps summed) */
-  int32_t refine_steps;   /* FP64 refinement steps (1 for FP64) */
+  int32_t refine_steps;   /* FP64 residual replacements (reliable updates / verified stops) */
   double rec_relres;      /* recursive (Krylov) relative residual at exit */
   double true_relres;     /* FP64 ||f - K u|| / ||f|| recomputed by the FP64 kernel */
   double seconds;         /* device wall time of the solve */
