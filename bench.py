@@ -56,6 +56,11 @@ def run_gpu(A, cfg, method, steps, warmup, tol):
     prec, pre = METHODS[method]
     c, x0 = make_problem(A, cfg)
     kw = dict(tol=tol, maxit=100000, precision=prec, precond=pre)
+    if pre == 1:
+        # the network is trained online on the Jacobi solves of the warm-up
+        # iterations (at least one); the timed iterations use the trained network
+        c.learned_setup(train_iters=max(warmup, 1), train_steps=400 * max(warmup, 1))
+        warmup = max(warmup, 1)
     if warmup > 0:
         log(f"[bench] warmup {warmup} SIMP iteration(s), method {method}")
         c.simp_run(x0, warmup, **kw)
@@ -163,6 +168,8 @@ def main():
         "solve_s_per_iter": st.solve_s / a.steps,
         "pcg_iters_per_solve": pcg_per_solve,
         "max_true_relres": st.max_true_relres,
+        "train_s_timed": st.train_s,
+        "fallbacks": int(st.fallbacks),
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": MI355X_PEAK["fp64_tflops"],
                      "unit": "TFLOP/s", "frac": achieved / MI355X_PEAK["fp64_tflops"],
                      "traffic": None,

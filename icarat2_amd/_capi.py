@@ -46,7 +46,19 @@ class SimpOpts(C.Structure):
 class SimpStats(C.Structure):
     _fields_ = [("seconds", C.c_double), ("solve_s", C.c_double), ("filter_s", C.c_double),
                 ("sens_s", C.c_double), ("update_s", C.c_double), ("pcg_iters", C.c_int64),
-                ("max_true_relres", C.c_double)]
+                ("max_true_relres", C.c_double), ("train_s", C.c_double),
+                ("fallbacks", C.c_int64)]
+
+
+class LearnedOpts(C.Structure):
+    _fields_ = [("train_iters", C.c_int32), ("train_steps", C.c_int32),
+                ("snapshot_every", C.c_int32), ("max_samples", C.c_int32),
+                ("batch_iters", C.c_int32), ("stall_iters", C.c_int32), ("lr", C.c_float),
+                ("reserved", C.c_int32), ("seed", C.c_uint64)]
+
+
+fp = C.POINTER(C.c_float)
+i64p = C.POINTER(C.c_int64)
 
 
 _SIGS = {
@@ -80,6 +92,25 @@ _SIGS = {
                                         C.c_double, i32p]),
     "icarat_hip_simp_run": (C.c_int, [C.c_void_p, C.POINTER(SimpOpts), dp, dp,
                                       C.POINTER(SimpStats)]),
+    "icarat_hip_learned_default_opts": (C.c_int, [C.POINTER(LearnedOpts)]),
+    "icarat_hip_learned_setup": (C.c_int, [C.c_void_p, C.POINTER(LearnedOpts)]),
+    "icarat_hip_learned_harvest": (C.c_int, [C.c_void_p, C.POINTER(SolveOpts), dp,
+                                             C.POINTER(SolveStats)]),
+    "icarat_hip_learned_train": (C.c_int, [C.c_void_p, C.c_int32, dp]),
+    "icarat_hip_learned_info": (C.c_int, [C.c_void_p, i32p, i64p, dp, i64p, i32p]),
+    "icarat_hip_learned_get_params": (C.c_int, [C.c_void_p, fp]),
+    "icarat_hip_learned_set_params": (C.c_int, [C.c_void_p, fp]),
+    "icarat_hip_learned_apply": (C.c_int, [C.c_void_p, dp, dp]),
+    "icarat_hip_unet_create": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_uint64,
+                                         C.POINTER(C.c_void_p)]),
+    "icarat_hip_unet_destroy": (C.c_int, [C.c_void_p]),
+    "icarat_hip_unet_last_error": (C.c_char_p, [C.c_void_p]),
+    "icarat_hip_unet_param_count": (C.c_int, [i64p]),
+    "icarat_hip_unet_set_params": (C.c_int, [C.c_void_p, fp]),
+    "icarat_hip_unet_get_params": (C.c_int, [C.c_void_p, fp]),
+    "icarat_hip_unet_forward": (C.c_int, [C.c_void_p, fp, fp]),
+    "icarat_hip_unet_grads": (C.c_int, [C.c_void_p, fp, fp, dp, fp]),
+    "icarat_hip_unet_train_step": (C.c_int, [C.c_void_p, fp, fp, C.c_float, dp]),
     "icarat_hip_time_kernel": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, dp, dp, dp]),
     "icarat_hip_gaussian_field": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_double,
                                             C.c_uint64, C.c_int32, C.c_double, C.c_double,
@@ -120,6 +151,90 @@ def _p(a):
 
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _f32(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(fp)
+
+
+def learned_default_opts(**kw):
+    """LearnedOpts with the library defaults, fields overridden by keyword."""
+    o = LearnedOpts()
+    rc = lib().icarat_hip_learned_default_opts(C.byref(o))
+    if rc:
+        raise IcaratError(rc, "learned_default_opts")
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+def unet_param_count():
+    n = C.c_int64()
+    lib().icarat_hip_unet_param_count(C.byref(n))
+    return n.value
+
+
+class UNet:
+    """The learned preconditioner's network alone on an X x Y x Z cell grid
+    (multiples of 4): x (Z, Y, X, 4) -> out (Z, Y, X, 3), FP32 host arrays."""
+
+    def __init__(self, X, Y, Z, seed=1):
+        self.L = lib()
+        self.shape = (Z, Y, X)
+        h = C.c_void_p()
+        rc = self.L.icarat_hip_unet_create(X, Y, Z, seed, C.byref(h))
+        if rc:
+            raise IcaratError(rc, "unet_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.icarat_hip_unet_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc, what):
+        if rc:
+            msg = self.L.icarat_hip_unet_last_error(self.h)
+            raise IcaratError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def set_params(self, flat):
+        a, p = _f32(flat)
+        assert a.size == unet_param_count()
+        self._chk(self.L.icarat_hip_unet_set_params(self.h, p), "unet_set_params")
+
+    def get_params(self):
+        out = np.zeros(unet_param_count(), dtype=np.float32)
+        self._chk(self.L.icarat_hip_unet_get_params(self.h, out.ctypes.data_as(fp)),
+                  "unet_get_params")
+        return out
+
+    def forward(self, x):
+        a, p = _f32(x)
+        assert a.shape == self.shape + (4,)
+        out = np.zeros(self.shape + (3,), dtype=np.float32)
+        self._chk(self.L.icarat_hip_unet_forward(self.h, p, out.ctypes.data_as(fp)), "unet_forward")
+        return out
+
+    def grads(self, x, target):
+        a, p = _f32(x)
+        t, tp = _f32(target)
+        g = np.zeros(unet_param_count(), dtype=np.float32)
+        loss = C.c_double()
+        self._chk(self.L.icarat_hip_unet_grads(self.h, p, tp, C.byref(loss), g.ctypes.data_as(fp)),
+                  "unet_grads")
+        return loss.value, g
+
+    def train_step(self, x, target, lr=1e-3):
+        a, p = _f32(x)
+        t, tp = _f32(target)
+        loss = C.c_double()
+        self._chk(self.L.icarat_hip_unet_train_step(self.h, p, tp, lr, C.byref(loss)),
+                  "unet_train_step")
+        return loss.value
 
 
 def device_available():
@@ -279,6 +394,49 @@ class Context:
         self._chk(self.L.icarat_hip_mma_update(self.h, _p(x), _p(_f64(dfdx)), g, _p(_f64(dgdx)),
                                                xmin, xmax, C.byref(outer)), "mma_update")
         return x, outer.value
+
+    # --- learned preconditioner
+    def learned_setup(self, opts=None, **kw):
+        o = opts if opts is not None else learned_default_opts(**kw)
+        self._chk(self.L.icarat_hip_learned_setup(self.h, C.byref(o)), "learned_setup")
+
+    def learned_harvest(self, tol=1e-8, maxit=10000, precision=FP64, want_u=False):
+        o = SolveOpts(tol, maxit, precision, PRECOND_JACOBI, 0.1, 200, 0)
+        st = SolveStats()
+        u = np.zeros(self.numeq) if want_u else None
+        self._chk(self.L.icarat_hip_learned_harvest(self.h, C.byref(o),
+                                                    None if u is None else _p(u), C.byref(st)),
+                  "learned_harvest")
+        return u, st
+
+    def learned_train(self, steps):
+        loss = C.c_double()
+        self._chk(self.L.icarat_hip_learned_train(self.h, steps, C.byref(loss)), "learned_train")
+        return loss.value
+
+    def learned_info(self):
+        ns, nst, n = C.c_int32(), C.c_int64(), C.c_int64()
+        loss = C.c_double()
+        g = (C.c_int32 * 3)()
+        self._chk(self.L.icarat_hip_learned_info(self.h, C.byref(ns), C.byref(nst), C.byref(loss),
+                                                 C.byref(n), g), "learned_info")
+        return {"samples": ns.value, "steps": nst.value, "loss": loss.value,
+                "n_params": n.value, "grid": tuple(g)}
+
+    def learned_get_params(self):
+        out = np.zeros(unet_param_count(), dtype=np.float32)
+        self._chk(self.L.icarat_hip_learned_get_params(self.h, out.ctypes.data_as(fp)),
+                  "learned_get_params")
+        return out
+
+    def learned_set_params(self, flat):
+        a, p = _f32(flat)
+        self._chk(self.L.icarat_hip_learned_set_params(self.h, p), "learned_set_params")
+
+    def learned_apply(self, r):
+        z = np.zeros(self.numeq)
+        self._chk(self.L.icarat_hip_learned_apply(self.h, _p(_f64(r)), _p(z)), "learned_apply")
+        return z
 
     def time_kernel(self, kernel, reps=20):
         ms, fl, by = C.c_double(), C.c_double(), C.c_double()

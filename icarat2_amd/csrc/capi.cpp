@@ -3,13 +3,25 @@
 #include <cstring>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "../../include/icarat_hip.h"
 #include "context.hpp"
+#include "learned.hpp"
 #include "mma.hpp"
+#include "unet.hpp"
 
 struct icarat_hip_ctx {
   icarat_hip::Context *c = nullptr;
+  std::string err;
+};
+
+struct icarat_hip_unet {
+  hipStream_t s = nullptr;
+  icarat_hip::UNet *net = nullptr;
+  icarat_hip::NetGrid g{0, 0, 0};
+  icarat_hip::DevBuf<__bf16> x;
+  icarat_hip::DevBuf<float> xf, out, t;
   std::string err;
 };
 
@@ -262,6 +274,225 @@ int icarat_hip_simp_run(icarat_hip_ctx *h, const icarat_simp_opts *o, double *x,
 int icarat_hip_time_kernel(icarat_hip_ctx *h, int32_t kernel, int32_t reps, double *avg_ms,
                            double *flops, double *bytes) {
   return guard(h, [&](icarat_hip::Context &c) { c.time_kernel(kernel, reps, avg_ms, flops, bytes); });
+}
+
+// ---- learned preconditioner ------------------------------------------------
+int icarat_hip_learned_default_opts(icarat_learned_opts *o) {
+  if (!o) return ICARAT_HIP_INVALID_ARGUMENT;
+  *o = icarat_learned_opts{};
+  o->train_iters = 10;
+  o->train_steps = 400;
+  o->snapshot_every = 64;
+  o->max_samples = 160;
+  o->batch_iters = 8;
+  o->stall_iters = 400;
+  o->lr = 1e-3f;
+  o->seed = 1;
+  return ICARAT_HIP_OK;
+}
+
+int icarat_hip_learned_setup(icarat_hip_ctx *h, const icarat_learned_opts *o) {
+  icarat_learned_opts d;
+  icarat_hip_learned_default_opts(&d);
+  return guard(h, [&](icarat_hip::Context &c) { c.learned_setup(o ? *o : d); });
+}
+
+int icarat_hip_learned_harvest(icarat_hip_ctx *h, const icarat_solve_opts *o, double *u,
+                               icarat_solve_stats *st) {
+  if (!o) return fail(h, ICARAT_HIP_INVALID_ARGUMENT, "null options");
+  icarat_solve_stats s{};
+  int rc = guard(h, [&](icarat_hip::Context &c) { c.learned_harvest(*o, u, &s); });
+  if (st) *st = s;
+  if (rc == ICARAT_HIP_OK && s.status != ICARAT_HIP_OK) {
+    h->err = "iteration cap reached before the tolerance";
+    return s.status;
+  }
+  return rc;
+}
+
+int icarat_hip_learned_train(icarat_hip_ctx *h, int32_t steps, double *loss) {
+  if (steps < 0) return fail(h, ICARAT_HIP_INVALID_ARGUMENT, "steps must be >= 0");
+  return guard(h, [&](icarat_hip::Context &c) {
+    const double l = c.learned().train(steps);
+    if (loss) *loss = l;
+  });
+}
+
+int icarat_hip_learned_info(const icarat_hip_ctx *hc, int32_t *samples, int64_t *steps,
+                            double *loss, int64_t *n_params, int32_t grid[3]) {
+  icarat_hip_ctx *h = const_cast<icarat_hip_ctx *>(hc);
+  return guard(h, [&](icarat_hip::Context &c) {
+    icarat_hip::LearnedPrecond &lp = c.learned();
+    if (samples) *samples = lp.samples();
+    if (steps) *steps = lp.total_steps();
+    if (loss) *loss = lp.last_loss();
+    if (n_params) *n_params = icarat_hip::UNet::host_param_count();
+    if (grid) {
+      grid[0] = lp.grid().X;
+      grid[1] = lp.grid().Y;
+      grid[2] = lp.grid().Z;
+    }
+  });
+}
+
+int icarat_hip_learned_get_params(const icarat_hip_ctx *hc, float *params) {
+  icarat_hip_ctx *h = const_cast<icarat_hip_ctx *>(hc);
+  if (!params) return fail(h, ICARAT_HIP_INVALID_ARGUMENT, "null argument");
+  return guard(h, [&](icarat_hip::Context &c) { c.learned().net().get_params(params); });
+}
+
+int icarat_hip_learned_set_params(icarat_hip_ctx *h, const float *params) {
+  if (!params) return fail(h, ICARAT_HIP_INVALID_ARGUMENT, "null argument");
+  return guard(h, [&](icarat_hip::Context &c) { c.learned().net().set_params(params); });
+}
+
+int icarat_hip_learned_apply(icarat_hip_ctx *h, const double *r, double *z) {
+  if (!r || !z) return fail(h, ICARAT_HIP_INVALID_ARGUMENT, "null argument");
+  return guard(h, [&](icarat_hip::Context &c) { c.learned_apply(r, z); });
+}
+
+}  // extern "C"
+
+// ---- the network alone ------------------------------------------------------
+namespace {
+template <class F>
+int unet_guard(icarat_hip_unet *u, F &&f) {
+  if (!u || !u->net) return ICARAT_HIP_INVALID_ARGUMENT;
+  try {
+    f(*u);
+    u->err.clear();
+    return ICARAT_HIP_OK;
+  } catch (const icarat_hip::Error &e) {
+    u->err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    u->err = "host allocation failed";
+    return ICARAT_HIP_OUT_OF_MEMORY;
+  } catch (const std::exception &e) {
+    u->err = e.what();
+    return ICARAT_HIP_INTERNAL;
+  }
+}
+
+// host x[v*4 + c] -> device BF16 x[v*8 + c] (channels 4..7 zero)
+void unet_upload(icarat_hip_unet &u, const float *x) {
+  const long long nv = u.g.nvox();
+  std::vector<float> x8(nv * 8, 0.f);
+  for (long long v = 0; v < nv; ++v)
+    for (int c = 0; c < 4; ++c) x8[v * 8 + c] = x[v * 4 + c];
+  HIPCHK(hipMemcpyAsync(u.xf.p, x8.data(), nv * 8 * sizeof(float), hipMemcpyHostToDevice, u.s));
+  icarat_hip::launch_f32_to_bf16(nv * 8, u.xf, u.x, u.s);
+}
+
+double unet_loss(icarat_hip_unet &u) {
+  double l2[2];
+  HIPCHK(hipMemcpyAsync(l2, u.net->loss2(), 2 * sizeof(double), hipMemcpyDeviceToHost, u.s));
+  HIPCHK(hipStreamSynchronize(u.s));
+  return l2[1] > 0.0 ? 0.5 * l2[0] / l2[1] : 0.0;
+}
+}  // namespace
+
+extern "C" {
+
+int icarat_hip_unet_create(int32_t X, int32_t Y, int32_t Z, uint64_t seed, icarat_hip_unet **out) {
+  if (!out) return ICARAT_HIP_INVALID_ARGUMENT;
+  *out = nullptr;
+  if (X <= 0 || Y <= 0 || Z <= 0 || X % 4 || Y % 4 || Z % 4) return ICARAT_HIP_INVALID_ARGUMENT;
+  icarat_hip_unet *u = new (std::nothrow) icarat_hip_unet();
+  if (!u) return ICARAT_HIP_OUT_OF_MEMORY;
+  try {
+    HIPCHK(hipStreamCreateWithFlags(&u->s, hipStreamNonBlocking));
+    u->g = icarat_hip::NetGrid{X, Y, Z};
+    const long long nv = u->g.nvox();
+    u->net = new icarat_hip::UNet(u->g, u->s, seed);
+    u->x.alloc(nv * 8);
+    u->xf.alloc(nv * 8);
+    u->out.alloc(nv * 8);
+    u->t.alloc(nv * 3);
+  } catch (const icarat_hip::Error &e) {
+    delete u->net;
+    if (u->s) (void)hipStreamDestroy(u->s);
+    delete u;
+    return e.code;
+  } catch (...) {
+    delete u->net;
+    if (u->s) (void)hipStreamDestroy(u->s);
+    delete u;
+    return ICARAT_HIP_INTERNAL;
+  }
+  *out = u;
+  return ICARAT_HIP_OK;
+}
+
+int icarat_hip_unet_destroy(icarat_hip_unet *u) {
+  if (!u) return ICARAT_HIP_INVALID_ARGUMENT;
+  if (u->s) (void)hipStreamSynchronize(u->s);
+  delete u->net;
+  u->x.release();
+  u->xf.release();
+  u->out.release();
+  u->t.release();
+  if (u->s) (void)hipStreamDestroy(u->s);
+  delete u;
+  return ICARAT_HIP_OK;
+}
+
+const char *icarat_hip_unet_last_error(const icarat_hip_unet *u) { return u ? u->err.c_str() : "null network"; }
+
+int icarat_hip_unet_param_count(int64_t *n) {
+  if (!n) return ICARAT_HIP_INVALID_ARGUMENT;
+  *n = icarat_hip::UNet::host_param_count();
+  return ICARAT_HIP_OK;
+}
+
+int icarat_hip_unet_set_params(icarat_hip_unet *u, const float *p) {
+  if (!p) return ICARAT_HIP_INVALID_ARGUMENT;
+  return unet_guard(u, [&](icarat_hip_unet &n) { n.net->set_params(p); });
+}
+
+int icarat_hip_unet_get_params(const icarat_hip_unet *uc, float *p) {
+  if (!p) return ICARAT_HIP_INVALID_ARGUMENT;
+  icarat_hip_unet *u = const_cast<icarat_hip_unet *>(uc);
+  return unet_guard(u, [&](icarat_hip_unet &n) { n.net->get_params(p); });
+}
+
+int icarat_hip_unet_forward(icarat_hip_unet *u, const float *x, float *out) {
+  if (!x || !out) return ICARAT_HIP_INVALID_ARGUMENT;
+  return unet_guard(u, [&](icarat_hip_unet &n) {
+    const long long nv = n.g.nvox();
+    unet_upload(n, x);
+    n.net->forward(n.x, n.out);
+    std::vector<float> o8(nv * 8);
+    HIPCHK(hipMemcpyAsync(o8.data(), n.out.p, nv * 8 * sizeof(float), hipMemcpyDeviceToHost, n.s));
+    HIPCHK(hipStreamSynchronize(n.s));
+    for (long long v = 0; v < nv; ++v)
+      for (int c = 0; c < 3; ++c) out[v * 3 + c] = o8[v * 8 + c];
+  });
+}
+
+int icarat_hip_unet_grads(icarat_hip_unet *u, const float *x, const float *target, double *loss,
+                          float *grads) {
+  if (!x || !target) return ICARAT_HIP_INVALID_ARGUMENT;
+  return unet_guard(u, [&](icarat_hip_unet &n) {
+    unet_upload(n, x);
+    HIPCHK(hipMemcpyAsync(n.t.p, target, n.g.nvox() * 3 * sizeof(float), hipMemcpyHostToDevice, n.s));
+    n.net->backward_only(n.x, n.t);
+    const double l = unet_loss(n);
+    if (loss) *loss = l;
+    if (grads) n.net->get_grads(grads);
+  });
+}
+
+int icarat_hip_unet_train_step(icarat_hip_unet *u, const float *x, const float *target, float lr,
+                               double *loss) {
+  if (!x || !target || !(lr > 0.f)) return ICARAT_HIP_INVALID_ARGUMENT;
+  return unet_guard(u, [&](icarat_hip_unet &n) {
+    unet_upload(n, x);
+    HIPCHK(hipMemcpyAsync(n.t.p, target, n.g.nvox() * 3 * sizeof(float), hipMemcpyHostToDevice, n.s));
+    n.net->train_step(n.x, n.t, lr);
+    const double l = unet_loss(n);
+    if (loss) *loss = l;
+  });
 }
 
 int icarat_hip_gaussian_field(int32_t nx, int32_t ny, int32_t nz, double h, uint64_t seed,

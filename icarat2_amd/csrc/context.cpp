@@ -7,7 +7,8 @@
 #include <cstring>
 #include <functional>
 
-#include "mixed.hpp"
+#include "learned.hpp"
+#include "mixed_kernels.hpp"
 #include "mma.hpp"
 
 namespace icarat_hip {
@@ -48,6 +49,13 @@ void Context::ensure_device() {
   HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   d_ke.alloc(576);
   h2d(d_ke.p, prob_.ke0(), 576, stream_);
+  {
+    float k32[576];
+    for (int i = 0; i < 576; ++i) k32[i] = (float)prob_.ke0()[i];
+    d_ke32.alloc(576);
+    h2d(d_ke32.p, k32, 576, stream_);
+    sync();
+  }
   for (DevBuf<double> *b : {&d_f, &d_b, &d_dinv, &d_x, &d_r, &d_q, &d_p[0], &d_p[1], &d_u, &d_nv,
                             &d_y, &d_dirlat}) {
     b->alloc(n3_);
@@ -55,14 +63,13 @@ void Context::ensure_device() {
   }
   d_dens.alloc(L_.nelx);
   d_young.alloc(L_.nelx);
+  d_young32.alloc(L_.nelx);
   for (auto &b : d_e) {
     b.alloc(L_.nelx);
     HIPCHK(hipMemsetAsync(b.p, 0, b.bytes(), stream_));
   }
-  const long long stencil_blocks =
-      (long long)((L_.NX + 31) / 32) * ((L_.NY + 7) / 8) * ((L_.NZ + 3) / 4);
   const long long filter_blocks = (long long)((L_.nx + 31) / 32) * ((L_.ny + 7) / 8) * ((L_.nz + 3) / 4);
-  const long long nparts = std::max({stencil_blocks, filter_blocks, 4096LL}) * 4;
+  const long long nparts = std::max({stencil_blocks(L_), filter_blocks, 4096LL}) * 4;
   d_partials.alloc(nparts);
   d_counter.alloc(4);
   HIPCHK(hipMemsetAsync(d_counter.p, 0, d_counter.bytes(), stream_));
@@ -79,11 +86,12 @@ void Context::ensure_device() {
 
 Context::~Context() {
   if (!dev_ready_) return;
-  for (int k = 0; k < 2; ++k) {
-    if (pcg_exec_[k]) (void)hipGraphExecDestroy(pcg_exec_[k]);
-    if (pcg_graph_[k]) (void)hipGraphDestroy(pcg_graph_[k]);
-  }
-  mixed_.reset();
+  for (int m = 0; m < 3; ++m)
+    for (int k = 0; k < 2; ++k) {
+      if (pcg_exec_[m][k]) (void)hipGraphExecDestroy(pcg_exec_[m][k]);
+      if (pcg_graph_[m][k]) (void)hipGraphDestroy(pcg_graph_[m][k]);
+    }
+  learned_.reset();
   mma_.reset();
   if (stream_) (void)hipStreamSynchronize(stream_);
   // buffers release themselves; the stream goes last
@@ -157,10 +165,10 @@ void Context::set_density_device() {
   require_dof();
   const auto &g = prob_.spec();
   launch_young(d_dens, d_young, L_.nelx, g.young1, g.young2, g.penalty, stream_);
+  launch_f64_to_f32(L_.nelx, d_young, d_young32, stream_);
   launch_diag_inv(L_, d_ke, d_young, d_free, d_dinv, nullptr, stream_);
   density_ready_ = true;
   solved_ = false;
-  if (mixed_) mixed_->invalidate();
 }
 
 void Context::apply_k(const double *u, double *y, int precision) {
@@ -171,8 +179,8 @@ void Context::apply_k(const double *u, double *y, int precision) {
   if (precision == ICARAT_HIP_FP64) {
     launch_apply_k64(L_, d_ke, d_young, d_free, d_y, d_q, stream_);
   } else if (precision == ICARAT_HIP_FP32 || precision == ICARAT_HIP_BF16) {
-    if (!mixed_) mixed_.reset(new MixedSolver(*this));
-    mixed_->apply(d_y, d_q, precision);
+    launch_apply_k_mixed(L_, precision == ICARAT_HIP_BF16 ? 1 : 0, d_ke32, d_young32, d_free, d_y,
+                         d_q, stream_);
   } else {
     throw Error(kInvalidArgument, "unknown precision");
   }
@@ -190,15 +198,20 @@ void Context::get_diag(double *d) {
   sync();
 }
 
-void Context::build_pcg_graph() {
-  if (pcg_exec_[0]) return;
+void Context::build_pcg_graph(int kmode) {
+  const int m = kmode + 1;
+  if (pcg_exec_[m][0]) return;
   for (int par = 0; par < 2; ++par) {
     HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     try {
       for (int j = 0; j < kBatch; ++j) {
         double *p_old = d_p[(j + par) & 1], *p_new = d_p[(j + par + 1) & 1];
-        launch_pcg_stencil64(L_, d_ke, d_young, d_free, d_r, d_dinv, p_old, p_new, d_q, d_pcg,
-                             d_partials, d_counter, stream_);
+        if (kmode < 0)
+          launch_pcg_stencil64(L_, d_ke, d_young, d_free, d_r, d_dinv, p_old, p_new, d_q, d_pcg,
+                               d_partials, d_counter, stream_);
+        else
+          launch_pcg_stencil_mfma64(L_, kmode, d_ke32, d_young32, d_free, d_r, d_dinv, p_old, p_new,
+                                    d_q, d_pcg, d_partials, stream_);
         launch_pcg_update64(n3_, d_x, d_r, p_new, d_q, d_dinv, d_pcg, d_partials, d_counter,
                             stream_);
       }
@@ -208,17 +221,17 @@ void Context::build_pcg_graph() {
       if (g) (void)hipGraphDestroy(g);
       throw;
     }
-    HIPCHK(hipStreamEndCapture(stream_, &pcg_graph_[par]));
-    HIPCHK(hipGraphInstantiate(&pcg_exec_[par], pcg_graph_[par], nullptr, nullptr, 0));
+    HIPCHK(hipStreamEndCapture(stream_, &pcg_graph_[m][par]));
+    HIPCHK(hipGraphInstantiate(&pcg_exec_[m][par], pcg_graph_[m][par], nullptr, nullptr, 0));
   }
 }
 
-PcgScalars Context::pcg64(double tol, int maxit, bool warm) {
-  build_pcg_graph();
+PcgScalars Context::pcg64(double tol, int maxit, bool warm, int kmode, double delta) {
+  build_pcg_graph(kmode);
   PcgScalars init{};
   init.tol = tol;
   init.maxit = maxit;
-  init.delta = 0.0;
+  init.delta = delta;
   *h_pcg.p = init;
   h2d(d_pcg.p, h_pcg.p, 1, stream_);
   if (warm) d2d(d_nv.p, d_x.p, n3_, stream_);
@@ -231,25 +244,45 @@ PcgScalars Context::pcg64(double tol, int maxit, bool warm) {
   }
   d2h(h_pcg.p, d_pcg.p, 1, stream_);
   sync();
+  if (batch_hook_) batch_hook_();
   replacements_ = 0;
-  while (!h_pcg.p->done || h_pcg.p->status == 0) {
-    // iterate until the recursive stop test, a cap or a breakdown
-    const long long max_batches = (long long)(maxit - h_pcg.p->iter) / kBatch + 3;
-    for (long long b = 0; b < max_batches && !h_pcg.p->done; ++b) {
-      HIPCHK(hipGraphLaunch(pcg_exec_[h_pcg.p->iter & 1], stream_));
-      d2h(h_pcg.p, d_pcg.p, 1, stream_);
-      sync();
+  fell_back_ = 0;
+  double best = std::sqrt(h_pcg.p->rr);
+  int stalls = 0;
+  while (!h_pcg.p->done || h_pcg.p->status == 0 || h_pcg.p->status == 3) {
+    if (!h_pcg.p->done) {
+      // iterate until the recursive stop test, a replacement, a cap or a breakdown
+      const long long max_batches = (long long)(maxit - h_pcg.p->iter) / kBatch + 3;
+      for (long long b = 0; b < max_batches && !h_pcg.p->done; ++b) {
+        HIPCHK(hipGraphLaunch(pcg_exec_[kmode + 1][h_pcg.p->iter & 1], stream_));
+        d2h(h_pcg.p, d_pcg.p, 1, stream_);
+        sync();
+        if (batch_hook_ && !h_pcg.p->done) batch_hook_();
+      }
+      if (!h_pcg.p->done) throw Error(kInternal, "PCG did not reach its stop test");
     }
-    if (!h_pcg.p->done) throw Error(kInternal, "PCG did not reach its stop test");
-    if (h_pcg.p->status != 0) break;
-    // verified stop: the FP64 true residual must meet the tolerance too;
-    // otherwise it replaces the recursive one and the iteration continues
+    if (h_pcg.p->status != 0 && h_pcg.p->status != 3) break;
+    // the FP64 true residual replaces the recursive one (verified stop /
+    // reliable update); the search direction is kept
     launch_apply_k64(L_, d_ke, d_young, d_free, d_x, d_y, stream_);
     launch_replace64(n3_, d_b, d_y, d_r, d_dinv, d_pcg, d_partials, stream_);
     d2h(h_pcg.p, d_pcg.p, 1, stream_);
     sync();
     if (h_pcg.p->done) break;
-    if (++replacements_ > 20 || h_pcg.p->iter > maxit) {
+    ++replacements_;
+    const double rn = std::sqrt(h_pcg.p->rr);
+    if (rn < 0.5 * best) {
+      best = rn;
+      stalls = 0;
+    } else if (++stalls >= 4 && kmode >= 0) {
+      // the low-precision contraction stopped making progress: continue the
+      // same iteration with the FP64 contraction
+      kmode = -1;
+      fell_back_ = 1;
+      stalls = 0;
+      build_pcg_graph(kmode);
+    }
+    if (h_pcg.p->iter > maxit || replacements_ > 100000) {
       h_pcg.p->status = 1;
       break;
     }
@@ -294,16 +327,24 @@ void Context::solve(const icarat_solve_opts &o, double *u_host, icarat_solve_sta
   }
   icarat_solve_stats S{};
   S.status = ICARAT_HIP_OK;
-  if (o.precision == ICARAT_HIP_FP64 && o.precond == ICARAT_HIP_PRECOND_JACOBI) {
-    PcgScalars r = pcg64(o.tol, o.maxit);
+  if (o.precision != ICARAT_HIP_FP64 && o.precision != ICARAT_HIP_FP32 &&
+      o.precision != ICARAT_HIP_BF16)
+    throw Error(kInvalidArgument, "unknown precision");
+  if (o.precond == ICARAT_HIP_PRECOND_JACOBI) {
+    const int kmode = o.precision == ICARAT_HIP_FP64 ? -1 : (o.precision == ICARAT_HIP_FP32 ? 0 : 1);
+    const double delta = kmode < 0 ? 0.0 : (o.inner_tol > 0.0 && o.inner_tol < 1.0 ? o.inner_tol : 0.1);
+    PcgScalars r = pcg64(o.tol, o.maxit, false, kmode, delta);
     S.iters = r.iter;
     S.refine_steps = replacements_;
+    S.fallback = fell_back_;
     S.rec_relres = r.bnorm2 > 0 ? std::sqrt(r.rr) / std::sqrt(r.bnorm2) : 0.0;
     if (r.status == 2) throw Error(kNumerical, "PCG breakdown (non-positive curvature or NaN)");
     if (r.status == 1) S.status = ICARAT_HIP_NOT_CONVERGED;
+  } else if (o.precond == ICARAT_HIP_PRECOND_LEARNED) {
+    if (!learned_) throw Error(kNotReady, "the learned preconditioner is not set up");
+    learned_->solve(o, S);
   } else {
-    if (!mixed_) mixed_.reset(new MixedSolver(*this));
-    mixed_->solve(o, S);
+    throw Error(kInvalidArgument, "unknown preconditioner");
   }
   assemble_nodal_solution();
   S.true_relres = true_residual_device();
@@ -460,8 +501,35 @@ void Context::simp_run(const icarat_simp_opts &o, double *x, double *hist, icara
     double t1 = now_s();
     S.filter_s += t1 - t0;
     icarat_solve_stats ss{};
-    solve(o.solve, nullptr, &ss);
+    icarat_solve_opts so = o.solve;
+    // the first train_iters solves of the preconditioner's lifetime are solved
+    // by Jacobi-PCG and feed the training set
+    const bool harvest = so.precond == ICARAT_HIP_PRECOND_LEARNED && learned_ &&
+                         learned_->harvested() < learned_->opts().train_iters;
+    if (harvest) {
+      so.precond = ICARAT_HIP_PRECOND_JACOBI;
+      learned_->begin_harvest();
+      batch_hook_ = [this]() { learned_->snapshot(); };
+      try {
+        solve(so, nullptr, &ss);
+      } catch (...) {
+        batch_hook_ = nullptr;
+        learned_->end_harvest(false);
+        throw;
+      }
+      batch_hook_ = nullptr;
+      learned_->end_harvest(ss.status == ICARAT_HIP_OK);
+      const int ti = std::max(1, learned_->opts().train_iters);
+      const int h = learned_->harvested();  // 1..ti
+      const int steps = learned_->opts().train_steps / ti + (h <= learned_->opts().train_steps % ti ? 1 : 0);
+      const double tt = now_s();
+      learned_->train(steps);
+      S.train_s += now_s() - tt;
+    } else {
+      solve(so, nullptr, &ss);
+    }
     S.pcg_iters += ss.iters;
+    S.fallbacks += ss.fallback;
     S.max_true_relres = std::max(S.max_true_relres, ss.true_relres);
     double t2 = now_s();
     S.solve_s += t2 - t1;
@@ -509,6 +577,54 @@ void Context::simp_run(const icarat_simp_opts &o, double *x, double *hist, icara
   sync();
   S.seconds = now_s() - t_all;
   if (st) *st = S;
+}
+
+// ---- learned preconditioner --------------------------------------------------
+void Context::learned_setup(const icarat_learned_opts &o) {
+  require_dof();
+  if (o.train_iters < 0 || o.train_steps < 0 || o.snapshot_every < 1 || o.max_samples < 1 ||
+      o.batch_iters < 1 || o.batch_iters > 64 || o.stall_iters < 1 || !(o.lr > 0.f))
+    throw Error(kInvalidArgument, "invalid learned-preconditioner options");
+  learned_.reset();
+  learned_.reset(new LearnedPrecond(*this, o));
+}
+
+LearnedPrecond &Context::learned() {
+  if (!learned_) throw Error(kNotReady, "the learned preconditioner is not set up");
+  return *learned_;
+}
+
+void Context::learned_harvest(const icarat_solve_opts &o, double *u_host, icarat_solve_stats *st) {
+  LearnedPrecond &lp = learned();
+  require_density();
+  icarat_solve_opts so = o;
+  so.precond = ICARAT_HIP_PRECOND_JACOBI;
+  lp.begin_harvest();
+  batch_hook_ = [&lp]() { lp.snapshot(); };
+  icarat_solve_stats ss{};
+  try {
+    solve(so, u_host, &ss);
+  } catch (...) {
+    batch_hook_ = nullptr;
+    lp.end_harvest(false);
+    throw;
+  }
+  batch_hook_ = nullptr;
+  lp.end_harvest(ss.status == ICARAT_HIP_OK);
+  if (st) *st = ss;
+}
+
+void Context::learned_apply(const double *r, double *z) {
+  LearnedPrecond &lp = learned();
+  require_density();
+  const int numeq = prob_.numeq();
+  h2d(d_numeq[0].p, r, numeq, stream_);
+  // constrained DOFs are ignored by M (D^-1 is zero there)
+  launch_numeq_to_lattice(L_, d_dof, numeq, d_numeq[0], d_dirval, d_y, stream_);
+  lp.apply(d_y, d_nv);
+  launch_lattice_to_numeq(L_, d_dof, numeq, d_nv, d_numeq[1], stream_);
+  d2h(z, d_numeq[1].p, numeq, stream_);
+  sync();
 }
 
 // ---- measurement hook ------------------------------------------------------

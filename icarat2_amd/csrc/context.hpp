@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -14,8 +15,8 @@
 
 namespace icarat_hip {
 
-class MixedSolver;
 class DeviceMMA;
+class LearnedPrecond;
 
 class Context {
  public:
@@ -53,12 +54,20 @@ class Context {
               double xmax, int use_projection, double beta);
 
   void simp_run(const icarat_simp_opts &o, double *x, double *hist, icarat_simp_stats *st);
+
+  // learned preconditioner: set-up, a harvesting Jacobi solve, z = M(r) on numeq vectors
+  void learned_setup(const icarat_learned_opts &o);
+  LearnedPrecond &learned();
+  void learned_harvest(const icarat_solve_opts &o, double *u_host, icarat_solve_stats *st);
+  void learned_apply(const double *r, double *z);
   void time_kernel(int kernel, int reps, double *avg_ms, double *flops, double *bytes);
 
   // ---- device buffers (public for the solver components) ----
   Lattice L_;
   long long n3_ = 0;  // 3 * Np
   DevBuf<double> d_ke;                // KE0 (576)
+  DevBuf<float> d_ke32;               // KE0 in FP32 for the MFMA contraction
+  DevBuf<float> d_young32;            // SIMP young in FP32
   DevBuf<uint8_t> d_free;             // per node free mask
   DevBuf<int> d_dof;                  // numnp*3 reference DOF numbers
   DevBuf<double> d_dirval;            // numnp*3 Dirichlet values (AoS)
@@ -92,18 +101,26 @@ class Context {
   double vol_e_ = 1.0;
   std::string last_error;
 
-  // PCG iteration batch captured as a graph, one per parity of the p buffers
+  // PCG iteration batch captured as a graph, per contraction mode (FP64 VALU,
+  // FP32 MFMA, BF16 MFMA) and parity of the p buffers
   static constexpr int kBatch = 16;
-  hipGraphExec_t pcg_exec_[2] = {nullptr, nullptr};
-  hipGraph_t pcg_graph_[2] = {nullptr, nullptr};
-  int replacements_ = 0;  // residual replacements of the last FP64 solve
+  hipGraphExec_t pcg_exec_[3][2] = {};
+  hipGraph_t pcg_graph_[3][2] = {};
+  int replacements_ = 0;  // residual replacements of the last solve
+  int fell_back_ = 0;     // the last solve switched to the FP64 contraction
 
-  std::unique_ptr<MixedSolver> mixed_;
   std::unique_ptr<DeviceMMA> mma_;
+  std::unique_ptr<LearnedPrecond> learned_;
+  // called by pcg64 after its set-up and after every batch of iterations
+  std::function<void()> batch_hook_;
 
-  // FP64 Jacobi-PCG on d_b -> d_x (from x = 0, or from the current d_x when
-  // warm), stopped on the FP64 true residual; returns the final scalars.
-  PcgScalars pcg64(double tol, int maxit, bool warm = false);
+  // Jacobi-PCG on d_b -> d_x with FP64 vectors and scalars (from x = 0, or
+  // from the current d_x when warm).  kmode selects the contraction of K.p:
+  // -1 FP64 VALU, 0 FP32 MFMA, 1 BF16 MFMA.  delta > 0 replaces the recursive
+  // residual by the FP64 true residual whenever it has dropped by delta; the
+  // FP64 true residual is the stop test.  A low-precision contraction that
+  // stalls is switched to FP64 in place.
+  PcgScalars pcg64(double tol, int maxit, bool warm = false, int kmode = -1, double delta = 0.0);
   void assemble_nodal_solution();         // d_u = d_x + Dirichlet values
   void sync() { HIPCHK(hipStreamSynchronize(stream_)); }
   // Device buffers are allocated on first device use, so mesh, numbering,
@@ -118,7 +135,7 @@ class Context {
   void upload_dof();
   void require_dof();
   void require_density();
-  void build_pcg_graph();
+  void build_pcg_graph(int kmode);
 };
 
 }  // namespace icarat_hip

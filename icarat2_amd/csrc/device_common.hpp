@@ -39,7 +39,10 @@ struct PcgScalars {
   double repl_norm;  // current replacement threshold (0: none)
   int iter, maxit;
   int done;          // 1 once converged, capped, broken down or a replacement is due
-  int status;        // 0 running/converged, 1 capped, 2 breakdown, 3 replacement due
+  int status;        // 0 running/converged, 1 capped, 2 breakdown, 3 replacement due,
+                     // 4 preconditioned residual not a descent direction
+  int restart;       // flexible PCG: next beta = 0
+  int pad_;
 };
 
 // Scalars of the device-resident OC bisection.

@@ -11,6 +11,8 @@
 // is deterministic and each node is written exactly once.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels.hpp"
 
 namespace icarat_hip {
@@ -136,6 +138,218 @@ __global__ __launch_bounds__(NT) void k_stencil64(
       y[2 * Np + n] = out[2];
       if constexpr (PCG) pq += U[13][0] * out[0] + U[13][1] * out[1] + U[13][2] * out[2];
     }
+  }
+  if constexpr (PCG) {
+    double v[1] = {pq};
+    store_partials<NT, 1>(v, partials, sh);
+  }
+}
+
+// ---- z-marching variant ----------------------------------------------------
+// A workgroup owns a ZX x ZY column of nodes and marches over the element
+// layers in z.  Each thread keeps the 3x3 node neighbourhood of two planes
+// (the layer's bottom and top) in registers; the layer's 4 elements around
+// the thread's node column give the last contribution of the bottom node
+// (finished and written) and the first of the top node (kept in registers).
+// The next plane and element layer are fetched from global memory before the
+// layer is contracted and reach a double-buffered LDS ring afterwards.
+constexpr int ZX = 32, ZY = 8;
+constexpr int ZPX = ZX + 2, ZPY = ZY + 2, ZPN = ZPX * ZPY;
+constexpr int ZLX = ZX + 1, ZLY = ZY + 1, ZLN = ZLX * ZLY;
+constexpr int ZPER = (ZPN + NT - 1) / NT;  // plane nodes per thread
+constexpr int ZEPER = (ZLN + NT - 1) / NT; // layer elements per thread
+
+template <bool PCG>
+struct PlaneFetch {
+  double v[ZPER][3];
+  double d[ZPER][3], pold[ZPER][3];  // PCG: dinv and p_old (v holds r)
+  double e[ZEPER];
+};
+
+template <bool PCG>
+__device__ __forceinline__ void fetch_plane(const Lattice &L, int x0, int y0, int zp,
+                                            const double *__restrict__ src,
+                                            const double *__restrict__ dinv,
+                                            const double *__restrict__ p_old,
+                                            const double *__restrict__ young, PlaneFetch<PCG> &F) {
+#pragma unroll
+  for (int k = 0; k < ZPER; ++k) {
+    const int t = threadIdx.x + k * NT;
+    const int gx = x0 - 1 + t % ZPX, gy = y0 - 1 + t / ZPX;
+    const bool in = t < ZPN && zp >= 0 && zp < L.NZ && gx >= 0 && gx < L.NX && gy >= 0 && gy < L.NY;
+    const long long n = in ? gx + (long long)L.NX * (gy + (long long)L.NY * zp) : 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      F.v[k][c] = in ? src[c * L.Np + n] : 0.0;
+      if constexpr (PCG) {
+        F.d[k][c] = in ? dinv[c * L.Np + n] : 0.0;
+        F.pold[k][c] = in ? p_old[c * L.Np + n] : 0.0;
+      }
+    }
+  }
+  // element layer zp-1 (between node planes zp-1 and zp)
+  const int ze = zp - 1;
+#pragma unroll
+  for (int k = 0; k < ZEPER; ++k) {
+    const int t = threadIdx.x + k * NT;
+    const int ex = x0 - 1 + t % ZLX, ey = y0 - 1 + t / ZLX;
+    const bool in = t < ZLN && ze >= 0 && ze < L.nz && ex >= 0 && ex < L.nx && ey >= 0 && ey < L.ny;
+    const long long ei = in ? ex + (long long)L.nx * (ey + (long long)L.ny * ze) : 0;
+    F.e[k] = in ? young[ei] : 0.0;
+  }
+}
+
+// Writes a fetched plane (and layer) into LDS; PCG: forms p = dinv r + beta
+// p_old and stores it to p_new for owned nodes.
+template <bool PCG>
+__device__ __forceinline__ void commit_plane(const Lattice &L, int x0, int y0, int zp, bool own_z,
+                                             double beta, const PlaneFetch<PCG> &F,
+                                             double *__restrict__ p_new, double (*sp)[ZPN],
+                                             double *se) {
+#pragma unroll
+  for (int k = 0; k < ZPER; ++k) {
+    const int t = threadIdx.x + k * NT;
+    if (t >= ZPN) continue;
+    double v[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      if constexpr (PCG) v[c] = F.d[k][c] * F.v[k][c] + beta * F.pold[k][c];
+      else v[c] = F.v[k][c];
+      sp[c][t] = v[c];
+    }
+    if constexpr (PCG) {
+      const int hx = t % ZPX, hy = t / ZPX;
+      const int gx = x0 - 1 + hx, gy = y0 - 1 + hy;
+      if (own_z && hx >= 1 && hx <= ZX && hy >= 1 && hy <= ZY && gx < L.NX && gy < L.NY) {
+        const long long n = gx + (long long)L.NX * (gy + (long long)L.NY * zp);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) p_new[c * L.Np + n] = v[c];
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < ZEPER; ++k) {
+    const int t = threadIdx.x + k * NT;
+    if (t < ZLN) se[t] = F.e[k];
+  }
+}
+
+// 3x3 neighbourhood of the thread's column in one LDS plane
+__device__ __forceinline__ void read_nbhd(const double (*sp)[ZPN], int tx, int ty, double (&U)[9][3]) {
+#pragma unroll
+  for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+    for (int dx = -1; dx <= 1; ++dx) {
+      const int h = (tx + 1 + dx) + ZPX * (ty + 1 + dy);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) U[(dx + 1) + 3 * (dy + 1)][c] = sp[c][h];
+    }
+}
+
+// Contributions of the 4 elements of one layer around the node column:
+// bot = rows of the bottom node (local z = 0), top = rows of the top node.
+__device__ __forceinline__ void layer_contract(const double *__restrict__ c_KE0,
+                                               const double (&Ulo)[9][3], const double (&Uhi)[9][3],
+                                               const double (&E)[4], double (&bot)[3],
+                                               double (&top)[3]) {
+  bot[0] = bot[1] = bot[2] = 0.0;
+  top[0] = top[1] = top[2] = 0.0;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int ex = (s & 1) - 1, ey = (s >> 1) - 1;
+    const int ab = lnode(-ex, -ey, 0), at = lnode(-ex, -ey, 1);
+    double tb0 = 0.0, tb1 = 0.0, tb2 = 0.0, tt0 = 0.0, tt1 = 0.0, tt2 = 0.0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const int o = (ex + lnode_x(b) + 1) + 3 * (ey + lnode_y(b) + 1);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const double u = lnode_z(b) ? Uhi[o][j] : Ulo[o][j];
+        tb0 += c_KE0[(3 * ab + 0) * 24 + 3 * b + j] * u;
+        tb1 += c_KE0[(3 * ab + 1) * 24 + 3 * b + j] * u;
+        tb2 += c_KE0[(3 * ab + 2) * 24 + 3 * b + j] * u;
+        tt0 += c_KE0[(3 * at + 0) * 24 + 3 * b + j] * u;
+        tt1 += c_KE0[(3 * at + 1) * 24 + 3 * b + j] * u;
+        tt2 += c_KE0[(3 * at + 2) * 24 + 3 * b + j] * u;
+      }
+    }
+    bot[0] += E[s] * tb0;
+    bot[1] += E[s] * tb1;
+    bot[2] += E[s] * tb2;
+    top[0] += E[s] * tt0;
+    top[1] += E[s] * tt1;
+    top[2] += E[s] * tt2;
+  }
+}
+
+template <bool PCG>
+__global__ __launch_bounds__(NT) void k_stencil64z(
+    Lattice L, int zchunk, const double *__restrict__ c_KE0, const double *__restrict__ young,
+    const uint8_t *__restrict__ fmask, const double *__restrict__ src,
+    const double *__restrict__ dinv, const double *__restrict__ p_old, double *__restrict__ p_new,
+    double *__restrict__ y, PcgScalars *S, double *partials) {
+  __shared__ double sp[2][3][ZPN];
+  __shared__ double se[2][ZLN];
+  __shared__ double sh[4];
+  double beta = 0.0;
+  if constexpr (PCG) {
+    if (S->done) return;
+    beta = S->beta;
+  }
+  const int x0 = blockIdx.x * ZX, y0 = blockIdx.y * ZY, z0 = blockIdx.z * zchunk;
+  const int z1 = min(z0 + zchunk, L.NZ);
+  const int tx = threadIdx.x % ZX, ty = threadIdx.x / ZX;
+  const int gx = x0 + tx, gy = y0 + ty;
+  const bool col_in = gx < L.NX && gy < L.NY;
+  PlaneFetch<PCG> F;
+  double Ulo[9][3], Uhi[9][3];
+  // Slot s holds node plane zb+1 and element layer zb of the step that reads
+  // it.  Plane z0-1 goes to registers through slot 1.
+  fetch_plane<PCG>(L, x0, y0, z0 - 1, src, dinv, p_old, young, F);
+  commit_plane<PCG>(L, x0, y0, z0 - 1, false, beta, F, p_new, sp[1], se[1]);
+  __syncthreads();
+  read_nbhd(sp[1], tx, ty, Ulo);
+  __syncthreads();
+  // plane z0 and layer z0-1 -> slot 0 for step 0
+  fetch_plane<PCG>(L, x0, y0, z0, src, dinv, p_old, young, F);
+  commit_plane<PCG>(L, x0, y0, z0, z0 < z1, beta, F, p_new, sp[0], se[0]);
+  __syncthreads();
+  double acc[3] = {0.0, 0.0, 0.0};
+  double pq = 0.0;
+  const int nsteps = z1 - z0 + 1;  // element layers z0-1 .. z1-1
+  for (int k = 0; k < nsteps; ++k) {
+    const int zb = z0 - 1 + k;
+    const int cur = k & 1, nxt = cur ^ 1;
+    const bool more = k + 1 < nsteps;
+    // prefetch plane zb+2 with layer zb+1 for the next step
+    if (more) fetch_plane<PCG>(L, x0, y0, zb + 2, src, dinv, p_old, young, F);
+    read_nbhd(sp[cur], tx, ty, Uhi);
+    double E[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int ex = (s & 1) - 1, ey = (s >> 1) - 1;
+      E[s] = se[cur][(tx + 1 + ex) + ZLX * (ty + 1 + ey)];
+    }
+    double bot[3], top[3];
+    layer_contract(c_KE0, Ulo, Uhi, E, bot, top);
+    if (k >= 1 && col_in) {
+      const long long n = gx + (long long)L.NX * (gy + (long long)L.NY * zb);
+      const unsigned m = fmask[n];
+      double out[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) out[c] = ((m >> c) & 1u) ? acc[c] + bot[c] : 0.0;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) y[c * L.Np + n] = out[c];
+      if constexpr (PCG) pq += Ulo[4][0] * out[0] + Ulo[4][1] * out[1] + Ulo[4][2] * out[2];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) acc[c] = top[c];
+#pragma unroll
+    for (int o = 0; o < 9; ++o)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) Ulo[o][c] = Uhi[o][c];
+    if (more) commit_plane<PCG>(L, x0, y0, zb + 2, zb + 2 < z1, beta, F, p_new, sp[nxt], se[nxt]);
+    __syncthreads();
   }
   if constexpr (PCG) {
     double v[1] = {pq};
@@ -361,8 +575,33 @@ static dim3 stencil_grid(const Lattice &L) {
   return dim3((L.NX + SX - 1) / SX, (L.NY + SY - 1) / SY, (L.NZ + SZ - 1) / SZ);
 }
 
+// z-chunk of the z-marching kernel: enough workgroups to fill the chip
+int stencil_zchunk(const Lattice &L) {
+  const long long cols = (long long)((L.NX + ZX - 1) / ZX) * ((L.NY + ZY - 1) / ZY);
+  int zc = 32;
+  while (zc > 4 && cols * ((L.NZ + zc - 1) / zc) < 1536) zc /= 2;
+  return zc;
+}
+
+static dim3 zgrid(const Lattice &L, int zc) {
+  return dim3((L.NX + ZX - 1) / ZX, (L.NY + ZY - 1) / ZY, (L.NZ + zc - 1) / zc);
+}
+
+long long stencil_blocks(const Lattice &L) {
+  const dim3 a = stencil_grid(L), b = zgrid(L, stencil_zchunk(L));
+  return std::max((long long)a.x * a.y * a.z, (long long)b.x * b.y * b.z);
+}
+
 void launch_apply_k64(const Lattice &L, const double *ke, const double *young,
                       const uint8_t *freemask, const double *u, double *y, hipStream_t s) {
+  const int zc = stencil_zchunk(L);
+  hipLaunchKernelGGL(k_stencil64z<false>, zgrid(L, zc), dim3(NT), 0, s, L, zc, ke, young, freemask,
+                     u, nullptr, nullptr, nullptr, y, nullptr, nullptr);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_apply_k64_tiled(const Lattice &L, const double *ke, const double *young,
+                            const uint8_t *freemask, const double *u, double *y, hipStream_t s) {
   hipLaunchKernelGGL(k_stencil64<false>, stencil_grid(L), dim3(NT), 0, s, L, ke, young, freemask, u,
                      nullptr, nullptr, nullptr, y, nullptr, nullptr, nullptr);
   HIPCHK(hipGetLastError());
@@ -372,9 +611,11 @@ void launch_pcg_stencil64(const Lattice &L, const double *ke, const double *youn
                           const uint8_t *freemask, const double *r, const double *dinv,
                           const double *p_old, double *p_new, double *q, PcgScalars *S,
                           double *partials, unsigned *counter, hipStream_t s, bool finalize) {
-  const dim3 g = stencil_grid(L);
-  hipLaunchKernelGGL(k_stencil64<true>, g, dim3(NT), 0, s, L, ke, young, freemask, r,
-                     dinv, p_old, p_new, q, S, partials, counter);
+  (void)counter;
+  const int zc = stencil_zchunk(L);
+  const dim3 g = zgrid(L, zc);
+  hipLaunchKernelGGL(k_stencil64z<true>, g, dim3(NT), 0, s, L, zc, ke, young, freemask, r, dinv,
+                     p_old, p_new, q, S, partials);
   HIPCHK(hipGetLastError());
   if (finalize) launch_finalize(kFinPcgAlpha, 1, partials, g.x * g.y * g.z, S, nullptr, nullptr, s);
 }

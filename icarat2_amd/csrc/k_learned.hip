@@ -1,0 +1,328 @@
+// k_learned.hip -- transfers between the node lattice and the U-Net's cell
+// grid, and the flexible-PCG reductions of the learned preconditioner.
+//
+// Cell (I,J,K) of the network grid covers 2x2x2 elements; its centre is node
+// (2I+1, 2J+1, 2K+1).  Restriction is full weighting over the 3x3x3 nodes
+// around the centre (1-D weights 1/2, 1, 1/2; normalised by 8), prolongation
+// is its transpose (trilinear interpolation from cell centres).
+#include <hip/hip_runtime.h>
+
+#include "learned_kernels.hpp"
+
+namespace icarat_hip {
+
+namespace {
+
+constexpr int NT = 256;
+constexpr int VEC_BLOCKS = 2048;
+
+inline unsigned blocks_for(long long n, int nt) { return (unsigned)((n + nt - 1) / nt); }
+
+__device__ __forceinline__ double w1(int d) { return d == 0 ? 1.0 : 0.5; }
+
+// s[cell][c] = R(dinv r)  (scale 1); partial sums of s^2
+__global__ __launch_bounds__(NT) void k_restrict_residual(Lattice L, NetGrid g, const double *__restrict__ r,
+                                                          const double *__restrict__ dinv,
+                                                          float *__restrict__ s3, double *partials) {
+  __shared__ double sh[4];
+  double acc = 0.0;
+  const long long ncell = g.nvox();
+  for (long long v = blockIdx.x * (long long)NT + threadIdx.x; v < ncell; v += (long long)gridDim.x * NT) {
+    const int I = (int)(v % g.X), J = (int)((v / g.X) % g.Y), K = (int)(v / ((long long)g.X * g.Y));
+    const int cx = 2 * I + 1, cy = 2 * J + 1, cz = 2 * K + 1;
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int dz = -1; dz <= 1; ++dz) {
+      const int z = cz + dz;
+      if (z < 0 || z >= L.NZ) continue;
+      for (int dy = -1; dy <= 1; ++dy) {
+        const int y = cy + dy;
+        if (y < 0 || y >= L.NY) continue;
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int x = cx + dx;
+          if (x < 0 || x >= L.NX) continue;
+          const double w = w1(dx) * w1(dy) * w1(dz) * 0.125;
+          const long long n = x + (long long)L.NX * (y + (long long)L.NY * z);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) s[c] += w * dinv[c * L.Np + n] * r[c * L.Np + n];
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      s3[v * 3 + c] = (float)s[c];
+      acc += s[c] * s[c];
+    }
+  }
+  double vv[1] = {acc};
+  store_partials<NT, 1>(vv, partials, sh);
+}
+
+// R(x) per cell (unscaled), for the training snapshots
+__global__ void k_restrict_vec(Lattice L, NetGrid g, const double *__restrict__ x,
+                               float *__restrict__ out3) {
+  const long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (v >= g.nvox()) return;
+  const int I = (int)(v % g.X), J = (int)((v / g.X) % g.Y), K = (int)(v / ((long long)g.X * g.Y));
+  const int cx = 2 * I + 1, cy = 2 * J + 1, cz = 2 * K + 1;
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int dz = -1; dz <= 1; ++dz) {
+    const int z = cz + dz;
+    if (z < 0 || z >= L.NZ) continue;
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int y = cy + dy;
+      if (y < 0 || y >= L.NY) continue;
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int xx = cx + dx;
+        if (xx < 0 || xx >= L.NX) continue;
+        const double w = w1(dx) * w1(dy) * w1(dz) * 0.125;
+        const long long n = xx + (long long)L.NX * (y + (long long)L.NY * z);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) s[c] += w * x[c * L.Np + n];
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) out3[v * 3 + c] = (float)s[c];
+}
+
+// scale[0] = sigma = RMS of s over the cells, scale[1] = 1 / sigma (0 if sigma == 0)
+__global__ void k_sigma(const double *sum2, long long ncell, double *scale) {
+  const double sig = sqrt(sum2[0] / (double)(3 * ncell));
+  scale[0] = sig;
+  scale[1] = sig > 0.0 ? 1.0 / sig : 0.0;
+}
+
+// network input: [density feature, s / sigma (3), 0 x 4] in BF16
+__global__ void k_pack_input(NetGrid g, const float *__restrict__ feat, const float *__restrict__ s3,
+                             const double *__restrict__ scale, __bf16 *__restrict__ xin,
+                             float *__restrict__ zr) {
+  const long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (v >= g.nvox()) return;
+  const float inv = (float)scale[1];
+  xin[v * 8 + 0] = (__bf16)feat[v];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const float a = s3[v * 3 + c] * inv;
+    xin[v * 8 + 1 + c] = (__bf16)a;
+    if (zr) zr[v * 3 + c] = a;
+  }
+#pragma unroll
+  for (int c = 4; c < 8; ++c) xin[v * 8 + c] = (__bf16)0.0f;
+}
+
+// log-stiffness feature of each cell: (log10 mean young + 9) / 9 clamped to [0,1]
+__global__ void k_density_feature(Lattice L, NetGrid g, const double *__restrict__ young,
+                                  float *__restrict__ feat) {
+  const long long v = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (v >= g.nvox()) return;
+  const int I = (int)(v % g.X), J = (int)((v / g.X) % g.Y), K = (int)(v / ((long long)g.X * g.Y));
+  double s = 0.0;
+  int cnt = 0;
+  for (int k = 0; k < 8; ++k) {
+    const int ex = 2 * I + (k & 1), ey = 2 * J + ((k >> 1) & 1), ez = 2 * K + (k >> 2);
+    if (ex < L.nx && ey < L.ny && ez < L.nz) {
+      s += young[ex + (long long)L.nx * (ey + (long long)L.ny * ez)];
+      ++cnt;
+    }
+  }
+  float f = 0.f;
+  if (cnt > 0 && s > 0.0) f = (float)((log10(s / cnt) + 9.0) / 9.0);
+  feat[v] = fminf(fmaxf(f, 0.f), 1.f);
+}
+
+// z = dinv r + sigma * P(out) on free DOFs
+__global__ void k_prolong_add(Lattice L, NetGrid g, const float *__restrict__ out, int cs,
+                              const double *__restrict__ scale, const double *__restrict__ r,
+                              const double *__restrict__ dinv, const uint8_t *__restrict__ fmask,
+                              double *__restrict__ z, int use_net) {
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (n >= L.numnp) return;
+  const int x = (int)(n % L.NX), y = (int)((n / L.NX) % L.NY), zz = (int)(n / ((long long)L.NX * L.NY));
+  double corr[3] = {0.0, 0.0, 0.0};
+  if (use_net) {
+    // cells whose centre is within one node of n, per dimension
+    int ci[3][2], nc[3];
+    double wc[3][2];
+    const int co[3] = {x, y, zz};
+    const int gd[3] = {g.X, g.Y, g.Z};
+    for (int d = 0; d < 3; ++d) {
+      nc[d] = 0;
+      if (co[d] & 1) {
+        const int c0 = (co[d] - 1) / 2;
+        if (c0 < gd[d]) { ci[d][0] = c0; wc[d][0] = 1.0; nc[d] = 1; }
+      } else {
+        const int c0 = co[d] / 2 - 1, c1 = co[d] / 2;
+        if (c0 >= 0 && c0 < gd[d]) { ci[d][nc[d]] = c0; wc[d][nc[d]] = 0.5; ++nc[d]; }
+        if (c1 < gd[d]) { ci[d][nc[d]] = c1; wc[d][nc[d]] = 0.5; ++nc[d]; }
+      }
+    }
+    for (int a = 0; a < nc[0]; ++a)
+      for (int b = 0; b < nc[1]; ++b)
+        for (int c = 0; c < nc[2]; ++c) {
+          const double w = wc[0][a] * wc[1][b] * wc[2][c];
+          const long long v = ci[0][a] + (long long)g.X * (ci[1][b] + (long long)g.Y * ci[2][c]);
+#pragma unroll
+          for (int k = 0; k < 3; ++k) corr[k] += w * (double)out[v * cs + k];
+        }
+  }
+  const double sig = scale[0];
+  const unsigned m = fmask[n];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const long long q = k * L.Np + n;
+    z[q] = ((m >> k) & 1u) ? dinv[q] * r[q] + sig * corr[k] : 0.0;
+  }
+}
+
+// target = (R x* - R x_k) / sigma_k - zr_k
+__global__ void k_make_target(long long n, const float *__restrict__ rxs, const float *__restrict__ rxk,
+                              double inv_sigma, const float *__restrict__ zr, float *__restrict__ t) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) t[i] = (float)(((double)rxs[i] - (double)rxk[i]) * inv_sigma) - zr[i];
+}
+
+// rho' = z.r and gamma = z.q (flexible PCG)
+__global__ __launch_bounds__(NT) void k_zdot(long long n, const double *__restrict__ z,
+                                             const double *__restrict__ r,
+                                             const double *__restrict__ q, PcgScalars *S,
+                                             double *partials) {
+  __shared__ double sh[8];
+  if (S->done) return;
+  double a = 0.0, b = 0.0;
+  const long long stride = (long long)gridDim.x * NT;
+  for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += stride) {
+    a += z[i] * r[i];
+    b += z[i] * q[i];
+  }
+  double v[2] = {a, b};
+  store_partials<NT, 2>(v, partials, sh);
+}
+
+__global__ void k_fpcg_beta(const double *__restrict__ partials, unsigned G, PcgScalars *S) {
+  __shared__ double sh[8];
+  if (S->done) return;
+  double tot[2];
+  sum_partials<NT, 2>(partials, G, tot, sh);
+  if (threadIdx.x != 0) return;
+  const double rho1 = tot[0], gam = tot[1];
+  if (!(rho1 > 0.0) || !isfinite(rho1) || !isfinite(gam)) {
+    S->status = 4;  // the preconditioned residual is not a descent direction
+    S->done = 1;
+    return;
+  }
+  // Polak-Ribiere: beta = z'.(r' - r) / rho with r - r' = alpha q
+  S->beta = S->restart ? 0.0 : -S->alpha * gam / S->rho;
+  S->restart = 0;
+  S->rho = rho1;
+}
+
+// x += alpha p; r -= alpha q; r.r (FPCG: rho comes from the preconditioned residual)
+__global__ __launch_bounds__(NT) void k_fpcg_update(long long n, double *__restrict__ x,
+                                                    double *__restrict__ r,
+                                                    const double *__restrict__ p,
+                                                    const double *__restrict__ q, PcgScalars *S,
+                                                    double *partials) {
+  __shared__ double sh[8];
+  if (S->done) return;
+  const double alpha = S->alpha;
+  double rr = 0.0;
+  const long long stride = (long long)gridDim.x * NT;
+  for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += stride) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    rr += ri * ri;
+  }
+  double v[1] = {rr};
+  store_partials<NT, 1>(v, partials, sh);
+}
+
+// stop tests of the flexible PCG after the update (rr only)
+__global__ void k_fpcg_check(const double *__restrict__ partials, unsigned G, PcgScalars *S) {
+  __shared__ double sh[8];
+  if (S->done) return;
+  double tot[1];
+  sum_partials<NT, 1>(partials, G, tot, sh);
+  if (threadIdx.x != 0) return;
+  S->rr = tot[0];
+  S->iter += 1;
+  const double rnorm = sqrt(tot[0]);
+  if (!isfinite(rnorm)) {
+    S->status = 2;
+    S->done = 1;
+  } else if (rnorm < S->tol * sqrt(S->bnorm2)) {
+    S->status = 0;
+    S->done = 1;
+  } else if (S->iter > S->maxit) {
+    S->status = 1;
+    S->done = 1;
+  } else if (rnorm < S->repl_norm) {
+    S->status = 3;
+    S->done = 1;
+  }
+}
+
+}  // namespace
+
+void launch_restrict_residual(const Lattice &L, const NetGrid &g, const double *r, const double *dinv,
+                              float *s3, double *partials, double *scale, hipStream_t s) {
+  hipLaunchKernelGGL(k_restrict_residual, dim3(VEC_BLOCKS), dim3(NT), 0, s, L, g, r, dinv, s3,
+                     partials);
+  HIPCHK(hipGetLastError());
+  launch_finalize(kFinOut, 1, partials, VEC_BLOCKS, nullptr, nullptr, scale + 2, s);
+  hipLaunchKernelGGL(k_sigma, dim3(1), dim3(1), 0, s, scale + 2, g.nvox(), scale);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_restrict_vec(const Lattice &L, const NetGrid &g, const double *x, float *out3,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_restrict_vec, dim3(blocks_for(g.nvox(), 256)), dim3(256), 0, s, L, g, x, out3);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_pack_input(const NetGrid &g, const float *feat, const float *s3, const double *scale,
+                       __bf16 *xin, float *zr, hipStream_t s) {
+  hipLaunchKernelGGL(k_pack_input, dim3(blocks_for(g.nvox(), 256)), dim3(256), 0, s, g, feat, s3,
+                     scale, xin, zr);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_density_feature(const Lattice &L, const NetGrid &g, const double *young, float *feat,
+                            hipStream_t s) {
+  hipLaunchKernelGGL(k_density_feature, dim3(blocks_for(g.nvox(), 256)), dim3(256), 0, s, L, g,
+                     young, feat);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_prolong_add(const Lattice &L, const NetGrid &g, const float *out, int cs,
+                        const double *scale, const double *r, const double *dinv,
+                        const uint8_t *fmask, double *z, int use_net, hipStream_t s) {
+  hipLaunchKernelGGL(k_prolong_add, dim3(blocks_for(L.numnp, 256)), dim3(256), 0, s, L, g, out, cs,
+                     scale, r, dinv, fmask, z, use_net);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_make_target(long long n, const float *rxs, const float *rxk, double inv_sigma,
+                        const float *zr, float *t, hipStream_t s) {
+  hipLaunchKernelGGL(k_make_target, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, rxs, rxk,
+                     inv_sigma, zr, t);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_zdot_beta(long long n, const double *z, const double *r, const double *q, PcgScalars *S,
+                      double *partials, hipStream_t s) {
+  hipLaunchKernelGGL(k_zdot, dim3(VEC_BLOCKS), dim3(NT), 0, s, n, z, r, q, S, partials);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_fpcg_beta, dim3(1), dim3(NT), 0, s, partials, (unsigned)VEC_BLOCKS, S);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_fpcg_update(long long n, double *x, double *r, const double *p, const double *q,
+                        PcgScalars *S, double *partials, hipStream_t s) {
+  hipLaunchKernelGGL(k_fpcg_update, dim3(VEC_BLOCKS), dim3(NT), 0, s, n, x, r, p, q, S, partials);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_fpcg_check, dim3(1), dim3(NT), 0, s, partials, (unsigned)VEC_BLOCKS, S);
+  HIPCHK(hipGetLastError());
+}
+
+}  // namespace icarat_hip

@@ -1,5 +1,10 @@
-// k_mixed.hip -- low-precision K·u on the matrix cores (MFMA) and the FP32
-// inner Jacobi-PCG that runs inside FP64 iterative refinement.
+// k_mixed.hip -- K·u with the element contraction on the matrix cores (MFMA)
+// in FP32 or BF16 with FP32 accumulation: the matvec of the mixed-precision
+// Jacobi-PCG, whose vectors, scalars and reductions stay FP64 and whose FP64
+// true residual periodically replaces the recursive one.
+//
+// The vectors stay FP64 (the kernel reads FP64 r, dinv, p_old and writes FP64
+// p and q); only the staged node planes and the contraction are FP32 / BF16.
 //
 // Element-centric contraction: W = KE0 · U, with KE0 (24x24, rows padded to
 // 32) as the MFMA A operand held in registers for the whole kernel and U the
@@ -131,16 +136,16 @@ __device__ __forceinline__ void contract_layer(const KeOperand<MODE> &K, const f
 // Loads node plane z of the window into LDS (3 components, zero outside the
 // mesh).  PCG: the plane is the new search direction p = dinv*r + beta*p_old,
 // written back for owned nodes.
-template <bool PCG>
+template <bool PCG, class VT>
 __device__ __forceinline__ void load_plane(const Lattice &L, int x0, int y0, int z, bool own_z,
-                                           const float *__restrict__ src,
-                                           const float *__restrict__ dinv,
-                                           const float *__restrict__ p_old,
-                                           float *__restrict__ p_new, float beta, float *pl) {
+                                           const VT *__restrict__ src,
+                                           const VT *__restrict__ dinv,
+                                           const VT *__restrict__ p_old,
+                                           VT *__restrict__ p_new, VT beta, float *pl) {
   for (int t = threadIdx.x; t < PN; t += NT) {
     const int hx = t % PX, hy = t / PX;
     const int gx = x0 - 1 + hx, gy = y0 - 1 + hy;
-    float v[3] = {0.f, 0.f, 0.f};
+    VT v[3] = {VT(0), VT(0), VT(0)};
     if (z >= 0 && z < L.NZ && gx >= 0 && gx < L.NX && gy >= 0 && gy < L.NY) {
       const long long n = gx + (long long)L.NX * (gy + (long long)L.NY * z);
 #pragma unroll
@@ -156,24 +161,24 @@ __device__ __forceinline__ void load_plane(const Lattice &L, int x0, int y0, int
       }
     }
 #pragma unroll
-    for (int c = 0; c < 3; ++c) pl[c * PN + t] = v[c];
+    for (int c = 0; c < 3; ++c) pl[c * PN + t] = (float)v[c];
   }
 }
 
-template <int MODE, bool PCG>
+template <int MODE, bool PCG, class VT>
 __global__ __launch_bounds__(NT) void k_stencil_mfma(
     Lattice L, int zchunk, const float *__restrict__ ke, const float *__restrict__ young,
-    const uint8_t *__restrict__ fmask, const float *__restrict__ src,
-    const float *__restrict__ dinv, const float *__restrict__ p_old, float *__restrict__ p_new,
-    float *__restrict__ y, PcgScalars *S, double *partials, unsigned *counter) {
+    const uint8_t *__restrict__ fmask, const VT *__restrict__ src,
+    const VT *__restrict__ dinv, const VT *__restrict__ p_old, VT *__restrict__ p_new,
+    VT *__restrict__ y, PcgScalars *S, double *partials, unsigned *counter) {
   __shared__ float sP[3][3 * PN];       // ring of node planes
   __shared__ float sW[LN * WSTRIDE];    // element forces of the current layer
   __shared__ float sE[LN];              // young of the current layer
   __shared__ double sh[4];
-  float beta = 0.f;
+  VT beta = VT(0);
   if constexpr (PCG) {
     if (S->done) return;
-    beta = (float)S->beta;
+    beta = (VT)S->beta;
   }
   const int x0 = blockIdx.x * TX, y0 = blockIdx.y * TY, z0 = blockIdx.z * zchunk;
   const int z1 = min(z0 + zchunk, L.NZ);  // owned node planes [z0, z1)
@@ -186,8 +191,8 @@ __global__ __launch_bounds__(NT) void k_stencil_mfma(
   float acc[3] = {0.f, 0.f, 0.f};
   double pq = 0.0;
   // planes z0-1 and z0 into the ring
-  load_plane<PCG>(L, x0, y0, z0 - 1, false, src, dinv, p_old, p_new, beta, sP[0]);
-  load_plane<PCG>(L, x0, y0, z0, z0 < z1, src, dinv, p_old, p_new, beta, sP[1]);
+  load_plane<PCG, VT>(L, x0, y0, z0 - 1, false, src, dinv, p_old, p_new, beta, sP[0]);
+  load_plane<PCG, VT>(L, x0, y0, z0, z0 < z1, src, dinv, p_old, p_new, beta, sP[1]);
   for (int lay = 0; lay <= z1 - z0; ++lay) {
     const int zb = z0 - 1 + lay;           // element layer between planes zb, zb+1
     const float *pl0 = sP[lay % 3], *pl1 = sP[(lay + 1) % 3];
@@ -203,8 +208,8 @@ __global__ __launch_bounds__(NT) void k_stencil_mfma(
     contract_layer<MODE>(K, pl0, pl1, sE, sW);
     // prefetch plane zb+2 for the next layer (the ring slot of plane zb-1 is free)
     if (lay + 1 <= z1 - z0)
-      load_plane<PCG>(L, x0, y0, zb + 2, zb + 2 < z1, src, dinv, p_old, p_new, beta,
-                      sP[(lay + 2) % 3]);
+      load_plane<PCG, VT>(L, x0, y0, zb + 2, zb + 2 < z1, src, dinv, p_old, p_new, beta,
+                          sP[(lay + 2) % 3]);
     __syncthreads();
     // gather: plane zb is complete (acc from the layer below + bottom forces here)
     float bot[3] = {0.f, 0.f, 0.f}, top[3] = {0.f, 0.f, 0.f};
@@ -226,7 +231,7 @@ __global__ __launch_bounds__(NT) void k_stencil_mfma(
 #pragma unroll
       for (int c = 0; c < 3; ++c) out[c] = ((mk >> c) & 1u) ? acc[c] + bot[c] : 0.f;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) y[c * L.Np + n] = out[c];
+      for (int c = 0; c < 3; ++c) y[c * L.Np + n] = (VT)out[c];
       if constexpr (PCG) {
         const int h = (tx + 1) + PX * (ty + 1);
 #pragma unroll
@@ -240,62 +245,6 @@ __global__ __launch_bounds__(NT) void k_stencil_mfma(
   if constexpr (PCG) {
     double v[1] = {pq};
     store_partials<NT, 1>(v, partials, sh);
-  }
-}
-
-// d += alpha p; r -= alpha q; rho' = r.dinv.r and r.r in FP64
-__global__ __launch_bounds__(NT) void k_pcg_update32(long long n, float *__restrict__ d,
-                                                     float *__restrict__ r,
-                                                     const float *__restrict__ p,
-                                                     const float *__restrict__ q,
-                                                     const float *__restrict__ dinv,
-                                                     PcgScalars *S, double *partials) {
-  __shared__ double sh[8];
-  if (S->done) return;
-  const float alpha = (float)S->alpha;
-  double rz = 0.0, rr = 0.0;
-  const long long stride = (long long)gridDim.x * NT;
-  for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += stride) {
-    d[i] += alpha * p[i];
-    const float ri = r[i] - alpha * q[i];
-    r[i] = ri;
-    rz += (double)(dinv[i] * ri) * (double)ri;
-    rr += (double)ri * (double)ri;
-  }
-  double v[2] = {rz, rr};
-  store_partials<NT, 2>(v, partials, sh);
-}
-
-// Reliable update: r32 = b - y (y = K64 x64, or 0 when y is null), rho =
-// r32.dinv.r32 and r.r in FP64; on the first call also d = p0 = 0.
-__global__ __launch_bounds__(NT) void k_replace32(long long n, const double *__restrict__ b,
-                                                  const double *__restrict__ y,
-                                                  float *__restrict__ r, float *__restrict__ d,
-                                                  float *__restrict__ p0,
-                                                  const float *__restrict__ dinv,
-                                                  double *partials) {
-  __shared__ double sh[8];
-  double rz = 0.0, rr = 0.0;
-  const long long stride = (long long)gridDim.x * NT;
-  for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += stride) {
-    const double r64 = y ? b[i] - y[i] : b[i];
-    const float ri = (float)r64;
-    r[i] = ri;
-    if (d) d[i] = 0.f;
-    if (p0) p0[i] = 0.f;
-    rz += (double)(dinv[i] * ri) * (double)ri;
-    rr += r64 * r64;
-  }
-  double v[2] = {rz, rr};
-  store_partials<NT, 2>(v, partials, sh);
-}
-
-// x64 += d32; d32 = 0
-__global__ void k_ir_accumulate(long long n, float *__restrict__ d, double *__restrict__ x) {
-  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (i < n) {
-    x[i] += (double)d[i];
-    d[i] = 0.f;
   }
 }
 
@@ -331,53 +280,31 @@ long long mixed_blocks(const Lattice &L) {
 }
 
 void launch_apply_k_mixed(const Lattice &L, int mode, const float *ke, const float *young,
-                          const uint8_t *fmask, const float *u, float *y, hipStream_t s) {
+                          const uint8_t *fmask, const double *u, double *y, hipStream_t s) {
   const int zc = mixed_zchunk(L);
   if (mode == 0)
-    hipLaunchKernelGGL((k_stencil_mfma<0, false>), mixed_grid(L, zc), dim3(NT), 0, s, L, zc, ke,
-                       young, fmask, u, nullptr, nullptr, nullptr, y, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL((k_stencil_mfma<0, false, double>), mixed_grid(L, zc), dim3(NT), 0, s, L, zc,
+                       ke, young, fmask, u, nullptr, nullptr, nullptr, y, nullptr, nullptr, nullptr);
   else
-    hipLaunchKernelGGL((k_stencil_mfma<1, false>), mixed_grid(L, zc), dim3(NT), 0, s, L, zc, ke,
-                       young, fmask, u, nullptr, nullptr, nullptr, y, nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL((k_stencil_mfma<1, false, double>), mixed_grid(L, zc), dim3(NT), 0, s, L, zc,
+                       ke, young, fmask, u, nullptr, nullptr, nullptr, y, nullptr, nullptr, nullptr);
   HIPCHK(hipGetLastError());
 }
 
-void launch_pcg_stencil_mixed(const Lattice &L, int mode, const float *ke, const float *young,
-                              const uint8_t *fmask, const float *r, const float *dinv,
-                              const float *p_old, float *p_new, float *q, PcgScalars *S,
-                              double *partials, hipStream_t s) {
+void launch_pcg_stencil_mfma64(const Lattice &L, int mode, const float *ke, const float *young,
+                               const uint8_t *fmask, const double *r, const double *dinv,
+                               const double *p_old, double *p_new, double *q, PcgScalars *S,
+                               double *partials, hipStream_t s, bool finalize) {
   const int zc = mixed_zchunk(L);
   const dim3 g = mixed_grid(L, zc);
   if (mode == 0)
-    hipLaunchKernelGGL((k_stencil_mfma<0, true>), g, dim3(NT), 0, s, L, zc, ke, young, fmask, r,
-                       dinv, p_old, p_new, q, S, partials, nullptr);
+    hipLaunchKernelGGL((k_stencil_mfma<0, true, double>), g, dim3(NT), 0, s, L, zc, ke, young, fmask,
+                       r, dinv, p_old, p_new, q, S, partials, nullptr);
   else
-    hipLaunchKernelGGL((k_stencil_mfma<1, true>), g, dim3(NT), 0, s, L, zc, ke, young, fmask, r,
-                       dinv, p_old, p_new, q, S, partials, nullptr);
+    hipLaunchKernelGGL((k_stencil_mfma<1, true, double>), g, dim3(NT), 0, s, L, zc, ke, young, fmask,
+                       r, dinv, p_old, p_new, q, S, partials, nullptr);
   HIPCHK(hipGetLastError());
-  launch_finalize(kFinPcgAlpha, 1, partials, g.x * g.y * g.z, S, nullptr, nullptr, s);
-}
-
-void launch_pcg_update32(long long n, float *d, float *r, const float *p, const float *q,
-                         const float *dinv, PcgScalars *S, double *partials, hipStream_t s) {
-  hipLaunchKernelGGL(k_pcg_update32, dim3(VEC_BLOCKS), dim3(NT), 0, s, n, d, r, p, q, dinv, S,
-                     partials);
-  HIPCHK(hipGetLastError());
-  launch_finalize(kFinPcgBeta, 2, partials, VEC_BLOCKS, S, nullptr, nullptr, s);
-}
-
-void launch_replace32(long long n, const double *b, const double *y, float *r, float *d,
-                      float *p0, const float *dinv, PcgScalars *S, double *partials, int init,
-                      hipStream_t s) {
-  hipLaunchKernelGGL(k_replace32, dim3(VEC_BLOCKS), dim3(NT), 0, s, n, b, y, r, d, p0, dinv,
-                     partials);
-  HIPCHK(hipGetLastError());
-  launch_finalize(init ? kFinPcgInit : kFinReplace, 2, partials, VEC_BLOCKS, S, nullptr, nullptr, s);
-}
-
-void launch_ir_accumulate(long long n, float *d, double *x, hipStream_t s) {
-  hipLaunchKernelGGL(k_ir_accumulate, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, d, x);
-  HIPCHK(hipGetLastError());
+  if (finalize) launch_finalize(kFinPcgAlpha, 1, partials, g.x * g.y * g.z, S, nullptr, nullptr, s);
 }
 
 void launch_f64_to_f32(long long n, const double *a, float *b, hipStream_t s) {

@@ -292,14 +292,17 @@ __global__ void k_copy_channels(long long nvox, int C, const __bf16 *__restrict_
   dst[v * Cd + doff + c] = (__bf16)s;
 }
 
-// relative L2 loss: partial sums of (out - t)^2 and t^2; dout = (out - t) / sum t^2
-__global__ void k_loss_terms(long long n, const float *__restrict__ out,
+// relative L2 loss over the first cu of cs output channels against a
+// cu-channel target: partial sums of (out - t)^2 and t^2; dout = (out - t) / sum t^2
+__global__ void k_loss_terms(long long n, int cs, int cu, const float *__restrict__ out,
                              const float *__restrict__ t, double *acc2) {
   __shared__ double sh[8];
   double a = 0.0, b = 0.0;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
-    const double d = (double)out[i] - (double)t[i];
+    const long long v = i / cu;
+    const int c = (int)(i % cu);
+    const double d = (double)out[v * cs + c] - (double)t[i];
     a += d * d;
     b += (double)t[i] * (double)t[i];
   }
@@ -311,12 +314,20 @@ __global__ void k_loss_terms(long long n, const float *__restrict__ out,
   }
 }
 
-__global__ void k_loss_grad(long long n, const float *__restrict__ out, const float *__restrict__ t,
-                            const double *acc2, __bf16 *__restrict__ dout) {
+__global__ void k_zero2(double *a) {
+  a[0] = 0.0;
+  a[1] = 0.0;
+}
+
+__global__ void k_loss_grad(long long n, int cs, int cu, const float *__restrict__ out,
+                            const float *__restrict__ t, const double *acc2,
+                            __bf16 *__restrict__ dout) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const long long v = i / cu;
+  const int c = (int)(i % cu);
   const float inv = (float)(1.0 / acc2[1]);
-  dout[i] = (__bf16)((out[i] - t[i]) * inv);
+  dout[v * cs + c] = (__bf16)((out[v * cs + c] - t[i]) * inv);
 }
 
 // Adam on FP32 masters; refreshes the BF16 weights and the flipped,
@@ -468,12 +479,14 @@ void launch_copy_channels(long long nvox, int C, const __bf16 *src, int Cs, int 
   HIPCHK(hipGetLastError());
 }
 
-void launch_loss(long long n, const float *out, const float *t, double *acc2, __bf16 *dout,
-                 hipStream_t s) {
-  HIPCHK(hipMemsetAsync(acc2, 0, 2 * sizeof(double), s));
+void launch_loss(long long nvox, int cs, int cu, const float *out, const float *t, double *acc2,
+                 __bf16 *dout, hipStream_t s) {
+  const long long n = nvox * cu;
+  hipLaunchKernelGGL(k_zero2, dim3(1), dim3(1), 0, s, acc2);
   hipLaunchKernelGGL(k_loss_terms, dim3(std::min<long long>(1024, blocks_for(n, 256))), dim3(256), 0,
-                     s, n, out, t, acc2);
-  hipLaunchKernelGGL(k_loss_grad, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, out, t, acc2, dout);
+                     s, n, cs, cu, out, t, acc2);
+  hipLaunchKernelGGL(k_loss_grad, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, cs, cu, out, t, acc2,
+                     dout);
   HIPCHK(hipGetLastError());
 }
 

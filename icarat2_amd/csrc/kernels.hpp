@@ -16,8 +16,13 @@ void launch_young(const double *x, double *young, long long nelx, double y1, dou
 // dinv[c*Np+n] = 1 / diag K at free DOFs, 0 elsewhere
 void launch_diag_inv(const Lattice &L, const double *ke, const double *young,
                      const uint8_t *freemask, double *dinv, double *diag, hipStream_t s);
+// number of workgroups of the largest K.u launch (sizes the partials buffer)
+long long stencil_blocks(const Lattice &L);
 // y = K u on the lattice; y masked to free DOFs.  u is used as given
-// (Dirichlet values included).
+// (Dirichlet values included).  The z-marching kernel; the _tiled variant is
+// the 3-D halo-tile kernel kept as a cross-check.
+void launch_apply_k64_tiled(const Lattice &L, const double *ke, const double *young,
+                            const uint8_t *freemask, const double *u, double *y, hipStream_t s);
 void launch_apply_k64(const Lattice &L, const double *ke, const double *young,
                       const uint8_t *freemask, const double *u, double *y, hipStream_t s);
 // One fused Jacobi-PCG iteration, stencil half: p_new = dinv*r + beta*p_old;

@@ -1,0 +1,76 @@
+// learned.hpp -- the learned preconditioner: a 3-D U-Net on a 2x coarsened
+// cell grid maps (stiffness feature, Jacobi-scaled residual) to the part of
+// the error that Jacobi misses; flexible PCG uses
+//     z = D^-1 r + sigma * P(net(feature, R(D^-1 r) / sigma)),
+// with a Jacobi-PCG fallback.  The network is trained online (Adam, on the
+// project's own forward/backward kernels) on residual/error pairs harvested
+// from the Jacobi-PCG iterates of the first design iterations.
+#pragma once
+#include <memory>
+#include <vector>
+
+#include "../../include/icarat_hip.h"
+#include "hip_util.hpp"
+#include "unet.hpp"
+
+namespace icarat_hip {
+
+class Context;
+
+class LearnedPrecond {
+ public:
+  LearnedPrecond(Context &ctx, const icarat_learned_opts &o);
+  ~LearnedPrecond();
+  const icarat_learned_opts &opts() const { return o_; }
+  UNet &net() { return *net_; }
+  const NetGrid &grid() const { return g0_; }
+
+  // Training data: snapshots of (x_k, r_k) taken every few batches of a
+  // Jacobi solve; after the solve, targets from the converged x.
+  void begin_harvest();
+  void snapshot();            // at a batch boundary of the running solve
+  void end_harvest(bool converged);  // targets need the converged x
+  int samples() const;
+  int harvested() const { return harvested_; }  // harvesting solves finished
+  // Adam steps on random harvested samples; returns the mean relative squared
+  // error ||net - target||^2 / ||target||^2 of the last (up to 20) steps, -1
+  // without samples.
+  double train(int steps);
+  double last_loss() const { return last_loss_; }
+  long long total_steps() const { return total_steps_; }
+  // Flexible PCG with the network preconditioner on ctx.d_b -> ctx.d_x.
+  void solve(const icarat_solve_opts &o, icarat_solve_stats &st);
+  // z = M(r) once (for tests): r, z lattice vectors on the device
+  void apply(const double *r, double *z);
+
+ private:
+  Context &ctx_;
+  icarat_learned_opts o_;
+  NetGrid g0_;
+  std::unique_ptr<UNet> net_;
+  long long ncell_;
+  DevBuf<float> feat_, s3_, out_;
+  DevBuf<__bf16> xin_;
+  DevBuf<double> scale_, z_, ones_;
+  // sample ring: valid_[k] once slot k holds a finished (input, target) pair
+  int cap_ = 0, next_ = 0;
+  std::vector<char> valid_;
+  DevBuf<__bf16> sx_;            // cap x ncell x 8
+  DevBuf<float> szr_, st_;       // cap x ncell x 3 (scaled residual part; R x_k then target)
+  DevBuf<double> ssig_;          // cap sigmas
+  std::vector<int> pending_;     // slots filled by the running harvest
+  DevBuf<float> rxs_;
+  bool harvesting_ = false;
+  int harvest_batches_ = 0;
+  int harvested_ = 0;
+  double last_loss_ = -1.0;
+  long long total_steps_ = 0;
+  uint64_t rng_;
+  // graphs of batches of flexible-PCG iterations [kmode + 1][parity]
+  hipGraph_t graph_[3][2] = {};
+  hipGraphExec_t exec_[3][2] = {};
+  void build_graph(int kmode);
+  void precondition(const double *r);  // z_ = M(r)
+};
+
+}  // namespace icarat_hip

@@ -1,0 +1,36 @@
+// learned_kernels.hpp -- launchers of the lattice <-> cell-grid transfers and
+// the flexible-PCG reductions of the learned preconditioner (k_learned.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+#include "hip_util.hpp"
+#include "unet_kernels.hpp"
+
+namespace icarat_hip {
+
+// s3 = R(dinv r) per cell; scale[0] = sigma (RMS of s3), scale[1] = 1/sigma,
+// scale[2] scratch
+void launch_restrict_residual(const Lattice &L, const NetGrid &g, const double *r, const double *dinv,
+                              float *s3, double *partials, double *scale, hipStream_t s);
+void launch_restrict_vec(const Lattice &L, const NetGrid &g, const double *x, float *out3,
+                         hipStream_t s);
+void launch_pack_input(const NetGrid &g, const float *feat, const float *s3, const double *scale,
+                       __bf16 *xin, float *zr, hipStream_t s);
+void launch_density_feature(const Lattice &L, const NetGrid &g, const double *young, float *feat,
+                            hipStream_t s);
+// z = dinv r + sigma P(out) (use_net = 0: z = dinv r)
+void launch_prolong_add(const Lattice &L, const NetGrid &g, const float *out, int cs,
+                        const double *scale, const double *r, const double *dinv,
+                        const uint8_t *fmask, double *z, int use_net, hipStream_t s);
+void launch_make_target(long long n, const float *rxs, const float *rxk, double inv_sigma,
+                        const float *zr, float *t, hipStream_t s);
+// rho = z.r, beta = -alpha z.q / rho_old (Polak-Ribiere; 0 after a restart);
+// status 4 when z.r <= 0
+void launch_zdot_beta(long long n, const double *z, const double *r, const double *q, PcgScalars *S,
+                      double *partials, hipStream_t s);
+// x += alpha p, r -= alpha q, r.r and the stop tests
+void launch_fpcg_update(long long n, double *x, double *r, const double *p, const double *q,
+                        PcgScalars *S, double *partials, hipStream_t s);
+
+}  // namespace icarat_hip

@@ -10,7 +10,7 @@ namespace icarat_hip {
 
 struct NetGrid {
   int X, Y, Z;
-  long long nvox() const { return (long long)X * Y * Z; }
+  __host__ __device__ long long nvox() const { return (long long)X * Y * Z; }
 };
 
 void launch_conv_fwd(const NetGrid &g, int taps, int Cin, int Cout, const __bf16 *x,
@@ -29,8 +29,10 @@ void launch_up_bwd(const NetGrid &gc, int C, int Ctot, int off, const __bf16 *dy
                    hipStream_t s);
 void launch_copy_channels(long long nvox, int C, const __bf16 *src, int Cs, int soff, __bf16 *dst,
                           int Cd, int doff, int accumulate, hipStream_t s);
-void launch_loss(long long n, const float *out, const float *t, double *acc2, __bf16 *dout,
-                 hipStream_t s);
+// relative L2 loss of the first cu of cs output channels against a cu-channel
+// target; acc2 = {sum (out-t)^2, sum t^2}; dout (cs channels, the unused ones left untouched)
+void launch_loss(long long nvox, int cs, int cu, const float *out, const float *t, double *acc2,
+                 __bf16 *dout, hipStream_t s);
 void launch_adam(long long n, int taps, int Cout, int Cin, float *p, const float *g, float *m,
                  float *v, float lr, float b1, float b2, float eps, int step, __bf16 *wbf,
                  __bf16 *wt, hipStream_t s);

@@ -179,6 +179,8 @@ typedef struct icarat_simp_stats {
   double solve_s, filter_s, sens_s, update_s;
   int64_t pcg_iters;       /* Krylov iterations, all design iterations */
   double max_true_relres;  /* largest FP64 true residual over the loop */
+  double train_s;          /* learned preconditioner: time spent training the network */
+  int64_t fallbacks;       /* learned preconditioner: solves that fell back to Jacobi-PCG */
 } icarat_simp_stats;
 
 /* The reference drivers' loop (sample/algorithm/opt-macro-oc/opt_macro_oc.cpp:91-178,
@@ -188,6 +190,70 @@ typedef struct icarat_simp_stats {
  * true relative residual}. */
 int icarat_hip_simp_run(icarat_hip_ctx *ctx, const icarat_simp_opts *opts, double *x,
                         double *hist, icarat_simp_stats *stats);
+
+/* ---- learned preconditioner --------------------------------------------------
+ * No reference counterpart (the reference solves with Scaling_CG only,
+ * analysis/solver/CG.cpp:56-100): a 3-level U-Net on a cell grid coarsened 2x
+ * from the node lattice corrects the Jacobi step inside flexible PCG,
+ *     z = D^-1 r + s P(net(log-stiffness, R(D^-1 r) / s)),  s = RMS of R(D^-1 r),
+ * R full-weighting restriction, P its transpose.  The network is trained on
+ * (residual, error) pairs harvested from Jacobi-PCG solves.  A solve with
+ * precond = ICARAT_HIP_PRECOND_LEARNED falls back to Jacobi-PCG from its
+ * current iterate when z.r <= 0 or the residual stalls. */
+typedef struct icarat_learned_opts {
+  int32_t train_iters;     /* simp_run: the first train_iters solves of the preconditioner's
+                              lifetime run Jacobi-PCG and are harvested */
+  int32_t train_steps;     /* simp_run: Adam steps in total, spread over the harvested iterations */
+  int32_t snapshot_every;  /* PCG iterations between two training snapshots of one solve */
+  int32_t max_samples;     /* capacity of the sample ring */
+  int32_t batch_iters;     /* flexible-PCG iterations per graph launch */
+  int32_t stall_iters;     /* fallback when the residual has not halved for this many iterations */
+  float lr;                /* Adam learning rate */
+  int32_t reserved;
+  uint64_t seed;           /* weight initialisation and sample order */
+} icarat_learned_opts;
+
+/* Fills *o with the defaults (10, 400, 64, 160, 8, 400, 1e-3, 0, 1). */
+int icarat_hip_learned_default_opts(icarat_learned_opts *o);
+/* (Re)creates the preconditioner of the context (o = NULL: defaults).  Needs make_dof. */
+int icarat_hip_learned_setup(icarat_hip_ctx *ctx, const icarat_learned_opts *o);
+/* Jacobi-PCG solve of the current density (as icarat_hip_solve with precond =
+ * JACOBI) that stores training snapshots; u may be NULL. */
+int icarat_hip_learned_harvest(icarat_hip_ctx *ctx, const icarat_solve_opts *opts, double *u,
+                               icarat_solve_stats *stats);
+/* Adam steps on the stored samples; *loss = mean ||net - t||^2 / ||t||^2 of the
+ * last steps (-1 without samples). */
+int icarat_hip_learned_train(icarat_hip_ctx *ctx, int32_t steps, double *loss);
+/* samples held, Adam steps taken, last loss, parameter count, cell grid {X,Y,Z};
+ * any output may be NULL */
+int icarat_hip_learned_info(const icarat_hip_ctx *ctx, int32_t *samples, int64_t *steps,
+                            double *loss, int64_t *n_params, int32_t grid[3]);
+/* Network parameters in the layout of icarat_hip_unet_get_params. */
+int icarat_hip_learned_get_params(const icarat_hip_ctx *ctx, float *params);
+int icarat_hip_learned_set_params(icarat_hip_ctx *ctx, const float *params);
+/* z = M(r) for a numeq residual r (density as last set). */
+int icarat_hip_learned_apply(icarat_hip_ctx *ctx, const double *r, double *z);
+
+/* The network alone on a cell grid (X, Y, Z multiples of 4), for checks of
+ * its kernels.  Activations are channels-last with x fastest: x[v*4 + c]
+ * (4 input channels; the kernels pad to 8 with zeros), out[v*3 + c].
+ * Parameters: per layer c0..c5 the weights W[tap][co][ci] (tap = 9(dz+1) +
+ * 3(dy+1) + (dx+1); 27 taps, 1 for c5) then the biases b[co]; channels
+ * (cin, cout) = (8,16) (16,32) (32,64) (96,32) (48,16) (16,3). */
+typedef struct icarat_hip_unet icarat_hip_unet;
+int icarat_hip_unet_create(int32_t X, int32_t Y, int32_t Z, uint64_t seed, icarat_hip_unet **out);
+int icarat_hip_unet_destroy(icarat_hip_unet *net);
+const char *icarat_hip_unet_last_error(const icarat_hip_unet *net);
+int icarat_hip_unet_param_count(int64_t *n);
+int icarat_hip_unet_set_params(icarat_hip_unet *net, const float *params);
+int icarat_hip_unet_get_params(const icarat_hip_unet *net, float *params);
+int icarat_hip_unet_forward(icarat_hip_unet *net, const float *x, float *out);
+/* loss = 0.5 ||out - t||^2 / ||t||^2 and its parameter gradient (no update) */
+int icarat_hip_unet_grads(icarat_hip_unet *net, const float *x, const float *target,
+                          double *loss, float *grads);
+/* one Adam step (b1 0.9, b2 0.999, eps 1e-8) on the loss above; *loss before the step */
+int icarat_hip_unet_train_step(icarat_hip_unet *net, const float *x, const float *target,
+                               float lr, double *loss);
 
 /* ---- measurement ------------------------------------------------------------ */
 /* No reference counterpart: a timing hook for benchmarks.  Launches one kernel

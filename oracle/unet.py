@@ -152,3 +152,84 @@ def adam(P, G, state, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
         mh = m / (1 - b1 ** t)
         vh = v / (1 - b2 ** t)
         P[k] -= lr * mh / (np.sqrt(vh) + eps)
+
+
+# ---------------------------------------------------------------------------
+# The preconditioner around the network: node lattice <-> cell grid.
+# Cell (I, J, K) covers elements 2I..2I+1 (and likewise in y, z); its centre is
+# node (2I+1, 2J+1, 2K+1).  Restriction is full weighting over the 3x3x3 nodes
+# around the centre (1-D weights 1/2, 1, 1/2, normalised by 8); prolongation
+# is its transpose without the 1/8 (trilinear interpolation from the centres).
+
+def cell_grid(nx, ny, nz):
+    """Cell grid (X, Y, Z) for nx x ny x nz elements: ceil-half rounded up to 4."""
+    r4 = lambda v: (v + 3) // 4 * 4
+    return r4((nx + 1) // 2), r4((ny + 1) // 2), r4((nz + 1) // 2)
+
+
+def _weights_1d(nn, ncell):
+    """W[I, i] = weight of node i in cell I along one axis (nn nodes)."""
+    W = np.zeros((ncell, nn))
+    for I in range(ncell):
+        c = 2 * I + 1
+        for d, w in ((-1, 0.5), (0, 1.0), (1, 0.5)):
+            if 0 <= c + d < nn:
+                W[I, c + d] = w
+    return W
+
+
+def restrict(v, grid):
+    """v: (NZ, NY, NX, C) nodal -> (Z, Y, X, C) per cell (full weighting / 8)."""
+    X, Y, Z = grid
+    NZ, NY, NX = v.shape[:3]
+    Wx, Wy, Wz = _weights_1d(NX, X), _weights_1d(NY, Y), _weights_1d(NZ, Z)
+    return np.einsum("Kk,Jj,Ii,kjic->KJIc", Wz, Wy, Wx, v) / 8.0
+
+
+def prolong(o, nodes):
+    """o: (Z, Y, X, C) per cell -> (NZ, NY, NX, C) nodal (transpose of restrict x 8)."""
+    NX, NY, NZ = nodes
+    Z, Y, X = o.shape[:3]
+    Wx, Wy, Wz = _weights_1d(NX, X), _weights_1d(NY, Y), _weights_1d(NZ, Z)
+    return np.einsum("Kk,Jj,Ii,KJIc->kjic", Wz, Wy, Wx, o)
+
+
+def stiffness_feature(young, grid):
+    """young: (nz, ny, nx) per element -> (Z, Y, X) clamp((log10 mean + 9) / 9, 0, 1)
+    over the elements of each cell."""
+    X, Y, Z = grid
+    nz, ny, nx = young.shape
+    f = np.zeros((Z, Y, X))
+    for K in range(Z):
+        for J in range(Y):
+            for I in range(X):
+                blk = young[2 * K:2 * K + 2, 2 * J:2 * J + 2, 2 * I:2 * I + 2]
+                if blk.size and blk.mean() > 0:
+                    f[K, J, I] = (np.log10(blk.mean()) + 9.0) / 9.0
+    return np.clip(f, 0.0, 1.0)
+
+
+def bf16_round(a):
+    """Round to bfloat16 (nearest, ties to even) and back to float64."""
+    f = np.ascontiguousarray(a, dtype=np.float32)
+    u = f.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def precondition(P, r, dinv, free, young):
+    """z = D^-1 r + s P(net(feature, R(D^-1 r) / s)) on the node lattice.
+    r, dinv: (NZ, NY, NX, 3); free: (NZ, NY, NX, 3) bool; young: (nz, ny, nx).
+    s = RMS of R(D^-1 r) over all cell components; network inputs rounded to
+    bfloat16 as the device kernels store them.  Returns z and the correction."""
+    NZ, NY, NX = r.shape[:3]
+    grid = cell_grid(NX - 1, NY - 1, NZ - 1)
+    s3 = restrict(dinv * r, grid)
+    sig = np.sqrt(np.mean(s3 * s3))
+    x = np.zeros(s3.shape[:3] + (8,))
+    x[..., 0] = stiffness_feature(young, grid)
+    x[..., 1:4] = s3 / sig if sig > 0 else 0.0
+    out, _ = forward(P, bf16_round(x))
+    corr = sig * prolong(out, (NX, NY, NZ))
+    z = np.where(free, dinv * r + corr, 0.0)
+    return z, np.where(free, corr, 0.0)

@@ -71,3 +71,42 @@ def test_adam_reduces_loss(oracle):
         _, d = U.loss(out, t)
         U.adam(P, U.backward(P, c, d), st, lr=3e-3)
     assert U.loss(U.forward(P, x)[0], t)[0] < 0.5 * L0
+
+
+def test_transfers_closed_forms(oracle):
+    """Restriction keeps a constant in interior cells, prolongation reproduces
+    a linear field at interior nodes (trilinear interpolation), and the two are
+    adjoint up to the factor 8."""
+    import unet as U
+    rng = np.random.default_rng(6)
+    nx, ny, nz = 14, 10, 6
+    g = U.cell_grid(nx, ny, nz)
+    assert g == (8, 8, 4)
+    NX, NY, NZ = nx + 1, ny + 1, nz + 1
+    v = np.full((NZ, NY, NX, 3), 2.5)
+    s = U.restrict(v, g)
+    assert np.allclose(s[1, 1:4, 1:6], 2.5)
+    # a linear function of the cell centres (2I+1, 2J+1, 2K+1) ...
+    K, J, I = np.meshgrid(np.arange(g[2]), np.arange(g[1]), np.arange(g[0]), indexing="ij")
+    lin = lambda x, y, z: 0.3 * x - 0.2 * y + 0.7 * z + 1.0
+    o = lin(2 * I + 1, 2 * J + 1, 2 * K + 1)[..., None]
+    p = U.prolong(o, (NX, NY, NZ))[..., 0]
+    # ... is interpolated exactly at nodes with centres on both sides
+    z, y, x = np.meshgrid(np.arange(NZ), np.arange(NY), np.arange(NX), indexing="ij")
+    inner = (x >= 1) & (x <= 2 * g[0] - 1) & (y >= 1) & (y <= 2 * g[1] - 1) & (z >= 1) & (z <= 2 * g[2] - 1)
+    inner &= (x < NX) & (y < NY) & (z < NZ)
+    assert np.allclose(p[inner], lin(x, y, z)[inner])
+    # adjointness
+    v = rng.standard_normal((NZ, NY, NX, 3))
+    o = rng.standard_normal((g[2], g[1], g[0], 3))
+    assert abs(8 * np.sum(U.restrict(v, g) * o) - np.sum(v * U.prolong(o, (NX, NY, NZ)))) < 1e-10
+
+
+def test_bf16_round_known_values(oracle):
+    import unet as U
+    a = np.array([1.0, 1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -2.0 - 2 ** -7, 3.0e-3, 0.0])
+    r = U.bf16_round(a)
+    assert r[0] == 1.0 and r[1] == 1.0            # tie to even (down)
+    assert r[2] == 1.0 + 2 ** -6                  # tie to even (up)
+    assert r[3] == -2.0                           # tie between -2 and -2 - 2^-6: even mantissa
+    assert abs(r[4] - 3.0e-3) <= 3.0e-3 * 2 ** -8 and r[5] == 0.0
