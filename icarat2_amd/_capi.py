@@ -111,6 +111,7 @@ _SIGS = {
     "icarat_hip_unet_forward": (C.c_int, [C.c_void_p, fp, fp]),
     "icarat_hip_unet_grads": (C.c_int, [C.c_void_p, fp, fp, dp, fp]),
     "icarat_hip_unet_train_step": (C.c_int, [C.c_void_p, fp, fp, C.c_float, dp]),
+    "icarat_hip_set_stencil": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, i32p]),
     "icarat_hip_time_kernel": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, dp, dp, dp]),
     "icarat_hip_gaussian_field": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_double,
                                             C.c_uint64, C.c_int32, C.c_double, C.c_double,
@@ -437,6 +438,14 @@ class Context:
         z = np.zeros(self.numeq)
         self._chk(self.L.icarat_hip_learned_apply(self.h, _p(_f64(r)), _p(z)), "learned_apply")
         return z
+
+    def set_stencil(self, kernel=0, sym_max=2):
+        """FP64 K.u kernel: 0 z-marching, 1 3-D tiles; highest KE0 symmetry
+        class allowed (0 memory, 1 box, 2 cube).  Returns the class used."""
+        used = C.c_int32()
+        self._chk(self.L.icarat_hip_set_stencil(self.h, kernel, sym_max, C.byref(used)),
+                  "set_stencil")
+        return used.value
 
     def time_kernel(self, kernel, reps=20):
         ms, fl, by = C.c_double(), C.c_double(), C.c_double()

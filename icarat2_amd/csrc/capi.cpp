@@ -271,6 +271,15 @@ int icarat_hip_simp_run(icarat_hip_ctx *h, const icarat_simp_opts *o, double *x,
   });
 }
 
+int icarat_hip_set_stencil(icarat_hip_ctx *h, int32_t kernel, int32_t sym_max, int32_t *sym_used) {
+  if (kernel < 0 || kernel > 1 || sym_max < 0 || sym_max > 2)
+    return fail(h, ICARAT_HIP_INVALID_ARGUMENT, "kernel must be 0 or 1 and sym_max 0..2");
+  return guard(h, [&](icarat_hip::Context &c) {
+    c.set_stencil(kernel, sym_max);
+    if (sym_used) *sym_used = c.kc_.sym;
+  });
+}
+
 int icarat_hip_time_kernel(icarat_hip_ctx *h, int32_t kernel, int32_t reps, double *avg_ms,
                            double *flops, double *bytes) {
   return guard(h, [&](icarat_hip::Context &c) { c.time_kernel(kernel, reps, avg_ms, flops, bytes); });

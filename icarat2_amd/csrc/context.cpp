@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 
@@ -49,6 +50,8 @@ void Context::ensure_device() {
   HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   d_ke.alloc(576);
   h2d(d_ke.p, prob_.ke0(), 576, stream_);
+  kc_ = make_ke_coef(prob_.ke0(), d_ke.p);
+  if (const char *g = std::getenv("ICARAT_HIP_GRAPHS")) use_graphs_ = std::string(g) != "0";
   {
     float k32[576];
     for (int i = 0; i < 576; ++i) k32[i] = (float)prob_.ke0()[i];
@@ -177,7 +180,7 @@ void Context::apply_k(const double *u, double *y, int precision) {
   h2d(d_numeq[0].p, u, numeq, stream_);
   launch_numeq_to_lattice(L_, d_dof, numeq, d_numeq[0], nullptr, d_y, stream_);
   if (precision == ICARAT_HIP_FP64) {
-    launch_apply_k64(L_, d_ke, d_young, d_free, d_y, d_q, stream_);
+    launch_apply_k64(L_, kc_, d_young, d_free, d_y, d_q, stream_);
   } else if (precision == ICARAT_HIP_FP32 || precision == ICARAT_HIP_BF16) {
     launch_apply_k_mixed(L_, precision == ICARAT_HIP_BF16 ? 1 : 0, d_ke32, d_young32, d_free, d_y,
                          d_q, stream_);
@@ -198,23 +201,26 @@ void Context::get_diag(double *d) {
   sync();
 }
 
+void Context::enqueue_pcg_batch(int kmode, int par) {
+  for (int j = 0; j < kBatch; ++j) {
+    double *p_old = d_p[(j + par) & 1], *p_new = d_p[(j + par + 1) & 1];
+    if (kmode < 0)
+      launch_pcg_stencil64(L_, kc_, d_young, d_free, d_r, d_dinv, p_old, p_new, d_q, d_pcg,
+                           d_partials, d_counter, stream_);
+    else
+      launch_pcg_stencil_mfma64(L_, kmode, d_ke32, d_young32, d_free, d_r, d_dinv, p_old, p_new,
+                                d_q, d_pcg, d_partials, stream_);
+    launch_pcg_update64(n3_, d_x, d_r, p_new, d_q, d_dinv, d_pcg, d_partials, d_counter, stream_);
+  }
+}
+
 void Context::build_pcg_graph(int kmode) {
   const int m = kmode + 1;
-  if (pcg_exec_[m][0]) return;
+  if (!use_graphs_ || pcg_exec_[m][0]) return;
   for (int par = 0; par < 2; ++par) {
     HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     try {
-      for (int j = 0; j < kBatch; ++j) {
-        double *p_old = d_p[(j + par) & 1], *p_new = d_p[(j + par + 1) & 1];
-        if (kmode < 0)
-          launch_pcg_stencil64(L_, d_ke, d_young, d_free, d_r, d_dinv, p_old, p_new, d_q, d_pcg,
-                               d_partials, d_counter, stream_);
-        else
-          launch_pcg_stencil_mfma64(L_, kmode, d_ke32, d_young32, d_free, d_r, d_dinv, p_old, p_new,
-                                    d_q, d_pcg, d_partials, stream_);
-        launch_pcg_update64(n3_, d_x, d_r, p_new, d_q, d_dinv, d_pcg, d_partials, d_counter,
-                            stream_);
-      }
+      enqueue_pcg_batch(kmode, par);
     } catch (...) {
       hipGraph_t g = nullptr;
       (void)hipStreamEndCapture(stream_, &g);
@@ -224,6 +230,13 @@ void Context::build_pcg_graph(int kmode) {
     HIPCHK(hipStreamEndCapture(stream_, &pcg_graph_[m][par]));
     HIPCHK(hipGraphInstantiate(&pcg_exec_[m][par], pcg_graph_[m][par], nullptr, nullptr, 0));
   }
+}
+
+void Context::launch_pcg_batch(int kmode, int par) {
+  if (use_graphs_)
+    HIPCHK(hipGraphLaunch(pcg_exec_[kmode + 1][par], stream_));
+  else
+    enqueue_pcg_batch(kmode, par);
 }
 
 PcgScalars Context::pcg64(double tol, int maxit, bool warm, int kmode, double delta) {
@@ -239,7 +252,7 @@ PcgScalars Context::pcg64(double tol, int maxit, bool warm, int kmode, double de
   launch_pcg_init64(n3_, d_b, d_x, d_r, d_p[0], d_dinv, d_pcg, d_partials, d_counter, stream_);
   if (warm) {
     d2d(d_x.p, d_nv.p, n3_, stream_);
-    launch_apply_k64(L_, d_ke, d_young, d_free, d_x, d_y, stream_);
+    launch_apply_k64(L_, kc_, d_young, d_free, d_x, d_y, stream_);
     launch_replace64(n3_, d_b, d_y, d_r, d_dinv, d_pcg, d_partials, stream_);
   }
   d2h(h_pcg.p, d_pcg.p, 1, stream_);
@@ -254,7 +267,7 @@ PcgScalars Context::pcg64(double tol, int maxit, bool warm, int kmode, double de
       // iterate until the recursive stop test, a replacement, a cap or a breakdown
       const long long max_batches = (long long)(maxit - h_pcg.p->iter) / kBatch + 3;
       for (long long b = 0; b < max_batches && !h_pcg.p->done; ++b) {
-        HIPCHK(hipGraphLaunch(pcg_exec_[kmode + 1][h_pcg.p->iter & 1], stream_));
+        launch_pcg_batch(kmode, h_pcg.p->iter & 1);
         d2h(h_pcg.p, d_pcg.p, 1, stream_);
         sync();
         if (batch_hook_ && !h_pcg.p->done) batch_hook_();
@@ -264,7 +277,7 @@ PcgScalars Context::pcg64(double tol, int maxit, bool warm, int kmode, double de
     if (h_pcg.p->status != 0 && h_pcg.p->status != 3) break;
     // the FP64 true residual replaces the recursive one (verified stop /
     // reliable update); the search direction is kept
-    launch_apply_k64(L_, d_ke, d_young, d_free, d_x, d_y, stream_);
+    launch_apply_k64(L_, kc_, d_young, d_free, d_x, d_y, stream_);
     launch_replace64(n3_, d_b, d_y, d_r, d_dinv, d_pcg, d_partials, stream_);
     d2h(h_pcg.p, d_pcg.p, 1, stream_);
     sync();
@@ -296,7 +309,7 @@ void Context::assemble_nodal_solution() {
 }
 
 double Context::true_residual_device() {
-  launch_apply_k64(L_, d_ke, d_young, d_free, d_u, d_y, stream_);
+  launch_apply_k64(L_, kc_, d_young, d_free, d_u, d_y, stream_);
   launch_residual_norms(n3_, d_f, d_y, d_scal, d_partials, d_counter, stream_);
   d2h(h_scal.p, d_scal.p, 2, stream_);
   sync();
@@ -322,7 +335,7 @@ void Context::solve(const icarat_solve_opts &o, double *u_host, icarat_solve_sta
   // right-hand side: fext minus the load of the prescribed displacements
   d2d(d_b.p, d_f.p, n3_, stream_);
   if (prob_.has_nonzero_dirichlet()) {
-    launch_apply_k64(L_, d_ke, d_young, d_free, d_dirlat, d_y, stream_);
+    launch_apply_k64(L_, kc_, d_young, d_free, d_dirlat, d_y, stream_);
     launch_axpby(n3_, -1.0, d_y, 1.0, d_b, stream_);
   }
   icarat_solve_stats S{};
@@ -627,6 +640,21 @@ void Context::learned_apply(const double *r, double *z) {
   sync();
 }
 
+void Context::set_stencil(int kernel, int sym_max) {
+  ensure_device();
+  KeCoef k = make_ke_coef(prob_.ke0(), d_ke.p);
+  if (k.sym > sym_max) {
+    // fall to the highest allowed class that still reproduces KE0
+    k.sym = 0;
+    if (sym_max >= 1) {
+      KeCoef box = make_ke_coef_class(prob_.ke0(), d_ke.p, 1);
+      if (box.sym == 1) k = box;
+    }
+  }
+  k.kernel = kernel;
+  kc_ = k;
+}
+
 // ---- measurement hook ------------------------------------------------------
 void Context::time_kernel(int kernel, int reps, double *avg_ms, double *flops, double *bytes) {
   require_density();
@@ -635,7 +663,7 @@ void Context::time_kernel(int kernel, int reps, double *avg_ms, double *flops, d
   const double N = (double)L_.numnp, E = (double)L_.nelx;
   double F = 0.0, B = 0.0;
   std::function<void()> launch;
-  if (kernel == 0) {
+  if (kernel == 0 || (kernel >= 3 && kernel <= 6)) {
     // p = dinv r + beta p_old with r = f (non-zero) and beta = 0, so p.q > 0
     // and the stop flag never rises between launches
     d2d(d_r.p, d_f.p, n3_, stream_);
@@ -645,10 +673,21 @@ void Context::time_kernel(int kernel, int reps, double *avg_ms, double *flops, d
     s0.maxit = 1 << 30;
     *h_pcg.p = s0;
     h2d(d_pcg.p, h_pcg.p, 1, stream_);
-    launch = [&]() {
-      launch_pcg_stencil64(L_, d_ke, d_young, d_free, d_r, d_dinv, d_p[0], d_p[1], d_q, d_pcg,
-                           d_partials, d_counter, stream_, false);
-    };
+    KeCoef kc = kc_;
+    if (kernel == 3) kc.sym = 0;                 // KE0 from memory
+    if (kernel == 4) kc.kernel = 1 - kc.kernel;  // the other FP64 kernel shape
+    if (kernel <= 4) {
+      launch = [this, kc]() {
+        launch_pcg_stencil64(L_, kc, d_young, d_free, d_r, d_dinv, d_p[0], d_p[1], d_q, d_pcg,
+                             d_partials, d_counter, stream_, false);
+      };
+    } else {
+      const int mode = kernel - 5;  // 0 FP32, 1 BF16 MFMA contraction
+      launch = [this, mode]() {
+        launch_pcg_stencil_mfma64(L_, mode, d_ke32, d_young32, d_free, d_r, d_dinv, d_p[0], d_p[1],
+                                  d_q, d_pcg, d_partials, stream_, false);
+      };
+    }
     F = 2.0 * 24 * 24 * E;                      // dense KE0 x U_e contraction
     B = 3.0 * N * 8 * 3 + 2.0 * 3 * N * 8 + E * 8 + N;  // r, dinv, p_old in; p, q out; young; mask
   } else if (kernel == 1) {

@@ -61,11 +61,13 @@ class Context {
   void learned_harvest(const icarat_solve_opts &o, double *u_host, icarat_solve_stats *st);
   void learned_apply(const double *r, double *z);
   void time_kernel(int kernel, int reps, double *avg_ms, double *flops, double *bytes);
+  void set_stencil(int kernel, int sym_max);
 
   // ---- device buffers (public for the solver components) ----
   Lattice L_;
   long long n3_ = 0;  // 3 * Np
   DevBuf<double> d_ke;                // KE0 (576)
+  KeCoef kc_;                         // KE0 as the FP64 stencil kernels take it
   DevBuf<float> d_ke32;               // KE0 in FP32 for the MFMA contraction
   DevBuf<float> d_young32;            // SIMP young in FP32
   DevBuf<uint8_t> d_free;             // per node free mask
@@ -106,6 +108,9 @@ class Context {
   static constexpr int kBatch = 16;
   hipGraphExec_t pcg_exec_[3][2] = {};
   hipGraph_t pcg_graph_[3][2] = {};
+  // Graphs can be switched off (ICARAT_HIP_GRAPHS=0) for tools that cannot
+  // trace graph launches; the kernels launched are the same.
+  bool use_graphs_ = true;
   int replacements_ = 0;  // residual replacements of the last solve
   int fell_back_ = 0;     // the last solve switched to the FP64 contraction
 
@@ -136,6 +141,8 @@ class Context {
   void require_dof();
   void require_density();
   void build_pcg_graph(int kmode);
+  void enqueue_pcg_batch(int kmode, int par);  // kBatch iterations on the stream
+  void launch_pcg_batch(int kmode, int par);   // as a graph, or enqueued directly
 };
 
 }  // namespace icarat_hip

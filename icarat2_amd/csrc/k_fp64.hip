@@ -12,8 +12,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 
 #include "kernels.hpp"
+#include "ke_sym_table.hpp"
 
 namespace icarat_hip {
 
@@ -28,8 +30,21 @@ __device__ __forceinline__ constexpr int nb(int dx, int dy, int dz) {
   return (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
 }
 
+// KE0[e] (e = row*24 + col, a compile-time constant after unrolling): from
+// memory (SYM 0) or as +-orbit value held in scalar registers (SYM 1 box, 2 cube).
+template <int SYM>
+__device__ __forceinline__ double kcoef(const double *__restrict__ c_KE0, const KeVals &kv, int e) {
+  if constexpr (SYM == 0) {
+    return c_KE0[e];
+  } else {
+    const int code = SYM == 1 ? KSYM_BOX[e] : KSYM_CUBE[e];
+    return code > 0 ? kv.v[code - 1] : -kv.v[-code - 1];
+  }
+}
+
 // y = sum over the 8 elements around a node of E_s * KE0[rows of the node] . u_e
-__device__ __forceinline__ void node_contract(const double *__restrict__ c_KE0,
+template <int SYM>
+__device__ __forceinline__ void node_contract(const double *__restrict__ c_KE0, const KeVals &kv,
                                               const double (&U)[27][3], const double (&E)[8],
                                               double (&y)[3]) {
   y[0] = y[1] = y[2] = 0.0;
@@ -44,9 +59,9 @@ __device__ __forceinline__ void node_contract(const double *__restrict__ c_KE0,
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
         const double u = U[o][j];
-        t0 += c_KE0[(3 * a + 0) * 24 + 3 * b + j] * u;
-        t1 += c_KE0[(3 * a + 1) * 24 + 3 * b + j] * u;
-        t2 += c_KE0[(3 * a + 2) * 24 + 3 * b + j] * u;
+        t0 += kcoef<SYM>(c_KE0, kv, (3 * a + 0) * 24 + 3 * b + j) * u;
+        t1 += kcoef<SYM>(c_KE0, kv, (3 * a + 1) * 24 + 3 * b + j) * u;
+        t2 += kcoef<SYM>(c_KE0, kv, (3 * a + 2) * 24 + 3 * b + j) * u;
       }
     }
     y[0] += E[s] * t0;
@@ -55,9 +70,9 @@ __device__ __forceinline__ void node_contract(const double *__restrict__ c_KE0,
   }
 }
 
-template <bool PCG>
+template <bool PCG, int SYM>
 __global__ __launch_bounds__(NT) void k_stencil64(
-    Lattice L, const double *__restrict__ c_KE0, const double *__restrict__ young,
+    Lattice L, const double *__restrict__ c_KE0, KeVals kv, const double *__restrict__ young,
     const uint8_t *__restrict__ fmask, const double *__restrict__ src, const double *__restrict__ dinv,
     const double *__restrict__ p_old, double *__restrict__ p_new, double *__restrict__ y,
     PcgScalars *S, double *partials, unsigned *counter) {
@@ -128,7 +143,7 @@ __global__ __launch_bounds__(NT) void k_stencil64(
         E[s] = se[(tx + 1 + ex) + EX * ((ty + 1 + ey) + EY * (iz + 1 + ez))];
       }
       double out[3];
-      node_contract(c_KE0, U, E, out);
+      node_contract<SYM>(c_KE0, kv, U, E, out);
       const long long n = gx + (long long)L.NX * (gy + (long long)L.NY * gz);
       const unsigned m = fmask[n];
 #pragma unroll
@@ -248,7 +263,8 @@ __device__ __forceinline__ void read_nbhd(const double (*sp)[ZPN], int tx, int t
 
 // Contributions of the 4 elements of one layer around the node column:
 // bot = rows of the bottom node (local z = 0), top = rows of the top node.
-__device__ __forceinline__ void layer_contract(const double *__restrict__ c_KE0,
+template <int SYM>
+__device__ __forceinline__ void layer_contract(const double *__restrict__ c_KE0, const KeVals &kv,
                                                const double (&Ulo)[9][3], const double (&Uhi)[9][3],
                                                const double (&E)[4], double (&bot)[3],
                                                double (&top)[3]) {
@@ -265,12 +281,12 @@ __device__ __forceinline__ void layer_contract(const double *__restrict__ c_KE0,
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
         const double u = lnode_z(b) ? Uhi[o][j] : Ulo[o][j];
-        tb0 += c_KE0[(3 * ab + 0) * 24 + 3 * b + j] * u;
-        tb1 += c_KE0[(3 * ab + 1) * 24 + 3 * b + j] * u;
-        tb2 += c_KE0[(3 * ab + 2) * 24 + 3 * b + j] * u;
-        tt0 += c_KE0[(3 * at + 0) * 24 + 3 * b + j] * u;
-        tt1 += c_KE0[(3 * at + 1) * 24 + 3 * b + j] * u;
-        tt2 += c_KE0[(3 * at + 2) * 24 + 3 * b + j] * u;
+        tb0 += kcoef<SYM>(c_KE0, kv, (3 * ab + 0) * 24 + 3 * b + j) * u;
+        tb1 += kcoef<SYM>(c_KE0, kv, (3 * ab + 1) * 24 + 3 * b + j) * u;
+        tb2 += kcoef<SYM>(c_KE0, kv, (3 * ab + 2) * 24 + 3 * b + j) * u;
+        tt0 += kcoef<SYM>(c_KE0, kv, (3 * at + 0) * 24 + 3 * b + j) * u;
+        tt1 += kcoef<SYM>(c_KE0, kv, (3 * at + 1) * 24 + 3 * b + j) * u;
+        tt2 += kcoef<SYM>(c_KE0, kv, (3 * at + 2) * 24 + 3 * b + j) * u;
       }
     }
     bot[0] += E[s] * tb0;
@@ -282,9 +298,9 @@ __device__ __forceinline__ void layer_contract(const double *__restrict__ c_KE0,
   }
 }
 
-template <bool PCG>
+template <bool PCG, int SYM>
 __global__ __launch_bounds__(NT) void k_stencil64z(
-    Lattice L, int zchunk, const double *__restrict__ c_KE0, const double *__restrict__ young,
+    Lattice L, int zchunk, const double *__restrict__ c_KE0, KeVals kv, const double *__restrict__ young,
     const uint8_t *__restrict__ fmask, const double *__restrict__ src,
     const double *__restrict__ dinv, const double *__restrict__ p_old, double *__restrict__ p_new,
     double *__restrict__ y, PcgScalars *S, double *partials) {
@@ -331,7 +347,7 @@ __global__ __launch_bounds__(NT) void k_stencil64z(
       E[s] = se[cur][(tx + 1 + ex) + ZLX * (ty + 1 + ey)];
     }
     double bot[3], top[3];
-    layer_contract(c_KE0, Ulo, Uhi, E, bot, top);
+    layer_contract<SYM>(c_KE0, kv, Ulo, Uhi, E, bot, top);
     if (k >= 1 && col_in) {
       const long long n = gx + (long long)L.NX * (gy + (long long)L.NY * zb);
       const unsigned m = fmask[n];
@@ -592,32 +608,80 @@ long long stencil_blocks(const Lattice &L) {
   return std::max((long long)a.x * a.y * a.z, (long long)b.x * b.y * b.z);
 }
 
-void launch_apply_k64(const Lattice &L, const double *ke, const double *young,
+KeCoef make_ke_coef_class(const double *ke, const double *ke_dev, int cls) {
+  KeCoef k;
+  k.ke = ke_dev;
+  double kmax = 0.0;
+  for (int e = 0; e < 576; ++e) kmax = std::max(kmax, std::fabs(ke[e]));
+  auto fits = [&](const signed char *code, const int *rep, int n) {
+    for (int e = 0; e < 576; ++e) {
+      const int c = code[e];
+      const double v = c > 0 ? ke[rep[c - 1]] : -ke[rep[-c - 1]];
+      if (!(std::fabs(v - ke[e]) <= 1e-13 * kmax)) return false;
+    }
+    for (int i = 0; i < n; ++i) k.vals.v[i] = ke[rep[i]];
+    return true;
+  };
+  if (cls == 2 && fits(KSYM_CUBE, KSYM_CUBE_REP, KSYM_CUBE_N)) k.sym = 2;
+  else if (cls == 1 && fits(KSYM_BOX, KSYM_BOX_REP, KSYM_BOX_N)) k.sym = 1;
+  return k;
+}
+
+KeCoef make_ke_coef(const double *ke, const double *ke_dev) {
+  KeCoef k = make_ke_coef_class(ke, ke_dev, 2);
+  if (k.sym == 0) k = make_ke_coef_class(ke, ke_dev, 1);
+  return k;
+}
+
+template <bool PCG, int SYM>
+static void launch_stencil64_t(const Lattice &L, const KeCoef &ke, const double *young,
+                               const uint8_t *freemask, const double *src, const double *dinv,
+                               const double *p_old, double *p_new, double *y, PcgScalars *S,
+                               double *partials, hipStream_t s, unsigned *nblocks) {
+  if (ke.kernel == 0) {
+    const int zc = stencil_zchunk(L);
+    const dim3 g = zgrid(L, zc);
+    hipLaunchKernelGGL((k_stencil64z<PCG, SYM>), g, dim3(NT), 0, s, L, zc, ke.ke, ke.vals, young,
+                       freemask, src, dinv, p_old, p_new, y, S, partials);
+    *nblocks = g.x * g.y * g.z;
+  } else {
+    const dim3 g = stencil_grid(L);
+    hipLaunchKernelGGL((k_stencil64<PCG, SYM>), g, dim3(NT), 0, s, L, ke.ke, ke.vals, young,
+                       freemask, src, dinv, p_old, p_new, y, S, partials, nullptr);
+    *nblocks = g.x * g.y * g.z;
+  }
+  HIPCHK(hipGetLastError());
+}
+
+template <bool PCG>
+static unsigned launch_stencil64(const Lattice &L, const KeCoef &ke, const double *young,
+                                 const uint8_t *freemask, const double *src, const double *dinv,
+                                 const double *p_old, double *p_new, double *y, PcgScalars *S,
+                                 double *partials, hipStream_t s) {
+  unsigned nb = 0;
+  switch (ke.sym) {
+    case 0: launch_stencil64_t<PCG, 0>(L, ke, young, freemask, src, dinv, p_old, p_new, y, S, partials, s, &nb); break;
+    case 1: launch_stencil64_t<PCG, 1>(L, ke, young, freemask, src, dinv, p_old, p_new, y, S, partials, s, &nb); break;
+    case 2: launch_stencil64_t<PCG, 2>(L, ke, young, freemask, src, dinv, p_old, p_new, y, S, partials, s, &nb); break;
+    default: throw Error(kInvalidArgument, "unknown KE0 symmetry class");
+  }
+  return nb;
+}
+
+void launch_apply_k64(const Lattice &L, const KeCoef &ke, const double *young,
                       const uint8_t *freemask, const double *u, double *y, hipStream_t s) {
-  const int zc = stencil_zchunk(L);
-  hipLaunchKernelGGL(k_stencil64z<false>, zgrid(L, zc), dim3(NT), 0, s, L, zc, ke, young, freemask,
-                     u, nullptr, nullptr, nullptr, y, nullptr, nullptr);
-  HIPCHK(hipGetLastError());
+  launch_stencil64<false>(L, ke, young, freemask, u, nullptr, nullptr, nullptr, y, nullptr, nullptr,
+                          s);
 }
 
-void launch_apply_k64_tiled(const Lattice &L, const double *ke, const double *young,
-                            const uint8_t *freemask, const double *u, double *y, hipStream_t s) {
-  hipLaunchKernelGGL(k_stencil64<false>, stencil_grid(L), dim3(NT), 0, s, L, ke, young, freemask, u,
-                     nullptr, nullptr, nullptr, y, nullptr, nullptr, nullptr);
-  HIPCHK(hipGetLastError());
-}
-
-void launch_pcg_stencil64(const Lattice &L, const double *ke, const double *young,
+void launch_pcg_stencil64(const Lattice &L, const KeCoef &ke, const double *young,
                           const uint8_t *freemask, const double *r, const double *dinv,
                           const double *p_old, double *p_new, double *q, PcgScalars *S,
                           double *partials, unsigned *counter, hipStream_t s, bool finalize) {
   (void)counter;
-  const int zc = stencil_zchunk(L);
-  const dim3 g = zgrid(L, zc);
-  hipLaunchKernelGGL(k_stencil64z<true>, g, dim3(NT), 0, s, L, zc, ke, young, freemask, r, dinv,
-                     p_old, p_new, q, S, partials);
-  HIPCHK(hipGetLastError());
-  if (finalize) launch_finalize(kFinPcgAlpha, 1, partials, g.x * g.y * g.z, S, nullptr, nullptr, s);
+  const unsigned nb =
+      launch_stencil64<true>(L, ke, young, freemask, r, dinv, p_old, p_new, q, S, partials, s);
+  if (finalize) launch_finalize(kFinPcgAlpha, 1, partials, nb, S, nullptr, nullptr, s);
 }
 
 void launch_pcg_update64(long long n, double *x, double *r, const double *p, const double *q,

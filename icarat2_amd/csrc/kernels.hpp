@@ -10,6 +10,25 @@ namespace icarat_hip {
 
 // ---- k_fp64.hip ----------------------------------------------------------
 // `ke` below is the 576-entry KE0 in device memory (row-major 24x24).
+
+// KE0 as the FP64 stencil kernels take it.  With sym > 0 the kernels use the
+// orbit values v (kernel arguments, resident in scalar registers) through the
+// compile-time orbit tables of ke_sym_table.hpp instead of loading KE0:
+// sym 1 = box symmetry (48 values), sym 2 = cube symmetry (12 values).
+struct KeVals {
+  double v[48];
+};
+struct KeCoef {
+  const double *ke = nullptr;  // device KE0
+  int sym = 0;
+  int kernel = 0;              // 0: z-marching column kernel, 1: 3-D tile kernel
+  KeVals vals{};
+};
+// Orbit values of a host KE0 (sym = 2 if the cube table reproduces it to
+// 1e-13 of max |KE0|, else 1 for the box table, else 0).
+KeCoef make_ke_coef(const double *ke_host, const double *ke_dev);
+// as make_ke_coef restricted to one class (sym = cls if it reproduces KE0, else 0)
+KeCoef make_ke_coef_class(const double *ke_host, const double *ke_dev, int cls);
 // young[e] = SIMP(x[e]) (reference problem/elasticity.hpp:13-17)
 void launch_young(const double *x, double *young, long long nelx, double y1, double y2, double p,
                   hipStream_t s);
@@ -19,15 +38,13 @@ void launch_diag_inv(const Lattice &L, const double *ke, const double *young,
 // number of workgroups of the largest K.u launch (sizes the partials buffer)
 long long stencil_blocks(const Lattice &L);
 // y = K u on the lattice; y masked to free DOFs.  u is used as given
-// (Dirichlet values included).  The z-marching kernel; the _tiled variant is
-// the 3-D halo-tile kernel kept as a cross-check.
-void launch_apply_k64_tiled(const Lattice &L, const double *ke, const double *young,
-                            const uint8_t *freemask, const double *u, double *y, hipStream_t s);
-void launch_apply_k64(const Lattice &L, const double *ke, const double *young,
+// (Dirichlet values included).  ke.kernel selects the z-marching or the 3-D
+// halo-tile kernel.
+void launch_apply_k64(const Lattice &L, const KeCoef &ke, const double *young,
                       const uint8_t *freemask, const double *u, double *y, hipStream_t s);
 // One fused Jacobi-PCG iteration, stencil half: p_new = dinv*r + beta*p_old;
 // q = K p_new; p.q reduced; alpha = rho / p.q.
-void launch_pcg_stencil64(const Lattice &L, const double *ke, const double *young,
+void launch_pcg_stencil64(const Lattice &L, const KeCoef &ke, const double *young,
                           const uint8_t *freemask, const double *r, const double *dinv,
                           const double *p_old, double *p_new, double *q, PcgScalars *S,
                           double *partials, unsigned *counter, hipStream_t s,

@@ -158,29 +158,33 @@ void LearnedPrecond::apply(const double *r, double *z) {
   ctx_.sync();
 }
 
-void LearnedPrecond::build_graph(int kmode) {
-  const int m = kmode + 1;
-  if (exec_[m][0]) return;
+void LearnedPrecond::enqueue_batch(int kmode, int par) {
   hipStream_t s = ctx_.stream();
   Context &c = ctx_;
   const Lattice &L = c.lattice();
-  const int batch = std::max(1, o_.batch_iters);
+  for (int j = 0; j < std::max(1, o_.batch_iters); ++j) {
+    double *p_old = c.d_p[(j + par) & 1], *p_new = c.d_p[(j + par + 1) & 1];
+    // p = z + beta p_old (the "dinv" operand is 1 at free DOFs), q = K p, alpha
+    if (kmode < 0)
+      launch_pcg_stencil64(L, c.kc_, c.d_young, c.d_free, z_, ones_, p_old, p_new, c.d_q, c.d_pcg,
+                           c.d_partials, c.d_counter, s);
+    else
+      launch_pcg_stencil_mfma64(L, kmode, c.d_ke32, c.d_young32, c.d_free, z_, ones_, p_old, p_new,
+                                c.d_q, c.d_pcg, c.d_partials, s);
+    launch_fpcg_update(c.n3_, c.d_x, c.d_r, p_new, c.d_q, c.d_pcg, c.d_partials, s);
+    precondition(c.d_r);
+    launch_zdot_beta(c.n3_, z_, c.d_r, c.d_q, c.d_pcg, c.d_partials, s);
+  }
+}
+
+void LearnedPrecond::build_graph(int kmode) {
+  const int m = kmode + 1;
+  if (!ctx_.use_graphs_ || exec_[m][0]) return;
+  hipStream_t s = ctx_.stream();
   for (int par = 0; par < 2; ++par) {
     HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     try {
-      for (int j = 0; j < batch; ++j) {
-        double *p_old = c.d_p[(j + par) & 1], *p_new = c.d_p[(j + par + 1) & 1];
-        // p = z + beta p_old (dinv = 1 at free DOFs), q = K p, alpha
-        if (kmode < 0)
-          launch_pcg_stencil64(L, c.d_ke, c.d_young, c.d_free, z_, ones_, p_old, p_new, c.d_q,
-                               c.d_pcg, c.d_partials, c.d_counter, s);
-        else
-          launch_pcg_stencil_mfma64(L, kmode, c.d_ke32, c.d_young32, c.d_free, z_, ones_, p_old,
-                                    p_new, c.d_q, c.d_pcg, c.d_partials, s);
-        launch_fpcg_update(c.n3_, c.d_x, c.d_r, p_new, c.d_q, c.d_pcg, c.d_partials, s);
-        precondition(c.d_r);
-        launch_zdot_beta(c.n3_, z_, c.d_r, c.d_q, c.d_pcg, c.d_partials, s);
-      }
+      enqueue_batch(kmode, par);
     } catch (...) {
       hipGraph_t g = nullptr;
       (void)hipStreamEndCapture(s, &g);
@@ -230,7 +234,10 @@ void LearnedPrecond::solve(const icarat_solve_opts &o, icarat_solve_stats &st) {
       const long long max_batches = (long long)(o.maxit - h->iter) / batch + 3;
       for (long long b = 0; b < max_batches && !h->done; ++b) {
         // the p buffers alternate each iteration; a batch starts at the parity of iter
-        HIPCHK(hipGraphLaunch(exec_[kmode + 1][h->iter & 1], s));
+        if (ctx_.use_graphs_)
+          HIPCHK(hipGraphLaunch(exec_[kmode + 1][h->iter & 1], s));
+        else
+          enqueue_batch(kmode, h->iter & 1);
         HIPCHK(hipMemcpyAsync(h, c.d_pcg.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
         c.sync();
         const double rn = std::sqrt(h->rr);
@@ -247,7 +254,7 @@ void LearnedPrecond::solve(const icarat_solve_opts &o, icarat_solve_stats &st) {
     }
     if (h->status != 0) break;
     // verified stop: the FP64 true residual replaces the recursive one
-    launch_apply_k64(L, c.d_ke, c.d_young, c.d_free, c.d_x, c.d_y, s);
+    launch_apply_k64(L, c.kc_, c.d_young, c.d_free, c.d_x, c.d_y, s);
     launch_replace64(c.n3_, c.d_b, c.d_y, c.d_r, c.d_dinv, c.d_pcg, c.d_partials, s);
     HIPCHK(hipMemcpyAsync(h, c.d_pcg.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
     c.sync();

@@ -70,6 +70,7 @@ class LearnedPrecond {
   hipGraph_t graph_[3][2] = {};
   hipGraphExec_t exec_[3][2] = {};
   void build_graph(int kmode);
+  void enqueue_batch(int kmode, int par);
   void precondition(const double *r);  // z_ = M(r)
 };
 

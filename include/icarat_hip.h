@@ -263,10 +263,22 @@ int icarat_hip_unet_train_step(icarat_hip_unet *net, const float *x, const float
  *   kernel 0: FP64 PCG stencil half (p update + K.p + p.q), 1152 flop/element
  *   kernel 1: FP64 PCG vector half (x, r update, r.z, r.r)
  *   kernel 2: density filter (isSens = false) at the set-up radius
+ *   kernel 3: as 0 with KE0 read from memory (no symmetry classes)
+ *   kernel 4: as 0 with the other FP64 kernel shape (see icarat_hip_set_stencil)
+ *   kernel 5: PCG stencil half with the FP32 MFMA contraction (FP64 vectors)
+ *   kernel 6: PCG stencil half with the BF16 MFMA contraction (FP64 vectors)
  * Needs set_density (and filter_setup for kernel 2).  Overwrites the solver's
  * work vectors. */
 int icarat_hip_time_kernel(icarat_hip_ctx *ctx, int32_t kernel, int32_t reps, double *avg_ms,
                            double *flops, double *bytes);
+
+/* No reference counterpart: selects the FP64 K.u kernel of this context.
+ * kernel: 0 = z-marching node columns (default), 1 = 3-D halo tiles.
+ * sym_max: highest KE0 symmetry class the kernel may use: 0 = KE0 read from
+ * memory, 1 = box orbits (48 values in registers), 2 = cube orbits (12 values,
+ * default); the class actually used is the highest one that reproduces KE0.
+ * *sym_used (may be NULL) returns it. */
+int icarat_hip_set_stencil(icarat_hip_ctx *ctx, int32_t kernel, int32_t sym_max, int32_t *sym_used);
 
 /* ---- inputs ---------------------------------------------------------------- */
 /* Seeded smooth density field (sum of nblob isotropic Gaussians, amplitude

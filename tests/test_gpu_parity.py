@@ -39,6 +39,29 @@ def test_apply_k_matches_oracle(gpu, oracle, dims):
     assert _rel(c.diag(), o.diag()) < 1e-14
 
 
+@pytest.mark.parametrize("kernel", [0, 1])
+@pytest.mark.parametrize("sym_max", [0, 1, 2])
+@pytest.mark.parametrize("lengths", [None, (40.0, 9.0, 21.0)])
+def test_apply_k_variants(gpu, oracle, kernel, sym_max, lengths):
+    """Every FP64 K.u kernel (z-marching / tiles) and KE0 source (memory, box
+    orbits, cube orbits) on a cubic-voxel and a box-voxel mesh."""
+    dims = (20, 9, 7)
+    L = lengths or tuple(float(d) for d in dims)
+    c = gpu.Context(*dims, *L)
+    o = oracle.Problem(*dims, *L)
+    for p in (c, o):
+        p.add_dirichlet([1, 1, 1, 0, 0, 0], [0.0] * 6, [[0, 0], [0, dims[1]], [0, dims[2]]])
+        p.make_dof()
+    rng = np.random.default_rng(14)
+    x = rng.uniform(1e-3, 1.0, c.nelx)
+    c.set_density(x)
+    o.set_density(x)
+    used = c.set_stencil(kernel, sym_max)
+    assert used == (min(sym_max, 2) if lengths is None else min(sym_max, 1))
+    u = rng.standard_normal(c.numeq)
+    assert _rel(c.apply_K(u), o.spmv(u)) < 1e-13
+
+
 def test_apply_k_partial_dirichlet(gpu, oracle):
     args = [([1, 0, 1], [0.0] * 3, [[0, 0], [0, 6], [0, 5]]),
             ([0, 1, 0], [0.0] * 3, [[9, 9], [0, 0], [0, 5]])]
