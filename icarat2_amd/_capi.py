@@ -54,7 +54,13 @@ class LearnedOpts(C.Structure):
     _fields_ = [("train_iters", C.c_int32), ("train_steps", C.c_int32),
                 ("snapshot_every", C.c_int32), ("max_samples", C.c_int32),
                 ("batch_iters", C.c_int32), ("stall_iters", C.c_int32), ("lr", C.c_float),
-                ("reserved", C.c_int32), ("seed", C.c_uint64)]
+                ("gate", C.c_float), ("seed", C.c_uint64)]
+
+
+class LearnedInfo(C.Structure):
+    _fields_ = [("samples", C.c_int32), ("active", C.c_int32), ("steps", C.c_int64),
+                ("loss", C.c_double), ("val_loss", C.c_double), ("gain", C.c_double),
+                ("n_params", C.c_int64), ("grid", C.c_int32 * 3), ("harvested", C.c_int32)]
 
 
 fp = C.POINTER(C.c_float)
@@ -97,7 +103,7 @@ _SIGS = {
     "icarat_hip_learned_harvest": (C.c_int, [C.c_void_p, C.POINTER(SolveOpts), dp,
                                              C.POINTER(SolveStats)]),
     "icarat_hip_learned_train": (C.c_int, [C.c_void_p, C.c_int32, dp]),
-    "icarat_hip_learned_info": (C.c_int, [C.c_void_p, i32p, i64p, dp, i64p, i32p]),
+    "icarat_hip_learned_info": (C.c_int, [C.c_void_p, C.POINTER(LearnedInfo)]),
     "icarat_hip_learned_get_params": (C.c_int, [C.c_void_p, fp]),
     "icarat_hip_learned_set_params": (C.c_int, [C.c_void_p, fp]),
     "icarat_hip_learned_apply": (C.c_int, [C.c_void_p, dp, dp]),
@@ -416,13 +422,11 @@ class Context:
         return loss.value
 
     def learned_info(self):
-        ns, nst, n = C.c_int32(), C.c_int64(), C.c_int64()
-        loss = C.c_double()
-        g = (C.c_int32 * 3)()
-        self._chk(self.L.icarat_hip_learned_info(self.h, C.byref(ns), C.byref(nst), C.byref(loss),
-                                                 C.byref(n), g), "learned_info")
-        return {"samples": ns.value, "steps": nst.value, "loss": loss.value,
-                "n_params": n.value, "grid": tuple(g)}
+        d = LearnedInfo()
+        self._chk(self.L.icarat_hip_learned_info(self.h, C.byref(d)), "learned_info")
+        out = {k: getattr(d, k) for k, _ in LearnedInfo._fields_}
+        out["grid"] = tuple(d.grid)
+        return out
 
     def learned_get_params(self):
         out = np.zeros(unet_param_count(), dtype=np.float32)

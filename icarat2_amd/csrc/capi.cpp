@@ -296,6 +296,7 @@ int icarat_hip_learned_default_opts(icarat_learned_opts *o) {
   o->batch_iters = 8;
   o->stall_iters = 400;
   o->lr = 1e-3f;
+  o->gate = 0.8f;
   o->seed = 1;
   return ICARAT_HIP_OK;
 }
@@ -327,20 +328,24 @@ int icarat_hip_learned_train(icarat_hip_ctx *h, int32_t steps, double *loss) {
   });
 }
 
-int icarat_hip_learned_info(const icarat_hip_ctx *hc, int32_t *samples, int64_t *steps,
-                            double *loss, int64_t *n_params, int32_t grid[3]) {
+int icarat_hip_learned_info(const icarat_hip_ctx *hc, icarat_learned_info *info) {
   icarat_hip_ctx *h = const_cast<icarat_hip_ctx *>(hc);
+  if (!info) return fail(h, ICARAT_HIP_INVALID_ARGUMENT, "null argument");
   return guard(h, [&](icarat_hip::Context &c) {
     icarat_hip::LearnedPrecond &lp = c.learned();
-    if (samples) *samples = lp.samples();
-    if (steps) *steps = lp.total_steps();
-    if (loss) *loss = lp.last_loss();
-    if (n_params) *n_params = icarat_hip::UNet::host_param_count();
-    if (grid) {
-      grid[0] = lp.grid().X;
-      grid[1] = lp.grid().Y;
-      grid[2] = lp.grid().Z;
-    }
+    icarat_learned_info d{};
+    d.samples = lp.samples();
+    d.active = lp.active() ? 1 : 0;
+    d.steps = lp.total_steps();
+    d.loss = lp.last_loss();
+    d.val_loss = lp.val_loss();
+    d.gain = lp.gain();
+    d.n_params = icarat_hip::UNet::host_param_count();
+    d.grid[0] = lp.grid().X;
+    d.grid[1] = lp.grid().Y;
+    d.grid[2] = lp.grid().Z;
+    d.harvested = lp.harvested();
+    *info = d;
   });
 }
 
@@ -352,7 +357,7 @@ int icarat_hip_learned_get_params(const icarat_hip_ctx *hc, float *params) {
 
 int icarat_hip_learned_set_params(icarat_hip_ctx *h, const float *params) {
   if (!params) return fail(h, ICARAT_HIP_INVALID_ARGUMENT, "null argument");
-  return guard(h, [&](icarat_hip::Context &c) { c.learned().net().set_params(params); });
+  return guard(h, [&](icarat_hip::Context &c) { c.learned().set_params(params); });
 }
 
 int icarat_hip_learned_apply(icarat_hip_ctx *h, const double *r, double *z) {

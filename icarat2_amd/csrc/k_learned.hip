@@ -130,11 +130,11 @@ __global__ void k_density_feature(Lattice L, NetGrid g, const double *__restrict
   feat[v] = fminf(fmaxf(f, 0.f), 1.f);
 }
 
-// z = dinv r + sigma * P(out) on free DOFs
+// z = dinv r + gain * sigma * P(out) on free DOFs
 __global__ void k_prolong_add(Lattice L, NetGrid g, const float *__restrict__ out, int cs,
                               const double *__restrict__ scale, const double *__restrict__ r,
                               const double *__restrict__ dinv, const uint8_t *__restrict__ fmask,
-                              double *__restrict__ z, int use_net) {
+                              double *__restrict__ z, int use_net, double gain) {
   const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (n >= L.numnp) return;
   const int x = (int)(n % L.NX), y = (int)((n / L.NX) % L.NY), zz = (int)(n / ((long long)L.NX * L.NY));
@@ -165,13 +165,18 @@ __global__ void k_prolong_add(Lattice L, NetGrid g, const float *__restrict__ ou
           for (int k = 0; k < 3; ++k) corr[k] += w * (double)out[v * cs + k];
         }
   }
-  const double sig = scale[0];
+  const double sig = use_net ? gain * scale[0] : 0.0;
   const unsigned m = fmask[n];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const long long q = k * L.Np + n;
     z[q] = ((m >> k) & 1u) ? dinv[q] * r[q] + sig * corr[k] : 0.0;
   }
+}
+
+__global__ void k_scale_f32(long long n, float a, float *__restrict__ x) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) x[i] *= a;
 }
 
 // target = (R x* - R x_k) / sigma_k - zr_k
@@ -296,9 +301,14 @@ void launch_density_feature(const Lattice &L, const NetGrid &g, const double *yo
 
 void launch_prolong_add(const Lattice &L, const NetGrid &g, const float *out, int cs,
                         const double *scale, const double *r, const double *dinv,
-                        const uint8_t *fmask, double *z, int use_net, hipStream_t s) {
+                        const uint8_t *fmask, double *z, int use_net, double gain, hipStream_t s) {
   hipLaunchKernelGGL(k_prolong_add, dim3(blocks_for(L.numnp, 256)), dim3(256), 0, s, L, g, out, cs,
-                     scale, r, dinv, fmask, z, use_net);
+                     scale, r, dinv, fmask, z, use_net, gain);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_scale_f32(long long n, float a, float *x, hipStream_t s) {
+  hipLaunchKernelGGL(k_scale_f32, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, a, x);
   HIPCHK(hipGetLastError());
 }
 

@@ -49,13 +49,7 @@ LearnedPrecond::LearnedPrecond(Context &ctx, const icarat_learned_opts &o) : ctx
   rng_ = o_.seed * 0x9e3779b97f4a7c15ULL + 1;
 }
 
-LearnedPrecond::~LearnedPrecond() {
-  for (int m = 0; m < 3; ++m)
-    for (int p = 0; p < 2; ++p) {
-      if (exec_[m][p]) (void)hipGraphExecDestroy(exec_[m][p]);
-      if (graph_[m][p]) (void)hipGraphDestroy(graph_[m][p]);
-    }
-}
+LearnedPrecond::~LearnedPrecond() { drop_graphs(); }
 
 // ---- harvesting -------------------------------------------------------------
 void LearnedPrecond::begin_harvest() {
@@ -93,16 +87,61 @@ void LearnedPrecond::end_harvest(bool converged) {
     std::vector<double> sig(cap_);
     HIPCHK(hipMemcpyAsync(sig.data(), ssig_.p, cap_ * sizeof(double), hipMemcpyDeviceToHost, s));
     ctx_.sync();
+    std::vector<int> made;
     for (int k : pending_) {
       // a zero residual carries no information
       if (!(sig[k] > 0.0)) continue;
       float *t = st_.p + (size_t)k * ncell_ * 3;
       launch_make_target(ncell_ * 3, rxs_, t, 1.0 / sig[k], szr_.p + (size_t)k * ncell_ * 3, t, s);
+      made.push_back(k);
+    }
+    if (!made.empty() && !gain_set_) {
+      // output scale: the median RMS of the first solve's targets, so that the
+      // network works at unit scale
+      std::vector<float> h((size_t)ncell_ * 3);
+      std::vector<double> rms;
+      for (int k : made) {
+        HIPCHK(hipMemcpyAsync(h.data(), st_.p + (size_t)k * ncell_ * 3, h.size() * sizeof(float),
+                              hipMemcpyDeviceToHost, s));
+        ctx_.sync();
+        double a = 0.0;
+        for (float v : h) a += (double)v * v;
+        rms.push_back(std::sqrt(a / (double)h.size()));
+      }
+      std::nth_element(rms.begin(), rms.begin() + rms.size() / 2, rms.end());
+      gain_ = rms[rms.size() / 2] > 0.0 ? rms[rms.size() / 2] : 1.0;
+      gain_set_ = true;
+    }
+    for (int k : made) {
+      launch_scale_f32(ncell_ * 3, (float)(1.0 / gain_), st_.p + (size_t)k * ncell_ * 3, s);
       valid_[k] = 1;
     }
     ctx_.sync();
   }
   pending_.clear();
+}
+
+void LearnedPrecond::set_params(const float *host) {
+  net_->set_params(host);
+  active_ = true;
+}
+
+double LearnedPrecond::evaluate() {
+  hipStream_t s = ctx_.stream();
+  double acc = 0.0;
+  int cnt = 0;
+  for (int k = 0; k < cap_; ++k) {
+    if (!valid_[k]) continue;
+    net_->eval_loss(sx_.p + (size_t)k * ncell_ * 8, st_.p + (size_t)k * ncell_ * 3);
+    double l2[2];
+    HIPCHK(hipMemcpyAsync(l2, net_->loss2(), 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+    ctx_.sync();
+    if (l2[1] > 0.0) {
+      acc += l2[0] / l2[1];
+      ++cnt;
+    }
+  }
+  return cnt ? acc / cnt : -1.0;
 }
 
 int LearnedPrecond::samples() const {
@@ -138,6 +177,9 @@ double LearnedPrecond::train(int steps) {
   }
   total_steps_ += steps;
   last_loss_ = cnt ? acc / cnt : -1.0;
+  // the gate: the network is used only if it explains enough of the targets
+  val_loss_ = evaluate();
+  active_ = val_loss_ >= 0.0 && val_loss_ < (double)o_.gate;
   return last_loss_;
 }
 
@@ -145,10 +187,14 @@ double LearnedPrecond::train(int steps) {
 void LearnedPrecond::precondition(const double *r) {
   hipStream_t s = ctx_.stream();
   const Lattice &L = ctx_.lattice();
+  if (!active_) {
+    launch_prolong_add(L, g0_, out_, 8, scale_, r, ctx_.d_dinv, ctx_.d_free, z_, 0, 0.0, s);
+    return;
+  }
   launch_restrict_residual(L, g0_, r, ctx_.d_dinv, s3_, ctx_.d_partials, scale_, s);
   launch_pack_input(g0_, feat_, s3_, scale_, xin_, nullptr, s);
   net_->forward(xin_, out_);
-  launch_prolong_add(L, g0_, out_, 8, scale_, r, ctx_.d_dinv, ctx_.d_free, z_, 1, s);
+  launch_prolong_add(L, g0_, out_, 8, scale_, r, ctx_.d_dinv, ctx_.d_free, z_, 1, gain_, s);
 }
 
 void LearnedPrecond::apply(const double *r, double *z) {
@@ -177,8 +223,24 @@ void LearnedPrecond::enqueue_batch(int kmode, int par) {
   }
 }
 
+void LearnedPrecond::drop_graphs() {
+  for (int m = 0; m < 3; ++m)
+    for (int p = 0; p < 2; ++p) {
+      if (exec_[m][p]) (void)hipGraphExecDestroy(exec_[m][p]);
+      if (graph_[m][p]) (void)hipGraphDestroy(graph_[m][p]);
+      exec_[m][p] = nullptr;
+      graph_[m][p] = nullptr;
+    }
+}
+
 void LearnedPrecond::build_graph(int kmode) {
   const int m = kmode + 1;
+  // the captured batch bakes in whether the network runs and its gain
+  if (graph_active_ != active_ || graph_gain_ != gain_) {
+    drop_graphs();
+    graph_active_ = active_;
+    graph_gain_ = gain_;
+  }
   if (!ctx_.use_graphs_ || exec_[m][0]) return;
   hipStream_t s = ctx_.stream();
   for (int par = 0; par < 2; ++par) {

@@ -23,6 +23,8 @@ class LearnedPrecond {
   ~LearnedPrecond();
   const icarat_learned_opts &opts() const { return o_; }
   UNet &net() { return *net_; }
+  // explicit parameters: the network is used as given (gain 1 until a harvest sets it)
+  void set_params(const float *host);
   const NetGrid &grid() const { return g0_; }
 
   // Training data: snapshots of (x_k, r_k) taken every few batches of a
@@ -37,6 +39,9 @@ class LearnedPrecond {
   // without samples.
   double train(int steps);
   double last_loss() const { return last_loss_; }
+  double val_loss() const { return val_loss_; }
+  double gain() const { return gain_; }
+  bool active() const { return active_; }
   long long total_steps() const { return total_steps_; }
   // Flexible PCG with the network preconditioner on ctx.d_b -> ctx.d_x.
   void solve(const icarat_solve_opts &o, icarat_solve_stats &st);
@@ -64,12 +69,20 @@ class LearnedPrecond {
   int harvest_batches_ = 0;
   int harvested_ = 0;
   double last_loss_ = -1.0;
+  double val_loss_ = -1.0;
+  double gain_ = 1.0;     // targets are stored divided by gain_; the correction is scaled by it
+  bool gain_set_ = false; // gain_ comes from the first harvested solve
+  bool active_ = false;   // the network passed the gate
+  bool graph_active_ = false;
+  double graph_gain_ = 0.0;
   long long total_steps_ = 0;
   uint64_t rng_;
   // graphs of batches of flexible-PCG iterations [kmode + 1][parity]
   hipGraph_t graph_[3][2] = {};
   hipGraphExec_t exec_[3][2] = {};
   void build_graph(int kmode);
+  void drop_graphs();
+  double evaluate();  // mean relative squared error over the stored samples
   void enqueue_batch(int kmode, int par);
   void precondition(const double *r);  // z_ = M(r)
 };

@@ -19,10 +19,11 @@ void launch_pack_input(const NetGrid &g, const float *feat, const float *s3, con
                        __bf16 *xin, float *zr, hipStream_t s);
 void launch_density_feature(const Lattice &L, const NetGrid &g, const double *young, float *feat,
                             hipStream_t s);
-// z = dinv r + sigma P(out) (use_net = 0: z = dinv r)
+// z = dinv r + gain sigma P(out) (use_net = 0: z = dinv r)
 void launch_prolong_add(const Lattice &L, const NetGrid &g, const float *out, int cs,
                         const double *scale, const double *r, const double *dinv,
-                        const uint8_t *fmask, double *z, int use_net, hipStream_t s);
+                        const uint8_t *fmask, double *z, int use_net, double gain, hipStream_t s);
+void launch_scale_f32(long long n, float a, float *x, hipStream_t s);
 void launch_make_target(long long n, const float *rxs, const float *rxk, double inv_sigma,
                         const float *zr, float *t, hipStream_t s);
 // rho = z.r, beta = -alpha z.q / rho_old (Polak-Ribiere; 0 after a restart);

@@ -184,6 +184,11 @@ void UNet::backward_only(const __bf16 *x, const float *target) {
   backward(x);
 }
 
+void UNet::eval_loss(const __bf16 *x, const float *target) {
+  forward(x, nullptr);
+  launch_loss(g_[0].nvox(), 8, kOut, out_, target, loss2_, dout_, s_);
+}
+
 void UNet::train_step(const __bf16 *x, const float *target, float lr) {
   backward_only(x, target);
   ++step_;

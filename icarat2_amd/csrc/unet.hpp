@@ -39,6 +39,7 @@ class UNet {
   // backward, Adam.  The loss terms land in loss2[0..1] (device).
   void train_step(const __bf16 *x, const float *target, float lr);
   void backward_only(const __bf16 *x, const float *target);  // forward + loss + grads
+  void eval_loss(const __bf16 *x, const float *target);      // forward + loss
   double *loss2() { return loss2_.p; }
   long long flops_forward() const;
 

@@ -209,11 +209,24 @@ typedef struct icarat_learned_opts {
   int32_t batch_iters;     /* flexible-PCG iterations per graph launch */
   int32_t stall_iters;     /* fallback when the residual has not halved for this many iterations */
   float lr;                /* Adam learning rate */
-  int32_t reserved;
+  float gate;              /* the network is used only while its mean relative squared error
+                              on the stored samples is below gate (else z = D^-1 r) */
   uint64_t seed;           /* weight initialisation and sample order */
 } icarat_learned_opts;
 
-/* Fills *o with the defaults (10, 400, 64, 160, 8, 400, 1e-3, 0, 1). */
+typedef struct icarat_learned_info {
+  int32_t samples;         /* (residual, error) pairs held */
+  int32_t active;          /* 1 if the network passed the gate and is used in solves */
+  int64_t steps;           /* Adam steps taken */
+  double loss;             /* mean relative squared error of the last training steps */
+  double val_loss;         /* mean relative squared error over all stored samples */
+  double gain;             /* output scale: median RMS of the targets in residual units */
+  int64_t n_params;        /* network parameters */
+  int32_t grid[3];         /* cell grid X, Y, Z */
+  int32_t harvested;       /* harvesting solves finished */
+} icarat_learned_info;
+
+/* Fills *o with the defaults (10, 400, 64, 160, 8, 400, 1e-3, 0.8, 1). */
 int icarat_hip_learned_default_opts(icarat_learned_opts *o);
 /* (Re)creates the preconditioner of the context (o = NULL: defaults).  Needs make_dof. */
 int icarat_hip_learned_setup(icarat_hip_ctx *ctx, const icarat_learned_opts *o);
@@ -221,14 +234,12 @@ int icarat_hip_learned_setup(icarat_hip_ctx *ctx, const icarat_learned_opts *o);
  * JACOBI) that stores training snapshots; u may be NULL. */
 int icarat_hip_learned_harvest(icarat_hip_ctx *ctx, const icarat_solve_opts *opts, double *u,
                                icarat_solve_stats *stats);
-/* Adam steps on the stored samples; *loss = mean ||net - t||^2 / ||t||^2 of the
- * last steps (-1 without samples). */
+/* Adam steps on the stored samples, then the gate on all samples; *loss = mean
+ * ||net - t||^2 / ||t||^2 of the last steps (-1 without samples). */
 int icarat_hip_learned_train(icarat_hip_ctx *ctx, int32_t steps, double *loss);
-/* samples held, Adam steps taken, last loss, parameter count, cell grid {X,Y,Z};
- * any output may be NULL */
-int icarat_hip_learned_info(const icarat_hip_ctx *ctx, int32_t *samples, int64_t *steps,
-                            double *loss, int64_t *n_params, int32_t grid[3]);
-/* Network parameters in the layout of icarat_hip_unet_get_params. */
+int icarat_hip_learned_info(const icarat_hip_ctx *ctx, icarat_learned_info *info);
+/* Network parameters in the layout of icarat_hip_unet_get_params.  Setting
+ * them switches the network on (gain 1 until a harvest sets the gain). */
 int icarat_hip_learned_get_params(const icarat_hip_ctx *ctx, float *params);
 int icarat_hip_learned_set_params(icarat_hip_ctx *ctx, const float *params);
 /* z = M(r) for a numeq residual r (density as last set). */

@@ -1,7 +1,9 @@
 """The learned preconditioner: z = M(r) against the numpy restatement
 (oracle/unet.py precondition), flexible PCG with a zero network against
 Jacobi-PCG, online training, the ablation (same solve with and without the
-network) and the design loop against the FP64 Jacobi loop.
+network) and the design loop against the FP64 Jacobi loop.  The network is used only
+while its relative squared error over the stored samples is below the gate
+(0.8); a network that fails the gate leaves flexible PCG at z = D^-1 r.
 Tolerances:
   M(r) with a zero network ...... D^-1 r to 1e-15 relative (no network term)
   M(r) network correction ....... 5e-2 normwise (BF16 network vs FP64 oracle)
@@ -105,11 +107,13 @@ def test_training_and_ablation(gpu):
         c.set_density(x)
         _, st = c.learned_harvest(tol=1e-8)
         assert st.status == 0
-    info = c.learned_info()
-    assert info["samples"] > 4
+    assert c.learned_info()["samples"] > 4
     loss = c.learned_train(300)
-    print(f"samples {info['samples']}, relative squared error after training {loss:.3f}")
-    assert 0.0 <= loss < 1.0
+    info = c.learned_info()
+    print(f"samples {info['samples']}, gain {info['gain']:.3g}, relative squared error: last "
+          f"steps {loss:.3f}, all samples {info['val_loss']:.3f}, network active {info['active']}")
+    assert loss >= 0.0 and info["val_loss"] >= 0.0 and info["steps"] == 300
+    assert info["active"] == (info["val_loss"] < 0.8)
     x = np.clip(base * (1.0 + 0.1 * rng.standard_normal(base.size)), 1e-3, 1.0)
     c.set_density(x)
     uj, sj = c.solve(tol=1e-8)
@@ -119,6 +123,9 @@ def test_training_and_ablation(gpu):
           f"relres {sl.true_relres:.2e})")
     assert sl.status == 0 and sl.true_relres <= 1e-8
     assert np.max(np.abs(ul - uj)) < 1e-6 * np.max(np.abs(uj))
+    if not info["active"]:
+        # a network that failed the gate is not used: flexible PCG with z = D^-1 r
+        assert sl.fallback == 0 and abs(sl.iters - sj.iters) <= 0.02 * sj.iters + 2
 
 
 def test_simp_learned_matches_fp64(gpu):
@@ -137,3 +144,5 @@ def test_simp_learned_matches_fp64(gpu):
     assert np.max(np.abs(xl - xj)) < 1e-6
     assert np.max(np.abs(hl[:, 0] - hj[:, 0]) / hj[:, 0]) < 1e-8
     assert np.all(hl[:, 3] <= 1e-10)
+    # the gate never lets the network make the loop slower than Jacobi by more than noise
+    assert sl.pcg_iters <= 1.05 * sj.pcg_iters
