@@ -33,7 +33,33 @@ METHODS = {
     "bf16": (2, 0),
     "learned": (1, 1),
 }
-MI355X_PEAK = {"fp64_tflops": 78.6, "hbm_gbs": 8000.0}
+MI355X_PEAK = {"fp64_tflops": 78.6, "fp32_mfma_tflops": 157.3, "bf16_mfma_tflops": 2500.0,
+               "hbm_gbs": 8000.0}
+# dominant kernel of each method: (timing-hook id, kernel, compute peak key)
+DOMINANT = {
+    "fp64": (0, "k_stencil64z<PCG> (FP64 VALU, KE0 orbit values in SGPRs)", "fp64_tflops"),
+    "learned": (0, "k_stencil64z<PCG> (FP64 VALU, KE0 orbit values in SGPRs)", "fp64_tflops"),
+    "fp32": (5, "k_stencil_mfma<FP32> (v_mfma_f32_16x16x4_f32, FP64 vectors)", "fp32_mfma_tflops"),
+    "bf16": (6, "k_stencil_mfma<BF16> (v_mfma_f32_16x16x32_bf16, FP64 vectors)", "bf16_mfma_tflops"),
+}
+
+
+def traffic_from_profiles(kernel_prefix):
+    """HBM bytes per launch of a kernel from the committed PMC summary
+    (profiles/pmc_traffic_c2.txt, FETCH_SIZE + WRITE_SIZE scaled by the copy
+    kernel's calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_c2.txt")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) >= 2 and parts[0] == "TRAFFIC_BYTES" and kernel_prefix in line:
+                try:
+                    return float(parts[-1])
+                except ValueError:
+                    return None
+    return None
 
 
 def log(msg):
@@ -141,13 +167,20 @@ def main():
     cfg = CONFIGS[a.config]
     c, x0, hist, st, dt = run_gpu(A, cfg, a.method, a.steps, a.warmup, a.tol)
     pcg_per_solve = float(hist[:, 2].mean())
-    # roofline of the dominant kernel (FP64 PCG stencil half), timed with HIP
-    # events on the library's stream
-    import numpy as np
+    # roofline of the dominant kernel, timed with HIP events on the library's
+    # stream (the stream the kernel is launched on)
     c.set_density(c.density_filter(x0))
-    ms, flops, nbytes = c.time_kernel(0, a.reps)
-    achieved = flops / (ms * 1e-3) / 1e12
+    kid, kname, peak_key = DOMINANT[a.method]
+    ms, flops, nbytes = c.time_kernel(kid, a.reps)
+    t_s = ms * 1e-3
+    peak_f = MI355X_PEAK[peak_key]
+    compute_bound = flops / (peak_f * 1e12) >= nbytes / (MI355X_PEAK["hbm_gbs"] * 1e9)
+    if compute_bound:
+        achieved, peak, unit = flops / t_s / 1e12, peak_f, "TFLOP/s"
+    else:
+        achieved, peak, unit = nbytes / t_s / 1e9, MI355X_PEAK["hbm_gbs"], "GB/s"
     ms_v, _, bytes_v = c.time_kernel(1, a.reps)
+    ms_c, _, bytes_c = c.time_kernel(7, a.reps)
     out = {
         "metric": "SIMP iters/s + solve time to 1e-8 FP64 residual: learned/mixed vs plain FP64",
         "value": a.steps / dt,
@@ -170,14 +203,15 @@ def main():
         "max_true_relres": st.max_true_relres,
         "train_s_timed": st.train_s,
         "fallbacks": int(st.fallbacks),
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": MI355X_PEAK["fp64_tflops"],
-                     "unit": "TFLOP/s", "frac": achieved / MI355X_PEAK["fp64_tflops"],
-                     "traffic": None,
-                     "kernel": "k_stencil64<PCG> (FP64 VALU; MI355X FP64 vector = matrix peak)",
-                     "avg_ms": ms, "algorithmic_flops": flops, "compulsory_bytes": nbytes,
-                     "hbm_gbs_at_compulsory_bytes": nbytes / (ms * 1e-3) / 1e9},
-        "vector_kernel": {"avg_ms": ms_v, "bytes": bytes_v,
+        "roofline": {"bound": "mfma" if compute_bound else "hbm", "achieved": achieved,
+                     "peak": peak, "unit": unit, "frac": achieved / peak,
+                     "traffic": traffic_from_profiles(kname.split("<")[0]),
+                     "kernel": kname, "avg_ms": ms, "algorithmic_flops": flops,
+                     "compulsory_bytes": nbytes, "hbm_gbs_at_compulsory_bytes": nbytes / t_s / 1e9},
+        "vector_kernel": {"kernel": "k_pcg_update64", "avg_ms": ms_v, "bytes": bytes_v,
                           "gbs": bytes_v / (ms_v * 1e-3) / 1e9},
+        "copy_kernel": {"kernel": "k_copy64", "avg_ms": ms_c, "bytes": bytes_c,
+                        "gbs": bytes_c / (ms_c * 1e-3) / 1e9},
     }
     if not a.no_cpu:
         log("[bench] cpu baseline sample")

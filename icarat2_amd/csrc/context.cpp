@@ -262,7 +262,12 @@ PcgScalars Context::pcg64(double tol, int maxit, bool warm, int kmode, double de
   fell_back_ = 0;
   double best = std::sqrt(h_pcg.p->rr);
   int stalls = 0;
+  // a low-precision contraction whose recursive residual stagnates never asks
+  // for a replacement: force one after kStallWindow iterations
+  constexpr int kStallWindow = 1000;
+  int last_repl = h_pcg.p->iter;
   while (!h_pcg.p->done || h_pcg.p->status == 0 || h_pcg.p->status == 3) {
+    bool forced = false;
     if (!h_pcg.p->done) {
       // iterate until the recursive stop test, a replacement, a cap or a breakdown
       const long long max_batches = (long long)(maxit - h_pcg.p->iter) / kBatch + 3;
@@ -271,10 +276,15 @@ PcgScalars Context::pcg64(double tol, int maxit, bool warm, int kmode, double de
         d2h(h_pcg.p, d_pcg.p, 1, stream_);
         sync();
         if (batch_hook_ && !h_pcg.p->done) batch_hook_();
+        if (kmode >= 0 && !h_pcg.p->done && h_pcg.p->iter - last_repl >= kStallWindow) {
+          forced = true;
+          break;
+        }
       }
-      if (!h_pcg.p->done) throw Error(kInternal, "PCG did not reach its stop test");
+      if (!h_pcg.p->done && !forced) throw Error(kInternal, "PCG did not reach its stop test");
     }
-    if (h_pcg.p->status != 0 && h_pcg.p->status != 3) break;
+    if (!forced && h_pcg.p->status != 0 && h_pcg.p->status != 3) break;
+    last_repl = h_pcg.p->iter;
     // the FP64 true residual replaces the recursive one (verified stop /
     // reliable update); the search direction is kept
     launch_apply_k64(L_, kc_, d_young, d_free, d_x, d_y, stream_);
@@ -287,13 +297,19 @@ PcgScalars Context::pcg64(double tol, int maxit, bool warm, int kmode, double de
     if (rn < 0.5 * best) {
       best = rn;
       stalls = 0;
-    } else if (++stalls >= 4 && kmode >= 0) {
+    } else if ((++stalls >= 4 || forced) && kmode >= 0) {
+      // (a forced replacement that did not halve the residual: the recursive
+      // residual stagnated for a whole window)
       // the low-precision contraction stopped making progress: continue the
       // same iteration with the FP64 contraction
       kmode = -1;
       fell_back_ = 1;
       stalls = 0;
       build_pcg_graph(kmode);
+      // restart the recurrence (beta = 0): the search directions built with
+      // the low-precision operator are not conjugate for the FP64 one
+      h_pcg.p->beta = 0.0;
+      h2d(d_pcg.p, h_pcg.p, 1, stream_);
     }
     if (h_pcg.p->iter > maxit || replacements_ > 100000) {
       h_pcg.p->status = 1;
@@ -709,6 +725,10 @@ void Context::time_kernel(int kernel, int reps, double *avg_ms, double *flops, d
     launch = [&]() { launch_density_filter(L_, ftab_, d_dens, d_e[2], d_e[3], 0, stream_); };
     F = 4.0 * ftab_.count * E;
     B = 3.0 * 8 * E;                            // x in; x_tilde, aa out
+  } else if (kernel == 7) {
+    launch = [&]() { launch_copy64(n3_, d_x, d_y, stream_); };
+    F = 0.0;
+    B = 2.0 * 8 * n3_;                          // x in, y out
   } else {
     throw Error(kInvalidArgument, "unknown kernel id");
   }

@@ -300,7 +300,7 @@ __device__ __forceinline__ void layer_contract(const double *__restrict__ c_KE0,
 
 template <bool PCG, int SYM>
 __global__ __launch_bounds__(NT) void k_stencil64z(
-    Lattice L, int zchunk, const double *__restrict__ c_KE0, KeVals kv, const double *__restrict__ young,
+    Lattice L, int nchunks, const double *__restrict__ c_KE0, KeVals kv, const double *__restrict__ young,
     const uint8_t *__restrict__ fmask, const double *__restrict__ src,
     const double *__restrict__ dinv, const double *__restrict__ p_old, double *__restrict__ p_new,
     double *__restrict__ y, PcgScalars *S, double *partials) {
@@ -312,8 +312,18 @@ __global__ __launch_bounds__(NT) void k_stencil64z(
     if (S->done) return;
     beta = S->beta;
   }
-  const int x0 = blockIdx.x * ZX, y0 = blockIdx.y * ZY, z0 = blockIdx.z * zchunk;
-  const int z1 = min(z0 + zchunk, L.NZ);
+  // XCD-aware tile order: workgroups go to the 8 XCDs round-robin by id, so
+  // each XCD is given a contiguous range of tiles (columns fastest): one XCD
+  // marches neighbouring columns of one z-slab and shares their halos in its L2.
+  const int nbx = (L.NX + ZX - 1) / ZX, nby = (L.NY + ZY - 1) / ZY, ncols = nbx * nby;
+  const unsigned nt = gridDim.x, b = blockIdx.x;
+  const unsigned q = nt / 8, rem = nt % 8, xcd = b % 8, slot = b / 8;
+  const unsigned T = (xcd < rem ? xcd * (q + 1) : rem * (q + 1) + (xcd - rem) * q) + slot;
+  const int col = (int)(T % ncols), zc = (int)(T / ncols);
+  const int x0 = (col % nbx) * ZX, y0 = (col / nbx) * ZY;
+  // balanced z-chunks: chunk zc covers node planes [z0, z1)
+  const int z0 = (int)((long long)zc * L.NZ / nchunks);
+  const int z1 = (int)((long long)(zc + 1) * L.NZ / nchunks);
   const int tx = threadIdx.x % ZX, ty = threadIdx.x / ZX;
   const int gx = x0 + tx, gy = y0 + ty;
   const bool col_in = gx < L.NX && gy < L.NY;
@@ -563,6 +573,13 @@ __global__ void k_lattice_to_numeq(Lattice L, const int *__restrict__ dof, int n
   }
 }
 
+// y = x: the reference stream for HBM bandwidth and traffic counters
+__global__ __launch_bounds__(NT) void k_copy64(long long n, const double *__restrict__ x,
+                                               double *__restrict__ y) {
+  const long long stride = (long long)gridDim.x * NT;
+  for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += stride) y[i] = x[i];
+}
+
 __global__ void k_axpby(long long n, double a, const double *__restrict__ x, double b,
                         double *__restrict__ y) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -572,6 +589,11 @@ __global__ void k_axpby(long long n, double a, const double *__restrict__ x, dou
 inline unsigned blocks_for(long long n, int nt) { return (unsigned)((n + nt - 1) / nt); }
 
 }  // namespace
+
+void launch_copy64(long long n, const double *x, double *y, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy64, dim3(VEC_BLOCKS), dim3(NT), 0, s, n, x, y);
+  HIPCHK(hipGetLastError());
+}
 
 void launch_young(const double *x, double *young, long long nelx, double y1, double y2, double p,
                   hipStream_t s) {
@@ -591,20 +613,34 @@ static dim3 stencil_grid(const Lattice &L) {
   return dim3((L.NX + SX - 1) / SX, (L.NY + SY - 1) / SY, (L.NZ + SZ - 1) / SZ);
 }
 
-// z-chunk of the z-marching kernel: enough workgroups to fill the chip
-int stencil_zchunk(const Lattice &L) {
-  const long long cols = (long long)((L.NX + ZX - 1) / ZX) * ((L.NY + ZY - 1) / ZY);
-  int zc = 32;
-  while (zc > 4 && cols * ((L.NZ + zc - 1) / zc) < 1536) zc /= 2;
-  return zc;
+// Number of balanced z-chunks of the z-marching kernel: as many as make the
+// columns x chunks one round of resident workgroups (2 per CU at the kernel's
+// register use), at least 2 node planes per chunk.
+static int resident_workgroups() {
+  static int cus = 0;
+  if (cus == 0) {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    cus = (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+           prop.multiProcessorCount > 0)
+              ? prop.multiProcessorCount
+              : 256;
+  }
+  return 2 * cus;
 }
 
-static dim3 zgrid(const Lattice &L, int zc) {
-  return dim3((L.NX + ZX - 1) / ZX, (L.NY + ZY - 1) / ZY, (L.NZ + zc - 1) / zc);
+int stencil_nchunks(const Lattice &L) {
+  const int ncols = ((L.NX + ZX - 1) / ZX) * ((L.NY + ZY - 1) / ZY);
+  return std::max(1, std::min(L.NZ / 2, resident_workgroups() / ncols));
+}
+
+static dim3 zgrid(const Lattice &L, int nchunks) {
+  const int ncols = ((L.NX + ZX - 1) / ZX) * ((L.NY + ZY - 1) / ZY);
+  return dim3((unsigned)(ncols * nchunks));
 }
 
 long long stencil_blocks(const Lattice &L) {
-  const dim3 a = stencil_grid(L), b = zgrid(L, stencil_zchunk(L));
+  const dim3 a = stencil_grid(L), b = zgrid(L, stencil_nchunks(L));
   return std::max((long long)a.x * a.y * a.z, (long long)b.x * b.y * b.z);
 }
 
@@ -639,9 +675,9 @@ static void launch_stencil64_t(const Lattice &L, const KeCoef &ke, const double 
                                const double *p_old, double *p_new, double *y, PcgScalars *S,
                                double *partials, hipStream_t s, unsigned *nblocks) {
   if (ke.kernel == 0) {
-    const int zc = stencil_zchunk(L);
-    const dim3 g = zgrid(L, zc);
-    hipLaunchKernelGGL((k_stencil64z<PCG, SYM>), g, dim3(NT), 0, s, L, zc, ke.ke, ke.vals, young,
+    const int nch = stencil_nchunks(L);
+    const dim3 g = zgrid(L, nch);
+    hipLaunchKernelGGL((k_stencil64z<PCG, SYM>), g, dim3(NT), 0, s, L, nch, ke.ke, ke.vals, young,
                        freemask, src, dinv, p_old, p_new, y, S, partials);
     *nblocks = g.x * g.y * g.z;
   } else {

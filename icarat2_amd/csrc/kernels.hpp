@@ -79,6 +79,8 @@ void launch_numeq_to_lattice(const Lattice &L, const int *dof, int numeq, const 
 void launch_lattice_to_numeq(const Lattice &L, const int *dof, int numeq, const double *lat,
                              double *v, hipStream_t s);
 void launch_axpby(long long n, double a, const double *x, double b, double *y, hipStream_t s);
+// y = x (plain copy; the calibration stream for bandwidth and traffic counters)
+void launch_copy64(long long n, const double *x, double *y, hipStream_t s);
 
 // ---- k_filter.hip ---------------------------------------------------------
 // Filter neighbourhood in device memory: lin[t] is the LDS offset of entry t

@@ -263,6 +263,18 @@ void LearnedPrecond::solve(const icarat_solve_opts &o, icarat_solve_stats &st) {
   hipStream_t s = c.stream();
   const Lattice &L = c.lattice();
   const int kmode = o.precision == ICARAT_HIP_FP64 ? -1 : (o.precision == ICARAT_HIP_FP32 ? 0 : 1);
+  if (!active_) {
+    // without the network M = D^-1 is fixed: plain Jacobi-PCG (whose beta
+    // keeps the short recurrence exact) instead of the flexible variant
+    const PcgScalars r = c.pcg64(o.tol, o.maxit, false, kmode, kmode < 0 ? 0.0 : 0.1);
+    if (r.status == 2) throw Error(kNumerical, "PCG breakdown");
+    st.iters = r.iter;
+    st.refine_steps = c.replacements_;
+    st.fallback = 0;
+    st.rec_relres = r.bnorm2 > 0.0 ? std::sqrt(r.rr) / std::sqrt(r.bnorm2) : 0.0;
+    st.status = r.status == 1 ? ICARAT_HIP_NOT_CONVERGED : ICARAT_HIP_OK;
+    return;
+  }
   build_graph(kmode);
   launch_density_feature(L, g0_, c.d_young, feat_, s);
   PcgScalars *h = c.h_pcg.p;

@@ -278,6 +278,7 @@ int icarat_hip_unet_train_step(icarat_hip_unet *net, const float *x, const float
  *   kernel 4: as 0 with the other FP64 kernel shape (see icarat_hip_set_stencil)
  *   kernel 5: PCG stencil half with the FP32 MFMA contraction (FP64 vectors)
  *   kernel 6: PCG stencil half with the BF16 MFMA contraction (FP64 vectors)
+ *   kernel 7: plain copy of one lattice vector (3 Np doubles; bandwidth reference)
  * Needs set_density (and filter_setup for kernel 2).  Overwrites the solver's
  * work vectors. */
 int icarat_hip_time_kernel(icarat_hip_ctx *ctx, int32_t kernel, int32_t reps, double *avg_ms,
