@@ -1,0 +1,251 @@
+// icarat_hip.hpp -- header-only C++ facade over the C ABI (icarat_hip.h) with
+// the class names, argument meaning and error behaviour of ICARAT's operator
+// API for the SIMP compliance path, so that a driver can switch by changing
+// types:
+//   LinearAnalysis::solve(solvName, log)     analysis/linear.hpp:55-104
+//   Filter::densityFilter / threshold / thresholdDerivative
+//                                            optimization/filter.hpp:21-45,106-155,216-248
+//   OC::UpdateVariables                      optimization/opt-solver/OCSolver.hpp:16-79
+//   MMASolver (ctor, Set*, Update, Reset)    optimization/opt-solver/MMASolver.hpp:62-116
+// Like the reference, a fatal condition prints to std::cerr and calls
+// exit(1); no exception crosses this header.  Mesh, material, boundary
+// conditions and loads are held by Device (MeshIcarat + Elastic::setParameter +
+// BCIcarat on a box, io/mesh_icarat.hpp:72-139, problem/elasticity.cpp:19-33,
+// io/boundary_condition_icarat.hpp:72-410).
+#ifndef ICARAT_HIP_HPP
+#define ICARAT_HIP_HPP
+
+#include <cstdlib>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "icarat_hip.h"
+
+namespace icarat_hip {
+
+inline void check(int rc, const icarat_hip_ctx *ctx, const char *what) {
+  if (rc == ICARAT_HIP_OK) return;
+  std::cerr << "Error : " << what << ": " << (ctx ? icarat_hip_last_error(ctx) : "")
+            << " (status " << rc << ")" << std::endl;
+  std::exit(1);
+}
+
+// Voxel box problem on the GPU: mesh, material, supports and loads.
+class Device {
+ public:
+  Device(int nx, int ny, int nz, double lx, double ly, double lz, double young1, double young2,
+         double poisson, double penalty) {
+    icarat_grid_desc d{nx, ny, nz, lx, ly, lz, young1, young2, poisson, penalty};
+    check(icarat_hip_create(&d, &ctx_), nullptr, "icarat_hip_create");
+    int64_t np = 0, ne = 0, neq = 0;
+    int32_t numeq = 0;
+    icarat_hip_counts(ctx_, &np, &ne, &neq, &numeq);
+    numnp_ = np;
+    nelx_ = ne;
+  }
+  ~Device() { icarat_hip_destroy(ctx_); }
+  Device(const Device &) = delete;
+  Device &operator=(const Device &) = delete;
+
+  // [[flag x6], [val x6], [[x0,x1],[y0,y1],[z0,z1]]] of a Dirichlet entry
+  void addDirichlet(const int32_t flag[6], const double val[6], const double range[6]) {
+    check(icarat_hip_add_dirichlet(ctx_, flag, val, range), ctx_, "add_dirichlet");
+  }
+  int makeDOF() {
+    int32_t n = 0;
+    check(icarat_hip_make_dof(ctx_, &n), ctx_, "makeDOF");
+    numeq_ = n;
+    return n;
+  }
+  // [[val x3], [[x0,x1],[y0,y1],[z0,z1]]] of a Neumann entry
+  void addNeumann(const double val[3], const double range[6]) {
+    check(icarat_hip_add_neumann(ctx_, val, range), ctx_, "add_neumann");
+  }
+  std::vector<int32_t> dof() const {
+    std::vector<int32_t> d(3 * numnp_);
+    check(icarat_hip_get_dof(ctx_, d.data()), ctx_, "dof");
+    return d;
+  }
+  std::vector<double> fext() const {
+    std::vector<double> f(numeq_);
+    check(icarat_hip_get_fext(ctx_, f.data()), ctx_, "fext");
+    return f;
+  }
+  // Elastic::setDesignS for every element: physical densities, nelx entries
+  void setDesign(const std::vector<double> &xphys) {
+    check(icarat_hip_set_density(ctx_, xphys.data(), (int64_t)xphys.size()), ctx_, "setDesign");
+  }
+  long long numnp() const { return numnp_; }
+  long long nelx() const { return nelx_; }
+  int numeq() const { return numeq_; }
+  icarat_hip_ctx *handle() const { return ctx_; }
+
+ private:
+  icarat_hip_ctx *ctx_ = nullptr;
+  long long numnp_ = 0, nelx_ = 0;
+  int numeq_ = -1;
+};
+
+// LinearAnalysis with matrix-free solver names next to the reference's:
+//   "HIP_MF_PCG"          FP64 Jacobi-PCG (Scaling_CG semantics, verified FP64 stop)
+//   "HIP_MF_PCG_FP32"     FP32 MFMA contraction inside FP64 reliable updates
+//   "HIP_MF_PCG_BF16"     BF16 MFMA contraction inside FP64 reliable updates
+//   "HIP_MF_FPCG_LEARNED" learned preconditioner (icarat_hip_learned_setup first)
+class LinearAnalysis {
+ public:
+  explicit LinearAnalysis(Device &dev, double tol = 1e-8, int maxIter = 10000)
+      : dev_(dev), tol_(tol), maxit_(maxIter) {}
+  void solve(const std::string &solvName, bool log) {
+    icarat_solve_opts o{tol_, maxit_, ICARAT_HIP_FP64, ICARAT_HIP_PRECOND_JACOBI, 0.1, 200, 0};
+    if (solvName == "HIP_MF_PCG_FP32") o.precision = ICARAT_HIP_FP32;
+    else if (solvName == "HIP_MF_PCG_BF16") o.precision = ICARAT_HIP_BF16;
+    else if (solvName == "HIP_MF_FPCG_LEARNED") o.precond = ICARAT_HIP_PRECOND_LEARNED;
+    else if (solvName != "HIP_MF_PCG") {
+      std::cerr << "Error : unknown solver name " << solvName << std::endl;
+      std::exit(1);
+    }
+    val_.assign(dev_.numeq() > 0 ? dev_.numeq() : 0, 0.0);
+    int rc = icarat_hip_solve(dev_.handle(), &o, val_.data(), &stats_);
+    // like Scaling_CG, reaching the iteration cap is reported, not fatal
+    if (rc != ICARAT_HIP_NOT_CONVERGED) check(rc, dev_.handle(), "solve");
+    if (log)
+      std::cout << "iteration: " << stats_.iters << "  residual norm (FP64 true): "
+                << stats_.true_relres << std::endl;
+  }
+  const std::vector<double> &val() const { return val_; }        // Value::val (numeq)
+  const icarat_solve_stats &stats() const { return stats_; }
+  // compliance f.u and dfds / dgds of the driver's sensitivity()
+  double compliance() {
+    double c = 0.0;
+    check(icarat_hip_compliance_sensitivity(dev_.handle(), 1.0, &c, nullptr, nullptr),
+          dev_.handle(), "compliance");
+    return c;
+  }
+  void sensitivity(double vnorm, std::vector<double> &dfds, std::vector<double> &dgds) {
+    dfds.resize(dev_.nelx());
+    dgds.resize(dev_.nelx());
+    check(icarat_hip_compliance_sensitivity(dev_.handle(), vnorm, nullptr, dfds.data(),
+                                            dgds.data()),
+          dev_.handle(), "sensitivity");
+  }
+
+ private:
+  Device &dev_;
+  double tol_;
+  int maxit_;
+  std::vector<double> val_;
+  icarat_solve_stats stats_{};
+};
+
+class Filter {
+ public:
+  Filter(Device &dev, double radius) : dev_(dev) {
+    check(icarat_hip_filter_setup(dev_.handle(), radius), dev_.handle(), "Filter");
+  }
+  std::vector<double> densityFilter(const std::vector<double> &design_s, bool isSens) {
+    std::vector<double> out(design_s.size());
+    check(icarat_hip_filter(dev_.handle(), design_s.data(), out.data(), isSens ? 1 : 0),
+          dev_.handle(), "densityFilter");
+    return out;
+  }
+  std::vector<double> threshold(const std::vector<double> &design_s, double beta, double T) {
+    std::vector<double> out(design_s.size());
+    check(icarat_hip_threshold(dev_.handle(), design_s.data(), beta, T, out.data(), 0),
+          dev_.handle(), "threshold");
+    return out;
+  }
+  std::vector<double> thresholdDerivative(const std::vector<double> &design_s, double beta,
+                                          double T) {
+    std::vector<double> out(design_s.size());
+    check(icarat_hip_threshold(dev_.handle(), design_s.data(), beta, T, out.data(), 1),
+          dev_.handle(), "thresholdDerivative");
+    return out;
+  }
+
+ private:
+  Device &dev_;
+};
+
+// Bounds must be uniform (every reference driver passes Optimize(nelx, 1, xmin, xmax)).
+inline bool uniform(const double *v, long long n) {
+  for (long long i = 1; i < n; ++i)
+    if (v[i] != v[0]) return false;
+  return true;
+}
+
+// OC::UpdateVariables with the drivers' volume callback computed on the device:
+// x' -> filter -> (projection) -> sum * element volume (opt_macro_oc.cpp:153-164).
+class OC {
+ public:
+  OC(Device &dev, int n) : dev_(dev), n_(n) {}
+  void UpdateVariables(std::vector<double> &xk, std::vector<double> &dfdx, double g,
+                       std::vector<double> &dgdx, std::vector<double> &xmin,
+                       std::vector<double> &xmax, bool useProjection = false,
+                       double beta = 1.0) {
+    if ((long long)xk.size() != n_ || !uniform(xmin.data(), n_) || !uniform(xmax.data(), n_)) {
+      std::cerr << "Error : OC::UpdateVariables needs n entries and uniform bounds" << std::endl;
+      std::exit(1);
+    }
+    int32_t steps = 0;
+    check(icarat_hip_oc_update(dev_.handle(), xk.data(), dfdx.data(), g, dgdx.data(), xmin[0],
+                               xmax[0], useProjection ? 1 : 0, beta, &steps),
+          dev_.handle(), "OC::UpdateVariables");
+  }
+
+ private:
+  Device &dev_;
+  long long n_;
+};
+
+// MMASolver(n, m = 1) on the device
+class MMASolver {
+ public:
+  MMASolver(Device &dev, int n, int m, double a = 0.0, double c = 100000.0, double d = 0.0)
+      : dev_(dev), n_(n) {
+    if (m != 1 || a != 0.0 || c != 100000.0 || d != 0.0 || n != dev.nelx()) {
+      std::cerr << "Error : MMASolver on the device supports n = nelx, m = 1, a = 0, c = 1e5, d = 0"
+                << std::endl;
+      std::exit(1);
+    }
+    check(icarat_hip_mma_reset(dev_.handle()), dev_.handle(), "MMASolver");
+  }
+  void SetAsymptotes(double init, double decrease, double increase) {
+    asyminit_ = init;
+    asymdec_ = decrease;
+    asyminc_ = increase;
+    push();
+  }
+  void SetMoveLimit(double m) {
+    move_ = m;
+    push();
+  }
+  void SetOscillation(double a) {
+    albefa_ = a;
+    push();
+  }
+  void ConstraintModification(bool) {}
+  void Update(double *xval, const double *dfdx, const double *gx, const double *dgdx,
+              const double *xmin, const double *xmax) {
+    if (!uniform(xmin, n_) || !uniform(xmax, n_)) {
+      std::cerr << "Error : MMASolver::Update needs uniform bounds" << std::endl;
+      std::exit(1);
+    }
+    check(icarat_hip_mma_update(dev_.handle(), xval, dfdx, gx[0], dgdx, xmin[0], xmax[0], nullptr),
+          dev_.handle(), "MMASolver::Update");
+  }
+  void Reset() { check(icarat_hip_mma_reset(dev_.handle()), dev_.handle(), "MMASolver::Reset"); }
+
+ private:
+  void push() {
+    check(icarat_hip_mma_set_params(dev_.handle(), move_, albefa_, asyminit_, asymdec_, asyminc_),
+          dev_.handle(), "MMASolver parameters");
+  }
+  Device &dev_;
+  long long n_;
+  double move_ = 0.15, albefa_ = 0.2, asyminit_ = 0.2, asymdec_ = 0.7, asyminc_ = 1.2;
+};
+
+}  // namespace icarat_hip
+
+#endif  // ICARAT_HIP_HPP

@@ -120,3 +120,57 @@ def test_device_calls_without_gpu_fail_loudly(capi):
     with pytest.raises(capi.IcaratError) as e:
         c.set_density(np.full(c.nelx, 0.5))
     assert e.value.code == capi.HIP_ERROR
+
+
+FACADE_DRIVER = r'''
+#include <cstdio>
+#include "icarat_hip.hpp"
+int main(int argc, char **argv) {
+  icarat_hip::Device d(6, 3, 2, 6.0, 3.0, 2.0, 1e-9, 1.0, 0.3, 3.0);
+  const int32_t flag[6] = {1, 1, 1, 0, 0, 0};
+  const double val[6] = {0, 0, 0, 0, 0, 0};
+  const double clamp[6] = {0, 0, 0, 3, 0, 2};
+  d.addDirichlet(flag, val, clamp);
+  const int numeq = d.makeDOF();
+  const double load[3] = {0.0, 0.0, -1.0};
+  const double tip[6] = {6, 6, 0, 0, 0, 0};
+  d.addNeumann(load, tip);
+  double s = 0.0;
+  for (double v : d.fext()) s += v;
+  std::printf("%d %lld %lld %.17g\n", numeq, d.numnp(), d.nelx(), s);
+  std::fflush(stdout);
+  if (argc > 1) {
+    // the first call that needs the device
+    std::vector<double> x(d.nelx(), 0.5);
+    d.setDesign(x);
+    icarat_hip::Filter f(d, 1.5);
+    std::vector<double> xf = f.densityFilter(x, false);
+    std::printf("filtered %.17g\n", xf[0]);
+  }
+  return 0;
+}
+'''
+
+
+def test_cpp_facade_compiles_and_keeps_reference_error_behaviour(capi, tmp_path):
+    """The header-only facade (reference class names over the C ABI) builds with
+    the host compiler, runs the host-side set-up, and turns a failing call into
+    the reference's cerr + exit(1)."""
+    import subprocess
+    src = tmp_path / "driver.cpp"
+    src.write_text(FACADE_DRIVER)
+    exe = tmp_path / "driver"
+    libdir = os.path.join(ROOT, "icarat2_amd")
+    subprocess.run(["g++", "-std=c++17", "-O1", "-I", os.path.join(ROOT, "include"), str(src),
+                    "-L", libdir, "-licarat_hip", f"-Wl,-rpath,{libdir}", "-o", str(exe)],
+                   check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    numeq, numnp, nelx, s = r.stdout.split()
+    assert (int(numeq), int(numnp), int(nelx)) == (3 * (7 * 4 * 3 - 4 * 3), 7 * 4 * 3, 36)
+    assert float(s) == -1.0  # one point load of -1 at the single tip node (6, 0, 0)
+    r = subprocess.run([str(exe), "device"], capture_output=True, text=True)
+    if capi.device_available():
+        assert r.returncode == 0 and "filtered" in r.stdout
+    else:
+        assert r.returncode == 1 and r.stderr.startswith("Error : setDesign")
