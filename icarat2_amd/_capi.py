@@ -77,6 +77,9 @@ _SIGS = {
     "icarat_hip_add_dirichlet": (C.c_int, [C.c_void_p, i32p, dp, dp]),
     "icarat_hip_make_dof": (C.c_int, [C.c_void_p, i32p]),
     "icarat_hip_add_neumann": (C.c_int, [C.c_void_p, dp, dp]),
+    "icarat_hip_new_load_case": (C.c_int, [C.c_void_p, i32p]),
+    "icarat_hip_select_load_case": (C.c_int, [C.c_void_p, C.c_int32]),
+    "icarat_hip_load_case_count": (C.c_int, [C.c_void_p, i32p]),
     "icarat_hip_get_dof": (C.c_int, [C.c_void_p, i32p]),
     "icarat_hip_get_fext": (C.c_int, [C.c_void_p, dp]),
     "icarat_hip_set_fext": (C.c_int, [C.c_void_p, dp]),
@@ -118,6 +121,15 @@ _SIGS = {
     "icarat_hip_unet_grads": (C.c_int, [C.c_void_p, fp, fp, dp, fp]),
     "icarat_hip_unet_train_step": (C.c_int, [C.c_void_p, fp, fp, C.c_float, dp]),
     "icarat_hip_set_stencil": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, i32p]),
+    "icarat_hip_batch_create": (C.c_int, [C.POINTER(C.c_void_p)]),
+    "icarat_hip_batch_destroy": (C.c_int, [C.c_void_p]),
+    "icarat_hip_batch_last_error": (C.c_char_p, [C.c_void_p]),
+    "icarat_hip_batch_add": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "icarat_hip_batch_info": (C.c_int, [C.c_void_p, i32p, i64p]),
+    "icarat_hip_batch_solve": (C.c_int, [C.c_void_p, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "icarat_hip_batch_simp_run": (C.c_int, [C.c_void_p, C.POINTER(SimpOpts), C.POINTER(dp),
+                                            C.POINTER(dp), C.POINTER(SimpStats)]),
+    "icarat_hip_batch_autotune": (C.c_int, [C.c_void_p, C.c_int32, i32p, dp]),
     "icarat_hip_time_kernel": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, dp, dp, dp]),
     "icarat_hip_gaussian_field": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_double,
                                             C.c_uint64, C.c_int32, C.c_double, C.c_double,
@@ -308,6 +320,19 @@ class Context:
         r = np.array(rng, dtype=np.float64).reshape(6)
         self._chk(self.L.icarat_hip_add_neumann(self.h, _p(v), _p(r)), "add_neumann")
 
+    def new_load_case(self):
+        i = C.c_int32()
+        self._chk(self.L.icarat_hip_new_load_case(self.h, C.byref(i)), "new_load_case")
+        return i.value
+
+    def select_load_case(self, i):
+        self._chk(self.L.icarat_hip_select_load_case(self.h, i), "select_load_case")
+
+    def load_case_count(self):
+        n = C.c_int32()
+        self._chk(self.L.icarat_hip_load_case_count(self.h, C.byref(n)), "load_case_count")
+        return n.value
+
     def dof(self):
         d = np.zeros(self.numnp * 3, dtype=np.int32)
         self._chk(self.L.icarat_hip_get_dof(self.h, d.ctypes.data_as(i32p)), "get_dof")
@@ -481,3 +506,67 @@ def cantilever(nx, ny, nz, **kw):
         v = -1.0 / (2 * ny) if j in (0, ny) else -1.0 / ny
         c.add_neumann([0.0, 0.0, v], [[nx, nx], [j, j], [0, 0]])
     return c
+
+
+class Batch:
+    """Independent problems (Context objects, kept alive by the batch) solved
+    together: one captured graph per PCG round with a branch per problem."""
+
+    def __init__(self, contexts=()):
+        self.L = lib()
+        h = C.c_void_p()
+        rc = self.L.icarat_hip_batch_create(C.byref(h))
+        if rc:
+            raise IcaratError(rc, "batch_create")
+        self.h = h
+        self.ctx = []
+        for c in contexts:
+            self.add(c)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.icarat_hip_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc, what):
+        if rc:
+            msg = self.L.icarat_hip_batch_last_error(self.h)
+            raise IcaratError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def add(self, c):
+        self._chk(self.L.icarat_hip_batch_add(self.h, c.h), "batch_add")
+        self.ctx.append(c)
+
+    def info(self):
+        n, k = C.c_int32(), C.c_int64()
+        self._chk(self.L.icarat_hip_batch_info(self.h, C.byref(n), C.byref(k)), "batch_info")
+        return n.value, k.value
+
+    def solve(self, tol=1e-8, maxit=10000, allow_cap=False):
+        o = SolveOpts(tol, maxit, FP64, PRECOND_JACOBI, 0.1, 200, 0)
+        st = (SolveStats * len(self.ctx))()
+        rc = self.L.icarat_hip_batch_solve(self.h, C.byref(o), st)
+        if rc and not (allow_cap and rc == NOT_CONVERGED):
+            self._chk(rc, "batch_solve")
+        return list(st)
+
+    def simp_run(self, x0s, n_iter, optimizer=0, use_projection=False, xmin=1e-4, xmax=1.0,
+                 tol=1e-8, maxit=10000):
+        o = SimpOpts(n_iter, optimizer, 1 if use_projection else 0, 0, xmin, xmax,
+                     SolveOpts(tol, maxit, FP64, PRECOND_JACOBI, 0.1, 200, 0))
+        xs = [np.array(x, dtype=np.float64) for x in x0s]
+        hs = [np.zeros(4 * max(n_iter, 1)) for _ in xs]
+        xp = (dp * len(xs))(*[_p(x) for x in xs])
+        hp = (dp * len(xs))(*[_p(h) for h in hs])
+        st = (SimpStats * len(xs))()
+        self._chk(self.L.icarat_hip_batch_simp_run(self.h, C.byref(o), xp, hp, st), "batch_simp_run")
+        return xs, [h[:4 * n_iter].reshape(n_iter, 4) for h in hs], list(st)
+
+    def autotune(self, reps=5):
+        cfg = C.c_int32()
+        ms = (C.c_double * 4)()
+        self._chk(self.L.icarat_hip_batch_autotune(self.h, reps, C.byref(cfg), ms), "batch_autotune")
+        return cfg.value, list(ms)

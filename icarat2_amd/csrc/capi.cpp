@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "../../include/icarat_hip.h"
+#include "batch.hpp"
 #include "context.hpp"
 #include "learned.hpp"
 #include "mma.hpp"
@@ -13,6 +14,11 @@
 
 struct icarat_hip_ctx {
   icarat_hip::Context *c = nullptr;
+  std::string err;
+};
+
+struct icarat_hip_batch {
+  icarat_hip::Batch *b = nullptr;
   std::string err;
 };
 
@@ -124,6 +130,23 @@ int icarat_hip_add_neumann(icarat_hip_ctx *h, const double val[3], const double 
     c.problem().add_neumann(val, range);
     c.mark_fext_dirty();
   });
+}
+
+int icarat_hip_new_load_case(icarat_hip_ctx *h, int32_t *index) {
+  return guard(h, [&](icarat_hip::Context &c) {
+    const int i = c.new_load_case();
+    if (index) *index = i;
+  });
+}
+
+int icarat_hip_select_load_case(icarat_hip_ctx *h, int32_t index) {
+  return guard(h, [&](icarat_hip::Context &c) { c.select_load_case(index); });
+}
+
+int icarat_hip_load_case_count(const icarat_hip_ctx *hc, int32_t *n) {
+  icarat_hip_ctx *h = const_cast<icarat_hip_ctx *>(hc);
+  if (!n) return fail(h, ICARAT_HIP_INVALID_ARGUMENT, "null argument");
+  return guard(h, [&](icarat_hip::Context &c) { *n = c.problem().num_load_cases(); });
 }
 
 int icarat_hip_get_dof(const icarat_hip_ctx *h, int32_t *dof) {
@@ -506,6 +529,106 @@ int icarat_hip_unet_train_step(icarat_hip_unet *u, const float *x, const float *
     n.net->train_step(n.x, n.t, lr);
     const double l = unet_loss(n);
     if (loss) *loss = l;
+  });
+}
+
+// ---- batched problems ---------------------------------------------------------
+}  // extern "C"
+
+namespace {
+template <class F>
+int batch_guard(icarat_hip_batch *b, F &&f) {
+  if (!b || !b->b) return ICARAT_HIP_INVALID_ARGUMENT;
+  try {
+    f(*b->b);
+    b->err.clear();
+    return ICARAT_HIP_OK;
+  } catch (const icarat_hip::Error &e) {
+    b->err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    b->err = "host allocation failed";
+    return ICARAT_HIP_OUT_OF_MEMORY;
+  } catch (const std::exception &e) {
+    b->err = e.what();
+    return ICARAT_HIP_INTERNAL;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int icarat_hip_batch_create(icarat_hip_batch **out) {
+  if (!out) return ICARAT_HIP_INVALID_ARGUMENT;
+  *out = nullptr;
+  icarat_hip_batch *h = new (std::nothrow) icarat_hip_batch();
+  if (!h) return ICARAT_HIP_OUT_OF_MEMORY;
+  try {
+    h->b = new icarat_hip::Batch();
+  } catch (const icarat_hip::Error &e) {
+    delete h;
+    return e.code;
+  } catch (...) {
+    delete h;
+    return ICARAT_HIP_INTERNAL;
+  }
+  *out = h;
+  return ICARAT_HIP_OK;
+}
+
+int icarat_hip_batch_destroy(icarat_hip_batch *h) {
+  if (!h) return ICARAT_HIP_INVALID_ARGUMENT;
+  delete h->b;
+  delete h;
+  return ICARAT_HIP_OK;
+}
+
+const char *icarat_hip_batch_last_error(const icarat_hip_batch *h) {
+  return h ? h->err.c_str() : "null batch";
+}
+
+int icarat_hip_batch_add(icarat_hip_batch *h, icarat_hip_ctx *ctx) {
+  if (!ctx || !ctx->c) return ICARAT_HIP_INVALID_ARGUMENT;
+  return batch_guard(h, [&](icarat_hip::Batch &b) { b.add(ctx->c); });
+}
+
+int icarat_hip_batch_info(const icarat_hip_batch *hc, int32_t *n, int64_t *launches) {
+  icarat_hip_batch *h = const_cast<icarat_hip_batch *>(hc);
+  return batch_guard(h, [&](icarat_hip::Batch &b) {
+    if (n) *n = b.size();
+    if (launches) *launches = b.graph_launches();
+  });
+}
+
+int icarat_hip_batch_solve(icarat_hip_batch *h, const icarat_solve_opts *o,
+                           icarat_solve_stats *stats) {
+  if (!o) return ICARAT_HIP_INVALID_ARGUMENT;
+  int cap = 0;
+  int rc = batch_guard(h, [&](icarat_hip::Batch &b) {
+    std::vector<icarat_solve_stats> s(b.size());
+    b.solve(*o, s.data());
+    for (int i = 0; i < b.size(); ++i) {
+      if (s[i].status != ICARAT_HIP_OK) cap = 1;
+      if (stats) stats[i] = s[i];
+    }
+  });
+  if (rc == ICARAT_HIP_OK && cap) {
+    h->err = "iteration cap reached before the tolerance in at least one problem";
+    return ICARAT_HIP_NOT_CONVERGED;
+  }
+  return rc;
+}
+
+int icarat_hip_batch_simp_run(icarat_hip_batch *h, const icarat_simp_opts *o, double *const *x,
+                              double *const *hist, icarat_simp_stats *stats) {
+  if (!o || !x) return ICARAT_HIP_INVALID_ARGUMENT;
+  return batch_guard(h, [&](icarat_hip::Batch &b) { b.simp_run(*o, x, hist, stats); });
+}
+
+int icarat_hip_batch_autotune(icarat_hip_batch *h, int32_t reps, int32_t *config, double *ms) {
+  return batch_guard(h, [&](icarat_hip::Batch &b) {
+    const int c = b.autotune(reps, ms);
+    if (config) *config = c;
   });
 }
 

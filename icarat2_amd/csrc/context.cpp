@@ -149,12 +149,47 @@ void Context::upload_dof() {
 void Context::sync_fext() {
   require_dof();
   const int numeq = prob_.numeq();
-  h2d(d_numeq[0].p, prob_.fext().data(), numeq, stream_);
-  launch_numeq_to_lattice(L_, d_dof, numeq, d_numeq[0], nullptr, d_f, stream_);
-  // constrained entries of the lattice load are 0 by construction
+  const int nc = prob_.num_load_cases();
+  if ((int)d_fcases_.size() != nc) {
+    d_fcases_.clear();
+    d_fcases_.resize(nc);
+    for (auto &b : d_fcases_) {
+      b.alloc(n3_);
+      // the conversion below writes the real nodes only: the padding stays 0
+      HIPCHK(hipMemsetAsync(b.p, 0, b.bytes(), stream_));
+    }
+  }
+  // constrained entries of the lattice loads are 0 by construction
+  for (int i = 0; i < nc; ++i) {
+    h2d(d_numeq[0].p, prob_.case_fext(i).data(), numeq, stream_);
+    launch_numeq_to_lattice(L_, d_dof, numeq, d_numeq[0], nullptr, d_fcases_[i], stream_);
+  }
+  d2d(d_f.p, d_fcases_[prob_.load_case()].p, n3_, stream_);
   sync();
   solved_ = false;
   fext_dirty_ = false;
+}
+
+void Context::use_load_case(int i) {
+  ensure_fext();
+  if (i < 0 || i >= (int)d_fcases_.size()) throw Error(kInvalidArgument, "no such load case");
+  d2d(d_f.p, d_fcases_[i].p, n3_, stream_);
+  solved_ = false;
+}
+
+int Context::new_load_case() {
+  const int i = prob_.new_load_case();
+  mark_fext_dirty();
+  return i;
+}
+
+void Context::select_load_case(int i) {
+  prob_.select_load_case(i);
+  if (fext_dirty_ || !dev_ready_) {
+    solved_ = false;
+    return;
+  }
+  use_load_case(i);
 }
 
 void Context::set_density(const double *xphys) {
@@ -348,12 +383,7 @@ void Context::solve(const icarat_solve_opts &o, double *u_host, icarat_solve_sta
   ensure_fext();
   sync();
   const double t0 = now_s();
-  // right-hand side: fext minus the load of the prescribed displacements
-  d2d(d_b.p, d_f.p, n3_, stream_);
-  if (prob_.has_nonzero_dirichlet()) {
-    launch_apply_k64(L_, kc_, d_young, d_free, d_dirlat, d_y, stream_);
-    launch_axpby(n3_, -1.0, d_y, 1.0, d_b, stream_);
-  }
+  prepare_rhs(stream_);
   icarat_solve_stats S{};
   S.status = ICARAT_HIP_OK;
   if (o.precision != ICARAT_HIP_FP64 && o.precision != ICARAT_HIP_FP32 &&
@@ -378,6 +408,11 @@ void Context::solve(const icarat_solve_opts &o, double *u_host, icarat_solve_sta
   assemble_nodal_solution();
   S.true_relres = true_residual_device();
   S.seconds = now_s() - t0;
+  copy_out_solution(u_host);
+  if (st) *st = S;
+}
+
+void Context::copy_out_solution(double *u_host) {
   if (u_host) {
     const int numeq = prob_.numeq();
     launch_lattice_to_numeq(L_, d_dof, numeq, d_u, d_numeq[0], stream_);
@@ -385,7 +420,53 @@ void Context::solve(const icarat_solve_opts &o, double *u_host, icarat_solve_sta
     sync();
   }
   solved_ = true;
-  if (st) *st = S;
+}
+
+void Context::prepare_rhs(hipStream_t s) {
+  // right-hand side: fext minus the load of the prescribed displacements
+  d2d(d_b.p, d_f.p, n3_, s);
+  if (prob_.has_nonzero_dirichlet()) {
+    launch_apply_k64(L_, kc_, d_young, d_free, d_dirlat, d_y, s);
+    launch_axpby(n3_, -1.0, d_y, 1.0, d_b, s);
+  }
+}
+
+// ---- pieces of the FP64 Jacobi-PCG driver for batched solves -----------------
+void Context::pcg_start(double tol, int maxit, hipStream_t s) {
+  require_density();
+  ensure_fext();
+  if (!(tol > 0.0) || maxit < 0) throw Error(kInvalidArgument, "tol must be > 0 and maxit >= 0");
+  prepare_rhs(s);
+  PcgScalars init{};
+  init.tol = tol;
+  init.maxit = maxit;
+  *h_pcg.p = init;
+  h2d(d_pcg.p, h_pcg.p, 1, s);
+  launch_pcg_init64(n3_, d_b, d_x, d_r, d_p[0], d_dinv, d_pcg, d_partials, d_counter, s);
+  d2h(h_pcg.p, d_pcg.p, 1, s);
+  replacements_ = 0;
+  fell_back_ = 0;
+}
+
+void Context::pcg_verify(hipStream_t s) {
+  // the FP64 true residual replaces the recursive one; CG continues if it fails the test
+  launch_apply_k64(L_, kc_, d_young, d_free, d_x, d_y, s);
+  launch_replace64(n3_, d_b, d_y, d_r, d_dinv, d_pcg, d_partials, s);
+  d2h(h_pcg.p, d_pcg.p, 1, s);
+  ++replacements_;
+}
+
+void Context::pcg_finish(icarat_solve_stats &S) {
+  const PcgScalars &r = *h_pcg.p;
+  S = icarat_solve_stats{};
+  S.iters = r.iter;
+  S.refine_steps = replacements_;
+  S.rec_relres = r.bnorm2 > 0 ? std::sqrt(r.rr) / std::sqrt(r.bnorm2) : 0.0;
+  S.status = r.done && r.status == 0 ? ICARAT_HIP_OK : ICARAT_HIP_NOT_CONVERGED;
+  if (r.status == 2) throw Error(kNumerical, "PCG breakdown (non-positive curvature or NaN)");
+  assemble_nodal_solution();
+  S.true_relres = true_residual_device();
+  solved_ = true;
 }
 
 void Context::compliance_sensitivity(double vnorm, double *c, double *dfds, double *dgds) {
@@ -496,116 +577,160 @@ int Context::oc_host(double *x, const double *dfdx, double g, const double *dgdx
 }
 
 // ---- the design loop ----------------------------------------------------
-void Context::simp_run(const icarat_simp_opts &o, double *x, double *hist, icarat_simp_stats *st) {
+// ---- the SIMP loop, in phases (shared by simp_run and the batch driver) ------
+void Context::simp_begin(const icarat_simp_opts &o, const double *x) {
   require_dof();
   ensure_fext();
   if (!filter_ready_) throw Error(kNotReady, "filter_setup has not been called");
   if (o.n_iter < 0) throw Error(kInvalidArgument, "n_iter must be >= 0");
-  if (o.optimizer != 0 && o.optimizer != 1) throw Error(kInvalidArgument, "optimizer must be 0 (OC) or 1 (MMA)");
-  const auto &g = prob_.spec();
-  const long long n = L_.nelx;
-  icarat_simp_stats S{};
+  if (o.optimizer != 0 && o.optimizer != 1)
+    throw Error(kInvalidArgument, "optimizer must be 0 (OC) or 1 (MMA)");
   sync();
-  const double t_all = now_s();
-  h2d(d_e[E_DES].p, x, n, stream_);
+  simp_ = SimpState{};
+  simp_.o = o;
+  simp_.t_all = now_s();
+  h2d(d_e[E_DES].p, x, L_.nelx, stream_);
   if (o.optimizer == 1) {
     if (!mma_) mma_.reset(new DeviceMMA(*this));
     mma_->reset();
   }
-  double V0 = 0.0;
-  for (int step = 1; step <= o.n_iter; ++step) {
-    double t0 = now_s();
-    // x_tilde = filter(x); x_hat = projection(x_tilde) or x_tilde
-    filter_device(d_e[E_DES], d_e[E_XF], 0);
-    const double beta = std::min(std::pow(2, (int)(step / 40)), 16.0);
-    double *xp = d_e[E_XF];
-    if (o.use_projection) {
-      launch_threshold(n, d_e[E_XF], beta, 0.5, d_e[E_XP], 0, stream_);
-      launch_threshold(n, d_e[E_XF], beta, 0.5, d_e[E_THD], 1, stream_);
-      xp = d_e[E_XP];
-    }
-    d2d(d_dens.p, xp, n, stream_);
-    set_density_device();
-    sync();
-    double t1 = now_s();
-    S.filter_s += t1 - t0;
-    icarat_solve_stats ss{};
-    icarat_solve_opts so = o.solve;
-    // the first train_iters solves of the preconditioner's lifetime are solved
-    // by Jacobi-PCG and feed the training set
-    const bool harvest = so.precond == ICARAT_HIP_PRECOND_LEARNED && learned_ &&
-                         learned_->harvested() < learned_->opts().train_iters;
-    if (harvest) {
-      so.precond = ICARAT_HIP_PRECOND_JACOBI;
-      learned_->begin_harvest();
-      batch_hook_ = [this]() { learned_->snapshot(); };
-      try {
-        solve(so, nullptr, &ss);
-      } catch (...) {
-        batch_hook_ = nullptr;
-        learned_->end_harvest(false);
-        throw;
-      }
-      batch_hook_ = nullptr;
-      learned_->end_harvest(ss.status == ICARAT_HIP_OK);
-      const int ti = std::max(1, learned_->opts().train_iters);
-      const int h = learned_->harvested();  // 1..ti
-      const int steps = learned_->opts().train_steps / ti + (h <= learned_->opts().train_steps % ti ? 1 : 0);
-      const double tt = now_s();
-      learned_->train(steps);
-      S.train_s += now_s() - tt;
-    } else {
-      solve(so, nullptr, &ss);
-    }
-    S.pcg_iters += ss.iters;
-    S.fallbacks += ss.fallback;
-    S.max_true_relres = std::max(S.max_true_relres, ss.true_relres);
-    double t2 = now_s();
-    S.solve_s += t2 - t1;
-    // compliance and volume
-    launch_dot(n3_, d_f, d_u, d_scal, d_partials, d_counter, stream_);
-    launch_volume(n, d_dens, vol_e_, d_scal.p + 1, d_partials, d_counter, stream_);
-    d2h(h_scal.p, d_scal.p, 2, stream_);
-    sync();
-    const double c = h_scal.p[0], vol = h_scal.p[1];
-    if (step == 1) V0 = vol;
-    launch_flush_small(n3_, d_u, d_nv, stream_);
-    launch_sensitivity64(L_, d_ke, d_dens, d_nv, g.young1, g.young2, g.penalty, vol_e_,
-                         o.optimizer == 1 ? V0 : 1.0, d_e[E_DFDS], d_e[E_DGDS], stream_);
-    if (o.use_projection) {
-      launch_mul(n, d_e[E_THD], d_e[E_DFDS], stream_);
-      launch_mul(n, d_e[E_THD], d_e[E_DGDS], stream_);
-    }
-    sync();
-    double t3 = now_s();
-    S.sens_s += t3 - t2;
-    filter_device(d_e[E_DFDS], d_e[E_TMP], 1);
-    std::swap(d_e[E_DFDS], d_e[E_TMP]);
-    filter_device(d_e[E_DGDS], d_e[E_TMP], 1);
-    std::swap(d_e[E_DGDS], d_e[E_TMP]);
-    sync();
-    double t4 = now_s();
-    S.filter_s += t4 - t3;
-    if (hist) {
-      hist[4 * (step - 1) + 0] = c;
-      hist[4 * (step - 1) + 1] = vol;
-      hist[4 * (step - 1) + 2] = ss.iters;
-      hist[4 * (step - 1) + 3] = ss.true_relres;
-    }
-    if (o.optimizer == 1) {
-      const double gc = step == 1 ? 0.0 : vol / V0 - 1.0;
-      mma_->update(d_e[E_DES], d_e[E_DFDS], gc, d_e[E_DGDS], o.xmin, o.xmax);
-    } else {
-      oc_device(d_e[E_DES], d_e[E_DFDS], d_e[E_DGDS], V0, o.xmin, o.xmax, o.use_projection,
-                beta);
-    }
-    sync();
-    S.update_s += now_s() - t4;
+}
+
+void Context::simp_pre(int step) {
+  const icarat_simp_opts &o = simp_.o;
+  const long long n = L_.nelx;
+  double t0 = now_s();
+  // x_tilde = filter(x); x_hat = projection(x_tilde) or x_tilde
+  filter_device(d_e[E_DES], d_e[E_XF], 0);
+  simp_.beta = std::min(std::pow(2, (int)(step / 40)), 16.0);
+  double *xp = d_e[E_XF];
+  if (o.use_projection) {
+    launch_threshold(n, d_e[E_XF], simp_.beta, 0.5, d_e[E_XP], 0, stream_);
+    launch_threshold(n, d_e[E_XF], simp_.beta, 0.5, d_e[E_THD], 1, stream_);
+    xp = d_e[E_XP];
   }
-  d2h(x, d_e[E_DES].p, n, stream_);
+  d2d(d_dens.p, xp, n, stream_);
+  set_density_device();
+  // volume of the physical design (V0 of the first iteration normalises MMA's constraint)
+  launch_volume(n, d_dens, vol_e_, d_scal.p + 1, d_partials, d_counter, stream_);
+  d2h(h_scal.p + 1, d_scal.p + 1, 1, stream_);
   sync();
-  S.seconds = now_s() - t_all;
-  if (st) *st = S;
+  simp_.vol = h_scal.p[1];
+  if (step == 1) simp_.V0 = simp_.vol;
+  simp_.c = 0.0;
+  simp_.relres = 0.0;
+  simp_.iters = 0;
+  simp_.S.filter_s += now_s() - t0;
+}
+
+void Context::simp_case_post(int lc, const icarat_solve_stats &ss) {
+  const icarat_simp_opts &o = simp_.o;
+  const auto &g = prob_.spec();
+  const long long n = L_.nelx;
+  const double t0 = now_s();
+  simp_.iters += ss.iters;
+  simp_.relres = std::max(simp_.relres, ss.true_relres);
+  simp_.S.pcg_iters += ss.iters;
+  simp_.S.fallbacks += ss.fallback;
+  simp_.S.max_true_relres = std::max(simp_.S.max_true_relres, ss.true_relres);
+  launch_dot(n3_, d_f, d_u, d_scal, d_partials, d_counter, stream_);
+  launch_flush_small(n3_, d_u, d_nv, stream_);
+  // dgds does not depend on u: the first case's is kept
+  launch_sensitivity64(L_, d_ke, d_dens, d_nv, g.young1, g.young2, g.penalty, vol_e_,
+                       o.optimizer == 1 ? simp_.V0 : 1.0, lc == 0 ? d_e[E_DFDS] : d_e[E_TMP],
+                       lc == 0 ? d_e[E_DGDS] : d_e[E_XP], stream_);
+  if (lc > 0) launch_axpby(n, 1.0, d_e[E_TMP], 1.0, d_e[E_DFDS], stream_);
+  d2h(h_scal.p, d_scal.p, 1, stream_);
+  sync();
+  simp_.c += h_scal.p[0];
+  simp_.S.sens_s += now_s() - t0;
+}
+
+void Context::simp_update(int step, double *hist) {
+  const icarat_simp_opts &o = simp_.o;
+  const long long n = L_.nelx;
+  double t2 = now_s();
+  if (o.use_projection) {
+    launch_mul(n, d_e[E_THD], d_e[E_DFDS], stream_);
+    launch_mul(n, d_e[E_THD], d_e[E_DGDS], stream_);
+  }
+  filter_device(d_e[E_DFDS], d_e[E_TMP], 1);
+  std::swap(d_e[E_DFDS], d_e[E_TMP]);
+  filter_device(d_e[E_DGDS], d_e[E_TMP], 1);
+  std::swap(d_e[E_DGDS], d_e[E_TMP]);
+  sync();
+  double t3 = now_s();
+  simp_.S.filter_s += t3 - t2;
+  if (hist) {
+    hist[4 * (step - 1) + 0] = simp_.c;
+    hist[4 * (step - 1) + 1] = simp_.vol;
+    hist[4 * (step - 1) + 2] = (double)simp_.iters;
+    hist[4 * (step - 1) + 3] = simp_.relres;
+  }
+  if (o.optimizer == 1) {
+    const double gc = step == 1 ? 0.0 : simp_.vol / simp_.V0 - 1.0;
+    mma_->update(d_e[E_DES], d_e[E_DFDS], gc, d_e[E_DGDS], o.xmin, o.xmax);
+  } else {
+    oc_device(d_e[E_DES], d_e[E_DFDS], d_e[E_DGDS], simp_.V0, o.xmin, o.xmax, o.use_projection,
+              simp_.beta);
+  }
+  sync();
+  simp_.S.update_s += now_s() - t3;
+}
+
+void Context::simp_end(double *x, icarat_simp_stats *st) {
+  d2h(x, d_e[E_DES].p, L_.nelx, stream_);
+  sync();
+  simp_.S.seconds = now_s() - simp_.t_all;
+  if (st) *st = simp_.S;
+}
+
+void Context::simp_run(const icarat_simp_opts &o, double *x, double *hist, icarat_simp_stats *st) {
+  simp_begin(o, x);
+  for (int step = 1; step <= o.n_iter; ++step) {
+    simp_pre(step);
+    // every load case: solve, compliance f.u, sensitivity; both summed over the
+    // cases (reference opt_multi_load.cpp:118-146)
+    const int ncase = prob_.num_load_cases();
+    for (int lc = 0; lc < ncase; ++lc) {
+      if (ncase > 1) use_load_case(lc);
+      const double t1 = now_s();
+      icarat_solve_stats ss{};
+      icarat_solve_opts so = o.solve;
+      // the first train_iters solves of the preconditioner's lifetime are
+      // solved by Jacobi-PCG and feed the training set
+      const bool harvest = so.precond == ICARAT_HIP_PRECOND_LEARNED && learned_ &&
+                           learned_->harvested() < learned_->opts().train_iters;
+      if (harvest) {
+        so.precond = ICARAT_HIP_PRECOND_JACOBI;
+        learned_->begin_harvest();
+        batch_hook_ = [this]() { learned_->snapshot(); };
+        try {
+          solve(so, nullptr, &ss);
+        } catch (...) {
+          batch_hook_ = nullptr;
+          learned_->end_harvest(false);
+          throw;
+        }
+        batch_hook_ = nullptr;
+        learned_->end_harvest(ss.status == ICARAT_HIP_OK);
+        const int ti = std::max(1, learned_->opts().train_iters);
+        const int h = learned_->harvested();  // 1..ti
+        const int steps =
+            learned_->opts().train_steps / ti + (h <= learned_->opts().train_steps % ti ? 1 : 0);
+        const double tt = now_s();
+        learned_->train(steps);
+        simp_.S.train_s += now_s() - tt;
+      } else {
+        solve(so, nullptr, &ss);
+      }
+      simp_.S.solve_s += now_s() - t1;
+      simp_case_post(lc, ss);
+    }
+    if (ncase > 1) use_load_case(prob_.load_case());
+    simp_update(step, hist);
+  }
+  simp_end(x, st);
 }
 
 // ---- learned preconditioner --------------------------------------------------
@@ -656,6 +781,26 @@ void Context::learned_apply(const double *r, double *z) {
   sync();
 }
 
+void Context::drop_graphs() {
+  for (int m = 0; m < 3; ++m)
+    for (int p = 0; p < 2; ++p) {
+      if (pcg_exec_[m][p]) (void)hipGraphExecDestroy(pcg_exec_[m][p]);
+      if (pcg_graph_[m][p]) (void)hipGraphDestroy(pcg_graph_[m][p]);
+      pcg_exec_[m][p] = nullptr;
+      pcg_graph_[m][p] = nullptr;
+    }
+  if (learned_) learned_->drop_graphs();
+}
+
+void Context::set_stencil_tuning(int kernel, int nchunks) {
+  ensure_device();
+  if (kernel < 0 || kernel > 1 || nchunks < 0) throw Error(kInvalidArgument, "bad stencil tuning");
+  sync();
+  kc_.kernel = kernel;
+  kc_.nchunks = nchunks;
+  drop_graphs();
+}
+
 void Context::set_stencil(int kernel, int sym_max) {
   ensure_device();
   KeCoef k = make_ke_coef(prob_.ke0(), d_ke.p);
@@ -668,7 +813,9 @@ void Context::set_stencil(int kernel, int sym_max) {
     }
   }
   k.kernel = kernel;
+  sync();
   kc_ = k;
+  drop_graphs();
 }
 
 // ---- measurement hook ------------------------------------------------------

@@ -28,6 +28,11 @@ class Context {
   hipStream_t stream() const { return stream_; }
 
   int make_dof();                          // numbering + device upload of masks
+  // load cases: add_neumann adds to the current case; solve uses it; simp_run
+  // sums compliance and sensitivities over all cases
+  int new_load_case();
+  void select_load_case(int i);
+  void use_load_case(int i);               // d_f <- case i (device copy)
   void sync_fext();                        // host fext -> device lattice vector
   void mark_fext_dirty() { fext_dirty_ = true; solved_ = false; }
   void ensure_fext() { if (fext_dirty_) sync_fext(); }
@@ -54,6 +59,20 @@ class Context {
               double xmax, int use_projection, double beta);
 
   void simp_run(const icarat_simp_opts &o, double *x, double *hist, icarat_simp_stats *st);
+  // the loop in phases: begin; per step: pre (filter, density, volume), per load
+  // case a solve then case_post (compliance, sensitivity), update (chain-rule
+  // filters, OC / MMA); end
+  void simp_begin(const icarat_simp_opts &o, const double *x);
+  void simp_pre(int step);
+  void simp_case_post(int lc, const icarat_solve_stats &ss);
+  void simp_update(int step, double *hist);
+  void simp_end(double *x, icarat_simp_stats *st);
+  struct SimpState {
+    icarat_simp_opts o{};
+    icarat_simp_stats S{};
+    double t_all = 0.0, V0 = 0.0, vol = 0.0, beta = 1.0, c = 0.0, relres = 0.0;
+    long long iters = 0;
+  } simp_;
 
   // learned preconditioner: set-up, a harvesting Jacobi solve, z = M(r) on numeq vectors
   void learned_setup(const icarat_learned_opts &o);
@@ -62,6 +81,10 @@ class Context {
   void learned_apply(const double *r, double *z);
   void time_kernel(int kernel, int reps, double *avg_ms, double *flops, double *bytes);
   void set_stencil(int kernel, int sym_max);
+  // kernel shape and z-chunk count of the FP64 K.u (keeps the symmetry class);
+  // captured graphs are rebuilt
+  void set_stencil_tuning(int kernel, int nchunks);
+  void drop_graphs();
 
   // ---- device buffers (public for the solver components) ----
   Lattice L_;
@@ -74,7 +97,8 @@ class Context {
   DevBuf<int> d_dof;                  // numnp*3 reference DOF numbers
   DevBuf<double> d_dirval;            // numnp*3 Dirichlet values (AoS)
   DevBuf<double> d_dirlat;            // Dirichlet values on the lattice (SoA, 0 at free DOFs)
-  DevBuf<double> d_f;                 // fext on the lattice (0 at constrained DOFs)
+  DevBuf<double> d_f;                 // fext of the current load case on the lattice (0 at constrained DOFs)
+  std::vector<DevBuf<double>> d_fcases_;  // every load case's fext on the lattice
   DevBuf<double> d_b;                 // right-hand side of the current solve
   DevBuf<double> d_dens, d_young;     // physical densities, SIMP young
   DevBuf<double> d_dinv;              // 1/diag K at free DOFs
@@ -127,6 +151,16 @@ class Context {
   // stalls is switched to FP64 in place.
   PcgScalars pcg64(double tol, int maxit, bool warm = false, int kmode = -1, double delta = 0.0);
   void assemble_nodal_solution();         // d_u = d_x + Dirichlet values
+  void copy_out_solution(double *u_host); // numeq copy of d_u (if u_host), marks solved
+  void prepare_rhs(hipStream_t s);        // d_b = f - K u_Dirichlet
+  // FP64 Jacobi-PCG in pieces for a driver that interleaves several contexts:
+  // start (rhs, init, scalars to h_pcg), verify (true residual replaces the
+  // recursive one, scalars to h_pcg), finish (nodal solution, true residual,
+  // stats).  The iterations themselves are enqueue_pcg_batch(-1, 0).
+  void pcg_start(double tol, int maxit, hipStream_t s);
+  void pcg_verify(hipStream_t s);
+  void pcg_finish(icarat_solve_stats &S);
+  void enqueue_pcg_batch(int kmode, int par);  // kBatch iterations on the stream
   void sync() { HIPCHK(hipStreamSynchronize(stream_)); }
   // Device buffers are allocated on first device use, so mesh, numbering,
   // loads and KE0 work without a GPU.
@@ -141,7 +175,6 @@ class Context {
   void require_dof();
   void require_density();
   void build_pcg_graph(int kmode);
-  void enqueue_pcg_batch(int kmode, int par);  // kBatch iterations on the stream
   void launch_pcg_batch(int kmode, int par);   // as a graph, or enqueued directly
 };
 

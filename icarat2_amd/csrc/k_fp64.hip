@@ -675,7 +675,8 @@ static void launch_stencil64_t(const Lattice &L, const KeCoef &ke, const double 
                                const double *p_old, double *p_new, double *y, PcgScalars *S,
                                double *partials, hipStream_t s, unsigned *nblocks) {
   if (ke.kernel == 0) {
-    const int nch = stencil_nchunks(L);
+    const int def = stencil_nchunks(L);
+    const int nch = ke.nchunks > 0 ? std::min(ke.nchunks, def) : def;
     const dim3 g = zgrid(L, nch);
     hipLaunchKernelGGL((k_stencil64z<PCG, SYM>), g, dim3(NT), 0, s, L, nch, ke.ke, ke.vals, young,
                        freemask, src, dinv, p_old, p_new, y, S, partials);

@@ -255,7 +255,7 @@ __global__ void k_fpcg_check(const double *__restrict__ partials, unsigned G, Pc
   if (!isfinite(rnorm)) {
     S->status = 2;
     S->done = 1;
-  } else if (rnorm < S->tol * sqrt(S->bnorm2)) {
+  } else if (rnorm < S->tol * sqrt(S->bnorm2) || rnorm == 0.0) {
     S->status = 0;
     S->done = 1;
   } else if (S->iter > S->maxit) {

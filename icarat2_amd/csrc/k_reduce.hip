@@ -60,7 +60,8 @@ __global__ __launch_bounds__(NT) void k_finalize(int op, const double *__restric
       if (!isfinite(rnorm) || !isfinite(rho1)) {
         S->status = 2;
         S->done = 1;
-      } else if (rnorm < S->tol * sqrt(S->bnorm2)) {
+      } else if (rnorm < S->tol * sqrt(S->bnorm2) || rnorm == 0.0) {
+        // (a zero residual is the exact solution, also for a zero right-hand side)
         S->status = 0;
         S->done = 1;
       } else if (S->iter > S->maxit) {
@@ -80,7 +81,7 @@ __global__ __launch_bounds__(NT) void k_finalize(int op, const double *__restric
       if (!isfinite(rnorm)) {
         S->status = 2;
         S->done = 1;
-      } else if (rnorm < S->tol * sqrt(S->bnorm2)) {
+      } else if (rnorm < S->tol * sqrt(S->bnorm2) || rnorm == 0.0) {
         S->status = 0;
         S->done = 1;
       } else {

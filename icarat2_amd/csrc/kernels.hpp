@@ -22,6 +22,7 @@ struct KeCoef {
   const double *ke = nullptr;  // device KE0
   int sym = 0;
   int kernel = 0;              // 0: z-marching column kernel, 1: 3-D tile kernel
+  int nchunks = 0;             // z-marching: z-chunks per column (0: default, capped by it)
   KeVals vals{};
 };
 // Orbit values of a host KE0 (sym = 2 if the cube table reproduces it to
@@ -37,6 +38,8 @@ void launch_diag_inv(const Lattice &L, const double *ke, const double *young,
                      const uint8_t *freemask, double *dinv, double *diag, hipStream_t s);
 // number of workgroups of the largest K.u launch (sizes the partials buffer)
 long long stencil_blocks(const Lattice &L);
+// default z-chunk count of the z-marching kernel (one round of resident workgroups)
+int stencil_nchunks(const Lattice &L);
 // y = K u on the lattice; y masked to free DOFs.  u is used as given
 // (Dirichlet values included).  ke.kernel selects the z-marching or the 3-D
 // halo-tile kernel.

@@ -23,6 +23,7 @@ class LearnedPrecond {
   ~LearnedPrecond();
   const icarat_learned_opts &opts() const { return o_; }
   UNet &net() { return *net_; }
+  void drop_graphs();  // captured batches (rebuilt at the next solve)
   // explicit parameters: the network is used as given (gain 1 until a harvest sets it)
   void set_params(const float *host);
   const NetGrid &grid() const { return g0_; }
@@ -81,7 +82,6 @@ class LearnedPrecond {
   hipGraph_t graph_[3][2] = {};
   hipGraphExec_t exec_[3][2] = {};
   void build_graph(int kmode);
-  void drop_graphs();
   double evaluate();  // mean relative squared error over the stored samples
   void enqueue_batch(int kmode, int par);
   void precondition(const double *r);  // z_ = M(r)

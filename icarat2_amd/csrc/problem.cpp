@@ -119,8 +119,21 @@ int Problem::make_dof() {
   for (long long n = 0; n < L_.numnp; ++n)
     for (int c = 0; c < 3; ++c)
       if ((dirflag_[n] >> c) & 1u) dof_[3 * n + c] = q++;
-  fext_.assign(numeq_, 0.0);
+  cases_.assign(1, std::vector<double>(numeq_, 0.0));
+  cur_ = 0;
   return numeq_;
+}
+
+int Problem::new_load_case() {
+  if (numeq_ < 0) throw Error(kNotReady, "make_dof must precede load cases");
+  cases_.emplace_back(numeq_, 0.0);
+  cur_ = (int)cases_.size() - 1;
+  return cur_;
+}
+
+void Problem::select_load_case(int i) {
+  if (i < 0 || i >= (int)cases_.size()) throw Error(kInvalidArgument, "no such load case");
+  cur_ = i;
 }
 
 bool Problem::has_nonzero_dirichlet() const {
@@ -148,7 +161,7 @@ void Problem::add_neumann(const double val[3], const double range[6]) {
       if (inside(n))
         for (int c = 0; c < 3; ++c) {
           const int q = dof_[3 * n + c];
-          if (q < numeq_) fext_[q] += val[c];
+          if (q < numeq_) fext()[q] += val[c];
         }
     return;
   }
@@ -200,7 +213,7 @@ void Problem::add_neumann(const double val[3], const double range[6]) {
           for (int s = 0; s < 4; ++s)
             for (int c = 0; c < 3; ++c) {
               const int q = dof_[3 * nd[s] + c];
-              if (q < numeq_) fext_[q] += fe[s][c];
+              if (q < numeq_) fext()[q] += fe[s][c];
             }
         }
       }

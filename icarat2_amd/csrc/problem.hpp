@@ -42,10 +42,18 @@ class Problem {
   bool has_nonzero_dirichlet() const;
 
   // Point load (degenerate box) or face-distributed load (reference
-  // io/boundary_condition_icarat.hpp:212-410).  Adds into fext (numeq order).
+  // io/boundary_condition_icarat.hpp:212-410).  Adds into the current load
+  // case's fext (numeq order).
   void add_neumann(const double val[3], const double range[6]);
-  std::vector<double> &fext() { return fext_; }
-  const std::vector<double> &fext() const { return fext_; }
+  std::vector<double> &fext() { return cases_[cur_]; }
+  const std::vector<double> &fext() const { return cases_[cur_]; }
+  // Load cases (one Force per case, reference opt_multi_load.cpp:74-84):
+  // make_dof leaves one empty case; new_load_case appends one and makes it current.
+  int new_load_case();
+  void select_load_case(int i);
+  int load_case() const { return cur_; }
+  int num_load_cases() const { return (int)cases_.size(); }
+  const std::vector<double> &case_fext(int i) const { return cases_[i]; }
 
   const double *ke0() const { return ke0_; }
   double element_volume() const { return vol_e_; }
@@ -63,7 +71,8 @@ class Problem {
   std::vector<double> dirval_;    // numnp*3
   std::vector<int32_t> dof_;      // numnp*3
   std::vector<uint8_t> free_;     // per node: bit c = free component c
-  std::vector<double> fext_;
+  std::vector<std::vector<double>> cases_{std::vector<double>()};
+  int cur_ = 0;
   int numeq_ = -1;
   double ke0_[576];
   double vol_e_ = 0.0;

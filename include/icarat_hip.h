@@ -104,6 +104,14 @@ int icarat_hip_make_dof(icarat_hip_ctx *ctx, int32_t *numeq);
  * element face with 4 nodes in the box (BCIcarat::input_neumann /
  * generateLoadMain, io/boundary_condition_icarat.hpp:212-410). */
 int icarat_hip_add_neumann(icarat_hip_ctx *ctx, const double val[3], const double range[6]);
+/* Load cases (one Force per case, reference sample/software/opt-multi-load/
+ * opt_multi_load.cpp:74-84).  make_dof leaves one empty case (index 0);
+ * new_load_case appends an empty case and makes it current.  add_neumann,
+ * get_fext / set_fext and solve act on the current case; simp_run sums the
+ * compliance and the sensitivities over all cases (opt_multi_load.cpp:118-146). */
+int icarat_hip_new_load_case(icarat_hip_ctx *ctx, int32_t *index);
+int icarat_hip_select_load_case(icarat_hip_ctx *ctx, int32_t index);
+int icarat_hip_load_case_count(const icarat_hip_ctx *ctx, int32_t *n);
 /* dof[3*node + c] = reference DOF number (Node::dof, problem/base.hpp:86) */
 int icarat_hip_get_dof(const icarat_hip_ctx *ctx, int32_t *dof);
 /* Force::fext_org (problem/base.hpp:131), numeq entries */
@@ -265,6 +273,37 @@ int icarat_hip_unet_grads(icarat_hip_unet *net, const float *x, const float *tar
 /* one Adam step (b1 0.9, b2 0.999, eps 1e-8) on the loss above; *loss before the step */
 int icarat_hip_unet_train_step(icarat_hip_unet *net, const float *x, const float *target,
                                float lr, double *loss);
+
+/* ---- batched problems ------------------------------------------------------------
+ * No reference counterpart (the reference runs one problem per process): several
+ * independent problems of any size, each a prepared context (mesh, supports,
+ * load cases, filter), solved together.  Per solve round the PCG iterations of
+ * every problem are one captured graph: a branch per problem on its context's
+ * stream, so the problems' kernels run concurrently; each problem keeps its own
+ * FP64 verified stop.  Batched solves are FP64 Jacobi-PCG.  A context added to a
+ * batch must outlive it and must not be used on its own while the batch runs. */
+typedef struct icarat_hip_batch icarat_hip_batch;
+int icarat_hip_batch_create(icarat_hip_batch **out);
+int icarat_hip_batch_destroy(icarat_hip_batch *b);
+const char *icarat_hip_batch_last_error(const icarat_hip_batch *b);
+int icarat_hip_batch_add(icarat_hip_batch *b, icarat_hip_ctx *ctx);
+/* problems in the batch and graph launches so far */
+int icarat_hip_batch_info(const icarat_hip_batch *b, int32_t *n, int64_t *graph_launches);
+/* one solve of every problem's current load case; stats[n] (may be NULL) */
+int icarat_hip_batch_solve(icarat_hip_batch *b, const icarat_solve_opts *opts,
+                           icarat_solve_stats *stats);
+/* the SIMP loop of every problem in lock step (as icarat_hip_simp_run each, load
+ * cases summed): x[i] (nelx of problem i) in/out, hist[i] (may be NULL, or hist
+ * itself NULL) 4 * n_iter per problem, stats[n] (may be NULL) */
+int icarat_hip_batch_simp_run(icarat_hip_batch *b, const icarat_simp_opts *opts,
+                              double *const *x, double *const *hist, icarat_simp_stats *stats);
+/* Autotuner: times reps whole-batch graph launches (16 PCG iterations of every
+ * problem) under each stencil configuration -- 0..2: z-marching columns with
+ * the default z-chunk count divided by 1, 2, 4; 3: 3-D halo tiles -- and keeps
+ * the fastest for every problem.  Needs densities on every problem; overwrites
+ * the solver state.  *config = the chosen configuration, ms[4] (may be NULL) the
+ * time of one launch per configuration. */
+int icarat_hip_batch_autotune(icarat_hip_batch *b, int32_t reps, int32_t *config, double *ms);
 
 /* ---- measurement ------------------------------------------------------------ */
 /* No reference counterpart: a timing hook for benchmarks.  Launches one kernel
