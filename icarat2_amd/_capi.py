@@ -125,6 +125,7 @@ _SIGS = {
     "icarat_hip_batch_destroy": (C.c_int, [C.c_void_p]),
     "icarat_hip_batch_last_error": (C.c_char_p, [C.c_void_p]),
     "icarat_hip_batch_add": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "icarat_hip_batch_set_budget": (C.c_int, [C.c_void_p, C.c_int64]),
     "icarat_hip_batch_info": (C.c_int, [C.c_void_p, i32p, i64p]),
     "icarat_hip_batch_solve": (C.c_int, [C.c_void_p, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "icarat_hip_batch_simp_run": (C.c_int, [C.c_void_p, C.POINTER(SimpOpts), C.POINTER(dp),
@@ -539,6 +540,9 @@ class Batch:
     def add(self, c):
         self._chk(self.L.icarat_hip_batch_add(self.h, c.h), "batch_add")
         self.ctx.append(c)
+
+    def set_budget(self, elements):
+        self._chk(self.L.icarat_hip_batch_set_budget(self.h, int(elements)), "batch_set_budget")
 
     def info(self):
         n, k = C.c_int32(), C.c_int64()

@@ -32,23 +32,29 @@ void Batch::add(Context *c) {
 }
 
 void Batch::drop_graph() {
-  if (ge_) (void)hipGraphExecDestroy(ge_);
-  if (g_) (void)hipGraphDestroy(g_);
-  ge_ = nullptr;
-  g_ = nullptr;
+  for (auto &kv : graphs_) {
+    if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
+    if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
+  }
+  graphs_.clear();
 }
 
-void Batch::build_graph() {
-  if (ge_) return;
+hipGraphExec_t Batch::graph_for(const std::vector<char> &run) {
+  const std::string key(run.begin(), run.end());
+  auto it = graphs_.find(key);
+  if (it != graphs_.end()) return it->second.second;
   const int n = size();
   while ((int)ev_.size() < 2 * n) {
     hipEvent_t e;
     HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     ev_.push_back(e);
   }
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
   HIPCHK(hipStreamBeginCapture(ms_, hipStreamCaptureModeGlobal));
   try {
     for (int i = 0; i < n; ++i) {
+      if (!run[i]) continue;
       // fork: problem i's iterations on its own stream after the batch stream
       HIPCHK(hipEventRecord(ev_[2 * i], ms_));
       HIPCHK(hipStreamWaitEvent(ctx_[i]->stream(), ev_[2 * i], 0));
@@ -58,97 +64,127 @@ void Batch::build_graph() {
       HIPCHK(hipStreamWaitEvent(ms_, ev_[2 * i + 1], 0));
     }
   } catch (...) {
-    hipGraph_t g = nullptr;
-    (void)hipStreamEndCapture(ms_, &g);
-    if (g) (void)hipGraphDestroy(g);
+    hipGraph_t gg = nullptr;
+    (void)hipStreamEndCapture(ms_, &gg);
+    if (gg) (void)hipGraphDestroy(gg);
     throw;
   }
-  HIPCHK(hipStreamEndCapture(ms_, &g_));
-  HIPCHK(hipGraphInstantiate(&ge_, g_, nullptr, nullptr, 0));
+  HIPCHK(hipStreamEndCapture(ms_, &g));
+  HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+  graphs_[key] = {g, ge};
+  return ge;
 }
 
-void Batch::solve_set(const std::vector<int> &act, double tol, int maxit,
-                      std::vector<icarat_solve_stats> &st) {
+void Batch::engine(double tol, int maxit, const std::function<bool(int)> &start,
+                   const std::function<void(int, const icarat_solve_stats &)> &done) {
   const int n = size();
-  std::vector<char> in(n, 0);
-  for (int i : act) in[i] = 1;
-  // every context finishes its own pending work before the batch stream uses it
-  for (Context *c : ctx_) c->sync();
-  for (int i = 0; i < n; ++i) {
-    Context &c = *ctx_[i];
-    if (in[i]) {
-      c.pcg_start(tol, maxit, ms_);
-    } else {
-      // parked: its branch of the graph skips every kernel
-      PcgScalars park{};
-      park.done = 1;
-      *c.h_pcg.p = park;
-      HIPCHK(hipMemcpyAsync(c.d_pcg.p, c.h_pcg.p, sizeof(PcgScalars), hipMemcpyHostToDevice, ms_));
+  std::vector<char> run(n, 0), admitted(n, 0), over(n, 0);
+  // admission order: largest problem first
+  std::vector<int> queue(n);
+  for (int i = 0; i < n; ++i) queue[i] = i;
+  std::stable_sort(queue.begin(), queue.end(),
+                   [&](int a, int b) { return ctx_[a]->lattice().nelx > ctx_[b]->lattice().nelx; });
+  size_t next = 0;
+  long long load = 0;
+  auto begin = [&](int i) {
+    ctx_[i]->sync();
+    run[i] = start(i) ? 1 : 0;
+    if (!run[i] && admitted[i] && !over[i]) {
+      over[i] = 1;
+      load -= ctx_[i]->lattice().nelx;
     }
-  }
+  };
+  auto admit = [&]() {
+    while (next < queue.size()) {
+      const int i = queue[next];
+      const long long e = ctx_[i]->lattice().nelx;
+      if (budget_ > 0 && load > 0 && load + e > budget_) break;
+      ++next;
+      admitted[i] = 1;
+      load += e;
+      begin(i);
+    }
+  };
+  admit();
   HIPCHK(hipStreamSynchronize(ms_));
-  build_graph();
-  // 0 running, 1 finished
-  std::vector<char> fin(n, 1);
-  for (int i : act) fin[i] = 0;
   for (;;) {
     // solves whose recursive residual passed: the FP64 true residual decides
-    std::vector<int> ver;
-    for (int i : act) {
-      if (fin[i]) continue;
+    std::vector<int> ver, fin;
+    for (int i = 0; i < n; ++i) {
+      if (!run[i]) continue;
       const PcgScalars &h = *ctx_[i]->h_pcg.p;
       if (h.done && h.status == 0) ver.push_back(i);
-      else if (h.done) fin[i] = 1;  // iteration cap or breakdown
+      else if (h.done) fin.push_back(i);  // iteration cap or breakdown
+      else if (h.iter > maxit) {
+        ctx_[i]->h_pcg.p->status = 1;
+        fin.push_back(i);
+      }
     }
     for (int i : ver) ctx_[i]->pcg_verify(ms_);
     if (!ver.empty()) HIPCHK(hipStreamSynchronize(ms_));
     for (int i : ver)
-      if (ctx_[i]->h_pcg.p->done) fin[i] = 1;
-    std::vector<int> run;
-    for (int i : act) {
-      if (fin[i]) continue;
-      PcgScalars &h = *ctx_[i]->h_pcg.p;
-      if (h.iter > maxit) {
-        h.status = 1;
-        fin[i] = 1;
-        continue;
-      }
-      run.push_back(i);
+      if (ctx_[i]->h_pcg.p->done) fin.push_back(i);
+    // finished solves: hand over, then the problem's next solve (if any)
+    for (int i : fin) {
+      icarat_solve_stats st{};
+      ctx_[i]->pcg_finish(st);
+      run[i] = 0;
+      done(i, st);
+      begin(i);
     }
-    if (run.empty()) break;
+    admit();  // problems that finished their loop free budget
+    if (!fin.empty() || next > 0) HIPCHK(hipStreamSynchronize(ms_));
+    bool any = false;
+    for (int i = 0; i < n; ++i) any = any || run[i];
+    if (!any && next >= queue.size()) break;
+    if (!any) continue;
+    // a solve that has just started or verified may already be done (zero
+    // right-hand side, or the true residual passed): handle it before launching
+    bool pending = false;
+    for (int i = 0; i < n; ++i)
+      if (run[i] && ctx_[i]->h_pcg.p->done) pending = true;
+    if (pending) continue;
     // the graph starts every branch at p-buffer parity 0
-    for (int i : run) {
+    for (int i = 0; i < n; ++i) {
+      if (!run[i]) continue;
       Context &c = *ctx_[i];
       if (c.h_pcg.p->iter & 1)
         HIPCHK(hipMemcpyAsync(c.d_p[0].p, c.d_p[1].p, c.d_p[0].bytes(), hipMemcpyDeviceToDevice, ms_));
     }
-    HIPCHK(hipGraphLaunch(ge_, ms_));
+    HIPCHK(hipGraphLaunch(graph_for(run), ms_));
     ++launches_;
-    for (int i : run) {
+    for (int i = 0; i < n; ++i) {
+      if (!run[i]) continue;
       Context &c = *ctx_[i];
       HIPCHK(hipMemcpyAsync(c.h_pcg.p, c.d_pcg.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, ms_));
     }
     HIPCHK(hipStreamSynchronize(ms_));
   }
-  st.assign(n, icarat_solve_stats{});
-  for (int i : act) ctx_[i]->pcg_finish(st[i]);
 }
 
 void Batch::solve(const icarat_solve_opts &o, icarat_solve_stats *st) {
   if (o.precision != ICARAT_HIP_FP64 || o.precond != ICARAT_HIP_PRECOND_JACOBI)
     throw Error(kInvalidArgument, "batched solves are FP64 Jacobi-PCG");
   if (!(o.tol > 0.0) || o.maxit < 0) throw Error(kInvalidArgument, "tol must be > 0 and maxit >= 0");
-  std::vector<int> act(size());
-  for (int i = 0; i < size(); ++i) act[i] = i;
-  const double t0 = now_s();
-  std::vector<icarat_solve_stats> s;
-  solve_set(act, o.tol, o.maxit, s);
-  const double dt = now_s() - t0;
-  for (int i = 0; i < size(); ++i) {
-    s[i].seconds = dt;
-    ctx_[i]->copy_out_solution(nullptr);
-    if (st) st[i] = s[i];
-  }
+  const int n = size();
+  std::vector<char> started(n, 0);
+  std::vector<double> t0(n, 0.0);
+  engine(
+      o.tol, o.maxit,
+      [&](int i) {
+        if (started[i]) return false;
+        started[i] = 1;
+        t0[i] = now_s();
+        ctx_[i]->pcg_start(o.tol, o.maxit, ms_);
+        return true;
+      },
+      [&](int i, const icarat_solve_stats &s) {
+        ctx_[i]->copy_out_solution(nullptr);
+        if (st) {
+          st[i] = s;
+          st[i].seconds = now_s() - t0[i];
+        }
+      });
 }
 
 void Batch::simp_run(const icarat_simp_opts &o, double *const *x, double *const *hist,
@@ -157,38 +193,37 @@ void Batch::simp_run(const icarat_simp_opts &o, double *const *x, double *const 
     throw Error(kInvalidArgument, "batched solves are FP64 Jacobi-PCG");
   if (!x) throw Error(kInvalidArgument, "null design array");
   const int n = size();
-  for (int i = 0; i < n; ++i) {
+  for (int i = 0; i < n; ++i)
     if (!x[i]) throw Error(kInvalidArgument, "null design vector");
-    ctx_[i]->simp_begin(o, x[i]);
-  }
-  int maxcase = 0;
-  for (Context *c : ctx_) maxcase = std::max(maxcase, c->problem().num_load_cases());
-  std::vector<icarat_solve_stats> s;
-  for (int step = 1; step <= o.n_iter; ++step) {
-    for (Context *c : ctx_) c->simp_pre(step);
-    for (int lc = 0; lc < maxcase; ++lc) {
-      std::vector<int> act;
-      for (int i = 0; i < n; ++i) {
-        const int nc = ctx_[i]->problem().num_load_cases();
-        if (nc <= lc) continue;
-        if (nc > 1) ctx_[i]->use_load_case(lc);
-        act.push_back(i);
-      }
-      const double t0 = now_s();
-      solve_set(act, o.solve.tol, o.solve.maxit, s);
-      const double dt = now_s() - t0;
-      for (int i : act) {
-        ctx_[i]->simp_.S.solve_s += dt;
-        ctx_[i]->copy_out_solution(nullptr);
-        ctx_[i]->simp_case_post(lc, s[i]);
-      }
-    }
-    for (int i = 0; i < n; ++i) {
-      Context &c = *ctx_[i];
-      if (c.problem().num_load_cases() > 1) c.use_load_case(c.problem().load_case());
-      c.simp_update(step, hist ? hist[i] : nullptr);
-    }
-  }
+  for (int i = 0; i < n; ++i) ctx_[i]->simp_begin(o, x[i]);
+  // per problem: design step and load case of the solve in flight
+  std::vector<int> step(n, 1), lc(n, -1);
+  std::vector<double> t_solve(n, 0.0);
+  engine(
+      o.solve.tol, o.solve.maxit,
+      [&](int i) {
+        Context &c = *ctx_[i];
+        if (step[i] > o.n_iter) return false;
+        const int nc = c.problem().num_load_cases();
+        if (lc[i] < 0) c.simp_pre(step[i]);  // a new design iteration
+        ++lc[i];
+        if (nc > 1) c.use_load_case(lc[i]);
+        t_solve[i] = now_s();
+        c.pcg_start(o.solve.tol, o.solve.maxit, ms_);
+        return true;
+      },
+      [&](int i, const icarat_solve_stats &s) {
+        Context &c = *ctx_[i];
+        c.simp_.S.solve_s += now_s() - t_solve[i];
+        c.copy_out_solution(nullptr);
+        c.simp_case_post(lc[i], s);
+        const int nc = c.problem().num_load_cases();
+        if (lc[i] + 1 < nc) return;  // the next case of this design iteration
+        if (nc > 1) c.use_load_case(c.problem().load_case());
+        c.simp_update(step[i], hist ? hist[i] : nullptr);
+        ++step[i];
+        lc[i] = -1;
+      });
   for (int i = 0; i < n; ++i) ctx_[i]->simp_end(x[i], st ? st + i : nullptr);
 }
 
@@ -219,10 +254,10 @@ int Batch::autotune(int reps, double *ms_out) {
     // launches time real iterations on the current densities
     for (Context *c : ctx_) c->sync();
     for (Context *c : ctx_) c->pcg_start(1e-300, 1 << 30, ms_);
-    build_graph();
-    HIPCHK(hipGraphLaunch(ge_, ms_));  // warm
+    hipGraphExec_t ge = graph_for(std::vector<char>(n, 1));
+    HIPCHK(hipGraphLaunch(ge, ms_));  // warm
     HIPCHK(hipEventRecord(a, ms_));
-    for (int r = 0; r < reps; ++r) HIPCHK(hipGraphLaunch(ge_, ms_));
+    for (int r = 0; r < reps; ++r) HIPCHK(hipGraphLaunch(ge, ms_));
     HIPCHK(hipEventRecord(b, ms_));
     HIPCHK(hipEventSynchronize(b));
     float t = 0.f;

@@ -1,13 +1,17 @@
 // batch.hpp -- several independent SIMP problems (contexts of any size) driven
-// together.  The PCG iterations of all of them are captured into one graph:
-// one branch per problem on that problem's stream, forked from and joined into
-// the batch stream through events, so the problems' kernels run concurrently
-// and small problems share the GPU.  The graph is launched until every
-// problem's solve has passed its FP64 verified stop; problems that are done
-// (or not taking part in a round) skip their kernels.
+// together.  Each problem advances through its own design loop; whenever
+// problems have a solve running, their PCG iterations are launched together as
+// one graph: one branch per running problem on that problem's stream, forked
+// from and joined into the batch stream through events, so the problems'
+// kernels run concurrently and small problems share the GPU with large ones.
+// A graph is captured once per set of running problems and reused.  Every
+// solve keeps its own FP64 verified stop.
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <functional>
+#include <map>
+#include <string>
 #include <vector>
 
 #include "../../include/icarat_hip.h"
@@ -28,8 +32,8 @@ class Batch {
 
   // one FP64 Jacobi-PCG solve of every context's current load case
   void solve(const icarat_solve_opts &o, icarat_solve_stats *st);
-  // the SIMP loop of every context in lock step; per design iteration, load
-  // case k of every problem that has one is solved in round k
+  // the SIMP loop of every context; each problem advances on its own (design
+  // phases between its solves), the solves of all problems share the launches
   void simp_run(const icarat_simp_opts &o, double *const *x, double *const *hist,
                 icarat_simp_stats *st);
   // Times whole-batch graph launches under each stencil configuration (applied
@@ -38,19 +42,30 @@ class Batch {
   // Returns the chosen configuration and the time of one launch per
   // configuration (ms[4]).
   int autotune(int reps, double *ms);
+  // Admission: problems enter the running set largest first while the running
+  // problems' elements stay within the budget (at least one runs; 0: no
+  // limit); an admitted problem runs its whole loop.
+  void set_budget(long long elements) { budget_ = elements; }
+  long long budget() const { return budget_; }
 
  private:
   std::vector<Context *> ctx_;
   hipStream_t ms_ = nullptr;
   std::vector<hipEvent_t> ev_;
-  hipGraph_t g_ = nullptr;
-  hipGraphExec_t ge_ = nullptr;
+  // one graph per set of problems with a running solve (key: membership bytes)
+  std::map<std::string, std::pair<hipGraph_t, hipGraphExec_t>> graphs_;
   long long launches_ = 0;
+  // two problems that each fill the GPU (measured on the 24-problem workload:
+  // 800k..1.5M elements within 1% of each other, all-at-once 2% slower)
+  long long budget_ = 1500000;
   void drop_graph();
-  void build_graph();
-  // Solves the contexts listed in act concurrently; the others are parked.
-  void solve_set(const std::vector<int> &act, double tol, int maxit,
-                 std::vector<icarat_solve_stats> &st);
+  hipGraphExec_t graph_for(const std::vector<char> &run);
+  // The solve engine: start(i) begins problem i's next solve (pcg_start on the
+  // batch stream) and returns false when problem i has no more solves; done(i,
+  // st) receives a finished solve.  Iterates every running solve in shared
+  // graph launches until no problem starts another solve.
+  void engine(double tol, int maxit, const std::function<bool(int)> &start,
+              const std::function<void(int, const icarat_solve_stats &)> &done);
   void apply_config(int cfg);
 };
 

@@ -592,6 +592,11 @@ int icarat_hip_batch_add(icarat_hip_batch *h, icarat_hip_ctx *ctx) {
   return batch_guard(h, [&](icarat_hip::Batch &b) { b.add(ctx->c); });
 }
 
+int icarat_hip_batch_set_budget(icarat_hip_batch *h, int64_t elements) {
+  if (elements < 0) return ICARAT_HIP_INVALID_ARGUMENT;
+  return batch_guard(h, [&](icarat_hip::Batch &b) { b.set_budget(elements); });
+}
+
 int icarat_hip_batch_info(const icarat_hip_batch *hc, int32_t *n, int64_t *launches) {
   icarat_hip_batch *h = const_cast<icarat_hip_batch *>(hc);
   return batch_guard(h, [&](icarat_hip::Batch &b) {

@@ -435,6 +435,8 @@ void Context::prepare_rhs(hipStream_t s) {
 void Context::pcg_start(double tol, int maxit, hipStream_t s) {
   require_density();
   ensure_fext();
+  // work on this context's stream (load-case switch, density) precedes stream s
+  sync();
   if (!(tol > 0.0) || maxit < 0) throw Error(kInvalidArgument, "tol must be > 0 and maxit >= 0");
   prepare_rhs(s);
   PcgScalars init{};

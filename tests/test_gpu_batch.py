@@ -142,3 +142,19 @@ def test_batch_autotune_keeps_results(gpu):
     for a, c in zip(st0, st1):
         assert c.status == 0 and c.true_relres <= 1e-9
         assert abs(a.iters - c.iters) <= max(2, 0.01 * a.iters)
+
+
+@pytest.mark.parametrize("budget", [0, 1])
+def test_batch_budget_does_not_change_results(gpu, budget):
+    """Admission order and concurrency change only the schedule."""
+    from icarat2_amd import workloads as W
+    specs = [_scaled(s, 2) for s in _batch_specs()]
+    ctxs = [W.build_context(gpu, s) for s in specs]
+    b = gpu.Batch(ctxs)
+    x0s = [W.initial_design(s) for s in specs]
+    xa, ha, _ = b.simp_run(x0s, 2, tol=1e-9)
+    b.set_budget(budget)
+    xb, hb, _ = b.simp_run(x0s, 2, tol=1e-9)
+    for x1, x2, h1, h2 in zip(xa, xb, ha, hb):
+        assert np.max(np.abs(x1 - x2)) < 1e-12
+        assert np.allclose(h1, h2, rtol=1e-12, atol=0)
