@@ -74,6 +74,7 @@ void Context::ensure_device() {
   const long long filter_blocks = (long long)((L_.nx + 31) / 32) * ((L_.ny + 7) / 8) * ((L_.nz + 3) / 4);
   const long long nparts = std::max({stencil_blocks(L_), filter_blocks, 4096LL}) * 4;
   d_partials.alloc(nparts);
+  d_partials2.alloc(2 * 4096);
   d_counter.alloc(4);
   HIPCHK(hipMemsetAsync(d_counter.p, 0, d_counter.bytes(), stream_));
   d_pcg.alloc(1);
@@ -237,6 +238,21 @@ void Context::get_diag(double *d) {
 }
 
 void Context::enqueue_pcg_batch(int kmode, int par) {
+  if (kmode < 0) {
+    // FP64: the finalizes are folded into the next kernel; one at the end of
+    // the batch leaves the scalars complete for the host
+    unsigned Gu = 0;
+    for (int j = 0; j < kBatch; ++j) {
+      double *p_old = d_p[(j + par) & 1], *p_new = d_p[(j + par + 1) & 1];
+      const unsigned Gs = launch_pcg_stencil64_fused(L_, kc_, d_young, d_free, d_r, d_dinv, p_old,
+                                                     p_new, d_q, d_pcg, d_partials, d_partials2,
+                                                     Gu, j, stream_);
+      Gu = launch_pcg_update64_fused(n3_, d_x, d_r, p_new, d_q, d_dinv, d_pcg, d_partials2,
+                                     d_partials, Gs, j, stream_);
+    }
+    launch_finalize(kFinPcgBeta, 2, d_partials2, Gu, d_pcg, nullptr, nullptr, stream_);
+    return;
+  }
   for (int j = 0; j < kBatch; ++j) {
     double *p_old = d_p[(j + par) & 1], *p_new = d_p[(j + par + 1) & 1];
     if (kmode < 0)
@@ -841,7 +857,19 @@ void Context::time_kernel(int kernel, int reps, double *avg_ms, double *flops, d
     KeCoef kc = kc_;
     if (kernel == 3) kc.sym = 0;                 // KE0 from memory
     if (kernel == 4) kc.kernel = 1 - kc.kernel;  // the other FP64 kernel shape
-    if (kernel <= 4) {
+    if (kernel == 0) {
+      // the fused stencil of the solver (in-batch iteration 1): its beta step
+      // reduces update partials; ones keep every stop test false
+      std::vector<double> ones(d_partials2.n, 1.0);
+      h2d(d_partials2.p, ones.data(), ones.size(), stream_);
+      h_pcg.p->rho_slot[0] = h_pcg.p->rho_slot[1] = 1.0;
+      h_pcg.p->iter_base = 0;
+      h2d(d_pcg.p, h_pcg.p, 1, stream_);
+      launch = [this]() {
+        launch_pcg_stencil64_fused(L_, kc_, d_young, d_free, d_r, d_dinv, d_p[0], d_p[1], d_q,
+                                   d_pcg, d_partials, d_partials2, 512, 1, stream_);
+      };
+    } else if (kernel <= 4) {
       launch = [this, kc]() {
         launch_pcg_stencil64(L_, kc, d_young, d_free, d_r, d_dinv, d_p[0], d_p[1], d_q, d_pcg,
                              d_partials, d_counter, stream_, false);
@@ -856,16 +884,21 @@ void Context::time_kernel(int kernel, int reps, double *avg_ms, double *flops, d
     F = 2.0 * 24 * 24 * E;                      // dense KE0 x U_e contraction
     B = 3.0 * N * 8 * 3 + 2.0 * 3 * N * 8 + E * 8 + N;  // r, dinv, p_old in; p, q out; young; mask
   } else if (kernel == 1) {
+    // the fused update of the solver: alpha = rho / p.q from stencil partials
+    // (ones: p.q = block count > 0), then the vector update
     PcgScalars s0{};
     s0.rho = 1.0;
-    s0.alpha = 0.0;
     s0.tol = 0.0;
     s0.maxit = 1 << 30;
+    s0.rho_slot[0] = s0.rho_slot[1] = 0.0;  // alpha = 0: the vectors do not drift
     *h_pcg.p = s0;
     h2d(d_pcg.p, h_pcg.p, 1, stream_);
-    launch = [&]() {
-      launch_pcg_update64(n3_, d_x, d_r, d_p[1], d_q, d_dinv, d_pcg, d_partials, d_counter,
-                          stream_, false);
+    const unsigned gs = (unsigned)std::min<long long>(stencil_blocks(L_), 4096);
+    std::vector<double> ones(gs, 1.0);
+    h2d(d_partials.p, ones.data(), gs, stream_);
+    launch = [this, gs]() {
+      launch_pcg_update64_fused(n3_, d_x, d_r, d_p[1], d_q, d_dinv, d_pcg, d_partials2,
+                                d_partials, gs, 1, stream_);
     };
     F = 8.0 * 3 * N;
     B = (5.0 + 2.0) * 3 * N * 8;                // x, r, p, q, dinv in; x, r out

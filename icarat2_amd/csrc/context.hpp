@@ -108,6 +108,7 @@ class Context {
   DevBuf<double> d_y;                 // scratch lattice vector
   DevBuf<double> d_numeq[2];          // numeq staging
   DevBuf<double> d_partials;
+  DevBuf<double> d_partials2;         // update partials of fused PCG batches
   DevBuf<unsigned> d_counter;
   DevBuf<PcgScalars> d_pcg;
   PinnedBuf<PcgScalars> h_pcg;

@@ -42,7 +42,11 @@ struct PcgScalars {
   int status;        // 0 running/converged, 1 capped, 2 breakdown, 3 replacement due,
                      // 4 preconditioned residual not a descent direction
   int restart;       // flexible PCG: next beta = 0
-  int pad_;
+  // Fused Jacobi-PCG batches (the finalizes folded into the next kernel):
+  // rho of in-batch iteration j lives in rho_slot[j & 1] so that a kernel never
+  // reads a field its own blocks write; iter_base = iterations before the batch.
+  int iter_base;
+  double rho_slot[2];
 };
 
 // Scalars of the device-resident OC bisection.
@@ -95,14 +99,28 @@ __device__ __forceinline__ void store_partials(double (&v)[NV], double *partials
 }
 
 // Fixed-order sum of G partials per quantity (call from one block of NT threads).
+// Each thread issues up to 8 loads per quantity before adding them, so the
+// latency of the partials (written by the previous kernel, possibly on another
+// XCD) is paid once per group rather than once per value.
 template <int NT, int NV>
 __device__ __forceinline__ void sum_partials(const double *partials, unsigned G, double (&tot)[NV],
                                              double *sh) {
+  constexpr int U = 8;
 #pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    double s = 0.0;
-    for (unsigned i = threadIdx.x; i < G; i += NT) s += partials[(size_t)k * G + i];
-    tot[k] = s;
+  for (int k = 0; k < NV; ++k) tot[k] = 0.0;
+  for (unsigned base = 0; base < G; base += NT * U) {
+    double v[NV][U];
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const unsigned i = base + u * NT + threadIdx.x;
+        v[k][u] = i < G ? partials[(size_t)k * G + i] : 0.0;
+      }
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+#pragma unroll
+      for (int u = 0; u < U; ++u) tot[k] += v[k][u];
   }
   block_sum<NT, NV>(tot, sh);
 }

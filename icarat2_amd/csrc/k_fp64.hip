@@ -25,6 +25,9 @@ constexpr int SX = 32, SY = 8, SZ = 4, NT = 256;
 constexpr int HX = SX + 2, HY = SY + 2, HZ = SZ + 2, HN = HX * HY * HZ;
 constexpr int EX = SX + 1, EY = SY + 1, EZ = SZ + 1, EN = EX * EY * EZ;
 constexpr int VEC_BLOCKS = 2048;
+// update kernel of fused batches: fewer blocks, so that every stencil block
+// reduces few partials
+constexpr int UPD_BLOCKS = 512;
 
 __device__ __forceinline__ constexpr int nb(int dx, int dy, int dz) {
   return (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
@@ -70,20 +73,88 @@ __device__ __forceinline__ void node_contract(const double *__restrict__ c_KE0, 
   }
 }
 
-template <bool PCG, int SYM>
+__device__ __forceinline__ bool first_thread() {
+  return threadIdx.x == 0 && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0;
+}
+
+// Beta step of a fused Jacobi-PCG batch, at the start of the stencil kernel of
+// in-batch iteration j >= 1: every block reduces the previous update's
+// partials (rho_j, r.r) in the same fixed order, so all blocks reach the same
+// beta_j = rho_j / rho_{j-1} and the same stop decision; the first thread
+// records them.  Returns false when the iteration must not run.
+__device__ __forceinline__ bool fused_beta(const double *__restrict__ pu, unsigned Gu, int j,
+                                           PcgScalars *S, double *sh, double &beta) {
+  double tot[2];
+  sum_partials<NT, 2>(pu, Gu, tot, sh);
+  const double rho1 = tot[0], rr = tot[1];
+  const double rho0 = S->rho_slot[(j - 1) & 1];
+  const double rnorm = sqrt(rr);
+  const int iter = S->iter_base + j;
+  int status = -1;
+  if (!isfinite(rnorm) || !isfinite(rho1)) status = 2;
+  else if (rnorm < S->tol * sqrt(S->bnorm2) || rnorm == 0.0) status = 0;
+  else if (iter > S->maxit) status = 1;
+  else if (rnorm < S->repl_norm) status = 3;
+  beta = rho1 / rho0;
+  if (first_thread()) {
+    S->rho = rho1;
+    S->rr = rr;
+    S->iter = iter;
+    S->beta = beta;
+    S->rho_slot[j & 1] = rho1;
+    if (status >= 0) {
+      S->status = status;
+      S->done = 1;
+    }
+  }
+  return status < 0;
+}
+
+// Alpha step of a fused batch, at the start of the update kernel of in-batch
+// iteration j: every block reduces the stencil's p.q partials.
+__device__ __forceinline__ bool fused_alpha(const double *__restrict__ ps, unsigned Gs, int j,
+                                            PcgScalars *S, double *sh, double &alpha) {
+  double tot[1];
+  sum_partials<NT, 1>(ps, Gs, tot, sh);
+  const double pq = tot[0];
+  alpha = S->rho_slot[j & 1] / pq;
+  const bool ok = pq > 0.0 && isfinite(alpha);
+  if (first_thread()) {
+    S->pq = pq;
+    S->alpha = alpha;
+    if (!ok) {
+      S->status = 2;
+      S->done = 1;
+    }
+  }
+  return ok;
+}
+
+// PCG stencil prologue: beta from the scalars, or (FUSE) from the previous
+// update's partials.  Returns false when the kernel must not run.
+template <bool PCG, bool FUSE>
+__device__ __forceinline__ bool pcg_prologue(PcgScalars *S, const double *pu, unsigned Gu, int j,
+                                             double *sh, double &beta) {
+  beta = 0.0;
+  if constexpr (PCG) {
+    if (S->done) return false;
+    if constexpr (FUSE) return fused_beta(pu, Gu, j, S, sh, beta);
+    beta = S->beta;
+  }
+  return true;
+}
+
+template <bool PCG, int SYM, bool FUSE>
 __global__ __launch_bounds__(NT) void k_stencil64(
     Lattice L, const double *__restrict__ c_KE0, KeVals kv, const double *__restrict__ young,
     const uint8_t *__restrict__ fmask, const double *__restrict__ src, const double *__restrict__ dinv,
     const double *__restrict__ p_old, double *__restrict__ p_new, double *__restrict__ y,
-    PcgScalars *S, double *partials, unsigned *counter) {
+    PcgScalars *S, double *partials, const double *__restrict__ pu, unsigned Gu, int j) {
   __shared__ double su[3][HN];
   __shared__ double se[EN];
-  __shared__ double sh[4];
-  double beta = 0.0;
-  if constexpr (PCG) {
-    if (S->done) return;
-    beta = S->beta;
-  }
+  __shared__ double sh[8];
+  double beta;
+  if (!pcg_prologue<PCG, FUSE>(S, pu, Gu, j, sh, beta)) return;
   const int x0 = blockIdx.x * SX, y0 = blockIdx.y * SY, z0 = blockIdx.z * SZ;
   const long long Np = L.Np;
   for (int t = threadIdx.x; t < HN; t += NT) {
@@ -298,20 +369,18 @@ __device__ __forceinline__ void layer_contract(const double *__restrict__ c_KE0,
   }
 }
 
-template <bool PCG, int SYM>
+template <bool PCG, int SYM, bool FUSE>
 __global__ __launch_bounds__(NT) void k_stencil64z(
     Lattice L, int nchunks, const double *__restrict__ c_KE0, KeVals kv, const double *__restrict__ young,
     const uint8_t *__restrict__ fmask, const double *__restrict__ src,
     const double *__restrict__ dinv, const double *__restrict__ p_old, double *__restrict__ p_new,
-    double *__restrict__ y, PcgScalars *S, double *partials) {
+    double *__restrict__ y, PcgScalars *S, double *partials, const double *__restrict__ pu,
+    unsigned Gu, int j) {
   __shared__ double sp[2][3][ZPN];
   __shared__ double se[2][ZLN];
-  __shared__ double sh[4];
-  double beta = 0.0;
-  if constexpr (PCG) {
-    if (S->done) return;
-    beta = S->beta;
-  }
+  __shared__ double sh[8];
+  double beta;
+  if (!pcg_prologue<PCG, FUSE>(S, pu, Gu, j, sh, beta)) return;
   // XCD-aware tile order: workgroups go to the 8 XCDs round-robin by id, so
   // each XCD is given a contiguous range of tiles (columns fastest): one XCD
   // marches neighbouring columns of one z-slab and shares their halos in its L2.
@@ -393,6 +462,33 @@ __global__ __launch_bounds__(NT) void k_pcg_update64(long long n, double *__rest
   __shared__ double sh[8];
   if (S->done) return;
   const double alpha = S->alpha;
+  double rz = 0.0, rr = 0.0;
+  const long long stride = (long long)gridDim.x * NT;
+  for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += stride) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    rz += (dinv[i] * ri) * ri;
+    rr += ri * ri;
+  }
+  double v[2] = {rz, rr};
+  store_partials<NT, 2>(v, partials, sh);
+}
+
+// Vector half of a fused batch: alpha from the stencil's partials in every
+// block, then x += alpha p, r -= alpha q and the partials of r.D^-1r, r.r.
+__global__ __launch_bounds__(NT) void k_pcg_update64f(long long n, double *__restrict__ x,
+                                                      double *__restrict__ r,
+                                                      const double *__restrict__ p,
+                                                      const double *__restrict__ q,
+                                                      const double *__restrict__ dinv,
+                                                      PcgScalars *S, double *partials,
+                                                      const double *__restrict__ ps, unsigned Gs,
+                                                      int j) {
+  __shared__ double sh[8];
+  if (S->done) return;
+  double alpha;
+  if (!fused_alpha(ps, Gs, j, S, sh, alpha)) return;
   double rz = 0.0, rr = 0.0;
   const long long stride = (long long)gridDim.x * NT;
   for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += stride) {
@@ -669,46 +765,58 @@ KeCoef make_ke_coef(const double *ke, const double *ke_dev) {
   return k;
 }
 
-template <bool PCG, int SYM>
+template <bool PCG, int SYM, bool FUSE>
 static void launch_stencil64_t(const Lattice &L, const KeCoef &ke, const double *young,
                                const uint8_t *freemask, const double *src, const double *dinv,
                                const double *p_old, double *p_new, double *y, PcgScalars *S,
-                               double *partials, hipStream_t s, unsigned *nblocks) {
+                               double *partials, const double *pu, unsigned Gu, int j,
+                               hipStream_t s, unsigned *nblocks) {
   if (ke.kernel == 0) {
     const int def = stencil_nchunks(L);
     const int nch = ke.nchunks > 0 ? std::min(ke.nchunks, def) : def;
     const dim3 g = zgrid(L, nch);
-    hipLaunchKernelGGL((k_stencil64z<PCG, SYM>), g, dim3(NT), 0, s, L, nch, ke.ke, ke.vals, young,
-                       freemask, src, dinv, p_old, p_new, y, S, partials);
+    hipLaunchKernelGGL((k_stencil64z<PCG, SYM, FUSE>), g, dim3(NT), 0, s, L, nch, ke.ke, ke.vals,
+                       young, freemask, src, dinv, p_old, p_new, y, S, partials, pu, Gu, j);
     *nblocks = g.x * g.y * g.z;
   } else {
     const dim3 g = stencil_grid(L);
-    hipLaunchKernelGGL((k_stencil64<PCG, SYM>), g, dim3(NT), 0, s, L, ke.ke, ke.vals, young,
-                       freemask, src, dinv, p_old, p_new, y, S, partials, nullptr);
+    hipLaunchKernelGGL((k_stencil64<PCG, SYM, FUSE>), g, dim3(NT), 0, s, L, ke.ke, ke.vals, young,
+                       freemask, src, dinv, p_old, p_new, y, S, partials, pu, Gu, j);
     *nblocks = g.x * g.y * g.z;
   }
   HIPCHK(hipGetLastError());
 }
 
-template <bool PCG>
+template <bool PCG, bool FUSE>
 static unsigned launch_stencil64(const Lattice &L, const KeCoef &ke, const double *young,
                                  const uint8_t *freemask, const double *src, const double *dinv,
                                  const double *p_old, double *p_new, double *y, PcgScalars *S,
-                                 double *partials, hipStream_t s) {
+                                 double *partials, const double *pu, unsigned Gu, int j,
+                                 hipStream_t s) {
   unsigned nb = 0;
   switch (ke.sym) {
-    case 0: launch_stencil64_t<PCG, 0>(L, ke, young, freemask, src, dinv, p_old, p_new, y, S, partials, s, &nb); break;
-    case 1: launch_stencil64_t<PCG, 1>(L, ke, young, freemask, src, dinv, p_old, p_new, y, S, partials, s, &nb); break;
-    case 2: launch_stencil64_t<PCG, 2>(L, ke, young, freemask, src, dinv, p_old, p_new, y, S, partials, s, &nb); break;
-    default: throw Error(kInvalidArgument, "unknown KE0 symmetry class");
+    case 0:
+      launch_stencil64_t<PCG, 0, FUSE>(L, ke, young, freemask, src, dinv, p_old, p_new, y, S,
+                                       partials, pu, Gu, j, s, &nb);
+      break;
+    case 1:
+      launch_stencil64_t<PCG, 1, FUSE>(L, ke, young, freemask, src, dinv, p_old, p_new, y, S,
+                                       partials, pu, Gu, j, s, &nb);
+      break;
+    case 2:
+      launch_stencil64_t<PCG, 2, FUSE>(L, ke, young, freemask, src, dinv, p_old, p_new, y, S,
+                                       partials, pu, Gu, j, s, &nb);
+      break;
+    default:
+      throw Error(kInvalidArgument, "unknown KE0 symmetry class");
   }
   return nb;
 }
 
 void launch_apply_k64(const Lattice &L, const KeCoef &ke, const double *young,
                       const uint8_t *freemask, const double *u, double *y, hipStream_t s) {
-  launch_stencil64<false>(L, ke, young, freemask, u, nullptr, nullptr, nullptr, y, nullptr, nullptr,
-                          s);
+  launch_stencil64<false, false>(L, ke, young, freemask, u, nullptr, nullptr, nullptr, y, nullptr,
+                                 nullptr, nullptr, 0, 0, s);
 }
 
 void launch_pcg_stencil64(const Lattice &L, const KeCoef &ke, const double *young,
@@ -716,9 +824,31 @@ void launch_pcg_stencil64(const Lattice &L, const KeCoef &ke, const double *youn
                           const double *p_old, double *p_new, double *q, PcgScalars *S,
                           double *partials, unsigned *counter, hipStream_t s, bool finalize) {
   (void)counter;
-  const unsigned nb =
-      launch_stencil64<true>(L, ke, young, freemask, r, dinv, p_old, p_new, q, S, partials, s);
+  const unsigned nb = launch_stencil64<true, false>(L, ke, young, freemask, r, dinv, p_old, p_new,
+                                                    q, S, partials, nullptr, 0, 0, s);
   if (finalize) launch_finalize(kFinPcgAlpha, 1, partials, nb, S, nullptr, nullptr, s);
+}
+
+unsigned launch_pcg_stencil64_fused(const Lattice &L, const KeCoef &ke, const double *young,
+                                    const uint8_t *freemask, const double *r, const double *dinv,
+                                    const double *p_old, double *p_new, double *q, PcgScalars *S,
+                                    double *partials_st, const double *partials_up, unsigned Gu,
+                                    int j, hipStream_t s) {
+  if (j == 0)
+    return launch_stencil64<true, false>(L, ke, young, freemask, r, dinv, p_old, p_new, q, S,
+                                         partials_st, nullptr, 0, 0, s);
+  return launch_stencil64<true, true>(L, ke, young, freemask, r, dinv, p_old, p_new, q, S,
+                                      partials_st, partials_up, Gu, j, s);
+}
+
+unsigned launch_pcg_update64_fused(long long n, double *x, double *r, const double *p,
+                                   const double *q, const double *dinv, PcgScalars *S,
+                                   double *partials_up, const double *partials_st, unsigned Gs,
+                                   int j, hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_update64f, dim3(UPD_BLOCKS), dim3(NT), 0, s, n, x, r, p, q, dinv, S,
+                     partials_up, partials_st, Gs, j);
+  HIPCHK(hipGetLastError());
+  return UPD_BLOCKS;
 }
 
 void launch_pcg_update64(long long n, double *x, double *r, const double *p, const double *q,

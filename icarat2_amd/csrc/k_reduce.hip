@@ -38,6 +38,8 @@ __global__ __launch_bounds__(NT) void k_finalize(int op, const double *__restric
       S->status = 0;
       S->done = tot[NV - 1] == 0.0 ? 1 : 0;
       S->repl_norm = S->delta > 0.0 ? S->delta * sqrt(tot[NV - 1]) : 0.0;
+      S->rho_slot[0] = tot[0];
+      S->iter_base = 0;
       break;
     case kFinPcgAlpha: {
       const double pqt = tot[0];
@@ -56,6 +58,9 @@ __global__ __launch_bounds__(NT) void k_finalize(int op, const double *__restric
       S->rho = rho1;
       S->rr = tot[NV - 1];
       S->iter += 1;
+      // the next batch starts here (fused batches read these)
+      S->rho_slot[0] = rho1;
+      S->iter_base = S->iter;
       const double rnorm = sqrt(tot[NV - 1]);
       if (!isfinite(rnorm) || !isfinite(rho1)) {
         S->status = 2;
@@ -76,6 +81,8 @@ __global__ __launch_bounds__(NT) void k_finalize(int op, const double *__restric
     case kFinReplace: {
       S->rho = tot[0];
       S->rr = tot[NV - 1];
+      S->rho_slot[0] = tot[0];
+      S->iter_base = S->iter;
       const double rnorm = sqrt(tot[NV - 1]);
       S->repl_norm = S->delta > 0.0 ? S->delta * rnorm : 0.0;
       if (!isfinite(rnorm)) {

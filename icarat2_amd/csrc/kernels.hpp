@@ -52,6 +52,22 @@ void launch_pcg_stencil64(const Lattice &L, const KeCoef &ke, const double *youn
                           const double *p_old, double *p_new, double *q, PcgScalars *S,
                           double *partials, unsigned *counter, hipStream_t s,
                           bool finalize = true);
+// Fused batches: the alpha and beta finalizes run inside the next kernel (every
+// block reduces the partials itself).  In-batch iteration j: the stencil (j >= 1
+// reduces partials_up of update j-1 over Gu blocks for beta and the stop test;
+// j == 0 takes beta from the scalars) writes partials_st and returns its block
+// count; the update reduces partials_st over Gs blocks for alpha and writes
+// partials_up (2 per block) and returns its block count.  A batch ends with
+// launch_finalize(kFinPcgBeta, 2, partials_up, Gu, ...).
+unsigned launch_pcg_stencil64_fused(const Lattice &L, const KeCoef &ke, const double *young,
+                                    const uint8_t *freemask, const double *r, const double *dinv,
+                                    const double *p_old, double *p_new, double *q, PcgScalars *S,
+                                    double *partials_st, const double *partials_up, unsigned Gu,
+                                    int j, hipStream_t s);
+unsigned launch_pcg_update64_fused(long long n, double *x, double *r, const double *p,
+                                   const double *q, const double *dinv, PcgScalars *S,
+                                   double *partials_up, const double *partials_st, unsigned Gs,
+                                   int j, hipStream_t s);
 // Vector half: x += alpha p; r -= alpha q; rho' = r.dinv.r; r.r; beta; stop test.
 void launch_pcg_update64(long long n, double *x, double *r, const double *p, const double *q,
                          const double *dinv, PcgScalars *S, double *partials, unsigned *counter,

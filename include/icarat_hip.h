@@ -314,8 +314,10 @@ int icarat_hip_batch_autotune(icarat_hip_batch *b, int32_t reps, int32_t *config
  * `reps` times back to back on the context's stream between two HIP events
  * and returns the average duration and the algorithmic work of one launch
  * (flops and compulsory HBM bytes as documented for each kernel):
- *   kernel 0: FP64 PCG stencil half (p update + K.p + p.q), 1152 flop/element
- *   kernel 1: FP64 PCG vector half (x, r update, r.z, r.r)
+ *   kernel 0: FP64 PCG stencil half as the solver runs it (beta step reduced from the
+ *             previous update's partials, p update + K.p + p.q), 1152 flop/element
+ *   kernel 1: FP64 PCG vector half as the solver runs it (alpha step reduced from the
+ *             stencil's partials, x, r update, r.z, r.r)
  *   kernel 2: density filter (isSens = false) at the set-up radius
  *   kernel 3: as 0 with KE0 read from memory (no symmetry classes)
  *   kernel 4: as 0 with the other FP64 kernel shape (see icarat_hip_set_stencil)
