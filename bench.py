@@ -39,8 +39,10 @@ MI355X_PEAK = {"fp64_tflops": 78.6, "fp32_mfma_tflops": 157.3, "bf16_mfma_tflops
                "hbm_gbs": 8000.0}
 # dominant kernel of each method: (timing-hook id, kernel, compute peak key)
 DOMINANT = {
-    "fp64": (0, "k_stencil64z<PCG> (FP64 VALU, KE0 orbit values in SGPRs)", "fp64_tflops"),
-    "learned": (0, "k_stencil64z<PCG> (FP64 VALU, KE0 orbit values in SGPRs)", "fp64_tflops"),
+    "fp64": (0, "k_stencil64z<PCG, cube, fused beta> (FP64 VALU, KE0 orbit values in SGPRs)",
+             "fp64_tflops"),
+    "learned": (0, "k_stencil64z<PCG, cube, fused beta> (FP64 VALU, KE0 orbit values in SGPRs)",
+                "fp64_tflops"),
     "fp32": (5, "k_stencil_mfma<FP32> (v_mfma_f32_16x16x4_f32, FP64 vectors)", "fp32_mfma_tflops"),
     "bf16": (6, "k_stencil_mfma<BF16> (v_mfma_f32_16x16x32_bf16, FP64 vectors)", "bf16_mfma_tflops"),
 }
@@ -264,7 +266,7 @@ def main():
                      "traffic": traffic_from_profiles(kname.split("<")[0]),
                      "kernel": kname, "avg_ms": ms, "algorithmic_flops": flops,
                      "compulsory_bytes": nbytes, "hbm_gbs_at_compulsory_bytes": nbytes / t_s / 1e9},
-        "vector_kernel": {"kernel": "k_pcg_update64", "avg_ms": ms_v, "bytes": bytes_v,
+        "vector_kernel": {"kernel": "k_pcg_update64f", "avg_ms": ms_v, "bytes": bytes_v,
                           "gbs": bytes_v / (ms_v * 1e-3) / 1e9},
         "copy_kernel": {"kernel": "k_copy64", "avg_ms": ms_c, "bytes": bytes_c,
                         "gbs": bytes_c / (ms_c * 1e-3) / 1e9},

@@ -856,7 +856,7 @@ void Context::time_kernel(int kernel, int reps, double *avg_ms, double *flops, d
     h2d(d_pcg.p, h_pcg.p, 1, stream_);
     KeCoef kc = kc_;
     if (kernel == 3) kc.sym = 0;                 // KE0 from memory
-    if (kernel == 4) kc.kernel = 1 - kc.kernel;  // the other FP64 kernel shape
+    if (kernel == 4) kc.kernel = kc.kernel == 1 ? 0 : 1;  // tiles, or z-marching if tiles are set
     if (kernel == 0) {
       // the fused stencil of the solver (in-batch iteration 1): its beta step
       // reduces update partials; ones keep every stop test false
