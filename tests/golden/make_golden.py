@@ -1,0 +1,97 @@
+#!/usr/bin/env python3
+"""Writes the optimizer fixtures of tests/golden from the reference's own
+optimizer sources compiled into oracle/_ref (MMASolver.cpp, OCSolver.hpp; built
+by oracle/Makefile where the reference tree is present).  The fixtures are
+inputs and outputs only:
+  mma_toy.npz   the reference's MMA toy problem (test_opt_solver.cpp:17-73):
+                design after each of 10 updates
+  mma_m1.npz    a 500-variable, one-constraint MMA run: per update the inputs
+                (dfdx, g, dgdx) and the design after the update
+  oc_filter.npz OC on an 8x6x5 mesh whose volume callback is the cone filter
+                (r = 1.5) summed, as the SIMP drivers pass it: inputs and the
+                updated design
+Run from the repository root: python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+import oracle as O  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def toy(s):
+    # the reference's toy problem (test_opt_solver.cpp:17-73)
+    f = s[0] ** 2 + s[1] ** 2 + s[2] ** 2
+    df = 2.0 * s
+    g = np.array([(s[0] - 5) ** 2 + (s[1] - 2) ** 2 + (s[2] - 1) ** 2 - 9.0,
+                  (s[0] - 3) ** 2 + (s[1] - 4) ** 2 + (s[2] - 3) ** 2 - 9.0])
+    dg = np.array([2 * (s[0] - 5), 2 * (s[1] - 2), 2 * (s[2] - 1),
+                   2 * (s[0] - 3), 2 * (s[1] - 4), 2 * (s[2] - 3)])
+    # dgdx[i * m + j]
+    dgx = np.array([dg[0], dg[3], dg[1], dg[4], dg[2], dg[5]])
+    return f, df, g, dgx
+
+
+def main():
+    if O.ref_lib() is None:
+        sys.exit("oracle/_ref is not built: run make -C oracle where the reference tree is present")
+    # MMA toy
+    m = O.MMA(3, 2, reference=True)
+    m.set(0.5, 0.1, 0.5, 0.7, 1.2)
+    s = np.array([4.0, 3.0, 2.0])
+    traj = []
+    for _ in range(10):
+        f, df, g, dg = toy(s)
+        m.update(s, df, g, dg, np.zeros(3), np.full(3, 5.0))
+        traj.append(s.copy())
+    np.savez(os.path.join(HERE, "mma_toy.npz"), x0=np.array([4.0, 3.0, 2.0]), traj=np.array(traj),
+             params=np.array([0.5, 0.1, 0.5, 0.7, 1.2]))
+    # MMA, n = 500, m = 1
+    rng = np.random.default_rng(6)
+    n = 500
+    x0 = rng.uniform(0.1, 0.9, n)
+    x = x0.copy()
+    mm = O.MMA(n, 1, reference=True)
+    dfs, gs, dgs, xs = [], [], [], []
+    for _ in range(6):
+        df = -rng.uniform(0.1, 3.0, n)
+        dg = np.full(n, 1.0 / n)
+        gval = np.array([x.mean() - 0.4])
+        mm.update(x, df, gval, dg, np.full(n, 1e-4), np.ones(n))
+        dfs.append(df)
+        gs.append(gval[0])
+        dgs.append(dg)
+        xs.append(x.copy())
+    np.savez(os.path.join(HERE, "mma_m1.npz"), x0=x0, dfdx=np.array(dfs), g=np.array(gs),
+             dgdx=np.array(dgs), x=np.array(xs))
+    # OC with the filtered-volume callback
+    rng = np.random.default_rng(7)
+    pr = O.Problem(8, 6, 5)
+    pr.make_dof()
+    F = O.Filter(pr, 1.5)
+    n = pr.nelx
+    x = rng.uniform(0.2, 0.6, n)
+    dc = -rng.uniform(0.01, 2.0, n)
+    dv = np.ones(n)
+    g = 0.35 * n
+
+    def vol(ptr, nn, user):
+        arr = np.ctypeslib.as_array(ptr, shape=(nn,)).copy()
+        return float(F.density_filter(arr, False).sum())
+
+    cb = O.REF_VOLUME_FN(vol)
+    x2 = x.copy()
+    O.ref_lib().ref_oc_update(n, O.P(x2), O.P(dc.copy()), g, O.P(dv.copy()),
+                              O.P(np.full(n, 1e-4)), O.P(np.ones(n)), cb, None)
+    np.savez(os.path.join(HERE, "oc_filter.npz"), dims=np.array([8, 6, 5]), radius=1.5, x=x,
+             dfdx=dc, dgdx=dv, g=g, x_new=x2)
+    print("wrote", sorted(f for f in os.listdir(HERE) if f.endswith(".npz")))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,5 +1,6 @@
 """Pins the CPU oracle against the reference's own known-answer tests, closed
 forms, invariants and independent formulations (no GPU)."""
+import os
 from fractions import Fraction
 
 import numpy as np
@@ -434,3 +435,32 @@ def test_gaussian_field_recipe(oracle):
     assert np.array_equal(x, x2)
     x3, _ = oracle.gaussian_field(24, 12, 10, seed=3)
     assert not np.array_equal(x, x3)
+
+
+# ---- committed fixtures from the compiled reference optimizers (tests/golden) ----
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_mma_toy_matches_golden(oracle):
+    d = np.load(os.path.join(GOLDEN, "mma_toy.npz"))
+    _, _, t = _run_toy(oracle, reference=False)
+    assert np.max(np.abs(t - d["traj"])) < 1e-12
+
+
+def test_mma_m1_matches_golden(oracle):
+    d = np.load(os.path.join(GOLDEN, "mma_m1.npz"))
+    n = d["x0"].size
+    x = d["x0"].copy()
+    m = oracle.MMA(n, 1)
+    for k in range(d["x"].shape[0]):
+        m.update(x, d["dfdx"][k], np.array([d["g"][k]]), d["dgdx"][k], np.full(n, 1e-4), np.ones(n))
+        assert np.max(np.abs(x - d["x"][k])) < 1e-11, k
+
+
+def test_oc_matches_golden(oracle):
+    d = np.load(os.path.join(GOLDEN, "oc_filter.npz"))
+    pr = oracle.Problem(*[int(v) for v in d["dims"]])
+    pr.make_dof()
+    F = oracle.Filter(pr, float(d["radius"]))
+    x1, cnt, _ = F.oc_update(d["x"], d["dfdx"], float(d["g"]), d["dgdx"])
+    assert np.max(np.abs(x1 - d["x_new"])) < 1e-14
