@@ -34,6 +34,8 @@ METHODS = {
     "fp32": (1, 0),
     "bf16": (2, 0),
     "learned": (1, 1),
+    # geometric multigrid V-cycle as the PCG preconditioner (FP64)
+    "mg": (0, 2),
 }
 MI355X_PEAK = {"fp64_tflops": 78.6, "fp32_mfma_tflops": 157.3, "bf16_mfma_tflops": 2500.0,
                "hbm_gbs": 8000.0}
@@ -43,6 +45,7 @@ DOMINANT = {
              "fp64_tflops"),
     "learned": (0, "k_stencil64z<PCG, cube, fused beta> (FP64 VALU, KE0 orbit values in SGPRs)",
                 "fp64_tflops"),
+    "mg": (0, "k_stencil64z<PCG, cube> (FP64 VALU, KE0 orbit values in SGPRs)", "fp64_tflops"),
     "fp32": (5, "k_stencil_mfma<FP32> (v_mfma_f32_16x16x4_f32, FP64 vectors)", "fp32_mfma_tflops"),
     "bf16": (6, "k_stencil_mfma<BF16> (v_mfma_f32_16x16x32_bf16, FP64 vectors)", "bf16_mfma_tflops"),
 }

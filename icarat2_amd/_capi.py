@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(HERE, "libicarat_hip.so")
 
 OK, INVALID_ARGUMENT, HIP_ERROR, NOT_READY, OUT_OF_MEMORY, NOT_CONVERGED, NUMERICAL, INTERNAL = range(8)
 FP64, FP32, BF16 = 0, 1, 2
-PRECOND_JACOBI, PRECOND_LEARNED = 0, 1
+PRECOND_JACOBI, PRECOND_LEARNED, PRECOND_MG = 0, 1, 2
 
 dp = C.POINTER(C.c_double)
 i32p = C.POINTER(C.c_int32)
@@ -63,6 +63,12 @@ class LearnedInfo(C.Structure):
                 ("n_params", C.c_int64), ("grid", C.c_int32 * 3), ("harvested", C.c_int32)]
 
 
+class MgOpts(C.Structure):
+    _fields_ = [("max_levels", C.c_int32), ("nu_pre", C.c_int32), ("nu_post", C.c_int32),
+                ("nu_coarse", C.c_int32), ("omega", C.c_double), ("batch_iters", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
 fp = C.POINTER(C.c_float)
 i64p = C.POINTER(C.c_int64)
 
@@ -110,6 +116,9 @@ _SIGS = {
     "icarat_hip_learned_get_params": (C.c_int, [C.c_void_p, fp]),
     "icarat_hip_learned_set_params": (C.c_int, [C.c_void_p, fp]),
     "icarat_hip_learned_apply": (C.c_int, [C.c_void_p, dp, dp]),
+    "icarat_hip_mg_default_opts": (C.c_int, [C.POINTER(MgOpts)]),
+    "icarat_hip_mg_setup": (C.c_int, [C.c_void_p, C.POINTER(MgOpts)]),
+    "icarat_hip_mg_info": (C.c_int, [C.c_void_p, i32p, i32p]),
     "icarat_hip_unet_create": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_uint64,
                                          C.POINTER(C.c_void_p)]),
     "icarat_hip_unet_destroy": (C.c_int, [C.c_void_p]),
@@ -476,6 +485,20 @@ class Context:
         self._chk(self.L.icarat_hip_set_stencil(self.h, kernel, sym_max, C.byref(used)),
                   "set_stencil")
         return used.value
+
+    # --- multigrid preconditioner
+    def mg_setup(self, **kw):
+        o = MgOpts()
+        self._chk(self.L.icarat_hip_mg_default_opts(C.byref(o)), "mg_default_opts")
+        for k, v in kw.items():
+            setattr(o, k, v)
+        self._chk(self.L.icarat_hip_mg_setup(self.h, C.byref(o)), "mg_setup")
+
+    def mg_info(self):
+        n = C.c_int32()
+        dims = (C.c_int32 * 60)()
+        self._chk(self.L.icarat_hip_mg_info(self.h, C.byref(n), dims), "mg_info")
+        return [tuple(dims[3 * l:3 * l + 3]) for l in range(n.value)]
 
     def time_kernel(self, kernel, reps=20):
         ms, fl, by = C.c_double(), C.c_double(), C.c_double()

@@ -10,6 +10,7 @@
 #include "context.hpp"
 #include "learned.hpp"
 #include "mma.hpp"
+#include "multigrid.hpp"
 #include "unet.hpp"
 
 struct icarat_hip_ctx {
@@ -386,6 +387,35 @@ int icarat_hip_learned_set_params(icarat_hip_ctx *h, const float *params) {
 int icarat_hip_learned_apply(icarat_hip_ctx *h, const double *r, double *z) {
   if (!r || !z) return fail(h, ICARAT_HIP_INVALID_ARGUMENT, "null argument");
   return guard(h, [&](icarat_hip::Context &c) { c.learned_apply(r, z); });
+}
+
+// ---- multigrid preconditioner -------------------------------------------------
+int icarat_hip_mg_default_opts(icarat_mg_opts *o) {
+  if (!o) return ICARAT_HIP_INVALID_ARGUMENT;
+  *o = icarat_mg_opts{10, 2, 2, 20, 1.6, 4, 0};
+  return ICARAT_HIP_OK;
+}
+
+int icarat_hip_mg_setup(icarat_hip_ctx *h, const icarat_mg_opts *o) {
+  icarat_mg_opts d;
+  icarat_hip_mg_default_opts(&d);
+  return guard(h, [&](icarat_hip::Context &c) { c.mg_setup(o ? *o : d); });
+}
+
+int icarat_hip_mg_info(const icarat_hip_ctx *hc, int32_t *levels, int32_t *dims) {
+  icarat_hip_ctx *h = const_cast<icarat_hip_ctx *>(hc);
+  return guard(h, [&](icarat_hip::Context &c) {
+    if (!c.mg_) throw icarat_hip::Error(ICARAT_HIP_NOT_READY, "no multigrid hierarchy");
+    const int n = c.mg_->levels();
+    if (levels) *levels = n;
+    if (dims)
+      for (int l = 0; l < n; ++l) {
+        const icarat_hip::Lattice &L = c.mg_->level(l).lattice();
+        dims[3 * l] = L.nx;
+        dims[3 * l + 1] = L.ny;
+        dims[3 * l + 2] = L.nz;
+      }
+  });
 }
 
 }  // extern "C"

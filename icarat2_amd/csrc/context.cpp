@@ -9,6 +9,7 @@
 #include <functional>
 
 #include "learned.hpp"
+#include "multigrid.hpp"
 #include "mixed_kernels.hpp"
 #include "mma.hpp"
 
@@ -96,6 +97,7 @@ Context::~Context() {
       if (pcg_graph_[m][k]) (void)hipGraphDestroy(pcg_graph_[m][k]);
     }
   learned_.reset();
+  mg_.reset();
   mma_.reset();
   if (stream_) (void)hipStreamSynchronize(stream_);
   // buffers release themselves; the stream goes last
@@ -418,6 +420,12 @@ void Context::solve(const icarat_solve_opts &o, double *u_host, icarat_solve_sta
   } else if (o.precond == ICARAT_HIP_PRECOND_LEARNED) {
     if (!learned_) throw Error(kNotReady, "the learned preconditioner is not set up");
     learned_->solve(o, S);
+  } else if (o.precond == ICARAT_HIP_PRECOND_MG) {
+    if (!mg_) {
+      icarat_mg_opts d{10, 2, 2, 20, 1.6, 4, 0};
+      mg_setup(d);
+    }
+    mg_->solve(o, S);
   } else {
     throw Error(kInvalidArgument, "unknown preconditioner");
   }
@@ -808,6 +816,17 @@ void Context::drop_graphs() {
       pcg_graph_[m][p] = nullptr;
     }
   if (learned_) learned_->drop_graphs();
+  if (mg_) mg_->drop_graphs();
+}
+
+void Context::mg_setup(const icarat_mg_opts &o) {
+  require_dof();
+  if (o.max_levels < 1 || o.nu_pre < 1 || o.nu_post < 1 || o.nu_coarse < 1 || !(o.omega > 0.0) ||
+      !(o.omega < 2.0) || o.batch_iters < 1 || o.batch_iters > 64)
+    throw Error(kInvalidArgument, "invalid multigrid options");
+  sync();
+  mg_.reset();
+  mg_.reset(new Multigrid(*this, o));
 }
 
 void Context::set_stencil_tuning(int kernel, int nchunks) {

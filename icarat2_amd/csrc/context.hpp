@@ -17,6 +17,7 @@ namespace icarat_hip {
 
 class DeviceMMA;
 class LearnedPrecond;
+class Multigrid;
 
 class Context {
  public:
@@ -141,6 +142,8 @@ class Context {
 
   std::unique_ptr<DeviceMMA> mma_;
   std::unique_ptr<LearnedPrecond> learned_;
+  std::unique_ptr<Multigrid> mg_;
+  void mg_setup(const icarat_mg_opts &o);
   // called by pcg64 after its set-up and after every batch of iterations
   std::function<void()> batch_hook_;
 

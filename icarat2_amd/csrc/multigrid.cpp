@@ -1,0 +1,236 @@
+// multigrid.cpp -- see multigrid.hpp.
+#include "multigrid.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+#include "context.hpp"
+#include "learned_kernels.hpp"
+#include "mg_kernels.hpp"
+
+namespace icarat_hip {
+
+namespace {
+// mu = lambda_max(diag(KE0)^-1 KE0) by power iteration on the similar symmetric
+// matrix diag^-1/2 KE0 diag^-1/2.  Element by element, x^T K x = sum_e E_e
+// x_e^T KE0 x_e <= mu sum_e E_e x_e^T diag(KE0) x_e = mu x^T D x, so mu bounds
+// lambda_max(D^-1 K) for every density field, and Jacobi damped by omega with
+// omega mu < 2 is a convergent (energy-norm contracting) smoother.
+double jacobi_bound(const double *ke) {
+  double s[24], v[24], w[24];
+  for (int i = 0; i < 24; ++i) {
+    s[i] = 1.0 / std::sqrt(ke[i * 24 + i]);
+    v[i] = 1.0 + 0.01 * i;
+  }
+  double lam = 0.0;
+  for (int it = 0; it < 500; ++it) {
+    double nw = 0.0;
+    for (int i = 0; i < 24; ++i) {
+      double a = 0.0;
+      for (int j = 0; j < 24; ++j) a += s[i] * ke[i * 24 + j] * s[j] * v[j];
+      w[i] = a;
+      nw += a * a;
+    }
+    nw = std::sqrt(nw);
+    double nv = 0.0;
+    for (int i = 0; i < 24; ++i) nv += v[i] * v[i];
+    lam = nw / std::sqrt(nv);
+    for (int i = 0; i < 24; ++i) v[i] = w[i] / nw;
+  }
+  return lam;
+}
+}  // namespace
+
+Multigrid::Multigrid(Context &fine, const icarat_mg_opts &o) : f_(fine), o_(o) {
+  f_.ensure_device();
+  const GridSpec g0 = f_.problem().spec();
+  GridSpec g = g0;
+  // coarsen while every dimension halves exactly and the grid keeps some size
+  while ((int)lv_.size() + 1 < std::max(1, o_.max_levels) && g.nx % 2 == 0 && g.ny % 2 == 0 &&
+         g.nz % 2 == 0 && (long long)(g.nx / 2) * (g.ny / 2) * (g.nz / 2) >= 8) {
+    g.nx /= 2;
+    g.ny /= 2;
+    g.nz /= 2;
+    std::unique_ptr<Context> c(new Context(g));
+    for (const auto &b : f_.problem().dirichlet_boxes()) c->problem().add_dirichlet(b.flag, b.val, b.range);
+    c->make_dof();
+    c->ensure_device();
+    lv_.push_back(std::move(c));
+  }
+  for (int l = 0; l < levels(); ++l)
+    omega_.push_back(o_.omega / jacobi_bound(level(l).problem().ke0()));
+  z_.alloc(f_.n3_);
+  y0_.alloc(f_.n3_);
+  r0_.alloc(f_.n3_);
+  ones_.alloc(f_.n3_);
+  hipStream_t s = f_.stream();
+  HIPCHK(hipMemsetAsync(z_.p, 0, z_.bytes(), s));
+  HIPCHK(hipMemsetAsync(y0_.p, 0, y0_.bytes(), s));
+  HIPCHK(hipMemsetAsync(r0_.p, 0, r0_.bytes(), s));
+  {
+    // 1 at free DOFs: the PCG stencil then forms p = z + beta p_old
+    const Lattice &L = f_.lattice();
+    std::vector<double> ones(f_.n3_, 0.0);
+    const auto &fm = f_.problem().free_mask();
+    for (long long n = 0; n < L.numnp; ++n)
+      for (int c = 0; c < 3; ++c) ones[c * L.Np + n] = ((fm[n] >> c) & 1u) ? 1.0 : 0.0;
+    HIPCHK(hipMemcpyAsync(ones_.p, ones.data(), ones_.bytes(), hipMemcpyHostToDevice, s));
+    f_.sync();
+  }
+}
+
+Multigrid::~Multigrid() { drop_graphs(); }
+
+void Multigrid::drop_graphs() {
+  for (int p = 0; p < 2; ++p) {
+    if (exec_[p]) (void)hipGraphExecDestroy(exec_[p]);
+    if (graph_[p]) (void)hipGraphDestroy(graph_[p]);
+    exec_[p] = nullptr;
+    graph_[p] = nullptr;
+  }
+}
+
+Context &Multigrid::level(int l) { return l == 0 ? f_ : *lv_[l - 1]; }
+
+void Multigrid::refresh() {
+  hipStream_t s = f_.stream();
+  for (int l = 1; l < levels(); ++l) {
+    Context &fc = level(l - 1), &cc = level(l);
+    launch_mg_young(fc.lattice(), cc.lattice(), fc.d_young, cc.d_young, s);
+    launch_diag_inv(cc.lattice(), cc.d_ke, cc.d_young, cc.d_free, cc.d_dinv, nullptr, s);
+  }
+}
+
+// x = V-cycle(b) on level l; every launch on the fine context's stream
+void Multigrid::cycle(int l, const double *b, double *x) {
+  hipStream_t s = f_.stream();
+  Context &c = level(l);
+  const Lattice &L = c.lattice();
+  const long long n = c.n3_;
+  double *y = l == 0 ? y0_.p : c.d_y.p;
+  double *r = l == 0 ? r0_.p : c.d_r.p;
+  const double w = omega_[l];
+  auto sweeps = [&](int k, bool from_zero) {
+    for (int i = 0; i < k; ++i) {
+      if (from_zero && i == 0) {
+        launch_mg_jacobi(n, w, c.d_dinv, b, nullptr, x, s);
+      } else {
+        launch_apply_k64(L, c.kc_, c.d_young, c.d_free, x, y, s);
+        launch_mg_jacobi(n, w, c.d_dinv, b, y, x, s);
+      }
+    }
+  };
+  if (l == levels() - 1) {
+    sweeps(std::max(1, o_.nu_coarse), true);
+    return;
+  }
+  sweeps(std::max(1, o_.nu_pre), true);
+  launch_apply_k64(L, c.kc_, c.d_young, c.d_free, x, y, s);
+  launch_mg_residual(n, b, y, r, s);
+  Context &cc = level(l + 1);
+  launch_mg_restrict(L, cc.lattice(), r, cc.d_free, cc.d_b, s);
+  cycle(l + 1, cc.d_b, cc.d_x);
+  launch_mg_prolong_add(L, cc.lattice(), cc.d_x, c.d_free, x, s);
+  sweeps(std::max(1, o_.nu_post), false);
+}
+
+void Multigrid::vcycle(const double *r, double *z) { cycle(0, r, z); }
+
+void Multigrid::enqueue_batch(int par) {
+  Context &c = f_;
+  hipStream_t s = c.stream();
+  const Lattice &L = c.lattice();
+  for (int j = 0; j < std::max(1, o_.batch_iters); ++j) {
+    double *p_old = c.d_p[(j + par) & 1], *p_new = c.d_p[(j + par + 1) & 1];
+    // p = z + beta p_old, q = K p, alpha
+    launch_pcg_stencil64(L, c.kc_, c.d_young, c.d_free, z_, ones_, p_old, p_new, c.d_q, c.d_pcg,
+                         c.d_partials, c.d_counter, s);
+    launch_fpcg_update(c.n3_, c.d_x, c.d_r, p_new, c.d_q, c.d_pcg, c.d_partials, s);
+    cycle(0, c.d_r, z_);
+    launch_zdot_beta(c.n3_, z_, c.d_r, c.d_q, c.d_pcg, c.d_partials, s);
+  }
+}
+
+void Multigrid::build_graph() {
+  if (!f_.use_graphs_ || exec_[0]) return;
+  hipStream_t s = f_.stream();
+  for (int par = 0; par < 2; ++par) {
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      enqueue_batch(par);
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(s, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HIPCHK(hipStreamEndCapture(s, &graph_[par]));
+    HIPCHK(hipGraphInstantiate(&exec_[par], graph_[par], nullptr, nullptr, 0));
+  }
+}
+
+void Multigrid::solve(const icarat_solve_opts &o, icarat_solve_stats &st) {
+  if (o.precision != ICARAT_HIP_FP64)
+    throw Error(kInvalidArgument, "the multigrid preconditioner runs the FP64 contraction");
+  Context &c = f_;
+  hipStream_t s = c.stream();
+  const Lattice &L = c.lattice();
+  refresh();
+  build_graph();
+  PcgScalars *h = c.h_pcg.p;
+  PcgScalars s0{};
+  s0.tol = o.tol;
+  s0.maxit = o.maxit;
+  *h = s0;
+  HIPCHK(hipMemcpyAsync(c.d_pcg.p, h, sizeof(PcgScalars), hipMemcpyHostToDevice, s));
+  launch_pcg_init64(c.n3_, c.d_b, c.d_x, c.d_r, c.d_p[0], c.d_dinv, c.d_pcg, c.d_partials,
+                    c.d_counter, s);
+  HIPCHK(hipMemcpyAsync(h, c.d_pcg.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
+  c.sync();
+  // z = M(r), rho = z.r, beta = 0
+  auto restart = [&]() {
+    h->restart = 1;
+    h->alpha = 0.0;
+    HIPCHK(hipMemcpyAsync(c.d_pcg.p, h, sizeof(PcgScalars), hipMemcpyHostToDevice, s));
+    cycle(0, c.d_r, z_);
+    launch_zdot_beta(c.n3_, z_, c.d_r, c.d_q, c.d_pcg, c.d_partials, s);
+    HIPCHK(hipMemcpyAsync(h, c.d_pcg.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
+    c.sync();
+  };
+  if (!h->done) restart();
+  const double bnorm = std::sqrt(h->bnorm2);
+  const int batch = std::max(1, o_.batch_iters);
+  int repl = 0;
+  while (!h->done || h->status == 0) {
+    if (!h->done) {
+      const long long max_batches = (long long)(o.maxit - h->iter) / batch + 3;
+      for (long long b = 0; b < max_batches && !h->done; ++b) {
+        if (c.use_graphs_)
+          HIPCHK(hipGraphLaunch(exec_[h->iter & 1], s));
+        else
+          enqueue_batch(h->iter & 1);
+        HIPCHK(hipMemcpyAsync(h, c.d_pcg.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
+        c.sync();
+      }
+      if (!h->done) throw Error(kInternal, "multigrid PCG did not reach its stop test");
+    }
+    if (h->status != 0) break;
+    // verified stop: the FP64 true residual replaces the recursive one
+    launch_apply_k64(L, c.kc_, c.d_young, c.d_free, c.d_x, c.d_y, s);
+    launch_replace64(c.n3_, c.d_b, c.d_y, c.d_r, c.d_dinv, c.d_pcg, c.d_partials, s);
+    HIPCHK(hipMemcpyAsync(h, c.d_pcg.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
+    c.sync();
+    if (h->done) break;
+    if (++repl > 50) break;
+    restart();
+  }
+  st.fallback = 0;
+  st.refine_steps = repl;
+  st.iters = h->iter;
+  if (h->done && (h->status == 2 || h->status == 4))
+    throw Error(kNumerical, "multigrid PCG breakdown (the V-cycle is not positive definite here)");
+  st.rec_relres = bnorm > 0.0 ? std::sqrt(h->rr) / bnorm : 0.0;
+  st.status = (h->done && h->status == 0) || bnorm == 0.0 ? ICARAT_HIP_OK : ICARAT_HIP_NOT_CONVERGED;
+}
+
+}  // namespace icarat_hip

@@ -86,6 +86,15 @@ double Problem::coord(long long n, int d) const {
 
 void Problem::add_dirichlet(const int flag[6], const double val[6], const double range[6]) {
   if (numeq_ >= 0) throw Error(kNotReady, "Dirichlet conditions must precede make_dof");
+  {
+    DirichletBox b;
+    for (int i = 0; i < 6; ++i) {
+      b.flag[i] = flag[i];
+      b.val[i] = val[i];
+      b.range[i] = range[i];
+    }
+    boxes_.push_back(b);
+  }
   const double tol = 1.0e-7;
   const double lo[3] = {range[0] - tol, range[2] - tol, range[4] - tol};
   const double hi[3] = {range[1] + tol, range[3] + tol, range[5] + tol};

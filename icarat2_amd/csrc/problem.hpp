@@ -31,6 +31,13 @@ class Problem {
   // Dirichlet box with tolerance 1e-7 (reference
   // io/boundary_condition_icarat.hpp:72-138).
   void add_dirichlet(const int flag[6], const double val[6], const double range[6]);
+  // the Dirichlet boxes as given (flag[6], val[6], range[6] each), for replaying
+  // them on a coarser grid of the same box
+  struct DirichletBox {
+    int flag[6];
+    double val[6], range[6];
+  };
+  const std::vector<DirichletBox> &dirichlet_boxes() const { return boxes_; }
   // Reference numbering: free DOFs first in node order, then the constrained
   // ones (reference io/boundary_condition_base.hpp:35-95).  Returns numeq.
   int make_dof();
@@ -66,6 +73,7 @@ class Problem {
  private:
   GridSpec g_;
   Lattice L_;
+  std::vector<DirichletBox> boxes_;
   std::vector<uint8_t> dirflag_;  // per node: bit c = constrained component c
   std::vector<uint8_t> hasdir_;
   std::vector<double> dirval_;    // numnp*3

@@ -44,7 +44,8 @@ enum {
 /* Preconditioner of the (inner) Krylov solve. */
 enum {
   ICARAT_HIP_PRECOND_JACOBI = 0,   /* reference "Scaling_CG" (analysis/solver/CG.cpp:56-100) */
-  ICARAT_HIP_PRECOND_LEARNED = 1   /* U-Net correction in flexible PCG, Jacobi fallback */
+  ICARAT_HIP_PRECOND_LEARNED = 1,  /* U-Net correction in flexible PCG, Jacobi fallback */
+  ICARAT_HIP_PRECOND_MG = 2        /* geometric multigrid V-cycle (see icarat_hip_mg_setup) */
 };
 
 typedef struct icarat_hip_ctx icarat_hip_ctx;
@@ -308,6 +309,32 @@ int icarat_hip_batch_simp_run(icarat_hip_batch *b, const icarat_simp_opts *opts,
  * the solver state.  *config = the chosen configuration, ms[4] (may be NULL) the
  * time of one launch per configuration. */
 int icarat_hip_batch_autotune(icarat_hip_batch *b, int32_t reps, int32_t *config, double *ms);
+
+/* ---- multigrid preconditioner --------------------------------------------------
+ * No reference counterpart (the reference solves with Jacobi-PCG): a hierarchy of
+ * 2x coarser voxel grids of the same box (coarse Young's moduli = mean of the 8
+ * children; the Dirichlet boxes replayed, so coarse constraints sit at the same
+ * coordinates; coarsening stops at an odd dimension), damped Jacobi smoothing
+ * with each level's element stiffness, trilinear transfers, and a symmetric
+ * V-cycle as the preconditioner of PCG.  The FP64 verified stop is unchanged.
+ * A solve with precond = ICARAT_HIP_PRECOND_MG sets the hierarchy up with the
+ * defaults when icarat_hip_mg_setup was not called. */
+typedef struct icarat_mg_opts {
+  int32_t max_levels;      /* levels including the fine grid */
+  int32_t nu_pre, nu_post; /* damped Jacobi sweeps before / after the coarse correction */
+  int32_t nu_coarse;       /* sweeps on the coarsest grid */
+  double omega;            /* Jacobi damping relative to 1 / mu, mu = lambda_max(diag(KE0)^-1
+                              KE0) >= lambda_max(D^-1 K) for any densities; in (0, 2) */
+  int32_t batch_iters;     /* PCG iterations per graph launch */
+  int32_t reserved;
+} icarat_mg_opts;
+/* Fills *o with the defaults (10, 2, 2, 20, 1.6, 4). */
+int icarat_hip_mg_default_opts(icarat_mg_opts *o);
+/* (Re)builds the hierarchy of the context (o = NULL: defaults).  Needs make_dof. */
+int icarat_hip_mg_setup(icarat_hip_ctx *ctx, const icarat_mg_opts *o);
+/* levels of the hierarchy (1 = no coarsening possible); dims (may be NULL) gets
+ * nx, ny, nz of every level, 3 * levels entries */
+int icarat_hip_mg_info(const icarat_hip_ctx *ctx, int32_t *levels, int32_t *dims);
 
 /* ---- measurement ------------------------------------------------------------ */
 /* No reference counterpart: a timing hook for benchmarks.  Launches one kernel
