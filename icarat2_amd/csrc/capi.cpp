@@ -11,6 +11,7 @@
 #include "learned.hpp"
 #include "mma.hpp"
 #include "multigrid.hpp"
+#include "multigrid.hpp"
 #include "unet.hpp"
 
 struct icarat_hip_ctx {
@@ -392,7 +393,7 @@ int icarat_hip_learned_apply(icarat_hip_ctx *h, const double *r, double *z) {
 // ---- multigrid preconditioner -------------------------------------------------
 int icarat_hip_mg_default_opts(icarat_mg_opts *o) {
   if (!o) return ICARAT_HIP_INVALID_ARGUMENT;
-  *o = icarat_mg_opts{10, 2, 2, 20, 1.6, 4, 0};
+  *o = icarat_hip::mg_default_opts();
   return ICARAT_HIP_OK;
 }
 

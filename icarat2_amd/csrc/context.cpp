@@ -421,10 +421,7 @@ void Context::solve(const icarat_solve_opts &o, double *u_host, icarat_solve_sta
     if (!learned_) throw Error(kNotReady, "the learned preconditioner is not set up");
     learned_->solve(o, S);
   } else if (o.precond == ICARAT_HIP_PRECOND_MG) {
-    if (!mg_) {
-      icarat_mg_opts d{10, 2, 2, 20, 1.6, 4, 0};
-      mg_setup(d);
-    }
+    if (!mg_) mg_setup(mg_default_opts());
     mg_->solve(o, S);
   } else {
     throw Error(kInvalidArgument, "unknown preconditioner");
@@ -822,7 +819,8 @@ void Context::drop_graphs() {
 void Context::mg_setup(const icarat_mg_opts &o) {
   require_dof();
   if (o.max_levels < 1 || o.nu_pre < 1 || o.nu_post < 1 || o.nu_coarse < 1 || !(o.omega > 0.0) ||
-      !(o.omega < 2.0) || o.batch_iters < 1 || o.batch_iters > 64)
+      !(o.omega < 2.0) || o.batch_iters < 1 || o.batch_iters > 64 || o.dense_dofs < 0 ||
+      o.direct_nodes < 0)
     throw Error(kInvalidArgument, "invalid multigrid options");
   sync();
   mg_.reset();

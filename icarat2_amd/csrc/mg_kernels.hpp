@@ -4,12 +4,16 @@
 
 #include "device_common.hpp"
 #include "hip_util.hpp"
+#include "kernels.hpp"
 
 namespace icarat_hip {
 
-// bc = R rf on the coarse lattice (transpose of trilinear prolongation), masked
-void launch_mg_restrict(const Lattice &Lf, const Lattice &Lc, const double *rf,
-                        const uint8_t *fmask_c, double *bc, hipStream_t s);
+// bc = R rf on the coarse lattice (transpose of trilinear prolongation), masked;
+// with yf: bc = R (rf - yf) (the fine residual formed on the fly); with dinv_c:
+// also xc = omega dinv_c bc (the first Jacobi sweep from zero)
+void launch_mg_restrict(const Lattice &Lf, const Lattice &Lc, const double *rf, const double *yf,
+                        const uint8_t *fmask_c, double *bc, const double *dinv_c, double omega,
+                        double *xc, hipStream_t s);
 // xf += P xc (trilinear interpolation), masked
 void launch_mg_prolong_add(const Lattice &Lf, const Lattice &Lc, const double *xc,
                            const uint8_t *fmask_f, double *xf, hipStream_t s);
@@ -20,5 +24,29 @@ void launch_mg_young(const Lattice &Lf, const Lattice &Lc, const double *yf, dou
 void launch_mg_jacobi(long long n, double omega, const double *dinv, const double *b,
                       const double *y, double *x, hipStream_t s);
 void launch_mg_residual(long long n, const double *b, const double *y, double *r, hipStream_t s);
+
+// One thread per node, K.x gathered from global memory (for coarse lattices,
+// where a z-marching launch is latency-bound), with the consumer fused in:
+enum MgDirectMode { MG_APPLY = 0, MG_JACOBI = 1, MG_RESIDUAL = 2 };
+//   MG_APPLY:    out = K x
+//   MG_JACOBI:   out = x + omega dinv (b - K x)   (out may not alias x)
+//   MG_RESIDUAL: out = b - K x
+// every output masked by the free DOFs
+void launch_mg_direct(int mode, const Lattice &L, const KeCoef &kc, const double *young,
+                      const uint8_t *fmask, const double *x, const double *b, const double *dinv,
+                      double omega, double *out, hipStream_t s);
+
+// Exact solve on a small lattice through the dense inverse of its K.  Dense
+// index d = c * numnp + n for lattice DOF c * Np + n; constrained DOFs carry
+// identity rows and columns.
+constexpr int MG_DENSE_MAX = 4096;  // largest 3 numnp solved densely
+// A = K of the lattice (n = 3 numnp, n x n, symmetric)
+void launch_mg_dense_assemble(const Lattice &L, const double *ke, const double *young,
+                              const uint8_t *fmask, double *A, hipStream_t s);
+// copies the lower triangle (column-major) over the upper one
+void launch_mg_dense_symmetrize(int n, double *A, hipStream_t s);
+// x = A b on the lattice (A symmetric, full)
+void launch_mg_dense_solve(const Lattice &L, const double *A, const double *b, double *x,
+                           hipStream_t s);
 
 }  // namespace icarat_hip

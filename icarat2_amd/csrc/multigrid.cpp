@@ -1,8 +1,12 @@
 // multigrid.cpp -- see multigrid.hpp.
 #include "multigrid.hpp"
 
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
 #include <algorithm>
 #include <cmath>
+#include <string>
 
 #include "context.hpp"
 #include "learned_kernels.hpp"
@@ -45,9 +49,13 @@ Multigrid::Multigrid(Context &fine, const icarat_mg_opts &o) : f_(fine), o_(o) {
   f_.ensure_device();
   const GridSpec g0 = f_.problem().spec();
   GridSpec g = g0;
-  // coarsen while every dimension halves exactly and the grid keeps some size
+  const long long dense_max = std::min<long long>(std::max(0, o_.dense_dofs), MG_DENSE_MAX);
+  auto dofs = [](const GridSpec &q) { return 3LL * (q.nx + 1) * (q.ny + 1) * (q.nz + 1); };
+  // coarsen while every dimension halves exactly and the grid keeps some size,
+  // up to the first grid small enough for the exact solve
   while ((int)lv_.size() + 1 < std::max(1, o_.max_levels) && g.nx % 2 == 0 && g.ny % 2 == 0 &&
-         g.nz % 2 == 0 && (long long)(g.nx / 2) * (g.ny / 2) * (g.nz / 2) >= 8) {
+         g.nz % 2 == 0 && (long long)(g.nx / 2) * (g.ny / 2) * (g.nz / 2) >= 8 &&
+         !(lv_.size() > 0 && dofs(g) <= dense_max)) {
     g.nx /= 2;
     g.ny /= 2;
     g.nz /= 2;
@@ -59,6 +67,20 @@ Multigrid::Multigrid(Context &fine, const icarat_mg_opts &o) : f_(fine), o_(o) {
   }
   for (int l = 0; l < levels(); ++l)
     omega_.push_back(o_.omega / jacobi_bound(level(l).problem().ke0()));
+  // smoothing kernels per level: the fused one-thread-per-node sweep where the
+  // z-marching launch is latency-bound; the smallest coarsest grids exactly
+  for (int l = 0; l < levels(); ++l)
+    direct_.push_back(level(l).lattice().numnp <= (long long)std::max(0, o_.direct_nodes) ? 1 : 0);
+  {
+    const long long nd = 3 * level(levels() - 1).lattice().numnp;
+    dense_ = nd <= dense_max;
+    if (dense_) {
+      ainv_.alloc(nd * nd);
+      info_.alloc(1);
+      if (rocblas_create_handle(&blas_) != rocblas_status_success)
+        throw Error(kInternal, "rocblas_create_handle failed");
+    }
+  }
   z_.alloc(f_.n3_);
   y0_.alloc(f_.n3_);
   r0_.alloc(f_.n3_);
@@ -67,6 +89,11 @@ Multigrid::Multigrid(Context &fine, const icarat_mg_opts &o) : f_(fine), o_(o) {
   HIPCHK(hipMemsetAsync(z_.p, 0, z_.bytes(), s));
   HIPCHK(hipMemsetAsync(y0_.p, 0, y0_.bytes(), s));
   HIPCHK(hipMemsetAsync(r0_.p, 0, r0_.bytes(), s));
+  for (int l = 1; l < levels(); ++l) {
+    Context &c = level(l);
+    for (DevBuf<double> *v : {&c.d_b, &c.d_x, &c.d_y, &c.d_r})
+      HIPCHK(hipMemsetAsync(v->p, 0, v->bytes(), s));
+  }
   {
     // 1 at free DOFs: the PCG stencil then forms p = z + beta p_old
     const Lattice &L = f_.lattice();
@@ -77,9 +104,13 @@ Multigrid::Multigrid(Context &fine, const icarat_mg_opts &o) : f_(fine), o_(o) {
     HIPCHK(hipMemcpyAsync(ones_.p, ones.data(), ones_.bytes(), hipMemcpyHostToDevice, s));
     f_.sync();
   }
+  zres_ = cycle(0, nullptr, false);
 }
 
-Multigrid::~Multigrid() { drop_graphs(); }
+Multigrid::~Multigrid() {
+  drop_graphs();
+  if (blas_) (void)rocblas_destroy_handle(blas_);
+}
 
 void Multigrid::drop_graphs() {
   for (int p = 0; p < 2; ++p) {
@@ -99,42 +130,83 @@ void Multigrid::refresh() {
     launch_mg_young(fc.lattice(), cc.lattice(), fc.d_young, cc.d_young, s);
     launch_diag_inv(cc.lattice(), cc.d_ke, cc.d_young, cc.d_free, cc.d_dinv, nullptr, s);
   }
+  if (dense_) {
+    // A = K of the coarsest grid, then A <- A^-1 by Cholesky (K is SPD on the
+    // free DOFs; constrained DOFs carry identity rows)
+    Context &cc = level(levels() - 1);
+    const int nd = (int)(3 * cc.lattice().numnp);
+    launch_mg_dense_assemble(cc.lattice(), cc.d_ke, cc.d_young, cc.d_free, ainv_, s);
+    auto chk = [](rocblas_status st, const char *what) {
+      if (st != rocblas_status_success)
+        throw Error(kInternal, std::string(what) + " failed: " + rocblas_status_to_string(st));
+    };
+    chk(rocblas_set_stream(blas_, s), "rocblas_set_stream");
+    chk(rocsolver_dpotrf(blas_, rocblas_fill_lower, nd, ainv_.p, nd, info_.p), "rocsolver_dpotrf");
+    int info = 0;
+    HIPCHK(hipMemcpyAsync(&info, info_.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (info != 0)
+      throw Error(kNumerical, "coarsest multigrid matrix is not positive definite (Cholesky pivot " +
+                                  std::to_string(info) + ")");
+    chk(rocsolver_dpotri(blas_, rocblas_fill_lower, nd, ainv_.p, nd, info_.p), "rocsolver_dpotri");
+    launch_mg_dense_symmetrize(nd, ainv_, s);
+  }
 }
 
-// x = V-cycle(b) on level l; every launch on the fine context's stream
-void Multigrid::cycle(int l, const double *b, double *x) {
+// V-cycle on level l; every launch on the fine context's stream.  Smoothing
+// ping-pongs between the level's x and y buffers where the fused sweep runs
+// (the z-marching path smooths x in place, with y holding K.x).
+double *Multigrid::cycle(int l, const double *b, bool emit) {
   hipStream_t s = f_.stream();
   Context &c = level(l);
   const Lattice &L = c.lattice();
   const long long n = c.n3_;
+  double *x = l == 0 ? z_.p : c.d_x.p;
   double *y = l == 0 ? y0_.p : c.d_y.p;
   double *r = l == 0 ? r0_.p : c.d_r.p;
   const double w = omega_[l];
-  auto sweeps = [&](int k, bool from_zero) {
-    for (int i = 0; i < k; ++i) {
-      if (from_zero && i == 0) {
-        launch_mg_jacobi(n, w, c.d_dinv, b, nullptr, x, s);
-      } else {
-        launch_apply_k64(L, c.kc_, c.d_young, c.d_free, x, y, s);
-        launch_mg_jacobi(n, w, c.d_dinv, b, y, x, s);
-      }
+  if (l == levels() - 1 && dense_) {
+    if (emit) launch_mg_dense_solve(L, ainv_, b, x, s);
+    return x;
+  }
+  double *cur = x, *oth = y;
+  auto sweep = [&]() {
+    if (direct_[l]) {
+      if (emit) launch_mg_direct(MG_JACOBI, L, c.kc_, c.d_young, c.d_free, cur, b, c.d_dinv, w, oth, s);
+      std::swap(cur, oth);
+    } else if (emit) {
+      launch_apply_k64(L, c.kc_, c.d_young, c.d_free, cur, oth, s);
+      launch_mg_jacobi(n, w, c.d_dinv, b, oth, cur, s);
     }
   };
+  // the first sweep from zero (levels >= 1: written by the restriction)
+  if (l == 0 && emit) launch_mg_jacobi(n, w, c.d_dinv, b, nullptr, x, s);
   if (l == levels() - 1) {
-    sweeps(std::max(1, o_.nu_coarse), true);
-    return;
+    for (int i = 1; i < std::max(1, o_.nu_coarse); ++i) sweep();
+    return cur;
   }
-  sweeps(std::max(1, o_.nu_pre), true);
-  launch_apply_k64(L, c.kc_, c.d_young, c.d_free, x, y, s);
-  launch_mg_residual(n, b, y, r, s);
+  for (int i = 1; i < std::max(1, o_.nu_pre); ++i) sweep();
   Context &cc = level(l + 1);
-  launch_mg_restrict(L, cc.lattice(), r, cc.d_free, cc.d_b, s);
-  cycle(l + 1, cc.d_b, cc.d_x);
-  launch_mg_prolong_add(L, cc.lattice(), cc.d_x, c.d_free, x, s);
-  sweeps(std::max(1, o_.nu_post), false);
+  if (emit) {
+    // the coarse level's first sweep from zero comes with its right-hand side
+    // (none before an exact coarse solve)
+    const bool exact = l + 1 == levels() - 1 && dense_;
+    const double *dinv_c = exact ? nullptr : cc.d_dinv.p;
+    if (direct_[l]) {
+      launch_mg_direct(MG_RESIDUAL, L, c.kc_, c.d_young, c.d_free, cur, b, nullptr, 0.0, r, s);
+      launch_mg_restrict(L, cc.lattice(), r, nullptr, cc.d_free, cc.d_b, dinv_c, omega_[l + 1],
+                         cc.d_x, s);
+    } else {
+      // residual b - K x formed inside the restriction
+      launch_apply_k64(L, c.kc_, c.d_young, c.d_free, cur, oth, s);
+      launch_mg_restrict(L, cc.lattice(), b, oth, cc.d_free, cc.d_b, dinv_c, omega_[l + 1], cc.d_x, s);
+    }
+  }
+  const double *xc = cycle(l + 1, cc.d_b, emit);
+  if (emit) launch_mg_prolong_add(L, cc.lattice(), xc, c.d_free, cur, s);
+  for (int i = 0; i < std::max(0, o_.nu_post); ++i) sweep();
+  return cur;
 }
-
-void Multigrid::vcycle(const double *r, double *z) { cycle(0, r, z); }
 
 void Multigrid::enqueue_batch(int par) {
   Context &c = f_;
@@ -143,11 +215,11 @@ void Multigrid::enqueue_batch(int par) {
   for (int j = 0; j < std::max(1, o_.batch_iters); ++j) {
     double *p_old = c.d_p[(j + par) & 1], *p_new = c.d_p[(j + par + 1) & 1];
     // p = z + beta p_old, q = K p, alpha
-    launch_pcg_stencil64(L, c.kc_, c.d_young, c.d_free, z_, ones_, p_old, p_new, c.d_q, c.d_pcg,
+    launch_pcg_stencil64(L, c.kc_, c.d_young, c.d_free, zres_, ones_, p_old, p_new, c.d_q, c.d_pcg,
                          c.d_partials, c.d_counter, s);
     launch_fpcg_update(c.n3_, c.d_x, c.d_r, p_new, c.d_q, c.d_pcg, c.d_partials, s);
-    cycle(0, c.d_r, z_);
-    launch_zdot_beta(c.n3_, z_, c.d_r, c.d_q, c.d_pcg, c.d_partials, s);
+    cycle(0, c.d_r, true);
+    launch_zdot_beta(c.n3_, zres_, c.d_r, c.d_q, c.d_pcg, c.d_partials, s);
   }
 }
 
@@ -192,8 +264,8 @@ void Multigrid::solve(const icarat_solve_opts &o, icarat_solve_stats &st) {
     h->restart = 1;
     h->alpha = 0.0;
     HIPCHK(hipMemcpyAsync(c.d_pcg.p, h, sizeof(PcgScalars), hipMemcpyHostToDevice, s));
-    cycle(0, c.d_r, z_);
-    launch_zdot_beta(c.n3_, z_, c.d_r, c.d_q, c.d_pcg, c.d_partials, s);
+    cycle(0, c.d_r, true);
+    launch_zdot_beta(c.n3_, zres_, c.d_r, c.d_q, c.d_pcg, c.d_partials, s);
     HIPCHK(hipMemcpyAsync(h, c.d_pcg.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
     c.sync();
   };

@@ -11,10 +11,27 @@
 
 #include "../../include/icarat_hip.h"
 #include "hip_util.hpp"
+#include "mg_kernels.hpp"
+
+struct _rocblas_handle;  // rocBLAS handle type (rocblas/rocblas.h)
 
 namespace icarat_hip {
 
 class Context;
+
+// the defaults of icarat_hip_mg_default_opts
+inline icarat_mg_opts mg_default_opts() {
+  icarat_mg_opts o{};
+  o.max_levels = 10;
+  o.nu_pre = 1;
+  o.nu_post = 1;
+  o.nu_coarse = 20;
+  o.omega = 1.8;
+  o.batch_iters = 4;
+  o.dense_dofs = 1024;
+  o.direct_nodes = 262144;
+  return o;
+}
 
 class Multigrid {
  public:
@@ -25,11 +42,13 @@ class Multigrid {
   Context &level(int l);
   // coarse Young's moduli and Jacobi diagonals after the fine density changed
   void refresh();
-  // z = M(r) on the fine lattice (device pointers; z may not alias r)
-  void vcycle(const double *r, double *z);
   // K u = b of the fine context (d_b -> d_x) by PCG with M = V-cycle
   void solve(const icarat_solve_opts &o, icarat_solve_stats &st);
   void drop_graphs();
+  // whether the coarsest level is solved exactly (dense inverse)
+  bool dense() const { return dense_; }
+  // whether level l smooths with the fused one-thread-per-node kernel
+  bool direct(int l) const { return direct_[l] != 0; }
 
  private:
   Context &f_;
@@ -37,9 +56,18 @@ class Multigrid {
   std::vector<std::unique_ptr<Context>> lv_;  // levels 1..L-1
   std::vector<double> omega_;                  // Jacobi damping per level
   DevBuf<double> z_, y0_, r0_, ones_;
+  std::vector<char> direct_;
+  bool dense_ = false;
+  DevBuf<double> ainv_;  // dense inverse of the coarsest K
+  DevBuf<int> info_;
+  ::_rocblas_handle *blas_ = nullptr;
+  double *zres_ = nullptr;  // buffer that holds the fine V-cycle's result
   hipGraph_t graph_[2] = {};
   hipGraphExec_t exec_[2] = {};
-  void cycle(int l, const double *b, double *x);
+  // V-cycle on level l with right-hand side b (the first sweep already in the
+  // level's x for l >= 1); returns the buffer holding the result.  emit =
+  // false only follows the buffers.
+  double *cycle(int l, const double *b, bool emit);
   void enqueue_batch(int par);
   void build_graph();
 };

@@ -322,13 +322,18 @@ int icarat_hip_batch_autotune(icarat_hip_batch *b, int32_t reps, int32_t *config
 typedef struct icarat_mg_opts {
   int32_t max_levels;      /* levels including the fine grid */
   int32_t nu_pre, nu_post; /* damped Jacobi sweeps before / after the coarse correction */
-  int32_t nu_coarse;       /* sweeps on the coarsest grid */
+  int32_t nu_coarse;       /* sweeps on the coarsest grid (unless it is solved exactly) */
   double omega;            /* Jacobi damping relative to 1 / mu, mu = lambda_max(diag(KE0)^-1
                               KE0) >= lambda_max(D^-1 K) for any densities; in (0, 2) */
   int32_t batch_iters;     /* PCG iterations per graph launch */
+  int32_t dense_dofs;      /* coarsening stops at the first grid of at most this many lattice
+                              DOFs (3 x nodes, capped at 4096), which is solved exactly by
+                              its dense inverse (0: never; the coarsest grid is smoothed) */
+  int32_t direct_nodes;    /* levels of at most this many nodes smooth with the fused
+                              one-thread-per-node sweep; larger ones with the z-marching K.u */
   int32_t reserved;
 } icarat_mg_opts;
-/* Fills *o with the defaults (10, 2, 2, 20, 1.6, 4). */
+/* Fills *o with the defaults (10, 1, 1, 20, 1.8, 4, 1024, 262144). */
 int icarat_hip_mg_default_opts(icarat_mg_opts *o);
 /* (Re)builds the hierarchy of the context (o = NULL: defaults).  Needs make_dof. */
 int icarat_hip_mg_setup(icarat_hip_ctx *ctx, const icarat_mg_opts *o);

@@ -10,11 +10,34 @@ pytestmark = pytest.mark.gpu
 
 def test_hierarchy_levels(gpu):
     c = gpu.cantilever(48, 16, 16)
+    c.mg_setup(dense_dofs=0)
+    assert c.mg_info() == [(48, 16, 16), (24, 8, 8), (12, 4, 4), (6, 2, 2)]
+    # coarsening stops at the first grid with at most dense_dofs lattice DOFs:
+    # 12x4x4 has 3 * 13 * 5 * 5 = 975
     c.mg_setup()
+    assert c.mg_info() == [(48, 16, 16), (24, 8, 8), (12, 4, 4)]
+    c.mg_setup(dense_dofs=600)
     assert c.mg_info() == [(48, 16, 16), (24, 8, 8), (12, 4, 4), (6, 2, 2)]
     c = gpu.cantilever(60, 20, 20)
     c.mg_setup(max_levels=2)
     assert c.mg_info() == [(60, 20, 20), (30, 10, 10)]
+
+
+@pytest.mark.parametrize("dims", [(8, 4, 4), (6, 4, 2)])
+def test_dense_inverse_is_exact(gpu, dims):
+    """One level solved by the dense inverse: the preconditioner is K^-1 of the
+    assembled free-DOF system, so PCG stops after one or two iterations.  This
+    pins the assembly (element scatter, constrained DOFs) to the matrix-free K."""
+    c = gpu.cantilever(*dims)
+    x, _ = gpu.gaussian_field(*dims, seed=3, smin=1.0, smax=3.0)
+    c.set_density(x)
+    c.mg_setup(max_levels=1)
+    assert c.mg_info() == [dims]
+    uj, sj = c.solve(tol=1e-12)
+    um, sm = c.solve(tol=1e-12, precond=gpu.PRECOND_MG)
+    print(f"{dims}: Jacobi {sj.iters} iterations, exact {sm.iters}")
+    assert sm.status == 0 and sm.iters <= 2 and sm.true_relres <= 1e-12
+    assert np.max(np.abs(um - uj)) < 1e-9 * np.max(np.abs(uj))
 
 
 @pytest.mark.parametrize("dims", [(48, 16, 16), (32, 24, 16)])
@@ -30,6 +53,33 @@ def test_mg_solve_agrees_with_jacobi(gpu, dims):
     assert sm.status == 0 and sm.true_relres <= 1e-9
     assert np.max(np.abs(um - uj)) < 1e-6 * np.max(np.abs(uj))
     assert sm.iters < sj.iters
+
+
+@pytest.mark.parametrize("dims", [(48, 16, 16), (60, 20, 20)])
+def test_mg_kernel_paths_agree(gpu, dims):
+    """The smoothing kernels (z-marching K.u + vector kernels, or the fused
+    one-thread-per-node sweep) run the same V-cycle: either gives the same
+    iterations (+-2, summation order only) and the same solution (1e-9 of max
+    |u|, all solves at 1e-10).  With the exact coarsest solve (dense inverse:
+    6x2x2 or 12x4x4 for (48, 16, 16)) the iterations do not grow."""
+    c = gpu.cantilever(*dims)
+    x, _ = gpu.gaussian_field(*dims, seed=5, smin=2.0, smax=6.0)
+    c.set_density(x)
+    res = {}
+    for dense, direct in ((0, 0), (0, 1 << 30), (600, 0), (600, 1 << 30), (1024, 1 << 30)):
+        c.mg_setup(dense_dofs=dense, direct_nodes=direct)
+        u, st = c.solve(tol=1e-10, precond=gpu.PRECOND_MG)
+        assert st.status == 0 and st.true_relres <= 1e-10
+        res[(dense, direct)] = (u, st.iters)
+        print(f"{dims} dense {dense} direct {direct}: {st.iters} iterations, "
+              f"{st.seconds * 1e3:.1f} ms")
+    u0, it0 = res[(0, 0)]
+    for k, (u, it) in res.items():
+        if k[0] == 0:
+            assert abs(it - it0) <= 2, k
+        else:
+            assert it <= it0 + 2, k
+        assert np.max(np.abs(u - u0)) < 1e-9 * np.max(np.abs(u0)), k
 
 
 def test_mg_simp_matches_jacobi(gpu):
