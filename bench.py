@@ -10,6 +10,12 @@ one SIMP design iteration on the GPU: density filter -> equilibrium solve to a
 sensitivity filter -> OC update.  Inputs are resident in HBM when the timed
 region starts.
 
+The default method solves with PCG preconditioned by a geometric multigrid
+V-cycle (FP64, same verified stop).  The plain method of the reference (FP64
+Jacobi-PCG) runs on the same workload in the same process; its rate, the
+speedup over it and the largest design difference after the timed
+iterations are reported beside the value.
+
 Prints one JSON line (see README of the benchmark contract)."""
 import argparse
 import json
@@ -39,33 +45,40 @@ METHODS = {
 }
 MI355X_PEAK = {"fp64_tflops": 78.6, "fp32_mfma_tflops": 157.3, "bf16_mfma_tflops": 2500.0,
                "hbm_gbs": 8000.0}
-# dominant kernel of each method: (timing-hook id, kernel, compute peak key)
+# dominant kernel of each method: (timing-hook id, kernel, compute peak key,
+# key of its PMC traffic line)
 DOMINANT = {
     "fp64": (0, "k_stencil64z<PCG, cube, fused beta> (FP64 VALU, KE0 orbit values in SGPRs)",
-             "fp64_tflops"),
+             "fp64_tflops", "k_stencil64z<true,2,true>"),
     "learned": (0, "k_stencil64z<PCG, cube, fused beta> (FP64 VALU, KE0 orbit values in SGPRs)",
-                "fp64_tflops"),
-    "mg": (0, "k_stencil64z<PCG, cube> (FP64 VALU, KE0 orbit values in SGPRs)", "fp64_tflops"),
-    "fp32": (5, "k_stencil_mfma<FP32> (v_mfma_f32_16x16x4_f32, FP64 vectors)", "fp32_mfma_tflops"),
-    "bf16": (6, "k_stencil_mfma<BF16> (v_mfma_f32_16x16x32_bf16, FP64 vectors)", "bf16_mfma_tflops"),
+                "fp64_tflops", "k_stencil64z<true,2,true>"),
+    "mg": (8, "k_stencil64z<apply, cube> (FP64 VALU, KE0 orbit values in SGPRs)", "fp64_tflops",
+           "k_stencil64z<false,2,false>"),
+    "fp32": (5, "k_stencil_mfma<FP32> (v_mfma_f32_16x16x4_f32, FP64 vectors)", "fp32_mfma_tflops",
+             "k_stencil_mfma<0>"),
+    "bf16": (6, "k_stencil_mfma<BF16> (v_mfma_f32_16x16x32_bf16, FP64 vectors)", "bf16_mfma_tflops",
+             "k_stencil_mfma<1>"),
 }
 
 
-def traffic_from_profiles(kernel_prefix):
-    """HBM bytes per launch of a kernel from the committed PMC summary
-    (profiles/pmc_traffic_c2.txt, FETCH_SIZE + WRITE_SIZE scaled by the copy
-    kernel's calibration), or None."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic_c2.txt")
-    if not os.path.exists(path):
+def traffic_from_profiles(key):
+    """HBM bytes per launch of a kernel from the committed PMC summaries
+    (profiles/pmc_traffic_*.txt: FETCH_SIZE + WRITE_SIZE scaled by the copy
+    kernel's calibration; lines `TRAFFIC_BYTES <kernel> <bytes>`), or None."""
+    d = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(d):
         return None
-    with open(path) as f:
-        for line in f:
-            parts = line.split()
-            if len(parts) >= 2 and parts[0] == "TRAFFIC_BYTES" and kernel_prefix in line:
-                try:
-                    return float(parts[-1])
-                except ValueError:
-                    return None
+    for name in sorted(os.listdir(d)):
+        if not (name.startswith("pmc_traffic_") and name.endswith(".txt")):
+            continue
+        with open(os.path.join(d, name)) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "TRAFFIC_BYTES" and parts[1] == key:
+                    try:
+                        return float(parts[2])
+                    except ValueError:
+                        return None
     return None
 
 
@@ -101,7 +114,7 @@ def run_gpu(A, cfg, method, steps, warmup, tol):
     t0 = time.perf_counter()
     x, hist, st = c.simp_run(x0, steps, **kw)
     dt = time.perf_counter() - t0
-    return c, x0, hist, st, dt
+    return c, x0, x, hist, st, dt
 
 
 def cpu_baseline(cfg, pcg_iters_per_solve, n_cg=60):
@@ -206,7 +219,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
-    ap.add_argument("--method", default="fp64", choices=sorted(METHODS))
+    ap.add_argument("--method", default="mg", choices=sorted(METHODS))
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the plain FP64 Jacobi-PCG run beside a faster method")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--reps", type=int, default=20, help="launches timed for the roofline")
@@ -221,17 +236,31 @@ def main():
         sys.exit(1)
     cfg = CONFIGS[a.config]
     if a.config == "c5":
-        if a.method != "fp64":
+        if a.method not in ("fp64", "mg"):
             print("bench.py: the batched workload runs FP64 Jacobi-PCG", file=sys.stderr)
             sys.exit(2)
         print(json.dumps(run_batch(A, a)), flush=True)
         return
-    c, x0, hist, st, dt = run_gpu(A, cfg, a.method, a.steps, a.warmup, a.tol)
+    c, x0, x, hist, st, dt = run_gpu(A, cfg, a.method, a.steps, a.warmup, a.tol)
     pcg_per_solve = float(hist[:, 2].mean())
+    # the plain method (FP64 Jacobi-PCG) on the same workload, same steps
+    plain = None
+    jacobi_iters = pcg_per_solve
+    if a.method != "fp64" and not a.no_plain:
+        import numpy as np
+        cp, _, xp, hp, sp, dtp = run_gpu(A, cfg, "fp64", a.steps, a.warmup, a.tol)
+        jacobi_iters = float(hp[:, 2].mean())
+        plain = {"method": "fp64 (Jacobi-PCG)", "value": a.steps / dtp,
+                 "solve_s_per_iter": sp.solve_s / a.steps, "pcg_iters_per_solve": jacobi_iters,
+                 "max_true_relres": sp.max_true_relres,
+                 "design_max_abs_diff": float(np.max(np.abs(x - xp))),
+                 "compliance_max_rel_diff": float(np.max(np.abs(hist[:, 0] - hp[:, 0]) /
+                                                         np.abs(hp[:, 0])))}
+        del cp
     # roofline of the dominant kernel, timed with HIP events on the library's
     # stream (the stream the kernel is launched on)
     c.set_density(c.density_filter(x0))
-    kid, kname, peak_key = DOMINANT[a.method]
+    kid, kname, peak_key, tkey = DOMINANT[a.method]
     ms, flops, nbytes = c.time_kernel(kid, a.reps)
     t_s = ms * 1e-3
     peak_f = MI355X_PEAK[peak_key]
@@ -253,7 +282,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if a.method == "fp64" else ("f32+f64" if a.method != "bf16" else "bf16+f64"),
+        "dtype": {"fp64": "f64", "mg": "f64", "fp32": "f32+f64", "learned": "f32+f64",
+                  "bf16": "bf16+f64"}[a.method],
         "data": "synthetic (seeded Gaussian-field densities, no external data)",
         "config": {"workload": f"{a.config}: {'x'.join(map(str, cfg['dims']))} voxel hexa8 "
                                f"cantilever SIMP (filter, solve to {a.tol:g}, sensitivities, OC)",
@@ -266,7 +296,7 @@ def main():
         "fallbacks": int(st.fallbacks),
         "roofline": {"bound": "mfma" if compute_bound else "hbm", "achieved": achieved,
                      "peak": peak, "unit": unit, "frac": achieved / peak,
-                     "traffic": traffic_from_profiles(kname.split("<")[0]),
+                     "traffic": traffic_from_profiles(tkey),
                      "kernel": kname, "avg_ms": ms, "algorithmic_flops": flops,
                      "compulsory_bytes": nbytes, "hbm_gbs_at_compulsory_bytes": nbytes / t_s / 1e9},
         "vector_kernel": {"kernel": "k_pcg_update64f", "avg_ms": ms_v, "bytes": bytes_v,
@@ -274,9 +304,15 @@ def main():
         "copy_kernel": {"kernel": "k_copy64", "avg_ms": ms_c, "bytes": bytes_c,
                         "gbs": bytes_c / (ms_c * 1e-3) / 1e9},
     }
+    if a.method == "mg":
+        out["config"]["mg_levels"] = c.mg_info()
+    if plain is not None:
+        out["plain_fp64"] = plain
+        out["speedup_vs_plain_fp64"] = out["value"] / plain["value"]
     if not a.no_cpu:
+        # the CPU runs the reference's Jacobi-PCG: scaled by Jacobi's iterations
         log("[bench] cpu baseline sample")
-        out["cpu_baseline"] = cpu_baseline(cfg, pcg_per_solve)
+        out["cpu_baseline"] = cpu_baseline(cfg, jacobi_iters)
     print(json.dumps(out), flush=True)
 
 

@@ -928,6 +928,12 @@ void Context::time_kernel(int kernel, int reps, double *avg_ms, double *flops, d
     launch = [&]() { launch_copy64(n3_, d_x, d_y, stream_); };
     F = 0.0;
     B = 2.0 * 8 * n3_;                          // x in, y out
+  } else if (kernel == 8) {
+    // y = K x as the multigrid smoother runs it on the fine lattice
+    d2d(d_r.p, d_f.p, n3_, stream_);
+    launch = [this]() { launch_apply_k64(L_, kc_, d_young, d_free, d_r, d_y, stream_); };
+    F = 2.0 * 24 * 24 * E;
+    B = 2.0 * 3 * N * 8 + E * 8 + N;            // x in, y out; young; mask
   } else {
     throw Error(kInvalidArgument, "unknown kernel id");
   }

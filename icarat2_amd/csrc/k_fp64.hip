@@ -776,14 +776,15 @@ void launch_apply_k64(const Lattice &L, const KeCoef &ke, const double *young,
                                  nullptr, nullptr, 0, 0, s);
 }
 
-void launch_pcg_stencil64(const Lattice &L, const KeCoef &ke, const double *young,
-                          const uint8_t *freemask, const double *r, const double *dinv,
-                          const double *p_old, double *p_new, double *q, PcgScalars *S,
-                          double *partials, unsigned *counter, hipStream_t s, bool finalize) {
+unsigned launch_pcg_stencil64(const Lattice &L, const KeCoef &ke, const double *young,
+                              const uint8_t *freemask, const double *r, const double *dinv,
+                              const double *p_old, double *p_new, double *q, PcgScalars *S,
+                              double *partials, unsigned *counter, hipStream_t s, bool finalize) {
   (void)counter;
   const unsigned nb = launch_stencil64<true, false>(L, ke, young, freemask, r, dinv, p_old, p_new,
                                                     q, S, partials, nullptr, 0, 0, s);
   if (finalize) launch_finalize(kFinPcgAlpha, 1, partials, nb, S, nullptr, nullptr, s);
+  return nb;
 }
 
 unsigned launch_pcg_stencil64_fused(const Lattice &L, const KeCoef &ke, const double *young,

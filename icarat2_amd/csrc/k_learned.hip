@@ -267,6 +267,104 @@ __global__ void k_fpcg_check(const double *__restrict__ partials, unsigned G, Pc
   }
 }
 
+// FPCG update with alpha folded in: every block sums the stencil's p.q
+// partials in the same order, so all reach the same alpha = rho / p.q and the
+// same breakdown decision; x += alpha p; r -= alpha q; r.r partials; with z0:
+// z0 = omega dinv r as well (the first sweep of a V-cycle on r).
+__global__ __launch_bounds__(NT) void k_fpcg_update_alpha(
+    long long n, double *__restrict__ x, double *__restrict__ r, const double *__restrict__ p,
+    const double *__restrict__ q, PcgScalars *S, const double *__restrict__ part_st, unsigned Gs,
+    double *part_up, const double *__restrict__ dinv, double omega, double *__restrict__ z0) {
+  __shared__ double sh[8];
+  if (S->done) return;
+  double tot[1];
+  sum_partials<NT, 1>(part_st, Gs, tot, sh);
+  const double pq = tot[0];
+  const double alpha = S->rho / pq;
+  const bool ok = pq > 0.0 && isfinite(alpha);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S->pq = pq;
+    S->alpha = alpha;
+    if (!ok) {
+      S->status = 2;
+      S->done = 1;
+    }
+  }
+  if (!ok) return;
+  double rr = 0.0;
+  const long long stride = (long long)gridDim.x * NT;
+  for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += stride) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * q[i];
+    r[i] = ri;
+    rr += ri * ri;
+    if (z0) z0[i] = omega * dinv[i] * ri;
+  }
+  double v[1] = {rr};
+  store_partials<NT, 1>(v, part_up, sh);
+}
+
+// z.r and z.q partials (no finalize)
+__global__ __launch_bounds__(NT) void k_zdot_partials(long long n, const double *__restrict__ z,
+                                                      const double *__restrict__ r,
+                                                      const double *__restrict__ q,
+                                                      double *part_z) {
+  __shared__ double sh[8];
+  double a = 0.0, b = 0.0;
+  const long long stride = (long long)gridDim.x * NT;
+  for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += stride) {
+    a += z[i] * r[i];
+    b += z[i] * q[i];
+  }
+  double v[2] = {a, b};
+  store_partials<NT, 2>(v, part_z, sh);
+}
+
+// One finalize per FPCG iteration: the stop tests of k_fpcg_check from the
+// update's r.r partials, then (still running) rho' = z.r and the Polak-Ribiere
+// beta of k_fpcg_beta from the z partials.
+__global__ void k_fpcg_check_beta(const double *__restrict__ part_up, unsigned Gu,
+                                  const double *__restrict__ part_z, unsigned Gz, PcgScalars *S) {
+  __shared__ double sh[8];
+  if (S->done) return;
+  double t1[1], t2[2];
+  sum_partials<NT, 1>(part_up, Gu, t1, sh);
+  sum_partials<NT, 2>(part_z, Gz, t2, sh);
+  if (threadIdx.x != 0) return;
+  S->rr = t1[0];
+  S->iter += 1;
+  const double rnorm = sqrt(t1[0]);
+  if (!isfinite(rnorm)) {
+    S->status = 2;
+    S->done = 1;
+    return;
+  }
+  if (rnorm < S->tol * sqrt(S->bnorm2) || rnorm == 0.0) {
+    S->status = 0;
+    S->done = 1;
+    return;
+  }
+  if (S->iter > S->maxit) {
+    S->status = 1;
+    S->done = 1;
+    return;
+  }
+  if (rnorm < S->repl_norm) {
+    S->status = 3;
+    S->done = 1;
+    return;
+  }
+  const double rho1 = t2[0], gam = t2[1];
+  if (!(rho1 > 0.0) || !isfinite(rho1) || !isfinite(gam)) {
+    S->status = 4;
+    S->done = 1;
+    return;
+  }
+  S->beta = S->restart ? 0.0 : -S->alpha * gam / S->rho;
+  S->restart = 0;
+  S->rho = rho1;
+}
+
 }  // namespace
 
 void launch_restrict_residual(const Lattice &L, const NetGrid &g, const double *r, const double *dinv,
@@ -332,6 +430,29 @@ void launch_fpcg_update(long long n, double *x, double *r, const double *p, cons
   hipLaunchKernelGGL(k_fpcg_update, dim3(VEC_BLOCKS), dim3(NT), 0, s, n, x, r, p, q, S, partials);
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(k_fpcg_check, dim3(1), dim3(NT), 0, s, partials, (unsigned)VEC_BLOCKS, S);
+  HIPCHK(hipGetLastError());
+}
+
+unsigned launch_fpcg_update_alpha(long long n, double *x, double *r, const double *p,
+                                  const double *q, PcgScalars *S, const double *part_st,
+                                  unsigned Gs, double *part_up, const double *dinv, double omega,
+                                  double *z0, hipStream_t s) {
+  hipLaunchKernelGGL(k_fpcg_update_alpha, dim3(VEC_BLOCKS), dim3(NT), 0, s, n, x, r, p, q, S,
+                     part_st, Gs, part_up, dinv, omega, z0);
+  HIPCHK(hipGetLastError());
+  return VEC_BLOCKS;
+}
+
+unsigned launch_zdot_partials(long long n, const double *z, const double *r, const double *q,
+                              double *part_z, hipStream_t s) {
+  hipLaunchKernelGGL(k_zdot_partials, dim3(VEC_BLOCKS), dim3(NT), 0, s, n, z, r, q, part_z);
+  HIPCHK(hipGetLastError());
+  return VEC_BLOCKS;
+}
+
+void launch_fpcg_check_beta(const double *part_up, unsigned Gu, const double *part_z, unsigned Gz,
+                            PcgScalars *S, hipStream_t s) {
+  hipLaunchKernelGGL(k_fpcg_check_beta, dim3(1), dim3(NT), 0, s, part_up, Gu, part_z, Gz, S);
   HIPCHK(hipGetLastError());
 }
 

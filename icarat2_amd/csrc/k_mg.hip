@@ -21,61 +21,84 @@ inline unsigned blocks_for(long long n, int nt) { return (unsigned)((n + nt - 1)
 
 __device__ __forceinline__ double w1(int d) { return d == 0 ? 1.0 : 0.5; }
 
-// sum over the 27 fine nodes around fine node (2I, 2J, 2K) of w * (rf - yf)
-// (yf == nullptr: of w * rf)
+// Restriction and prolongation run fixed, unrolled stencils: indices outside
+// the lattice are clamped and carry weight 0, so the loads of a plane are
+// independent of each other and in flight together.
+__device__ __forceinline__ void tap(int c, int dim, int d, double &w, int &q) {
+  // fine coordinate c + d in [0, dim): 1-D trilinear weight (1 centre, 1/2 side)
+  const int v = c + d;
+  const bool in = v >= 0 && v < dim;
+  w = in ? (d == 0 ? 1.0 : 0.5) : 0.0;
+  q = in ? v : 0;
+}
+
+// s = sum over the 27 fine nodes around fine node (2I, 2J, 2K) of w * (rf - yf)
+// (HAS_Y = false: of w * rf)
+template <bool HAS_Y>
 __device__ __forceinline__ void restrict_node(const Lattice &Lf, const Lattice &Lc,
                                               const double *__restrict__ rf,
                                               const double *__restrict__ yf, long long n,
                                               double (&s)[3]) {
   const int I = (int)(n % Lc.NX), J = (int)((n / Lc.NX) % Lc.NY), K = (int)(n / ((long long)Lc.NX * Lc.NY));
-  s[0] = s[1] = s[2] = 0.0;
-  for (int dz = -1; dz <= 1; ++dz) {
-    const int z = 2 * K + dz;
-    if (z < 0 || z >= Lf.NZ) continue;
-    for (int dy = -1; dy <= 1; ++dy) {
-      const int y = 2 * J + dy;
-      if (y < 0 || y >= Lf.NY) continue;
-      for (int dx = -1; dx <= 1; ++dx) {
-        const int x = 2 * I + dx;
-        if (x < 0 || x >= Lf.NX) continue;
-        const double w = w1(dx) * w1(dy) * w1(dz);
-        const long long f = x + (long long)Lf.NX * (y + (long long)Lf.NY * z);
+  double wx[3], wy[3], wz[3];
+  int qx[3], qy[3], qz[3];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) s[c] += w * (yf ? rf[c * Lf.Np + f] - yf[c * Lf.Np + f] : rf[c * Lf.Np + f]);
+  for (int d = 0; d < 3; ++d) {
+    tap(2 * I, Lf.NX, d - 1, wx[d], qx[d]);
+    tap(2 * J, Lf.NY, d - 1, wy[d], qy[d]);
+    tap(2 * K, Lf.NZ, d - 1, wz[d], qz[d]);
+  }
+  s[0] = s[1] = s[2] = 0.0;
+  // one fine plane at a time: 9 nodes x 3 components (x 2 vectors) in flight
+#pragma unroll 1
+  for (int dz = 0; dz < 3; ++dz)
+#pragma unroll
+    for (int dy = 0; dy < 3; ++dy) {
+      const long long row = (long long)Lf.NX * (qy[dy] + (long long)Lf.NY * qz[dz]);
+      const double wyz = wy[dy] * wz[dz];
+#pragma unroll
+      for (int dx = 0; dx < 3; ++dx) {
+        const long long f = row + qx[dx];
+        const double w = wx[dx] * wyz;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const double v = HAS_Y ? rf[c * Lf.Np + f] - yf[c * Lf.Np + f] : rf[c * Lf.Np + f];
+          s[c] += w * v;
+        }
       }
     }
-  }
 }
 
-// (P xc) at fine node n: the (up to 8) coarse nodes around it, weighted
+// (P xc) at fine node n: the 2x2x2 coarse nodes around it, weighted (a fine
+// coordinate 2I coincides with coarse I: weights 1 and 0; 2I+1: 1/2 and 1/2)
 __device__ __forceinline__ void prolong_node(const Lattice &Lf, const Lattice &Lc,
                                              const double *__restrict__ xc, long long n,
                                              double (&s)[3]) {
-  const int x = (int)(n % Lf.NX), y = (int)((n / Lf.NX) % Lf.NY), z = (int)(n / ((long long)Lf.NX * Lf.NY));
-  const int co[3] = {x, y, z};
+  const int co[3] = {(int)(n % Lf.NX), (int)((n / Lf.NX) % Lf.NY), (int)(n / ((long long)Lf.NX * Lf.NY))};
   const int cd[3] = {Lc.NX, Lc.NY, Lc.NZ};
-  int ci[3][2], nc[3];
-  double wc[3][2];
+  double w[3][2];
+  int q[3][2];
+#pragma unroll
   for (int d = 0; d < 3; ++d) {
-    ci[d][0] = co[d] / 2;
-    ci[d][1] = co[d] / 2 + 1;
-    if (co[d] & 1) {
-      wc[d][0] = wc[d][1] = 0.5;
-      nc[d] = ci[d][1] < cd[d] ? 2 : 1;
-    } else {
-      wc[d][0] = 1.0;
-      wc[d][1] = 0.0;
-      nc[d] = 1;
-    }
+    const int lo = co[d] >> 1;
+    const bool odd = co[d] & 1;
+    const bool hi_in = odd && lo + 1 < cd[d];
+    w[d][0] = odd ? 0.5 : 1.0;
+    w[d][1] = hi_in ? 0.5 : 0.0;
+    q[d][0] = lo;
+    q[d][1] = hi_in ? lo + 1 : lo;
   }
   s[0] = s[1] = s[2] = 0.0;
-  for (int a = 0; a < nc[0]; ++a)
-    for (int b = 0; b < nc[1]; ++b)
-      for (int c2 = 0; c2 < nc[2]; ++c2) {
-        const double w = wc[0][a] * wc[1][b] * wc[2][c2];
-        const long long q = ci[0][a] + (long long)Lc.NX * (ci[1][b] + (long long)Lc.NY * ci[2][c2]);
 #pragma unroll
-        for (int c = 0; c < 3; ++c) s[c] += w * xc[c * Lc.Np + q];
+  for (int c2 = 0; c2 < 2; ++c2)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const double wt = w[0][a] * w[1][b] * w[2][c2];
+        const long long qq = q[0][a] + (long long)Lc.NX * (q[1][b] + (long long)Lc.NY * q[2][c2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) s[c] += wt * xc[c * Lc.Np + qq];
       }
 }
 
@@ -128,6 +151,7 @@ __device__ __forceinline__ void direct_out(long long Np, long long n, unsigned m
   }
 }
 
+template <bool HAS_Y>
 __global__ void k_mg_restrict(Lattice Lf, Lattice Lc, const double *__restrict__ rf,
                               const double *__restrict__ yf, const uint8_t *__restrict__ fmask_c,
                               double *__restrict__ bc, const double *__restrict__ dinv_c,
@@ -135,7 +159,7 @@ __global__ void k_mg_restrict(Lattice Lf, Lattice Lc, const double *__restrict__
   const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (n >= Lc.numnp) return;
   double s[3];
-  restrict_node(Lf, Lc, rf, yf, n, s);
+  restrict_node<HAS_Y>(Lf, Lc, rf, yf, n, s);
   const unsigned m = fmask_c[n];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -182,6 +206,29 @@ __global__ void k_mg_jacobi(long long n, double omega, const double *__restrict_
     x[i] += omega * dinv[i] * (b[i] - y[i]);
   else
     x[i] = omega * dinv[i] * b[i];
+}
+
+constexpr int DOT_BLOCKS = 2048;
+
+__global__ __launch_bounds__(256) void k_mg_jacobi_dot(long long n, double omega,
+                                                       const double *__restrict__ dinv,
+                                                       const double *__restrict__ b,
+                                                       const double *__restrict__ y,
+                                                       double *__restrict__ x,
+                                                       const double *__restrict__ q,
+                                                       double *part) {
+  __shared__ double sh[8];
+  double zr = 0.0, zq = 0.0;
+  const long long stride = (long long)gridDim.x * 256;
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += stride) {
+    const double bi = b[i];
+    const double xi = x[i] + omega * dinv[i] * (bi - y[i]);
+    x[i] = xi;
+    zr += xi * bi;
+    zq += xi * q[i];
+  }
+  double v[2] = {zr, zq};
+  store_partials<256, 2>(v, part, sh);
 }
 
 // r = b - y
@@ -253,21 +300,21 @@ __global__ void k_mg_dense_symmetrize(int n, double *__restrict__ A) {
   if (i < j) A[q] = A[j + (long long)i * n];
 }
 
-// x = A b: one wave per row, lanes over the columns
+// x = A b: one block per row, threads over the columns, block reduction
 __global__ __launch_bounds__(256) void k_mg_dense_solve(Lattice L, const double *__restrict__ A,
                                                         const double *__restrict__ b,
                                                         double *__restrict__ x) {
-  const int n = (int)(3 * L.numnp);
-  const int row = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (row >= n) return;
+  __shared__ double sh[4];
+  const int n = (int)(3 * L.numnp), np1 = (int)L.numnp;
+  const int row = blockIdx.x;
   double s = 0.0;
-  for (int j = lane; j < n; j += 64) {
-    const long long lat = (j / L.numnp) * L.Np + j % L.numnp;
-    s += A[(long long)row * n + j] * b[lat];
-  }
+  for (int j = threadIdx.x; j < n; j += 256)
+    s += A[(long long)row * n + j] * b[(long long)(j / np1) * L.Np + j % np1];
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-  if (lane == 0) x[(row / L.numnp) * L.Np + row % L.numnp] = s;
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) x[(long long)(row / np1) * L.Np + row % np1] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
 
 template <int SYM>
@@ -291,8 +338,12 @@ void direct_dispatch(int mode, const Lattice &L, const KeCoef &kc, const double 
 void launch_mg_restrict(const Lattice &Lf, const Lattice &Lc, const double *rf, const double *yf,
                         const uint8_t *fmask_c, double *bc, const double *dinv_c, double omega,
                         double *xc, hipStream_t s) {
-  hipLaunchKernelGGL(k_mg_restrict, dim3(blocks_for(Lc.numnp, 256)), dim3(256), 0, s, Lf, Lc, rf,
-                     yf, fmask_c, bc, dinv_c, omega, xc);
+  if (yf)
+    hipLaunchKernelGGL(k_mg_restrict<true>, dim3(blocks_for(Lc.numnp, 256)), dim3(256), 0, s, Lf, Lc,
+                       rf, yf, fmask_c, bc, dinv_c, omega, xc);
+  else
+    hipLaunchKernelGGL(k_mg_restrict<false>, dim3(blocks_for(Lc.numnp, 256)), dim3(256), 0, s, Lf, Lc,
+                       rf, yf, fmask_c, bc, dinv_c, omega, xc);
   HIPCHK(hipGetLastError());
 }
 
@@ -321,7 +372,7 @@ void launch_mg_dense_symmetrize(int n, double *A, hipStream_t s) {
 void launch_mg_dense_solve(const Lattice &L, const double *ainv, const double *b, double *x,
                            hipStream_t s) {
   if (3 * L.numnp > MG_DENSE_MAX) throw Error(kInvalidArgument, "lattice too large for the dense solve");
-  hipLaunchKernelGGL(k_mg_dense_solve, dim3(blocks_for(3 * L.numnp, 4)), dim3(256), 0, s, L, ainv, b, x);
+  hipLaunchKernelGGL(k_mg_dense_solve, dim3((unsigned)(3 * L.numnp)), dim3(256), 0, s, L, ainv, b, x);
   HIPCHK(hipGetLastError());
 }
 
@@ -342,6 +393,15 @@ void launch_mg_jacobi(long long n, double omega, const double *dinv, const doubl
                       const double *y, double *x, hipStream_t s) {
   hipLaunchKernelGGL(k_mg_jacobi, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, omega, dinv, b, y, x);
   HIPCHK(hipGetLastError());
+}
+
+unsigned launch_mg_jacobi_dot(long long n, double omega, const double *dinv, const double *b,
+                              const double *y, double *x, const double *q, double *part,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(k_mg_jacobi_dot, dim3(DOT_BLOCKS), dim3(256), 0, s, n, omega, dinv, b, y, x, q,
+                     part);
+  HIPCHK(hipGetLastError());
+  return DOT_BLOCKS;
 }
 
 void launch_mg_residual(long long n, const double *b, const double *y, double *r, hipStream_t s) {

@@ -46,12 +46,13 @@ int stencil_nchunks(const Lattice &L);
 void launch_apply_k64(const Lattice &L, const KeCoef &ke, const double *young,
                       const uint8_t *freemask, const double *u, double *y, hipStream_t s);
 // One fused Jacobi-PCG iteration, stencil half: p_new = dinv*r + beta*p_old;
-// q = K p_new; p.q reduced; alpha = rho / p.q.
-void launch_pcg_stencil64(const Lattice &L, const KeCoef &ke, const double *young,
-                          const uint8_t *freemask, const double *r, const double *dinv,
-                          const double *p_old, double *p_new, double *q, PcgScalars *S,
-                          double *partials, unsigned *counter, hipStream_t s,
-                          bool finalize = true);
+// q = K p_new; p.q partials (returns their count); with finalize, alpha = rho /
+// p.q by a finalize kernel.
+unsigned launch_pcg_stencil64(const Lattice &L, const KeCoef &ke, const double *young,
+                              const uint8_t *freemask, const double *r, const double *dinv,
+                              const double *p_old, double *p_new, double *q, PcgScalars *S,
+                              double *partials, unsigned *counter, hipStream_t s,
+                              bool finalize = true);
 // Fused batches: the alpha and beta finalizes run inside the next kernel (every
 // block reduces the partials itself).  In-batch iteration j: the stencil (j >= 1
 // reduces partials_up of update j-1 over Gu blocks for beta and the stop test;

@@ -33,5 +33,20 @@ void launch_zdot_beta(long long n, const double *z, const double *r, const doubl
 // x += alpha p, r -= alpha q, r.r and the stop tests
 void launch_fpcg_update(long long n, double *x, double *r, const double *p, const double *q,
                         PcgScalars *S, double *partials, hipStream_t s);
+// Flexible PCG with the scalar steps folded into neighbouring kernels (used by
+// the multigrid solve).  launch_fpcg_update_alpha: alpha = rho / p.q from the
+// stencil's partials (Gs blocks) in every block, x and r updated, r.r partials
+// to part_up, and with z0 != nullptr also z0 = omega dinv r.  Returns the
+// partials count.
+unsigned launch_fpcg_update_alpha(long long n, double *x, double *r, const double *p,
+                                  const double *q, PcgScalars *S, const double *part_st,
+                                  unsigned Gs, double *part_up, const double *dinv, double omega,
+                                  double *z0, hipStream_t s);
+// z.r and z.q partials; returns their count
+unsigned launch_zdot_partials(long long n, const double *z, const double *r, const double *q,
+                              double *part_z, hipStream_t s);
+// stop tests from the r.r partials, then rho' = z.r and beta (Polak-Ribiere)
+void launch_fpcg_check_beta(const double *part_up, unsigned Gu, const double *part_z, unsigned Gz,
+                            PcgScalars *S, hipStream_t s);
 
 }  // namespace icarat_hip

@@ -156,7 +156,7 @@ void Multigrid::refresh() {
 // V-cycle on level l; every launch on the fine context's stream.  Smoothing
 // ping-pongs between the level's x and y buffers where the fused sweep runs
 // (the z-marching path smooths x in place, with y holding K.x).
-double *Multigrid::cycle(int l, const double *b, bool emit) {
+double *Multigrid::cycle(int l, const double *b, bool emit, bool x0_ready, ZDot *zd) {
   hipStream_t s = f_.stream();
   Context &c = level(l);
   const Lattice &L = c.lattice();
@@ -180,7 +180,7 @@ double *Multigrid::cycle(int l, const double *b, bool emit) {
     }
   };
   // the first sweep from zero (levels >= 1: written by the restriction)
-  if (l == 0 && emit) launch_mg_jacobi(n, w, c.d_dinv, b, nullptr, x, s);
+  if (l == 0 && emit && !x0_ready) launch_mg_jacobi(n, w, c.d_dinv, b, nullptr, x, s);
   if (l == levels() - 1) {
     for (int i = 1; i < std::max(1, o_.nu_coarse); ++i) sweep();
     return cur;
@@ -204,7 +204,18 @@ double *Multigrid::cycle(int l, const double *b, bool emit) {
   }
   const double *xc = cycle(l + 1, cc.d_b, emit);
   if (emit) launch_mg_prolong_add(L, cc.lattice(), xc, c.d_free, cur, s);
-  for (int i = 0; i < std::max(0, o_.nu_post); ++i) sweep();
+  const int post = std::max(0, o_.nu_post);
+  for (int i = 0; i < post; ++i) {
+    if (i == post - 1 && l == 0 && zd && !direct_[l]) {
+      // the last fine sweep also yields the partials of z.r and z.q
+      if (emit) {
+        launch_apply_k64(L, c.kc_, c.d_young, c.d_free, cur, oth, s);
+        zd->count = launch_mg_jacobi_dot(n, w, c.d_dinv, b, oth, cur, zd->q, zd->part, s);
+      }
+    } else {
+      sweep();
+    }
+  }
   return cur;
 }
 
@@ -212,14 +223,24 @@ void Multigrid::enqueue_batch(int par) {
   Context &c = f_;
   hipStream_t s = c.stream();
   const Lattice &L = c.lattice();
+  // the fine level's first sweep comes out of the update unless the fine level
+  // is itself solved exactly
+  const bool x0_fused = !(levels() == 1 && dense_);
   for (int j = 0; j < std::max(1, o_.batch_iters); ++j) {
     double *p_old = c.d_p[(j + par) & 1], *p_new = c.d_p[(j + par + 1) & 1];
-    // p = z + beta p_old, q = K p, alpha
-    launch_pcg_stencil64(L, c.kc_, c.d_young, c.d_free, zres_, ones_, p_old, p_new, c.d_q, c.d_pcg,
-                         c.d_partials, c.d_counter, s);
-    launch_fpcg_update(c.n3_, c.d_x, c.d_r, p_new, c.d_q, c.d_pcg, c.d_partials, s);
-    cycle(0, c.d_r, true);
-    launch_zdot_beta(c.n3_, zres_, c.d_r, c.d_q, c.d_pcg, c.d_partials, s);
+    // p = z + beta p_old, q = K p, p.q partials
+    const unsigned gs = launch_pcg_stencil64(L, c.kc_, c.d_young, c.d_free, zres_, ones_, p_old,
+                                             p_new, c.d_q, c.d_pcg, c.d_partials, c.d_counter, s,
+                                             false);
+    // alpha, x and r, r.r partials, z0 = omega D^-1 r
+    const unsigned gu = launch_fpcg_update_alpha(c.n3_, c.d_x, c.d_r, p_new, c.d_q, c.d_pcg,
+                                                 c.d_partials, gs, c.d_partials2, c.d_dinv,
+                                                 omega_[0], x0_fused ? z_.p : nullptr, s);
+    ZDot zd{c.d_q, c.d_partials, 0};
+    cycle(0, c.d_r, true, x0_fused, &zd);
+    if (!zd.count) zd.count = launch_zdot_partials(c.n3_, zres_, c.d_r, c.d_q, c.d_partials, s);
+    // stop tests, then rho and beta
+    launch_fpcg_check_beta(c.d_partials2, gu, c.d_partials, zd.count, c.d_pcg, s);
   }
 }
 

@@ -64,10 +64,18 @@ class Multigrid {
   double *zres_ = nullptr;  // buffer that holds the fine V-cycle's result
   hipGraph_t graph_[2] = {};
   hipGraphExec_t exec_[2] = {};
-  // V-cycle on level l with right-hand side b (the first sweep already in the
-  // level's x for l >= 1); returns the buffer holding the result.  emit =
-  // false only follows the buffers.
-  double *cycle(int l, const double *b, bool emit);
+  // Partials of z.r and z.q produced by the fine level's last sweep (count
+  // stays 0 when that sweep cannot produce them).
+  struct ZDot {
+    const double *q;
+    double *part;
+    unsigned count;
+  };
+  // V-cycle on level l with right-hand side b; returns the buffer holding the
+  // result.  The first sweep from zero is already in the level's x for l >= 1
+  // (written by the restriction) and for l = 0 when x0_ready.  emit = false
+  // only follows the buffers.
+  double *cycle(int l, const double *b, bool emit, bool x0_ready = false, ZDot *zd = nullptr);
   void enqueue_batch(int par);
   void build_graph();
 };

@@ -356,6 +356,8 @@ int icarat_hip_mg_info(const icarat_hip_ctx *ctx, int32_t *levels, int32_t *dims
  *   kernel 5: PCG stencil half with the FP32 MFMA contraction (FP64 vectors)
  *   kernel 6: PCG stencil half with the BF16 MFMA contraction (FP64 vectors)
  *   kernel 7: plain copy of one lattice vector (3 Np doubles; bandwidth reference)
+ *   kernel 8: FP64 y = K x as the multigrid smoother runs it (no PCG prologue),
+ *             1152 flop/element
  * Needs set_density (and filter_setup for kernel 2).  Overwrites the solver's
  * work vectors. */
 int icarat_hip_time_kernel(icarat_hip_ctx *ctx, int32_t kernel, int32_t reps, double *avg_ms,

@@ -92,6 +92,8 @@ class Device {
 //   "HIP_MF_PCG_FP32"     FP32 MFMA contraction inside FP64 reliable updates
 //   "HIP_MF_PCG_BF16"     BF16 MFMA contraction inside FP64 reliable updates
 //   "HIP_MF_FPCG_LEARNED" learned preconditioner (icarat_hip_learned_setup first)
+//   "HIP_MF_PCG_MG"       FP64 PCG preconditioned by a geometric multigrid V-cycle
+//                         (hierarchy built on first use, icarat_hip_mg_setup to tune)
 class LinearAnalysis {
  public:
   explicit LinearAnalysis(Device &dev, double tol = 1e-8, int maxIter = 10000)
@@ -101,6 +103,7 @@ class LinearAnalysis {
     if (solvName == "HIP_MF_PCG_FP32") o.precision = ICARAT_HIP_FP32;
     else if (solvName == "HIP_MF_PCG_BF16") o.precision = ICARAT_HIP_BF16;
     else if (solvName == "HIP_MF_FPCG_LEARNED") o.precond = ICARAT_HIP_PRECOND_LEARNED;
+    else if (solvName == "HIP_MF_PCG_MG") o.precond = ICARAT_HIP_PRECOND_MG;
     else if (solvName != "HIP_MF_PCG") {
       std::cerr << "Error : unknown solver name " << solvName << std::endl;
       std::exit(1);
