@@ -572,8 +572,8 @@ class Batch:
         self._chk(self.L.icarat_hip_batch_info(self.h, C.byref(n), C.byref(k)), "batch_info")
         return n.value, k.value
 
-    def solve(self, tol=1e-8, maxit=10000, allow_cap=False):
-        o = SolveOpts(tol, maxit, FP64, PRECOND_JACOBI, 0.1, 200, 0)
+    def solve(self, tol=1e-8, maxit=10000, allow_cap=False, precond=PRECOND_JACOBI):
+        o = SolveOpts(tol, maxit, FP64, precond, 0.1, 200, 0)
         st = (SolveStats * len(self.ctx))()
         rc = self.L.icarat_hip_batch_solve(self.h, C.byref(o), st)
         if rc and not (allow_cap and rc == NOT_CONVERGED):
@@ -581,9 +581,9 @@ class Batch:
         return list(st)
 
     def simp_run(self, x0s, n_iter, optimizer=0, use_projection=False, xmin=1e-4, xmax=1.0,
-                 tol=1e-8, maxit=10000):
+                 tol=1e-8, maxit=10000, precond=PRECOND_JACOBI):
         o = SimpOpts(n_iter, optimizer, 1 if use_projection else 0, 0, xmin, xmax,
-                     SolveOpts(tol, maxit, FP64, PRECOND_JACOBI, 0.1, 200, 0))
+                     SolveOpts(tol, maxit, FP64, precond, 0.1, 200, 0))
         xs = [np.array(x, dtype=np.float64) for x in x0s]
         hs = [np.zeros(4 * max(n_iter, 1)) for _ in xs]
         xp = (dp * len(xs))(*[_p(x) for x in xs])

@@ -40,7 +40,9 @@ void Batch::drop_graph() {
 }
 
 hipGraphExec_t Batch::graph_for(const std::vector<char> &run) {
-  const std::string key(run.begin(), run.end());
+  // captured branches depend on the preconditioner of each context
+  std::string key(run.begin(), run.end());
+  for (Context *c : ctx_) key.push_back((char)c->batch_precond_);
   auto it = graphs_.find(key);
   if (it != graphs_.end()) return it->second.second;
   const int n = size();
@@ -162,9 +164,15 @@ void Batch::engine(double tol, int maxit, const std::function<bool(int)> &start,
   }
 }
 
+void Batch::set_precond(const icarat_solve_opts &o) {
+  if (o.precision != ICARAT_HIP_FP64 ||
+      (o.precond != ICARAT_HIP_PRECOND_JACOBI && o.precond != ICARAT_HIP_PRECOND_MG))
+    throw Error(kInvalidArgument, "batched solves are FP64 Jacobi-PCG or multigrid PCG");
+  for (Context *c : ctx_) c->batch_precond_ = o.precond;
+}
+
 void Batch::solve(const icarat_solve_opts &o, icarat_solve_stats *st) {
-  if (o.precision != ICARAT_HIP_FP64 || o.precond != ICARAT_HIP_PRECOND_JACOBI)
-    throw Error(kInvalidArgument, "batched solves are FP64 Jacobi-PCG");
+  set_precond(o);
   if (!(o.tol > 0.0) || o.maxit < 0) throw Error(kInvalidArgument, "tol must be > 0 and maxit >= 0");
   const int n = size();
   std::vector<char> started(n, 0);
@@ -189,8 +197,7 @@ void Batch::solve(const icarat_solve_opts &o, icarat_solve_stats *st) {
 
 void Batch::simp_run(const icarat_simp_opts &o, double *const *x, double *const *hist,
                      icarat_simp_stats *st) {
-  if (o.solve.precision != ICARAT_HIP_FP64 || o.solve.precond != ICARAT_HIP_PRECOND_JACOBI)
-    throw Error(kInvalidArgument, "batched solves are FP64 Jacobi-PCG");
+  set_precond(o.solve);
   if (!x) throw Error(kInvalidArgument, "null design array");
   const int n = size();
   for (int i = 0; i < n; ++i)
@@ -248,6 +255,8 @@ int Batch::autotune(int reps, double *ms_out) {
   hipEvent_t a, b;
   HIPCHK(hipEventCreate(&a));
   HIPCHK(hipEventCreate(&b));
+  // timed on Jacobi-PCG launches (the stencil configuration is shared)
+  for (Context *c : ctx_) c->batch_precond_ = ICARAT_HIP_PRECOND_JACOBI;
   for (int cfg = 0; cfg < 4; ++cfg) {
     apply_config(cfg);
     // every problem iterates (unreachable tolerance, no cap in reach): the

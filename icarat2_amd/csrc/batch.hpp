@@ -30,7 +30,8 @@ class Batch {
   int size() const { return (int)ctx_.size(); }
   long long graph_launches() const { return launches_; }
 
-  // one FP64 Jacobi-PCG solve of every context's current load case
+  // one FP64 solve of every context's current load case (Jacobi-PCG or
+  // multigrid PCG, o.precond)
   void solve(const icarat_solve_opts &o, icarat_solve_stats *st);
   // the SIMP loop of every context; each problem advances on its own (design
   // phases between its solves), the solves of all problems share the launches
@@ -67,6 +68,7 @@ class Batch {
   void engine(double tol, int maxit, const std::function<bool(int)> &start,
               const std::function<void(int, const icarat_solve_stats &)> &done);
   void apply_config(int cfg);
+  void set_precond(const icarat_solve_opts &o);
 };
 
 }  // namespace icarat_hip

@@ -210,6 +210,7 @@ void Context::set_density_device() {
   launch_diag_inv(L_, d_ke, d_young, d_free, d_dinv, nullptr, stream_);
   density_ready_ = true;
   solved_ = false;
+  ++density_version_;
 }
 
 void Context::apply_k(const double *u, double *y, int precision) {
@@ -240,6 +241,10 @@ void Context::get_diag(double *d) {
 }
 
 void Context::enqueue_pcg_batch(int kmode, int par) {
+  if (batch_precond_ == ICARAT_HIP_PRECOND_MG) {
+    mg_->enqueue_batch(par);
+    return;
+  }
   if (kmode < 0) {
     // FP64: the finalizes are folded into the next kernel; one at the end of
     // the batch leaves the scalars complete for the host
@@ -460,6 +465,15 @@ void Context::pcg_start(double tol, int maxit, hipStream_t s) {
   sync();
   if (!(tol > 0.0) || maxit < 0) throw Error(kInvalidArgument, "tol must be > 0 and maxit >= 0");
   prepare_rhs(s);
+  replacements_ = 0;
+  fell_back_ = 0;
+  if (batch_precond_ == ICARAT_HIP_PRECOND_MG) {
+    if (!mg_) mg_setup(mg_default_opts());
+    mg_->start(tol, maxit, s);
+    return;
+  }
+  if (batch_precond_ != ICARAT_HIP_PRECOND_JACOBI)
+    throw Error(kInvalidArgument, "interleaved solves run Jacobi or multigrid PCG");
   PcgScalars init{};
   init.tol = tol;
   init.maxit = maxit;
@@ -467,11 +481,14 @@ void Context::pcg_start(double tol, int maxit, hipStream_t s) {
   h2d(d_pcg.p, h_pcg.p, 1, s);
   launch_pcg_init64(n3_, d_b, d_x, d_r, d_p[0], d_dinv, d_pcg, d_partials, d_counter, s);
   d2h(h_pcg.p, d_pcg.p, 1, s);
-  replacements_ = 0;
-  fell_back_ = 0;
 }
 
 void Context::pcg_verify(hipStream_t s) {
+  if (batch_precond_ == ICARAT_HIP_PRECOND_MG) {
+    mg_->verify(s);
+    ++replacements_;
+    return;
+  }
   // the FP64 true residual replaces the recursive one; CG continues if it fails the test
   launch_apply_k64(L_, kc_, d_young, d_free, d_x, d_y, s);
   launch_replace64(n3_, d_b, d_y, d_r, d_dinv, d_pcg, d_partials, s);

@@ -157,14 +157,20 @@ class Context {
   void assemble_nodal_solution();         // d_u = d_x + Dirichlet values
   void copy_out_solution(double *u_host); // numeq copy of d_u (if u_host), marks solved
   void prepare_rhs(hipStream_t s);        // d_b = f - K u_Dirichlet
-  // FP64 Jacobi-PCG in pieces for a driver that interleaves several contexts:
-  // start (rhs, init, scalars to h_pcg), verify (true residual replaces the
+  // FP64 PCG in pieces for a driver that interleaves several contexts: start
+  // (rhs, init, scalars to h_pcg), verify (true residual replaces the
   // recursive one, scalars to h_pcg), finish (nodal solution, true residual,
-  // stats).  The iterations themselves are enqueue_pcg_batch(-1, 0).
+  // stats).  The iterations themselves are enqueue_pcg_batch(-1, 0).  The
+  // preconditioner is batch_precond_: Jacobi, or the multigrid V-cycle (set up
+  // with the defaults on first use).
   void pcg_start(double tol, int maxit, hipStream_t s);
   void pcg_verify(hipStream_t s);
   void pcg_finish(icarat_solve_stats &S);
   void enqueue_pcg_batch(int kmode, int par);  // kBatch iterations on the stream
+  int batch_precond_ = ICARAT_HIP_PRECOND_JACOBI;
+  // bumped whenever the densities change (the multigrid hierarchy refreshes
+  // its coarse operators once per version)
+  unsigned long long density_version_ = 0;
   void sync() { HIPCHK(hipStreamSynchronize(stream_)); }
   // Device buffers are allocated on first device use, so mesh, numbering,
   // loads and KE0 work without a GPU.

@@ -365,6 +365,14 @@ __global__ void k_fpcg_check_beta(const double *__restrict__ part_up, unsigned G
   S->rho = rho1;
 }
 
+// a restart of the flexible PCG (beta = 0 at the next beta step) unless done
+__global__ void k_fpcg_restart_flag(PcgScalars *S) {
+  if (threadIdx.x == 0 && !S->done) {
+    S->restart = 1;
+    S->alpha = 0.0;
+  }
+}
+
 }  // namespace
 
 void launch_restrict_residual(const Lattice &L, const NetGrid &g, const double *r, const double *dinv,
@@ -448,6 +456,11 @@ unsigned launch_zdot_partials(long long n, const double *z, const double *r, con
   hipLaunchKernelGGL(k_zdot_partials, dim3(VEC_BLOCKS), dim3(NT), 0, s, n, z, r, q, part_z);
   HIPCHK(hipGetLastError());
   return VEC_BLOCKS;
+}
+
+void launch_fpcg_restart_flag(PcgScalars *S, hipStream_t s) {
+  hipLaunchKernelGGL(k_fpcg_restart_flag, dim3(1), dim3(64), 0, s, S);
+  HIPCHK(hipGetLastError());
 }
 
 void launch_fpcg_check_beta(const double *part_up, unsigned Gu, const double *part_z, unsigned Gz,

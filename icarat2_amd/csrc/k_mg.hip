@@ -19,88 +19,17 @@ namespace {
 
 inline unsigned blocks_for(long long n, int nt) { return (unsigned)((n + nt - 1) / nt); }
 
-__device__ __forceinline__ double w1(int d) { return d == 0 ? 1.0 : 0.5; }
-
-// Restriction and prolongation run fixed, unrolled stencils: indices outside
-// the lattice are clamped and carry weight 0, so the loads of a plane are
-// independent of each other and in flight together.
-__device__ __forceinline__ void tap(int c, int dim, int d, double &w, int &q) {
-  // fine coordinate c + d in [0, dim): 1-D trilinear weight (1 centre, 1/2 side)
-  const int v = c + d;
-  const bool in = v >= 0 && v < dim;
-  w = in ? (d == 0 ? 1.0 : 0.5) : 0.0;
-  q = in ? v : 0;
-}
-
-// s = sum over the 27 fine nodes around fine node (2I, 2J, 2K) of w * (rf - yf)
-// (HAS_Y = false: of w * rf)
-template <bool HAS_Y>
-__device__ __forceinline__ void restrict_node(const Lattice &Lf, const Lattice &Lc,
-                                              const double *__restrict__ rf,
-                                              const double *__restrict__ yf, long long n,
-                                              double (&s)[3]) {
-  const int I = (int)(n % Lc.NX), J = (int)((n / Lc.NX) % Lc.NY), K = (int)(n / ((long long)Lc.NX * Lc.NY));
-  double wx[3], wy[3], wz[3];
-  int qx[3], qy[3], qz[3];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    tap(2 * I, Lf.NX, d - 1, wx[d], qx[d]);
-    tap(2 * J, Lf.NY, d - 1, wy[d], qy[d]);
-    tap(2 * K, Lf.NZ, d - 1, wz[d], qz[d]);
-  }
-  s[0] = s[1] = s[2] = 0.0;
-  // one fine plane at a time: 9 nodes x 3 components (x 2 vectors) in flight
-#pragma unroll 1
-  for (int dz = 0; dz < 3; ++dz)
-#pragma unroll
-    for (int dy = 0; dy < 3; ++dy) {
-      const long long row = (long long)Lf.NX * (qy[dy] + (long long)Lf.NY * qz[dz]);
-      const double wyz = wy[dy] * wz[dz];
-#pragma unroll
-      for (int dx = 0; dx < 3; ++dx) {
-        const long long f = row + qx[dx];
-        const double w = wx[dx] * wyz;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const double v = HAS_Y ? rf[c * Lf.Np + f] - yf[c * Lf.Np + f] : rf[c * Lf.Np + f];
-          s[c] += w * v;
-        }
-      }
-    }
-}
-
-// (P xc) at fine node n: the 2x2x2 coarse nodes around it, weighted (a fine
-// coordinate 2I coincides with coarse I: weights 1 and 0; 2I+1: 1/2 and 1/2)
-__device__ __forceinline__ void prolong_node(const Lattice &Lf, const Lattice &Lc,
-                                             const double *__restrict__ xc, long long n,
-                                             double (&s)[3]) {
-  const int co[3] = {(int)(n % Lf.NX), (int)((n / Lf.NX) % Lf.NY), (int)(n / ((long long)Lf.NX * Lf.NY))};
-  const int cd[3] = {Lc.NX, Lc.NY, Lc.NZ};
-  double w[3][2];
-  int q[3][2];
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    const int lo = co[d] >> 1;
-    const bool odd = co[d] & 1;
-    const bool hi_in = odd && lo + 1 < cd[d];
-    w[d][0] = odd ? 0.5 : 1.0;
-    w[d][1] = hi_in ? 0.5 : 0.0;
-    q[d][0] = lo;
-    q[d][1] = hi_in ? lo + 1 : lo;
-  }
-  s[0] = s[1] = s[2] = 0.0;
-#pragma unroll
-  for (int c2 = 0; c2 < 2; ++c2)
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int a = 0; a < 2; ++a) {
-        const double wt = w[0][a] * w[1][b] * w[2][c2];
-        const long long qq = q[0][a] + (long long)Lc.NX * (q[1][b] + (long long)Lc.NY * q[2][c2]);
-#pragma unroll
-        for (int c = 0; c < 3; ++c) s[c] += wt * xc[c * Lc.Np + qq];
-      }
-}
+// Restriction and prolongation are tiled through LDS.  A restriction block
+// owns RTX x RTY x RTZ coarse nodes and stages the fine region under them (one
+// fine node of halo on each side) with the residual formed on load, so each
+// fine value is read from HBM about once rather than by every coarse node that
+// weighs it.  A prolongation block owns PTX x PTY x PTZ fine nodes and stages
+// the coarse nodes around them.  Lattice nodes outside the grid enter as 0,
+// which is the masked transfer's value there.
+constexpr int RTX = 16, RTY = 4, RTZ = 4;
+constexpr int RFX = 2 * RTX + 1, RFY = 2 * RTY + 1, RFZ = 2 * RTZ + 1, RFN = RFX * RFY * RFZ;
+constexpr int PTX = 32, PTY = 4, PTZ = 2;
+constexpr int PCX = PTX / 2 + 1, PCY = PTY / 2 + 1, PCZ = PTZ / 2 + 1, PCN = PCX * PCY * PCZ;
 
 // (K x) at node n from global memory: the 27 neighbouring displacements (zero
 // outside the lattice) and the 8 surrounding elements (zero outside)
@@ -151,15 +80,61 @@ __device__ __forceinline__ void direct_out(long long Np, long long n, unsigned m
   }
 }
 
+// bc = R (rf - yf) (or R rf), masked; with dinv_c also xc = omega dinv_c bc
 template <bool HAS_Y>
-__global__ void k_mg_restrict(Lattice Lf, Lattice Lc, const double *__restrict__ rf,
-                              const double *__restrict__ yf, const uint8_t *__restrict__ fmask_c,
-                              double *__restrict__ bc, const double *__restrict__ dinv_c,
-                              double omega, double *__restrict__ xc) {
-  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (n >= Lc.numnp) return;
-  double s[3];
-  restrict_node<HAS_Y>(Lf, Lc, rf, yf, n, s);
+__global__ __launch_bounds__(256) void k_mg_restrict(Lattice Lf, Lattice Lc,
+                                                     const double *__restrict__ rf,
+                                                     const double *__restrict__ yf,
+                                                     const uint8_t *__restrict__ fmask_c,
+                                                     double *__restrict__ bc,
+                                                     const double *__restrict__ dinv_c,
+                                                     double omega, double *__restrict__ xc) {
+  __shared__ double sv[3][RFN];
+  const int I0 = blockIdx.x * RTX, J0 = blockIdx.y * RTY, K0 = blockIdx.z * RTZ;
+  const int fx0 = 2 * I0 - 1, fy0 = 2 * J0 - 1, fz0 = 2 * K0 - 1;
+  // all loads of the staging are issued before any is stored (load phase into
+  // registers, then the LDS phase)
+  constexpr int NK = (RFN + 255) / 256;
+  double v[NK][3];
+#pragma unroll
+  for (int k = 0; k < NK; ++k) {
+    const int t = threadIdx.x + 256 * k;
+    const int hx = t % RFX, hy = (t / RFX) % RFY, hz = t / (RFX * RFY);
+    const int gx = fx0 + hx, gy = fy0 + hy, gz = fz0 + hz;
+    const bool in = t < RFN && gx >= 0 && gx < Lf.NX && gy >= 0 && gy < Lf.NY && gz >= 0 && gz < Lf.NZ;
+    const long long f = in ? gx + (long long)Lf.NX * (gy + (long long)Lf.NY * gz) : 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double a = in ? rf[c * Lf.Np + f] : 0.0;
+      const double b = (HAS_Y && in) ? yf[c * Lf.Np + f] : 0.0;
+      v[k][c] = a - b;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NK; ++k) {
+    const int t = threadIdx.x + 256 * k;
+    if (t < RFN) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) sv[c][t] = v[k][c];
+    }
+  }
+  __syncthreads();
+  const int tx = threadIdx.x % RTX, ty = (threadIdx.x / RTX) % RTY, tz = threadIdx.x / (RTX * RTY);
+  const int I = I0 + tx, J = J0 + ty, K = K0 + tz;
+  if (I >= Lc.NX || J >= Lc.NY || K >= Lc.NZ) return;
+  double s[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+      for (int dx = -1; dx <= 1; ++dx) {
+        const double w = (dx == 0 ? 1.0 : 0.5) * (dy == 0 ? 1.0 : 0.5) * (dz == 0 ? 1.0 : 0.5);
+        const int h = (2 * tx + 1 + dx) + RFX * ((2 * ty + 1 + dy) + RFY * (2 * tz + 1 + dz));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) s[c] += w * sv[c][h];
+      }
+  const long long n = I + (long long)Lc.NX * (J + (long long)Lc.NY * K);
   const unsigned m = fmask_c[n];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -169,16 +144,61 @@ __global__ void k_mg_restrict(Lattice Lf, Lattice Lc, const double *__restrict__
   }
 }
 
-__global__ void k_mg_prolong_add(Lattice Lf, Lattice Lc, const double *__restrict__ xc,
-                                 const uint8_t *__restrict__ fmask_f, double *__restrict__ xf) {
-  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (n >= Lf.numnp) return;
-  double s[3];
-  prolong_node(Lf, Lc, xc, n, s);
-  const unsigned m = fmask_f[n];
+// xf += P xc (trilinear), masked by the fine free DOFs
+__global__ __launch_bounds__(256) void k_mg_prolong_add(Lattice Lf, Lattice Lc,
+                                                        const double *__restrict__ xc,
+                                                        const uint8_t *__restrict__ fmask_f,
+                                                        double *__restrict__ xf) {
+  __shared__ double sc[3][PCN];
+  const int fx0 = blockIdx.x * PTX, fy0 = blockIdx.y * PTY, fz0 = blockIdx.z * PTZ;
+  const int I0 = fx0 / 2, J0 = fy0 / 2, K0 = fz0 / 2;
+  const int tx = threadIdx.x % PTX, ty = (threadIdx.x / PTX) % PTY, tz = threadIdx.x / (PTX * PTY);
+  const int x = fx0 + tx, y = fy0 + ty, z = fz0 + tz;
+  const bool own = x < Lf.NX && y < Lf.NY && z < Lf.NZ;
+  const long long n = own ? x + (long long)Lf.NX * (y + (long long)Lf.NY * z) : 0;
+  // the fine values and mask are loaded before the coarse staging, so both
+  // round trips overlap
+  double xo[3] = {0.0, 0.0, 0.0};
+  unsigned m = 0;
+  if (own) {
+    m = fmask_f[n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) xo[c] = xf[c * Lf.Np + n];
+  }
+  for (int t = threadIdx.x; t < PCN; t += 256) {
+    const int hx = t % PCX, hy = (t / PCX) % PCY, hz = t / (PCX * PCY);
+    const int gx = I0 + hx, gy = J0 + hy, gz = K0 + hz;
+    double v[3] = {0.0, 0.0, 0.0};
+    if (gx < Lc.NX && gy < Lc.NY && gz < Lc.NZ) {
+      const long long q = gx + (long long)Lc.NX * (gy + (long long)Lc.NY * gz);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[c] = xc[c * Lc.Np + q];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) sc[c][t] = v[c];
+  }
+  __syncthreads();
+  if (!own) return;
+  // local coarse index of the lower neighbour and the 1-D weights (an even
+  // fine coordinate sits on a coarse node: weights 1, 0; odd: 1/2, 1/2)
+  const int lx = tx >> 1, ly = ty >> 1, lz = tz >> 1;
+  const double wx1 = (tx & 1) ? 0.5 : 0.0, wy1 = (ty & 1) ? 0.5 : 0.0, wz1 = (tz & 1) ? 0.5 : 0.0;
+  const double wx[2] = {1.0 - wx1, wx1}, wy[2] = {1.0 - wy1, wy1}, wz[2] = {1.0 - wz1, wz1};
+  double s[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int c2 = 0; c2 < 2; ++c2)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        const double w = wx[a] * wy[b] * wz[c2];
+        const int h = (lx + a) + PCX * ((ly + b) + PCY * (lz + c2));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) s[c] += w * sc[c][h];
+      }
 #pragma unroll
   for (int c = 0; c < 3; ++c)
-    if ((m >> c) & 1u) xf[c * Lf.Np + n] += s[c];
+    if ((m >> c) & 1u) xf[c * Lf.Np + n] = xo[c] + s[c];
 }
 
 // coarse element = mean of its 8 children
@@ -338,12 +358,24 @@ void direct_dispatch(int mode, const Lattice &L, const KeCoef &kc, const double 
 void launch_mg_restrict(const Lattice &Lf, const Lattice &Lc, const double *rf, const double *yf,
                         const uint8_t *fmask_c, double *bc, const double *dinv_c, double omega,
                         double *xc, hipStream_t s) {
+  if (Lf.NX != 2 * Lc.NX - 1 || Lf.NY != 2 * Lc.NY - 1 || Lf.NZ != 2 * Lc.NZ - 1)
+    throw Error(kInvalidArgument, "restriction: lattices are not 2x apart");
+  const dim3 g((Lc.NX + RTX - 1) / RTX, (Lc.NY + RTY - 1) / RTY, (Lc.NZ + RTZ - 1) / RTZ);
   if (yf)
-    hipLaunchKernelGGL(k_mg_restrict<true>, dim3(blocks_for(Lc.numnp, 256)), dim3(256), 0, s, Lf, Lc,
-                       rf, yf, fmask_c, bc, dinv_c, omega, xc);
+    hipLaunchKernelGGL(k_mg_restrict<true>, g, dim3(256), 0, s, Lf, Lc, rf, yf, fmask_c, bc, dinv_c,
+                       omega, xc);
   else
-    hipLaunchKernelGGL(k_mg_restrict<false>, dim3(blocks_for(Lc.numnp, 256)), dim3(256), 0, s, Lf, Lc,
-                       rf, yf, fmask_c, bc, dinv_c, omega, xc);
+    hipLaunchKernelGGL(k_mg_restrict<false>, g, dim3(256), 0, s, Lf, Lc, rf, yf, fmask_c, bc, dinv_c,
+                       omega, xc);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_mg_prolong_add(const Lattice &Lf, const Lattice &Lc, const double *xc,
+                           const uint8_t *fmask_f, double *xf, hipStream_t s) {
+  if (Lf.NX != 2 * Lc.NX - 1 || Lf.NY != 2 * Lc.NY - 1 || Lf.NZ != 2 * Lc.NZ - 1)
+    throw Error(kInvalidArgument, "prolongation: lattices are not 2x apart");
+  const dim3 g((Lf.NX + PTX - 1) / PTX, (Lf.NY + PTY - 1) / PTY, (Lf.NZ + PTZ - 1) / PTZ);
+  hipLaunchKernelGGL(k_mg_prolong_add, g, dim3(256), 0, s, Lf, Lc, xc, fmask_f, xf);
   HIPCHK(hipGetLastError());
 }
 
@@ -373,13 +405,6 @@ void launch_mg_dense_solve(const Lattice &L, const double *ainv, const double *b
                            hipStream_t s) {
   if (3 * L.numnp > MG_DENSE_MAX) throw Error(kInvalidArgument, "lattice too large for the dense solve");
   hipLaunchKernelGGL(k_mg_dense_solve, dim3((unsigned)(3 * L.numnp)), dim3(256), 0, s, L, ainv, b, x);
-  HIPCHK(hipGetLastError());
-}
-
-void launch_mg_prolong_add(const Lattice &Lf, const Lattice &Lc, const double *xc,
-                           const uint8_t *fmask_f, double *xf, hipStream_t s) {
-  hipLaunchKernelGGL(k_mg_prolong_add, dim3(blocks_for(Lf.numnp, 256)), dim3(256), 0, s, Lf, Lc,
-                     xc, fmask_f, xf);
   HIPCHK(hipGetLastError());
 }
 

@@ -45,6 +45,8 @@ unsigned launch_fpcg_update_alpha(long long n, double *x, double *r, const doubl
 // z.r and z.q partials; returns their count
 unsigned launch_zdot_partials(long long n, const double *z, const double *r, const double *q,
                               double *part_z, hipStream_t s);
+// the next beta step restarts (beta = 0) unless the solve is done
+void launch_fpcg_restart_flag(PcgScalars *S, hipStream_t s);
 // stop tests from the r.r partials, then rho' = z.r and beta (Polak-Ribiere)
 void launch_fpcg_check_beta(const double *part_up, unsigned Gu, const double *part_z, unsigned Gz,
                             PcgScalars *S, hipStream_t s);

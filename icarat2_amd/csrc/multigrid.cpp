@@ -110,6 +110,8 @@ Multigrid::Multigrid(Context &fine, const icarat_mg_opts &o) : f_(fine), o_(o) {
 Multigrid::~Multigrid() {
   drop_graphs();
   if (blas_) (void)rocblas_destroy_handle(blas_);
+  for (hipEvent_t e : ev_)
+    if (e) (void)hipEventDestroy(e);
 }
 
 void Multigrid::drop_graphs() {
@@ -262,39 +264,72 @@ void Multigrid::build_graph() {
   }
 }
 
+void Multigrid::enqueue_restart() {
+  Context &c = f_;
+  hipStream_t s = c.stream();
+  launch_fpcg_restart_flag(c.d_pcg, s);
+  cycle(0, c.d_r, true);
+  launch_zdot_beta(c.n3_, zres_, c.d_r, c.d_q, c.d_pcg, c.d_partials, s);
+}
+
+void Multigrid::start(double tol, int maxit, hipStream_t s_ext) {
+  Context &c = f_;
+  hipStream_t s = c.stream();
+  for (hipEvent_t &e : ev_)
+    if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  if (s_ext != s) {
+    HIPCHK(hipEventRecord(ev_[0], s_ext));
+    HIPCHK(hipStreamWaitEvent(s, ev_[0], 0));
+  }
+  if (refreshed_ != c.density_version_) {
+    refresh();
+    refreshed_ = c.density_version_;
+  }
+  PcgScalars *h = c.h_pcg.p;
+  PcgScalars s0{};
+  s0.tol = tol;
+  s0.maxit = maxit;
+  *h = s0;
+  HIPCHK(hipMemcpyAsync(c.d_pcg.p, h, sizeof(PcgScalars), hipMemcpyHostToDevice, s));
+  launch_pcg_init64(c.n3_, c.d_b, c.d_x, c.d_r, c.d_p[0], c.d_dinv, c.d_pcg, c.d_partials,
+                    c.d_counter, s);
+  // z = M(r), rho = z.r, beta = 0 (a zero right-hand side is done already)
+  enqueue_restart();
+  HIPCHK(hipMemcpyAsync(h, c.d_pcg.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+}
+
+void Multigrid::verify(hipStream_t s_ext) {
+  Context &c = f_;
+  hipStream_t s = c.stream();
+  if (s_ext != s) {
+    HIPCHK(hipEventRecord(ev_[0], s_ext));
+    HIPCHK(hipStreamWaitEvent(s, ev_[0], 0));
+  }
+  // the FP64 true residual replaces the recursive one; if it fails the test
+  // the iteration restarts from z = M(r) with beta = 0
+  launch_apply_k64(c.lattice(), c.kc_, c.d_young, c.d_free, c.d_x, c.d_y, s);
+  launch_replace64(c.n3_, c.d_b, c.d_y, c.d_r, c.d_dinv, c.d_pcg, c.d_partials, s);
+  enqueue_restart();
+  HIPCHK(hipMemcpyAsync(c.h_pcg.p, c.d_pcg.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
+  if (s_ext != s) {
+    HIPCHK(hipEventRecord(ev_[1], s));
+    HIPCHK(hipStreamWaitEvent(s_ext, ev_[1], 0));
+  }
+}
+
 void Multigrid::solve(const icarat_solve_opts &o, icarat_solve_stats &st) {
   if (o.precision != ICARAT_HIP_FP64)
     throw Error(kInvalidArgument, "the multigrid preconditioner runs the FP64 contraction");
   Context &c = f_;
   hipStream_t s = c.stream();
-  const Lattice &L = c.lattice();
-  refresh();
+  start(o.tol, o.maxit, s);
   build_graph();
   PcgScalars *h = c.h_pcg.p;
-  PcgScalars s0{};
-  s0.tol = o.tol;
-  s0.maxit = o.maxit;
-  *h = s0;
-  HIPCHK(hipMemcpyAsync(c.d_pcg.p, h, sizeof(PcgScalars), hipMemcpyHostToDevice, s));
-  launch_pcg_init64(c.n3_, c.d_b, c.d_x, c.d_r, c.d_p[0], c.d_dinv, c.d_pcg, c.d_partials,
-                    c.d_counter, s);
-  HIPCHK(hipMemcpyAsync(h, c.d_pcg.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
-  c.sync();
-  // z = M(r), rho = z.r, beta = 0
-  auto restart = [&]() {
-    h->restart = 1;
-    h->alpha = 0.0;
-    HIPCHK(hipMemcpyAsync(c.d_pcg.p, h, sizeof(PcgScalars), hipMemcpyHostToDevice, s));
-    cycle(0, c.d_r, true);
-    launch_zdot_beta(c.n3_, zres_, c.d_r, c.d_q, c.d_pcg, c.d_partials, s);
-    HIPCHK(hipMemcpyAsync(h, c.d_pcg.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
-    c.sync();
-  };
-  if (!h->done) restart();
   const double bnorm = std::sqrt(h->bnorm2);
   const int batch = std::max(1, o_.batch_iters);
   int repl = 0;
-  while (!h->done || h->status == 0) {
+  for (;;) {
     if (!h->done) {
       const long long max_batches = (long long)(o.maxit - h->iter) / batch + 3;
       for (long long b = 0; b < max_batches && !h->done; ++b) {
@@ -308,14 +343,11 @@ void Multigrid::solve(const icarat_solve_opts &o, icarat_solve_stats &st) {
       if (!h->done) throw Error(kInternal, "multigrid PCG did not reach its stop test");
     }
     if (h->status != 0) break;
-    // verified stop: the FP64 true residual replaces the recursive one
-    launch_apply_k64(L, c.kc_, c.d_young, c.d_free, c.d_x, c.d_y, s);
-    launch_replace64(c.n3_, c.d_b, c.d_y, c.d_r, c.d_dinv, c.d_pcg, c.d_partials, s);
-    HIPCHK(hipMemcpyAsync(h, c.d_pcg.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
+    // verified stop
+    verify(s);
     c.sync();
     if (h->done) break;
     if (++repl > 50) break;
-    restart();
   }
   st.fallback = 0;
   st.refine_steps = repl;

@@ -44,6 +44,17 @@ class Multigrid {
   void refresh();
   // K u = b of the fine context (d_b -> d_x) by PCG with M = V-cycle
   void solve(const icarat_solve_opts &o, icarat_solve_stats &st);
+  // The solve in pieces (the fine context's d_b must be ready), each ordered
+  // after the work on stream s and before later work on s:
+  // start: coarse operators refreshed if the densities changed, PCG state
+  // initialised, z = M(r) with beta = 0, scalars to the host copy (blocks);
+  // enqueue_batch(par): batch_iters iterations on the fine context's stream
+  // (p buffers at parity par); verify: the FP64 true residual replaces the
+  // recursive one and a restart z = M(r), beta = 0 follows, scalars to the
+  // host copy once s has completed.
+  void start(double tol, int maxit, hipStream_t s);
+  void enqueue_batch(int par);
+  void verify(hipStream_t s);
   void drop_graphs();
   // whether the coarsest level is solved exactly (dense inverse)
   bool dense() const { return dense_; }
@@ -76,8 +87,10 @@ class Multigrid {
   // (written by the restriction) and for l = 0 when x0_ready.  emit = false
   // only follows the buffers.
   double *cycle(int l, const double *b, bool emit, bool x0_ready = false, ZDot *zd = nullptr);
-  void enqueue_batch(int par);
   void build_graph();
+  void enqueue_restart();                      // restart flag, z = M(r), beta = 0
+  unsigned long long refreshed_ = ~0ULL;       // density version of the coarse operators
+  hipEvent_t ev_[2] = {};                      // ordering between stream s and the fine stream
 };
 
 }  // namespace icarat_hip

@@ -281,8 +281,11 @@ int icarat_hip_unet_train_step(icarat_hip_unet *net, const float *x, const float
  * load cases, filter), solved together.  Per solve round the PCG iterations of
  * every problem are one captured graph: a branch per problem on its context's
  * stream, so the problems' kernels run concurrently; each problem keeps its own
- * FP64 verified stop.  Batched solves are FP64 Jacobi-PCG.  A context added to a
- * batch must outlive it and must not be used on its own while the batch runs. */
+ * FP64 verified stop.  Batched solves are FP64 Jacobi-PCG or FP64 multigrid PCG
+ * (opts precond ICARAT_HIP_PRECOND_MG; each problem's hierarchy is set up with
+ * the defaults unless icarat_hip_mg_setup was called on its context).  A context
+ * added to a batch must outlive it and must not be used on its own while the
+ * batch runs. */
 typedef struct icarat_hip_batch icarat_hip_batch;
 int icarat_hip_batch_create(icarat_hip_batch **out);
 int icarat_hip_batch_destroy(icarat_hip_batch *b);

@@ -6,7 +6,8 @@ solves of the same density and against the oracle's compliance.
 Batch: one captured graph per PCG round over problems of uneven size; every
 problem must give what it gives alone (same kernels, same order: iteration
 counts equal, solutions and designs to 1e-12), keep its FP64 verified stop,
-and survive the autotuner's reconfiguration.
+and survive the autotuner's reconfiguration.  Both batched preconditioners
+(Jacobi and the multigrid V-cycle) are held to that.
 """
 import numpy as np
 import pytest
@@ -85,18 +86,20 @@ def _scaled(spec, factor):
     return s
 
 
-def test_batch_solve_equals_individual_solves(gpu):
+@pytest.mark.parametrize("mg", [False, True])
+def test_batch_solve_equals_individual_solves(gpu, mg):
     from icarat2_amd import workloads as W
+    pre = gpu.PRECOND_MG if mg else gpu.PRECOND_JACOBI
     specs = [_scaled(s, 2) for s in _batch_specs()]
     ctxs = [W.build_context(gpu, s) for s in specs]
     alone = []
     for c, s in zip(ctxs, specs):
         x = W.initial_design(s) * (1.0 + 0.3 * np.sin(np.arange(c.nelx)))
         c.set_density(np.clip(x, 1e-3, 1.0))
-        u, st = c.solve(tol=1e-9)
+        u, st = c.solve(tol=1e-9, precond=pre)
         alone.append((c.fext() @ u, st))
     b = gpu.Batch(ctxs)
-    st = b.solve(tol=1e-9)
+    st = b.solve(tol=1e-9, precond=pre)
     n, launches = b.info()
     assert n == len(ctxs) and launches > 0
     for c, (c0, s0), s1 in zip(ctxs, alone, st):
@@ -108,16 +111,18 @@ def test_batch_solve_equals_individual_solves(gpu):
         assert abs(c1 - c0) <= 1e-12 * abs(c0)
 
 
-def test_batch_simp_equals_individual_loops(gpu):
+@pytest.mark.parametrize("mg", [False, True])
+def test_batch_simp_equals_individual_loops(gpu, mg):
     from icarat2_amd import workloads as W
+    pre = gpu.PRECOND_MG if mg else gpu.PRECOND_JACOBI
     specs = [_scaled(s, 2) for s in _batch_specs()]
     single = []
     for s in specs:
         c = W.build_context(gpu, s)
-        single.append(c.simp_run(W.initial_design(s), 3, tol=1e-9))
+        single.append(c.simp_run(W.initial_design(s), 3, tol=1e-9, precond=pre))
     ctxs = [W.build_context(gpu, s) for s in specs]
     b = gpu.Batch(ctxs)
-    xs, hists, sts = b.simp_run([W.initial_design(s) for s in specs], 3, tol=1e-9)
+    xs, hists, sts = b.simp_run([W.initial_design(s) for s in specs], 3, tol=1e-9, precond=pre)
     for (x1, h1, s1), x2, h2, s2 in zip(single, xs, hists, sts):
         print(f"iters alone {s1.pcg_iters} batched {s2.pcg_iters}; max design diff "
               f"{np.max(np.abs(x1 - x2)):.2e}")
