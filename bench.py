@@ -167,29 +167,37 @@ def cpu_baseline(cfg, pcg_iters_per_solve, n_cg=60):
 
 def run_batch(A, a):
     """The batched workload: 24 seeded problems (sizes 32^3..160x80x80, 1..4 load
-    cases each) in lock step; one captured graph per PCG round."""
+    cases each); each problem runs its own design loop and the running solves
+    share one captured graph per launch."""
     import numpy as np
     from icarat2_amd import workloads as W
+    pre = METHODS[a.method][1]
     specs = W.batch_workload()
     ctxs = [W.build_context(A, s) for s in specs]
     x0s = [W.initial_design(s) for s in specs]
     for c, s, x in zip(ctxs, specs, x0s):
         c.set_density(c.density_filter(x))
     b = A.Batch(ctxs)
+    if a.budget is not None:
+        b.set_budget(a.budget)
     log("[bench] autotune")
     cfg, cfg_ms = b.autotune(reps=3)
-    if a.warmup > 0:
-        log(f"[bench] warmup {a.warmup} batched SIMP iteration(s)")
-        b.simp_run(x0s, a.warmup, tol=a.tol, maxit=100000)
-    log(f"[bench] timed {a.steps} batched SIMP iteration(s) of {len(specs)} problems")
-    t0 = time.perf_counter()
-    xs, hists, sts = b.simp_run(x0s, a.steps, tol=a.tol, maxit=100000)
-    dt = time.perf_counter() - t0
+
+    def timed(precond):
+        if a.warmup > 0:
+            log(f"[bench] warmup {a.warmup} batched SIMP iteration(s), precond {precond}")
+            b.simp_run(x0s, a.warmup, tol=a.tol, maxit=100000, precond=precond)
+        log(f"[bench] timed {a.steps} batched SIMP iteration(s) of {len(specs)} problems")
+        t0 = time.perf_counter()
+        out = b.simp_run(x0s, a.steps, tol=a.tol, maxit=100000, precond=precond)
+        return out, time.perf_counter() - t0
+
+    (xs, hists, sts), dt = timed(pre)
     # the same problems one after the other (each alone on the GPU)
     log("[bench] sequential reference: each problem alone")
     t1 = time.perf_counter()
     for c, x in zip(ctxs, x0s):
-        c.simp_run(x, a.steps, tol=a.tol, maxit=100000)
+        c.simp_run(x, a.steps, tol=a.tol, maxit=100000, precond=pre)
     dt_seq = time.perf_counter() - t1
     iters = [float(h[:, 2].sum()) / a.steps for h in hists]  # PCG iterations per design iteration
     out = {
@@ -200,16 +208,25 @@ def run_batch(A, a):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (seeded problem specs, uniform initial designs)",
         "config": {"workload": "c5: 24 independent problems (seeds 100..123), 32^3..160x80x80, "
-                               "1-4 face load cases each, lock-step SIMP with OC",
+                               "1-4 face load cases each, one design loop per problem, OC",
+                   "method": a.method,
                    "elements": int(sum(c.nelx for c in ctxs)),
                    "load_cases": int(sum(len(s["loads"]) for s in specs)),
-                   "autotune_config": cfg, "autotune_ms_per_launch": cfg_ms},
+                   "autotune_config": cfg, "autotune_ms_per_launch": cfg_ms,
+                   "admission_budget": a.budget if a.budget is not None else "default"},
         "sequential_value": len(specs) * a.steps / dt_seq,
         "batch_speedup_vs_sequential": dt_seq / dt,
         "max_true_relres": max(st.max_true_relres for st in sts),
         "pcg_iters_per_design_iter": iters,
         "graph_launches": b.info()[1],
     }
+    if pre != 0 and not a.no_plain:
+        (xp, hp, sp), dtp = timed(0)
+        out["plain_fp64"] = {
+            "method": "fp64 (Jacobi-PCG), batched", "value": len(specs) * a.steps / dtp,
+            "max_true_relres": max(st.max_true_relres for st in sp),
+            "design_max_abs_diff": float(max(np.max(np.abs(x1 - x2)) for x1, x2 in zip(xs, xp)))}
+        out["speedup_vs_plain_fp64"] = out["value"] / out["plain_fp64"]["value"]
     return out
 
 
@@ -225,6 +242,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--reps", type=int, default=20, help="launches timed for the roofline")
+    ap.add_argument("--budget", type=int, default=None,
+                    help="c5: admission budget in elements (0: all problems at once)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; the multi-GPU (slab "
@@ -237,7 +256,8 @@ def main():
     cfg = CONFIGS[a.config]
     if a.config == "c5":
         if a.method not in ("fp64", "mg"):
-            print("bench.py: the batched workload runs FP64 Jacobi-PCG", file=sys.stderr)
+            print("bench.py: the batched workload runs FP64 Jacobi-PCG or multigrid PCG",
+                  file=sys.stderr)
             sys.exit(2)
         print(json.dumps(run_batch(A, a)), flush=True)
         return
