@@ -144,7 +144,8 @@ __global__ __launch_bounds__(256) void k_mg_restrict(Lattice Lf, Lattice Lc,
   }
 }
 
-// xf += P xc (trilinear), masked by the fine free DOFs
+// xf += P xc (ADD) or xf = P xc (trilinear), masked by the fine free DOFs
+template <bool ADD>
 __global__ __launch_bounds__(256) void k_mg_prolong_add(Lattice Lf, Lattice Lc,
                                                         const double *__restrict__ xc,
                                                         const uint8_t *__restrict__ fmask_f,
@@ -162,8 +163,10 @@ __global__ __launch_bounds__(256) void k_mg_prolong_add(Lattice Lf, Lattice Lc,
   unsigned m = 0;
   if (own) {
     m = fmask_f[n];
+    if constexpr (ADD) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) xo[c] = xf[c * Lf.Np + n];
+      for (int c = 0; c < 3; ++c) xo[c] = xf[c * Lf.Np + n];
+    }
   }
   for (int t = threadIdx.x; t < PCN; t += 256) {
     const int hx = t % PCX, hy = (t / PCX) % PCY, hz = t / (PCX * PCY);
@@ -371,11 +374,14 @@ void launch_mg_restrict(const Lattice &Lf, const Lattice &Lc, const double *rf, 
 }
 
 void launch_mg_prolong_add(const Lattice &Lf, const Lattice &Lc, const double *xc,
-                           const uint8_t *fmask_f, double *xf, hipStream_t s) {
+                           const uint8_t *fmask_f, double *xf, hipStream_t s, bool add) {
   if (Lf.NX != 2 * Lc.NX - 1 || Lf.NY != 2 * Lc.NY - 1 || Lf.NZ != 2 * Lc.NZ - 1)
     throw Error(kInvalidArgument, "prolongation: lattices are not 2x apart");
   const dim3 g((Lf.NX + PTX - 1) / PTX, (Lf.NY + PTY - 1) / PTY, (Lf.NZ + PTZ - 1) / PTZ);
-  hipLaunchKernelGGL(k_mg_prolong_add, g, dim3(256), 0, s, Lf, Lc, xc, fmask_f, xf);
+  if (add)
+    hipLaunchKernelGGL(k_mg_prolong_add<true>, g, dim3(256), 0, s, Lf, Lc, xc, fmask_f, xf);
+  else
+    hipLaunchKernelGGL(k_mg_prolong_add<false>, g, dim3(256), 0, s, Lf, Lc, xc, fmask_f, xf);
   HIPCHK(hipGetLastError());
 }
 

@@ -14,9 +14,10 @@ namespace icarat_hip {
 void launch_mg_restrict(const Lattice &Lf, const Lattice &Lc, const double *rf, const double *yf,
                         const uint8_t *fmask_c, double *bc, const double *dinv_c, double omega,
                         double *xc, hipStream_t s);
-// xf += P xc (trilinear interpolation), masked
+// xf += P xc (trilinear interpolation), masked; add = false: xf = P xc at the
+// free DOFs (constrained DOFs keep their value)
 void launch_mg_prolong_add(const Lattice &Lf, const Lattice &Lc, const double *xc,
-                           const uint8_t *fmask_f, double *xf, hipStream_t s);
+                           const uint8_t *fmask_f, double *xf, hipStream_t s, bool add = true);
 // coarse Young's moduli: mean over the 8 children
 void launch_mg_young(const Lattice &Lf, const Lattice &Lc, const double *yf, double *yc,
                      hipStream_t s);

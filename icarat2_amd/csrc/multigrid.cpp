@@ -158,6 +158,12 @@ void Multigrid::refresh() {
 // V-cycle on level l; every launch on the fine context's stream.  Smoothing
 // ping-pongs between the level's x and y buffers where the fused sweep runs
 // (the z-marching path smooths x in place, with y holding K.x).
+bool Multigrid::starts_swept(int l) const {
+  const bool coarsest = l == levels() - 1;
+  if (coarsest) return !dense_;
+  return o_.nu_pre > 0;
+}
+
 double *Multigrid::cycle(int l, const double *b, bool emit, bool x0_ready, ZDot *zd) {
   hipStream_t s = f_.stream();
   Context &c = level(l);
@@ -182,19 +188,22 @@ double *Multigrid::cycle(int l, const double *b, bool emit, bool x0_ready, ZDot 
     }
   };
   // the first sweep from zero (levels >= 1: written by the restriction)
-  if (l == 0 && emit && !x0_ready) launch_mg_jacobi(n, w, c.d_dinv, b, nullptr, x, s);
+  if (l == 0 && emit && !x0_ready && starts_swept(0)) launch_mg_jacobi(n, w, c.d_dinv, b, nullptr, x, s);
   if (l == levels() - 1) {
     for (int i = 1; i < std::max(1, o_.nu_coarse); ++i) sweep();
     return cur;
   }
-  for (int i = 1; i < std::max(1, o_.nu_pre); ++i) sweep();
+  const int pre = std::max(0, o_.nu_pre);
+  for (int i = 1; i < pre; ++i) sweep();
   Context &cc = level(l + 1);
   if (emit) {
     // the coarse level's first sweep from zero comes with its right-hand side
-    // (none before an exact coarse solve)
-    const bool exact = l + 1 == levels() - 1 && dense_;
-    const double *dinv_c = exact ? nullptr : cc.d_dinv.p;
-    if (direct_[l]) {
+    const double *dinv_c = starts_swept(l + 1) ? cc.d_dinv.p : nullptr;
+    if (pre == 0) {
+      // no pre-smoothing: x = 0, the residual is b
+      launch_mg_restrict(L, cc.lattice(), b, nullptr, cc.d_free, cc.d_b, dinv_c, omega_[l + 1],
+                         cc.d_x, s);
+    } else if (direct_[l]) {
       launch_mg_direct(MG_RESIDUAL, L, c.kc_, c.d_young, c.d_free, cur, b, nullptr, 0.0, r, s);
       launch_mg_restrict(L, cc.lattice(), r, nullptr, cc.d_free, cc.d_b, dinv_c, omega_[l + 1],
                          cc.d_x, s);
@@ -205,7 +214,8 @@ double *Multigrid::cycle(int l, const double *b, bool emit, bool x0_ready, ZDot 
     }
   }
   const double *xc = cycle(l + 1, cc.d_b, emit);
-  if (emit) launch_mg_prolong_add(L, cc.lattice(), xc, c.d_free, cur, s);
+  // x = 0 without pre-smoothing: the correction is the iterate
+  if (emit) launch_mg_prolong_add(L, cc.lattice(), xc, c.d_free, cur, s, pre > 0);
   const int post = std::max(0, o_.nu_post);
   for (int i = 0; i < post; ++i) {
     if (i == post - 1 && l == 0 && zd && !direct_[l]) {
@@ -226,8 +236,8 @@ void Multigrid::enqueue_batch(int par) {
   hipStream_t s = c.stream();
   const Lattice &L = c.lattice();
   // the fine level's first sweep comes out of the update unless the fine level
-  // is itself solved exactly
-  const bool x0_fused = !(levels() == 1 && dense_);
+  // does not start with one (solved exactly, or no pre-smoothing)
+  const bool x0_fused = starts_swept(0);
   for (int j = 0; j < std::max(1, o_.batch_iters); ++j) {
     double *p_old = c.d_p[(j + par) & 1], *p_new = c.d_p[(j + par + 1) & 1];
     // p = z + beta p_old, q = K p, p.q partials

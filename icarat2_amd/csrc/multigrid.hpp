@@ -89,6 +89,9 @@ class Multigrid {
   double *cycle(int l, const double *b, bool emit, bool x0_ready = false, ZDot *zd = nullptr);
   void build_graph();
   void enqueue_restart();                      // restart flag, z = M(r), beta = 0
+  // whether level l's V-cycle starts with the sweep x = omega D^-1 b (a
+  // smoothed coarsest level, or any other level with pre-smoothing)
+  bool starts_swept(int l) const;
   unsigned long long refreshed_ = ~0ULL;       // density version of the coarse operators
   hipEvent_t ev_[2] = {};                      // ordering between stream s and the fine stream
 };

@@ -324,7 +324,9 @@ int icarat_hip_batch_autotune(icarat_hip_batch *b, int32_t reps, int32_t *config
  * defaults when icarat_hip_mg_setup was not called. */
 typedef struct icarat_mg_opts {
   int32_t max_levels;      /* levels including the fine grid */
-  int32_t nu_pre, nu_post; /* damped Jacobi sweeps before / after the coarse correction */
+  int32_t nu_pre, nu_post; /* damped Jacobi sweeps before (>= 0) / after (>= 1) the coarse
+                              correction; nu_pre != nu_post gives a non-symmetric V-cycle,
+                              which the flexible PCG accepts */
   int32_t nu_coarse;       /* sweeps on the coarsest grid (unless it is solved exactly) */
   double omega;            /* Jacobi damping relative to 1 / mu, mu = lambda_max(diag(KE0)^-1
                               KE0) >= lambda_max(D^-1 K) for any densities; in (0, 2) */
