@@ -73,8 +73,7 @@ __device__ __forceinline__ void direct_out(long long Np, long long n, unsigned m
   for (int c = 0; c < 3; ++c) {
     const long long q = c * Np + n;
     double v;
-    if constexpr (MODE == MG_APPLY) v = y[c];
-    else if constexpr (MODE == MG_JACOBI) v = xc[c] + omega * dinv[q] * (b[q] - y[c]);
+    if constexpr (MODE == MG_JACOBI) v = xc[c] + omega * dinv[q] * (b[q] - y[c]);
     else v = b[q] - y[c];
     out[q] = ((m >> c) & 1u) ? v : 0.0;
   }
@@ -254,13 +253,6 @@ __global__ __launch_bounds__(256) void k_mg_jacobi_dot(long long n, double omega
   store_partials<256, 2>(v, part, sh);
 }
 
-// r = b - y
-__global__ void k_mg_residual(long long n, const double *__restrict__ b, const double *__restrict__ y,
-                              double *__restrict__ r) {
-  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (i < n) r[i] = b[i] - y[i];
-}
-
 template <int SYM, int MODE>
 __global__ __launch_bounds__(256) void k_mg_direct(Lattice L, const double *__restrict__ c_KE0,
                                                    KeVals kv, const double *__restrict__ young,
@@ -345,10 +337,7 @@ void direct_dispatch(int mode, const Lattice &L, const KeCoef &kc, const double 
                      const uint8_t *fmask, const double *x, const double *b, const double *dinv,
                      double omega, double *out, hipStream_t s) {
   const dim3 g(blocks_for(L.numnp, 256)), t(256);
-  if (mode == MG_APPLY)
-    hipLaunchKernelGGL((k_mg_direct<SYM, MG_APPLY>), g, t, 0, s, L, kc.ke, kc.vals, young, fmask, x,
-                       b, dinv, omega, out);
-  else if (mode == MG_JACOBI)
+  if (mode == MG_JACOBI)
     hipLaunchKernelGGL((k_mg_direct<SYM, MG_JACOBI>), g, t, 0, s, L, kc.ke, kc.vals, young, fmask,
                        x, b, dinv, omega, out);
   else
@@ -435,15 +424,10 @@ unsigned launch_mg_jacobi_dot(long long n, double omega, const double *dinv, con
   return DOT_BLOCKS;
 }
 
-void launch_mg_residual(long long n, const double *b, const double *y, double *r, hipStream_t s) {
-  hipLaunchKernelGGL(k_mg_residual, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, b, y, r);
-  HIPCHK(hipGetLastError());
-}
-
 void launch_mg_direct(int mode, const Lattice &L, const KeCoef &kc, const double *young,
                       const uint8_t *fmask, const double *x, const double *b, const double *dinv,
                       double omega, double *out, hipStream_t s) {
-  if (mode < MG_APPLY || mode > MG_RESIDUAL) throw Error(kInvalidArgument, "unknown direct mode");
+  if (mode != MG_JACOBI && mode != MG_RESIDUAL) throw Error(kInvalidArgument, "unknown direct mode");
   if (kc.sym == 2)
     direct_dispatch<2>(mode, L, kc, young, fmask, x, b, dinv, omega, out, s);
   else if (kc.sym == 1)

@@ -24,7 +24,6 @@ void launch_mg_young(const Lattice &Lf, const Lattice &Lc, const double *yf, dou
 // damped Jacobi: y == nullptr: x = omega dinv b; else x += omega dinv (b - y)
 void launch_mg_jacobi(long long n, double omega, const double *dinv, const double *b,
                       const double *y, double *x, hipStream_t s);
-void launch_mg_residual(long long n, const double *b, const double *y, double *r, hipStream_t s);
 // x += omega dinv (b - y) and the partials of x.b and x.q (the flexible PCG's
 // z.r and z.q when this is a V-cycle's last sweep on r); returns their count
 unsigned launch_mg_jacobi_dot(long long n, double omega, const double *dinv, const double *b,
@@ -33,8 +32,7 @@ unsigned launch_mg_jacobi_dot(long long n, double omega, const double *dinv, con
 
 // One thread per node, K.x gathered from global memory (for coarse lattices,
 // where a z-marching launch is latency-bound), with the consumer fused in:
-enum MgDirectMode { MG_APPLY = 0, MG_JACOBI = 1, MG_RESIDUAL = 2 };
-//   MG_APPLY:    out = K x
+enum MgDirectMode { MG_JACOBI = 1, MG_RESIDUAL = 2 };
 //   MG_JACOBI:   out = x + omega dinv (b - K x)   (out may not alias x)
 //   MG_RESIDUAL: out = b - K x
 // every output masked by the free DOFs

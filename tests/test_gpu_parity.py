@@ -6,7 +6,7 @@ seeded inputs.  Tolerances (FP64 path):
   compliance ................. 1e-8 relative (solves at the configs' 1e-8)
   sensitivities .............. 1e-8 relative (solves at 1e-11), 1e-12 given the same u
   OC / MMA update ............ 1e-12 / 1e-9 given the same inputs
-  SIMP design after 20 its ... 1e-6 (the north-star bar for design iterates)
+  SIMP design after 20 its ... 1e-6 (the tolerance on design iterates)
 """
 import numpy as np
 import pytest
