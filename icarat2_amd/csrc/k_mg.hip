@@ -203,7 +203,9 @@ __global__ __launch_bounds__(256) void k_mg_prolong_add(Lattice Lf, Lattice Lc,
     if ((m >> c) & 1u) xf[c * Lf.Np + n] = xo[c] + s[c];
 }
 
-// coarse element = mean of its 8 children
+// coarse element from its 8 children: arithmetic mean (MODE 0) or geometric
+// mean (MODE 1) of the values
+template <int MODE>
 __global__ void k_mg_young(Lattice Lf, Lattice Lc, const double *__restrict__ yf,
                            double *__restrict__ yc) {
   const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -213,9 +215,10 @@ __global__ void k_mg_young(Lattice Lf, Lattice Lc, const double *__restrict__ yf
 #pragma unroll
   for (int k = 0; k < 8; ++k) {
     const int x = 2 * I + (k & 1), y = 2 * J + ((k >> 1) & 1), z = 2 * K + (k >> 2);
-    s += yf[x + (long long)Lf.nx * (y + (long long)Lf.ny * z)];
+    const double v = yf[x + (long long)Lf.nx * (y + (long long)Lf.ny * z)];
+    s += MODE == 1 ? log(v) : v;
   }
-  yc[e] = 0.125 * s;
+  yc[e] = MODE == 1 ? exp(0.125 * s) : 0.125 * s;
 }
 
 // x = omega dinv b (first sweep from zero) or x += omega dinv (b - y)
@@ -404,8 +407,11 @@ void launch_mg_dense_solve(const Lattice &L, const double *ainv, const double *b
 }
 
 void launch_mg_young(const Lattice &Lf, const Lattice &Lc, const double *yf, double *yc,
-                     hipStream_t s) {
-  hipLaunchKernelGGL(k_mg_young, dim3(blocks_for(Lc.nelx, 256)), dim3(256), 0, s, Lf, Lc, yf, yc);
+                     hipStream_t s, bool geometric) {
+  if (geometric)
+    hipLaunchKernelGGL(k_mg_young<1>, dim3(blocks_for(Lc.nelx, 256)), dim3(256), 0, s, Lf, Lc, yf, yc);
+  else
+    hipLaunchKernelGGL(k_mg_young<0>, dim3(blocks_for(Lc.nelx, 256)), dim3(256), 0, s, Lf, Lc, yf, yc);
   HIPCHK(hipGetLastError());
 }
 

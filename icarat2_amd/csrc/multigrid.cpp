@@ -127,9 +127,16 @@ Context &Multigrid::level(int l) { return l == 0 ? f_ : *lv_[l - 1]; }
 
 void Multigrid::refresh() {
   hipStream_t s = f_.stream();
+  const GridSpec &g = f_.problem().spec();
   for (int l = 1; l < levels(); ++l) {
     Context &fc = level(l - 1), &cc = level(l);
-    launch_mg_young(fc.lattice(), cc.lattice(), fc.d_young, cc.d_young, s);
+    if (o_.coarse_avg == 2) {
+      // the SIMP law of the mean density
+      launch_mg_young(fc.lattice(), cc.lattice(), fc.d_dens, cc.d_dens, s, false);
+      launch_young(cc.d_dens, cc.d_young, cc.lattice().nelx, g.young1, g.young2, g.penalty, s);
+    } else {
+      launch_mg_young(fc.lattice(), cc.lattice(), fc.d_young, cc.d_young, s, o_.coarse_avg == 1);
+    }
     launch_diag_inv(cc.lattice(), cc.d_ke, cc.d_young, cc.d_free, cc.d_dinv, nullptr, s);
   }
   if (dense_) {

@@ -99,6 +99,23 @@ def test_mg_nonsymmetric_cycles(gpu, shape):
     assert np.max(np.abs(um - uj)) < 1e-6 * np.max(np.abs(uj))
 
 
+@pytest.mark.parametrize("avg", [1, 2])
+def test_mg_coarse_averaging_rules(gpu, avg):
+    """Coarse Young's moduli by the geometric mean or by the SIMP law of the mean
+    density (the default is the arithmetic mean): other coarse operators, the
+    same verified solve (1e-6 of max |u| of Jacobi-PCG, both at 1e-9)."""
+    dims = (48, 16, 16)
+    c = gpu.cantilever(*dims)
+    x, _ = gpu.gaussian_field(*dims, seed=13, smin=2.0, smax=6.0)
+    c.set_density(x)
+    uj, _ = c.solve(tol=1e-9)
+    c.mg_setup(coarse_avg=avg)
+    um, sm = c.solve(tol=1e-9, precond=gpu.PRECOND_MG)
+    print(f"coarse_avg {avg}: {sm.iters} iterations")
+    assert sm.status == 0 and sm.true_relres <= 1e-9
+    assert np.max(np.abs(um - uj)) < 1e-6 * np.max(np.abs(uj))
+
+
 def test_mg_simp_matches_jacobi(gpu):
     dims = (60, 20, 20)
     x0 = np.full(24000, 0.3)
