@@ -203,8 +203,8 @@ __global__ __launch_bounds__(256) void k_mg_prolong_add(Lattice Lf, Lattice Lc,
     if ((m >> c) & 1u) xf[c * Lf.Np + n] = xo[c] + s[c];
 }
 
-// coarse element from its 8 children: arithmetic mean (MODE 0) or geometric
-// mean (MODE 1) of the values
+// coarse element from its 8 children: arithmetic mean (MODE 0), geometric mean
+// (1), quadratic mean (3) or maximum (4) of the values
 template <int MODE>
 __global__ void k_mg_young(Lattice Lf, Lattice Lc, const double *__restrict__ yf,
                            double *__restrict__ yc) {
@@ -216,9 +216,15 @@ __global__ void k_mg_young(Lattice Lf, Lattice Lc, const double *__restrict__ yf
   for (int k = 0; k < 8; ++k) {
     const int x = 2 * I + (k & 1), y = 2 * J + ((k >> 1) & 1), z = 2 * K + (k >> 2);
     const double v = yf[x + (long long)Lf.nx * (y + (long long)Lf.ny * z)];
-    s += MODE == 1 ? log(v) : v;
+    if constexpr (MODE == 1) s += log(v);
+    else if constexpr (MODE == 3) s += v * v;
+    else if constexpr (MODE == 4) s = fmax(s, v);
+    else s += v;
   }
-  yc[e] = MODE == 1 ? exp(0.125 * s) : 0.125 * s;
+  if constexpr (MODE == 1) yc[e] = exp(0.125 * s);
+  else if constexpr (MODE == 3) yc[e] = sqrt(0.125 * s);
+  else if constexpr (MODE == 4) yc[e] = s;
+  else yc[e] = 0.125 * s;
 }
 
 // x = omega dinv b (first sweep from zero) or x += omega dinv (b - y)
@@ -407,11 +413,14 @@ void launch_mg_dense_solve(const Lattice &L, const double *ainv, const double *b
 }
 
 void launch_mg_young(const Lattice &Lf, const Lattice &Lc, const double *yf, double *yc,
-                     hipStream_t s, bool geometric) {
-  if (geometric)
-    hipLaunchKernelGGL(k_mg_young<1>, dim3(blocks_for(Lc.nelx, 256)), dim3(256), 0, s, Lf, Lc, yf, yc);
-  else
-    hipLaunchKernelGGL(k_mg_young<0>, dim3(blocks_for(Lc.nelx, 256)), dim3(256), 0, s, Lf, Lc, yf, yc);
+                     hipStream_t s, int mode) {
+  const dim3 g(blocks_for(Lc.nelx, 256)), t(256);
+  switch (mode) {
+    case 1: hipLaunchKernelGGL(k_mg_young<1>, g, t, 0, s, Lf, Lc, yf, yc); break;
+    case 3: hipLaunchKernelGGL(k_mg_young<3>, g, t, 0, s, Lf, Lc, yf, yc); break;
+    case 4: hipLaunchKernelGGL(k_mg_young<4>, g, t, 0, s, Lf, Lc, yf, yc); break;
+    default: hipLaunchKernelGGL(k_mg_young<0>, g, t, 0, s, Lf, Lc, yf, yc); break;
+  }
   HIPCHK(hipGetLastError());
 }
 

@@ -18,10 +18,10 @@ void launch_mg_restrict(const Lattice &Lf, const Lattice &Lc, const double *rf, 
 // free DOFs (constrained DOFs keep their value)
 void launch_mg_prolong_add(const Lattice &Lf, const Lattice &Lc, const double *xc,
                            const uint8_t *fmask_f, double *xf, hipStream_t s, bool add = true);
-// coarse element values from the 8 children: arithmetic mean, or geometric
-// mean (geometric = true)
+// coarse element values from the 8 children: mode 0 arithmetic mean, 1
+// geometric mean, 3 quadratic mean, 4 maximum
 void launch_mg_young(const Lattice &Lf, const Lattice &Lc, const double *yf, double *yc,
-                     hipStream_t s, bool geometric = false);
+                     hipStream_t s, int mode = 0);
 // damped Jacobi: y == nullptr: x = omega dinv b; else x += omega dinv (b - y)
 void launch_mg_jacobi(long long n, double omega, const double *dinv, const double *b,
                       const double *y, double *x, hipStream_t s);

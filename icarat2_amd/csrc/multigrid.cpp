@@ -132,10 +132,10 @@ void Multigrid::refresh() {
     Context &fc = level(l - 1), &cc = level(l);
     if (o_.coarse_avg == 2) {
       // the SIMP law of the mean density
-      launch_mg_young(fc.lattice(), cc.lattice(), fc.d_dens, cc.d_dens, s, false);
+      launch_mg_young(fc.lattice(), cc.lattice(), fc.d_dens, cc.d_dens, s, 0);
       launch_young(cc.d_dens, cc.d_young, cc.lattice().nelx, g.young1, g.young2, g.penalty, s);
     } else {
-      launch_mg_young(fc.lattice(), cc.lattice(), fc.d_young, cc.d_young, s, o_.coarse_avg == 1);
+      launch_mg_young(fc.lattice(), cc.lattice(), fc.d_young, cc.d_young, s, o_.coarse_avg);
     }
     launch_diag_inv(cc.lattice(), cc.d_ke, cc.d_young, cc.d_free, cc.d_dinv, nullptr, s);
   }

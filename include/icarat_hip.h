@@ -338,7 +338,7 @@ typedef struct icarat_mg_opts {
                               one-thread-per-node sweep; larger ones with the z-marching K.u */
   int32_t coarse_avg;      /* a coarse element's Young's modulus from its 8 children:
                               0 arithmetic mean, 1 geometric mean, 2 the SIMP law of the
-                              mean density */
+                              mean density, 3 quadratic mean, 4 maximum */
 } icarat_mg_opts;
 /* Fills *o with the defaults (10, 1, 1, 20, 1.8, 4, 1024, 262144). */
 int icarat_hip_mg_default_opts(icarat_mg_opts *o);

@@ -99,11 +99,12 @@ def test_mg_nonsymmetric_cycles(gpu, shape):
     assert np.max(np.abs(um - uj)) < 1e-6 * np.max(np.abs(uj))
 
 
-@pytest.mark.parametrize("avg", [1, 2])
+@pytest.mark.parametrize("avg", [1, 2, 3, 4])
 def test_mg_coarse_averaging_rules(gpu, avg):
-    """Coarse Young's moduli by the geometric mean or by the SIMP law of the mean
-    density (the default is the arithmetic mean): other coarse operators, the
-    same verified solve (1e-6 of max |u| of Jacobi-PCG, both at 1e-9)."""
+    """Coarse Young's moduli by the geometric mean, the SIMP law of the mean
+    density, the quadratic mean or the maximum of the children (the default is
+    the arithmetic mean): other coarse operators, the same verified solve (1e-6
+    of max |u| of Jacobi-PCG, both at 1e-9)."""
     dims = (48, 16, 16)
     c = gpu.cantilever(*dims)
     x, _ = gpu.gaussian_field(*dims, seed=13, smin=2.0, smax=6.0)
