@@ -28,7 +28,7 @@ class GridDesc(C.Structure):
 class SolveOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("maxit", C.c_int32), ("precision", C.c_int32),
                 ("precond", C.c_int32), ("inner_tol", C.c_double), ("max_refine", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("warm", C.c_int32)]
 
 
 class SolveStats(C.Structure):
@@ -377,8 +377,8 @@ class Context:
         return d
 
     def solve(self, tol=1e-8, maxit=10000, precision=FP64, precond=PRECOND_JACOBI,
-              inner_tol=0.1, max_refine=200, want_u=True, allow_cap=False):
-        o = SolveOpts(tol, maxit, precision, precond, inner_tol, max_refine, 0)
+              inner_tol=0.1, max_refine=200, want_u=True, allow_cap=False, warm=False):
+        o = SolveOpts(tol, maxit, precision, precond, inner_tol, max_refine, 1 if warm else 0)
         st = SolveStats()
         u = np.zeros(self.numeq) if want_u else None
         rc = self.L.icarat_hip_solve(self.h, C.byref(o), None if u is None else _p(u), C.byref(st))
@@ -508,9 +508,10 @@ class Context:
 
     def simp_run(self, x0, n_iter, optimizer=0, use_projection=False, xmin=1e-4, xmax=1.0,
                  tol=1e-8, maxit=10000, precision=FP64, precond=PRECOND_JACOBI, inner_tol=0.1,
-                 max_refine=200):
+                 max_refine=200, warm=False):
         o = SimpOpts(n_iter, optimizer, 1 if use_projection else 0, 0, xmin, xmax,
-                     SolveOpts(tol, maxit, precision, precond, inner_tol, max_refine, 0))
+                     SolveOpts(tol, maxit, precision, precond, inner_tol, max_refine,
+                               1 if warm else 0))
         x = np.array(x0, dtype=np.float64)
         hist = np.zeros(4 * max(n_iter, 1))
         st = SimpStats()
@@ -572,8 +573,8 @@ class Batch:
         self._chk(self.L.icarat_hip_batch_info(self.h, C.byref(n), C.byref(k)), "batch_info")
         return n.value, k.value
 
-    def solve(self, tol=1e-8, maxit=10000, allow_cap=False, precond=PRECOND_JACOBI):
-        o = SolveOpts(tol, maxit, FP64, precond, 0.1, 200, 0)
+    def solve(self, tol=1e-8, maxit=10000, allow_cap=False, precond=PRECOND_JACOBI, warm=False):
+        o = SolveOpts(tol, maxit, FP64, precond, 0.1, 200, 1 if warm else 0)
         st = (SolveStats * len(self.ctx))()
         rc = self.L.icarat_hip_batch_solve(self.h, C.byref(o), st)
         if rc and not (allow_cap and rc == NOT_CONVERGED):
@@ -581,9 +582,9 @@ class Batch:
         return list(st)
 
     def simp_run(self, x0s, n_iter, optimizer=0, use_projection=False, xmin=1e-4, xmax=1.0,
-                 tol=1e-8, maxit=10000, precond=PRECOND_JACOBI):
+                 tol=1e-8, maxit=10000, precond=PRECOND_JACOBI, warm=False):
         o = SimpOpts(n_iter, optimizer, 1 if use_projection else 0, 0, xmin, xmax,
-                     SolveOpts(tol, maxit, FP64, precond, 0.1, 200, 0))
+                     SolveOpts(tol, maxit, FP64, precond, 0.1, 200, 1 if warm else 0))
         xs = [np.array(x, dtype=np.float64) for x in x0s]
         hs = [np.zeros(4 * max(n_iter, 1)) for _ in xs]
         xp = (dp * len(xs))(*[_p(x) for x in xs])

@@ -183,7 +183,7 @@ void Batch::solve(const icarat_solve_opts &o, icarat_solve_stats *st) {
         if (started[i]) return false;
         started[i] = 1;
         t0[i] = now_s();
-        ctx_[i]->pcg_start(o.tol, o.maxit, ms_);
+        ctx_[i]->pcg_start(o.tol, o.maxit, ms_, o.warm != 0);
         return true;
       },
       [&](int i, const icarat_solve_stats &s) {
@@ -216,7 +216,7 @@ void Batch::simp_run(const icarat_simp_opts &o, double *const *x, double *const 
         ++lc[i];
         if (nc > 1) c.use_load_case(lc[i]);
         t_solve[i] = now_s();
-        c.pcg_start(o.solve.tol, o.solve.maxit, ms_);
+        c.pcg_start(o.solve.tol, o.solve.maxit, ms_, o.solve.warm != 0);
         return true;
       },
       [&](int i, const icarat_solve_stats &s) {

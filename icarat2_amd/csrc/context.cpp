@@ -168,6 +168,7 @@ void Context::sync_fext() {
     launch_numeq_to_lattice(L_, d_dof, numeq, d_numeq[0], nullptr, d_fcases_[i], stream_);
   }
   d2d(d_f.p, d_fcases_[prob_.load_case()].p, n3_, stream_);
+  active_case_ = prob_.load_case();
   sync();
   solved_ = false;
   fext_dirty_ = false;
@@ -177,7 +178,26 @@ void Context::use_load_case(int i) {
   ensure_fext();
   if (i < 0 || i >= (int)d_fcases_.size()) throw Error(kInvalidArgument, "no such load case");
   d2d(d_f.p, d_fcases_[i].p, n3_, stream_);
+  active_case_ = i;
   solved_ = false;
+}
+
+bool Context::load_warm_guess(hipStream_t s) {
+  const int c = active_case_;
+  if (c >= (int)xcase_valid_.size() || !xcase_valid_[c]) return false;
+  d2d(d_x.p, d_xcases_[c].p, n3_, s);
+  return true;
+}
+
+void Context::save_solution(hipStream_t s) {
+  const int c = active_case_;
+  if (c >= (int)d_xcases_.size()) {
+    d_xcases_.resize(c + 1);
+    xcase_valid_.resize(c + 1, 0);
+  }
+  if (!d_xcases_[c].p) d_xcases_[c].alloc(n3_);
+  d2d(d_xcases_[c].p, d_x.p, n3_, s);
+  xcase_valid_[c] = 1;
 }
 
 int Context::new_load_case() {
@@ -415,7 +435,8 @@ void Context::solve(const icarat_solve_opts &o, double *u_host, icarat_solve_sta
   if (o.precond == ICARAT_HIP_PRECOND_JACOBI) {
     const int kmode = o.precision == ICARAT_HIP_FP64 ? -1 : (o.precision == ICARAT_HIP_FP32 ? 0 : 1);
     const double delta = kmode < 0 ? 0.0 : (o.inner_tol > 0.0 && o.inner_tol < 1.0 ? o.inner_tol : 0.1);
-    PcgScalars r = pcg64(o.tol, o.maxit, false, kmode, delta);
+    const bool warm = o.warm && kmode < 0 && load_warm_guess(stream_);
+    PcgScalars r = pcg64(o.tol, o.maxit, warm, kmode, delta);
     S.iters = r.iter;
     S.refine_steps = replacements_;
     S.fallback = fell_back_;
@@ -427,12 +448,13 @@ void Context::solve(const icarat_solve_opts &o, double *u_host, icarat_solve_sta
     learned_->solve(o, S);
   } else if (o.precond == ICARAT_HIP_PRECOND_MG) {
     if (!mg_) mg_setup(mg_default_opts());
-    mg_->solve(o, S);
+    mg_->solve(o, S, o.warm && load_warm_guess(stream_));
   } else {
     throw Error(kInvalidArgument, "unknown preconditioner");
   }
   assemble_nodal_solution();
   S.true_relres = true_residual_device();
+  save_solution(stream_);
   S.seconds = now_s() - t0;
   copy_out_solution(u_host);
   if (st) *st = S;
@@ -458,7 +480,7 @@ void Context::prepare_rhs(hipStream_t s) {
 }
 
 // ---- pieces of the FP64 Jacobi-PCG driver for batched solves -----------------
-void Context::pcg_start(double tol, int maxit, hipStream_t s) {
+void Context::pcg_start(double tol, int maxit, hipStream_t s, bool warm) {
   require_density();
   ensure_fext();
   // work on this context's stream (load-case switch, density) precedes stream s
@@ -469,7 +491,7 @@ void Context::pcg_start(double tol, int maxit, hipStream_t s) {
   fell_back_ = 0;
   if (batch_precond_ == ICARAT_HIP_PRECOND_MG) {
     if (!mg_) mg_setup(mg_default_opts());
-    mg_->start(tol, maxit, s);
+    mg_->start(tol, maxit, s, warm && load_warm_guess(s));
     return;
   }
   if (batch_precond_ != ICARAT_HIP_PRECOND_JACOBI)
@@ -479,7 +501,15 @@ void Context::pcg_start(double tol, int maxit, hipStream_t s) {
   init.maxit = maxit;
   *h_pcg.p = init;
   h2d(d_pcg.p, h_pcg.p, 1, s);
+  const bool guess = warm && load_warm_guess(s);
+  if (guess) d2d(d_nv.p, d_x.p, n3_, s);
+  // x = 0, r = b, p = 0, bnorm2 = b.b; with a guess, r = b - K x replaces r
   launch_pcg_init64(n3_, d_b, d_x, d_r, d_p[0], d_dinv, d_pcg, d_partials, d_counter, s);
+  if (guess) {
+    d2d(d_x.p, d_nv.p, n3_, s);
+    launch_apply_k64(L_, kc_, d_young, d_free, d_x, d_y, s);
+    launch_replace64(n3_, d_b, d_y, d_r, d_dinv, d_pcg, d_partials, s);
+  }
   d2h(h_pcg.p, d_pcg.p, 1, s);
 }
 
@@ -506,6 +536,7 @@ void Context::pcg_finish(icarat_solve_stats &S) {
   if (r.status == 2) throw Error(kNumerical, "PCG breakdown (non-positive curvature or NaN)");
   assemble_nodal_solution();
   S.true_relres = true_residual_device();
+  save_solution(stream_);
   solved_ = true;
 }
 

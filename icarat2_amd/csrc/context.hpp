@@ -163,11 +163,19 @@ class Context {
   // stats).  The iterations themselves are enqueue_pcg_batch(-1, 0).  The
   // preconditioner is batch_precond_: Jacobi, or the multigrid V-cycle (set up
   // with the defaults on first use).
-  void pcg_start(double tol, int maxit, hipStream_t s);
+  void pcg_start(double tol, int maxit, hipStream_t s, bool warm = false);
   void pcg_verify(hipStream_t s);
   void pcg_finish(icarat_solve_stats &S);
   void enqueue_pcg_batch(int kmode, int par);  // kBatch iterations on the stream
   int batch_precond_ = ICARAT_HIP_PRECOND_JACOBI;
+  // Warm starts: the last solution of each load case.  load_warm_guess copies
+  // the active case's into d_x (false: none, d_x untouched); save_solution
+  // records d_x for the active case.
+  bool load_warm_guess(hipStream_t s);
+  void save_solution(hipStream_t s);
+  int active_case_ = 0;                   // load case whose fext is in d_f
+  std::vector<DevBuf<double>> d_xcases_;
+  std::vector<char> xcase_valid_;
   // bumped whenever the densities change (the multigrid hierarchy refreshes
   // its coarse operators once per version)
   unsigned long long density_version_ = 0;

@@ -289,7 +289,7 @@ void Multigrid::enqueue_restart() {
   launch_zdot_beta(c.n3_, zres_, c.d_r, c.d_q, c.d_pcg, c.d_partials, s);
 }
 
-void Multigrid::start(double tol, int maxit, hipStream_t s_ext) {
+void Multigrid::start(double tol, int maxit, hipStream_t s_ext, bool warm) {
   Context &c = f_;
   hipStream_t s = c.stream();
   for (hipEvent_t &e : ev_)
@@ -308,9 +308,17 @@ void Multigrid::start(double tol, int maxit, hipStream_t s_ext) {
   s0.maxit = maxit;
   *h = s0;
   HIPCHK(hipMemcpyAsync(c.d_pcg.p, h, sizeof(PcgScalars), hipMemcpyHostToDevice, s));
+  // x = 0, r = b, bnorm2 = b.b; with a guess, r = b - K x and the stop test on it
+  if (warm) HIPCHK(hipMemcpyAsync(c.d_nv.p, c.d_x.p, c.d_x.bytes(), hipMemcpyDeviceToDevice, s));
   launch_pcg_init64(c.n3_, c.d_b, c.d_x, c.d_r, c.d_p[0], c.d_dinv, c.d_pcg, c.d_partials,
                     c.d_counter, s);
-  // z = M(r), rho = z.r, beta = 0 (a zero right-hand side is done already)
+  if (warm) {
+    HIPCHK(hipMemcpyAsync(c.d_x.p, c.d_nv.p, c.d_x.bytes(), hipMemcpyDeviceToDevice, s));
+    launch_apply_k64(c.lattice(), c.kc_, c.d_young, c.d_free, c.d_x, c.d_y, s);
+    launch_replace64(c.n3_, c.d_b, c.d_y, c.d_r, c.d_dinv, c.d_pcg, c.d_partials, s);
+  }
+  // z = M(r), rho = z.r, beta = 0 (a zero right-hand side, or a guess that
+  // already passes, is done)
   enqueue_restart();
   HIPCHK(hipMemcpyAsync(h, c.d_pcg.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));
@@ -335,12 +343,12 @@ void Multigrid::verify(hipStream_t s_ext) {
   }
 }
 
-void Multigrid::solve(const icarat_solve_opts &o, icarat_solve_stats &st) {
+void Multigrid::solve(const icarat_solve_opts &o, icarat_solve_stats &st, bool warm) {
   if (o.precision != ICARAT_HIP_FP64)
     throw Error(kInvalidArgument, "the multigrid preconditioner runs the FP64 contraction");
   Context &c = f_;
   hipStream_t s = c.stream();
-  start(o.tol, o.maxit, s);
+  start(o.tol, o.maxit, s, warm);
   build_graph();
   PcgScalars *h = c.h_pcg.p;
   const double bnorm = std::sqrt(h->bnorm2);

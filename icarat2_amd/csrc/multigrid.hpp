@@ -42,17 +42,19 @@ class Multigrid {
   Context &level(int l);
   // coarse Young's moduli and Jacobi diagonals after the fine density changed
   void refresh();
-  // K u = b of the fine context (d_b -> d_x) by PCG with M = V-cycle
-  void solve(const icarat_solve_opts &o, icarat_solve_stats &st);
+  // K u = b of the fine context (d_b -> d_x) by PCG with M = V-cycle; warm:
+  // from the guess in d_x instead of 0
+  void solve(const icarat_solve_opts &o, icarat_solve_stats &st, bool warm = false);
   // The solve in pieces (the fine context's d_b must be ready), each ordered
   // after the work on stream s and before later work on s:
   // start: coarse operators refreshed if the densities changed, PCG state
-  // initialised, z = M(r) with beta = 0, scalars to the host copy (blocks);
+  // initialised (from 0, or with warm from the guess in d_x: r = b - K x),
+  // z = M(r) with beta = 0, scalars to the host copy (blocks);
   // enqueue_batch(par): batch_iters iterations on the fine context's stream
   // (p buffers at parity par); verify: the FP64 true residual replaces the
   // recursive one and a restart z = M(r), beta = 0 follows, scalars to the
   // host copy once s has completed.
-  void start(double tol, int maxit, hipStream_t s);
+  void start(double tol, int maxit, hipStream_t s, bool warm = false);
   void enqueue_batch(int par);
   void verify(hipStream_t s);
   void drop_graphs();

@@ -71,7 +71,9 @@ typedef struct icarat_solve_opts {
                              residual replaces the FP32 recursive one whenever the latter
                              has dropped by delta since the last replacement (default 0.1) */
   int32_t max_refine;     /* mixed precision: cap on residual replacements (default 200) */
-  int32_t reserved;
+  int32_t warm;           /* 1: start the iteration from this load case's last solution on
+                             this context (if any) instead of 0 -- same stop test; the
+                             reference starts from 0 (FP64 Jacobi and multigrid PCG) */
 } icarat_solve_opts;
 
 typedef struct icarat_solve_stats {
