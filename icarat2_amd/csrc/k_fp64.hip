@@ -326,13 +326,8 @@ __device__ __forceinline__ void layer_contract(const double *__restrict__ c_KE0,
   }
 }
 
-// Residency: the PCG variants hold p_old and D^-1 in their fetch registers and
-// run 2 workgroups per CU; the plain K.u runs 3 (a few spilled values are
-// cheaper than the lost latency hiding).
-constexpr int ZWG_PCG = 2, ZWG_APPLY = 3;
-
 template <bool PCG, int SYM, bool FUSE>
-__global__ __launch_bounds__(NT, PCG ? ZWG_PCG : ZWG_APPLY) void k_stencil64z(
+__global__ __launch_bounds__(NT) void k_stencil64z(
     Lattice L, int nchunks, const double *__restrict__ c_KE0, KeVals kv, const double *__restrict__ young,
     const uint8_t *__restrict__ fmask, const double *__restrict__ src,
     const double *__restrict__ dinv, const double *__restrict__ p_old, double *__restrict__ p_new,
@@ -674,7 +669,7 @@ static dim3 stencil_grid(const Lattice &L) {
 // Number of balanced z-chunks of the z-marching kernel: as many as make the
 // columns x chunks one round of resident workgroups (2 per CU at the kernel's
 // register use), at least 2 node planes per chunk.
-static int compute_units() {
+static int resident_workgroups() {
   static int cus = 0;
   if (cus == 0) {
     int dev = 0;
@@ -684,16 +679,13 @@ static int compute_units() {
               ? prop.multiProcessorCount
               : 256;
   }
-  return cus;
+  return 2 * cus;
 }
 
-// z-chunks per column so that the grid is one round of resident workgroups
-static int zchunks(const Lattice &L, int per_cu) {
+int stencil_nchunks(const Lattice &L) {
   const int ncols = ((L.NX + ZX - 1) / ZX) * ((L.NY + ZY - 1) / ZY);
-  return std::max(1, std::min(L.NZ / 2, per_cu * compute_units() / ncols));
+  return std::max(1, std::min(L.NZ / 2, resident_workgroups() / ncols));
 }
-
-int stencil_nchunks(const Lattice &L) { return zchunks(L, ZWG_PCG); }
 
 static dim3 zgrid(const Lattice &L, int nchunks) {
   const int ncols = ((L.NX + ZX - 1) / ZX) * ((L.NY + ZY - 1) / ZY);
@@ -737,10 +729,8 @@ static void launch_stencil64_t(const Lattice &L, const KeCoef &ke, const double 
                                double *partials, const double *pu, unsigned Gu, int j,
                                hipStream_t s, unsigned *nblocks) {
   if (ke.kernel == 0) {
-    // tuning (ke.nchunks) is given against the PCG variant's default
     const int def = stencil_nchunks(L);
-    const int base = PCG ? def : zchunks(L, ZWG_APPLY);
-    const int nch = ke.nchunks > 0 ? std::max(1, (int)((long long)std::min(ke.nchunks, def) * base / def)) : base;
+    const int nch = ke.nchunks > 0 ? std::min(ke.nchunks, def) : def;
     const dim3 g = zgrid(L, nch);
     hipLaunchKernelGGL((k_stencil64z<PCG, SYM, FUSE>), g, dim3(NT), 0, s, L, nch, ke.ke, ke.vals,
                        young, freemask, src, dinv, p_old, p_new, y, S, partials, pu, Gu, j);
