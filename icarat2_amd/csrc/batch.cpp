@@ -16,8 +16,8 @@ Batch::Batch() { HIPCHK(hipStreamCreateWithFlags(&ms_, hipStreamNonBlocking)); }
 
 Batch::~Batch() {
   if (ms_) (void)hipStreamSynchronize(ms_);
-  drop_graph();
   for (hipEvent_t e : ev_) (void)hipEventDestroy(e);
+  if (fork_) (void)hipEventDestroy(fork_);
   if (ms_) (void)hipStreamDestroy(ms_);
 }
 
@@ -28,53 +28,33 @@ void Batch::add(Context *c) {
   if (!c->problem().dof_ready()) throw Error(kNotReady, "make_dof has not been called");
   c->ensure_device();
   ctx_.push_back(c);
-  drop_graph();
 }
 
-void Batch::drop_graph() {
-  for (auto &kv : graphs_) {
-    if (kv.second.second) (void)hipGraphExecDestroy(kv.second.second);
-    if (kv.second.first) (void)hipGraphDestroy(kv.second.first);
-  }
-  graphs_.clear();
-}
-
-hipGraphExec_t Batch::graph_for(const std::vector<char> &run) {
-  // captured branches depend on the preconditioner of each context
-  std::string key(run.begin(), run.end());
-  for (Context *c : ctx_) key.push_back((char)c->batch_precond_);
-  auto it = graphs_.find(key);
-  if (it != graphs_.end()) return it->second.second;
+// One batch of iterations of every running problem: each problem's own
+// captured graph on its own stream (separate hardware queues run them
+// concurrently), after the work queued on the batch stream, with its scalars
+// copied back; the batch stream then waits for every problem.
+void Batch::launch_running(const std::vector<char> &run) {
   const int n = size();
-  while ((int)ev_.size() < 2 * n) {
+  if (!fork_) HIPCHK(hipEventCreateWithFlags(&fork_, hipEventDisableTiming));
+  while ((int)ev_.size() < n) {
     hipEvent_t e;
     HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     ev_.push_back(e);
   }
-  hipGraph_t g = nullptr;
-  hipGraphExec_t ge = nullptr;
-  HIPCHK(hipStreamBeginCapture(ms_, hipStreamCaptureModeGlobal));
-  try {
-    for (int i = 0; i < n; ++i) {
-      if (!run[i]) continue;
-      // fork: problem i's iterations on its own stream after the batch stream
-      HIPCHK(hipEventRecord(ev_[2 * i], ms_));
-      HIPCHK(hipStreamWaitEvent(ctx_[i]->stream(), ev_[2 * i], 0));
-      ctx_[i]->enqueue_pcg_batch(-1, 0);
-      // join back into the batch stream
-      HIPCHK(hipEventRecord(ev_[2 * i + 1], ctx_[i]->stream()));
-      HIPCHK(hipStreamWaitEvent(ms_, ev_[2 * i + 1], 0));
-    }
-  } catch (...) {
-    hipGraph_t gg = nullptr;
-    (void)hipStreamEndCapture(ms_, &gg);
-    if (gg) (void)hipGraphDestroy(gg);
-    throw;
+  // graphs are captured before any stream is made to wait
+  for (int i = 0; i < n; ++i)
+    if (run[i]) ctx_[i]->build_solver_graph();
+  HIPCHK(hipEventRecord(fork_, ms_));
+  for (int i = 0; i < n; ++i) {
+    if (!run[i]) continue;
+    Context &c = *ctx_[i];
+    HIPCHK(hipStreamWaitEvent(c.stream(), fork_, 0));
+    c.launch_solver_batch(c.h_pcg.p->iter & 1);
+    HIPCHK(hipMemcpyAsync(c.h_pcg.p, c.d_pcg.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, c.stream()));
+    HIPCHK(hipEventRecord(ev_[i], c.stream()));
+    HIPCHK(hipStreamWaitEvent(ms_, ev_[i], 0));
   }
-  HIPCHK(hipStreamEndCapture(ms_, &g));
-  HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-  graphs_[key] = {g, ge};
-  return ge;
 }
 
 void Batch::engine(double tol, int maxit, const std::function<bool(int)> &start,
@@ -146,20 +126,8 @@ void Batch::engine(double tol, int maxit, const std::function<bool(int)> &start,
     for (int i = 0; i < n; ++i)
       if (run[i] && ctx_[i]->h_pcg.p->done) pending = true;
     if (pending) continue;
-    // the graph starts every branch at p-buffer parity 0
-    for (int i = 0; i < n; ++i) {
-      if (!run[i]) continue;
-      Context &c = *ctx_[i];
-      if (c.h_pcg.p->iter & 1)
-        HIPCHK(hipMemcpyAsync(c.d_p[0].p, c.d_p[1].p, c.d_p[0].bytes(), hipMemcpyDeviceToDevice, ms_));
-    }
-    HIPCHK(hipGraphLaunch(graph_for(run), ms_));
+    launch_running(run);
     ++launches_;
-    for (int i = 0; i < n; ++i) {
-      if (!run[i]) continue;
-      Context &c = *ctx_[i];
-      HIPCHK(hipMemcpyAsync(c.h_pcg.p, c.d_pcg.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, ms_));
-    }
     HIPCHK(hipStreamSynchronize(ms_));
   }
 }
@@ -243,7 +211,6 @@ void Batch::apply_config(int cfg) {
     else
       c->set_stencil_tuning(0, std::max(1, stencil_nchunks(c->lattice()) >> cfg));
   }
-  drop_graph();
 }
 
 int Batch::autotune(int reps, double *ms_out) {
@@ -263,10 +230,11 @@ int Batch::autotune(int reps, double *ms_out) {
     // launches time real iterations on the current densities
     for (Context *c : ctx_) c->sync();
     for (Context *c : ctx_) c->pcg_start(1e-300, 1 << 30, ms_);
-    hipGraphExec_t ge = graph_for(std::vector<char>(n, 1));
-    HIPCHK(hipGraphLaunch(ge, ms_));  // warm
+    const std::vector<char> all(n, 1);
+    launch_running(all);  // warm (graphs built)
+    HIPCHK(hipStreamSynchronize(ms_));
     HIPCHK(hipEventRecord(a, ms_));
-    for (int r = 0; r < reps; ++r) HIPCHK(hipGraphLaunch(ge, ms_));
+    for (int r = 0; r < reps; ++r) launch_running(all);
     HIPCHK(hipEventRecord(b, ms_));
     HIPCHK(hipEventSynchronize(b));
     float t = 0.f;

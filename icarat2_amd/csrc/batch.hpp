@@ -1,11 +1,10 @@
 // batch.hpp -- several independent SIMP problems (contexts of any size) driven
 // together.  Each problem advances through its own design loop; whenever
-// problems have a solve running, their PCG iterations are launched together as
-// one graph: one branch per running problem on that problem's stream, forked
+// problems have a solve running, their PCG iterations are launched together:
+// each running problem's own captured graph on that problem's stream, forked
 // from and joined into the batch stream through events, so the problems'
-// kernels run concurrently and small problems share the GPU with large ones.
-// A graph is captured once per set of running problems and reused.  Every
-// solve keeps its own FP64 verified stop.
+// kernels run concurrently on separate hardware queues and small problems
+// share the GPU with large ones.  Every solve keeps its own FP64 verified stop.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -37,7 +36,7 @@ class Batch {
   // phases between its solves), the solves of all problems share the launches
   void simp_run(const icarat_simp_opts &o, double *const *x, double *const *hist,
                 icarat_simp_stats *st);
-  // Times whole-batch graph launches under each stencil configuration (applied
+  // Times whole-batch launches under each stencil configuration (applied
   // to every context: z-marching with 1/1, 1/2 or 1/4 of the default z-chunks,
   // or 3-D tiles) and keeps the fastest.  Needs densities on every context.
   // Returns the chosen configuration and the time of one launch per
@@ -52,15 +51,14 @@ class Batch {
  private:
   std::vector<Context *> ctx_;
   hipStream_t ms_ = nullptr;
-  std::vector<hipEvent_t> ev_;
-  // one graph per set of problems with a running solve (key: membership bytes)
-  std::map<std::string, std::pair<hipGraph_t, hipGraphExec_t>> graphs_;
+  std::vector<hipEvent_t> ev_;  // per problem: its batch of iterations done
+  hipEvent_t fork_ = nullptr;   // the batch stream's work before the launches
   long long launches_ = 0;
-  // two problems that each fill the GPU (measured on the 24-problem workload:
-  // 800k..1.5M elements within 1% of each other, all-at-once 2% slower)
-  long long budget_ = 1500000;
-  void drop_graph();
-  hipGraphExec_t graph_for(const std::vector<char> &run);
+  // no limit: on the 24-problem workload all problems at once ran 1.3x faster
+  // than one after another, against 1.2x with a 3M-element budget and 1.1x
+  // with 1.5M
+  long long budget_ = 0;
+  void launch_running(const std::vector<char> &run);
   // The solve engine: start(i) begins problem i's next solve (pcg_start on the
   // batch stream) and returns false when problem i has no more solves; done(i,
   // st) receives a finished solve.  Iterates every running solve in shared

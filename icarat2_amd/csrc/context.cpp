@@ -260,11 +260,29 @@ void Context::get_diag(double *d) {
   sync();
 }
 
-void Context::enqueue_pcg_batch(int kmode, int par) {
-  if (batch_precond_ == ICARAT_HIP_PRECOND_MG) {
+void Context::enqueue_solver_batch(int par) {
+  if (batch_precond_ == ICARAT_HIP_PRECOND_MG)
     mg_->enqueue_batch(par);
-    return;
-  }
+  else
+    enqueue_pcg_batch(-1, par);
+}
+
+void Context::build_solver_graph() {
+  if (batch_precond_ == ICARAT_HIP_PRECOND_MG)
+    mg_->build_graph();
+  else
+    build_pcg_graph(-1);
+}
+
+void Context::launch_solver_batch(int par) {
+  build_solver_graph();
+  if (batch_precond_ == ICARAT_HIP_PRECOND_MG)
+    mg_->launch_batch(par);
+  else
+    launch_pcg_batch(-1, par);
+}
+
+void Context::enqueue_pcg_batch(int kmode, int par) {
   if (kmode < 0) {
     // FP64: the finalizes are folded into the next kernel; one at the end of
     // the batch leaves the scalars complete for the host

@@ -160,13 +160,18 @@ class Context {
   // FP64 PCG in pieces for a driver that interleaves several contexts: start
   // (rhs, init, scalars to h_pcg), verify (true residual replaces the
   // recursive one, scalars to h_pcg), finish (nodal solution, true residual,
-  // stats).  The iterations themselves are enqueue_pcg_batch(-1, 0).  The
-  // preconditioner is batch_precond_: Jacobi, or the multigrid V-cycle (set up
-  // with the defaults on first use).
+  // stats).  The iterations themselves are enqueue_solver_batch /
+  // launch_solver_batch.  The preconditioner is batch_precond_: Jacobi, or the
+  // multigrid V-cycle (set up with the defaults on first use).
   void pcg_start(double tol, int maxit, hipStream_t s, bool warm = false);
   void pcg_verify(hipStream_t s);
   void pcg_finish(icarat_solve_stats &S);
-  void enqueue_pcg_batch(int kmode, int par);  // kBatch iterations on the stream
+  void enqueue_pcg_batch(int kmode, int par);  // kBatch Jacobi-PCG iterations on the stream
+  // one batch of iterations of the interleaved solve (batch_precond_): enqueued
+  // (for a caller's capture), or launched as this context's own graph
+  void enqueue_solver_batch(int par);
+  void launch_solver_batch(int par);
+  void build_solver_graph();  // captures the graphs launch_solver_batch uses
   int batch_precond_ = ICARAT_HIP_PRECOND_JACOBI;
   // Warm starts: the last solution of each load case.  load_warm_guess copies
   // the active case's into d_x (false: none, d_x untouched); save_solution

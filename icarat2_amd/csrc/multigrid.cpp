@@ -263,6 +263,14 @@ void Multigrid::enqueue_batch(int par) {
   }
 }
 
+void Multigrid::launch_batch(int par) {
+  build_graph();
+  if (f_.use_graphs_)
+    HIPCHK(hipGraphLaunch(exec_[par & 1], f_.stream()));
+  else
+    enqueue_batch(par & 1);
+}
+
 void Multigrid::build_graph() {
   if (!f_.use_graphs_ || exec_[0]) return;
   hipStream_t s = f_.stream();

@@ -56,6 +56,10 @@ class Multigrid {
   // host copy once s has completed.
   void start(double tol, int maxit, hipStream_t s, bool warm = false);
   void enqueue_batch(int par);
+  // enqueue_batch(par) through this hierarchy's captured graph (built on first
+  // use) on the fine context's stream
+  void launch_batch(int par);
+  void build_graph();  // the two parity graphs (no-op if built or graphs are off)
   void verify(hipStream_t s);
   void drop_graphs();
   // whether the coarsest level is solved exactly (dense inverse)
@@ -89,7 +93,6 @@ class Multigrid {
   // (written by the restriction) and for l = 0 when x0_ready.  emit = false
   // only follows the buffers.
   double *cycle(int l, const double *b, bool emit, bool x0_ready = false, ZDot *zd = nullptr);
-  void build_graph();
   void enqueue_restart();                      // restart flag, z = M(r), beta = 0
   // whether level l's V-cycle starts with the sweep x = omega D^-1 b (a
   // smoothed coarsest level, or any other level with pre-smoothing)

@@ -295,7 +295,7 @@ const char *icarat_hip_batch_last_error(const icarat_hip_batch *b);
 int icarat_hip_batch_add(icarat_hip_batch *b, icarat_hip_ctx *ctx);
 /* Admission control: problems join the running set largest first while the
  * running problems' elements stay within `elements` (at least one runs; 0: all
- * run at once); an admitted problem runs its whole loop.  Default 1,500,000. */
+ * run at once); an admitted problem runs its whole loop.  Default 0. */
 int icarat_hip_batch_set_budget(icarat_hip_batch *b, int64_t elements);
 /* problems in the batch and graph launches so far */
 int icarat_hip_batch_info(const icarat_hip_batch *b, int32_t *n, int64_t *graph_launches);
