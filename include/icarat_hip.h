@@ -280,9 +280,10 @@ int icarat_hip_unet_train_step(icarat_hip_unet *net, const float *x, const float
 /* ---- batched problems ------------------------------------------------------------
  * No reference counterpart (the reference runs one problem per process): several
  * independent problems of any size, each a prepared context (mesh, supports,
- * load cases, filter), solved together.  Per solve round the PCG iterations of
- * every problem are one captured graph: a branch per problem on its context's
- * stream, so the problems' kernels run concurrently; each problem keeps its own
+ * load cases, filter), solved together.  Per launch every running problem's own
+ * captured graph (a batch of its iterations) goes on its context's stream, forked
+ * from and joined into the batch stream, so the problems run concurrently on
+ * separate hardware queues; each problem keeps its own
  * FP64 verified stop.  Batched solves are FP64 Jacobi-PCG or FP64 multigrid PCG
  * (opts precond ICARAT_HIP_PRECOND_MG; each problem's hierarchy is set up with
  * the defaults unless icarat_hip_mg_setup was called on its context).  A context
@@ -297,7 +298,7 @@ int icarat_hip_batch_add(icarat_hip_batch *b, icarat_hip_ctx *ctx);
  * running problems' elements stay within `elements` (at least one runs; 0: all
  * run at once); an admitted problem runs its whole loop.  Default 0. */
 int icarat_hip_batch_set_budget(icarat_hip_batch *b, int64_t elements);
-/* problems in the batch and graph launches so far */
+/* problems in the batch and launches (one graph per running problem each) so far */
 int icarat_hip_batch_info(const icarat_hip_batch *b, int32_t *n, int64_t *graph_launches);
 /* one solve of every problem's current load case; stats[n] (may be NULL) */
 int icarat_hip_batch_solve(icarat_hip_batch *b, const icarat_solve_opts *opts,
@@ -307,8 +308,8 @@ int icarat_hip_batch_solve(icarat_hip_batch *b, const icarat_solve_opts *opts,
  * itself NULL) 4 * n_iter per problem, stats[n] (may be NULL) */
 int icarat_hip_batch_simp_run(icarat_hip_batch *b, const icarat_simp_opts *opts,
                               double *const *x, double *const *hist, icarat_simp_stats *stats);
-/* Autotuner: times reps whole-batch graph launches (16 PCG iterations of every
- * problem) under each stencil configuration -- 0..2: z-marching columns with
+/* Autotuner: times reps whole-batch launches (a batch of Jacobi-PCG iterations of
+ * every problem) under each stencil configuration -- 0..2: z-marching columns with
  * the default z-chunk count divided by 1, 2, 4; 3: 3-D halo tiles -- and keeps
  * the fastest for every problem.  Needs densities on every problem; overwrites
  * the solver state.  *config = the chosen configuration, ms[4] (may be NULL) the
