@@ -77,6 +77,8 @@ Multigrid::Multigrid(Context &fine, const icarat_mg_opts &o) : f_(fine), o_(o) {
     if (dense_) {
       ainv_.alloc(nd * nd);
       info_.alloc(1);
+      HIPCHK(hipHostMalloc((void **)&h_info_, sizeof(int), hipHostMallocDefault));
+      *h_info_ = 0;
       if (rocblas_create_handle(&blas_) != rocblas_status_success)
         throw Error(kInternal, "rocblas_create_handle failed");
     }
@@ -109,6 +111,7 @@ Multigrid::Multigrid(Context &fine, const icarat_mg_opts &o) : f_(fine), o_(o) {
 
 Multigrid::~Multigrid() {
   drop_graphs();
+  if (h_info_) (void)hipHostFree(h_info_);
   if (blas_) (void)rocblas_destroy_handle(blas_);
   for (hipEvent_t e : ev_)
     if (e) (void)hipEventDestroy(e);
@@ -150,13 +153,10 @@ void Multigrid::refresh() {
         throw Error(kInternal, std::string(what) + " failed: " + rocblas_status_to_string(st));
     };
     chk(rocblas_set_stream(blas_, s), "rocblas_set_stream");
+    // the factorization's status is read once the solve has synchronised
+    // (check_factor): a failed factor only makes the V-cycle's output useless
     chk(rocsolver_dpotrf(blas_, rocblas_fill_lower, nd, ainv_.p, nd, info_.p), "rocsolver_dpotrf");
-    int info = 0;
-    HIPCHK(hipMemcpyAsync(&info, info_.p, sizeof(int), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    if (info != 0)
-      throw Error(kNumerical, "coarsest multigrid matrix is not positive definite (Cholesky pivot " +
-                                  std::to_string(info) + ")");
+    HIPCHK(hipMemcpyAsync(h_info_, info_.p, sizeof(int), hipMemcpyDeviceToHost, s));
     chk(rocsolver_dpotri(blas_, rocblas_fill_lower, nd, ainv_.p, nd, info_.p), "rocsolver_dpotri");
     launch_mg_dense_symmetrize(nd, ainv_, s);
   }
@@ -329,7 +329,16 @@ void Multigrid::start(double tol, int maxit, hipStream_t s_ext, bool warm) {
   // already passes, is done)
   enqueue_restart();
   HIPCHK(hipMemcpyAsync(h, c.d_pcg.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
-  HIPCHK(hipStreamSynchronize(s));
+  if (s_ext != s) {
+    HIPCHK(hipEventRecord(ev_[1], s));
+    HIPCHK(hipStreamWaitEvent(s_ext, ev_[1], 0));
+  }
+}
+
+void Multigrid::check_factor() const {
+  if (dense_ && *h_info_ != 0)
+    throw Error(kNumerical, "coarsest multigrid matrix is not positive definite (Cholesky pivot " +
+                                std::to_string(*h_info_) + ")");
 }
 
 void Multigrid::verify(hipStream_t s_ext) {
@@ -358,6 +367,8 @@ void Multigrid::solve(const icarat_solve_opts &o, icarat_solve_stats &st, bool w
   hipStream_t s = c.stream();
   start(o.tol, o.maxit, s, warm);
   build_graph();
+  c.sync();
+  check_factor();
   PcgScalars *h = c.h_pcg.p;
   const double bnorm = std::sqrt(h->bnorm2);
   const int batch = std::max(1, o_.batch_iters);

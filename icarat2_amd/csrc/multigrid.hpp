@@ -49,7 +49,7 @@ class Multigrid {
   // after the work on stream s and before later work on s:
   // start: coarse operators refreshed if the densities changed, PCG state
   // initialised (from 0, or with warm from the guess in d_x: r = b - K x),
-  // z = M(r) with beta = 0, scalars to the host copy (blocks);
+  // z = M(r) with beta = 0, scalars to the host copy once s has completed;
   // enqueue_batch(par): batch_iters iterations on the fine context's stream
   // (p buffers at parity par); verify: the FP64 true residual replaces the
   // recursive one and a restart z = M(r), beta = 0 follows, scalars to the
@@ -61,6 +61,9 @@ class Multigrid {
   void launch_batch(int par);
   void build_graph();  // the two parity graphs (no-op if built or graphs are off)
   void verify(hipStream_t s);
+  // throws if the last refresh's Cholesky factorization failed (call after the
+  // fine context's stream has completed the start)
+  void check_factor() const;
   void drop_graphs();
   // whether the coarsest level is solved exactly (dense inverse)
   bool dense() const { return dense_; }
@@ -77,6 +80,7 @@ class Multigrid {
   bool dense_ = false;
   DevBuf<double> ainv_;  // dense inverse of the coarsest K
   DevBuf<int> info_;
+  int *h_info_ = nullptr;  // pinned copy of the last factorization's status
   ::_rocblas_handle *blas_ = nullptr;
   double *zres_ = nullptr;  // buffer that holds the fine V-cycle's result
   hipGraph_t graph_[2] = {};
