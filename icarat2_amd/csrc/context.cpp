@@ -545,7 +545,7 @@ void Context::pcg_verify(hipStream_t s) {
 }
 
 void Context::pcg_finish(icarat_solve_stats &S) {
-  if (batch_precond_ == ICARAT_HIP_PRECOND_MG) mg_->check_factor();
+  if (batch_precond_ == ICARAT_HIP_PRECOND_MG) mg_->finish();
   const PcgScalars &r = *h_pcg.p;
   S = icarat_solve_stats{};
   S.iters = r.iter;

@@ -76,6 +76,10 @@ Multigrid::Multigrid(Context &fine, const icarat_mg_opts &o) : f_(fine), o_(o) {
     dense_ = nd <= dense_max;
     if (dense_) {
       ainv_.alloc(nd * nd);
+      anext_.alloc(nd * nd);
+      HIPCHK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+      HIPCHK(hipEventCreateWithFlags(&ev_side_, hipEventDisableTiming));
+      HIPCHK(hipEventCreateWithFlags(&ev_fact_, hipEventDisableTiming));
       info_.alloc(1);
       HIPCHK(hipHostMalloc((void **)&h_info_, sizeof(int), hipHostMallocDefault));
       *h_info_ = 0;
@@ -110,8 +114,12 @@ Multigrid::Multigrid(Context &fine, const icarat_mg_opts &o) : f_(fine), o_(o) {
 }
 
 Multigrid::~Multigrid() {
+  if (side_) (void)hipStreamSynchronize(side_);
   drop_graphs();
   if (h_info_) (void)hipHostFree(h_info_);
+  if (ev_side_) (void)hipEventDestroy(ev_side_);
+  if (ev_fact_) (void)hipEventDestroy(ev_fact_);
+  if (side_) (void)hipStreamDestroy(side_);
   if (blas_) (void)rocblas_destroy_handle(blas_);
   for (hipEvent_t e : ev_)
     if (e) (void)hipEventDestroy(e);
@@ -143,23 +151,62 @@ void Multigrid::refresh() {
     launch_diag_inv(cc.lattice(), cc.d_ke, cc.d_young, cc.d_free, cc.d_dinv, nullptr, s);
   }
   if (dense_) {
-    // A = K of the coarsest grid, then A <- A^-1 by Cholesky (K is SPD on the
-    // free DOFs; constrained DOFs carry identity rows)
-    Context &cc = level(levels() - 1);
-    const int nd = (int)(3 * cc.lattice().numnp);
-    launch_mg_dense_assemble(cc.lattice(), cc.d_ke, cc.d_young, cc.d_free, ainv_, s);
-    auto chk = [](rocblas_status st, const char *what) {
-      if (st != rocblas_status_success)
-        throw Error(kInternal, std::string(what) + " failed: " + rocblas_status_to_string(st));
-    };
-    chk(rocblas_set_stream(blas_, s), "rocblas_set_stream");
-    // the factorization's status is read once the solve has synchronised
-    // (check_factor): a failed factor only makes the V-cycle's output useless
-    chk(rocsolver_dpotrf(blas_, rocblas_fill_lower, nd, ainv_.p, nd, info_.p), "rocsolver_dpotrf");
-    HIPCHK(hipMemcpyAsync(h_info_, info_.p, sizeof(int), hipMemcpyDeviceToHost, s));
-    chk(rocsolver_dpotri(blas_, rocblas_fill_lower, nd, ainv_.p, nd, info_.p), "rocsolver_dpotri");
-    launch_mg_dense_symmetrize(nd, ainv_, s);
+    if (!have_inverse_) {
+      factor_into(ainv_, s);
+      have_inverse_ = true;
+      swap_pending_ = false;
+    } else {
+      // factored on the side stream while the first iterations run with the
+      // previous density's inverse; swapped in at a fixed batch (before_batch)
+      HIPCHK(hipEventRecord(ev_side_, s));
+      HIPCHK(hipStreamWaitEvent(side_, ev_side_, 0));
+      factor_into(anext_, side_);
+      HIPCHK(hipEventRecord(ev_fact_, side_));
+      swap_pending_ = true;
+    }
   }
+}
+
+// A = K of the coarsest grid (SPD on the free DOFs; constrained DOFs carry
+// identity rows), then dst = A^-1 by Cholesky, all on stream s.  The
+// factorization's status is read once the solve has synchronised
+// (check_factor): a failed factor only makes the V-cycle's output useless.
+void Multigrid::factor_into(double *dst, hipStream_t s) {
+  Context &cc = level(levels() - 1);
+  const int nd = (int)(3 * cc.lattice().numnp);
+  launch_mg_dense_assemble(cc.lattice(), cc.d_ke, cc.d_young, cc.d_free, dst, s);
+  auto chk = [](rocblas_status st, const char *what) {
+    if (st != rocblas_status_success)
+      throw Error(kInternal, std::string(what) + " failed: " + rocblas_status_to_string(st));
+  };
+  chk(rocblas_set_stream(blas_, s), "rocblas_set_stream");
+  chk(rocsolver_dpotrf(blas_, rocblas_fill_lower, nd, dst, nd, info_.p), "rocsolver_dpotrf");
+  HIPCHK(hipMemcpyAsync(h_info_, info_.p, sizeof(int), hipMemcpyDeviceToHost, s));
+  chk(rocsolver_dpotri(blas_, rocblas_fill_lower, nd, dst, nd, info_.p), "rocsolver_dpotri");
+  launch_mg_dense_symmetrize(nd, dst, s);
+}
+
+void Multigrid::swap_inverse(hipStream_t s) {
+  if (!swap_pending_) return;
+  HIPCHK(hipStreamWaitEvent(s, ev_fact_, 0));
+  HIPCHK(hipMemcpyAsync(ainv_.p, anext_.p, ainv_.bytes(), hipMemcpyDeviceToDevice, s));
+  swap_pending_ = false;
+}
+
+void Multigrid::before_batch(hipStream_t s) {
+  if (batches_ == kSwapAfter && swap_pending_) {
+    swap_inverse(s);
+    // the Krylov directions so far used the previous preconditioner: restart
+    // from z = M(r) with the new one (beta = 0)
+    enqueue_restart();
+  }
+  ++batches_;
+}
+
+void Multigrid::finish() {
+  swap_inverse(f_.stream());
+  f_.sync();
+  check_factor();
 }
 
 // V-cycle on level l; every launch on the fine context's stream.  Smoothing
@@ -265,6 +312,7 @@ void Multigrid::enqueue_batch(int par) {
 
 void Multigrid::launch_batch(int par) {
   build_graph();
+  before_batch(f_.stream());
   if (f_.use_graphs_)
     HIPCHK(hipGraphLaunch(exec_[par & 1], f_.stream()));
   else
@@ -306,7 +354,10 @@ void Multigrid::start(double tol, int maxit, hipStream_t s_ext, bool warm) {
     HIPCHK(hipEventRecord(ev_[0], s_ext));
     HIPCHK(hipStreamWaitEvent(s, ev_[0], 0));
   }
+  batches_ = 0;
   if (refreshed_ != c.density_version_) {
+    // a factorization still pending belongs to an older density
+    swap_inverse(s);
     refresh();
     refreshed_ = c.density_version_;
   }
@@ -368,7 +419,6 @@ void Multigrid::solve(const icarat_solve_opts &o, icarat_solve_stats &st, bool w
   start(o.tol, o.maxit, s, warm);
   build_graph();
   c.sync();
-  check_factor();
   PcgScalars *h = c.h_pcg.p;
   const double bnorm = std::sqrt(h->bnorm2);
   const int batch = std::max(1, o_.batch_iters);
@@ -377,6 +427,7 @@ void Multigrid::solve(const icarat_solve_opts &o, icarat_solve_stats &st, bool w
     if (!h->done) {
       const long long max_batches = (long long)(o.maxit - h->iter) / batch + 3;
       for (long long b = 0; b < max_batches && !h->done; ++b) {
+        before_batch(s);
         if (c.use_graphs_)
           HIPCHK(hipGraphLaunch(exec_[h->iter & 1], s));
         else
@@ -393,6 +444,7 @@ void Multigrid::solve(const icarat_solve_opts &o, icarat_solve_stats &st, bool w
     if (h->done) break;
     if (++repl > 50) break;
   }
+  finish();
   st.fallback = 0;
   st.refine_steps = repl;
   st.iters = h->iter;

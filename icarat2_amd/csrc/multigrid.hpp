@@ -61,9 +61,12 @@ class Multigrid {
   void launch_batch(int par);
   void build_graph();  // the two parity graphs (no-op if built or graphs are off)
   void verify(hipStream_t s);
-  // throws if the last refresh's Cholesky factorization failed (call after the
-  // fine context's stream has completed the start)
+  // throws if the last refresh's Cholesky factorization failed (call once the
+  // fine context's stream has completed the factorization, e.g. after finish)
   void check_factor() const;
+  // end of a solve: the new inverse in place (if still pending), the stream
+  // synchronised, the factorization's status checked
+  void finish();
   void drop_graphs();
   // whether the coarsest level is solved exactly (dense inverse)
   bool dense() const { return dense_; }
@@ -81,6 +84,20 @@ class Multigrid {
   DevBuf<double> ainv_;  // dense inverse of the coarsest K
   DevBuf<int> info_;
   int *h_info_ = nullptr;  // pinned copy of the last factorization's status
+  // Overlapped refresh: a new density's coarsest inverse is factored into
+  // anext_ on the side stream while the solve's first kSwapAfter batches run
+  // with the previous inverse, then copied into ainv_ before the next batch.
+  // The swap point is fixed, so every run takes the same path.
+  static constexpr int kSwapAfter = 2;
+  DevBuf<double> anext_;
+  hipStream_t side_ = nullptr;
+  hipEvent_t ev_side_ = nullptr, ev_fact_ = nullptr;
+  bool have_inverse_ = false;  // ainv_ holds the inverse of this or an earlier density
+  bool swap_pending_ = false;
+  int batches_ = 0;            // batches launched since the solve's start
+  void factor_into(double *dst, hipStream_t s);
+  void swap_inverse(hipStream_t s);  // ainv_ = anext_ once factored (if pending)
+  void before_batch(hipStream_t s);  // the swap at its fixed batch
   ::_rocblas_handle *blas_ = nullptr;
   double *zres_ = nullptr;  // buffer that holds the fine V-cycle's result
   hipGraph_t graph_[2] = {};
