@@ -39,7 +39,7 @@ class SolveStats(C.Structure):
 
 class SimpOpts(C.Structure):
     _fields_ = [("n_iter", C.c_int32), ("optimizer", C.c_int32), ("use_projection", C.c_int32),
-                ("reserved", C.c_int32), ("xmin", C.c_double), ("xmax", C.c_double),
+                ("filter", C.c_int32), ("xmin", C.c_double), ("xmax", C.c_double),
                 ("solve", SolveOpts)]
 
 
@@ -98,6 +98,8 @@ _SIGS = {
     "icarat_hip_compliance_sensitivity": (C.c_int, [C.c_void_p, C.c_double, dp, dp, dp]),
     "icarat_hip_filter_setup": (C.c_int, [C.c_void_p, C.c_double]),
     "icarat_hip_filter": (C.c_int, [C.c_void_p, dp, dp, C.c_int32]),
+    "icarat_hip_helm_setup": (C.c_int, [C.c_void_p, C.c_double]),
+    "icarat_hip_helm_filter": (C.c_int, [C.c_void_p, dp, dp, C.c_int32]),
     "icarat_hip_threshold": (C.c_int, [C.c_void_p, dp, C.c_double, C.c_double, dp, C.c_int32]),
     "icarat_hip_oc_update": (C.c_int, [C.c_void_p, dp, dp, C.c_double, dp, C.c_double,
                                        C.c_double, C.c_int32, C.c_double, i32p]),
@@ -409,6 +411,17 @@ class Context:
                   "filter")
         return out
 
+    def helm_setup(self, radius):
+        """FilterHelmholtz(fem, element, node, radius) on this box."""
+        self._chk(self.L.icarat_hip_helm_setup(self.h, radius), "helm_setup")
+
+    def helm_filter(self, x, is_sens=False):
+        """FilterHelmholtz::densityFilter(x, solver, log, is_sens)."""
+        out = np.zeros(self.nelx)
+        self._chk(self.L.icarat_hip_helm_filter(self.h, _p(_f64(x)), _p(out),
+                                                1 if is_sens else 0), "helm_filter")
+        return out
+
     def threshold(self, x, beta, T=0.5, deriv=False):
         out = np.zeros(self.nelx)
         self._chk(self.L.icarat_hip_threshold(self.h, _p(_f64(x)), beta, T, _p(out),
@@ -508,8 +521,9 @@ class Context:
 
     def simp_run(self, x0, n_iter, optimizer=0, use_projection=False, xmin=1e-4, xmax=1.0,
                  tol=1e-8, maxit=10000, precision=FP64, precond=PRECOND_JACOBI, inner_tol=0.1,
-                 max_refine=200, warm=False):
-        o = SimpOpts(n_iter, optimizer, 1 if use_projection else 0, 0, xmin, xmax,
+                 max_refine=200, warm=False, helmholtz=False):
+        o = SimpOpts(n_iter, optimizer, 1 if use_projection else 0, 1 if helmholtz else 0,
+                     xmin, xmax,
                      SolveOpts(tol, maxit, precision, precond, inner_tol, max_refine,
                                1 if warm else 0))
         x = np.array(x0, dtype=np.float64)

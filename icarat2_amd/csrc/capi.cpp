@@ -233,6 +233,15 @@ int icarat_hip_filter(icarat_hip_ctx *h, const double *in, double *out, int32_t 
   return guard(h, [&](icarat_hip::Context &c) { c.filter_host(in, out, is_sens ? 1 : 0); });
 }
 
+int icarat_hip_helm_setup(icarat_hip_ctx *h, double radius) {
+  return guard(h, [&](icarat_hip::Context &c) { c.helm_setup(radius); });
+}
+
+int icarat_hip_helm_filter(icarat_hip_ctx *h, const double *in, double *out, int32_t is_sens) {
+  if (!in || !out) return fail(h, ICARAT_HIP_INVALID_ARGUMENT, "null argument");
+  return guard(h, [&](icarat_hip::Context &c) { c.helm_host(in, out, is_sens ? 1 : 0); });
+}
+
 int icarat_hip_threshold(icarat_hip_ctx *h, const double *in, double beta, double T, double *out,
                          int32_t deriv) {
   if (!in || !out) return fail(h, ICARAT_HIP_INVALID_ARGUMENT, "null argument");

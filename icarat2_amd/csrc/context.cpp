@@ -11,6 +11,7 @@
 #include "learned.hpp"
 #include "multigrid.hpp"
 #include "mixed_kernels.hpp"
+#include "helm.hpp"
 #include "mma.hpp"
 
 namespace icarat_hip {
@@ -630,6 +631,53 @@ void Context::threshold_host(const double *in, double beta, double T, double *ou
   sync();
 }
 
+void Context::helm_setup(double radius) {
+  helm_.reset();
+  helm_.reset(new HelmFilter(*this, radius));
+}
+
+void Context::helm_host(const double *in, double *out, int is_sens) {
+  if (!helm_) throw Error(kNotReady, "helm_setup has not been called");
+  h2d(d_e[E_TMP].p, in, L_.nelx, stream_);
+  helm_->apply(d_e[E_TMP], d_e[E_XF], is_sens);
+  d2h(out, d_e[E_XF].p, L_.nelx, stream_);
+  sync();
+}
+
+void Context::loop_filter(const double *in, double *out, int is_sens) {
+  if (simp_.o.filter == 1) helm_->apply(in, out, is_sens);
+  else filter_device(in, out, is_sens);
+}
+
+int Context::oc_device_helm(double *x, const double *dc, const double *dv, double g, double xmin,
+                            double xmax, int use_projection, double beta) {
+  if (!helm_) throw Error(kNotReady, "helm_setup has not been called");
+  OcState s0{};
+  s0.l0 = 0.0;
+  s0.l1 = 1.0e4;
+  s0.g = g;
+  s0.maxiter = 100;
+  *h_oc.p = s0;
+  h2d(d_oc.p, h_oc.p, 1, stream_);
+  // x' at lambda -> E_TMP, filtered -> E_XP (both free while the design is
+  // updated), volume of its projection -> bisection update
+  constexpr int kSteps = 4;
+  for (int b = 0; b < 100 / kSteps + 2; ++b) {
+    for (int k = 0; k < kSteps; ++k) {
+      launch_oc_candidate(L_.nelx, x, dc, dv, xmin, xmax, d_oc, d_e[E_TMP], stream_);
+      helm_->apply(d_e[E_TMP], d_e[E_XP], 0);
+      launch_oc_volume(L_.nelx, d_e[E_XP], use_projection, beta, vol_e_, d_oc, d_partials,
+                       stream_);
+    }
+    d2h(h_oc.p, d_oc.p, 1, stream_);
+    sync();
+    if (h_oc.p->done) break;
+  }
+  if (!h_oc.p->done) throw Error(kInternal, "OC bisection did not stop");
+  launch_oc_final(L_.nelx, x, dc, dv, xmin, xmax, d_oc, stream_);
+  return h_oc.p->counter;
+}
+
 int Context::oc_device(double *x, const double *dc, const double *dv, double g, double xmin,
                        double xmax, int use_projection, double beta) {
   if (!filter_ready_) throw Error(kNotReady, "filter_setup has not been called");
@@ -671,7 +719,10 @@ int Context::oc_host(double *x, const double *dfdx, double g, const double *dgdx
 void Context::simp_begin(const icarat_simp_opts &o, const double *x) {
   require_dof();
   ensure_fext();
-  if (!filter_ready_) throw Error(kNotReady, "filter_setup has not been called");
+  if (o.filter != 0 && o.filter != 1)
+    throw Error(kInvalidArgument, "filter must be 0 (cone) or 1 (Helmholtz)");
+  if (o.filter == 0 && !filter_ready_) throw Error(kNotReady, "filter_setup has not been called");
+  if (o.filter == 1 && !helm_) throw Error(kNotReady, "helm_setup has not been called");
   if (o.n_iter < 0) throw Error(kInvalidArgument, "n_iter must be >= 0");
   if (o.optimizer != 0 && o.optimizer != 1)
     throw Error(kInvalidArgument, "optimizer must be 0 (OC) or 1 (MMA)");
@@ -691,7 +742,7 @@ void Context::simp_pre(int step) {
   const long long n = L_.nelx;
   double t0 = now_s();
   // x_tilde = filter(x); x_hat = projection(x_tilde) or x_tilde
-  filter_device(d_e[E_DES], d_e[E_XF], 0);
+  loop_filter(d_e[E_DES], d_e[E_XF], 0);
   simp_.beta = std::min(std::pow(2, (int)(step / 40)), 16.0);
   double *xp = d_e[E_XF];
   if (o.use_projection) {
@@ -744,9 +795,9 @@ void Context::simp_update(int step, double *hist) {
     launch_mul(n, d_e[E_THD], d_e[E_DFDS], stream_);
     launch_mul(n, d_e[E_THD], d_e[E_DGDS], stream_);
   }
-  filter_device(d_e[E_DFDS], d_e[E_TMP], 1);
+  loop_filter(d_e[E_DFDS], d_e[E_TMP], 1);
   std::swap(d_e[E_DFDS], d_e[E_TMP]);
-  filter_device(d_e[E_DGDS], d_e[E_TMP], 1);
+  loop_filter(d_e[E_DGDS], d_e[E_TMP], 1);
   std::swap(d_e[E_DGDS], d_e[E_TMP]);
   sync();
   double t3 = now_s();
@@ -760,6 +811,9 @@ void Context::simp_update(int step, double *hist) {
   if (o.optimizer == 1) {
     const double gc = step == 1 ? 0.0 : simp_.vol / simp_.V0 - 1.0;
     mma_->update(d_e[E_DES], d_e[E_DFDS], gc, d_e[E_DGDS], o.xmin, o.xmax);
+  } else if (o.filter == 1) {
+    oc_device_helm(d_e[E_DES], d_e[E_DFDS], d_e[E_DGDS], simp_.V0, o.xmin, o.xmax,
+                   o.use_projection, simp_.beta);
   } else {
     oc_device(d_e[E_DES], d_e[E_DFDS], d_e[E_DGDS], simp_.V0, o.xmin, o.xmax, o.use_projection,
               simp_.beta);
@@ -1001,6 +1055,13 @@ void Context::time_kernel(int kernel, int reps, double *avg_ms, double *flops, d
     launch = [this]() { launch_apply_k64(L_, kc_, d_young, d_free, d_r, d_y, stream_); };
     F = 2.0 * 24 * 24 * E;
     B = 2.0 * 3 * N * 8 + E * 8 + N;            // x in, y out; young; mask
+  } else if (kernel == 9) {
+    // one Helmholtz filter pass (density mode): element loads, the six 1-D
+    // transforms (FP64 GEMMs) with the spectral scaling, element read-back
+    if (!helm_) throw Error(kNotReady, "helm_setup has not been called");
+    launch = [&]() { helm_->apply(d_dens, d_e[E_XF], 0); };
+    F = 2.0 * 2.0 * N * (L_.NX + L_.NY + L_.NZ);
+    B = 2.0 * 8 * E;                            // x in, x_tilde out
   } else {
     throw Error(kInvalidArgument, "unknown kernel id");
   }

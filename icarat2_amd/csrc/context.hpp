@@ -16,6 +16,7 @@
 namespace icarat_hip {
 
 class DeviceMMA;
+class HelmFilter;
 class LearnedPrecond;
 class Multigrid;
 
@@ -54,6 +55,9 @@ class Context {
   void filter_device(const double *in, double *out, int is_sens);  // device pointers
   void filter_host(const double *in, double *out, int is_sens);
   void threshold_host(const double *in, double beta, double T, double *out, int deriv);
+  // Helmholtz PDE filter (FilterHelmholtz): set-up and both modes on host vectors
+  void helm_setup(double radius);
+  void helm_host(const double *in, double *out, int is_sens);
   int oc_device(double *x, const double *dc, const double *dv, double g, double xmin, double xmax,
                 int use_projection, double beta);
   int oc_host(double *x, const double *dfdx, double g, const double *dgdx, double xmin,
@@ -141,6 +145,12 @@ class Context {
   int fell_back_ = 0;     // the last solve switched to the FP64 contraction
 
   std::unique_ptr<DeviceMMA> mma_;
+  std::unique_ptr<HelmFilter> helm_;
+  // the design loop's filter (simp_.o.filter): cone or Helmholtz, device vectors
+  void loop_filter(const double *in, double *out, int is_sens);
+  // OC bisection whose volume callback filters with the Helmholtz filter
+  int oc_device_helm(double *x, const double *dc, const double *dv, double g, double xmin,
+                     double xmax, int use_projection, double beta);
   std::unique_ptr<LearnedPrecond> learned_;
   std::unique_ptr<Multigrid> mg_;
   void mg_setup(const icarat_mg_opts &o);

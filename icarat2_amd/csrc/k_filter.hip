@@ -149,6 +149,34 @@ __global__ void k_oc_final(long long n, double *__restrict__ x, const double *__
   x[i] = oc_candidate(x[i], dc[i], dv[i], S->lam_last, xmin, xmax);
 }
 
+// x' = oc_candidate(x) at the bisection's current lambda (skipped once it is done)
+__global__ void k_oc_candidate(long long n, const double *__restrict__ x,
+                               const double *__restrict__ dc, const double *__restrict__ dv,
+                               double xmin, double xmax, const OcState *S,
+                               double *__restrict__ out) {
+  if (S->done) return;
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = oc_candidate(x[i], dc[i], dv[i], 0.5 * (S->l1 + S->l0), xmin, xmax);
+}
+
+// block partials of sum (projected) xf * vol_e for the kFinOc update
+__global__ __launch_bounds__(NT) void k_oc_volume(long long n, const double *__restrict__ xf,
+                                                  int use_projection, double beta, double vol_e,
+                                                  const OcState *S, double *partials) {
+#pragma clang fp contract(off)
+  __shared__ double sh[4];
+  if (S->done) return;
+  double s = 0.0;
+  const long long stride = (long long)gridDim.x * NT;
+  for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += stride) {
+    const double v = use_projection ? proj(xf[i], beta, 0.5) : xf[i];
+    s += v * vol_e;
+  }
+  double v[1] = {s};
+  store_partials<NT, 1>(v, partials, sh);
+}
+
 __global__ void k_threshold(long long n, const double *__restrict__ x, double beta, double T,
                             double *__restrict__ out, int deriv) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -223,6 +251,22 @@ void launch_oc_final(long long n, double *x, const double *dc, const double *dv,
   hipLaunchKernelGGL(k_oc_final, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, x, dc, dv, xmin,
                      xmax, S);
   HIPCHK(hipGetLastError());
+}
+
+void launch_oc_candidate(long long n, const double *x, const double *dc, const double *dv,
+                         double xmin, double xmax, const OcState *S, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_oc_candidate, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, x, dc, dv, xmin,
+                     xmax, S, out);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_oc_volume(long long n, const double *xf, int use_projection, double beta, double vol_e,
+                      OcState *S, double *partials, hipStream_t s) {
+  constexpr unsigned G = 1024;
+  hipLaunchKernelGGL(k_oc_volume, dim3(G), dim3(NT), 0, s, n, xf, use_projection, beta, vol_e, S,
+                     partials);
+  HIPCHK(hipGetLastError());
+  launch_finalize(kFinOc, 1, partials, G, nullptr, S, nullptr, s);
 }
 
 void launch_threshold(long long n, const double *x, double beta, double T, double *out, int deriv,

@@ -128,6 +128,14 @@ void launch_oc_step(const Lattice &L, const FilterTable &T, const double *x, con
                     double *partials, unsigned *counter, hipStream_t s);
 void launch_oc_final(long long n, double *x, const double *dc, const double *dv, double xmin,
                      double xmax, const OcState *S, hipStream_t s);
+// The OC bisection step in pieces, for a filter that is not a fused stencil
+// (the Helmholtz filter): out = x' at the current lambda; then, after the
+// caller has filtered out, the volume of the (projected) filtered x' and the
+// bisection update of the device state.  Both skip once the bisection is done.
+void launch_oc_candidate(long long n, const double *x, const double *dc, const double *dv,
+                         double xmin, double xmax, const OcState *S, double *out, hipStream_t s);
+void launch_oc_volume(long long n, const double *xf, int use_projection, double beta, double vol_e,
+                      OcState *S, double *partials, hipStream_t s);
 // sum of x (or of the projection of x) times vol_e, into *out
 void launch_volume(long long n, const double *x, double vol_e, double *out, double *partials,
                    unsigned *counter, hipStream_t s);

@@ -156,6 +156,17 @@ int icarat_hip_filter(icarat_hip_ctx *ctx, const double *in, double *out, int32_
  * (optimization/filter.hpp:216-248). */
 int icarat_hip_threshold(icarat_hip_ctx *ctx, const double *in, double beta, double T,
                          double *out, int32_t deriv);
+/* FilterHelmholtz<E,N>(fem, element, node, radius) (optimization/filter_helm.hpp:69-75):
+ * the Helmholtz PDE filter on the voxel box, natural boundary conditions. */
+int icarat_hip_helm_setup(icarat_hip_ctx *ctx, double radius);
+/* FilterHelmholtz::densityFilter(design_s, solvName, log, sensFlag)
+ * (optimization/filter_helm.hpp:76-135, 191-257): is_sens = sensFlag.  The
+ * reference assembles the nodal system sum_e (N N^T + (r/(2 sqrt 3))^2 B^T B) jac
+ * and solves it with the named iterative solver ("Eigen_CG", relative residual
+ * 1e-10); here the same system is solved directly by fast diagonalisation (the
+ * separable voxel operator, FP64 GEMMs), so the result is the exact solution to
+ * FP64 rounding.  threshold / thresholdDerivative are icarat_hip_threshold. */
+int icarat_hip_helm_filter(icarat_hip_ctx *ctx, const double *in, double *out, int32_t is_sens);
 /* OC::UpdateVariables with the drivers' volume callback (filter, optional
  * projection, sum * volume) (optimization/opt-solver/OCSolver.hpp:20-79,
  * opt_macro_oc.cpp:153-164).  x updated in place; *steps = bisection steps. */
@@ -180,7 +191,9 @@ typedef struct icarat_simp_opts {
   int32_t n_iter;          /* design iterations */
   int32_t optimizer;       /* 0 = OC, 1 = MMA (m = 1 volume constraint) */
   int32_t use_projection;  /* 0 = plain method (no tanh projection) */
-  int32_t reserved;
+  int32_t filter;          /* 0 = cone filter (icarat_hip_filter_setup, optimization/filter.hpp),
+                              1 = Helmholtz filter (icarat_hip_helm_setup, filter_helm.hpp; the
+                              loop of opt-mma-HelmholtzFilter/opt_mma_Helmholtz.cpp:84-150) */
   double xmin, xmax;       /* Optimize(nelx, 1, xmin, xmax) (optimization/opt_base.hpp:20-24) */
   icarat_solve_opts solve;
 } icarat_simp_opts;
@@ -196,7 +209,8 @@ typedef struct icarat_simp_stats {
 
 /* The reference drivers' loop (sample/algorithm/opt-macro-oc/opt_macro_oc.cpp:91-178,
  * opt-macro-mma-nondesign/opt_macro_mma_nondesign.cpp:90-178) with the cone
- * filter set up by icarat_hip_filter_setup.  x: design variables in/out
+ * filter set up by icarat_hip_filter_setup, or the Helmholtz filter set up by
+ * icarat_hip_helm_setup (opts->filter = 1).  x: design variables in/out
  * (nelx).  hist (may be NULL): per iteration {compliance, volume, iterations,
  * true relative residual}. */
 int icarat_hip_simp_run(icarat_hip_ctx *ctx, const icarat_simp_opts *opts, double *x,
@@ -368,7 +382,9 @@ int icarat_hip_mg_info(const icarat_hip_ctx *ctx, int32_t *levels, int32_t *dims
  *   kernel 7: plain copy of one lattice vector (3 Np doubles; bandwidth reference)
  *   kernel 8: FP64 y = K x as the multigrid smoother runs it (no PCG prologue),
  *             1152 flop/element
- * Needs set_density (and filter_setup for kernel 2).  Overwrites the solver's
+ *   kernel 9: one Helmholtz filter pass (icarat_hip_helm_setup): 4 numnp (NX+NY+NZ)
+ *             flop of 1-D transforms (FP64 GEMMs), 16 B per element in and out
+ * Needs set_density (and filter_setup for kernel 2, helm_setup for kernel 9).  Overwrites the solver's
  * work vectors. */
 int icarat_hip_time_kernel(icarat_hip_ctx *ctx, int32_t kernel, int32_t reps, double *avg_ms,
                            double *flops, double *bytes);

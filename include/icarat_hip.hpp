@@ -5,6 +5,8 @@
 //   LinearAnalysis::solve(solvName, log)     analysis/linear.hpp:55-104
 //   Filter::densityFilter / threshold / thresholdDerivative
 //                                            optimization/filter.hpp:21-45,106-155,216-248
+//   FilterHelmholtz::densityFilter / threshold / thresholdDerivative
+//                                            optimization/filter_helm.hpp:21-62,76-181
 //   OC::UpdateVariables                      optimization/opt-solver/OCSolver.hpp:16-79
 //   MMASolver (ctor, Set*, Update, Reset)    optimization/opt-solver/MMASolver.hpp:62-116
 // Like the reference, a fatal condition prints to std::cerr and calls
@@ -149,6 +151,40 @@ class Filter {
   std::vector<double> densityFilter(const std::vector<double> &design_s, bool isSens) {
     std::vector<double> out(design_s.size());
     check(icarat_hip_filter(dev_.handle(), design_s.data(), out.data(), isSens ? 1 : 0),
+          dev_.handle(), "densityFilter");
+    return out;
+  }
+  std::vector<double> threshold(const std::vector<double> &design_s, double beta, double T) {
+    std::vector<double> out(design_s.size());
+    check(icarat_hip_threshold(dev_.handle(), design_s.data(), beta, T, out.data(), 0),
+          dev_.handle(), "threshold");
+    return out;
+  }
+  std::vector<double> thresholdDerivative(const std::vector<double> &design_s, double beta,
+                                          double T) {
+    std::vector<double> out(design_s.size());
+    check(icarat_hip_threshold(dev_.handle(), design_s.data(), beta, T, out.data(), 1),
+          dev_.handle(), "thresholdDerivative");
+    return out;
+  }
+
+ private:
+  Device &dev_;
+};
+
+// FilterHelmholtz (optimization/filter_helm.hpp:21-62): the solver name and log
+// flag of densityFilter are accepted for signature compatibility; the nodal
+// system is solved directly (fast diagonalisation) whatever the name.
+class FilterHelmholtz {
+ public:
+  FilterHelmholtz(Device &dev, double radius) : dev_(dev) {
+    check(icarat_hip_helm_setup(dev_.handle(), radius), dev_.handle(), "FilterHelmholtz");
+  }
+  std::vector<double> densityFilter(const std::vector<double> &design_s,
+                                    const std::string & /*solvName*/, int /*log*/,
+                                    bool sensFlag) {
+    std::vector<double> out(design_s.size());
+    check(icarat_hip_helm_filter(dev_.handle(), design_s.data(), out.data(), sensFlag ? 1 : 0),
           dev_.handle(), "densityFilter");
     return out;
   }

@@ -432,6 +432,206 @@ void threshold_deriv(i64 n, const double *x, double beta, double T, double *out)
               std::tanh(T * beta));
 }
 
+// ------------------------------------------------------- Helmholtz filter --
+// Helmholtz PDE filter (reference optimization/filter_helm.hpp:76-257; B. S.
+// Lazarov and O. Sigmund, Filters in topology optimization based on
+// Helmholtz-type differential equations, 2010).  On every element
+//   Ke = sum_ip (N N^T + c B^T B) jac,  c = (r / (2 sqrt 3))^2   (:191-223)
+//   fe = sum_ip x_e N jac, divided by ipmax when sensFlag          (:215-221)
+// with B the gradient matrix dN/dX (bmatrix.hpp:52-58, :111-117), assembled
+// over all nodes with no boundary condition, K val = F solved by Jacobi-PCG
+// from zero to a relative residual tol ("Eigen_CG", analysis/solver/
+// solvers.cpp:63-91, default tol 1e-10 solvers.hpp:89), and per element
+//   out_e = sum_ip N(ip) . val_e, divided by ipmax unless sensFlag (:229-257).
+// dim 2: quad4 with the 2x2 rule in the reference's point order
+// (basis/gauss_integration.cpp:72-88), dim 3: hexa8 with 2x2x2 (:268-289).
+// Nodes are numbered x fastest, elements x fastest, local nodes
+// counter-clockwise on the bottom face then the top (io/mesh_icarat.hpp:480-511).
+struct Helm {
+  int dim = 3, nx = 1, ny = 1, nz = 1;
+  double h[3] = {1, 1, 1};
+  double radius = 0.0;
+  i64 numnp = 0, nelx = 0;
+  int ne = 8, nip = 8;
+  std::vector<int> conn;     // nelx * ne
+  std::vector<double> ke;    // nelx * ne * ne
+  std::vector<double> nj;    // nelx * nip * ne: N_a(ip) jac(ip)
+  std::vector<double> nq;    // nip * ne: N_a(ip) (the same on every element)
+  std::vector<double> diag;  // numnp
+  int last_iters = 0;
+};
+
+const int kSign2[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+
+void helm_build(Helm &H) {
+  const int d = H.dim, ne = H.ne;
+  const int NX = H.nx + 1, NY = H.ny + 1;
+  H.numnp = (i64)NX * NY * (d == 3 ? H.nz + 1 : 1);
+  H.nelx = (i64)H.nx * H.ny * (d == 3 ? H.nz : 1);
+  H.conn.assign(H.nelx * ne, 0);
+  H.ke.assign(H.nelx * ne * ne, 0.0);
+  H.nj.assign(H.nelx * H.nip * ne, 0.0);
+  H.nq.assign(H.nip * ne, 0.0);
+  H.diag.assign(H.numnp, 0.0);
+  const double g = 1.0 / std::sqrt(3.0);
+  double pts[8][3];
+  if (d == 2) {
+    const double p2[4][2] = {{-g, -g}, {g, -g}, {-g, g}, {g, g}};
+    for (int ip = 0; ip < 4; ++ip) pts[ip][0] = p2[ip][0], pts[ip][1] = p2[ip][1], pts[ip][2] = 0;
+  } else {
+    for (int ip = 0; ip < 8; ++ip)
+      for (int k = 0; k < 3; ++k) pts[ip][k] = kSign[ip][k] * g;
+  }
+  const double c = std::pow(H.radius / (2 * std::sqrt(3.0)), 2);
+  for (i64 e = 0; e < H.nelx; ++e) {
+    const i64 ex = e % H.nx, ey = (e / H.nx) % H.ny, ez = d == 3 ? e / ((i64)H.nx * H.ny) : 0;
+    double Xe[8][3];
+    for (int a = 0; a < ne; ++a) {
+      const int sx = d == 2 ? kSign2[a][0] : kSign[a][0];
+      const int sy = d == 2 ? kSign2[a][1] : kSign[a][1];
+      const int sz = d == 2 ? -1 : kSign[a][2];
+      const i64 ix = ex + (sx > 0), iy = ey + (sy > 0), iz = ez + (sz > 0);
+      H.conn[e * ne + a] = (int)(ix + NX * (iy + (i64)NY * iz));
+      Xe[a][0] = H.h[0] * ix;
+      Xe[a][1] = H.h[1] * iy;
+      Xe[a][2] = H.h[2] * iz;
+    }
+    double *K = &H.ke[e * ne * ne];
+    for (int ip = 0; ip < H.nip; ++ip) {
+      double N[8], dr[3][8];
+      for (int a = 0; a < ne; ++a) {
+        const double s[3] = {(double)(d == 2 ? kSign2[a][0] : kSign[a][0]),
+                             (double)(d == 2 ? kSign2[a][1] : kSign[a][1]),
+                             d == 2 ? 0.0 : (double)kSign[a][2]};
+        double f[3];
+        for (int k = 0; k < d; ++k) f[k] = 0.5 * (1.0 + s[k] * pts[ip][k]);
+        N[a] = 1.0;
+        for (int k = 0; k < d; ++k) N[a] *= f[k];
+        for (int k = 0; k < d; ++k) {
+          double v = 0.5 * s[k];
+          for (int l = 0; l < d; ++l)
+            if (l != k) v *= f[l];
+          dr[k][a] = v;
+        }
+      }
+      double J[3][3] = {{0}}, Ji[3][3] = {{0}}, det;
+      for (int i = 0; i < d; ++i)
+        for (int j = 0; j < d; ++j) {
+          double s = 0.0;
+          for (int a = 0; a < ne; ++a) s += dr[i][a] * Xe[a][j];
+          J[i][j] = s;
+        }
+      if (d == 2) {
+        det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        Ji[0][0] = J[1][1] / det;
+        Ji[0][1] = -J[0][1] / det;
+        Ji[1][0] = -J[1][0] / det;
+        Ji[1][1] = J[0][0] / det;
+      } else {
+        det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+              J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+              J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+        Ji[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+        Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+        Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+        Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
+        Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+        Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+        Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+        Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+        Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+      }
+      double dX[3][8];
+      for (int i = 0; i < d; ++i)
+        for (int a = 0; a < ne; ++a) {
+          double s = 0.0;
+          for (int k = 0; k < d; ++k) s += Ji[i][k] * dr[k][a];
+          dX[i][a] = s;
+        }
+      const double jac = det * 1.0;
+      for (int a = 0; a < ne; ++a) {
+        for (int b = 0; b < ne; ++b) {
+          double bb = 0.0;
+          for (int k = 0; k < d; ++k) bb += dX[k][a] * dX[k][b];
+          K[a * ne + b] += (N[a] * N[b] + c * bb) * jac;
+        }
+        H.nj[(e * H.nip + ip) * ne + a] = N[a] * jac;
+        H.nq[ip * ne + a] = N[a];
+      }
+    }
+    for (int a = 0; a < ne; ++a) H.diag[H.conn[e * ne + a]] += K[a * ne + a];
+  }
+}
+
+// y = K x, element by element (the assembled globalK of filter_helm.hpp:117-118)
+void helm_apply(const Helm &H, const double *x, double *y) {
+  std::fill(y, y + H.numnp, 0.0);
+  const int ne = H.ne;
+  for (i64 e = 0; e < H.nelx; ++e) {
+    const int *cn = &H.conn[e * ne];
+    const double *K = &H.ke[e * ne * ne];
+    for (int a = 0; a < ne; ++a) {
+      double s = 0.0;
+      for (int b = 0; b < ne; ++b) s += K[a * ne + b] * x[cn[b]];
+      y[cn[a]] += s;
+    }
+  }
+}
+
+// Jacobi-preconditioned CG from zero, stop at ||r|| <= tol ||b|| (the
+// ConjugateGradient + DiagonalPreconditioner of "Eigen_CG").  Returns the
+// iteration count, -1 on the cap.
+int helm_pcg(const Helm &H, const double *b, double *x, double tol, int maxit) {
+  const i64 n = H.numnp;
+  std::vector<double> r(b, b + n), z(n), p(n), q(n);
+  std::fill(x, x + n, 0.0);
+  const double bn2 = dot(b, b, (int)n);
+  if (bn2 == 0.0) return 0;
+  for (i64 i = 0; i < n; ++i) z[i] = r[i] / H.diag[i];
+  p = z;
+  double rho = dot(r.data(), z.data(), (int)n);
+  for (int it = 1; it <= maxit; ++it) {
+    helm_apply(H, p.data(), q.data());
+    const double alpha = rho / dot(p.data(), q.data(), (int)n);
+    for (i64 i = 0; i < n; ++i) {
+      x[i] += alpha * p[i];
+      r[i] -= alpha * q[i];
+    }
+    if (dot(r.data(), r.data(), (int)n) <= tol * tol * bn2) return it;
+    for (i64 i = 0; i < n; ++i) z[i] = r[i] / H.diag[i];
+    const double rho1 = dot(r.data(), z.data(), (int)n);
+    const double beta = rho1 / rho;
+    rho = rho1;
+    for (i64 i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+  }
+  return -1;
+}
+
+// FilterHelmholtz::densityFilter(design_s, "Eigen_CG", 0, sensFlag)
+// (filter_helm.hpp:76-135).  Returns the CG iteration count (-1: cap).
+int helm_filter(Helm &H, const double *in, double *out, int sens, double tol, int maxit) {
+  const int ne = H.ne;
+  std::vector<double> F(H.numnp, 0.0), val(H.numnp, 0.0);
+  for (i64 e = 0; e < H.nelx; ++e) {
+    double fe[8] = {0};
+    for (int ip = 0; ip < H.nip; ++ip)
+      for (int a = 0; a < ne; ++a) fe[a] += in[e] * H.nj[(e * H.nip + ip) * ne + a];
+    for (int a = 0; a < ne; ++a) F[H.conn[e * ne + a]] += sens ? fe[a] / H.nip : fe[a];
+  }
+  const int it = helm_pcg(H, F.data(), val.data(), tol, maxit);
+  for (i64 e = 0; e < H.nelx; ++e) {
+    double s = 0.0;
+    for (int ip = 0; ip < H.nip; ++ip) {
+      double t = 0.0;
+      for (int a = 0; a < ne; ++a) t += H.nq[ip * ne + a] * val[H.conn[e * ne + a]];
+      s += t;
+    }
+    out[e] = sens ? s : s / H.nip;
+  }
+  H.last_iters = it;
+  return it;
+}
+
 // ------------------------------------------------------------------- MMA --
 // Svanberg's MMA with the dual interior-point subproblem solver, restated
 // step for step from the published method and the reference's parameter
@@ -1096,11 +1296,15 @@ void orc_threshold_deriv(long long n, const double *x, double beta, double T, do
 // projection -> sum x_hat * vol) (reference
 // optimization/opt-solver/OCSolver.hpp:20-79, opt_macro_oc.cpp:153-164).
 // Returns the number of bisection steps; lambda of the last step in *lam.
-int orc_oc_update(void *f, double *x, const double *dfdx, double g, const double *dgdx,
-                  const double *xmin, const double *xmax, int use_projection, double beta,
-                  double *lam_out) {
-  Filter &F = *(Filter *)f;
-  const i64 n = F.n;
+}  // extern "C"
+
+// OC with the drivers' volume callback: every bisection step filters x' with
+// `filt` (density mode), projects it if asked and sums the volume
+// (optimization/opt-solver/OCSolver.hpp:20-79, opt_macro_oc.cpp:153-164).
+template <class Filt>
+int oc_update_impl(i64 n, double vol_e, Filt filt, double *x, const double *dfdx, double g,
+                   const double *dgdx, const double *xmin, const double *xmax,
+                   int use_projection, double beta, double *lam_out) {
   const double iota = 0.5, movelimit = 0.15, lambdaeps = 1.0e-5;
   const int maxiter = 100;
   double l0 = 0.0, l1 = 1.0e4, lam = 0.0;
@@ -1117,14 +1321,14 @@ int orc_oc_update(void *f, double *x, const double *dfdx, double g, const double
       else if (v > hi) v = hi;
       xn[i] = v;
     }
-    density_filter(F, xn.data(), xf.data(), 0);
+    filt(xn.data(), xf.data());
     const double *xs = xf.data();
     if (use_projection) {
       threshold(n, xf.data(), beta, 0.5, xt.data());
       xs = xt.data();
     }
     double vol = 0.0;
-    for (i64 i = 0; i < n; ++i) vol += xs[i] * F.P->vol_e;
+    for (i64 i = 0; i < n; ++i) vol += xs[i] * vol_e;
     if (vol - g > 0.0) l0 = lam;
     else l1 = lam;
     ++counter;
@@ -1132,6 +1336,17 @@ int orc_oc_update(void *f, double *x, const double *dfdx, double g, const double
   std::copy(xn.begin(), xn.end(), x);
   if (lam_out) *lam_out = lam;
   return counter;
+}
+
+extern "C" {
+
+int orc_oc_update(void *f, double *x, const double *dfdx, double g, const double *dgdx,
+                  const double *xmin, const double *xmax, int use_projection, double beta,
+                  double *lam_out) {
+  Filter &F = *(Filter *)f;
+  return oc_update_impl(
+      F.n, F.P->vol_e, [&](const double *in, double *out) { density_filter(F, in, out, 0); }, x,
+      dfdx, g, dgdx, xmin, xmax, use_projection, beta, lam_out);
 }
 
 void *orc_mma_create(int n, int m, double a, double c, double d) { return new MMA(n, m, a, c, d); }
@@ -1157,10 +1372,13 @@ void orc_mma_state(void *h, double *low, double *upp) {
 // opt_macro_oc.cpp:91-178, opt-macro-mma-nondesign/opt_macro_mma_nondesign.cpp
 // :90-178).  x holds the design variables on entry and exit.  hist receives
 // per iteration {compliance, volume, pcg iterations, true residual}.
-int orc_simp_run(void *h, void *f, const orc_simp_opts *o, double *x, double *hist,
-                 orc_simp_stats *st) {
+}  // extern "C"
+
+// filt(in, out, isSens) is the density filter of the loop (cone or Helmholtz)
+template <class Filt>
+int simp_run_impl(void *h, Filt filt, const orc_simp_opts *o, double *x, double *hist,
+                  orc_simp_stats *st) {
   Problem &P = *(Problem *)h;
-  Filter &F = *(Filter *)f;
   const i64 n = P.nelx;
   orc_simp_stats S;
   std::memset(&S, 0, sizeof(S));
@@ -1172,7 +1390,7 @@ int orc_simp_run(void *h, void *f, const orc_simp_opts *o, double *x, double *hi
   double V0 = 0.0;
   for (int step = 1; step <= o->n_iter; ++step) {
     double t0 = now_s();
-    density_filter(F, x, xf.data(), 0);
+    filt(x, xf.data(), 0);
     double beta = std::min(std::pow(2, (int)(step / 40)), 16.0);
     const double *xp = xf.data();
     if (o->use_projection) {
@@ -1203,9 +1421,9 @@ int orc_simp_run(void *h, void *f, const orc_simp_opts *o, double *x, double *hi
       }
     double t4 = now_s();
     S.sens_s += t4 - t3;
-    density_filter(F, dfds.data(), tmp.data(), 1);
+    filt(dfds.data(), tmp.data(), 1);
     dfds.swap(tmp);
-    density_filter(F, dgds.data(), tmp.data(), 1);
+    filt(dgds.data(), tmp.data(), 1);
     dgds.swap(tmp);
     double t5 = now_s();
     S.filter_s += t5 - t4;
@@ -1219,8 +1437,9 @@ int orc_simp_run(void *h, void *f, const orc_simp_opts *o, double *x, double *hi
       double g = step == 1 ? 0.0 : vol / V0 - 1.0;
       mma->update(x, dfds.data(), &g, dgds.data(), xmin.data(), xmax.data());
     } else {
-      orc_oc_update(f, x, dfds.data(), V0, dgds.data(), xmin.data(), xmax.data(),
-                    o->use_projection, beta, nullptr);
+      oc_update_impl(
+          n, P.vol_e, [&](const double *in, double *out) { filt(in, out, 0); }, x, dfds.data(),
+          V0, dgds.data(), xmin.data(), xmax.data(), o->use_projection, beta, nullptr);
     }
     S.update_s += now_s() - t5;
   }
@@ -1228,6 +1447,70 @@ int orc_simp_run(void *h, void *f, const orc_simp_opts *o, double *x, double *hi
   S.total_s = now_s() - t_all;
   if (st) *st = S;
   return 0;
+}
+
+extern "C" {
+
+int orc_simp_run(void *h, void *f, const orc_simp_opts *o, double *x, double *hist,
+                 orc_simp_stats *st) {
+  Filter &F = *(Filter *)f;
+  return simp_run_impl(
+      h, [&](const double *in, double *out, int sens) { density_filter(F, in, out, sens); }, o, x,
+      hist, st);
+}
+
+// ---------------------------------------------------------- Helmholtz ----
+// FilterHelmholtz<E,N>(fem, element, node, radius) on a structured grid of
+// quad4 (dim 2) or hexa8 (dim 3) elements of size hx x hy (x hz)
+// (optimization/filter_helm.hpp:69-75).
+void *orc_helm_create(int dim, int nx, int ny, int nz, double hx, double hy, double hz,
+                      double radius) {
+  Helm *H = new Helm;
+  H->dim = dim;
+  H->nx = nx;
+  H->ny = ny;
+  H->nz = dim == 3 ? nz : 1;
+  H->h[0] = hx;
+  H->h[1] = hy;
+  H->h[2] = hz;
+  H->radius = radius;
+  H->ne = H->nip = dim == 2 ? 4 : 8;
+  helm_build(*H);
+  return H;
+}
+void orc_helm_destroy(void *h) { delete (Helm *)h; }
+long long orc_helm_numnp(void *h) { return ((Helm *)h)->numnp; }
+// element matrix Ke (ne x ne) of element e, and its connectivity
+void orc_helm_element(void *h, long long e, double *ke, int *conn) {
+  const Helm &H = *(Helm *)h;
+  std::copy(&H.ke[e * H.ne * H.ne], &H.ke[(e + 1) * H.ne * H.ne], ke);
+  std::copy(&H.conn[e * H.ne], &H.conn[(e + 1) * H.ne], conn);
+}
+// y = K x (the assembled Helmholtz matrix, numnp)
+void orc_helm_apply(void *h, const double *x, double *y) { helm_apply(*(Helm *)h, x, y); }
+// densityFilter(in, "Eigen_CG", 0, sens) with CG tolerance tol; returns the
+// CG iterations (-1 at the cap)
+int orc_helm_filter(void *h, const double *in, double *out, int sens, double tol, int maxit) {
+  return helm_filter(*(Helm *)h, in, out, sens, tol, maxit);
+}
+// OC whose volume callback filters with the Helmholtz filter
+int orc_helm_oc_update(void *h, void *hp, double *x, const double *dfdx, double g,
+                       const double *dgdx, const double *xmin, const double *xmax,
+                       int use_projection, double beta, double tol) {
+  Helm &H = *(Helm *)h;
+  const Problem &P = *(Problem *)hp;
+  return oc_update_impl(
+      H.nelx, P.vol_e, [&](const double *in, double *out) { helm_filter(H, in, out, 0, tol, 100000); },
+      x, dfdx, g, dgdx, xmin, xmax, use_projection, beta, nullptr);
+}
+// The SIMP loop with the Helmholtz filter in place of the cone filter
+// (sample/algorithm/opt-mma-HelmholtzFilter/opt_mma_Helmholtz.cpp:84-150).
+int orc_simp_run_helm(void *h, void *helm, const orc_simp_opts *o, double tol, double *x,
+                      double *hist, orc_simp_stats *st) {
+  Helm &H = *(Helm *)helm;
+  return simp_run_impl(
+      h, [&](const double *in, double *out, int sens) { helm_filter(H, in, out, sens, tol, 100000); },
+      o, x, hist, st);
 }
 
 // Seeded smooth density field: sum of nblob isotropic Gaussians (amplitude

@@ -80,6 +80,16 @@ def lib():
                                                 C.c_ulonglong, C.c_int, C.c_double, C.c_double,
                                                 C.c_double, C.c_double, C.c_double, dp]),
             "orc_num_threads": (C.c_int, []),
+            "orc_helm_create": (vp, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                     C.c_double, C.c_double]),
+            "orc_helm_destroy": (None, [vp]), "orc_helm_numnp": (C.c_longlong, [vp]),
+            "orc_helm_element": (None, [vp, C.c_longlong, dp, ip]),
+            "orc_helm_apply": (None, [vp, dp, dp]),
+            "orc_helm_filter": (C.c_int, [vp, dp, dp, C.c_int, C.c_double, C.c_int]),
+            "orc_helm_oc_update": (C.c_int, [vp, vp, dp, dp, C.c_double, dp, dp, dp, C.c_int,
+                                             C.c_double, C.c_double]),
+            "orc_simp_run_helm": (C.c_int, [vp, vp, C.POINTER(SimpOpts), C.c_double, dp, dp,
+                                            C.POINTER(SimpStats)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -304,6 +314,71 @@ class Filter:
         hist = np.zeros(4 * n_iter)
         st = SimpStats()
         self.L.orc_simp_run(self.prob.h, self.h, C.byref(o), P(x), P(hist), C.byref(st))
+        return x, hist.reshape(n_iter, 4), st
+
+
+class HelmFilter:
+    """Helmholtz PDE filter (reference optimization/filter_helm.hpp) on a
+    structured grid: dim 2 = quad4 (nz ignored), dim 3 = hexa8.  `tol` is the
+    relative residual of its Jacobi-PCG solve (the reference's Eigen_CG
+    default is 1e-10)."""
+
+    def __init__(self, dims, radius, h=(1.0, 1.0, 1.0), dim=3, tol=1e-13):
+        self.L = lib()
+        nx, ny = dims[0], dims[1]
+        nz = dims[2] if dim == 3 else 1
+        self.dim, self.dims, self.radius, self.tol = dim, (nx, ny, nz), radius, tol
+        self.nelx = nx * ny * nz
+        self.ne = 4 if dim == 2 else 8
+        self.h = self.L.orc_helm_create(dim, nx, ny, nz, h[0], h[1], h[2], radius)
+        self.numnp = self.L.orc_helm_numnp(self.h)
+        self.iters = 0
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.orc_helm_destroy(self.h)
+            self.h = None
+
+    def element(self, e):
+        ke = np.zeros(self.ne * self.ne)
+        conn = np.zeros(self.ne, dtype=np.int32)
+        self.L.orc_helm_element(self.h, e, P(ke), PI(conn))
+        return ke.reshape(self.ne, self.ne), conn
+
+    def apply(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        y = np.zeros(self.numnp)
+        self.L.orc_helm_apply(self.h, P(v), P(y))
+        return y
+
+    def density_filter(self, x, is_sens=False, tol=None, maxit=100000):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros(self.nelx)
+        it = self.L.orc_helm_filter(self.h, P(x), P(out), 1 if is_sens else 0,
+                                    self.tol if tol is None else tol, maxit)
+        if it < 0:
+            raise RuntimeError("Helmholtz filter CG did not converge")
+        self.iters = it
+        return out
+
+    def oc_update(self, prob, x, dfdx, g, dgdx, xmin=1e-4, xmax=1.0, use_projection=False,
+                  beta=1.0):
+        n = self.nelx
+        x = np.array(x, dtype=np.float64)
+        cnt = self.L.orc_helm_oc_update(self.h, prob.h, P(x), P(np.ascontiguousarray(dfdx)), g,
+                                        P(np.ascontiguousarray(dgdx)), P(np.full(n, xmin)),
+                                        P(np.full(n, xmax)), 1 if use_projection else 0, beta,
+                                        self.tol)
+        return x, cnt
+
+    def simp_run(self, prob, x0, n_iter, optimizer=0, use_projection=False, cg_tol=1e-8,
+                 cg_maxit=10000, xmin=1e-4, xmax=1.0):
+        o = SimpOpts(n_iter, optimizer, 1 if use_projection else 0, cg_tol, cg_maxit, xmin, xmax)
+        x = np.array(x0, dtype=np.float64)
+        hist = np.zeros(4 * n_iter)
+        st = SimpStats()
+        self.L.orc_simp_run_helm(prob.h, self.h, C.byref(o), self.tol, P(x), P(hist),
+                                 C.byref(st))
         return x, hist.reshape(n_iter, 4), st
 
 
