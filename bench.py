@@ -86,7 +86,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_problem(A, cfg):
+def make_problem(A, cfg, helm=False):
     nx, ny, nz = cfg["dims"]
     c = A.cantilever(nx, ny, nz)
     if cfg["field"] is None:
@@ -95,13 +95,15 @@ def make_problem(A, cfg):
     else:
         x0, _ = A.gaussian_field(nx, ny, nz, cfg["field"])
     c.filter_setup(cfg["rmin"])
+    if helm:
+        c.helm_setup(cfg["rmin"])
     return c, x0
 
 
-def run_gpu(A, cfg, method, steps, warmup, tol):
+def run_gpu(A, cfg, method, steps, warmup, tol, helm=False):
     prec, pre = METHODS[method]
-    c, x0 = make_problem(A, cfg)
-    kw = dict(tol=tol, maxit=100000, precision=prec, precond=pre)
+    c, x0 = make_problem(A, cfg, helm)
+    kw = dict(tol=tol, maxit=100000, precision=prec, precond=pre, helmholtz=helm)
     if pre == 1:
         # the network is trained online on the Jacobi solves of the warm-up
         # iterations (at least one); the timed iterations use the trained network
@@ -117,10 +119,13 @@ def run_gpu(A, cfg, method, steps, warmup, tol):
     return c, x0, x, hist, st, dt
 
 
-def cpu_baseline(cfg, pcg_iters_per_solve, n_cg=60):
+def cpu_baseline(cfg, pcg_iters_per_solve, n_cg=60, helm=False):
     """Times the FP64 CPU oracle (assembled CSR + Jacobi-PCG + filter + OC) on a
     bounded sample of one SIMP iteration and scales the PCG part to the
-    iteration count the GPU needed on the same design."""
+    iteration count the GPU needed on the same design.  With the Helmholtz
+    filter, one filter solve (Jacobi-PCG to 1e-10, as the reference's Eigen_CG)
+    is timed and counted once per filter call: 3 per iteration plus one per OC
+    bisection step."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle as O
@@ -149,8 +154,18 @@ def cpu_baseline(cfg, pcg_iters_per_solve, n_cg=60):
     dvf = F.density_filter(dg, True)
     t_filter += time.perf_counter() - t
     t = time.perf_counter()
-    F.oc_update(x0, dcf, xf.sum(), dvf)
+    _, oc_steps, _ = F.oc_update(x0, dcf, xf.sum(), dvf)
     t_oc = time.perf_counter() - t
+    helm_note = ""
+    if helm:
+        H = O.HelmFilter((nx, ny, nz), cfg["rmin"], tol=1e-10)
+        t = time.perf_counter()
+        H.density_filter(x0)
+        t_h = time.perf_counter() - t
+        t_filter = 3 * t_h
+        t_oc += oc_steps * t_h
+        helm_note = (f"; Helmholtz filter: one solve timed ({t_h:.2f}s, {H.iters} CG "
+                     f"iterations), counted 3 + {oc_steps} (OC steps) times")
     per_iter = t_filter + t_asm + t_cg * pcg_iters_per_solve + t_sens + t_oc
     return {
         "value": 1.0 / per_iter,
@@ -161,7 +176,7 @@ def cpu_baseline(cfg, pcg_iters_per_solve, n_cg=60):
                    f"unavailable offline) on {nx}x{ny}x{nz}: filter + CSR assembly "
                    f"({t_asm:.2f}s) + {it} timed Jacobi-PCG iterations ({t_cg * 1e3:.1f} ms each, "
                    f"scaled to the GPU's {pcg_iters_per_solve:.0f} iterations/solve) + "
-                   f"sensitivity + 2 chain-rule filters + OC ({t_oc:.2f}s)"),
+                   f"sensitivity + 2 chain-rule filters + OC ({t_oc:.2f}s)" + helm_note),
     }
 
 
@@ -242,6 +257,9 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--reps", type=int, default=20, help="launches timed for the roofline")
+    ap.add_argument("--filter", default="cone", choices=["cone", "helm"],
+                    help="density filter of the loop: cone (filter.hpp) or Helmholtz "
+                         "(filter_helm.hpp)")
     ap.add_argument("--budget", type=int, default=None,
                     help="c5: admission budget in elements (0: all problems at once)")
     a = ap.parse_args()
@@ -261,14 +279,15 @@ def main():
             sys.exit(2)
         print(json.dumps(run_batch(A, a)), flush=True)
         return
-    c, x0, x, hist, st, dt = run_gpu(A, cfg, a.method, a.steps, a.warmup, a.tol)
+    helm = a.filter == "helm"
+    c, x0, x, hist, st, dt = run_gpu(A, cfg, a.method, a.steps, a.warmup, a.tol, helm)
     pcg_per_solve = float(hist[:, 2].mean())
     # the plain method (FP64 Jacobi-PCG) on the same workload, same steps
     plain = None
     jacobi_iters = pcg_per_solve
     if a.method != "fp64" and not a.no_plain:
         import numpy as np
-        cp, _, xp, hp, sp, dtp = run_gpu(A, cfg, "fp64", a.steps, a.warmup, a.tol)
+        cp, _, xp, hp, sp, dtp = run_gpu(A, cfg, "fp64", a.steps, a.warmup, a.tol, helm)
         jacobi_iters = float(hp[:, 2].mean())
         plain = {"method": "fp64 (Jacobi-PCG)", "value": a.steps / dtp,
                  "solve_s_per_iter": sp.solve_s / a.steps, "pcg_iters_per_solve": jacobi_iters,
@@ -306,9 +325,10 @@ def main():
                   "bf16": "bf16+f64"}[a.method],
         "data": "synthetic (seeded Gaussian-field densities, no external data)",
         "config": {"workload": f"{a.config}: {'x'.join(map(str, cfg['dims']))} voxel hexa8 "
-                               f"cantilever SIMP (filter, solve to {a.tol:g}, sensitivities, OC)",
+                               f"cantilever SIMP ({'Helmholtz ' if helm else ''}filter, solve "
+                               f"to {a.tol:g}, sensitivities, OC)",
                    "method": a.method, "elements": int(c.nelx), "free_dofs": int(c.numeq),
-                   "rmin": cfg["rmin"]},
+                   "rmin": cfg["rmin"], "filter": "helmholtz" if helm else "cone"},
         "solve_s_per_iter": st.solve_s / a.steps,
         "pcg_iters_per_solve": pcg_per_solve,
         "max_true_relres": st.max_true_relres,
@@ -327,13 +347,21 @@ def main():
     }
     if a.method == "mg":
         out["config"]["mg_levels"] = c.mg_info()
+    if helm:
+        ms_h, fl_h, by_h = c.time_kernel(9, a.reps)
+        out["helm_filter"] = {"kernel": "Helmholtz filter pass (element loads, 6 FP64 GEMM "
+                                        "transforms + spectral scaling, read-back)",
+                              "avg_ms": ms_h, "flops": fl_h,
+                              "tflops": fl_h / (ms_h * 1e-3) / 1e12,
+                              "frac_fp64_peak": fl_h / (ms_h * 1e-3) / 1e12 /
+                              MI355X_PEAK["fp64_tflops"]}
     if plain is not None:
         out["plain_fp64"] = plain
         out["speedup_vs_plain_fp64"] = out["value"] / plain["value"]
     if not a.no_cpu:
         # the CPU runs the reference's Jacobi-PCG: scaled by Jacobi's iterations
         log("[bench] cpu baseline sample")
-        out["cpu_baseline"] = cpu_baseline(cfg, jacobi_iters)
+        out["cpu_baseline"] = cpu_baseline(cfg, jacobi_iters, helm=helm)
     print(json.dumps(out), flush=True)
 
 

@@ -660,14 +660,21 @@ int Context::oc_device_helm(double *x, const double *dc, const double *dv, doubl
   *h_oc.p = s0;
   h2d(d_oc.p, h_oc.p, 1, stream_);
   // x' at lambda -> E_TMP, filtered -> E_XP (both free while the design is
-  // updated), volume of its projection -> bisection update
+  // updated), volume of its projection -> bisection update.  Without the
+  // projection the filter is skipped: the Helmholtz filter preserves the
+  // volume exactly (A 1 = M 1 = (V_e / 8) S^T 1 with no boundary condition, so
+  // sum_e (W x)_e = sum_e x_e), and the volume of x' is the volume of its
+  // filtered field to rounding.
   constexpr int kSteps = 4;
   for (int b = 0; b < 100 / kSteps + 2; ++b) {
     for (int k = 0; k < kSteps; ++k) {
       launch_oc_candidate(L_.nelx, x, dc, dv, xmin, xmax, d_oc, d_e[E_TMP], stream_);
-      helm_->apply(d_e[E_TMP], d_e[E_XP], 0);
-      launch_oc_volume(L_.nelx, d_e[E_XP], use_projection, beta, vol_e_, d_oc, d_partials,
-                       stream_);
+      const double *xf = d_e[E_TMP];
+      if (use_projection) {
+        helm_->apply(d_e[E_TMP], d_e[E_XP], 0);
+        xf = d_e[E_XP];
+      }
+      launch_oc_volume(L_.nelx, xf, use_projection, beta, vol_e_, d_oc, d_partials, stream_);
     }
     d2h(h_oc.p, d_oc.p, 1, stream_);
     sync();
