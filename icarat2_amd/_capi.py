@@ -99,6 +99,7 @@ _SIGS = {
     "icarat_hip_filter_setup": (C.c_int, [C.c_void_p, C.c_double]),
     "icarat_hip_filter": (C.c_int, [C.c_void_p, dp, dp, C.c_int32]),
     "icarat_hip_helm_setup": (C.c_int, [C.c_void_p, C.c_double]),
+    "icarat_hip_mises": (C.c_int, [C.c_void_p, dp, dp]),
     "icarat_hip_helm_filter": (C.c_int, [C.c_void_p, dp, dp, C.c_int32]),
     "icarat_hip_threshold": (C.c_int, [C.c_void_p, dp, C.c_double, C.c_double, dp, C.c_int32]),
     "icarat_hip_oc_update": (C.c_int, [C.c_void_p, dp, dp, C.c_double, dp, C.c_double,
@@ -410,6 +411,13 @@ class Context:
         self._chk(self.L.icarat_hip_filter(self.h, _p(_f64(x)), _p(out), 1 if is_sens else 0),
                   "filter")
         return out
+
+    def mises(self):
+        """Element-mean stress (nelx x 6) and von Mises stress of the last solve."""
+        m = np.zeros(self.nelx)
+        st = np.zeros(6 * self.nelx)
+        self._chk(self.L.icarat_hip_mises(self.h, _p(m), _p(st)), "mises")
+        return m, st.reshape(self.nelx, 6)
 
     def helm_setup(self, radius):
         """FilterHelmholtz(fem, element, node, radius) on this box."""

@@ -224,6 +224,11 @@ int icarat_hip_compliance_sensitivity(icarat_hip_ctx *h, double vnorm, double *c
   return guard(h, [&](icarat_hip::Context &c) { c.compliance_sensitivity(vnorm, cc, dfds, dgds); });
 }
 
+int icarat_hip_mises(icarat_hip_ctx *h, double *mises, double *stress) {
+  if (!mises) return fail(h, ICARAT_HIP_INVALID_ARGUMENT, "null argument");
+  return guard(h, [&](icarat_hip::Context &c) { c.mises_host(mises, stress); });
+}
+
 int icarat_hip_filter_setup(icarat_hip_ctx *h, double radius) {
   return guard(h, [&](icarat_hip::Context &c) { c.filter_setup(radius); });
 }

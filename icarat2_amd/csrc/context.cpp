@@ -581,6 +581,22 @@ void Context::compliance_sensitivity(double vnorm, double *c, double *dfds, doub
   if (c) *c = h_scal.p[0];
 }
 
+void Context::mises_host(double *mises, double *stress) {
+  require_density();
+  ensure_fext();
+  if (!solved_) throw Error(kNotReady, "no solution for the current density");
+  const auto &g = prob_.spec();
+  const double h[3] = {g.lx / g.nx, g.ly / g.ny, g.lz / g.nz};
+  DevBuf<double> d_st;
+  if (stress) d_st.alloc(6 * (size_t)L_.nelx);
+  launch_flush_small(n3_, d_u, d_nv, stream_);
+  launch_mises64(L_, d_dens, d_nv, g.young1, g.young2, g.penalty, g.poisson, h, d_e[6],
+                 stress ? d_st.p : nullptr, stream_);
+  d2h(mises, d_e[6].p, L_.nelx, stream_);
+  if (stress) d2h(stress, d_st.p, 6 * L_.nelx, stream_);
+  sync();
+}
+
 // ---- filter / OC ----------------------------------------------------------
 // element work vectors
 enum { E_DFDS = 0, E_DGDS = 1, E_AA = 2, E_XF = 3, E_XP = 4, E_THD = 5, E_TMP = 6, E_DES = 7 };

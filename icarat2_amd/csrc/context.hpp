@@ -50,6 +50,8 @@ class Context {
   double true_residual_device();          // ||f - K d_u|| / ||f||
   double true_residual_host(const double *u);
   void compliance_sensitivity(double vnorm, double *c, double *dfds, double *dgds);
+  // element-mean stress (nelx x 6, may be null) and von Mises stress of the last solve
+  void mises_host(double *mises, double *stress);
 
   void filter_setup(double radius);
   void filter_device(const double *in, double *out, int is_sens);  // device pointers

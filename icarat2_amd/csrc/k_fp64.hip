@@ -603,6 +603,54 @@ __global__ __launch_bounds__(NT) void k_sensitivity64(Lattice L, const double *_
   dgds[e] = vol_e / vnorm;
 }
 
+// Element-mean stress and von Mises stress (reference problem/elasticity.cpp:
+// 166-209).  The mean of B over the 2x2x2 Gauss points of a box voxel is B at
+// its centre (B is multilinear), where dN_a/dX_j = s_aj / (4 h_j), s_aj = +-1
+// the corner's side; so strain = B(0) u_e and stress = De(E(x_e)) strain.
+__global__ __launch_bounds__(NT) void k_mises64(Lattice L, const double *__restrict__ dens,
+                                                const double *__restrict__ nv, double y1,
+                                                double y2, double p, double nu, double ihx,
+                                                double ihy, double ihz,
+                                                double *__restrict__ mises,
+                                                double *__restrict__ stress) {
+  const long long e = blockIdx.x * (long long)NT + threadIdx.x;
+  if (e >= L.nelx) return;
+  const int i = (int)(e % L.nx), j = (int)((e / L.nx) % L.ny), k = (int)(e / ((long long)L.nx * L.ny));
+  const long long n0 = i + (long long)L.NX * (j + (long long)L.NY * k);
+  // G[c][d] = du_c / dX_d at the centre
+  double G[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  const double ih[3] = {0.25 * ihx, 0.25 * ihy, 0.25 * ihz};
+#pragma unroll
+  for (int a = 0; a < 8; ++a) {
+    const long long n = n0 + lnode_x(a) + (long long)L.NX * (lnode_y(a) + (long long)L.NY * lnode_z(a));
+    const double sg[3] = {lnode_x(a) ? ih[0] : -ih[0], lnode_y(a) ? ih[1] : -ih[1],
+                          lnode_z(a) ? ih[2] : -ih[2]};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double u = nv[c * L.Np + n];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) G[c][d] += sg[d] * u;
+    }
+  }
+  const double eps[6] = {G[0][0], G[1][1], G[2][2], G[0][1] + G[1][0], G[1][2] + G[2][1],
+                         G[2][0] + G[0][2]};
+  const double x = dens[e];
+  const double xp = pow(x, p);
+  const double E = (1.0 - xp) * y1 + xp * y2;
+  const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+  const double mu = E / (2.0 * (1.0 + nu));
+  const double tr = eps[0] + eps[1] + eps[2];
+  const double s[6] = {lam * tr + 2.0 * mu * eps[0], lam * tr + 2.0 * mu * eps[1],
+                       lam * tr + 2.0 * mu * eps[2], mu * eps[3], mu * eps[4], mu * eps[5]};
+  const double q = s[0] * s[0] + s[1] * s[1] + s[2] * s[2] - s[0] * s[1] - s[1] * s[2] -
+                   s[2] * s[0] + 3.0 * (s[3] * s[3] + s[4] * s[4] + s[5] * s[5]);
+  mises[e] = sqrt(fmax(q, 0.0));
+  if (stress) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) stress[6 * e + c] = s[c];
+  }
+}
+
 __global__ void k_numeq_to_lattice(Lattice L, const int *__restrict__ dof, int numeq,
                                    const double *__restrict__ v, const double *__restrict__ dirval,
                                    double *__restrict__ lat) {
@@ -859,6 +907,14 @@ void launch_sensitivity64(const Lattice &L, const double *ke, const double *dens
                           double vnorm, double *dfds, double *dgds, hipStream_t s) {
   hipLaunchKernelGGL(k_sensitivity64, dim3(blocks_for(L.nelx, NT)), dim3(NT), 0, s, L, ke, dens,
                      nodeval, y1, y2, p, vol_e, vnorm, dfds, dgds);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_mises64(const Lattice &L, const double *dens, const double *nodeval, double y1,
+                    double y2, double p, double nu, const double h[3], double *mises,
+                    double *stress, hipStream_t s) {
+  hipLaunchKernelGGL(k_mises64, dim3(blocks_for(L.nelx, NT)), dim3(NT), 0, s, L, dens, nodeval, y1,
+                     y2, p, nu, 1.0 / h[0], 1.0 / h[1], 1.0 / h[2], mises, stress);
   HIPCHK(hipGetLastError());
 }
 

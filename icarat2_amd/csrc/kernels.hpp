@@ -93,6 +93,11 @@ void launch_flush_small(long long n, const double *u, double *v, hipStream_t s);
 void launch_sensitivity64(const Lattice &L, const double *ke, const double *dens,
                           const double *nodeval, double y1, double y2, double p, double vol_e,
                           double vnorm, double *dfds, double *dgds, hipStream_t s);
+// element-mean stress (stress may be null; nelx x 6 Voigt) and von Mises stress
+// of the nodal values, voxel edge lengths h (Elastic::makeMisesStress)
+void launch_mises64(const Lattice &L, const double *dens, const double *nodeval, double y1,
+                    double y2, double p, double nu, const double h[3], double *mises,
+                    double *stress, hipStream_t s);
 // lattice <-> reference numeq ordering (dof[3*n+c] is the reference DOF number)
 void launch_numeq_to_lattice(const Lattice &L, const int *dof, int numeq, const double *v,
                              const double *dirval, double *lat, hipStream_t s);

@@ -147,6 +147,12 @@ int icarat_hip_true_residual(icarat_hip_ctx *ctx, const double *u, double *relre
 int icarat_hip_compliance_sensitivity(icarat_hip_ctx *ctx, double vnorm, double *c, double *dfds,
                                       double *dgds);
 
+/* Element-mean stress and von Mises stress of the last solve
+ * (Elastic::makeMisesStress, problem/elasticity.cpp:166-209): stress = mean over
+ * the Gauss points of De(E(x_e)) B u_e, mises = sqrt(stress^T V stress).
+ * mises: nelx entries; stress (may be NULL): nelx x 6, Voigt [xx,yy,zz,xy,yz,zx]. */
+int icarat_hip_mises(icarat_hip_ctx *ctx, double *mises, double *stress);
+
 /* ---- optimization (Filter / OC / MMASolver) ------------------------------ */
 /* Filter<E,N>(fem, element, node, radius) (optimization/filter.hpp:63-70). */
 int icarat_hip_filter_setup(icarat_hip_ctx *ctx, double radius);

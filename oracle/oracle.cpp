@@ -97,59 +97,65 @@ void De3(double E, double nu, double D[6][6]) {
   D[3][3] = D[4][4] = D[5][5] = mu;
 }
 
-// Ke = sum_ip B^T De B jac over the 2x2x2 Gauss rule, points (+-1/sqrt3)^3 in
-// the listed order, weight 1 (reference basis/bmatrix.hpp:90-162,
-// basis/gauss_integration.cpp:268-289, problem/elasticity.cpp:211-255).
-// Returns the element volume (sum of jac).
-double element_ke(const double Xe[8][3], double E, double nu, double Ke[576]) {
+// Strain-displacement matrix B (6x24, DOF order 3a+d, Voigt [xx,yy,zz,xy,yz,zx]
+// with engineering shears) at Gauss point ip of the 2x2x2 rule, points
+// (+-1/sqrt3)^3 in the listed order, weight 1 (reference basis/bmatrix.hpp:
+// 90-162, basis/gauss_integration.cpp:268-289).  Returns jac = det J * weight.
+double element_B(const double Xe[8][3], int ip, double B[6][24]) {
   const double g = 1.0 / std::sqrt(3.0);
   const double pts[8][3] = {{-g, -g, -g}, {g, -g, -g}, {g, g, -g}, {-g, g, -g},
                             {-g, -g, g},  {g, -g, g},  {g, g, g},  {-g, g, g}};
+  double dr[3][8];
+  dNdr_h8(pts[ip], dr);
+  double J[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+      for (int a = 0; a < 8; ++a) s += dr[i][a] * Xe[a][j];
+      J[i][j] = s;
+    }
+  double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+               J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+               J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+  double Ji[3][3];
+  Ji[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
+  Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
+  Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
+  Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
+  Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
+  Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
+  Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
+  Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
+  Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
+  double dX[3][8];
+  for (int i = 0; i < 3; ++i)
+    for (int a = 0; a < 8; ++a)
+      dX[i][a] = Ji[i][0] * dr[0][a] + Ji[i][1] * dr[1][a] + Ji[i][2] * dr[2][a];
+  std::memset(B, 0, sizeof(double) * 6 * 24);
+  for (int a = 0; a < 8; ++a) {
+    B[0][3 * a] = dX[0][a];
+    B[1][3 * a + 1] = dX[1][a];
+    B[2][3 * a + 2] = dX[2][a];
+    B[3][3 * a] = dX[1][a];
+    B[3][3 * a + 1] = dX[0][a];
+    B[4][3 * a + 1] = dX[2][a];
+    B[4][3 * a + 2] = dX[1][a];
+    B[5][3 * a] = dX[2][a];
+    B[5][3 * a + 2] = dX[0][a];
+  }
+  return det * 1.0;
+}
+
+// Ke = sum_ip B^T De B jac over the 2x2x2 Gauss rule (reference
+// problem/elasticity.cpp:211-255).  Returns the element volume (sum of jac).
+double element_ke(const double Xe[8][3], double E, double nu, double Ke[576]) {
   double D[6][6];
   De3(E, nu, D);
   std::memset(Ke, 0, sizeof(double) * 576);
   double vol = 0.0;
   for (int ip = 0; ip < 8; ++ip) {
-    double dr[3][8];
-    dNdr_h8(pts[ip], dr);
-    double J[3][3];
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) {
-        double s = 0.0;
-        for (int a = 0; a < 8; ++a) s += dr[i][a] * Xe[a][j];
-        J[i][j] = s;
-      }
-    double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
-                 J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
-                 J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
-    double Ji[3][3];
-    Ji[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
-    Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
-    Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
-    Ji[1][0] = (J[1][2] * J[2][0] - J[1][0] * J[2][2]) / det;
-    Ji[1][1] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) / det;
-    Ji[1][2] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) / det;
-    Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
-    Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
-    Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
-    double dX[3][8];
-    for (int i = 0; i < 3; ++i)
-      for (int a = 0; a < 8; ++a)
-        dX[i][a] = Ji[i][0] * dr[0][a] + Ji[i][1] * dr[1][a] + Ji[i][2] * dr[2][a];
     double B[6][24];
-    std::memset(B, 0, sizeof(B));
-    for (int a = 0; a < 8; ++a) {
-      B[0][3 * a] = dX[0][a];
-      B[1][3 * a + 1] = dX[1][a];
-      B[2][3 * a + 2] = dX[2][a];
-      B[3][3 * a] = dX[1][a];
-      B[3][3 * a + 1] = dX[0][a];
-      B[4][3 * a + 1] = dX[2][a];
-      B[4][3 * a + 2] = dX[1][a];
-      B[5][3 * a] = dX[2][a];
-      B[5][3 * a + 2] = dX[0][a];
-    }
-    double jac = det * 1.0;
+    const double jac = element_B(Xe, ip, B);
     double DB[6][24];
     for (int i = 0; i < 6; ++i)
       for (int c = 0; c < 24; ++c) {
@@ -1254,6 +1260,47 @@ void orc_sensitivity(void *h, const double *nodeval, double vnorm, double *dfds,
     }
     dfds[e] = -E * s;
     dgds[e] = P.vol_e / vnorm;
+  }
+}
+
+// Element-mean stress and von Mises stress (reference problem/elasticity.cpp:
+// 166-209, Elastic::makeMisesStress): strain = sum_ip B u_e / ipmax, stress =
+// sum_ip De(E(x_e)) B u_e / ipmax, mises = sqrt(stress^T V stress) with
+// V = [1 -.5 -.5; -.5 1 -.5; -.5 -.5 1] (+) 3 I.  u_e from the nodal values
+// (Dirichlet values included), De with the SIMP Young's modulus of the
+// densities last set.  stress (may be NULL): nelx x 6 Voigt.
+void orc_mises(void *h, const double *nodeval, double *mises, double *stress) {
+  Problem &P = *(Problem *)h;
+#pragma omp parallel for schedule(static)
+  for (i64 e = 0; e < P.nelx; ++e) {
+    double Xe[8][3], ue[24], D[6][6];
+    element_X(P, e, Xe);
+    for (int a = 0; a < 8; ++a)
+      for (int d = 0; d < 3; ++d) ue[3 * a + d] = nodeval[3 * (i64)P.conn[8 * e + a] + d];
+    De3(simp(P.young1, P.young2, P.dens[e], P.penalty), P.poisson, D);
+    const double fac = 1.0 / 8;
+    double sig[6] = {0, 0, 0, 0, 0, 0};
+    for (int ip = 0; ip < 8; ++ip) {
+      double B[6][24], eps[6];
+      element_B(Xe, ip, B);
+      for (int i = 0; i < 6; ++i) {
+        double t = 0.0;
+        for (int c = 0; c < 24; ++c) t += B[i][c] * ue[c];
+        eps[i] = t;
+      }
+      for (int i = 0; i < 6; ++i) {
+        double t = 0.0;
+        for (int k = 0; k < 6; ++k) t += D[i][k] * eps[k];
+        sig[i] += t * fac;
+      }
+    }
+    const double q = sig[0] * (sig[0] - 0.5 * sig[1] - 0.5 * sig[2]) +
+                     sig[1] * (-0.5 * sig[0] + sig[1] - 0.5 * sig[2]) +
+                     sig[2] * (-0.5 * sig[0] - 0.5 * sig[1] + sig[2]) +
+                     3.0 * (sig[3] * sig[3] + sig[4] * sig[4] + sig[5] * sig[5]);
+    mises[e] = std::sqrt(q);
+    if (stress)
+      for (int i = 0; i < 6; ++i) stress[6 * e + i] = sig[i];
   }
 }
 

@@ -80,6 +80,7 @@ def lib():
                                                 C.c_ulonglong, C.c_int, C.c_double, C.c_double,
                                                 C.c_double, C.c_double, C.c_double, dp]),
             "orc_num_threads": (C.c_int, []),
+            "orc_mises": (None, [vp, dp, dp, dp]),
             "orc_helm_create": (vp, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                      C.c_double, C.c_double]),
             "orc_helm_destroy": (None, [vp]), "orc_helm_numnp": (C.c_longlong, [vp]),
@@ -253,6 +254,15 @@ class Problem:
         dg = np.zeros(self.nelx)
         self.L.orc_sensitivity(self.h, P(np.ascontiguousarray(nodeval)), vnorm, P(df), P(dg))
         return df, dg
+
+    def mises(self, nodeval):
+        """Element-mean stress (nelx x 6) and von Mises stress of the nodal
+        values, densities as last set (Elastic::makeMisesStress)."""
+        nodeval = np.ascontiguousarray(nodeval, dtype=np.float64)
+        m = np.zeros(self.nelx)
+        st = np.zeros(6 * self.nelx)
+        self.L.orc_mises(self.h, P(nodeval), P(m), P(st))
+        return m, st.reshape(self.nelx, 6)
 
     def gaussian_field(self, seed, nblob=32, smin=4.0, smax=12.0, mean=0.3, lo=1e-3, hi=1.0):
         return gaussian_field(self.nx, self.ny, self.nz, seed, nblob, smin, smax, mean, lo, hi)

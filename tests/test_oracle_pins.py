@@ -464,3 +464,73 @@ def test_oc_matches_golden(oracle):
     F = oracle.Filter(pr, float(d["radius"]))
     x1, cnt, _ = F.oc_update(d["x"], d["dfdx"], float(d["g"]), d["dgdx"])
     assert np.max(np.abs(x1 - d["x_new"])) < 1e-14
+
+
+# ------------------------------------------------------------ Mises stress
+def _d0(nu):
+    lam = nu / ((1 + nu) * (1 - 2 * nu))
+    mu = 1 / (2 * (1 + nu))
+    D = np.full((3, 3), lam) + 2 * mu * np.eye(3)
+    return np.block([[D, np.zeros((3, 3))], [np.zeros((3, 3)), mu * np.eye(3)]])
+
+
+def _vm(s):
+    return np.sqrt(s[:, 0] ** 2 + s[:, 1] ** 2 + s[:, 2] ** 2 - s[:, 0] * s[:, 1] -
+                   s[:, 1] * s[:, 2] - s[:, 2] * s[:, 0] +
+                   3 * (s[:, 3] ** 2 + s[:, 4] ** 2 + s[:, 5] ** 2))
+
+
+def test_mises_affine_field_closed_form(oracle):
+    """problem/elasticity.cpp:166-209: for u = G X the trilinear interpolation is
+    exact, the strain is constant eps = [Gxx, Gyy, Gzz, Gxy+Gyx, Gyz+Gzy, Gzx+Gxz],
+    stress = E(x) D0 eps and mises = sqrt(s^T V s)."""
+    pr = oracle.Problem(4, 3, 2, 2.0, 4.5, 1.0, young1=1e-3, young2=2.0, poisson=0.3,
+                        penalty=3.0)
+    pr.make_dof()
+    rng = np.random.default_rng(5)
+    x = rng.uniform(0.05, 1.0, pr.nelx)
+    pr.set_density(x)
+    G = rng.standard_normal((3, 3))
+    X = pr.coords()
+    nodeval = (X @ G.T).reshape(-1)
+    m, st = pr.mises(nodeval)
+    eps = np.array([G[0, 0], G[1, 1], G[2, 2], G[0, 1] + G[1, 0], G[1, 2] + G[2, 1],
+                    G[2, 0] + G[0, 2]])
+    E = (1 - x ** 3) * 1e-3 + x ** 3 * 2.0
+    ref = E[:, None] * (_d0(0.3) @ eps)[None, :]
+    assert np.max(np.abs(st - ref)) < 1e-13 * np.max(np.abs(ref))
+    assert np.max(np.abs(m - _vm(ref)) / _vm(ref)) < 1e-13
+
+
+def test_mises_rigid_motion_is_stress_free(oracle):
+    pr = oracle.cantilever(5, 3, 3)
+    pr.set_density(np.full(pr.nelx, 0.7))
+    X = pr.coords()
+    w = np.array([0.3, -0.2, 0.5])          # infinitesimal rotation + translation
+    u = np.cross(w, X) + np.array([1.0, -2.0, 0.5])
+    m, st = pr.mises(u.reshape(-1))
+    assert np.max(m) < 1e-13 and np.max(np.abs(st)) < 1e-13
+
+
+def test_mises_gauss_mean_is_centroid_value(oracle):
+    """The mean of B over the 2x2x2 points equals B at the element centre
+    (B is multilinear), so the stress of a random field is E D0 B(0) u_e."""
+    nx, ny, nz, hx, hy, hz = 3, 2, 2, 0.5, 1.0, 2.0
+    pr = oracle.Problem(nx, ny, nz, nx * hx, ny * hy, nz * hz)
+    pr.make_dof()
+    x = np.full(pr.nelx, 0.6)
+    pr.set_density(x)
+    u = np.random.default_rng(6).standard_normal(3 * pr.numnp)
+    m, st = pr.mises(u)
+    conn = pr.connectivity()
+    sgn = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
+                    [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]], dtype=float)
+    dN = sgn / (4 * np.array([hx, hy, hz]))  # dN_a/dX at the centre
+    E = (1 - 0.6 ** 3) * 1e-9 + 0.6 ** 3 * 1.0
+    for e in range(pr.nelx):
+        ue = u.reshape(-1, 3)[conn[e]]
+        Gr = ue.T @ dN                       # du_i/dX_j
+        eps = np.array([Gr[0, 0], Gr[1, 1], Gr[2, 2], Gr[0, 1] + Gr[1, 0],
+                        Gr[1, 2] + Gr[2, 1], Gr[2, 0] + Gr[0, 2]])
+        ref = E * (_d0(0.3) @ eps)
+        assert np.max(np.abs(st[e] - ref)) < 1e-13 * np.max(np.abs(ref))
