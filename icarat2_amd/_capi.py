@@ -47,7 +47,7 @@ class SimpStats(C.Structure):
     _fields_ = [("seconds", C.c_double), ("solve_s", C.c_double), ("filter_s", C.c_double),
                 ("sens_s", C.c_double), ("update_s", C.c_double), ("pcg_iters", C.c_int64),
                 ("max_true_relres", C.c_double), ("train_s", C.c_double),
-                ("fallbacks", C.c_int64)]
+                ("fallbacks", C.c_int64), ("inner_iters", C.c_int64)]
 
 
 class LearnedOpts(C.Structure):
@@ -100,6 +100,15 @@ _SIGS = {
     "icarat_hip_filter": (C.c_int, [C.c_void_p, dp, dp, C.c_int32]),
     "icarat_hip_helm_setup": (C.c_int, [C.c_void_p, C.c_double]),
     "icarat_hip_mises": (C.c_int, [C.c_void_p, dp, dp]),
+    "icarat_hip_gcmma_set_params": (C.c_int, [C.c_void_p] + [C.c_double] * 5),
+    "icarat_hip_gcmma_reset": (C.c_int, [C.c_void_p]),
+    "icarat_hip_gcmma_outer": (C.c_int, [C.c_void_p, dp, dp, C.c_double, dp, C.c_double, dp,
+                                         C.c_double, C.c_double]),
+    "icarat_hip_gcmma_inner": (C.c_int, [C.c_void_p, dp, C.c_double, C.c_double, dp,
+                                         C.c_double, dp, C.c_double, dp, C.c_double,
+                                         C.c_double]),
+    "icarat_hip_gcmma_concheck": (C.c_int, [C.c_void_p, C.c_double, C.c_double,
+                                            C.POINTER(C.c_int32)]),
     "icarat_hip_helm_filter": (C.c_int, [C.c_void_p, dp, dp, C.c_int32]),
     "icarat_hip_threshold": (C.c_int, [C.c_void_p, dp, C.c_double, C.c_double, dp, C.c_int32]),
     "icarat_hip_oc_update": (C.c_int, [C.c_void_p, dp, dp, C.c_double, dp, C.c_double,
@@ -418,6 +427,34 @@ class Context:
         st = np.zeros(6 * self.nelx)
         self._chk(self.L.icarat_hip_mises(self.h, _p(m), _p(st)), "mises")
         return m, st.reshape(self.nelx, 6)
+
+    def gcmma_set_params(self, move=0.15, albefa=0.2, asyminit=0.2, asymdec=0.7, asyminc=1.2):
+        self._chk(self.L.icarat_hip_gcmma_set_params(self.h, move, albefa, asyminit, asymdec,
+                                                     asyminc), "gcmma_set_params")
+
+    def gcmma_reset(self):
+        self._chk(self.L.icarat_hip_gcmma_reset(self.h), "gcmma_reset")
+
+    def gcmma_outer(self, xval, f0x, df0dx, fx, dfdx, xmin=1e-4, xmax=1.0):
+        xm = np.zeros(self.nelx)
+        self._chk(self.L.icarat_hip_gcmma_outer(self.h, _p(xm), _p(_f64(xval)), f0x,
+                                                _p(_f64(df0dx)), fx, _p(_f64(dfdx)), xmin, xmax),
+                  "gcmma_outer")
+        return xm
+
+    def gcmma_inner(self, xmma, f0xnew, fxnew, xval, f0x, df0dx, fx, dfdx, xmin=1e-4,
+                    xmax=1.0):
+        xm = np.array(xmma, dtype=np.float64)
+        self._chk(self.L.icarat_hip_gcmma_inner(self.h, _p(xm), f0xnew, fxnew, _p(_f64(xval)),
+                                                f0x, _p(_f64(df0dx)), fx, _p(_f64(dfdx)), xmin,
+                                                xmax), "gcmma_inner")
+        return xm
+
+    def gcmma_concheck(self, f0xnew, fxnew):
+        ok = C.c_int32()
+        self._chk(self.L.icarat_hip_gcmma_concheck(self.h, f0xnew, fxnew, C.byref(ok)),
+                  "gcmma_concheck")
+        return bool(ok.value)
 
     def helm_setup(self, radius):
         """FilterHelmholtz(fem, element, node, radius) on this box."""

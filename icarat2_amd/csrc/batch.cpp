@@ -165,6 +165,8 @@ void Batch::solve(const icarat_solve_opts &o, icarat_solve_stats *st) {
 
 void Batch::simp_run(const icarat_simp_opts &o, double *const *x, double *const *hist,
                      icarat_simp_stats *st) {
+  if (o.optimizer == 2)
+    throw Error(kInvalidArgument, "the batched loop runs OC or MMA (GCMMA: icarat_hip_simp_run)");
   set_precond(o.solve);
   if (!x) throw Error(kInvalidArgument, "null design array");
   const int n = size();

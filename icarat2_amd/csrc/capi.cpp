@@ -8,6 +8,7 @@
 #include "../../include/icarat_hip.h"
 #include "batch.hpp"
 #include "context.hpp"
+#include "gcmma.hpp"
 #include "learned.hpp"
 #include "mma.hpp"
 #include "multigrid.hpp"
@@ -222,6 +223,43 @@ int icarat_hip_true_residual(icarat_hip_ctx *h, const double *u, double *relres)
 int icarat_hip_compliance_sensitivity(icarat_hip_ctx *h, double vnorm, double *cc, double *dfds,
                                       double *dgds) {
   return guard(h, [&](icarat_hip::Context &c) { c.compliance_sensitivity(vnorm, cc, dfds, dgds); });
+}
+
+int icarat_hip_gcmma_set_params(icarat_hip_ctx *h, double move, double albefa, double asyminit,
+                                double asymdec, double asyminc) {
+  return guard(h, [&](icarat_hip::Context &c) {
+    c.gcmma().set_params(move, albefa, asyminit, asymdec, asyminc);
+  });
+}
+
+int icarat_hip_gcmma_reset(icarat_hip_ctx *h) {
+  return guard(h, [&](icarat_hip::Context &c) { c.gcmma().reset(); });
+}
+
+int icarat_hip_gcmma_outer(icarat_hip_ctx *h, double *xmma, const double *xval, double f0x,
+                           const double *df0dx, double fx, const double *dfdx, double xmin,
+                           double xmax) {
+  if (!xmma || !xval || !df0dx || !dfdx) return fail(h, ICARAT_HIP_INVALID_ARGUMENT, "null argument");
+  return guard(h, [&](icarat_hip::Context &c) {
+    c.gcmma_host(0, xmma, 0.0, 0.0, xval, f0x, df0dx, fx, dfdx, xmin, xmax);
+  });
+}
+
+int icarat_hip_gcmma_inner(icarat_hip_ctx *h, double *xmma, double f0xnew, double fxnew,
+                           const double *xval, double f0x, const double *df0dx, double fx,
+                           const double *dfdx, double xmin, double xmax) {
+  if (!xmma || !xval || !df0dx || !dfdx) return fail(h, ICARAT_HIP_INVALID_ARGUMENT, "null argument");
+  return guard(h, [&](icarat_hip::Context &c) {
+    c.gcmma_host(1, xmma, f0xnew, fxnew, xval, f0x, df0dx, fx, dfdx, xmin, xmax);
+  });
+}
+
+int icarat_hip_gcmma_concheck(icarat_hip_ctx *h, double f0xnew, double fxnew,
+                              int32_t *conservative) {
+  if (!conservative) return fail(h, ICARAT_HIP_INVALID_ARGUMENT, "null argument");
+  return guard(h, [&](icarat_hip::Context &c) {
+    *conservative = c.gcmma().concheck(f0xnew, fxnew) ? 1 : 0;
+  });
 }
 
 int icarat_hip_mises(icarat_hip_ctx *h, double *mises, double *stress) {

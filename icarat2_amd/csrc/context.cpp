@@ -11,6 +11,7 @@
 #include "learned.hpp"
 #include "multigrid.hpp"
 #include "mixed_kernels.hpp"
+#include "gcmma.hpp"
 #include "helm.hpp"
 #include "mma.hpp"
 
@@ -747,8 +748,8 @@ void Context::simp_begin(const icarat_simp_opts &o, const double *x) {
   if (o.filter == 0 && !filter_ready_) throw Error(kNotReady, "filter_setup has not been called");
   if (o.filter == 1 && !helm_) throw Error(kNotReady, "helm_setup has not been called");
   if (o.n_iter < 0) throw Error(kInvalidArgument, "n_iter must be >= 0");
-  if (o.optimizer != 0 && o.optimizer != 1)
-    throw Error(kInvalidArgument, "optimizer must be 0 (OC) or 1 (MMA)");
+  if (o.optimizer != 0 && o.optimizer != 1 && o.optimizer != 2)
+    throw Error(kInvalidArgument, "optimizer must be 0 (OC), 1 (MMA) or 2 (GCMMA)");
   sync();
   simp_ = SimpState{};
   simp_.o = o;
@@ -852,7 +853,136 @@ void Context::simp_end(double *x, icarat_simp_stats *st) {
   if (st) *st = simp_.S;
 }
 
+DeviceGCMMA &Context::gcmma() {
+  ensure_device();
+  if (!gcmma_) gcmma_.reset(new DeviceGCMMA(*this));
+  return *gcmma_;
+}
+
+void Context::gcmma_host(int mode, double *xmma, double f0xnew, double fxnew, const double *xval,
+                         double f0x, const double *df0dx, double fx, const double *dfdx,
+                         double xmin, double xmax) {
+  DeviceGCMMA &G = gcmma();
+  const long long n = L_.nelx;
+  h2d(d_e[E_DES].p, xval, n, stream_);
+  h2d(d_e[E_DFDS].p, df0dx, n, stream_);
+  h2d(d_e[E_DGDS].p, dfdx, n, stream_);
+  if (mode == 0) {
+    G.outer(d_e[E_XF], d_e[E_DES], f0x, d_e[E_DFDS], fx, d_e[E_DGDS], xmin, xmax);
+  } else {
+    h2d(d_e[E_XF].p, xmma, n, stream_);
+    G.inner(d_e[E_XF], f0xnew, fxnew, d_e[E_DES], f0x, d_e[E_DFDS], fx, d_e[E_DGDS], xmin, xmax);
+  }
+  d2h(xmma, d_e[E_XF].p, n, stream_);
+  sync();
+}
+
+double Context::gcmma_evaluate(const double *xd, bool sens, double *vol) {
+  const icarat_simp_opts &o = simp_.o;
+  const long long n = L_.nelx;
+  double t0 = now_s();
+  loop_filter(xd, d_e[E_XF], 0);
+  double *xp = d_e[E_XF];
+  if (o.use_projection) {
+    launch_threshold(n, d_e[E_XF], simp_.beta, 0.5, d_e[E_XP], 0, stream_);
+    launch_threshold(n, d_e[E_XF], simp_.beta, 0.5, d_e[E_THD], 1, stream_);
+    xp = d_e[E_XP];
+  }
+  d2d(d_dens.p, xp, n, stream_);
+  set_density_device();
+  launch_volume(n, d_dens, vol_e_, d_scal.p + 1, d_partials, d_counter, stream_);
+  d2h(h_scal.p + 1, d_scal.p + 1, 1, stream_);
+  sync();
+  *vol = h_scal.p[1];
+  simp_.S.filter_s += now_s() - t0;
+  double c = 0.0;
+  const int ncase = prob_.num_load_cases();
+  for (int lc = 0; lc < ncase; ++lc) {
+    if (ncase > 1) use_load_case(lc);
+    const double t1 = now_s();
+    icarat_solve_stats ss{};
+    solve(o.solve, nullptr, &ss);
+    simp_.S.solve_s += now_s() - t1;
+    if (sens) {
+      simp_.c = 0.0;
+      simp_case_post(lc, ss);
+      c += simp_.c;
+    } else {
+      simp_.iters += ss.iters;
+      simp_.relres = std::max(simp_.relres, ss.true_relres);
+      simp_.S.pcg_iters += ss.iters;
+      simp_.S.max_true_relres = std::max(simp_.S.max_true_relres, ss.true_relres);
+      launch_dot(n3_, d_f, d_u, d_scal, d_partials, d_counter, stream_);
+      d2h(h_scal.p, d_scal.p, 1, stream_);
+      sync();
+      c += h_scal.p[0];
+    }
+  }
+  if (ncase > 1) use_load_case(prob_.load_case());
+  if (sens) {
+    const double t2 = now_s();
+    if (o.use_projection) {
+      launch_mul(n, d_e[E_THD], d_e[E_DFDS], stream_);
+      launch_mul(n, d_e[E_THD], d_e[E_DGDS], stream_);
+    }
+    loop_filter(d_e[E_DFDS], d_e[E_TMP], 1);
+    std::swap(d_e[E_DFDS], d_e[E_TMP]);
+    loop_filter(d_e[E_DGDS], d_e[E_TMP], 1);
+    std::swap(d_e[E_DGDS], d_e[E_TMP]);
+    sync();
+    simp_.S.filter_s += now_s() - t2;
+  }
+  return c;
+}
+
+void Context::gcmma_run(const icarat_simp_opts &o, double *x, double *hist,
+                        icarat_simp_stats *st) {
+  simp_begin(o, x);
+  DeviceGCMMA &G = gcmma();
+  G.reset();
+  DevBuf<double> xnew(L_.nelx);
+  for (int step = 1; step <= o.n_iter; ++step) {
+    // beta schedule of the GCMMA driver (opt_macro_gcmma.cpp:91-93)
+    const int bt = (int)std::max((step - 100) / 40.0, 1.0);
+    simp_.beta = std::min((double)bt, 8.0);
+    simp_.iters = 0;
+    simp_.relres = 0.0;
+    double vol = 0.0;
+    const double c = gcmma_evaluate(d_e[E_DES], true, &vol);
+    if (step == 1) simp_.V0 = vol;
+    const double g = vol - simp_.V0;
+    double t0 = now_s();
+    G.outer(xnew, d_e[E_DES], c, d_e[E_DFDS], g, d_e[E_DGDS], o.xmin, o.xmax);
+    simp_.S.update_s += now_s() - t0;
+    double vnew = 0.0;
+    double cnew = gcmma_evaluate(xnew, false, &vnew);
+    double gnew = vnew - simp_.V0;
+    bool ok = G.concheck(cnew, gnew);
+    for (int inner = 0; !ok && inner < 15; ++inner) {
+      t0 = now_s();
+      G.inner(xnew, cnew, gnew, d_e[E_DES], c, d_e[E_DFDS], g, d_e[E_DGDS], o.xmin, o.xmax);
+      simp_.S.update_s += now_s() - t0;
+      cnew = gcmma_evaluate(xnew, false, &vnew);
+      gnew = vnew - simp_.V0;
+      ok = G.concheck(cnew, gnew);
+      simp_.S.inner_iters += 1;
+    }
+    d2d(d_e[E_DES].p, xnew.p, L_.nelx, stream_);
+    if (hist) {
+      hist[4 * (step - 1) + 0] = c;
+      hist[4 * (step - 1) + 1] = vol;
+      hist[4 * (step - 1) + 2] = (double)simp_.iters;
+      hist[4 * (step - 1) + 3] = simp_.relres;
+    }
+  }
+  simp_end(x, st);
+}
+
 void Context::simp_run(const icarat_simp_opts &o, double *x, double *hist, icarat_simp_stats *st) {
+  if (o.optimizer == 2) {
+    gcmma_run(o, x, hist, st);
+    return;
+  }
   simp_begin(o, x);
   for (int step = 1; step <= o.n_iter; ++step) {
     simp_pre(step);

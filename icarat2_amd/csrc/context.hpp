@@ -15,6 +15,7 @@
 
 namespace icarat_hip {
 
+class DeviceGCMMA;
 class DeviceMMA;
 class HelmFilter;
 class LearnedPrecond;
@@ -148,6 +149,19 @@ class Context {
 
   std::unique_ptr<DeviceMMA> mma_;
   std::unique_ptr<HelmFilter> helm_;
+  std::unique_ptr<DeviceGCMMA> gcmma_;
+  DeviceGCMMA &gcmma();                   // created on first use
+  // GCMMA steps on host vectors: mode 0 OuterUpdate, 1 InnerUpdate (xmma in/out)
+  void gcmma_host(int mode, double *xmma, double f0xnew, double fxnew, const double *xval,
+                  double f0x, const double *df0dx, double fx, const double *dfdx, double xmin,
+                  double xmax);
+  // the GCMMA driver's loop (optimizer 2 of simp_run)
+  void gcmma_run(const icarat_simp_opts &o, double *x, double *hist, icarat_simp_stats *st);
+  // filter -> projection -> densities -> solve of every load case at the design
+  // xd (device); returns the compliance summed over the cases and the volume
+  // in *vol; with sens, also the sensitivities with the projection derivative
+  // and the chain-rule filter applied (E_DFDS, E_DGDS)
+  double gcmma_evaluate(const double *xd, bool sens, double *vol);
   // the design loop's filter (simp_.o.filter): cone or Helmholtz, device vectors
   void loop_filter(const double *in, double *out, int is_sens);
   // OC bisection whose volume callback filters with the Helmholtz filter

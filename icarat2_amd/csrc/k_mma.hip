@@ -107,7 +107,127 @@ __global__ __launch_bounds__(NT) void k_mma_dual(long long n, double lam, MmaArr
   store_partials<NT, 2>(v, partials, sh);
 }
 
+// GCMMA subproblem (reference GCMMASolver.cpp:486-560 Asymp when asymp != 0,
+// then :593-640 GenSub): MMA's asymptotes and history shift, move limits
+// without a second clamp to the bounds, raa0 in the objective and raa1 in the
+// constraint.  Partials: sum p0/(U-x) + q0/(x-L) and sum p1/(U-x) + q1/(x-L).
+__global__ __launch_bounds__(NT) void k_gcmma_sub(long long n, const double *__restrict__ x,
+                                                  const double *__restrict__ df0,
+                                                  const double *__restrict__ dg, MmaParams P,
+                                                  double raa0, double raa1, int asymp,
+                                                  MmaArrays A, double *partials) {
+  __shared__ double sh[8];
+  const double span = P.xmax - P.xmin;
+  const double w = fmax(P.xmamieps, span);
+  const double winv = 1.0 / w;
+  double s0 = 0.0, s1 = 0.0;
+  const long long stride = (long long)gridDim.x * NT;
+  for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += stride) {
+    const double xi = x[i];
+    double lo = A.low[i], up = A.upp[i];
+    if (asymp) {
+      const double xo1 = A.xold1[i], xo2 = A.xold2[i];
+      if (P.iter < 3) {
+        lo = xi - P.asyminit * span;
+        up = xi + P.asyminit * span;
+      } else {
+        const double osc = (xi - xo1) * (xo1 - xo2);
+        const double gam = osc < 0.0 ? P.asymdec : (osc > 0.0 ? P.asyminc : 1.0);
+        lo = xi - gam * (xo1 - lo);
+        up = xi + gam * (up - xo1);
+        lo = fmin(fmax(lo, xi - 100.0 * w), xi - 1.0e-5 * w);
+        up = fmin(fmax(up, xi + 1.0e-5 * w), xi + 100.0 * w);
+        const double xl = P.xmin - 1.0e-6, xh = P.xmax + 1.0e-6;
+        if (xi < xl) {
+          lo = xi - (xh - xi) / 0.9;
+          up = xi + (xh - xi) / 0.9;
+        }
+        if (xi > xh) {
+          lo = xi - (xi - xl) / 0.9;
+          up = xi + (xi - xl) / 0.9;
+        }
+      }
+      A.low[i] = lo;
+      A.upp[i] = up;
+      A.xold2[i] = xo1;
+      A.xold1[i] = xi;
+    }
+    A.alpha[i] = fmax(fmax(P.xmin, lo + P.albefa * (xi - lo)), xi - P.move * span);
+    A.beta[i] = fmin(fmin(P.xmax, up - P.albefa * (up - xi)), xi + P.move * span);
+    const double ux = up - xi, xl2 = xi - lo;
+    const double uxinv = 1.0 / ux, xlinv = 1.0 / xl2;
+    const double ux2 = ux * ux, xl2sq = xl2 * xl2;
+    const double f = df0[i], g = dg[i];
+    const double pq0 = 0.001 * fabs(f) + raa0 * winv;
+    const double pq1 = 0.001 * fabs(g) + raa1 * winv;
+    const double p0 = ux2 * (fmax(0.0, f) + pq0), q0 = xl2sq * (fmax(0.0, -f) + pq0);
+    const double p1 = ux2 * (fmax(0.0, g) + pq1), q1 = xl2sq * (fmax(0.0, -g) + pq1);
+    A.p0[i] = p0;
+    A.q0[i] = q0;
+    A.p1[i] = p1;
+    A.q1[i] = q1;
+    s0 += p0 * uxinv + q0 * xlinv;
+    s1 += p1 * uxinv + q1 * xlinv;
+  }
+  double v[2] = {s0, s1};
+  store_partials<NT, 2>(v, partials, sh);
+}
+
+// ComputeApprox (GCMMASolver.cpp:642-661) without the constants: the two
+// approximation sums at xm
+__global__ __launch_bounds__(NT) void k_gcmma_approx(long long n, const double *__restrict__ xm,
+                                                     MmaArrays A, double *partials) {
+  __shared__ double sh[8];
+  double s0 = 0.0, s1 = 0.0;
+  const long long stride = (long long)gridDim.x * NT;
+  for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += stride) {
+    const double uxinv = 1.0 / (A.upp[i] - xm[i]), xlinv = 1.0 / (xm[i] - A.low[i]);
+    s0 += A.p0[i] * uxinv + A.q0[i] * xlinv;
+    s1 += A.p1[i] * uxinv + A.q1[i] * xlinv;
+  }
+  double v[2] = {s0, s1};
+  store_partials<NT, 2>(v, partials, sh);
+}
+
+// the raa growth coefficient of RaaUpdate (GCMMASolver.cpp:562-576)
+__global__ __launch_bounds__(NT) void k_gcmma_raacof(long long n, const double *__restrict__ xm,
+                                                     const double *__restrict__ xv, double winv,
+                                                     MmaArrays A, double *partials) {
+  __shared__ double sh[4];
+  double s = 0.0;
+  const long long stride = (long long)gridDim.x * NT;
+  for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += stride) {
+    const double up = A.upp[i], lo = A.low[i], d = xm[i] - xv[i];
+    s += (d / (up - xm[i])) * (d / (xm[i] - lo)) * ((up - lo) * winv);
+  }
+  double v[1] = {s};
+  store_partials<NT, 1>(v, partials, sh);
+}
+
 }  // namespace
+
+void launch_gcmma_sub(long long n, const double *x, const double *df0, const double *dg,
+                      const MmaParams &P, double raa0, double raa1, int asymp, const MmaArrays &A,
+                      double *partials, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_gcmma_sub, dim3(BLOCKS), dim3(NT), 0, s, n, x, df0, dg, P, raa0, raa1,
+                     asymp, A, partials);
+  HIPCHK(hipGetLastError());
+  launch_finalize(kFinOut, 2, partials, BLOCKS, nullptr, nullptr, out, s);
+}
+
+void launch_gcmma_approx(long long n, const double *xm, const MmaArrays &A, double *partials,
+                         double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_gcmma_approx, dim3(BLOCKS), dim3(NT), 0, s, n, xm, A, partials);
+  HIPCHK(hipGetLastError());
+  launch_finalize(kFinOut, 2, partials, BLOCKS, nullptr, nullptr, out, s);
+}
+
+void launch_gcmma_raacof(long long n, const double *xm, const double *xv, double winv,
+                         const MmaArrays &A, double *partials, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_gcmma_raacof, dim3(BLOCKS), dim3(NT), 0, s, n, xm, xv, winv, A, partials);
+  HIPCHK(hipGetLastError());
+  launch_finalize(kFinOut, 1, partials, BLOCKS, nullptr, nullptr, out, s);
+}
 
 void launch_mma_gensub(long long n, const double *x, const double *dfdx, const double *dgdx,
                        const MmaParams &P, const MmaArrays &A, double *partials,

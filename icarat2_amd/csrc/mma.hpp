@@ -8,6 +8,14 @@ namespace icarat_hip {
 
 class Context;
 
+// The dual interior-point iteration of MMA / GCMMA for one constraint
+// (SolveDIP, reference MMASolver.cpp:88-154, GCMMASolver.cpp:128-196): the
+// element passes on the device (x(lambda) and the dual sums), the 1x1 Newton
+// system on the host.  b is the constraint constant of the subproblem; x
+// receives x(lambda*).  Returns the outer (epsilon) loops.
+int solve_dual_m1(Context &ctx, long long n, const MmaArrays &A, double *x, double b, double a,
+                  double c, int *newton_steps);
+
 class DeviceMMA {
  public:
   explicit DeviceMMA(Context &ctx);

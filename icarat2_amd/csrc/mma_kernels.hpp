@@ -28,4 +28,18 @@ void launch_mma_gensub(long long n, const double *x, const double *dfdx, const d
 void launch_mma_dual(long long n, double lam, const MmaArrays &A, double *x, double *partials,
                      unsigned *counter, double *out, hipStream_t s);
 
+// GCMMA (reference optimization/opt-solver/GCMMASolver.cpp) for one constraint:
+// the subproblem at x (asymp != 0: move the asymptotes and shift the history
+// first, as OuterUpdate does); out[0] = sum p0/(U-x) + q0/(x-L), out[1] = the
+// same of p1, q1.
+void launch_gcmma_sub(long long n, const double *x, const double *df0, const double *dg,
+                      const MmaParams &P, double raa0, double raa1, int asymp, const MmaArrays &A,
+                      double *partials, double *out, hipStream_t s);
+// out[0..1] = the two approximation sums at xm
+void launch_gcmma_approx(long long n, const double *xm, const MmaArrays &A, double *partials,
+                         double *out, hipStream_t s);
+// out[0] = sum ((xm-xv)/(U-xm)) ((xm-xv)/(xm-L)) (U-L) winv
+void launch_gcmma_raacof(long long n, const double *xm, const double *xv, double winv,
+                         const MmaArrays &A, double *partials, double *out, hipStream_t s);
+
 }  // namespace icarat_hip

@@ -192,10 +192,33 @@ int icarat_hip_mma_reset(icarat_hip_ctx *ctx);
 int icarat_hip_mma_update(icarat_hip_ctx *ctx, double *x, const double *dfdx, double g,
                           const double *dgdx, double xmin, double xmax, int32_t *outer);
 
+/* GCMMASolver(n = nelx, m = 1, a = 0, c = 1000, d = 0) (optimization/opt-solver/
+ * GCMMASolver.hpp:62-85) on the device, for the single volume constraint fx:
+ * SetMoveLimit / SetOscillation / SetAsymptotes, and Reset (which here also
+ * restores raa0 = raa = 1).  outer = OuterUpdate(xmma, xval, f0x, df0dx, &fx,
+ * dfdx, xmin, xmax); inner = InnerUpdate(xmma, f0xnew, &fxnew, xval, f0x, df0dx,
+ * &fx, dfdx, xmin, xmax) with xmma in/out; concheck = ConCheck(f0xnew, &fxnew).
+ * Bounds are uniform. */
+int icarat_hip_gcmma_set_params(icarat_hip_ctx *ctx, double move, double albefa,
+                                double asyminit, double asymdec, double asyminc);
+int icarat_hip_gcmma_reset(icarat_hip_ctx *ctx);
+int icarat_hip_gcmma_outer(icarat_hip_ctx *ctx, double *xmma, const double *xval, double f0x,
+                           const double *df0dx, double fx, const double *dfdx, double xmin,
+                           double xmax);
+int icarat_hip_gcmma_inner(icarat_hip_ctx *ctx, double *xmma, double f0xnew, double fxnew,
+                           const double *xval, double f0x, const double *df0dx, double fx,
+                           const double *dfdx, double xmin, double xmax);
+int icarat_hip_gcmma_concheck(icarat_hip_ctx *ctx, double f0xnew, double fxnew,
+                              int32_t *conservative);
+
 /* ---- SIMP driver loop ------------------------------------------------------ */
 typedef struct icarat_simp_opts {
   int32_t n_iter;          /* design iterations */
-  int32_t optimizer;       /* 0 = OC, 1 = MMA (m = 1 volume constraint) */
+  int32_t optimizer;       /* 0 = OC, 1 = MMA (m = 1 volume constraint), 2 = GCMMA (the loop of
+                              opt-macro-gcmma/opt_macro_gcmma.cpp:84-212: projection beta =
+                              min(int(max((k - 100) / 40, 1)), 8) when use_projection, constraint
+                              volume - V0, inner iterations until conservative; icarat_hip_simp_run
+                              only) */
   int32_t use_projection;  /* 0 = plain method (no tanh projection) */
   int32_t filter;          /* 0 = cone filter (icarat_hip_filter_setup, optimization/filter.hpp),
                               1 = Helmholtz filter (icarat_hip_helm_setup, filter_helm.hpp; the
@@ -211,6 +234,7 @@ typedef struct icarat_simp_stats {
   double max_true_relres;  /* largest FP64 true residual over the loop */
   double train_s;          /* learned preconditioner: time spent training the network */
   int64_t fallbacks;       /* learned preconditioner: solves that fell back to Jacobi-PCG */
+  int64_t inner_iters;     /* GCMMA: inner iterations, all design iterations */
 } icarat_simp_stats;
 
 /* The reference drivers' loop (sample/algorithm/opt-macro-oc/opt_macro_oc.cpp:91-178,

@@ -658,6 +658,12 @@ struct MMA {
   // Move asymptotes and build the convex separable approximation.
   void subproblem(const double *x, const double *df, const double *g,
                   const double *dg, const double *xmin, const double *xmax) {
+    move_asymptotes(x, xmin, xmax);
+    approximation(x, df, g, dg, xmin, xmax);
+  }
+
+  // Asymptotes L, U of this outer iteration (reference MMASolver.cpp:404-458).
+  void move_asymptotes(const double *x, const double *xmin, const double *xmax) {
     ++iter;
     for (int i = 0; i < n; ++i) {
       double span = xmax[i] - xmin[i];
@@ -683,6 +689,11 @@ struct MMA {
         }
       }
     }
+  }
+
+  // Move limits and the approximation's p, q and b (MMASolver.cpp:460-512).
+  void approximation(const double *x, const double *df, const double *g, const double *dg,
+                     const double *xmin, const double *xmax) {
     for (int i = 0; i < n; ++i) {
       double span = xmax[i] - xmin[i];
       double al = std::max(xmin[i], low[i] + albefa * (x[i] - low[i]));
@@ -862,6 +873,128 @@ struct MMA {
     xold2 = xold1;
     std::copy(x, x + n, xold1.begin());
     solve_dual(x);
+  }
+};
+
+// ----------------------------------------------------------------- GCMMA --
+// Globally convergent MMA (K. Svanberg, A class of globally convergent
+// optimization methods based on conservative convex separable approximations,
+// SIAM J. Optim. 12, 2002) with the reference's choices (optimization/
+// opt-solver/GCMMASolver.cpp:41-661): a = 0, c = 1000, d = 0; raa0 = raa_j = 1
+// at construction, scaled by 0.1 (floor 1e-6) at every outer iteration and
+// grown by the inner iterations until the approximations are conservative.
+// The dual interior-point solver is the one of MMA above.
+struct GCMMA : MMA {
+  double raa0eps = 1.0e-6, raaeps = 1.0e-6;
+  std::vector<double> raa, r, fapp;
+  double r0 = 0.0, f0app = 0.0;
+  GCMMA(int n_, int m_, double ai, double ci, double di)
+      : MMA(n_, m_, ai, ci, di), raa(m_, 1.0), r(m_), fapp(m_) {
+    raa0 = 1.0;
+  }
+
+  // Asymp (GCMMASolver.cpp:486-560): MMA's asymptotes, then raa0/raa shrink
+  void asymptotes(const double *x, const double *xmin, const double *xmax) {
+    move_asymptotes(x, xmin, xmax);
+    raa0 = std::max(raa0eps, 0.1 * raa0);
+    for (int j = 0; j < m; ++j) raa[j] = std::max(raaeps, 0.1 * raa[j]);
+  }
+
+  // GenSub (GCMMASolver.cpp:593-640): move limits without clamping to the
+  // bounds again, raa0 in the objective and raa_j in constraint j, and the
+  // constants r0, r_j that make the approximations exact at x.
+  void gensub(const double *x, double f0x, const double *df0, const double *fx,
+              const double *dfx, const double *xmin, const double *xmax) {
+    r0 = 0.0;
+    std::fill(r.begin(), r.end(), 0.0);
+    for (int i = 0; i < n; ++i) {
+      const double span = xmax[i] - xmin[i];
+      alpha[i] = std::max(std::max(xmin[i], low[i] + albefa * (x[i] - low[i])),
+                          x[i] - move * span);
+      beta[i] = std::min(std::min(xmax[i], upp[i] - albefa * (upp[i] - x[i])),
+                         x[i] + move * span);
+      const double uxinv = 1.0 / (upp[i] - x[i]), xlinv = 1.0 / (x[i] - low[i]);
+      const double inv = 1.0 / std::max(xmamieps, span);
+      const double ux2 = std::pow(upp[i] - x[i], 2.0), xl2 = std::pow(x[i] - low[i], 2.0);
+      const double pq0 = 0.001 * std::abs(df0[i]) + raa0 * inv;
+      p0[i] = ux2 * (std::max(0.0, df0[i]) + pq0);
+      q0[i] = xl2 * (std::max(0.0, -1.0 * df0[i]) + pq0);
+      r0 += p0[i] * uxinv + q0[i] * xlinv;
+      for (int j = 0; j < m; ++j) {
+        const double gij = dfx[(size_t)i * m + j];
+        const double pq = 0.001 * std::abs(gij) + raa[j] * inv;
+        pij[(size_t)i * m + j] = ux2 * (std::max(0.0, gij) + pq);
+        qij[(size_t)i * m + j] = xl2 * (std::max(0.0, -1.0 * gij) + pq);
+        r[j] += pij[(size_t)i * m + j] * uxinv + qij[(size_t)i * m + j] * xlinv;
+      }
+    }
+    r0 = f0x - r0;
+    for (int j = 0; j < m; ++j) {
+      r[j] = fx[j] - r[j];
+      b[j] = -r[j];
+    }
+  }
+
+  // ComputeApprox (GCMMASolver.cpp:642-661): the approximations at xmma
+  void approx(const double *xm) {
+    f0app = 0.0;
+    std::fill(fapp.begin(), fapp.end(), 0.0);
+    for (int i = 0; i < n; ++i) {
+      const double uxinv = 1.0 / (upp[i] - xm[i]), xlinv = 1.0 / (xm[i] - low[i]);
+      f0app += p0[i] * uxinv + q0[i] * xlinv;
+      for (int j = 0; j < m; ++j)
+        fapp[j] += pij[(size_t)i * m + j] * uxinv + qij[(size_t)i * m + j] * xlinv;
+    }
+    f0app += r0;
+    for (int j = 0; j < m; ++j) fapp[j] += r[j];
+  }
+
+  // RaaUpdate (GCMMASolver.cpp:562-591): grow raa where the approximation
+  // under-estimated the true value at xmma
+  void raa_update(const double *xm, const double *xv, double f0new, const double *fnew,
+                  const double *xmin, const double *xmax) {
+    double cof = 0.0;
+    for (int i = 0; i < n; ++i) {
+      const double w = std::max(xmamieps, xmax[i] - xmin[i]);
+      const double a1 = (xm[i] - xv[i]) / (upp[i] - xm[i]);
+      const double a2 = (xm[i] - xv[i]) / (xm[i] - low[i]);
+      cof += a1 * a2 * ((upp[i] - low[i]) / w);
+    }
+    cof = std::max(1e-12, cof);
+    if (f0new > f0app + 0.5 * epsimin)
+      raa0 = std::min(1.1 * (raa0 + (1.0 / cof) * (f0new - f0app)), 10.0 * raa0);
+    for (int j = 0; j < m; ++j)
+      if (fnew[j] > fapp[j] + 0.5 * epsimin)
+        raa[j] = std::min(1.1 * (raa[j] + (1.0 / cof) * (fnew[j] - fapp[j])), 10.0 * raa[j]);
+  }
+
+  bool conservative(double f0new, const double *fnew) const {
+    if (f0app + epsimin < f0new) return false;
+    for (int j = 0; j < m; ++j)
+      if (fapp[j] + epsimin < fnew[j]) return false;
+    return true;
+  }
+
+  void outer_update(double *xm, const double *xv, double f0x, const double *df0,
+                    const double *fx, const double *dfx, const double *xmin,
+                    const double *xmax) {
+    asymptotes(xv, xmin, xmax);
+    gensub(xv, f0x, df0, fx, dfx, xmin, xmax);
+    xold2 = xold1;
+    std::copy(xv, xv + n, xold1.begin());
+    std::copy(xv, xv + n, xm);
+    solve_dual(xm);
+    approx(xm);
+  }
+
+  void inner_update(double *xm, double f0new, const double *fnew, const double *xv, double f0x,
+                    const double *df0, const double *fx, const double *dfx,
+                    const double *xmin, const double *xmax) {
+    raa_update(xm, xv, f0new, fnew, xmin, xmax);
+    gensub(xv, f0x, df0, fx, dfx, xmin, xmax);
+    std::copy(xv, xv + n, xm);
+    solve_dual(xm);
+    approx(xm);
   }
 };
 
@@ -1411,6 +1544,122 @@ void orc_mma_state(void *h, double *low, double *upp) {
   MMA &M = *(MMA *)h;
   std::copy(M.low.begin(), M.low.end(), low);
   std::copy(M.upp.begin(), M.upp.end(), upp);
+}
+
+void *orc_gcmma_create(int n, int m, double a, double c, double d) {
+  return new GCMMA(n, m, a, c, d);
+}
+void orc_gcmma_destroy(void *h) { delete (GCMMA *)h; }
+void orc_gcmma_set(void *h, double move, double albefa, double asyminit, double asymdec,
+                   double asyminc) {
+  GCMMA &G = *(GCMMA *)h;
+  G.move = move; G.albefa = albefa; G.asyminit = asyminit; G.asymdec = asymdec; G.asyminc = asyminc;
+}
+// GCMMASolver::OuterUpdate / InnerUpdate / ConCheck (GCMMASolver.hpp:69-85)
+void orc_gcmma_outer(void *h, double *xmma, const double *xval, double f0x, const double *df0dx,
+                     const double *fx, const double *dfdx, const double *xmin,
+                     const double *xmax) {
+  ((GCMMA *)h)->outer_update(xmma, xval, f0x, df0dx, fx, dfdx, xmin, xmax);
+}
+void orc_gcmma_inner(void *h, double *xmma, double f0xnew, const double *fxnew,
+                     const double *xval, double f0x, const double *df0dx, const double *fx,
+                     const double *dfdx, const double *xmin, const double *xmax) {
+  ((GCMMA *)h)->inner_update(xmma, f0xnew, fxnew, xval, f0x, df0dx, fx, dfdx, xmin, xmax);
+}
+int orc_gcmma_concheck(void *h, double f0xnew, const double *fxnew) {
+  return ((GCMMA *)h)->conservative(f0xnew, fxnew) ? 1 : 0;
+}
+// {raa0, raa[0..m), f0app, fapp[0..m)}
+void orc_gcmma_state(void *h, double *out) {
+  const GCMMA &G = *(GCMMA *)h;
+  out[0] = G.raa0;
+  for (int j = 0; j < G.m; ++j) out[1 + j] = G.raa[j];
+  out[1 + G.m] = G.f0app;
+  for (int j = 0; j < G.m; ++j) out[2 + G.m + j] = G.fapp[j];
+}
+
+// The GCMMA driver's loop (sample/algorithm/opt-macro-gcmma/opt_macro_gcmma.cpp:
+// 84-212): filter -> projection with beta = min(int(max((k - 100) / 40, 1)), 8)
+// (none when use_projection = 0) -> solve -> f = compliance, g = volume - V0 ->
+// sensitivities with the projection derivative -> chain rule -> OuterUpdate;
+// then the new design is evaluated (filter, projection, solve) and inner
+// iterations (at most 15) run until the approximations are conservative.
+// hist per iteration: {compliance, volume, PCG iterations of all solves,
+// inner iterations}.
+int orc_simp_run_gcmma(void *h, void *f, const orc_simp_opts *o, double *x, double *hist,
+                       orc_simp_stats *st) {
+  Problem &P = *(Problem *)h;
+  Filter &F = *(Filter *)f;
+  const i64 n = P.nelx;
+  orc_simp_stats S;
+  std::memset(&S, 0, sizeof(S));
+  const double t_all = now_s();
+  std::vector<double> xf(n), xt(n), thd(n, 1.0), dfds(n), dgds(n), tmp(n), xnew(n);
+  std::vector<double> u(P.numeq), nodeval(P.numnp * 3);
+  std::vector<double> xmin(n, o->xmin), xmax(n, o->xmax);
+  GCMMA G((int)n, 1, 0.0, 1000.0, 0.0);
+  double V0 = 0.0;
+  long long its = 0;
+  // filter, projection, solve at design xd; returns compliance, sets vol
+  auto evaluate = [&](const double *xd, double beta, double &vol) {
+    density_filter(F, xd, xf.data(), 0);
+    const double *xp = xf.data();
+    if (o->use_projection) {
+      threshold(n, xf.data(), beta, 0.5, xt.data());
+      threshold_deriv(n, xf.data(), beta, 0.5, thd.data());
+      xp = xt.data();
+    }
+    orc_set_density(h, xp);
+    double rn = 0.0;
+    const int it = orc_scaling_cg(h, P.fext.data(), u.data(), o->cg_tol, o->cg_maxit, &rn);
+    its += it;
+    S.pcg_iters += it;
+    vol = 0.0;
+    for (i64 e = 0; e < n; ++e) vol += xp[e] * P.vol_e;
+    return orc_compliance(h, u.data());
+  };
+  for (int step = 1; step <= o->n_iter; ++step) {
+    its = 0;
+    const int bt = (int)std::max((step - 100) / 40.0, 1.0);
+    const double beta = std::min((double)bt, 8.0);
+    double vol = 0.0;
+    const double c = evaluate(x, beta, vol);
+    if (step == 1) V0 = vol;
+    const double g = vol - V0;
+    orc_node_values(h, u.data(), nodeval.data());
+    orc_sensitivity(h, nodeval.data(), 1.0, dfds.data(), dgds.data());
+    for (i64 e = 0; e < n; ++e) {
+      dfds[e] *= thd[e];
+      dgds[e] *= thd[e];
+    }
+    density_filter(F, dfds.data(), tmp.data(), 1);
+    dfds.swap(tmp);
+    density_filter(F, dgds.data(), tmp.data(), 1);
+    dgds.swap(tmp);
+    G.outer_update(xnew.data(), x, c, dfds.data(), &g, dgds.data(), xmin.data(), xmax.data());
+    double vnew = 0.0;
+    double cnew = evaluate(xnew.data(), beta, vnew);
+    double gnew = vnew - V0;
+    bool ok = G.conservative(cnew, &gnew);
+    int inner = 0;
+    for (; !ok && inner < 15; ++inner) {
+      G.inner_update(xnew.data(), cnew, &gnew, x, c, dfds.data(), &g, dgds.data(), xmin.data(),
+                     xmax.data());
+      cnew = evaluate(xnew.data(), beta, vnew);
+      gnew = vnew - V0;
+      ok = G.conservative(cnew, &gnew);
+    }
+    std::copy(xnew.begin(), xnew.end(), x);
+    if (hist) {
+      hist[4 * (step - 1) + 0] = c;
+      hist[4 * (step - 1) + 1] = vol;
+      hist[4 * (step - 1) + 2] = (double)its;
+      hist[4 * (step - 1) + 3] = inner;
+    }
+  }
+  S.total_s = now_s() - t_all;
+  if (st) *st = S;
+  return 0;
 }
 
 // One SIMP design loop as the reference drivers run it (filter -> optional

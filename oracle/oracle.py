@@ -81,6 +81,16 @@ def lib():
                                                 C.c_double, C.c_double, C.c_double, dp]),
             "orc_num_threads": (C.c_int, []),
             "orc_mises": (None, [vp, dp, dp, dp]),
+            "orc_gcmma_create": (vp, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double]),
+            "orc_gcmma_destroy": (None, [vp]),
+            "orc_gcmma_set": (None, [vp] + [C.c_double] * 5),
+            "orc_gcmma_outer": (None, [vp, dp, dp, C.c_double, dp, dp, dp, dp, dp]),
+            "orc_gcmma_inner": (None, [vp, dp, C.c_double, dp, dp, C.c_double, dp, dp, dp, dp,
+                                       dp]),
+            "orc_gcmma_concheck": (C.c_int, [vp, C.c_double, dp]),
+            "orc_gcmma_state": (None, [vp, dp]),
+            "orc_simp_run_gcmma": (C.c_int, [vp, vp, C.POINTER(SimpOpts), dp, dp,
+                                             C.POINTER(SimpStats)]),
             "orc_helm_create": (vp, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                      C.c_double, C.c_double]),
             "orc_helm_destroy": (None, [vp]), "orc_helm_numnp": (C.c_longlong, [vp]),
@@ -116,6 +126,14 @@ def ref_lib():
         L.ref_mma_update.argtypes = [vp, dp, dp, dp, dp, dp, dp]
         L.ref_oc_update.argtypes = [C.c_int, dp, dp, C.c_double, dp, dp, dp, C.c_void_p,
                                     C.c_void_p]
+        L.ref_gcmma_create.restype = vp
+        L.ref_gcmma_create.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double]
+        L.ref_gcmma_destroy.argtypes = [vp]
+        L.ref_gcmma_set.argtypes = [vp] + [C.c_double] * 5
+        L.ref_gcmma_outer.argtypes = [vp, dp, dp, C.c_double, dp, dp, dp, dp, dp]
+        L.ref_gcmma_inner.argtypes = [vp, dp, C.c_double, dp, dp, C.c_double, dp, dp, dp, dp, dp]
+        L.ref_gcmma_concheck.restype = C.c_int
+        L.ref_gcmma_concheck.argtypes = [vp, C.c_double, dp]
         _REF = L
     return _REF
 
@@ -426,6 +444,66 @@ class MMA:
         if getattr(self, "h", None):
             (self.L.ref_mma_destroy if self.ref else self.L.orc_mma_destroy)(self.h)
             self.h = None
+
+
+class GCMMA:
+    """GCMMASolver(n, m, a, c, d) (reference optimization/opt-solver/GCMMASolver.hpp);
+    reference=True drives the compiled reference in oracle/_ref."""
+
+    def __init__(self, n, m, a=0.0, c=1000.0, d=0.0, reference=False):
+        self.ref = reference
+        self.L = ref_lib() if reference else lib()
+        self.n, self.m = n, m
+        self.pre = "ref_gcmma_" if reference else "orc_gcmma_"
+        self.h = getattr(self.L, self.pre + "create")(n, m, a, c, d)
+
+    def _f(self, name):
+        return getattr(self.L, self.pre + name)
+
+    def set(self, move, albefa, asyminit, asymdec, asyminc):
+        self._f("set")(self.h, move, albefa, asyminit, asymdec, asyminc)
+
+    def outer(self, xval, f0x, df0dx, fx, dfdx, xmin, xmax):
+        xm = np.zeros(self.n)
+        a = [np.ascontiguousarray(v, dtype=np.float64) for v in (xval, df0dx, fx, dfdx, xmin, xmax)]
+        self._f("outer")(self.h, P(xm), P(a[0]), f0x, P(a[1]), P(a[2]), P(a[3]), P(a[4]), P(a[5]))
+        return xm
+
+    def inner(self, xmma, f0xnew, fxnew, xval, f0x, df0dx, fx, dfdx, xmin, xmax):
+        xm = np.array(xmma, dtype=np.float64)
+        a = [np.ascontiguousarray(v, dtype=np.float64)
+             for v in (fxnew, xval, df0dx, fx, dfdx, xmin, xmax)]
+        self._f("inner")(self.h, P(xm), f0xnew, P(a[0]), P(a[1]), f0x, P(a[2]), P(a[3]),
+                         P(a[4]), P(a[5]), P(a[6]))
+        return xm
+
+    def concheck(self, f0xnew, fxnew):
+        return bool(self._f("concheck")(self.h, f0xnew,
+                                        P(np.ascontiguousarray(fxnew, dtype=np.float64))))
+
+    def state(self):
+        """{raa0, raa[m], f0app, fapp[m]} (oracle only)."""
+        out = np.zeros(2 + 2 * self.m)
+        self.L.orc_gcmma_state(self.h, P(out))
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self._f("destroy")(self.h)
+            self.h = None
+
+
+def simp_run_gcmma(prob, filt, x0, n_iter, use_projection=True, cg_tol=1e-8, cg_maxit=10000,
+                   xmin=1e-4, xmax=1.0):
+    """The GCMMA driver's loop (opt-macro-gcmma/opt_macro_gcmma.cpp:84-212) with
+    the cone filter `filt`; hist rows {compliance, volume, PCG iterations,
+    inner iterations}."""
+    o = SimpOpts(n_iter, 2, 1 if use_projection else 0, cg_tol, cg_maxit, xmin, xmax)
+    x = np.array(x0, dtype=np.float64)
+    hist = np.zeros(4 * n_iter)
+    st = SimpStats()
+    lib().orc_simp_run_gcmma(prob.h, filt.h, C.byref(o), P(x), P(hist), C.byref(st))
+    return x, hist.reshape(n_iter, 4), st
 
 
 def cantilever(nx, ny, nz, edge_load=True):

@@ -1,16 +1,17 @@
 // ref_optim_wrap.cpp -- C entry points around the reference's own std-only
 // optimizers, compiled from the reference sources where they lie
-// (optimization/opt-solver/MMASolver.cpp, optimization/opt-solver/OCSolver.hpp)
+// (optimization/opt-solver/MMASolver.cpp, GCMMASolver.cpp, OCSolver.hpp)
 // into oracle/_ref/libref_optim.so by oracle/Makefile.
 //
 // TEST INFRASTRUCTURE ONLY: used by tests/ to pin the oracle's OC and MMA
-// restatements against the reference implementation on seeded inputs.
+// and GCMMA restatements against the reference implementation on seeded inputs.
 // Nothing in the product links or loads it.
 #include <algorithm>
 #include <cmath>
 #include <iostream>
 #include <vector>
 
+#include "optimization/opt-solver/GCMMASolver.hpp"
 #include "optimization/opt-solver/MMASolver.hpp"
 #include "optimization/opt-solver/OCSolver.hpp"
 
@@ -30,6 +31,32 @@ void ref_mma_set(void *h, double move, double albefa, double asyminit, double as
 void ref_mma_update(void *h, double *x, const double *dfdx, const double *gx,
                     const double *dgdx, const double *xmin, const double *xmax) {
   ((icarat::MMASolver *)h)->Update(x, dfdx, gx, dgdx, xmin, xmax);
+}
+
+void *ref_gcmma_create(int n, int m, double a, double c, double d) {
+  return new icarat::GCMMASolver(n, m, a, c, d);
+}
+void ref_gcmma_destroy(void *h) { delete (icarat::GCMMASolver *)h; }
+void ref_gcmma_set(void *h, double move, double albefa, double asyminit, double asymdec,
+                   double asyminc) {
+  auto *G = (icarat::GCMMASolver *)h;
+  G->SetMoveLimit(move);
+  G->SetOscillation(albefa);
+  G->SetAsymptotes(asyminit, asymdec, asyminc);
+}
+void ref_gcmma_outer(void *h, double *xmma, const double *xval, double f0x, const double *df0dx,
+                     const double *fx, const double *dfdx, const double *xmin,
+                     const double *xmax) {
+  ((icarat::GCMMASolver *)h)->OuterUpdate(xmma, xval, f0x, df0dx, fx, dfdx, xmin, xmax);
+}
+void ref_gcmma_inner(void *h, double *xmma, double f0xnew, const double *fxnew,
+                     const double *xval, double f0x, const double *df0dx, const double *fx,
+                     const double *dfdx, const double *xmin, const double *xmax) {
+  ((icarat::GCMMASolver *)h)->InnerUpdate(xmma, f0xnew, fxnew, xval, f0x, df0dx, fx, dfdx, xmin,
+                                          xmax);
+}
+int ref_gcmma_concheck(void *h, double f0xnew, const double *fxnew) {
+  return ((icarat::GCMMASolver *)h)->ConCheck(f0xnew, fxnew) ? 1 : 0;
 }
 
 typedef double (*ref_volume_fn)(const double *x, int n, void *user);
