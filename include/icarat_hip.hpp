@@ -9,6 +9,8 @@
 //                                            optimization/filter_helm.hpp:21-62,76-181
 //   OC::UpdateVariables                      optimization/opt-solver/OCSolver.hpp:16-79
 //   MMASolver (ctor, Set*, Update, Reset)    optimization/opt-solver/MMASolver.hpp:62-116
+//   GCMMASolver (ctor, Set*, OuterUpdate, InnerUpdate, ConCheck, Reset)
+//                                            optimization/opt-solver/GCMMASolver.hpp:62-85
 // Like the reference, a fatal condition prints to std::cerr and calls
 // exit(1); no exception crosses this header.  Mesh, material, boundary
 // conditions and loads are held by Device (MeshIcarat + Elastic::setParameter +
@@ -279,6 +281,74 @@ class MMASolver {
   void push() {
     check(icarat_hip_mma_set_params(dev_.handle(), move_, albefa_, asyminit_, asymdec_, asyminc_),
           dev_.handle(), "MMASolver parameters");
+  }
+  Device &dev_;
+  long long n_;
+  double move_ = 0.15, albefa_ = 0.2, asyminit_ = 0.2, asymdec_ = 0.7, asyminc_ = 1.2;
+};
+
+// GCMMASolver(n, m = 1) on the device (optimization/opt-solver/GCMMASolver.hpp:62-85)
+class GCMMASolver {
+ public:
+  GCMMASolver(Device &dev, int n, int m, double a = 0.0, double c = 1000.0, double d = 0.0)
+      : dev_(dev), n_(n) {
+    if (m != 1 || a != 0.0 || c != 1000.0 || d != 0.0 || n != dev.nelx()) {
+      std::cerr << "Error : GCMMASolver on the device supports n = nelx, m = 1, a = 0, c = 1000, "
+                   "d = 0"
+                << std::endl;
+      std::exit(1);
+    }
+    check(icarat_hip_gcmma_reset(dev_.handle()), dev_.handle(), "GCMMASolver");
+  }
+  void SetAsymptotes(double init, double decrease, double increase) {
+    asyminit_ = init;
+    asymdec_ = decrease;
+    asyminc_ = increase;
+    push();
+  }
+  void SetMoveLimit(double m) {
+    move_ = m;
+    push();
+  }
+  void SetOscillation(double a) {
+    albefa_ = a;
+    push();
+  }
+  void OuterUpdate(double *xmma, const double *xval, double f0x, const double *df0dx,
+                   const double *fx, const double *dfdx, const double *xmin,
+                   const double *xmax) {
+    bounds(xmin, xmax);
+    check(icarat_hip_gcmma_outer(dev_.handle(), xmma, xval, f0x, df0dx, fx[0], dfdx, xmin[0],
+                                 xmax[0]),
+          dev_.handle(), "GCMMASolver::OuterUpdate");
+  }
+  void InnerUpdate(double *xmma, double f0xnew, const double *fxnew, const double *xval,
+                   double f0x, const double *df0dx, const double *fx, const double *dfdx,
+                   const double *xmin, const double *xmax) {
+    bounds(xmin, xmax);
+    check(icarat_hip_gcmma_inner(dev_.handle(), xmma, f0xnew, fxnew[0], xval, f0x, df0dx, fx[0],
+                                 dfdx, xmin[0], xmax[0]),
+          dev_.handle(), "GCMMASolver::InnerUpdate");
+  }
+  bool ConCheck(double f0xnew, const double *fxnew) {
+    int32_t ok = 0;
+    check(icarat_hip_gcmma_concheck(dev_.handle(), f0xnew, fxnew[0], &ok), dev_.handle(),
+          "GCMMASolver::ConCheck");
+    return ok != 0;
+  }
+  void Reset() { check(icarat_hip_gcmma_reset(dev_.handle()), dev_.handle(), "GCMMASolver::Reset"); }
+
+ private:
+  void bounds(const double *xmin, const double *xmax) {
+    if (!uniform(xmin, n_) || !uniform(xmax, n_)) {
+      std::cerr << "Error : GCMMASolver needs uniform bounds" << std::endl;
+      std::exit(1);
+    }
+  }
+  void push() {
+    check(icarat_hip_gcmma_set_params(dev_.handle(), move_, albefa_, asyminit_, asymdec_,
+                                      asyminc_),
+          dev_.handle(), "GCMMASolver parameters");
   }
   Device &dev_;
   long long n_;
