@@ -37,6 +37,13 @@ class SolveStats(C.Structure):
                 ("status", C.c_int32)]
 
 
+class TomlInfo(C.Structure):
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+                ("lx", C.c_double), ("ly", C.c_double), ("lz", C.c_double),
+                ("designs_0", C.c_double), ("filter_radius", C.c_double),
+                ("n_dirichlet", C.c_int32), ("n_neumann", C.c_int32)]
+
+
 class SimpOpts(C.Structure):
     _fields_ = [("n_iter", C.c_int32), ("optimizer", C.c_int32), ("use_projection", C.c_int32),
                 ("filter", C.c_int32), ("xmin", C.c_double), ("xmax", C.c_double),
@@ -100,6 +107,9 @@ _SIGS = {
     "icarat_hip_filter": (C.c_int, [C.c_void_p, dp, dp, C.c_int32]),
     "icarat_hip_helm_setup": (C.c_int, [C.c_void_p, C.c_double]),
     "icarat_hip_mises": (C.c_int, [C.c_void_p, dp, dp]),
+    "icarat_hip_create_from_toml": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p),
+                                              C.c_void_p, C.c_char_p, C.c_int32]),
+    "icarat_hip_write_vtk": (C.c_int, [C.c_void_p, C.c_char_p, dp]),
     "icarat_hip_gcmma_set_params": (C.c_int, [C.c_void_p] + [C.c_double] * 5),
     "icarat_hip_gcmma_reset": (C.c_int, [C.c_void_p]),
     "icarat_hip_gcmma_outer": (C.c_int, [C.c_void_p, dp, dp, C.c_double, dp, C.c_double, dp,
@@ -310,6 +320,30 @@ class Context:
         a, b, c, e = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int32()
         self.L.icarat_hip_counts(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(e))
         self.numnp, self.nelx, self.neq, self.numeq = a.value, b.value, c.value, e.value
+
+    @classmethod
+    def from_toml(cls, path):
+        """A context from one of ICARAT's TOML configuration files; returns
+        (context, TomlInfo)."""
+        L = lib()
+        h = C.c_void_p()
+        info = TomlInfo()
+        err = C.create_string_buffer(512)
+        rc = L.icarat_hip_create_from_toml(os.fsencode(path), C.byref(h), C.byref(info), err, 512)
+        if rc:
+            raise IcaratError(rc, "create_from_toml: " + err.value.decode())
+        self = cls.__new__(cls)
+        self.L, self.h = L, h
+        self.nx, self.ny, self.nz = info.nx, info.ny, info.nz
+        a, b, c, e = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int32()
+        L.icarat_hip_counts(h, C.byref(a), C.byref(b), C.byref(c), C.byref(e))
+        self.numnp, self.nelx, self.neq, self.numeq = a.value, b.value, c.value, e.value
+        return self, info
+
+    def write_vtk(self, path, density=None):
+        """The drivers' result file (legacy VTK) of the current state."""
+        d = None if density is None else _p(_f64(density))
+        self._chk(self.L.icarat_hip_write_vtk(self.h, os.fsencode(path), d), "write_vtk")
 
     def close(self):
         if getattr(self, "h", None):

@@ -34,6 +34,18 @@ struct icarat_hip_unet {
   std::string err;
 };
 
+namespace icarat_hip {
+icarat_hip_ctx *wrap_context(Context *c) {
+  icarat_hip_ctx *h = new icarat_hip_ctx();
+  h->c = c;
+  return h;
+}
+Context *unwrap_context(icarat_hip_ctx *h) { return h ? h->c : nullptr; }
+void set_context_error(icarat_hip_ctx *h, const std::string &m) {
+  if (h) h->err = m;
+}
+}  // namespace icarat_hip
+
 namespace {
 
 template <class F>

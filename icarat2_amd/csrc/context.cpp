@@ -598,6 +598,16 @@ void Context::mises_host(double *mises, double *stress) {
   sync();
 }
 
+void Context::nodal_values_host(double *u) {
+  if (!solved_) throw Error(kNotReady, "no solution for the current density");
+  launch_flush_small(n3_, d_u, d_nv, stream_);
+  std::vector<double> lat(n3_);
+  d2h(lat.data(), d_nv.p, n3_, stream_);
+  sync();
+  for (long long n = 0; n < L_.numnp; ++n)
+    for (int c = 0; c < 3; ++c) u[3 * n + c] = lat[c * L_.Np + n];
+}
+
 // ---- filter / OC ----------------------------------------------------------
 // element work vectors
 enum { E_DFDS = 0, E_DGDS = 1, E_AA = 2, E_XF = 3, E_XP = 4, E_THD = 5, E_TMP = 6, E_DES = 7 };

@@ -53,6 +53,9 @@ class Context {
   void compliance_sensitivity(double vnorm, double *c, double *dfds, double *dgds);
   // element-mean stress (nelx x 6, may be null) and von Mises stress of the last solve
   void mises_host(double *mises, double *stress);
+  // nodal values of the last solve (numnp x 3, node order; Dirichlet values
+  // included, |v| < 1e-15 flushed as analysis/analysis_utility.hpp:26-28)
+  void nodal_values_host(double *u);
 
   void filter_setup(double radius);
   void filter_device(const double *in, double *out, int is_sens);  // device pointers

@@ -96,6 +96,31 @@ int icarat_hip_counts(const icarat_hip_ctx *ctx, int64_t *numnp, int64_t *nelx, 
 /* 1 if a HIP device is usable from this process, else 0 */
 int icarat_hip_device_available(void);
 
+/* ---- configuration and result files ------------------------------------- */
+typedef struct icarat_toml_info {
+  int32_t nx, ny, nz;      /* [mesh] division */
+  double lx, ly, lz;       /* [mesh] length */
+  double designs_0;        /* [material] designs_0 (1 if absent): the initial design */
+  double filter_radius;    /* [this_problem] filterR (0 if absent) */
+  int32_t n_dirichlet;     /* entries of [boundary_condition] dirichlet applied */
+  int32_t n_neumann;       /* entries of [boundary_condition] neumann applied */
+} icarat_toml_info;
+/* A context from one of ICARAT's configuration files (toml::parse + MeshIcarat +
+ * Elastic::setParameter + BCIcarat, io/mesh_icarat.hpp:72-139,
+ * problem/elasticity.cpp:19-33, io/boundary_condition_icarat.hpp:72-264): [mesh]
+ * dimension 3 / type "hexa8" / division / length, [material] young / poisson /
+ * penalty / designs_0, [boundary_condition] dirichlet and neumann boxes (applied in
+ * that order, with make_dof between them), [this_problem] filterR.  Other keys are
+ * ignored; other element types are rejected.  info and err (err_len bytes, the
+ * message of a failure) may be NULL. */
+int icarat_hip_create_from_toml(const char *path, icarat_hip_ctx **out, icarat_toml_info *info,
+                                char *err, int32_t err_len);
+/* The drivers' result file (output(), opt_macro_oc.cpp:214-236, with the writer of
+ * io/export_vtk.hpp:47-190): legacy ASCII VTK, nodes, hexa8 cells, and, when a
+ * solution exists, POINT_DATA "displacement" and CELL_DATA "mises"; CELL_DATA
+ * "density" when density (nelx) is not NULL. */
+int icarat_hip_write_vtk(icarat_hip_ctx *ctx, const char *path, const double *density);
+
 /* ---- boundary conditions (BCIcarat) -------------------------------------- */
 /* Dirichlet box: flag[6]/val[6]/range {x0,x1,y0,y1,z0,z1}, tolerance 1e-7
  * (io/boundary_condition_icarat.hpp:72-138, io/boundary_condition_base.hpp:19-33). */
