@@ -326,13 +326,17 @@ __device__ __forceinline__ void layer_contract(const double *__restrict__ c_KE0,
   }
 }
 
-template <bool PCG, int SYM, bool FUSE>
+// JAC (with PCG = false): the multigrid smoother's last fine sweep fused into
+// the epilogue.  The node's own value x is the centre of the bottom plane, so
+// y receives x' = x + omega dinv (b - K x) (out of place: neighbouring columns
+// and the next z-chunk still read x) and the block partials of x'.b and x'.q.
+template <bool PCG, int SYM, bool FUSE, bool JAC>
 __global__ __launch_bounds__(NT) void k_stencil64z(
     Lattice L, int nchunks, const double *__restrict__ c_KE0, KeVals kv, const double *__restrict__ young,
     const uint8_t *__restrict__ fmask, const double *__restrict__ src,
     const double *__restrict__ dinv, const double *__restrict__ p_old, double *__restrict__ p_new,
     double *__restrict__ y, PcgScalars *S, double *partials, const double *__restrict__ pu,
-    unsigned Gu, int j) {
+    unsigned Gu, int j, JacEpilogue je) {
   __shared__ double sp[2][3][ZPN];
   __shared__ double se[2][ZLN];
   __shared__ double sh[8];
@@ -367,7 +371,7 @@ __global__ __launch_bounds__(NT) void k_stencil64z(
   commit_plane<PCG>(L, x0, y0, z0, z0 < z1, beta, F, p_new, sp[0], se[0]);
   __syncthreads();
   double acc[3] = {0.0, 0.0, 0.0};
-  double pq = 0.0;
+  double pq = 0.0, zr = 0.0, zq = 0.0;
   const int nsteps = z1 - z0 + 1;  // element layers z0-1 .. z1-1
   for (int k = 0; k < nsteps; ++k) {
     const int zb = z0 - 1 + k;
@@ -390,8 +394,20 @@ __global__ __launch_bounds__(NT) void k_stencil64z(
       double out[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) out[c] = ((m >> c) & 1u) ? acc[c] + bot[c] : 0.0;
+      if constexpr (JAC) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) y[c * L.Np + n] = out[c];
+        for (int c = 0; c < 3; ++c) {
+          const long long ic = c * L.Np + n;
+          const double bc = je.b[ic];
+          const double xn = Ulo[4][c] + je.omega * dinv[ic] * (bc - out[c]);
+          y[ic] = xn;
+          zr += xn * bc;
+          zq += xn * je.q[ic];
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) y[c * L.Np + n] = out[c];
+      }
       if constexpr (PCG) pq += Ulo[4][0] * out[0] + Ulo[4][1] * out[1] + Ulo[4][2] * out[2];
     }
 #pragma unroll
@@ -406,6 +422,10 @@ __global__ __launch_bounds__(NT) void k_stencil64z(
   if constexpr (PCG) {
     double v[1] = {pq};
     store_partials<NT, 1>(v, partials, sh);
+  }
+  if constexpr (JAC) {
+    double v[2] = {zr, zq};
+    store_partials<NT, 2>(v, partials, sh);
   }
 }
 
@@ -770,20 +790,22 @@ KeCoef make_ke_coef(const double *ke, const double *ke_dev) {
   return k;
 }
 
-template <bool PCG, int SYM, bool FUSE>
+template <bool PCG, int SYM, bool FUSE, bool JAC = false>
 static void launch_stencil64_t(const Lattice &L, const KeCoef &ke, const double *young,
                                const uint8_t *freemask, const double *src, const double *dinv,
                                const double *p_old, double *p_new, double *y, PcgScalars *S,
                                double *partials, const double *pu, unsigned Gu, int j,
-                               hipStream_t s, unsigned *nblocks) {
+                               hipStream_t s, unsigned *nblocks, JacEpilogue je = JacEpilogue{}) {
   if (ke.kernel == 0) {
     const int def = stencil_nchunks(L);
     const int nch = ke.nchunks > 0 ? std::min(ke.nchunks, def) : def;
     const dim3 g = zgrid(L, nch);
-    hipLaunchKernelGGL((k_stencil64z<PCG, SYM, FUSE>), g, dim3(NT), 0, s, L, nch, ke.ke, ke.vals,
-                       young, freemask, src, dinv, p_old, p_new, y, S, partials, pu, Gu, j);
+    hipLaunchKernelGGL((k_stencil64z<PCG, SYM, FUSE, JAC>), g, dim3(NT), 0, s, L, nch, ke.ke,
+                       ke.vals, young, freemask, src, dinv, p_old, p_new, y, S, partials, pu, Gu,
+                       j, je);
     *nblocks = g.x * g.y * g.z;
   } else {
+    if (JAC) throw Error(kInternal, "the fused smoother needs the z-marching kernel");
     const dim3 g = stencil_grid(L);
     hipLaunchKernelGGL((k_stencil64<PCG, SYM, FUSE>), g, dim3(NT), 0, s, L, ke.ke, ke.vals, young,
                        freemask, src, dinv, p_old, p_new, y, S, partials, pu, Gu, j);
@@ -811,6 +833,32 @@ static unsigned launch_stencil64(const Lattice &L, const KeCoef &ke, const doubl
     case 2:
       launch_stencil64_t<PCG, 2, FUSE>(L, ke, young, freemask, src, dinv, p_old, p_new, y, S,
                                        partials, pu, Gu, j, s, &nb);
+      break;
+    default:
+      throw Error(kInvalidArgument, "unknown KE0 symmetry class");
+  }
+  return nb;
+}
+
+unsigned launch_apply_k64_jacobi(const Lattice &L, const KeCoef &ke, const double *young,
+                                 const uint8_t *freemask, const double *x, double *xout,
+                                 const double *b, const double *dinv, double omega,
+                                 const double *q, double *partials, hipStream_t s) {
+  if (ke.kernel != 0) throw Error(kInternal, "the fused smoother needs the z-marching kernel");
+  const JacEpilogue je{b, q, omega};
+  unsigned nb = 0;
+  switch (ke.sym) {
+    case 0:
+      launch_stencil64_t<false, 0, false, true>(L, ke, young, freemask, x, dinv, nullptr, nullptr,
+                                                xout, nullptr, partials, nullptr, 0, 0, s, &nb, je);
+      break;
+    case 1:
+      launch_stencil64_t<false, 1, false, true>(L, ke, young, freemask, x, dinv, nullptr, nullptr,
+                                                xout, nullptr, partials, nullptr, 0, 0, s, &nb, je);
+      break;
+    case 2:
+      launch_stencil64_t<false, 2, false, true>(L, ke, young, freemask, x, dinv, nullptr, nullptr,
+                                                xout, nullptr, partials, nullptr, 0, 0, s, &nb, je);
       break;
     default:
       throw Error(kInvalidArgument, "unknown KE0 symmetry class");

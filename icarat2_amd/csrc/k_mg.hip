@@ -245,7 +245,7 @@ __global__ __launch_bounds__(256) void k_mg_jacobi_dot(long long n, double omega
                                                        const double *__restrict__ dinv,
                                                        const double *__restrict__ b,
                                                        const double *__restrict__ y,
-                                                       double *__restrict__ x,
+                                                       const double *xin, double *xout,
                                                        const double *__restrict__ q,
                                                        double *part) {
   __shared__ double sh[8];
@@ -253,8 +253,8 @@ __global__ __launch_bounds__(256) void k_mg_jacobi_dot(long long n, double omega
   const long long stride = (long long)gridDim.x * 256;
   for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n; i += stride) {
     const double bi = b[i];
-    const double xi = x[i] + omega * dinv[i] * (bi - y[i]);
-    x[i] = xi;
+    const double xi = xin[i] + omega * dinv[i] * (bi - y[i]);
+    xout[i] = xi;
     zr += xi * bi;
     zq += xi * q[i];
   }
@@ -431,10 +431,10 @@ void launch_mg_jacobi(long long n, double omega, const double *dinv, const doubl
 }
 
 unsigned launch_mg_jacobi_dot(long long n, double omega, const double *dinv, const double *b,
-                              const double *y, double *x, const double *q, double *part,
-                              hipStream_t s) {
-  hipLaunchKernelGGL(k_mg_jacobi_dot, dim3(DOT_BLOCKS), dim3(256), 0, s, n, omega, dinv, b, y, x, q,
-                     part);
+                              const double *y, const double *xin, double *xout, const double *q,
+                              double *part, hipStream_t s) {
+  hipLaunchKernelGGL(k_mg_jacobi_dot, dim3(DOT_BLOCKS), dim3(256), 0, s, n, omega, dinv, b, y, xin,
+                     xout, q, part);
   HIPCHK(hipGetLastError());
   return DOT_BLOCKS;
 }

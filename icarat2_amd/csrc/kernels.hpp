@@ -45,6 +45,20 @@ int stencil_nchunks(const Lattice &L);
 // halo-tile kernel.
 void launch_apply_k64(const Lattice &L, const KeCoef &ke, const double *young,
                       const uint8_t *freemask, const double *u, double *y, hipStream_t s);
+// Arguments of the fused smoother epilogue of the z-marching K.u.
+struct JacEpilogue {
+  const double *b = nullptr;
+  const double *q = nullptr;
+  double omega = 0.0;
+};
+// The multigrid smoother's sweep with K.x in one z-marching pass:
+// xout = x + omega dinv (b - K x) (x unchanged) and the block partials of
+// xout.b and xout.q (2 per block, the layout of store_partials); returns the
+// block count.  Needs the z-marching kernel shape.
+unsigned launch_apply_k64_jacobi(const Lattice &L, const KeCoef &ke, const double *young,
+                                 const uint8_t *freemask, const double *x, double *xout,
+                                 const double *b, const double *dinv, double omega,
+                                 const double *q, double *partials, hipStream_t s);
 // One fused Jacobi-PCG iteration, stencil half: p_new = dinv*r + beta*p_old;
 // q = K p_new; p.q partials (returns their count); with finalize, alpha = rho /
 // p.q by a finalize kernel.

@@ -25,11 +25,12 @@ void launch_mg_young(const Lattice &Lf, const Lattice &Lc, const double *yf, dou
 // damped Jacobi: y == nullptr: x = omega dinv b; else x += omega dinv (b - y)
 void launch_mg_jacobi(long long n, double omega, const double *dinv, const double *b,
                       const double *y, double *x, hipStream_t s);
-// x += omega dinv (b - y) and the partials of x.b and x.q (the flexible PCG's
-// z.r and z.q when this is a V-cycle's last sweep on r); returns their count
+// xout = xin + omega dinv (b - y) (xout may be xin) and the partials of
+// xout.b and xout.q (the flexible PCG's z.r and z.q when this is a V-cycle's
+// last sweep on r); returns their count
 unsigned launch_mg_jacobi_dot(long long n, double omega, const double *dinv, const double *b,
-                              const double *y, double *x, const double *q, double *part,
-                              hipStream_t s);
+                              const double *y, const double *xin, double *xout, const double *q,
+                              double *part, hipStream_t s);
 
 // One thread per node, K.x gathered from global memory (for coarse lattices,
 // where a z-marching launch is latency-bound), with the consumer fused in:

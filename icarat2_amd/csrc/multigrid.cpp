@@ -272,12 +272,25 @@ double *Multigrid::cycle(int l, const double *b, bool emit, bool x0_ready, ZDot 
   if (emit) launch_mg_prolong_add(L, cc.lattice(), xc, c.d_free, cur, s, pre > 0);
   const int post = std::max(0, o_.nu_post);
   for (int i = 0; i < post; ++i) {
-    if (i == post - 1 && l == 0 && zd && !direct_[l]) {
-      // the last fine sweep also yields the partials of z.r and z.q
+    if (i == post - 1 && l == 0 && !direct_[l]) {
+      // The last fine sweep runs out of place (the V-cycle's result is always
+      // the other buffer, whatever the kernel shape) and also yields the
+      // partials of z.r and z.q.  With the z-marching K.u the sweep is fused
+      // into the K.u's epilogue; else K.x goes through r.
       if (emit) {
-        launch_apply_k64(L, c.kc_, c.d_young, c.d_free, cur, oth, s);
-        zd->count = launch_mg_jacobi_dot(n, w, c.d_dinv, b, oth, cur, zd->q, zd->part, s);
+        double *part = zd ? zd->part : c.d_partials.p;
+        const double *q = zd ? zd->q : c.d_q.p;
+        unsigned cnt;
+        if (c.kc_.kernel == 0) {
+          cnt = launch_apply_k64_jacobi(L, c.kc_, c.d_young, c.d_free, cur, oth, b, c.d_dinv, w, q,
+                                        part, s);
+        } else {
+          launch_apply_k64(L, c.kc_, c.d_young, c.d_free, cur, r, s);
+          cnt = launch_mg_jacobi_dot(n, w, c.d_dinv, b, r, cur, oth, q, part, s);
+        }
+        if (zd) zd->count = cnt;
       }
+      std::swap(cur, oth);
     } else {
       sweep();
     }
