@@ -1104,7 +1104,7 @@ void Context::mg_setup(const icarat_mg_opts &o) {
   require_dof();
   if (o.max_levels < 1 || o.nu_pre < 0 || o.nu_post < 1 || o.nu_coarse < 1 || !(o.omega > 0.0) ||
       !(o.omega < 2.0) || o.batch_iters < 1 || o.batch_iters > 64 || o.dense_dofs < 0 ||
-      o.direct_nodes < 0 || o.coarse_avg < 0 || o.coarse_avg > 4)
+      o.direct_nodes < 0 || o.coarse_avg < 0 || o.coarse_avg > 5)
     throw Error(kInvalidArgument, "invalid multigrid options");
   sync();
   mg_.reset();

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include "contract.hpp"
+#include "galerkin_kernels.hpp"
 #include "mg_kernels.hpp"
 
 namespace icarat_hip {
@@ -384,11 +385,12 @@ void launch_mg_prolong_add(const Lattice &Lf, const Lattice &Lc, const double *x
 }
 
 void launch_mg_dense_assemble(const Lattice &L, const double *ke, const double *young,
-                              const uint8_t *fmask, double *A, hipStream_t s) {
+                              const uint8_t *fmask, double *A, hipStream_t s, const double *kmat) {
   const long long n = 3 * L.numnp;
   if (n > MG_DENSE_MAX) throw Error(kInvalidArgument, "lattice too large for the dense solve");
   hipLaunchKernelGGL(k_mg_dense_zero, dim3(blocks_for(n * n, 256)), dim3(256), 0, s, n * n, A);
-  for (int cls = 0; cls < 8; ++cls) {
+  if (kmat) launch_gal_dense_scatter(L, kmat, A, s);
+  for (int cls = 0; cls < 8 && !kmat; ++cls) {
     const int px = cls & 1, py = (cls >> 1) & 1, pz = cls >> 2;
     const long long cnt = (long long)((L.nx - px + 1) / 2) * ((L.ny - py + 1) / 2) * ((L.nz - pz + 1) / 2) * 576;
     if (cnt == 0) continue;

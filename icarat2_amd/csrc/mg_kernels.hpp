@@ -48,7 +48,8 @@ void launch_mg_direct(int mode, const Lattice &L, const KeCoef &kc, const double
 constexpr int MG_DENSE_MAX = 4096;  // largest 3 numnp solved densely
 // A = K of the lattice (n = 3 numnp, n x n, symmetric)
 void launch_mg_dense_assemble(const Lattice &L, const double *ke, const double *young,
-                              const uint8_t *fmask, double *A, hipStream_t s);
+                              const uint8_t *fmask, double *A, hipStream_t s,
+                              const double *kmat = nullptr);
 // copies the lower triangle (column-major) over the upper one
 void launch_mg_dense_symmetrize(int n, double *A, hipStream_t s);
 // x = A b on the lattice (A symmetric, full)

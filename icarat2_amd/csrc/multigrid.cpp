@@ -9,6 +9,7 @@
 #include <string>
 
 #include "context.hpp"
+#include "galerkin_kernels.hpp"
 #include "learned_kernels.hpp"
 #include "mg_kernels.hpp"
 
@@ -65,12 +66,28 @@ Multigrid::Multigrid(Context &fine, const icarat_mg_opts &o) : f_(fine), o_(o) {
     c->ensure_device();
     lv_.push_back(std::move(c));
   }
+  galerkin_ = o_.coarse_avg == 5 && levels() > 1;
   for (int l = 0; l < levels(); ++l)
-    omega_.push_back(o_.omega / jacobi_bound(level(l).problem().ke0()));
+    omega_.push_back(galerkin_ && l > 0 ? 1.0  // folded into the level's dinv
+                                        : o_.omega / jacobi_bound(level(l).problem().ke0()));
   // smoothing kernels per level: the fused one-thread-per-node sweep where the
-  // z-marching launch is latency-bound; the smallest coarsest grids exactly
+  // z-marching launch is latency-bound (every Galerkin level: their operators
+  // are explicit element matrices); the smallest coarsest grids exactly
   for (int l = 0; l < levels(); ++l)
-    direct_.push_back(level(l).lattice().numnp <= (long long)std::max(0, o_.direct_nodes) ? 1 : 0);
+    direct_.push_back((galerkin_ && l > 0) ||
+                              level(l).lattice().numnp <= (long long)std::max(0, o_.direct_nodes)
+                          ? 1
+                          : 0);
+  if (galerkin_) {
+    kmat_.resize(levels());
+    long long maxe = 0;
+    for (int l = 1; l < levels(); ++l) {
+      kmat_[l].alloc(576 * (size_t)level(l).lattice().nelx);
+      maxe = std::max(maxe, level(l).lattice().nelx);
+    }
+    mu_.alloc(maxe);
+    mumax_.alloc(levels());
+  }
   {
     const long long nd = 3 * level(levels() - 1).lattice().numnp;
     dense_ = nd <= dense_max;
@@ -141,6 +158,14 @@ void Multigrid::refresh() {
   const GridSpec &g = f_.problem().spec();
   for (int l = 1; l < levels(); ++l) {
     Context &fc = level(l - 1), &cc = level(l);
+    if (galerkin_) {
+      // R A P element by element from the level above, then the damped
+      // Jacobi factor theta / (mu_max diag)
+      launch_gal_build(fc.lattice(), cc.lattice(), l == 1 ? fc.d_ke.p : kmat_[l - 1].p,
+                       l == 1 ? fc.d_young.p : nullptr, fc.d_free, cc.d_free, kmat_[l], mu_, s);
+      launch_gal_dinv(cc.lattice(), kmat_[l], cc.d_free, mu_, mumax_.p + l, o_.omega, cc.d_dinv, s);
+      continue;
+    }
     if (o_.coarse_avg == 2) {
       // the SIMP law of the mean density
       launch_mg_young(fc.lattice(), cc.lattice(), fc.d_dens, cc.d_dens, s, 0);
@@ -174,7 +199,8 @@ void Multigrid::refresh() {
 void Multigrid::factor_into(double *dst, hipStream_t s) {
   Context &cc = level(levels() - 1);
   const int nd = (int)(3 * cc.lattice().numnp);
-  launch_mg_dense_assemble(cc.lattice(), cc.d_ke, cc.d_young, cc.d_free, dst, s);
+  launch_mg_dense_assemble(cc.lattice(), cc.d_ke, cc.d_young, cc.d_free, dst, s,
+                           galerkin_ ? kmat_[levels() - 1].p : nullptr);
   auto chk = [](rocblas_status st, const char *what) {
     if (st != rocblas_status_success)
       throw Error(kInternal, std::string(what) + " failed: " + rocblas_status_to_string(st));
@@ -218,6 +244,18 @@ bool Multigrid::starts_swept(int l) const {
   return o_.nu_pre > 0;
 }
 
+void Multigrid::coarse_op(int l, int mode, const double *x, const double *b, double w,
+                          double *out, hipStream_t s) {
+  Context &c = level(l);
+  if (galerkin_ && l > 0) {
+    launch_gal_apply(mode == MG_JACOBI ? GAL_JACOBI : GAL_RESIDUAL, c.lattice(), kmat_[l], c.d_free,
+                     x, b, c.d_dinv, out, s);
+    return;
+  }
+  launch_mg_direct(mode, c.lattice(), c.kc_, c.d_young, c.d_free, x, b,
+                   mode == MG_JACOBI ? c.d_dinv.p : nullptr, w, out, s);
+}
+
 double *Multigrid::cycle(int l, const double *b, bool emit, bool x0_ready, ZDot *zd) {
   hipStream_t s = f_.stream();
   Context &c = level(l);
@@ -234,7 +272,7 @@ double *Multigrid::cycle(int l, const double *b, bool emit, bool x0_ready, ZDot 
   double *cur = x, *oth = y;
   auto sweep = [&]() {
     if (direct_[l]) {
-      if (emit) launch_mg_direct(MG_JACOBI, L, c.kc_, c.d_young, c.d_free, cur, b, c.d_dinv, w, oth, s);
+      if (emit) coarse_op(l, MG_JACOBI, cur, b, w, oth, s);
       std::swap(cur, oth);
     } else if (emit) {
       launch_apply_k64(L, c.kc_, c.d_young, c.d_free, cur, oth, s);
@@ -258,7 +296,7 @@ double *Multigrid::cycle(int l, const double *b, bool emit, bool x0_ready, ZDot 
       launch_mg_restrict(L, cc.lattice(), b, nullptr, cc.d_free, cc.d_b, dinv_c, omega_[l + 1],
                          cc.d_x, s);
     } else if (direct_[l]) {
-      launch_mg_direct(MG_RESIDUAL, L, c.kc_, c.d_young, c.d_free, cur, b, nullptr, 0.0, r, s);
+      coarse_op(l, MG_RESIDUAL, cur, b, 0.0, r, s);
       launch_mg_restrict(L, cc.lattice(), r, nullptr, cc.d_free, cc.d_b, dinv_c, omega_[l + 1],
                          cc.d_x, s);
     } else {

@@ -80,6 +80,15 @@ class Multigrid {
   std::vector<double> omega_;                  // Jacobi damping per level
   DevBuf<double> z_, y0_, r0_, ones_;
   std::vector<char> direct_;
+  // Galerkin coarse operators (coarse_avg 5): explicit element matrices per
+  // coarse level (kmat_[l], 576 per element), the per-element smoothing bounds
+  // and their maximum per level; the damping is folded into the levels' dinv
+  bool galerkin_ = false;
+  std::vector<DevBuf<double>> kmat_;
+  DevBuf<double> mu_, mumax_;
+  // level l >= 1 operator with the fused consumer (MG_JACOBI / MG_RESIDUAL)
+  void coarse_op(int l, int mode, const double *x, const double *b, double w, double *out,
+                 hipStream_t s);
   bool dense_ = false;
   DevBuf<double> ainv_;  // dense inverse of the coarsest K
   DevBuf<int> info_;

@@ -408,9 +408,11 @@ typedef struct icarat_mg_opts {
                               its dense inverse (0: never; the coarsest grid is smoothed) */
   int32_t direct_nodes;    /* levels of at most this many nodes smooth with the fused
                               one-thread-per-node sweep; larger ones with the z-marching K.u */
-  int32_t coarse_avg;      /* a coarse element's Young's modulus from its 8 children:
-                              0 arithmetic mean, 1 geometric mean, 2 the SIMP law of the
-                              mean density, 3 quadratic mean, 4 maximum */
+  int32_t coarse_avg;      /* coarse operators: a coarse element's Young's modulus from its 8
+                              children (0 arithmetic mean, 1 geometric mean, 2 the SIMP law of
+                              the mean density, 3 quadratic mean, 4 maximum) times the coarse
+                              KE0; or 5: Galerkin, R A P element by element (explicit 24x24
+                              matrices per coarse element, 4.6 KB each) */
 } icarat_mg_opts;
 /* Fills *o with the defaults (10, 1, 1, 20, 1.8, 4, 1024, 262144). */
 int icarat_hip_mg_default_opts(icarat_mg_opts *o);

@@ -99,12 +99,13 @@ def test_mg_nonsymmetric_cycles(gpu, shape):
     assert np.max(np.abs(um - uj)) < 1e-6 * np.max(np.abs(uj))
 
 
-@pytest.mark.parametrize("avg", [1, 2, 3, 4])
+@pytest.mark.parametrize("avg", [1, 2, 3, 4, 5])
 def test_mg_coarse_averaging_rules(gpu, avg):
     """Coarse Young's moduli by the geometric mean, the SIMP law of the mean
     density, the quadratic mean or the maximum of the children (the default is
-    the arithmetic mean): other coarse operators, the same verified solve (1e-6
-    of max |u| of Jacobi-PCG, both at 1e-9)."""
+    the arithmetic mean), or Galerkin operators R A P (5): other coarse
+    operators, the same verified solve (1e-6 of max |u| of Jacobi-PCG, both at
+    1e-9)."""
     dims = (48, 16, 16)
     c = gpu.cantilever(*dims)
     x, _ = gpu.gaussian_field(*dims, seed=13, smin=2.0, smax=6.0)
@@ -131,3 +132,53 @@ def test_mg_simp_matches_jacobi(gpu):
     assert np.max(np.abs(xm - xj)) < 1e-6
     assert np.max(np.abs(hm[:, 0] - hj[:, 0]) / hj[:, 0]) < 1e-8
     assert np.all(hm[:, 3] <= 1e-10)
+
+
+def _prolong(v, dims):
+    """Trilinear interpolation of a coarse nodal field (node-major, 3 comps) to
+    the 2x finer lattice."""
+    nx, ny, nz = dims
+    vc = v.reshape(nz // 2 + 1, ny // 2 + 1, nx // 2 + 1, 3)
+    out = np.zeros((nz + 1, ny + 1, nx + 1, 3))
+    for k in range(nz + 1):
+        for j in range(ny + 1):
+            for i in range(nx + 1):
+                acc = 0.0
+                for (K, wk) in ([(k // 2, 1.0)] if k % 2 == 0 else [(k // 2, .5), (k // 2 + 1, .5)]):
+                    for (J, wj) in ([(j // 2, 1.0)] if j % 2 == 0 else [(j // 2, .5), (j // 2 + 1, .5)]):
+                        for (I, wi) in ([(i // 2, 1.0)] if i % 2 == 0 else [(i // 2, .5), (i // 2 + 1, .5)]):
+                            acc = acc + wk * wj * wi * vc[K, J, I]
+                out[k, j, i] = acc
+    return out.reshape(-1, 3)
+
+
+@pytest.mark.parametrize("avg,exact", [(5, True), (0, False)])
+def test_galerkin_two_grid_is_exact_on_coarse_solutions(gpu, avg, exact):
+    """Two levels, no pre-smoothing, the coarse level solved exactly: with the
+    Galerkin operator R A P the coarse correction reproduces any solution in
+    the range of P, so PCG stops after one iteration; with a rediscretised
+    coarse operator (arithmetic mean of E) on a high-contrast field it does not.
+    This pins the element-by-element Galerkin matrices (weights, masks,
+    assembly) against the matrix-free fine operator and the transfers."""
+    dims = (8, 4, 4)
+    c = gpu.cantilever(*dims)
+    rng = np.random.default_rng(31)
+    c.set_density(rng.uniform(1e-3, 1.0, c.nelx) ** 0.5)
+    c.mg_setup(max_levels=2, nu_pre=0, nu_post=1, dense_dofs=4096, coarse_avg=avg)
+    assert c.mg_info() == [(8, 4, 4), (4, 2, 2)]
+    cn = (dims[0] // 2 + 1) * (dims[1] // 2 + 1) * (dims[2] // 2 + 1)
+    vc = rng.standard_normal((cn, 3))
+    vc[::dims[0] // 2 + 1] = 0.0            # coarse nodes on the clamped face x = 0
+    uf = _prolong(vc.reshape(-1), dims)
+    dof = c.dof().reshape(-1, 3)
+    ustar = np.zeros(c.numeq)
+    free = dof < c.numeq
+    ustar[dof[free]] = uf[free]
+    c.set_fext(c.apply_K(ustar))
+    u, st = c.solve(tol=1e-10, precond=gpu.PRECOND_MG)
+    assert st.status == 0 and st.true_relres <= 1e-10
+    assert np.max(np.abs(u - ustar)) < 1e-8 * np.max(np.abs(ustar))
+    if exact:
+        assert st.iters <= 1, st.iters
+    else:
+        assert st.iters > 1
