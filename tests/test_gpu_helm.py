@@ -12,6 +12,7 @@ pytestmark = pytest.mark.gpu
     ((9, 7, 4), (4.5, 14.0, 2.0), 1.2),   # box voxels 0.5 x 2 x 0.5
     ((20, 10, 6), None, 4.0),
     ((2, 2, 2), None, 2.0),
+    ((4, 90, 90), None, 1.5),             # long y and z axes, short x
 ])
 def test_helm_filter_matches_oracle(gpu, oracle, dims, L, r):
     """Both modes against the oracle's assembled system solved by Jacobi-PCG to
