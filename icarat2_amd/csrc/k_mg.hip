@@ -317,6 +317,15 @@ __global__ void k_mg_dense_constrain(Lattice L, const uint8_t *__restrict__ fmas
   if (!fr || !fc) A[q] = r == c ? 1.0 : 0.0;
 }
 
+// diagonal of the free DOFs times (1 + shift)
+__global__ void k_mg_dense_shift(Lattice L, const uint8_t *__restrict__ fmask, double shift,
+                                 double *__restrict__ A) {
+  const long long n = 3 * L.numnp;
+  const long long r = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  if ((fmask[r % L.numnp] >> (r / L.numnp)) & 1u) A[r * n + r] *= 1.0 + shift;
+}
+
 // column-major: A(i, j) = A[i + j n]; the lower triangle (i > j) is valid
 __global__ void k_mg_dense_symmetrize(int n, double *__restrict__ A) {
   const long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
@@ -398,6 +407,13 @@ void launch_mg_dense_assemble(const Lattice &L, const double *ke, const double *
                        px, py, pz, A);
   }
   hipLaunchKernelGGL(k_mg_dense_constrain, dim3(blocks_for(n * n, 256)), dim3(256), 0, s, L, fmask, A);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_mg_dense_shift(int n, const Lattice &L, const uint8_t *fmask, double shift, double *A,
+                           hipStream_t s) {
+  if (n != 3 * L.numnp) throw Error(kInternal, "dense shift: size mismatch");
+  hipLaunchKernelGGL(k_mg_dense_shift, dim3(blocks_for(n, 256)), dim3(256), 0, s, L, fmask, shift, A);
   HIPCHK(hipGetLastError());
 }
 

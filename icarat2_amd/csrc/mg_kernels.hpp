@@ -50,6 +50,9 @@ constexpr int MG_DENSE_MAX = 4096;  // largest 3 numnp solved densely
 void launch_mg_dense_assemble(const Lattice &L, const double *ke, const double *young,
                               const uint8_t *fmask, double *A, hipStream_t s,
                               const double *kmat = nullptr);
+// the diagonal of the free DOFs of the assembled dense A (n = 3 numnp) times (1 + shift)
+void launch_mg_dense_shift(int n, const Lattice &L, const uint8_t *fmask, double shift, double *A,
+                           hipStream_t s);
 // copies the lower triangle (column-major) over the upper one
 void launch_mg_dense_symmetrize(int n, double *A, hipStream_t s);
 // x = A b on the lattice (A symmetric, full)

@@ -177,7 +177,7 @@ void Multigrid::refresh() {
   }
   if (dense_) {
     if (!have_inverse_) {
-      factor_into(ainv_, s);
+      factor_checked(ainv_, s);
       have_inverse_ = true;
       swap_pending_ = false;
     } else {
@@ -185,7 +185,7 @@ void Multigrid::refresh() {
       // previous density's inverse; swapped in at a fixed batch (before_batch)
       HIPCHK(hipEventRecord(ev_side_, s));
       HIPCHK(hipStreamWaitEvent(side_, ev_side_, 0));
-      factor_into(anext_, side_);
+      factor_into(anext_, side_, 0.0);
       HIPCHK(hipEventRecord(ev_fact_, side_));
       swap_pending_ = true;
     }
@@ -193,14 +193,16 @@ void Multigrid::refresh() {
 }
 
 // A = K of the coarsest grid (SPD on the free DOFs; constrained DOFs carry
-// identity rows), then dst = A^-1 by Cholesky, all on stream s.  The
-// factorization's status is read once the solve has synchronised
-// (check_factor): a failed factor only makes the V-cycle's output useless.
-void Multigrid::factor_into(double *dst, hipStream_t s) {
+// identity rows), plus shift * diag(A) on the free DOFs when shift > 0, then
+// dst = A^-1 by Cholesky, all on stream s.  The factorization's status goes to
+// the pinned h_info_ (read by factor_checked and swap_inverse before the
+// inverse is used).
+void Multigrid::factor_into(double *dst, hipStream_t s, double shift) {
   Context &cc = level(levels() - 1);
   const int nd = (int)(3 * cc.lattice().numnp);
   launch_mg_dense_assemble(cc.lattice(), cc.d_ke, cc.d_young, cc.d_free, dst, s,
                            galerkin_ ? kmat_[levels() - 1].p : nullptr);
+  if (shift > 0.0) launch_mg_dense_shift(nd, cc.lattice(), cc.d_free, shift, dst, s);
   auto chk = [](rocblas_status st, const char *what) {
     if (st != rocblas_status_success)
       throw Error(kInternal, std::string(what) + " failed: " + rocblas_status_to_string(st));
@@ -212,11 +214,29 @@ void Multigrid::factor_into(double *dst, hipStream_t s) {
   launch_mg_dense_symmetrize(nd, dst, s);
 }
 
+// Synchronous factorization into dst on s, checked before the inverse is used;
+// a Cholesky breakdown (non-positive pivot in floating point) is retried once
+// with the diagonal of the free DOFs raised by a relative 1e-10.
+void Multigrid::factor_checked(double *dst, hipStream_t s) {
+  factor_into(dst, s, 0.0);
+  HIPCHK(hipStreamSynchronize(s));
+  if (*h_info_ == 0) return;
+  factor_into(dst, s, 1e-10);
+  HIPCHK(hipStreamSynchronize(s));
+  check_factor();
+}
+
 void Multigrid::swap_inverse(hipStream_t s) {
   if (!swap_pending_) return;
+  swap_pending_ = false;
+  // the side stream's status is read before its inverse replaces the current one
+  HIPCHK(hipEventSynchronize(ev_fact_));
+  if (*h_info_ != 0) {
+    factor_checked(ainv_, s);
+    return;
+  }
   HIPCHK(hipStreamWaitEvent(s, ev_fact_, 0));
   HIPCHK(hipMemcpyAsync(ainv_.p, anext_.p, ainv_.bytes(), hipMemcpyDeviceToDevice, s));
-  swap_pending_ = false;
 }
 
 void Multigrid::before_batch(hipStream_t s) {

@@ -26,7 +26,7 @@ inline icarat_mg_opts mg_default_opts() {
   o.nu_pre = 1;
   o.nu_post = 1;
   o.nu_coarse = 20;
-  o.omega = 1.8;
+  o.omega = 1.95;
   o.batch_iters = 4;
   o.dense_dofs = 1024;
   o.direct_nodes = 262144;
@@ -104,7 +104,8 @@ class Multigrid {
   bool have_inverse_ = false;  // ainv_ holds the inverse of this or an earlier density
   bool swap_pending_ = false;
   int batches_ = 0;            // batches launched since the solve's start
-  void factor_into(double *dst, hipStream_t s);
+  void factor_into(double *dst, hipStream_t s, double shift);
+  void factor_checked(double *dst, hipStream_t s);
   void swap_inverse(hipStream_t s);  // ainv_ = anext_ once factored (if pending)
   void before_batch(hipStream_t s);  // the swap at its fixed batch
   ::_rocblas_handle *blas_ = nullptr;

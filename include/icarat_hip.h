@@ -414,7 +414,7 @@ typedef struct icarat_mg_opts {
                               KE0; or 5: Galerkin, R A P element by element (explicit 24x24
                               matrices per coarse element, 4.6 KB each) */
 } icarat_mg_opts;
-/* Fills *o with the defaults (10, 1, 1, 20, 1.8, 4, 1024, 262144). */
+/* Fills *o with the defaults (10, 1, 1, 20, 1.95, 4, 1024, 262144). */
 int icarat_hip_mg_default_opts(icarat_mg_opts *o);
 /* (Re)builds the hierarchy of the context (o = NULL: defaults).  Needs make_dof. */
 int icarat_hip_mg_setup(icarat_hip_ctx *ctx, const icarat_mg_opts *o);
