@@ -278,6 +278,134 @@ __global__ __launch_bounds__(256) void k_mg_direct(Lattice L, const double *__re
   direct_out<MODE>(L.Np, n, fmask[n], y, xc, b, dinv, omega, out);
 }
 
+// The coarse correction and the post-smoothing sweep of a direct level in one
+// pass: xp = x + P xc (masked by the fine free DOFs, as k_mg_prolong_add<true>
+// forms it), out = xp + omega dinv (b - K xp) (masked; xp itself is not
+// stored).  A block owns QTX x QTY x QTZ fine nodes; it stages the coarse nodes
+// under them, then xp on the tile and one node of halo, then contracts from LDS.
+constexpr int QTX = 16, QTY = 4, QTZ = 4;
+constexpr int QHX = QTX + 2, QHY = QTY + 2, QHZ = QTZ + 2, QHN = QHX * QHY * QHZ;
+// Halo fine node g reads coarse nodes g / 2 and g / 2 + 1 (the second with
+// weight 0 when g is even): local coarse indices 0 .. QT / 2 + 2.
+constexpr int QCX = QTX / 2 + 3, QCY = QTY / 2 + 3, QCZ = QTZ / 2 + 3, QCN = QCX * QCY * QCZ;
+
+template <int SYM>
+__global__ __launch_bounds__(256) void k_mg_prolong_jacobi(
+    Lattice Lf, Lattice Lc, const double *__restrict__ c_KE0, KeVals kv,
+    const double *__restrict__ young, const uint8_t *__restrict__ fmask,
+    const double *__restrict__ xcoarse, const double *__restrict__ x, const double *__restrict__ b,
+    const double *__restrict__ dinv, double omega, double *__restrict__ out) {
+  __shared__ double sc[3][QCN];
+  __shared__ double su[3][QHN];
+  const int fx0 = blockIdx.x * QTX, fy0 = blockIdx.y * QTY, fz0 = blockIdx.z * QTZ;
+  // coarse node C0 + h covers fine halo nodes from 2 (C0 + h) - 1 to 2 (C0 + h) + 1
+  const int cx0 = fx0 / 2 - 1, cy0 = fy0 / 2 - 1, cz0 = fz0 / 2 - 1;
+  const int tx = threadIdx.x % QTX, ty = (threadIdx.x / QTX) % QTY, tz = threadIdx.x / (QTX * QTY);
+  const int i = fx0 + tx, j = fy0 + ty, k = fz0 + tz;
+  const bool own = i < Lf.NX && j < Lf.NY && k < Lf.NZ;
+  const long long n = own ? i + (long long)Lf.NX * (j + (long long)Lf.NY * k) : 0;
+  // every global load is issued before the first barrier: the coarse nodes,
+  // the halo's x and mask, and the owned node's elements, b and dinv
+  constexpr int NC = (QCN + 255) / 256, NH = (QHN + 255) / 256;
+  double cv[NC][3];
+#pragma unroll
+  for (int r = 0; r < NC; ++r) {
+    const int t = threadIdx.x + 256 * r;
+    const int gx = cx0 + t % QCX, gy = cy0 + (t / QCX) % QCY, gz = cz0 + t / (QCX * QCY);
+    const bool in = t < QCN && gx >= 0 && gx < Lc.NX && gy >= 0 && gy < Lc.NY && gz >= 0 && gz < Lc.NZ;
+    const long long q = in ? gx + (long long)Lc.NX * (gy + (long long)Lc.NY * gz) : 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) cv[r][c] = in ? xcoarse[c * Lc.Np + q] : 0.0;
+  }
+  double hv[NH][3];
+  unsigned hm[NH];
+  bool hin[NH];
+#pragma unroll
+  for (int r = 0; r < NH; ++r) {
+    const int t = threadIdx.x + 256 * r;
+    const int gx = fx0 - 1 + t % QHX, gy = fy0 - 1 + (t / QHX) % QHY, gz = fz0 - 1 + t / (QHX * QHY);
+    hin[r] = t < QHN && gx >= 0 && gx < Lf.NX && gy >= 0 && gy < Lf.NY && gz >= 0 && gz < Lf.NZ;
+    const long long m = hin[r] ? gx + (long long)Lf.NX * (gy + (long long)Lf.NY * gz) : 0;
+    hm[r] = hin[r] ? fmask[m] : 0u;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) hv[r][c] = hin[r] ? x[c * Lf.Np + m] : 0.0;
+  }
+  double E[8], bo[3], dv[3];
+  unsigned mo = 0;
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    const int ex = i + (s & 1) - 1, ey = j + ((s >> 1) & 1) - 1, ez = k + ((s >> 2) & 1) - 1;
+    const bool in = own && ex >= 0 && ex < Lf.nx && ey >= 0 && ey < Lf.ny && ez >= 0 && ez < Lf.nz;
+    E[s] = in ? young[ex + (long long)Lf.nx * (ey + (long long)Lf.ny * ez)] : 0.0;
+  }
+  if (own) {
+    mo = fmask[n];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      bo[c] = b[c * Lf.Np + n];
+      dv[c] = dinv[c * Lf.Np + n];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < NC; ++r) {
+    const int t = threadIdx.x + 256 * r;
+    if (t < QCN) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) sc[c][t] = cv[r][c];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < NH; ++r) {
+    const int t = threadIdx.x + 256 * r;
+    if (t >= QHN) continue;
+    double v[3] = {0.0, 0.0, 0.0};
+    if (hin[r]) {
+      const int gx = fx0 - 1 + t % QHX, gy = fy0 - 1 + (t / QHX) % QHY, gz = fz0 - 1 + t / (QHX * QHY);
+      // lower coarse neighbour (local) and 1-D weights, as in k_mg_prolong_add
+      const int lx = (gx >> 1) - cx0, ly = (gy >> 1) - cy0, lz = (gz >> 1) - cz0;
+      const double wx1 = (gx & 1) ? 0.5 : 0.0, wy1 = (gy & 1) ? 0.5 : 0.0, wz1 = (gz & 1) ? 0.5 : 0.0;
+      const double wx[2] = {1.0 - wx1, wx1}, wy[2] = {1.0 - wy1, wy1}, wz[2] = {1.0 - wz1, wz1};
+      double s[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+      for (int c2 = 0; c2 < 2; ++c2)
+#pragma unroll
+        for (int bb = 0; bb < 2; ++bb)
+#pragma unroll
+          for (int a = 0; a < 2; ++a) {
+            const double w = wx[a] * wy[bb] * wz[c2];
+            const int h = (lx + a) + QCX * ((ly + bb) + QCY * (lz + c2));
+#pragma unroll
+            for (int c = 0; c < 3; ++c) s[c] += w * sc[c][h];
+          }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[c] = ((hm[r] >> c) & 1u) ? hv[r][c] + s[c] : hv[r][c];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) su[c][t] = v[c];
+  }
+  __syncthreads();
+  if (!own) return;
+  double U[27][3];
+#pragma unroll
+  for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int h = (tx + 1 + dx) + QHX * ((ty + 1 + dy) + QHY * (tz + 1 + dz));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) U[nb(dx, dy, dz)][c] = su[c][h];
+      }
+  double y[3];
+  node_contract<SYM>(c_KE0, kv, U, E, y);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double v = U[13][c] + omega * dv[c] * (bo[c] - y[c]);
+    out[c * Lf.Np + n] = ((mo >> c) & 1u) ? v : 0.0;
+  }
+}
+
 // ---- dense exact solve on the coarsest lattice -----------------------------
 
 __global__ void k_mg_dense_zero(long long nn, double *__restrict__ A) {
@@ -455,6 +583,26 @@ unsigned launch_mg_jacobi_dot(long long n, double omega, const double *dinv, con
                      xout, q, part);
   HIPCHK(hipGetLastError());
   return DOT_BLOCKS;
+}
+
+void launch_mg_prolong_jacobi(const Lattice &Lf, const Lattice &Lc, const KeCoef &kc,
+                              const double *young, const uint8_t *fmask, const double *xcoarse,
+                              const double *x, const double *b, const double *dinv, double omega,
+                              double *out, hipStream_t s) {
+  if (Lf.NX != 2 * Lc.NX - 1 || Lf.NY != 2 * Lc.NY - 1 || Lf.NZ != 2 * Lc.NZ - 1)
+    throw Error(kInvalidArgument, "prolongation: lattices are not 2x apart");
+  if (out == x) throw Error(kInternal, "prolong_jacobi: out may not alias x");
+  const dim3 g((Lf.NX + QTX - 1) / QTX, (Lf.NY + QTY - 1) / QTY, (Lf.NZ + QTZ - 1) / QTZ), t(256);
+  if (kc.sym == 2)
+    hipLaunchKernelGGL(k_mg_prolong_jacobi<2>, g, t, 0, s, Lf, Lc, kc.ke, kc.vals, young, fmask,
+                       xcoarse, x, b, dinv, omega, out);
+  else if (kc.sym == 1)
+    hipLaunchKernelGGL(k_mg_prolong_jacobi<1>, g, t, 0, s, Lf, Lc, kc.ke, kc.vals, young, fmask,
+                       xcoarse, x, b, dinv, omega, out);
+  else
+    hipLaunchKernelGGL(k_mg_prolong_jacobi<0>, g, t, 0, s, Lf, Lc, kc.ke, kc.vals, young, fmask,
+                       xcoarse, x, b, dinv, omega, out);
+  HIPCHK(hipGetLastError());
 }
 
 void launch_mg_direct(int mode, const Lattice &L, const KeCoef &kc, const double *young,

@@ -42,6 +42,14 @@ void launch_mg_direct(int mode, const Lattice &L, const KeCoef &kc, const double
                       const uint8_t *fmask, const double *x, const double *b, const double *dinv,
                       double omega, double *out, hipStream_t s);
 
+// out = xp + omega dinv (b - K xp) with xp = x + P xcoarse at the fine free
+// DOFs (x elsewhere): launch_mg_prolong_add followed by an MG_JACOBI sweep, in
+// one pass and without storing xp (out may not alias x)
+void launch_mg_prolong_jacobi(const Lattice &Lf, const Lattice &Lc, const KeCoef &kc,
+                              const double *young, const uint8_t *fmask, const double *xcoarse,
+                              const double *x, const double *b, const double *dinv, double omega,
+                              double *out, hipStream_t s);
+
 // Exact solve on a small lattice through the dense inverse of its K.  Dense
 // index d = c * numnp + n for lattice DOF c * Np + n; constrained DOFs carry
 // identity rows and columns.

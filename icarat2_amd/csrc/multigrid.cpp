@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <string>
 
 #include "context.hpp"
@@ -48,6 +49,7 @@ double jacobi_bound(const double *ke) {
 
 Multigrid::Multigrid(Context &fine, const icarat_mg_opts &o) : f_(fine), o_(o) {
   f_.ensure_device();
+  if (const char *e = std::getenv("ICARAT_HIP_MG_FUSE")) fuse_post_ = std::string(e) != "0";
   const GridSpec g0 = f_.problem().spec();
   GridSpec g = g0;
   const long long dense_max = std::min<long long>(std::max(0, o_.dense_dofs), MG_DENSE_MAX);
@@ -326,10 +328,22 @@ double *Multigrid::cycle(int l, const double *b, bool emit, bool x0_ready, ZDot 
     }
   }
   const double *xc = cycle(l + 1, cc.d_b, emit);
-  // x = 0 without pre-smoothing: the correction is the iterate
-  if (emit) launch_mg_prolong_add(L, cc.lattice(), xc, c.d_free, cur, s, pre > 0);
   const int post = std::max(0, o_.nu_post);
-  for (int i = 0; i < post; ++i) {
+  // a direct level's first post-sweep reads x + P xc straight from the
+  // correction (same arithmetic as the two kernels, one pass fewer)
+  const bool fused = fuse_post_ && direct_[l] && !(galerkin_ && l > 0) && pre > 0 && post > 0;
+  int first = 0;
+  if (fused) {
+    if (emit)
+      launch_mg_prolong_jacobi(L, cc.lattice(), c.kc_, c.d_young, c.d_free, xc, cur, b, c.d_dinv, w,
+                               oth, s);
+    std::swap(cur, oth);
+    first = 1;
+  } else if (emit) {
+    // x = 0 without pre-smoothing: the correction is the iterate
+    launch_mg_prolong_add(L, cc.lattice(), xc, c.d_free, cur, s, pre > 0);
+  }
+  for (int i = first; i < post; ++i) {
     if (i == post - 1 && l == 0 && !direct_[l]) {
       // The last fine sweep runs out of place (the V-cycle's result is always
       // the other buffer, whatever the kernel shape) and also yields the

@@ -182,3 +182,30 @@ def test_galerkin_two_grid_is_exact_on_coarse_solutions(gpu, avg, exact):
         assert st.iters <= 1, st.iters
     else:
         assert st.iters > 1
+
+
+@pytest.mark.parametrize("dims,direct", [((48, 16, 16), 1 << 30), ((60, 20, 20), 1 << 30),
+                                         ((64, 32, 32), 4096), ((30, 10, 14), 1 << 30)])
+def test_fused_correction_sweep_matches_separate_kernels(gpu, monkeypatch, dims, direct):
+    """On the one-thread-per-node levels the coarse correction x + P e and the
+    first post-sweep run as one kernel.  It forms the same values in the same
+    order as the prolongation and sweep kernels it replaces, so the solve takes
+    the same iterations and the solutions agree to rounding (1e-12 of max |u|).
+    Grids include tiles cut by the lattice edge in every axis (30 x 10 x 14:
+    nodes 31 x 11 x 15 against 16 x 4 x 4 tiles) and a hierarchy with both
+    kernel shapes (direct_nodes 4096)."""
+    c = gpu.cantilever(*dims)
+    x, _ = gpu.gaussian_field(*dims, seed=21, smin=2.0, smax=6.0)
+    c.set_density(x)
+    res = {}
+    for fuse in ("0", "1"):
+        monkeypatch.setenv("ICARAT_HIP_MG_FUSE", fuse)
+        c.mg_setup(direct_nodes=direct)
+        u, st = c.solve(tol=1e-10, precond=gpu.PRECOND_MG)
+        assert st.status == 0 and st.true_relres <= 1e-10
+        res[fuse] = (u, st.iters)
+        print(f"{dims} fuse {fuse}: {st.iters} iterations, {st.seconds * 1e3:.2f} ms, "
+              f"levels {c.mg_info()}")
+    (u0, it0), (u1, it1) = res["0"], res["1"]
+    assert it0 == it1
+    assert np.max(np.abs(u1 - u0)) <= 1e-12 * np.max(np.abs(u0))
