@@ -49,7 +49,7 @@ double jacobi_bound(const double *ke) {
 
 Multigrid::Multigrid(Context &fine, const icarat_mg_opts &o) : f_(fine), o_(o) {
   f_.ensure_device();
-  if (const char *e = std::getenv("ICARAT_HIP_MG_FUSE")) fuse_post_ = std::string(e) != "0";
+  if (const char *e = std::getenv("ICARAT_HIP_MG_FUSE")) fuse_ = std::string(e) != "0";
   const GridSpec g0 = f_.problem().spec();
   GridSpec g = g0;
   const long long dense_max = std::min<long long>(std::max(0, o_.dense_dofs), MG_DENSE_MAX);
@@ -331,7 +331,7 @@ double *Multigrid::cycle(int l, const double *b, bool emit, bool x0_ready, ZDot 
   const int post = std::max(0, o_.nu_post);
   // a direct level's first post-sweep reads x + P xc straight from the
   // correction (same arithmetic as the two kernels, one pass fewer)
-  const bool fused = fuse_post_ && direct_[l] && !(galerkin_ && l > 0) && pre > 0 && post > 0;
+  const bool fused = fuse_ && direct_[l] && !(galerkin_ && l > 0) && pre > 0 && post > 0;
   int first = 0;
   if (fused) {
     if (emit)

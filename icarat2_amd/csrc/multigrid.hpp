@@ -82,7 +82,7 @@ class Multigrid {
   std::vector<char> direct_;
   // direct levels: the coarse correction and the first post-sweep in one
   // kernel (launch_mg_prolong_jacobi); ICARAT_HIP_MG_FUSE=0 keeps them apart
-  bool fuse_post_ = true;
+  bool fuse_ = true;
   // Galerkin coarse operators (coarse_avg 5): explicit element matrices per
   // coarse level (kmat_[l], 576 per element), the per-element smoothing bounds
   // and their maximum per level; the damping is folded into the levels' dinv
