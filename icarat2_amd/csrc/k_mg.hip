@@ -20,6 +20,18 @@ namespace {
 
 inline unsigned blocks_for(long long n, int nt) { return (unsigned)((n + nt - 1) / nt); }
 
+// XCD-aware tile order for a 1-D grid over nbx x nby x nbz tiles: workgroups go
+// to the 8 XCDs round-robin by id, so each XCD is given a contiguous range of
+// tiles (x fastest) and neighbouring tiles share their halos in its L2.
+__device__ __forceinline__ void xcd_tile(int nbx, int nby, int &bx, int &by, int &bz) {
+  const unsigned nt = gridDim.x, b = blockIdx.x;
+  const unsigned q = nt / 8, rem = nt % 8, xcd = b % 8, slot = b / 8;
+  const unsigned T = (xcd < rem ? xcd * (q + 1) : rem * (q + 1) + (xcd - rem) * q) + slot;
+  bx = (int)(T % nbx);
+  by = (int)((T / nbx) % nby);
+  bz = (int)(T / ((unsigned)nbx * nby));
+}
+
 // Restriction and prolongation are tiled through LDS.  A restriction block
 // owns RTX x RTY x RTZ coarse nodes and stages the fine region under them (one
 // fine node of halo on each side) with the residual formed on load, so each
@@ -90,7 +102,9 @@ __global__ __launch_bounds__(256) void k_mg_restrict(Lattice Lf, Lattice Lc,
                                                      const double *__restrict__ dinv_c,
                                                      double omega, double *__restrict__ xc) {
   __shared__ double sv[3][RFN];
-  const int I0 = blockIdx.x * RTX, J0 = blockIdx.y * RTY, K0 = blockIdx.z * RTZ;
+  int bx, by, bz;
+  xcd_tile((Lc.NX + RTX - 1) / RTX, (Lc.NY + RTY - 1) / RTY, bx, by, bz);
+  const int I0 = bx * RTX, J0 = by * RTY, K0 = bz * RTZ;
   const int fx0 = 2 * I0 - 1, fy0 = 2 * J0 - 1, fz0 = 2 * K0 - 1;
   // all loads of the staging are issued before any is stored (load phase into
   // registers, then the LDS phase)
@@ -151,7 +165,9 @@ __global__ __launch_bounds__(256) void k_mg_prolong_add(Lattice Lf, Lattice Lc,
                                                         const uint8_t *__restrict__ fmask_f,
                                                         double *__restrict__ xf) {
   __shared__ double sc[3][PCN];
-  const int fx0 = blockIdx.x * PTX, fy0 = blockIdx.y * PTY, fz0 = blockIdx.z * PTZ;
+  int bx, by, bz;
+  xcd_tile((Lf.NX + PTX - 1) / PTX, (Lf.NY + PTY - 1) / PTY, bx, by, bz);
+  const int fx0 = bx * PTX, fy0 = by * PTY, fz0 = bz * PTZ;
   const int I0 = fx0 / 2, J0 = fy0 / 2, K0 = fz0 / 2;
   const int tx = threadIdx.x % PTX, ty = (threadIdx.x / PTX) % PTY, tz = threadIdx.x / (PTX * PTY);
   const int x = fx0 + tx, y = fy0 + ty, z = fz0 + tz;
@@ -499,7 +515,8 @@ void launch_mg_restrict(const Lattice &Lf, const Lattice &Lc, const double *rf, 
                         double *xc, hipStream_t s) {
   if (Lf.NX != 2 * Lc.NX - 1 || Lf.NY != 2 * Lc.NY - 1 || Lf.NZ != 2 * Lc.NZ - 1)
     throw Error(kInvalidArgument, "restriction: lattices are not 2x apart");
-  const dim3 g((Lc.NX + RTX - 1) / RTX, (Lc.NY + RTY - 1) / RTY, (Lc.NZ + RTZ - 1) / RTZ);
+  const dim3 g((unsigned)(((Lc.NX + RTX - 1) / RTX) * ((Lc.NY + RTY - 1) / RTY) *
+                          ((Lc.NZ + RTZ - 1) / RTZ)));
   if (yf)
     hipLaunchKernelGGL(k_mg_restrict<true>, g, dim3(256), 0, s, Lf, Lc, rf, yf, fmask_c, bc, dinv_c,
                        omega, xc);
@@ -513,7 +530,8 @@ void launch_mg_prolong_add(const Lattice &Lf, const Lattice &Lc, const double *x
                            const uint8_t *fmask_f, double *xf, hipStream_t s, bool add) {
   if (Lf.NX != 2 * Lc.NX - 1 || Lf.NY != 2 * Lc.NY - 1 || Lf.NZ != 2 * Lc.NZ - 1)
     throw Error(kInvalidArgument, "prolongation: lattices are not 2x apart");
-  const dim3 g((Lf.NX + PTX - 1) / PTX, (Lf.NY + PTY - 1) / PTY, (Lf.NZ + PTZ - 1) / PTZ);
+  const dim3 g((unsigned)(((Lf.NX + PTX - 1) / PTX) * ((Lf.NY + PTY - 1) / PTY) *
+                          ((Lf.NZ + PTZ - 1) / PTZ)));
   if (add)
     hipLaunchKernelGGL(k_mg_prolong_add<true>, g, dim3(256), 0, s, Lf, Lc, xc, fmask_f, xf);
   else
