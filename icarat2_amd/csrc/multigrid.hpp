@@ -25,7 +25,7 @@ inline icarat_mg_opts mg_default_opts() {
   o.max_levels = 10;
   o.nu_pre = 1;
   o.nu_post = 1;
-  o.nu_coarse = 20;
+  o.nu_coarse = 6;
   o.omega = 1.95;
   o.batch_iters = 4;
   o.dense_dofs = 1024;

@@ -405,7 +405,9 @@ typedef struct icarat_mg_opts {
   int32_t batch_iters;     /* PCG iterations per graph launch */
   int32_t dense_dofs;      /* coarsening stops at the first grid of at most this many lattice
                               DOFs (3 x nodes, capped at 4096), which is solved exactly by
-                              its dense inverse (0: never; the coarsest grid is smoothed) */
+                              its dense inverse; a coarsest grid that cannot be halved
+                              further is solved exactly up to 1.5 x this many (0: never;
+                              the coarsest grid is smoothed) */
   int32_t direct_nodes;    /* levels of at most this many nodes smooth with the fused
                               one-thread-per-node sweep; larger ones with the z-marching K.u */
   int32_t coarse_avg;      /* coarse operators: a coarse element's Young's modulus from its 8
@@ -414,7 +416,7 @@ typedef struct icarat_mg_opts {
                               KE0; or 5: Galerkin, R A P element by element (explicit 24x24
                               matrices per coarse element, 4.6 KB each) */
 } icarat_mg_opts;
-/* Fills *o with the defaults (10, 1, 1, 20, 1.95, 4, 1024, 262144). */
+/* Fills *o with the defaults (10, 1, 1, 6, 1.95, 4, 1024, 262144). */
 int icarat_hip_mg_default_opts(icarat_mg_opts *o);
 /* (Re)builds the hierarchy of the context (o = NULL: defaults).  Needs make_dof. */
 int icarat_hip_mg_setup(icarat_hip_ctx *ctx, const icarat_mg_opts *o);
