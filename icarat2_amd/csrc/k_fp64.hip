@@ -121,9 +121,10 @@ __global__ __launch_bounds__(NT) void k_stencil64(
     if (gx >= 0 && gx < L.NX && gy >= 0 && gy < L.NY && gz >= 0 && gz < L.NZ) {
       const long long n = gx + (long long)L.NX * (gy + (long long)L.NY * gz);
       if constexpr (PCG) {
-        v0 = dinv[n] * src[n] + beta * p_old[n];
-        v1 = dinv[Np + n] * src[Np + n] + beta * p_old[Np + n];
-        v2 = dinv[2 * Np + n] * src[2 * Np + n] + beta * p_old[2 * Np + n];
+        // dinv == nullptr: p = src + beta p_old (src is already preconditioned)
+        v0 = (dinv ? dinv[n] * src[n] : src[n]) + beta * p_old[n];
+        v1 = (dinv ? dinv[Np + n] * src[Np + n] : src[Np + n]) + beta * p_old[Np + n];
+        v2 = (dinv ? dinv[2 * Np + n] * src[2 * Np + n] : src[2 * Np + n]) + beta * p_old[2 * Np + n];
         if (hx >= 1 && hx <= SX && hy >= 1 && hy <= SY && hz >= 1 && hz <= SZ) {
           p_new[n] = v0;
           p_new[Np + n] = v1;
@@ -225,7 +226,8 @@ __device__ __forceinline__ void fetch_plane(const Lattice &L, int x0, int y0, in
     for (int c = 0; c < 3; ++c) {
       F.v[k][c] = in ? src[c * L.Np + n] : 0.0;
       if constexpr (PCG) {
-        F.d[k][c] = in ? dinv[c * L.Np + n] : 0.0;
+        // dinv == nullptr: p = src + beta p_old (src is already preconditioned)
+        F.d[k][c] = in ? (dinv ? dinv[c * L.Np + n] : 1.0) : 0.0;
         F.pold[k][c] = in ? p_old[c * L.Np + n] : 0.0;
       }
     }

@@ -59,7 +59,8 @@ unsigned launch_apply_k64_jacobi(const Lattice &L, const KeCoef &ke, const doubl
                                  const uint8_t *freemask, const double *x, double *xout,
                                  const double *b, const double *dinv, double omega,
                                  const double *q, double *partials, hipStream_t s);
-// One fused Jacobi-PCG iteration, stencil half: p_new = dinv*r + beta*p_old;
+// One fused Jacobi-PCG iteration, stencil half: p_new = dinv*r + beta*p_old
+// (dinv == nullptr: p_new = r + beta*p_old, for a preconditioned r);
 // q = K p_new; p.q partials (returns their count); with finalize, alpha = rho /
 // p.q by a finalize kernel.
 unsigned launch_pcg_stencil64(const Lattice &L, const KeCoef &ke, const double *young,

@@ -120,7 +120,6 @@ Multigrid::Multigrid(Context &fine, const icarat_mg_opts &o) : f_(fine), o_(o) {
   z_.alloc(f_.n3_);
   y0_.alloc(f_.n3_);
   r0_.alloc(f_.n3_);
-  ones_.alloc(f_.n3_);
   hipStream_t s = f_.stream();
   HIPCHK(hipMemsetAsync(z_.p, 0, z_.bytes(), s));
   HIPCHK(hipMemsetAsync(y0_.p, 0, y0_.bytes(), s));
@@ -130,16 +129,7 @@ Multigrid::Multigrid(Context &fine, const icarat_mg_opts &o) : f_(fine), o_(o) {
     for (DevBuf<double> *v : {&c.d_b, &c.d_x, &c.d_y, &c.d_r})
       HIPCHK(hipMemsetAsync(v->p, 0, v->bytes(), s));
   }
-  {
-    // 1 at free DOFs: the PCG stencil then forms p = z + beta p_old
-    const Lattice &L = f_.lattice();
-    std::vector<double> ones(f_.n3_, 0.0);
-    const auto &fm = f_.problem().free_mask();
-    for (long long n = 0; n < L.numnp; ++n)
-      for (int c = 0; c < 3; ++c) ones[c * L.Np + n] = ((fm[n] >> c) & 1u) ? 1.0 : 0.0;
-    HIPCHK(hipMemcpyAsync(ones_.p, ones.data(), ones_.bytes(), hipMemcpyHostToDevice, s));
-    f_.sync();
-  }
+  f_.sync();
   zres_ = cycle(0, nullptr, false);
 }
 
@@ -391,7 +381,7 @@ void Multigrid::enqueue_batch(int par) {
   for (int j = 0; j < std::max(1, o_.batch_iters); ++j) {
     double *p_old = c.d_p[(j + par) & 1], *p_new = c.d_p[(j + par + 1) & 1];
     // p = z + beta p_old, q = K p, p.q partials
-    const unsigned gs = launch_pcg_stencil64(L, c.kc_, c.d_young, c.d_free, zres_, ones_, p_old,
+    const unsigned gs = launch_pcg_stencil64(L, c.kc_, c.d_young, c.d_free, zres_, nullptr, p_old,
                                              p_new, c.d_q, c.d_pcg, c.d_partials, c.d_counter, s,
                                              false);
     // alpha, x and r, r.r partials, z0 = omega D^-1 r

@@ -78,7 +78,7 @@ class Multigrid {
   icarat_mg_opts o_;
   std::vector<std::unique_ptr<Context>> lv_;  // levels 1..L-1
   std::vector<double> omega_;                  // Jacobi damping per level
-  DevBuf<double> z_, y0_, r0_, ones_;
+  DevBuf<double> z_, y0_, r0_;
   std::vector<char> direct_;
   // direct levels: the coarse correction and the first post-sweep in one
   // kernel (launch_mg_prolong_jacobi); ICARAT_HIP_MG_FUSE=0 keeps them apart
