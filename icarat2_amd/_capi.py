@@ -107,6 +107,7 @@ _SIGS = {
     "icarat_hip_filter": (C.c_int, [C.c_void_p, dp, dp, C.c_int32]),
     "icarat_hip_helm_setup": (C.c_int, [C.c_void_p, C.c_double]),
     "icarat_hip_mises": (C.c_int, [C.c_void_p, dp, dp]),
+    "icarat_hip_mises_noalloc": (C.c_int, [C.c_void_p, dp, dp]),
     "icarat_hip_create_from_toml": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p),
                                               C.c_void_p, C.c_char_p, C.c_int32]),
     "icarat_hip_write_vtk": (C.c_int, [C.c_void_p, C.c_char_p, dp]),
@@ -460,6 +461,13 @@ class Context:
         m = np.zeros(self.nelx)
         st = np.zeros(6 * self.nelx)
         self._chk(self.L.icarat_hip_mises(self.h, _p(m), _p(st)), "mises")
+        return m, st.reshape(self.nelx, 6)
+
+    def mises_noalloc(self):
+        """As mises(), through the entry point that allocates no device memory."""
+        m = np.zeros(self.nelx)
+        st = np.zeros(6 * self.nelx)
+        self._chk(self.L.icarat_hip_mises_noalloc(self.h, _p(m), _p(st)), "mises_noalloc")
         return m, st.reshape(self.nelx, 6)
 
     def gcmma_set_params(self, move=0.15, albefa=0.2, asyminit=0.2, asymdec=0.7, asyminc=1.2):

@@ -279,6 +279,11 @@ int icarat_hip_mises(icarat_hip_ctx *h, double *mises, double *stress) {
   return guard(h, [&](icarat_hip::Context &c) { c.mises_host(mises, stress); });
 }
 
+int icarat_hip_mises_noalloc(icarat_hip_ctx *h, double *mises, double *stress) {
+  if (!mises) return fail(h, ICARAT_HIP_INVALID_ARGUMENT, "null argument");
+  return guard(h, [&](icarat_hip::Context &c) { c.mises_noalloc(mises, stress); });
+}
+
 int icarat_hip_filter_setup(icarat_hip_ctx *h, double radius) {
   return guard(h, [&](icarat_hip::Context &c) { c.filter_setup(radius); });
 }

@@ -50,6 +50,8 @@ void Context::ensure_device() {
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
     throw Error(kHipError, "no HIP device available");
+  HIPCHK(hipGetDevice(&device_));
+  noalloc_init_device(device_);
   HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   d_ke.alloc(576);
   h2d(d_ke.p, prob_.ke0(), 576, stream_);

@@ -21,6 +21,11 @@ class HelmFilter;
 class LearnedPrecond;
 class Multigrid;
 
+// the fixed device scratch of the no-allocation entry point (noalloc.hip):
+// its per-device address is looked up by the device set-up of a context
+void noalloc_init_device(int device);
+long long noalloc_max_elements();
+
 class Context {
  public:
   explicit Context(const GridSpec &g);
@@ -53,6 +58,10 @@ class Context {
   void compliance_sensitivity(double vnorm, double *c, double *dfds, double *dgds);
   // element-mean stress (nelx x 6, may be null) and von Mises stress of the last solve
   void mises_host(double *mises, double *stress);
+  // as mises_host with the element stresses staged in the library's fixed
+  // device scratch array instead of a per-call allocation (at most
+  // noalloc_max_elements() elements when stress is requested)
+  void mises_noalloc(double *mises, double *stress);
   // nodal values of the last solve (numnp x 3, node order; Dirichlet values
   // included, |v| < 1e-15 flushed as analysis/analysis_utility.hpp:26-28)
   void nodal_values_host(double *u);
@@ -221,6 +230,7 @@ class Context {
  private:
   Problem prob_;
   hipStream_t stream_ = nullptr;
+  int device_ = 0;  // the device current at set-up (the stream's device)
   bool dev_ready_ = false;
   bool dof_uploaded_ = false;
   void upload_dof();

@@ -177,6 +177,14 @@ int icarat_hip_compliance_sensitivity(icarat_hip_ctx *ctx, double vnorm, double 
  * the Gauss points of De(E(x_e)) B u_e, mises = sqrt(stress^T V stress).
  * mises: nelx entries; stress (may be NULL): nelx x 6, Voigt [xx,yy,zz,xy,yz,zx]. */
 int icarat_hip_mises(icarat_hip_ctx *ctx, double *mises, double *stress);
+/* No reference counterpart: icarat_hip_mises for callers that cannot allocate
+ * device memory (code under graph capture, tight serving loops).  The same
+ * outputs; the element stresses are staged in a fixed device array of the
+ * library (48 MiB per GPU, held until the process ends) instead of a per-call
+ * allocation, so the call allocates nothing.  With stress != NULL the context
+ * may have at most 1,048,576 elements (else ICARAT_HIP_INVALID_ARGUMENT).  All
+ * calls on one GPU share that array: calls that overlap on one GPU race. */
+int icarat_hip_mises_noalloc(icarat_hip_ctx *ctx, double *mises, double *stress);
 
 /* ---- optimization (Filter / OC / MMASolver) ------------------------------ */
 /* Filter<E,N>(fem, element, node, radius) (optimization/filter.hpp:63-70). */
