@@ -76,6 +76,11 @@ class MgOpts(C.Structure):
                 ("dense_dofs", C.c_int32), ("direct_nodes", C.c_int32), ("coarse_avg", C.c_int32)]
 
 
+class MgStatus(C.Structure):
+    _fields_ = [("levels", C.c_int32), ("dense", C.c_int32), ("direct_fine", C.c_int32),
+                ("fine_fp32", C.c_int32), ("factorizations", C.c_int64), ("shifted", C.c_int64)]
+
+
 fp = C.POINTER(C.c_float)
 i64p = C.POINTER(C.c_int64)
 
@@ -163,6 +168,7 @@ _SIGS = {
     "icarat_hip_batch_simp_run": (C.c_int, [C.c_void_p, C.POINTER(SimpOpts), C.POINTER(dp),
                                             C.POINTER(dp), C.POINTER(SimpStats)]),
     "icarat_hip_batch_autotune": (C.c_int, [C.c_void_p, C.c_int32, i32p, dp]),
+    "icarat_hip_mg_status": (C.c_int, [C.c_void_p, C.POINTER(MgStatus)]),
     "icarat_hip_time_kernel": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, dp, dp, dp]),
     "icarat_hip_gaussian_field": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_double,
                                             C.c_uint64, C.c_int32, C.c_double, C.c_double,
@@ -599,6 +605,11 @@ class Context:
         dims = (C.c_int32 * 60)()
         self._chk(self.L.icarat_hip_mg_info(self.h, C.byref(n), dims), "mg_info")
         return [tuple(dims[3 * l:3 * l + 3]) for l in range(n.value)]
+
+    def mg_status(self):
+        st = MgStatus()
+        self._chk(self.L.icarat_hip_mg_status(self.h, C.byref(st)), "mg_status")
+        return st
 
     def time_kernel(self, kernel, reps=20):
         ms, fl, by = C.c_double(), C.c_double(), C.c_double()

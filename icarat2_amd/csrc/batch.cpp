@@ -105,7 +105,7 @@ void Batch::engine(double tol, int maxit, const std::function<bool(int)> &start,
     for (int i : ver) ctx_[i]->pcg_verify(ms_);
     if (!ver.empty()) HIPCHK(hipStreamSynchronize(ms_));
     for (int i : ver)
-      if (ctx_[i]->h_pcg.p->done) fin.push_back(i);
+      if (ctx_[i]->pcg_after_verify()) fin.push_back(i);
     // finished solves: hand over, then the problem's next solve (if any)
     for (int i : fin) {
       icarat_solve_stats st{};

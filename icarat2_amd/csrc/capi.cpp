@@ -488,6 +488,22 @@ int icarat_hip_mg_info(const icarat_hip_ctx *hc, int32_t *levels, int32_t *dims)
   });
 }
 
+int icarat_hip_mg_status(const icarat_hip_ctx *hc, icarat_mg_status *st) {
+  icarat_hip_ctx *h = const_cast<icarat_hip_ctx *>(hc);
+  if (!st) return fail(h, ICARAT_HIP_INVALID_ARGUMENT, "null argument");
+  return guard(h, [&](icarat_hip::Context &c) {
+    if (!c.mg_) throw icarat_hip::Error(ICARAT_HIP_NOT_READY, "no multigrid hierarchy");
+    icarat_mg_status o{};
+    o.levels = c.mg_->levels();
+    o.dense = c.mg_->dense() ? 1 : 0;
+    o.direct_fine = c.mg_->direct(0) ? 1 : 0;
+    o.fine_fp32 = c.mg_->fine_fp32() ? 1 : 0;
+    o.factorizations = c.mg_->factorizations();
+    o.shifted = c.mg_->shifted();
+    *st = o;
+  });
+}
+
 }  // extern "C"
 
 // ---- the network alone ------------------------------------------------------

@@ -1,5 +1,6 @@
 // context.cpp -- device state and host orchestration of one SIMP problem.
 #include "context.hpp"
+#include "learned_kernels.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -64,6 +65,7 @@ void Context::ensure_device() {
     h2d(d_ke32.p, k32, 576, stream_);
     sync();
   }
+  kc_.ke32 = d_ke32.p;
   for (DevBuf<double> *b : {&d_f, &d_b, &d_dinv, &d_x, &d_r, &d_q, &d_p[0], &d_p[1], &d_u, &d_nv,
                             &d_y, &d_dirlat}) {
     b->alloc(n3_);
@@ -513,6 +515,7 @@ void Context::pcg_start(double tol, int maxit, hipStream_t s, bool warm) {
   fell_back_ = 0;
   if (batch_precond_ == ICARAT_HIP_PRECOND_MG) {
     if (!mg_) mg_setup(mg_default_opts());
+    mg_->set_fine_fp32(false);  // batched multigrid solves run the FP64 V-cycle
     mg_->start(tol, maxit, s, warm && load_warm_guess(s));
     return;
   }
@@ -546,6 +549,11 @@ void Context::pcg_verify(hipStream_t s) {
   launch_replace64(n3_, d_b, d_y, d_r, d_dinv, d_pcg, d_partials, s);
   d2h(h_pcg.p, d_pcg.p, 1, s);
   ++replacements_;
+}
+
+bool Context::pcg_after_verify() {
+  if (batch_precond_ == ICARAT_HIP_PRECOND_MG) return mg_->after_verify();
+  return h_pcg.p->done != 0;
 }
 
 void Context::pcg_finish(icarat_solve_stats &S) {
@@ -1134,6 +1142,7 @@ void Context::set_stencil(int kernel, int sym_max) {
     }
   }
   k.kernel = kernel;
+  k.ke32 = d_ke32.p;
   sync();
   kc_ = k;
   drop_graphs();
@@ -1220,6 +1229,41 @@ void Context::time_kernel(int kernel, int reps, double *avg_ms, double *flops, d
     launch = [this]() { launch_apply_k64(L_, kc_, d_young, d_free, d_r, d_y, stream_); };
     F = 2.0 * 24 * 24 * E;
     B = 2.0 * 3 * N * 8 + E * 8 + N;            // x in, y out; young; mask
+  } else if (kernel == 10) {
+    // the multigrid solve's PCG stencil: p = z + beta p_old (no dinv: z is the
+    // V-cycle's result), q = K p, p.q partials; beta = 0 from the scalars
+    d2d(d_r.p, d_f.p, n3_, stream_);
+    PcgScalars s0{};
+    s0.rho = 1.0;
+    s0.tol = 0.0;
+    s0.maxit = 1 << 30;
+    *h_pcg.p = s0;
+    h2d(d_pcg.p, h_pcg.p, 1, stream_);
+    launch = [this]() {
+      launch_pcg_stencil64(L_, kc_, d_young, d_free, d_r, nullptr, d_p[0], d_p[1], d_q, d_pcg,
+                           d_partials, d_counter, stream_, false);
+    };
+    F = 2.0 * 24 * 24 * E;
+    B = 2.0 * 3 * N * 8 + 2.0 * 3 * N * 8 + E * 8 + N;  // z, p_old in; p, q out; young; mask
+  } else if (kernel == 11) {
+    // the multigrid solve's vector update: alpha = rho / p.q from stencil
+    // partials (ones; rho = 0 keeps the vectors fixed), x, r updated, r.r
+    // partials, z0 = omega dinv r for the V-cycle's first sweep
+    PcgScalars s0{};
+    s0.rho = 0.0;
+    s0.tol = 0.0;
+    s0.maxit = 1 << 30;
+    *h_pcg.p = s0;
+    h2d(d_pcg.p, h_pcg.p, 1, stream_);
+    const unsigned gs = (unsigned)std::min<long long>(stencil_blocks(L_), 4096);
+    std::vector<double> ones(gs, 1.0);
+    h2d(d_partials.p, ones.data(), gs, stream_);
+    launch = [this, gs]() {
+      launch_fpcg_update_alpha(n3_, d_x, d_r, d_p[1], d_q, d_pcg, d_partials, gs, d_partials2,
+                               d_dinv, 1.0, d_y.p, stream_);
+    };
+    F = 8.0 * 3 * N;
+    B = (5.0 + 3.0) * 3 * N * 8;                // x, p, r, q, dinv in; x, r, z0 out
   } else if (kernel == 9) {
     // one Helmholtz filter pass (density mode): element loads, the six 1-D
     // transforms (FP64 GEMMs) with the spectral scaling, element read-back

@@ -203,6 +203,9 @@ class Context {
   // multigrid V-cycle (set up with the defaults on first use).
   void pcg_start(double tol, int maxit, hipStream_t s, bool warm = false);
   void pcg_verify(hipStream_t s);
+  // once the verification's scalars are on the host: true if the solve ends
+  // there (multigrid: passed or stalled at its floor; Jacobi: passed)
+  bool pcg_after_verify();
   void pcg_finish(icarat_solve_stats &S);
   void enqueue_pcg_batch(int kmode, int par);  // kBatch Jacobi-PCG iterations on the stream
   // one batch of iterations of the interleaved solve (batch_precond_): enqueued

@@ -16,8 +16,8 @@ __device__ __forceinline__ constexpr int nb(int dx, int dy, int dz) {
 
 // KE0[e] (e = row*24 + col, a compile-time constant after unrolling): from
 // memory (SYM 0) or as +-orbit value held in scalar registers (SYM 1 box, 2 cube).
-template <int SYM>
-__device__ __forceinline__ double kcoef(const double *__restrict__ c_KE0, const KeVals &kv, int e) {
+template <int SYM, class T>
+__device__ __forceinline__ T kcoef(const T *__restrict__ c_KE0, const KeValsT<T> &kv, int e) {
   if constexpr (SYM == 0) {
     return c_KE0[e];
   } else {

@@ -2,6 +2,7 @@
 // local-node table, deterministic wave/block reductions and the
 // "last block finalises" pattern used by every reduction in the library.
 #pragma once
+#include <cmath>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -23,6 +24,21 @@ struct Lattice {
 __host__ __device__ constexpr int lnode(int ox, int oy, int oz) {
   return (ox == 0 ? (oy == 0 ? 0 : 3) : (oy == 0 ? 1 : 2)) + 4 * oz;
 }
+// x^p of the SIMP law (problem/elasticity.hpp:13-17).  For the usual p = 3
+// the cube is formed with the exact error terms of its two products, which
+// gives the correctly rounded x^3 (the value a correctly rounded pow returns)
+// on the device and on the host alike; any other p goes to pow.
+__host__ __device__ inline double simp_pow(double x, double p) {
+  if (p == 3.0) {
+    const double s = x * x;
+    const double es = fma(x, x, -s);  // x^2 = s + es exactly
+    const double t = s * x;
+    const double et = fma(s, x, -t);  // s x = t + et exactly
+    return t + fma(es, x, et);
+  }
+  return pow(x, p);
+}
+
 __host__ __device__ constexpr int lnode_x(int a) { return ((a & 3) == 1 || (a & 3) == 2) ? 1 : 0; }
 __host__ __device__ constexpr int lnode_y(int a) { return ((a & 3) >= 2) ? 1 : 0; }
 __host__ __device__ constexpr int lnode_z(int a) { return a >> 2; }

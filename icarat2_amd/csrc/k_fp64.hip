@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 
 #include "contract.hpp"
 #include "kernels.hpp"
@@ -202,20 +203,25 @@ constexpr int ZPX = ZX + 2, ZPY = ZY + 2, ZPN = ZPX * ZPY;
 constexpr int ZLX = ZX + 1, ZLY = ZY + 1, ZLN = ZLX * ZLY;
 constexpr int ZPER = (ZPN + NT - 1) / NT;  // plane nodes per thread
 constexpr int ZEPER = (ZLN + NT - 1) / NT; // layer elements per thread
+// output of the z-marching K.u: the node-value type, FP64 for the fused sweep
+template <bool JAC, class T>
+using OutT = typename std::conditional<JAC, double, T>::type;
 
-template <bool PCG>
+// T is the type of the node values, LDS planes and contraction (double; float
+// for the multigrid smoother's FP32 fine level, PCG = false only).
+template <bool PCG, class T = double>
 struct PlaneFetch {
-  double v[ZPER][3];
+  T v[ZPER][3];
   double d[ZPER][3], pold[ZPER][3];  // PCG: dinv and p_old (v holds r)
-  double e[ZEPER];
+  T e[ZEPER];
 };
 
-template <bool PCG>
+template <bool PCG, class T>
 __device__ __forceinline__ void fetch_plane(const Lattice &L, int x0, int y0, int zp,
-                                            const double *__restrict__ src,
+                                            const T *__restrict__ src,
                                             const double *__restrict__ dinv,
                                             const double *__restrict__ p_old,
-                                            const double *__restrict__ young, PlaneFetch<PCG> &F) {
+                                            const T *__restrict__ young, PlaneFetch<PCG, T> &F) {
 #pragma unroll
   for (int k = 0; k < ZPER; ++k) {
     const int t = threadIdx.x + k * NT;
@@ -224,7 +230,7 @@ __device__ __forceinline__ void fetch_plane(const Lattice &L, int x0, int y0, in
     const long long n = in ? gx + (long long)L.NX * (gy + (long long)L.NY * zp) : 0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      F.v[k][c] = in ? src[c * L.Np + n] : 0.0;
+      F.v[k][c] = in ? src[c * L.Np + n] : T(0);
       if constexpr (PCG) {
         // dinv == nullptr: p = src + beta p_old (src is already preconditioned)
         F.d[k][c] = in ? (dinv ? dinv[c * L.Np + n] : 1.0) : 0.0;
@@ -240,22 +246,21 @@ __device__ __forceinline__ void fetch_plane(const Lattice &L, int x0, int y0, in
     const int ex = x0 - 1 + t % ZLX, ey = y0 - 1 + t / ZLX;
     const bool in = t < ZLN && ze >= 0 && ze < L.nz && ex >= 0 && ex < L.nx && ey >= 0 && ey < L.ny;
     const long long ei = in ? ex + (long long)L.nx * (ey + (long long)L.ny * ze) : 0;
-    F.e[k] = in ? young[ei] : 0.0;
+    F.e[k] = in ? young[ei] : T(0);
   }
 }
 
 // Writes a fetched plane (and layer) into LDS; PCG: forms p = dinv r + beta
 // p_old and stores it to p_new for owned nodes.
-template <bool PCG>
+template <bool PCG, class T>
 __device__ __forceinline__ void commit_plane(const Lattice &L, int x0, int y0, int zp, bool own_z,
-                                             double beta, const PlaneFetch<PCG> &F,
-                                             double *__restrict__ p_new, double (*sp)[ZPN],
-                                             double *se) {
+                                             double beta, const PlaneFetch<PCG, T> &F,
+                                             double *__restrict__ p_new, T (*sp)[ZPN], T *se) {
 #pragma unroll
   for (int k = 0; k < ZPER; ++k) {
     const int t = threadIdx.x + k * NT;
     if (t >= ZPN) continue;
-    double v[3];
+    T v[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       if constexpr (PCG) v[c] = F.d[k][c] * F.v[k][c] + beta * F.pold[k][c];
@@ -280,7 +285,8 @@ __device__ __forceinline__ void commit_plane(const Lattice &L, int x0, int y0, i
 }
 
 // 3x3 neighbourhood of the thread's column in one LDS plane
-__device__ __forceinline__ void read_nbhd(const double (*sp)[ZPN], int tx, int ty, double (&U)[9][3]) {
+template <class T>
+__device__ __forceinline__ void read_nbhd(const T (*sp)[ZPN], int tx, int ty, T (&U)[9][3]) {
 #pragma unroll
   for (int dy = -1; dy <= 1; ++dy)
 #pragma unroll
@@ -293,24 +299,23 @@ __device__ __forceinline__ void read_nbhd(const double (*sp)[ZPN], int tx, int t
 
 // Contributions of the 4 elements of one layer around the node column:
 // bot = rows of the bottom node (local z = 0), top = rows of the top node.
-template <int SYM>
-__device__ __forceinline__ void layer_contract(const double *__restrict__ c_KE0, const KeVals &kv,
-                                               const double (&Ulo)[9][3], const double (&Uhi)[9][3],
-                                               const double (&E)[4], double (&bot)[3],
-                                               double (&top)[3]) {
-  bot[0] = bot[1] = bot[2] = 0.0;
-  top[0] = top[1] = top[2] = 0.0;
+template <int SYM, class T>
+__device__ __forceinline__ void layer_contract(const T *__restrict__ c_KE0, const KeValsT<T> &kv,
+                                               const T (&Ulo)[9][3], const T (&Uhi)[9][3],
+                                               const T (&E)[4], T (&bot)[3], T (&top)[3]) {
+  bot[0] = bot[1] = bot[2] = T(0);
+  top[0] = top[1] = top[2] = T(0);
 #pragma unroll
   for (int s = 0; s < 4; ++s) {
     const int ex = (s & 1) - 1, ey = (s >> 1) - 1;
     const int ab = lnode(-ex, -ey, 0), at = lnode(-ex, -ey, 1);
-    double tb0 = 0.0, tb1 = 0.0, tb2 = 0.0, tt0 = 0.0, tt1 = 0.0, tt2 = 0.0;
+    T tb0 = T(0), tb1 = T(0), tb2 = T(0), tt0 = T(0), tt1 = T(0), tt2 = T(0);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
       const int o = (ex + lnode_x(b) + 1) + 3 * (ey + lnode_y(b) + 1);
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
-        const double u = lnode_z(b) ? Uhi[o][j] : Ulo[o][j];
+        const T u = lnode_z(b) ? Uhi[o][j] : Ulo[o][j];
         tb0 += kcoef<SYM>(c_KE0, kv, (3 * ab + 0) * 24 + 3 * b + j) * u;
         tb1 += kcoef<SYM>(c_KE0, kv, (3 * ab + 1) * 24 + 3 * b + j) * u;
         tb2 += kcoef<SYM>(c_KE0, kv, (3 * ab + 2) * 24 + 3 * b + j) * u;
@@ -332,15 +337,17 @@ __device__ __forceinline__ void layer_contract(const double *__restrict__ c_KE0,
 // the epilogue.  The node's own value x is the centre of the bottom plane, so
 // y receives x' = x + omega dinv (b - K x) (out of place: neighbouring columns
 // and the next z-chunk still read x) and the block partials of x'.b and x'.q.
-template <bool PCG, int SYM, bool FUSE, bool JAC>
+// The epilogue computes in FP64 and y, b, dinv, q are FP64 whatever T is.
+template <bool PCG, int SYM, bool FUSE, bool JAC, class T = double, class TO = OutT<JAC, T>>
 __global__ __launch_bounds__(NT) void k_stencil64z(
-    Lattice L, int nchunks, const double *__restrict__ c_KE0, KeVals kv, const double *__restrict__ young,
-    const uint8_t *__restrict__ fmask, const double *__restrict__ src,
+    Lattice L, int nchunks, const T *__restrict__ c_KE0, KeValsT<T> kv, const T *__restrict__ young,
+    const uint8_t *__restrict__ fmask, const T *__restrict__ src,
     const double *__restrict__ dinv, const double *__restrict__ p_old, double *__restrict__ p_new,
-    double *__restrict__ y, PcgScalars *S, double *partials, const double *__restrict__ pu,
+    TO *__restrict__ y, PcgScalars *S, double *partials, const double *__restrict__ pu,
     unsigned Gu, int j, JacEpilogue je) {
-  __shared__ double sp[2][3][ZPN];
-  __shared__ double se[2][ZLN];
+  static_assert(!PCG || std::is_same<T, double>::value, "the PCG stencil is FP64");
+  __shared__ T sp[2][3][ZPN];
+  __shared__ T se[2][ZLN];
   __shared__ double sh[8];
   double beta;
   if (!pcg_prologue<PCG, FUSE>(S, pu, Gu, j, sh, beta)) return;
@@ -350,8 +357,8 @@ __global__ __launch_bounds__(NT) void k_stencil64z(
   const int nbx = (L.NX + ZX - 1) / ZX, nby = (L.NY + ZY - 1) / ZY, ncols = nbx * nby;
   const unsigned nt = gridDim.x, b = blockIdx.x;
   const unsigned q = nt / 8, rem = nt % 8, xcd = b % 8, slot = b / 8;
-  const unsigned T = (xcd < rem ? xcd * (q + 1) : rem * (q + 1) + (xcd - rem) * q) + slot;
-  const int col = (int)(T % ncols), zc = (int)(T / ncols);
+  const unsigned T_ = (xcd < rem ? xcd * (q + 1) : rem * (q + 1) + (xcd - rem) * q) + slot;
+  const int col = (int)(T_ % ncols), zc = (int)(T_ / ncols);
   const int x0 = (col % nbx) * ZX, y0 = (col / nbx) * ZY;
   // balanced z-chunks: chunk zc covers node planes [z0, z1)
   const int z0 = (int)((long long)zc * L.NZ / nchunks);
@@ -359,20 +366,20 @@ __global__ __launch_bounds__(NT) void k_stencil64z(
   const int tx = threadIdx.x % ZX, ty = threadIdx.x / ZX;
   const int gx = x0 + tx, gy = y0 + ty;
   const bool col_in = gx < L.NX && gy < L.NY;
-  PlaneFetch<PCG> F;
-  double Ulo[9][3], Uhi[9][3];
+  PlaneFetch<PCG, T> F;
+  T Ulo[9][3], Uhi[9][3];
   // Slot s holds node plane zb+1 and element layer zb of the step that reads
   // it.  Plane z0-1 goes to registers through slot 1.
-  fetch_plane<PCG>(L, x0, y0, z0 - 1, src, dinv, p_old, young, F);
-  commit_plane<PCG>(L, x0, y0, z0 - 1, false, beta, F, p_new, sp[1], se[1]);
+  fetch_plane<PCG, T>(L, x0, y0, z0 - 1, src, dinv, p_old, young, F);
+  commit_plane<PCG, T>(L, x0, y0, z0 - 1, false, beta, F, p_new, sp[1], se[1]);
   __syncthreads();
   read_nbhd(sp[1], tx, ty, Ulo);
   __syncthreads();
   // plane z0 and layer z0-1 -> slot 0 for step 0
-  fetch_plane<PCG>(L, x0, y0, z0, src, dinv, p_old, young, F);
-  commit_plane<PCG>(L, x0, y0, z0, z0 < z1, beta, F, p_new, sp[0], se[0]);
+  fetch_plane<PCG, T>(L, x0, y0, z0, src, dinv, p_old, young, F);
+  commit_plane<PCG, T>(L, x0, y0, z0, z0 < z1, beta, F, p_new, sp[0], se[0]);
   __syncthreads();
-  double acc[3] = {0.0, 0.0, 0.0};
+  T acc[3] = {T(0), T(0), T(0)};
   double pq = 0.0, zr = 0.0, zq = 0.0;
   const int nsteps = z1 - z0 + 1;  // element layers z0-1 .. z1-1
   for (int k = 0; k < nsteps; ++k) {
@@ -380,35 +387,35 @@ __global__ __launch_bounds__(NT) void k_stencil64z(
     const int cur = k & 1, nxt = cur ^ 1;
     const bool more = k + 1 < nsteps;
     // prefetch plane zb+2 with layer zb+1 for the next step
-    if (more) fetch_plane<PCG>(L, x0, y0, zb + 2, src, dinv, p_old, young, F);
+    if (more) fetch_plane<PCG, T>(L, x0, y0, zb + 2, src, dinv, p_old, young, F);
     read_nbhd(sp[cur], tx, ty, Uhi);
-    double E[4];
+    T E[4];
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       const int ex = (s & 1) - 1, ey = (s >> 1) - 1;
       E[s] = se[cur][(tx + 1 + ex) + ZLX * (ty + 1 + ey)];
     }
-    double bot[3], top[3];
+    T bot[3], top[3];
     layer_contract<SYM>(c_KE0, kv, Ulo, Uhi, E, bot, top);
     if (k >= 1 && col_in) {
       const long long n = gx + (long long)L.NX * (gy + (long long)L.NY * zb);
       const unsigned m = fmask[n];
-      double out[3];
+      T out[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) out[c] = ((m >> c) & 1u) ? acc[c] + bot[c] : 0.0;
+      for (int c = 0; c < 3; ++c) out[c] = ((m >> c) & 1u) ? acc[c] + bot[c] : T(0);
       if constexpr (JAC) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const long long ic = c * L.Np + n;
           const double bc = je.b[ic];
-          const double xn = Ulo[4][c] + je.omega * dinv[ic] * (bc - out[c]);
+          const double xn = (double)Ulo[4][c] + je.omega * dinv[ic] * (bc - (double)out[c]);
           y[ic] = xn;
           zr += xn * bc;
           zq += xn * je.q[ic];
         }
       } else {
 #pragma unroll
-        for (int c = 0; c < 3; ++c) y[c * L.Np + n] = out[c];
+        for (int c = 0; c < 3; ++c) y[c * L.Np + n] = (TO)out[c];
       }
       if constexpr (PCG) pq += Ulo[4][0] * out[0] + Ulo[4][1] * out[1] + Ulo[4][2] * out[2];
     }
@@ -418,7 +425,7 @@ __global__ __launch_bounds__(NT) void k_stencil64z(
     for (int o = 0; o < 9; ++o)
 #pragma unroll
       for (int c = 0; c < 3; ++c) Ulo[o][c] = Uhi[o][c];
-    if (more) commit_plane<PCG>(L, x0, y0, zb + 2, zb + 2 < z1, beta, F, p_new, sp[nxt], se[nxt]);
+    if (more) commit_plane<PCG, T>(L, x0, y0, zb + 2, zb + 2 < z1, beta, F, p_new, sp[nxt], se[nxt]);
     __syncthreads();
   }
   if constexpr (PCG) {
@@ -550,9 +557,11 @@ __global__ __launch_bounds__(NT) void k_dot(long long n, const double *__restric
 
 __global__ void k_young(const double *__restrict__ x, double *__restrict__ young, long long n,
                         double y1, double y2, double p) {
+  // no contraction: the same two roundings as the host's SIMP law
+#pragma clang fp contract(off)
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double xp = pow(x[i], p);
+  const double xp = simp_pow(x[i], p);
   young[i] = (1.0 - xp) * y1 + xp * y2;
 }
 
@@ -618,7 +627,7 @@ __global__ __launch_bounds__(NT) void k_sensitivity64(Lattice L, const double *_
     s += ue[r] * t;
   }
   const double x = dens[e];
-  const double xp = pow(x, p);
+  const double xp = simp_pow(x, p);
   const double yo = (1.0 - xp) * y1 + xp * y2;
   const double dyo = p * pow(x, p - 1.0) * (y2 - y1);
   dfds[e] = -(dyo / yo) * (yo * s);
@@ -657,7 +666,7 @@ __global__ __launch_bounds__(NT) void k_mises64(Lattice L, const double *__restr
   const double eps[6] = {G[0][0], G[1][1], G[2][2], G[0][1] + G[1][0], G[1][2] + G[2][1],
                          G[2][0] + G[0][2]};
   const double x = dens[e];
-  const double xp = pow(x, p);
+  const double xp = simp_pow(x, p);
   const double E = (1.0 - xp) * y1 + xp * y2;
   const double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
   const double mu = E / (2.0 * (1.0 + nu));
@@ -783,6 +792,7 @@ KeCoef make_ke_coef_class(const double *ke, const double *ke_dev, int cls) {
   };
   if (cls == 2 && fits(KSYM_CUBE, KSYM_CUBE_REP, KSYM_CUBE_N)) k.sym = 2;
   else if (cls == 1 && fits(KSYM_BOX, KSYM_BOX_REP, KSYM_BOX_N)) k.sym = 1;
+  for (int i = 0; i < 48; ++i) k.vals32.v[i] = (float)k.vals.v[i];
   return k;
 }
 
@@ -866,6 +876,41 @@ unsigned launch_apply_k64_jacobi(const Lattice &L, const KeCoef &ke, const doubl
       throw Error(kInvalidArgument, "unknown KE0 symmetry class");
   }
   return nb;
+}
+
+template <bool JAC>
+static unsigned launch_k32(const Lattice &L, const KeCoef &ke, const float *young32,
+                           const uint8_t *freemask, const float *x, double *y,
+                           const double *dinv, double *partials, JacEpilogue je, hipStream_t s) {
+  if (ke.sym == 0 && !ke.ke32) throw Error(kInternal, "FP32 K.u without an FP32 KE0");
+  const int def = stencil_nchunks(L);
+  const int nch = ke.nchunks > 0 ? std::min(ke.nchunks, def) : def;
+  const dim3 g = zgrid(L, nch);
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, g, dim3(NT), 0, s, L, nch, ke.ke32, ke.vals32, young32, freemask, x,
+                       dinv, nullptr, nullptr, y, nullptr, partials, nullptr, 0u, 0, je);
+  };
+  switch (ke.sym) {
+    case 0: go(k_stencil64z<false, 0, false, JAC, float, double>); break;
+    case 1: go(k_stencil64z<false, 1, false, JAC, float, double>); break;
+    case 2: go(k_stencil64z<false, 2, false, JAC, float, double>); break;
+    default: throw Error(kInvalidArgument, "unknown KE0 symmetry class");
+  }
+  HIPCHK(hipGetLastError());
+  return g.x * g.y * g.z;
+}
+
+void launch_apply_k32(const Lattice &L, const KeCoef &ke, const float *young32,
+                      const uint8_t *freemask, const float *x, double *y, hipStream_t s) {
+  launch_k32<false>(L, ke, young32, freemask, x, y, nullptr, nullptr, JacEpilogue{}, s);
+}
+
+unsigned launch_apply_k32_jacobi(const Lattice &L, const KeCoef &ke, const float *young32,
+                                 const uint8_t *freemask, const float *x, double *xout,
+                                 const double *b, const double *dinv, double omega,
+                                 const double *q, double *partials, hipStream_t s) {
+  return launch_k32<true>(L, ke, young32, freemask, x, xout, dinv, partials,
+                          JacEpilogue{b, q, omega}, s);
 }
 
 void launch_apply_k64(const Lattice &L, const KeCoef &ke, const double *young,

@@ -271,10 +271,11 @@ __global__ void k_fpcg_check(const double *__restrict__ partials, unsigned G, Pc
 // partials in the same order, so all reach the same alpha = rho / p.q and the
 // same breakdown decision; x += alpha p; r -= alpha q; r.r partials; with z0:
 // z0 = omega dinv r as well (the first sweep of a V-cycle on r).
+template <class TZ>
 __global__ __launch_bounds__(NT) void k_fpcg_update_alpha(
     long long n, double *__restrict__ x, double *__restrict__ r, const double *__restrict__ p,
     const double *__restrict__ q, PcgScalars *S, const double *__restrict__ part_st, unsigned Gs,
-    double *part_up, const double *__restrict__ dinv, double omega, double *__restrict__ z0) {
+    double *part_up, const double *__restrict__ dinv, double omega, TZ *__restrict__ z0) {
   __shared__ double sh[8];
   if (S->done) return;
   double tot[1];
@@ -298,7 +299,7 @@ __global__ __launch_bounds__(NT) void k_fpcg_update_alpha(
     const double ri = r[i] - alpha * q[i];
     r[i] = ri;
     rr += ri * ri;
-    if (z0) z0[i] = omega * dinv[i] * ri;
+    if (z0) z0[i] = (TZ)(omega * dinv[i] * ri);
   }
   double v[1] = {rr};
   store_partials<NT, 1>(v, part_up, sh);
@@ -445,8 +446,18 @@ unsigned launch_fpcg_update_alpha(long long n, double *x, double *r, const doubl
                                   const double *q, PcgScalars *S, const double *part_st,
                                   unsigned Gs, double *part_up, const double *dinv, double omega,
                                   double *z0, hipStream_t s) {
-  hipLaunchKernelGGL(k_fpcg_update_alpha, dim3(VEC_BLOCKS), dim3(NT), 0, s, n, x, r, p, q, S,
-                     part_st, Gs, part_up, dinv, omega, z0);
+  hipLaunchKernelGGL(k_fpcg_update_alpha<double>, dim3(VEC_BLOCKS), dim3(NT), 0, s, n, x, r, p, q,
+                     S, part_st, Gs, part_up, dinv, omega, z0);
+  HIPCHK(hipGetLastError());
+  return VEC_BLOCKS;
+}
+
+unsigned launch_fpcg_update_alpha(long long n, double *x, double *r, const double *p,
+                                  const double *q, PcgScalars *S, const double *part_st,
+                                  unsigned Gs, double *part_up, const double *dinv, double omega,
+                                  float *z0, hipStream_t s) {
+  hipLaunchKernelGGL(k_fpcg_update_alpha<float>, dim3(VEC_BLOCKS), dim3(NT), 0, s, n, x, r, p, q,
+                     S, part_st, Gs, part_up, dinv, omega, z0);
   HIPCHK(hipGetLastError());
   return VEC_BLOCKS;
 }

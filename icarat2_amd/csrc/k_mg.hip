@@ -159,11 +159,11 @@ __global__ __launch_bounds__(256) void k_mg_restrict(Lattice Lf, Lattice Lc,
 }
 
 // xf += P xc (ADD) or xf = P xc (trilinear), masked by the fine free DOFs
-template <bool ADD>
+template <bool ADD, class TF = double>
 __global__ __launch_bounds__(256) void k_mg_prolong_add(Lattice Lf, Lattice Lc,
                                                         const double *__restrict__ xc,
                                                         const uint8_t *__restrict__ fmask_f,
-                                                        double *__restrict__ xf) {
+                                                        TF *__restrict__ xf) {
   __shared__ double sc[3][PCN];
   int bx, by, bz;
   xcd_tile((Lf.NX + PTX - 1) / PTX, (Lf.NY + PTY - 1) / PTY, bx, by, bz);
@@ -181,7 +181,7 @@ __global__ __launch_bounds__(256) void k_mg_prolong_add(Lattice Lf, Lattice Lc,
     m = fmask_f[n];
     if constexpr (ADD) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) xo[c] = xf[c * Lf.Np + n];
+      for (int c = 0; c < 3; ++c) xo[c] = (double)xf[c * Lf.Np + n];
     }
   }
   for (int t = threadIdx.x; t < PCN; t += 256) {
@@ -217,7 +217,14 @@ __global__ __launch_bounds__(256) void k_mg_prolong_add(Lattice Lf, Lattice Lc,
       }
 #pragma unroll
   for (int c = 0; c < 3; ++c)
-    if ((m >> c) & 1u) xf[c * Lf.Np + n] = xo[c] + s[c];
+    if ((m >> c) & 1u) xf[c * Lf.Np + n] = (TF)(xo[c] + s[c]);
+}
+
+// x = omega dinv b in FP32 (the first sweep from zero of an FP32 level)
+__global__ void k_mg_jacobi0_f32(long long n, double omega, const double *__restrict__ dinv,
+                                 const double *__restrict__ b, float *__restrict__ x) {
+  const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (i < n) x[i] = (float)(omega * dinv[i] * b[i]);
 }
 
 // coarse element from its 8 children: arithmetic mean (MODE 0), geometric mean
@@ -509,6 +516,23 @@ void direct_dispatch(int mode, const Lattice &L, const KeCoef &kc, const double 
 }
 
 }  // namespace
+
+void launch_mg_prolong_add_f32(const Lattice &Lf, const Lattice &Lc, const double *xc,
+                               const uint8_t *fmask_f, float *xf, hipStream_t s) {
+  if (Lf.NX != 2 * Lc.NX - 1 || Lf.NY != 2 * Lc.NY - 1 || Lf.NZ != 2 * Lc.NZ - 1)
+    throw Error(kInvalidArgument, "prolongation: lattices are not 2x apart");
+  const dim3 g((unsigned)(((Lf.NX + PTX - 1) / PTX) * ((Lf.NY + PTY - 1) / PTY) *
+                          ((Lf.NZ + PTZ - 1) / PTZ)));
+  hipLaunchKernelGGL((k_mg_prolong_add<true, float>), g, dim3(256), 0, s, Lf, Lc, xc, fmask_f, xf);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_mg_jacobi0_f32(long long n, double omega, const double *dinv, const double *b,
+                           float *x, hipStream_t s) {
+  hipLaunchKernelGGL(k_mg_jacobi0_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n,
+                     omega, dinv, b, x);
+  HIPCHK(hipGetLastError());
+}
 
 void launch_mg_restrict(const Lattice &Lf, const Lattice &Lc, const double *rf, const double *yf,
                         const uint8_t *fmask_c, double *bc, const double *dinv_c, double omega,

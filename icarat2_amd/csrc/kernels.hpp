@@ -15,15 +15,20 @@ namespace icarat_hip {
 // orbit values v (kernel arguments, resident in scalar registers) through the
 // compile-time orbit tables of ke_sym_table.hpp instead of loading KE0:
 // sym 1 = box symmetry (48 values), sym 2 = cube symmetry (12 values).
-struct KeVals {
-  double v[48];
+template <class T>
+struct KeValsT {
+  T v[48];
 };
+using KeVals = KeValsT<double>;
 struct KeCoef {
   const double *ke = nullptr;  // device KE0
   int sym = 0;
   int kernel = 0;              // 0: z-marching column kernel, 1: 3-D tile kernel
   int nchunks = 0;             // z-marching: z-chunks per column (0: default, capped by it)
   KeVals vals{};
+  // the same in FP32 for the FP32 z-marching K.u (multigrid fine level)
+  const float *ke32 = nullptr; // device KE0 in FP32
+  KeValsT<float> vals32{};
 };
 // Orbit values of a host KE0 (sym = 2 if the cube table reproduces it to
 // 1e-13 of max |KE0|, else 1 for the box table, else 0).
@@ -45,6 +50,17 @@ int stencil_nchunks(const Lattice &L);
 // halo-tile kernel.
 void launch_apply_k64(const Lattice &L, const KeCoef &ke, const double *young,
                       const uint8_t *freemask, const double *u, double *y, hipStream_t s);
+// FP32 z-marching K.u (FP32 node values, young32 and KE0 in FP32, FP32
+// accumulation): y = K x masked to free DOFs, stored in FP64; the _jacobi form is the fused
+// smoother sweep of launch_apply_k64_jacobi with an FP32 x and the FP64
+// epilogue (xout, b, dinv, q and the partials in FP64).  Either ignores
+// ke.kernel (always the z-marching shape).
+void launch_apply_k32(const Lattice &L, const KeCoef &ke, const float *young32,
+                      const uint8_t *freemask, const float *x, double *y, hipStream_t s);
+unsigned launch_apply_k32_jacobi(const Lattice &L, const KeCoef &ke, const float *young32,
+                                 const uint8_t *freemask, const float *x, double *xout,
+                                 const double *b, const double *dinv, double omega,
+                                 const double *q, double *partials, hipStream_t s);
 // Arguments of the fused smoother epilogue of the z-marching K.u.
 struct JacEpilogue {
   const double *b = nullptr;

@@ -42,6 +42,11 @@ unsigned launch_fpcg_update_alpha(long long n, double *x, double *r, const doubl
                                   const double *q, PcgScalars *S, const double *part_st,
                                   unsigned Gs, double *part_up, const double *dinv, double omega,
                                   double *z0, hipStream_t s);
+// the same with z0 in FP32 (the FP32 fine level of the multigrid V-cycle)
+unsigned launch_fpcg_update_alpha(long long n, double *x, double *r, const double *p,
+                                  const double *q, PcgScalars *S, const double *part_st,
+                                  unsigned Gs, double *part_up, const double *dinv, double omega,
+                                  float *z0, hipStream_t s);
 // z.r and z.q partials; returns their count
 unsigned launch_zdot_partials(long long n, const double *z, const double *r, const double *q,
                               double *part_z, hipStream_t s);

@@ -18,6 +18,13 @@ void launch_mg_restrict(const Lattice &Lf, const Lattice &Lc, const double *rf, 
 // free DOFs (constrained DOFs keep their value)
 void launch_mg_prolong_add(const Lattice &Lf, const Lattice &Lc, const double *xc,
                            const uint8_t *fmask_f, double *xf, hipStream_t s, bool add = true);
+// FP32 fine level of the multigrid V-cycle: the prolongation added into an
+// FP32 xf (accumulated in FP64), and the first sweep from zero x = omega dinv b
+// written in FP32
+void launch_mg_prolong_add_f32(const Lattice &Lf, const Lattice &Lc, const double *xc,
+                               const uint8_t *fmask_f, float *xf, hipStream_t s);
+void launch_mg_jacobi0_f32(long long n, double omega, const double *dinv, const double *b,
+                           float *x, hipStream_t s);
 // coarse element values from the 8 children: mode 0 arithmetic mean, 1
 // geometric mean, 3 quadratic mean, 4 maximum
 void launch_mg_young(const Lattice &Lf, const Lattice &Lc, const double *yf, double *yc,

@@ -110,11 +110,15 @@ Multigrid::Multigrid(Context &fine, const icarat_mg_opts &o) : f_(fine), o_(o) {
       HIPCHK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
       HIPCHK(hipEventCreateWithFlags(&ev_side_, hipEventDisableTiming));
       HIPCHK(hipEventCreateWithFlags(&ev_fact_, hipEventDisableTiming));
-      info_.alloc(1);
-      HIPCHK(hipHostMalloc((void **)&h_info_, sizeof(int), hipHostMallocDefault));
-      *h_info_ = 0;
-      if (rocblas_create_handle(&blas_) != rocblas_status_success)
-        throw Error(kInternal, "rocblas_create_handle failed");
+      for (int k = 0; k < 2; ++k) {
+        info_[k].alloc(1);
+        HIPCHK(hipHostMalloc((void **)&h_info_[k], sizeof(int), hipHostMallocDefault));
+        *h_info_[k] = 0;
+        if (rocblas_create_handle(&blas_[k]) != rocblas_status_success)
+          throw Error(kInternal, "rocblas_create_handle failed");
+        if (rocblas_set_stream(blas_[k], k == 0 ? f_.stream() : side_) != rocblas_status_success)
+          throw Error(kInternal, "rocblas_set_stream failed");
+      }
     }
   }
   z_.alloc(f_.n3_);
@@ -136,11 +140,13 @@ Multigrid::Multigrid(Context &fine, const icarat_mg_opts &o) : f_(fine), o_(o) {
 Multigrid::~Multigrid() {
   if (side_) (void)hipStreamSynchronize(side_);
   drop_graphs();
-  if (h_info_) (void)hipHostFree(h_info_);
+  for (int k = 0; k < 2; ++k) {
+    if (h_info_[k]) (void)hipHostFree(h_info_[k]);
+    if (blas_[k]) (void)rocblas_destroy_handle(blas_[k]);
+  }
   if (ev_side_) (void)hipEventDestroy(ev_side_);
   if (ev_fact_) (void)hipEventDestroy(ev_fact_);
   if (side_) (void)hipStreamDestroy(side_);
-  if (blas_) (void)rocblas_destroy_handle(blas_);
   for (hipEvent_t e : ev_)
     if (e) (void)hipEventDestroy(e);
 }
@@ -180,7 +186,7 @@ void Multigrid::refresh() {
   }
   if (dense_) {
     if (!have_inverse_) {
-      factor_checked(ainv_, s);
+      factor_checked(ainv_);
       have_inverse_ = true;
       swap_pending_ = false;
     } else {
@@ -188,7 +194,7 @@ void Multigrid::refresh() {
       // previous density's inverse; swapped in at a fixed batch (before_batch)
       HIPCHK(hipEventRecord(ev_side_, s));
       HIPCHK(hipStreamWaitEvent(side_, ev_side_, 0));
-      factor_into(anext_, side_, 0.0);
+      factor_into(anext_, 1, 0.0);
       HIPCHK(hipEventRecord(ev_fact_, side_));
       swap_pending_ = true;
     }
@@ -197,11 +203,13 @@ void Multigrid::refresh() {
 
 // A = K of the coarsest grid (SPD on the free DOFs; constrained DOFs carry
 // identity rows), plus shift * diag(A) on the free DOFs when shift > 0, then
-// dst = A^-1 by Cholesky, all on stream s.  The factorization's status goes to
-// the pinned h_info_ (read by factor_checked and swap_inverse before the
-// inverse is used).
-void Multigrid::factor_into(double *dst, hipStream_t s, double shift) {
+// dst = A^-1 by Cholesky, on the fine context's stream (side = 0) or the side
+// stream (side = 1), each with its own rocBLAS handle and status buffers.  The
+// factorization's status goes to the pinned h_info_[side] (read by
+// factor_checked and swap_inverse before the inverse is used).
+void Multigrid::factor_into(double *dst, int side, double shift) {
   Context &cc = level(levels() - 1);
+  hipStream_t s = side ? side_ : f_.stream();
   const int nd = (int)(3 * cc.lattice().numnp);
   launch_mg_dense_assemble(cc.lattice(), cc.d_ke, cc.d_young, cc.d_free, dst, s,
                            galerkin_ ? kmat_[levels() - 1].p : nullptr);
@@ -210,22 +218,29 @@ void Multigrid::factor_into(double *dst, hipStream_t s, double shift) {
     if (st != rocblas_status_success)
       throw Error(kInternal, std::string(what) + " failed: " + rocblas_status_to_string(st));
   };
-  chk(rocblas_set_stream(blas_, s), "rocblas_set_stream");
-  chk(rocsolver_dpotrf(blas_, rocblas_fill_lower, nd, dst, nd, info_.p), "rocsolver_dpotrf");
-  HIPCHK(hipMemcpyAsync(h_info_, info_.p, sizeof(int), hipMemcpyDeviceToHost, s));
-  chk(rocsolver_dpotri(blas_, rocblas_fill_lower, nd, dst, nd, info_.p), "rocsolver_dpotri");
+  rocblas_handle hb = blas_[side];
+  int *info = info_[side].p;
+  chk(rocsolver_dpotrf(hb, rocblas_fill_lower, nd, dst, nd, info), "rocsolver_dpotrf");
+  HIPCHK(hipMemcpyAsync(h_info_[side], info, sizeof(int), hipMemcpyDeviceToHost, s));
+  chk(rocsolver_dpotri(hb, rocblas_fill_lower, nd, dst, nd, info), "rocsolver_dpotri");
   launch_mg_dense_symmetrize(nd, dst, s);
+  ++n_fact_;
 }
 
-// Synchronous factorization into dst on s, checked before the inverse is used;
-// a Cholesky breakdown (non-positive pivot in floating point) is retried once
-// with the diagonal of the free DOFs raised by a relative 1e-10.
-void Multigrid::factor_checked(double *dst, hipStream_t s) {
-  factor_into(dst, s, 0.0);
-  HIPCHK(hipStreamSynchronize(s));
-  if (*h_info_ == 0) return;
-  factor_into(dst, s, 1e-10);
-  HIPCHK(hipStreamSynchronize(s));
+// Synchronous factorization into dst on the fine context's stream, checked
+// before the inverse is used.  A Cholesky breakdown (a non-positive pivot in
+// floating point) is redone once with the diagonal of the free DOFs raised by
+// a relative 1e-10 and counted in shifted().  The coarsest operator of a
+// near-void field can be that ill-conditioned: its smallest eigenvalue can sit
+// below the n eps ||A|| backward error of the factorization (see the CPU test
+// of the 12x4x4 coarsest matrix).
+void Multigrid::factor_checked(double *dst) {
+  factor_into(dst, 0, 0.0);
+  HIPCHK(hipStreamSynchronize(f_.stream()));
+  if (*h_info_[0] == 0) return;
+  ++n_shifted_;
+  factor_into(dst, 0, 1e-10);
+  HIPCHK(hipStreamSynchronize(f_.stream()));
   check_factor();
 }
 
@@ -234,8 +249,13 @@ void Multigrid::swap_inverse(hipStream_t s) {
   swap_pending_ = false;
   // the side stream's status is read before its inverse replaces the current one
   HIPCHK(hipEventSynchronize(ev_fact_));
-  if (*h_info_ != 0) {
-    factor_checked(ainv_, s);
+  if (*h_info_[1] != 0) {
+    HIPCHK(hipStreamWaitEvent(f_.stream(), ev_fact_, 0));
+    factor_checked(ainv_);
+    if (s != f_.stream()) {
+      HIPCHK(hipEventRecord(ev_side_, f_.stream()));
+      HIPCHK(hipStreamWaitEvent(s, ev_side_, 0));
+    }
     return;
   }
   HIPCHK(hipStreamWaitEvent(s, ev_fact_, 0));
@@ -279,6 +299,25 @@ void Multigrid::coarse_op(int l, int mode, const double *x, const double *b, dou
                    mode == MG_JACOBI ? c.d_dinv.p : nullptr, w, out, s);
 }
 
+bool Multigrid::fine_fp32_capable() const {
+  return levels() > 1 && !direct_[0] && o_.nu_pre == 1 && o_.nu_post == 1;
+}
+
+void Multigrid::set_fine_fp32(bool on) {
+  if (on == fp32_) return;
+  if (on) {
+    if (o_.nu_pre != 1 || o_.nu_post != 1)
+      throw Error(kInvalidArgument, "the FP32 fine level needs nu_pre = nu_post = 1");
+    if (!z32_.p) {
+      z32_.alloc(f_.n3_);
+      HIPCHK(hipMemsetAsync(z32_.p, 0, z32_.bytes(), f_.stream()));
+      f_.sync();
+    }
+  }
+  fp32_ = on;
+  drop_graphs();
+}
+
 double *Multigrid::cycle(int l, const double *b, bool emit, bool x0_ready, ZDot *zd) {
   hipStream_t s = f_.stream();
   Context &c = level(l);
@@ -288,6 +327,32 @@ double *Multigrid::cycle(int l, const double *b, bool emit, bool x0_ready, ZDot 
   double *y = l == 0 ? y0_.p : c.d_y.p;
   double *r = l == 0 ? r0_.p : c.d_r.p;
   const double w = omega_[l];
+  if (l == 0 && fine_fp32()) {
+    // FP32 fine level: x0 = omega D^-1 b (written by the update when
+    // x0_ready), the residual b - K x0 restricted with K x0 in FP32, the
+    // coarse correction added in FP32, and the last sweep with its FP64
+    // epilogue into the result buffer (with the z.r and z.q partials)
+    if (zres_ && zres_ != y0_.p) throw Error(kInternal, "FP32 fine level: unexpected result buffer");
+    Context &cc = level(1);
+    if (emit) {
+      if (!x0_ready) launch_mg_jacobi0_f32(n, w, c.d_dinv, b, z32_, s);
+      // K x0 in FP64 storage (r0_): the restriction then reads the same
+      // types as on the FP64 path
+      launch_apply_k32(L, c.kc_, c.d_young32, c.d_free, z32_, r0_, s);
+      const double *dinv_c = starts_swept(1) ? cc.d_dinv.p : nullptr;
+      launch_mg_restrict(L, cc.lattice(), b, r0_, cc.d_free, cc.d_b, dinv_c, omega_[1], cc.d_x, s);
+    }
+    const double *xc = cycle(1, cc.d_b, emit);
+    if (emit) {
+      launch_mg_prolong_add_f32(L, cc.lattice(), xc, c.d_free, z32_.p, s);
+      double *part = zd ? zd->part : c.d_partials.p;
+      const double *q = zd ? zd->q : c.d_q.p;
+      const unsigned cnt = launch_apply_k32_jacobi(L, c.kc_, c.d_young32, c.d_free, z32_, y0_, b,
+                                                   c.d_dinv, w, q, part, s);
+      if (zd) zd->count = cnt;
+    }
+    return y0_.p;
+  }
   if (l == levels() - 1 && dense_) {
     if (emit) launch_mg_dense_solve(L, ainv_, b, x, s);
     return x;
@@ -385,9 +450,13 @@ void Multigrid::enqueue_batch(int par) {
                                              p_new, c.d_q, c.d_pcg, c.d_partials, c.d_counter, s,
                                              false);
     // alpha, x and r, r.r partials, z0 = omega D^-1 r
-    const unsigned gu = launch_fpcg_update_alpha(c.n3_, c.d_x, c.d_r, p_new, c.d_q, c.d_pcg,
-                                                 c.d_partials, gs, c.d_partials2, c.d_dinv,
-                                                 omega_[0], x0_fused ? z_.p : nullptr, s);
+    const unsigned gu =
+        fine_fp32()
+            ? launch_fpcg_update_alpha(c.n3_, c.d_x, c.d_r, p_new, c.d_q, c.d_pcg, c.d_partials,
+                                       gs, c.d_partials2, c.d_dinv, omega_[0], z32_.p, s)
+            : launch_fpcg_update_alpha(c.n3_, c.d_x, c.d_r, p_new, c.d_q, c.d_pcg, c.d_partials,
+                                       gs, c.d_partials2, c.d_dinv, omega_[0],
+                                       x0_fused ? z_.p : nullptr, s);
     ZDot zd{c.d_q, c.d_partials, 0};
     cycle(0, c.d_r, true, x0_fused, &zd);
     if (!zd.count) zd.count = launch_zdot_partials(c.n3_, zres_, c.d_r, c.d_q, c.d_partials, s);
@@ -441,6 +510,9 @@ void Multigrid::start(double tol, int maxit, hipStream_t s_ext, bool warm) {
     HIPCHK(hipStreamWaitEvent(s, ev_[0], 0));
   }
   batches_ = 0;
+  best_ = INFINITY;
+  since_best_ = 0;
+  verifications_ = 0;
   if (refreshed_ != c.density_version_) {
     // a factorization still pending belongs to an older density
     swap_inverse(s);
@@ -472,10 +544,33 @@ void Multigrid::start(double tol, int maxit, hipStream_t s_ext, bool warm) {
   }
 }
 
+bool Multigrid::after_verify() {
+  PcgScalars *h = f_.h_pcg.p;
+  if (h->done) return true;  // the verified residual passes
+  hipStream_t s = f_.stream();
+  ++verifications_;
+  const double bnorm = std::sqrt(h->bnorm2);
+  const double tr = bnorm > 0.0 ? std::sqrt(h->rr) / bnorm : 0.0;
+  if (tr < best_) {
+    best_ = tr;
+    since_best_ = 0;
+    if (!xbest_.p) xbest_.alloc(f_.n3_);
+    HIPCHK(hipMemcpyAsync(xbest_.p, f_.d_x.p, f_.d_x.bytes(), hipMemcpyDeviceToDevice, s));
+  } else {
+    ++since_best_;
+  }
+  if (since_best_ < kStallVerifications && verifications_ <= kMaxVerifications) return false;
+  // stalled at the floor: the best verified iterate, reported as not converged
+  HIPCHK(hipMemcpyAsync(f_.d_x.p, xbest_.p, f_.d_x.bytes(), hipMemcpyDeviceToDevice, s));
+  h->done = 1;
+  h->status = 1;
+  return true;
+}
+
 void Multigrid::check_factor() const {
-  if (dense_ && *h_info_ != 0)
+  if (dense_ && *h_info_[0] != 0)
     throw Error(kNumerical, "coarsest multigrid matrix is not positive definite (Cholesky pivot " +
-                                std::to_string(*h_info_) + ")");
+                                std::to_string(*h_info_[0]) + ")");
 }
 
 void Multigrid::verify(hipStream_t s_ext) {
@@ -498,8 +593,10 @@ void Multigrid::verify(hipStream_t s_ext) {
 }
 
 void Multigrid::solve(const icarat_solve_opts &o, icarat_solve_stats &st, bool warm) {
-  if (o.precision != ICARAT_HIP_FP64)
-    throw Error(kInvalidArgument, "the multigrid preconditioner runs the FP64 contraction");
+  if (o.precision != ICARAT_HIP_FP64 && o.precision != ICARAT_HIP_FP32)
+    throw Error(kInvalidArgument,
+                "the multigrid preconditioner runs an FP64 or an FP32 fine level (no BF16)");
+  set_fine_fp32(o.precision == ICARAT_HIP_FP32);
   Context &c = f_;
   hipStream_t s = c.stream();
   start(o.tol, o.maxit, s, warm);
@@ -527,8 +624,8 @@ void Multigrid::solve(const icarat_solve_opts &o, icarat_solve_stats &st, bool w
     // verified stop
     verify(s);
     c.sync();
-    if (h->done) break;
-    if (++repl > 50) break;
+    ++repl;
+    if (after_verify()) break;
   }
   finish();
   st.fallback = 0;

@@ -61,6 +61,15 @@ class Multigrid {
   void launch_batch(int par);
   void build_graph();  // the two parity graphs (no-op if built or graphs are off)
   void verify(hipStream_t s);
+  // After a verification's scalars reached the host copy: true when the solve
+  // ends -- the verified residual passed, or it stalled.  Near the FP64 floor
+  // of a solution (|u| far above |f|, where rounding u itself costs a fraction
+  // of the tolerance) the verified residual stops improving and every restart
+  // adds rounding to x; the best verified iterate is kept, and after
+  // kStallVerifications verifications without a new best (or
+  // kMaxVerifications in all) it is put back in x and the host copy marked
+  // capped (status 1, reported as not converged).
+  bool after_verify();
   // throws if the last refresh's Cholesky factorization failed (call once the
   // fine context's stream has completed the factorization, e.g. after finish)
   void check_factor() const;
@@ -68,10 +77,23 @@ class Multigrid {
   // synchronised, the factorization's status checked
   void finish();
   void drop_graphs();
+  // FP32 fine level (solve precision ICARAT_HIP_FP32): the fine level's
+  // smoothing sweeps, its residual and the transfers to and from level 1 run
+  // on FP32 vectors with the FP32 K.u (young and KE0 in FP32, FP32
+  // accumulation); the sweep that produces the V-cycle's result computes and
+  // stores in FP64, and the coarse levels stay FP64.  Applies where the fine
+  // level uses the z-marching K.u; needs nu_pre = nu_post = 1.  Captured
+  // graphs are rebuilt when the mode changes.
+  void set_fine_fp32(bool on);
+  bool fine_fp32() const { return fp32_ && fine_fp32_capable(); }
   // whether the coarsest level is solved exactly (dense inverse)
   bool dense() const { return dense_; }
   // whether level l smooths with the fused one-thread-per-node kernel
   bool direct(int l) const { return direct_[l] != 0; }
+  // coarsest-level Cholesky factorizations so far, and those redone with the
+  // diagonal shift after a breakdown
+  long long factorizations() const { return n_fact_; }
+  long long shifted() const { return n_shifted_; }
 
  private:
   Context &f_;
@@ -79,6 +101,13 @@ class Multigrid {
   std::vector<std::unique_ptr<Context>> lv_;  // levels 1..L-1
   std::vector<double> omega_;                  // Jacobi damping per level
   DevBuf<double> z_, y0_, r0_;
+  DevBuf<double> xbest_;  // best verified iterate of a solve that stalls at its floor
+  static constexpr int kStallVerifications = 4, kMaxVerifications = 50;
+  double best_ = 0.0;
+  int since_best_ = 0, verifications_ = 0;
+  bool fp32_ = false;
+  DevBuf<float> z32_;  // FP32 fine level: the iterate
+  bool fine_fp32_capable() const;
   std::vector<char> direct_;
   // direct levels: the coarse correction and the first post-sweep in one
   // kernel (launch_mg_prolong_jacobi); ICARAT_HIP_MG_FUSE=0 keeps them apart
@@ -94,8 +123,12 @@ class Multigrid {
                  hipStream_t s);
   bool dense_ = false;
   DevBuf<double> ainv_;  // dense inverse of the coarsest K
-  DevBuf<int> info_;
-  int *h_info_ = nullptr;  // pinned copy of the last factorization's status
+  // Factorization status per stream (index 0: the fine context's stream, 1:
+  // the side stream), each with a pinned host copy, and one rocBLAS handle
+  // (with its workspace) per stream, so the two factorizations share nothing.
+  DevBuf<int> info_[2];
+  int *h_info_[2] = {nullptr, nullptr};
+  long long n_fact_ = 0, n_shifted_ = 0;
   // Overlapped refresh: a new density's coarsest inverse is factored into
   // anext_ on the side stream while the solve's first kSwapAfter batches run
   // with the previous inverse, then copied into ainv_ before the next batch.
@@ -107,11 +140,12 @@ class Multigrid {
   bool have_inverse_ = false;  // ainv_ holds the inverse of this or an earlier density
   bool swap_pending_ = false;
   int batches_ = 0;            // batches launched since the solve's start
-  void factor_into(double *dst, hipStream_t s, double shift);
-  void factor_checked(double *dst, hipStream_t s);
+  // side = 0: on the fine context's stream with handle 0, else on side_
+  void factor_into(double *dst, int side, double shift);
+  void factor_checked(double *dst);
   void swap_inverse(hipStream_t s);  // ainv_ = anext_ once factored (if pending)
   void before_batch(hipStream_t s);  // the swap at its fixed batch
-  ::_rocblas_handle *blas_ = nullptr;
+  ::_rocblas_handle *blas_[2] = {nullptr, nullptr};
   double *zres_ = nullptr;  // buffer that holds the fine V-cycle's result
   hipGraph_t graph_[2] = {};
   hipGraphExec_t exec_[2] = {};

@@ -20,44 +20,55 @@ inline int corner_sign(int a, int d) {
 }  // namespace
 
 double hexa8_stiffness(double hx, double hy, double hz, double nu, double ke[576]) {
-  const double lam = nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
-  const double mu = 1.0 / (2 * (1.0 + nu));
-  const double g = 1.0 / std::sqrt(3.0);
-  const double h[3] = {hx, hy, hz};
-  std::memset(ke, 0, sizeof(double) * 576);
-  double vol = 0.0;
+  // The 2x2x2 rule integrates the trilinear element exactly; evaluated in
+  // extended precision and rounded once, every entry is the FP64 rounding of
+  // the exact integral, so entries equal in exact arithmetic (the symmetry
+  // orbits the stencil kernels rely on) are bit-identical.
+  using R = long double;
+  const R lam = (R)nu / ((1 + (R)nu) * (1 - 2 * (R)nu));
+  const R mu = 1 / (2 * (1 + (R)nu));
+  const R g = 1 / std::sqrt((R)3);
+  const R h[3] = {hx, hy, hz};
+  R acc[576];
+  for (int i = 0; i < 576; ++i) acc[i] = 0;
+  R vol = 0;
   // Gauss points in the reference order (basis/gauss_integration.cpp:268-289)
   for (int ip = 0; ip < 8; ++ip) {
-    const double r[3] = {corner_sign(ip, 0) * g, corner_sign(ip, 1) * g, corner_sign(ip, 2) * g};
+    const R r[3] = {corner_sign(ip, 0) * g, corner_sign(ip, 1) * g, corner_sign(ip, 2) * g};
     // Jacobian of the map from the unit cube [-1,1]^3 to the element
-    double J[3] = {0.0, 0.0, 0.0};
-    double dNr[8][3];
+    R J[3] = {0, 0, 0};
+    R dNr[8][3];
     for (int a = 0; a < 8; ++a) {
-      const double s[3] = {(double)corner_sign(a, 0), (double)corner_sign(a, 1),
-                           (double)corner_sign(a, 2)};
-      dNr[a][0] = 0.125 * s[0] * (1.0 + s[1] * r[1]) * (1.0 + s[2] * r[2]);
-      dNr[a][1] = 0.125 * s[1] * (1.0 + s[2] * r[2]) * (1.0 + s[0] * r[0]);
-      dNr[a][2] = 0.125 * s[2] * (1.0 + s[0] * r[0]) * (1.0 + s[1] * r[1]);
+      const R s[3] = {(R)corner_sign(a, 0), (R)corner_sign(a, 1), (R)corner_sign(a, 2)};
+      dNr[a][0] = s[0] * (1 + s[1] * r[1]) * (1 + s[2] * r[2]) / 8;
+      dNr[a][1] = s[1] * (1 + s[2] * r[2]) * (1 + s[0] * r[0]) / 8;
+      dNr[a][2] = s[2] * (1 + s[0] * r[0]) * (1 + s[1] * r[1]) / 8;
       for (int d = 0; d < 3; ++d) J[d] += dNr[a][d] * (lnode_x(a) * (d == 0) + lnode_y(a) * (d == 1) + lnode_z(a) * (d == 2)) * h[d];
     }
-    const double detw = J[0] * J[1] * J[2] * 1.0;
-    double gr[8][3];
+    const R detw = J[0] * J[1] * J[2];
+    R gr[8][3];
     for (int a = 0; a < 8; ++a)
       for (int d = 0; d < 3; ++d) gr[a][d] = dNr[a][d] / J[d];
     // K_{ai,bj} += w detJ (lam g_ai g_bj + mu (g_aj g_bi + delta_ij g_a.g_b))
     for (int a = 0; a < 8; ++a)
       for (int b = 0; b < 8; ++b) {
-        const double gg = gr[a][0] * gr[b][0] + gr[a][1] * gr[b][1] + gr[a][2] * gr[b][2];
+        const R gg = gr[a][0] * gr[b][0] + gr[a][1] * gr[b][1] + gr[a][2] * gr[b][2];
         for (int i = 0; i < 3; ++i)
           for (int j = 0; j < 3; ++j) {
-            double v = lam * gr[a][i] * gr[b][j] + mu * gr[a][j] * gr[b][i];
+            R v = lam * gr[a][i] * gr[b][j] + mu * gr[a][j] * gr[b][i];
             if (i == j) v += mu * gg;
-            ke[(3 * a + i) * 24 + 3 * b + j] += v * detw;
+            acc[(3 * a + i) * 24 + 3 * b + j] += v * detw;
           }
       }
     vol += detw;
   }
-  return vol;
+  // an entry that vanishes exactly comes out at the rounding level of the
+  // extended sums (1e-19 of the largest); every other exact entry is a ratio
+  // of small integers times lam, mu and the element sizes, far above 1e-12
+  R amax = 0;
+  for (int i = 0; i < 576; ++i) amax = std::max(amax, std::fabs(acc[i]));
+  for (int i = 0; i < 576; ++i) ke[i] = std::fabs(acc[i]) < 1e-12L * amax ? 0.0 : (double)acc[i];
+  return (double)vol;
 }
 
 Problem::Problem(const GridSpec &g) : g_(g) {

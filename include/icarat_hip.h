@@ -431,6 +431,17 @@ int icarat_hip_mg_setup(icarat_hip_ctx *ctx, const icarat_mg_opts *o);
 /* levels of the hierarchy (1 = no coarsening possible); dims (may be NULL) gets
  * nx, ny, nz of every level, 3 * levels entries */
 int icarat_hip_mg_info(const icarat_hip_ctx *ctx, int32_t *levels, int32_t *dims);
+typedef struct icarat_mg_status {
+  int32_t levels;          /* levels of the hierarchy */
+  int32_t dense;           /* 1 if the coarsest level is solved exactly by its dense inverse */
+  int32_t direct_fine;     /* 1 if the fine level smooths with the one-thread-per-node kernel */
+  int32_t fine_fp32;       /* 1 if the last solve ran the FP32 fine level (precision FP32) */
+  int64_t factorizations;  /* Cholesky factorizations of the coarsest matrix so far */
+  int64_t shifted;         /* of which broke down and were redone with the diagonal of the
+                              free DOFs raised by a relative 1e-10 */
+} icarat_mg_status;
+/* State of the context's hierarchy (NOT_READY without one). */
+int icarat_hip_mg_status(const icarat_hip_ctx *ctx, icarat_mg_status *status);
 
 /* ---- measurement ------------------------------------------------------------ */
 /* No reference counterpart: a timing hook for benchmarks.  Launches one kernel
@@ -451,6 +462,10 @@ int icarat_hip_mg_info(const icarat_hip_ctx *ctx, int32_t *levels, int32_t *dims
  *             1152 flop/element
  *   kernel 9: one Helmholtz filter pass (icarat_hip_helm_setup): 4 numnp (NX+NY+NZ)
  *             flop of 1-D transforms (FP64 GEMMs), 16 B per element in and out
+ *   kernel 10: FP64 PCG stencil of the multigrid solve (p = z + beta p_old from the
+ *             V-cycle's z, no D^-1; q = K p, p.q), 1152 flop/element
+ *   kernel 11: FP64 vector update of the multigrid solve (alpha step reduced from the
+ *             stencil's partials, x, r update, r.r, z0 = omega D^-1 r)
  * Needs set_density (and filter_setup for kernel 2, helm_setup for kernel 9).  Overwrites the solver's
  * work vectors. */
 int icarat_hip_time_kernel(icarat_hip_ctx *ctx, int32_t kernel, int32_t reps, double *avg_ms,

@@ -76,9 +76,10 @@ struct Problem {
 const int kSign[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
                          {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
 
-void dNdr_h8(const double r[3], double d[3][8]) {
+template <class R>
+void dNdr_h8(const R r[3], R d[3][8]) {
   for (int a = 0; a < 8; ++a) {
-    double s0 = kSign[a][0], s1 = kSign[a][1], s2 = kSign[a][2];
+    R s0 = kSign[a][0], s1 = kSign[a][1], s2 = kSign[a][2];
     d[0][a] = 0.125 * s0 * (1.0 + s1 * r[1]) * (1.0 + s2 * r[2]);
     d[1][a] = 0.125 * s1 * (1.0 + s2 * r[2]) * (1.0 + s0 * r[0]);
     d[2][a] = 0.125 * s2 * (1.0 + s0 * r[0]) * (1.0 + s1 * r[1]);
@@ -87,10 +88,12 @@ void dNdr_h8(const double r[3], double d[3][8]) {
 
 // Isotropic 3-D elasticity matrix, Voigt [xx,yy,zz,xy,yz,zx] with engineering
 // shears (reference problem/elasticity.cpp:72-93).
-void De3(double E, double nu, double D[6][6]) {
-  std::memset(D, 0, sizeof(double) * 36);
-  double lam = E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
-  double mu = E / (2 * (1.0 + nu));
+template <class R>
+void De3(R E, R nu, R D[6][6]) {
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) D[i][j] = 0;
+  R lam = E * nu / ((1 + nu) * (1 - 2 * nu));
+  R mu = E / (2 * (1 + nu));
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) D[i][j] = lam;
   for (int i = 0; i < 3; ++i) D[i][i] = lam + 2 * mu;
@@ -101,23 +104,24 @@ void De3(double E, double nu, double D[6][6]) {
 // with engineering shears) at Gauss point ip of the 2x2x2 rule, points
 // (+-1/sqrt3)^3 in the listed order, weight 1 (reference basis/bmatrix.hpp:
 // 90-162, basis/gauss_integration.cpp:268-289).  Returns jac = det J * weight.
-double element_B(const double Xe[8][3], int ip, double B[6][24]) {
-  const double g = 1.0 / std::sqrt(3.0);
-  const double pts[8][3] = {{-g, -g, -g}, {g, -g, -g}, {g, g, -g}, {-g, g, -g},
-                            {-g, -g, g},  {g, -g, g},  {g, g, g},  {-g, g, g}};
-  double dr[3][8];
+template <class R>
+R element_B(const double Xe[8][3], int ip, R B[6][24]) {
+  const R g = 1 / std::sqrt((R)3);
+  const R pts[8][3] = {{-g, -g, -g}, {g, -g, -g}, {g, g, -g}, {-g, g, -g},
+                       {-g, -g, g},  {g, -g, g},  {g, g, g},  {-g, g, g}};
+  R dr[3][8];
   dNdr_h8(pts[ip], dr);
-  double J[3][3];
+  R J[3][3];
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) {
-      double s = 0.0;
-      for (int a = 0; a < 8; ++a) s += dr[i][a] * Xe[a][j];
+      R s = 0;
+      for (int a = 0; a < 8; ++a) s += dr[i][a] * (R)Xe[a][j];
       J[i][j] = s;
     }
-  double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+  R det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
                J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
                J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
-  double Ji[3][3];
+  R Ji[3][3];
   Ji[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
   Ji[0][1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) / det;
   Ji[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
@@ -127,11 +131,12 @@ double element_B(const double Xe[8][3], int ip, double B[6][24]) {
   Ji[2][0] = (J[1][0] * J[2][1] - J[1][1] * J[2][0]) / det;
   Ji[2][1] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) / det;
   Ji[2][2] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / det;
-  double dX[3][8];
+  R dX[3][8];
   for (int i = 0; i < 3; ++i)
     for (int a = 0; a < 8; ++a)
       dX[i][a] = Ji[i][0] * dr[0][a] + Ji[i][1] * dr[1][a] + Ji[i][2] * dr[2][a];
-  std::memset(B, 0, sizeof(double) * 6 * 24);
+  for (int i = 0; i < 6; ++i)
+    for (int c = 0; c < 24; ++c) B[i][c] = 0;
   for (int a = 0; a < 8; ++a) {
     B[0][3 * a] = dX[0][a];
     B[1][3 * a + 1] = dX[1][a];
@@ -143,40 +148,68 @@ double element_B(const double Xe[8][3], int ip, double B[6][24]) {
     B[5][3 * a] = dX[2][a];
     B[5][3 * a + 2] = dX[0][a];
   }
-  return det * 1.0;
+  return det;
 }
 
 // Ke = sum_ip B^T De B jac over the 2x2x2 Gauss rule (reference
 // problem/elasticity.cpp:211-255).  Returns the element volume (sum of jac).
+// The rule integrates the trilinear element exactly, so Ke is evaluated in
+// extended precision (long double) and rounded once: every entry is the FP64
+// rounding of the exact integral (equal entries of the exact matrix stay
+// bit-identical), which makes the operator the same, bit for bit, for any
+// implementation that rounds the exact element matrix.
 double element_ke(const double Xe[8][3], double E, double nu, double Ke[576]) {
-  double D[6][6];
-  De3(E, nu, D);
-  std::memset(Ke, 0, sizeof(double) * 576);
-  double vol = 0.0;
+  using R = long double;
+  R D[6][6];
+  De3<R>(E, nu, D);
+  R acc[576];
+  for (int i = 0; i < 576; ++i) acc[i] = 0;
+  R vol = 0;
   for (int ip = 0; ip < 8; ++ip) {
-    double B[6][24];
-    const double jac = element_B(Xe, ip, B);
-    double DB[6][24];
+    R B[6][24];
+    const R jac = element_B<R>(Xe, ip, B);
+    R DB[6][24];
     for (int i = 0; i < 6; ++i)
       for (int c = 0; c < 24; ++c) {
-        double s = 0.0;
+        R s = 0;
         for (int k = 0; k < 6; ++k) s += D[i][k] * B[k][c];
         DB[i][c] = s;
       }
     for (int r = 0; r < 24; ++r)
       for (int c = 0; c < 24; ++c) {
-        double s = 0.0;
+        R s = 0;
         for (int k = 0; k < 6; ++k) s += B[k][r] * DB[k][c];
-        Ke[r * 24 + c] += s * jac;
+        acc[r * 24 + c] += s * jac;
       }
     vol += jac;
   }
-  return vol;
+  // an entry that vanishes exactly comes out at the rounding level of the
+  // extended sums (1e-19 of the largest); every other exact entry is a ratio
+  // of small integers times lam, mu and the element sizes, far above 1e-12
+  R amax = 0;
+  for (int i = 0; i < 576; ++i) amax = std::max(amax, std::fabs(acc[i]));
+  for (int i = 0; i < 576; ++i) Ke[i] = std::fabs(acc[i]) < 1e-12L * amax ? 0.0 : (double)acc[i];
+  return (double)vol;
+}
+
+// x^p of the SIMP law: for p = 3 the correctly rounded cube (the two products
+// with their exact error terms), what a correctly rounded std::pow returns;
+// otherwise std::pow.
+inline double simp_pow(double x, double p) {
+  if (p == 3.0) {
+    const double s = x * x;
+    const double es = std::fma(x, x, -s);
+    const double t = s * x;
+    const double et = std::fma(s, x, -t);
+    return t + std::fma(es, x, et);
+  }
+  return std::pow(x, p);
 }
 
 // SIMP interpolation (reference problem/elasticity.hpp:13-17).
 inline double simp(double y1, double y2, double x, double p) {
-  return (1.0 - std::pow(x, p)) * y1 + std::pow(x, p) * y2;
+  const double xp = simp_pow(x, p);
+  return (1.0 - xp) * y1 + xp * y2;
 }
 
 void element_X(const Problem &P, i64 e, double Xe[8][3]) {
@@ -1257,6 +1290,77 @@ void orc_set_density(void *h, const double *xphys) {
   assemble_values(P);
 }
 
+// Densities without the assembled K (meshes whose CSR would not fit, e.g.
+// 256x128x128 with ~1e9 non-zeros): the element-by-element operators below
+// need only the SIMP young.
+void orc_set_density_mf(void *h, const double *xphys) {
+  Problem &P = *(Problem *)h;
+  P.dens.assign(xphys, xphys + P.nelx);
+  P.young.resize(P.nelx);
+  for (i64 e = 0; e < P.nelx; ++e) P.young[e] = simp(P.young1, P.young2, xphys[e], P.penalty);
+  P.rowptr.clear();
+  P.col.clear();
+  P.val.clear();
+  P.patternBuilt = false;
+}
+
+// K.u on numeq vectors gathered node by node: row (n, d) of the reference's
+// assembled K (misc/assembling.hpp:18-39) is sum over the <= 8 elements
+// around node n of young_e * KE0[rows of n] . u_e, u_e gathered with the
+// Dirichlet values at constrained DOFs (misc/assembling.hpp:295-358).
+// Parallel over nodes, each row summed in a fixed element order.
+void orc_apply_k_nodal(void *h, const double *u, const double *dirval_nodes, double *y) {
+  Problem &P = *(Problem *)h;
+  const int nx = P.nx, ny = P.ny, nz = P.nz;
+  const i64 NX = nx + 1, NY = ny + 1;
+#pragma omp parallel for schedule(static)
+  for (i64 n = 0; n < P.numnp; ++n) {
+    const int i = (int)(n % NX), j = (int)((n / NX) % NY), k = (int)(n / (NX * NY));
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int s = 0; s < 8; ++s) {
+      const int ei = i - 1 + (s & 1), ej = j - 1 + ((s >> 1) & 1), ek = k - 1 + ((s >> 2) & 1);
+      if (ei < 0 || ei >= nx || ej < 0 || ej >= ny || ek < 0 || ek >= nz) continue;
+      const i64 e = ei + (i64)nx * (ej + (i64)ny * ek);
+      int a = -1;
+      double ue[24];
+      for (int b = 0; b < 8; ++b) {
+        const i64 nb = P.conn[8 * e + b];
+        if (nb == n) a = b;
+        for (int d = 0; d < 3; ++d) {
+          const int q = P.dof[3 * nb + d];
+          ue[3 * b + d] = q < P.numeq ? u[q] : (dirval_nodes ? dirval_nodes[3 * nb + d] : 0.0);
+        }
+      }
+      const double E = P.young[e];
+      for (int d = 0; d < 3; ++d) {
+        double t = 0.0;
+        for (int c = 0; c < 24; ++c) t += E * P.KE0[(3 * a + d) * 24 + c] * ue[c];
+        acc[d] += t;
+      }
+    }
+    for (int d = 0; d < 3; ++d) {
+      const int q = P.dof[3 * n + d];
+      if (q < P.numeq) y[q] = acc[d];
+    }
+  }
+}
+
+// ||fext - K u|| / ||fext|| with the node-by-node K.u (the true residual of
+// analysis/linear.hpp:90-103 without an assembled matrix).
+double orc_true_residual_nodal(void *h, const double *sol) {
+  Problem &P = *(Problem *)h;
+  std::vector<double> dv(P.numnp * 3), y(P.numeq);
+  orc_dirichlet_values(h, dv.data());
+  orc_apply_k_nodal(h, sol, dv.data(), y.data());
+  double a = 0.0, b = 0.0;
+  for (int i = 0; i < P.numeq; ++i) {
+    const double r = P.fext[i] - y[i];
+    a += r * r;
+    b += P.fext[i] * P.fext[i];
+  }
+  return std::sqrt(a) / std::sqrt(b);
+}
+
 long long orc_nnz(void *h) { return ((Problem *)h)->rowptr.empty() ? 0 : ((Problem *)h)->rowptr.back(); }
 
 void orc_spmv(void *h, const double *x, double *y) { spmv(*(Problem *)h, x, y); }
@@ -1379,7 +1483,7 @@ void orc_sensitivity(void *h, const double *nodeval, double vnorm, double *dfds,
 #pragma omp parallel for schedule(static)
   for (i64 e = 0; e < P.nelx; ++e) {
     double dens = P.dens[e];
-    double young = (1.0 - std::pow(dens, p)) * y1 + std::pow(dens, p) * y2;
+    double young = simp(y1, y2, dens, p);
     double dyoung = p * std::pow(dens, p - 1.0) * (y2 - y1);
     double E = dyoung / young;
     double ue[24];

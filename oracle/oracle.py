@@ -56,6 +56,9 @@ def lib():
             "orc_set_density": (None, [vp, dp]), "orc_nnz": (C.c_longlong, [vp]),
             "orc_spmv": (None, [vp, dp, dp]), "orc_diag": (None, [vp, dp]),
             "orc_apply_k_elementwise": (None, [vp, dp, dp, dp]),
+            "orc_set_density_mf": (None, [vp, dp]),
+            "orc_apply_k_nodal": (None, [vp, dp, dp, dp]),
+            "orc_true_residual_nodal": (C.c_double, [vp, dp]),
             "orc_scaling_cg": (C.c_int, [vp, dp, dp, C.c_double, C.c_int, dp]),
             "orc_true_residual": (C.c_double, [vp, dp]),
             "orc_node_values": (None, [vp, dp, dp]),
@@ -227,6 +230,24 @@ class Problem:
     def set_density(self, x):
         self.x = np.ascontiguousarray(x, dtype=np.float64)
         self.L.orc_set_density(self.h, P(self.x))
+
+    def set_density_matrix_free(self, x):
+        """Densities without assembling K (for meshes whose CSR does not fit);
+        spmv / scaling_cg / diag need set_density instead."""
+        self.x = np.ascontiguousarray(x, dtype=np.float64)
+        self.L.orc_set_density_mf(self.h, P(self.x))
+
+    def apply_k_nodal(self, u, dirval=None):
+        """K.u row by row (node-centric gather over the <= 8 elements of each
+        node), in parallel; needs only the densities."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        y = np.zeros(self.numeq)
+        dv = None if dirval is None else np.ascontiguousarray(dirval, dtype=np.float64)
+        self.L.orc_apply_k_nodal(self.h, P(u), None if dv is None else P(dv), P(y))
+        return y
+
+    def true_residual_nodal(self, sol):
+        return self.L.orc_true_residual_nodal(self.h, P(np.ascontiguousarray(sol)))
 
     def nnz(self):
         return self.L.orc_nnz(self.h)

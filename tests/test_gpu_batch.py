@@ -163,3 +163,70 @@ def test_batch_budget_does_not_change_results(gpu, budget):
     for x1, x2, h1, h2 in zip(xa, xb, ha, hb):
         assert np.max(np.abs(x1 - x2)) < 1e-12
         assert np.allclose(h1, h2, rtol=1e-12, atol=0)
+
+
+def _oracle_problem(oracle, spec):
+    """The oracle's problem of one batched spec: the clamped face, then every
+    load case's face load (face_range and the unit tractions of workloads)."""
+    from icarat2_amd import workloads as W
+    dims = spec["dims"]
+    o = oracle.Problem(*dims)
+    o.add_dirichlet([1, 1, 1, 0, 0, 0], [0.0] * 6, W.face_range(spec["clamp"], dims))
+    o.make_dof()
+    return o
+
+
+@pytest.mark.slow
+def test_full_batched_workload(gpu, oracle):
+    """All 24 problems of the batched workload (seeds 100..123, 32^3 to
+    160x80x80, 1-4 face load cases each) in one batch, multigrid PCG, 2 design
+    iterations: every solve at the verified FP64 residual, four problems equal
+    to their standalone loops, and the smallest problem's first compliance
+    (summed over its cases) against the oracle's own solves."""
+    from icarat2_amd import workloads as W
+    specs = W.batch_workload()
+    assert len(specs) == 24
+    ctxs = [W.build_context(gpu, s) for s in specs]
+    x0s = [W.initial_design(s) for s in specs]
+    b = gpu.Batch(ctxs)
+    xs, hists, sts = b.simp_run(x0s, 2, tol=1e-8, maxit=100000, precond=gpu.PRECOND_MG)
+    for s, h, st in zip(specs, hists, sts):
+        assert np.all(h[:, 3] <= 1e-8), (s["seed"], h[:, 3])
+        assert st.max_true_relres <= 1e-8
+    print("batched: PCG iterations per problem", [int(st.pcg_iters) for st in sts])
+    # four problems alone: smallest, largest, most load cases, and the first
+    order = sorted(range(24), key=lambda i: np.prod(specs[i]["dims"]))
+    picks = [order[0], order[-1], max(range(24), key=lambda i: len(specs[i]["loads"]))]
+    for i in range(24):
+        if len(set(picks)) >= 4:
+            break
+        picks.append(i)
+    picks = sorted(set(picks))
+    assert len(picks) == 4
+    for i in picks:
+        c = W.build_context(gpu, specs[i])
+        x1, h1, s1 = c.simp_run(x0s[i], 2, tol=1e-8, maxit=100000, precond=gpu.PRECOND_MG)
+        print(f"problem {specs[i]['seed']} {specs[i]['dims']}: alone {s1.pcg_iters} "
+              f"batched {sts[i].pcg_iters} iterations")
+        assert s1.pcg_iters == sts[i].pcg_iters
+        assert np.max(np.abs(x1 - xs[i])) < 1e-12
+        assert np.allclose(h1, hists[i], rtol=1e-12, atol=0)
+    # the oracle's compliance of the smallest problem's first design
+    i = order[0]
+    s = specs[i]
+    c = ctxs[i]
+    xf = c.density_filter(x0s[i])
+    o = _oracle_problem(oracle, s)
+    o.set_density(xf)
+    comp = 0.0
+    for lc, (face, val) in enumerate(s["loads"]):
+        q = _oracle_problem(oracle, s)
+        q.add_neumann(val, W.face_range(face, s["dims"]))
+        q.set_density(xf)
+        if not np.any(q.fext()):
+            continue  # a face load entirely on clamped DOFs: u = 0, no compliance
+        u, it, _ = q.scaling_cg(tol=1e-12, maxit=200000)
+        assert q.true_residual(u) < 1e-10
+        comp += q.compliance(u)
+    print(f"problem {s['seed']}: first compliance batched {hists[i][0, 0]:.15g} oracle {comp:.15g}")
+    assert abs(hists[i][0, 0] - comp) < 1e-8 * abs(comp)

@@ -1,0 +1,82 @@
+"""The 256x128x128 mixed-precision configuration at full size (4,194,304
+elements, 12,780,288 free DOFs; Gaussian-field density seed 3, rmin 1.5): the
+K.u contraction of every precision against the oracle's node-by-node apply,
+the FP64 multigrid solve and the solve with the FP32 fine level judged by the
+oracle's own recomputation of the true residual (analysis/linear.hpp:93-103),
+and the FP64-oracle error of u, compliance and sensitivities per precision.
+
+The tolerance is 3e-8: this field leaves 47% of the filtered densities below
+0.01, |u| reaches 3.9e7 with |f| = 1, and rounding u to FP64 alone costs about
+1e-8 of relative residual (a random half-ulp perturbation adds 1.8e-8 in
+quadrature), so the verified residual of an FP64 iterate stalls between 1.5e-8
+and 4e-8 depending on where the iteration stands; 3e-8 is the tightest
+tolerance the FP64 multigrid solve reaches reliably.
+"""
+import numpy as np
+import pytest
+
+pytestmark = [pytest.mark.gpu, pytest.mark.slow]
+
+DIMS = (256, 128, 128)
+TOL = 3e-8
+
+
+@pytest.fixture(scope="module")
+def c3(gpu, oracle):
+    c = gpu.cantilever(*DIMS)
+    x0, _ = gpu.gaussian_field(*DIMS, 3)
+    c.filter_setup(1.5)
+    xf = c.density_filter(x0)
+    c.set_density(xf)
+    o = oracle.cantilever(*DIMS)
+    o.set_density_matrix_free(xf)
+    assert o.numeq == c.numeq == 12780288
+    return c, o
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b)) / np.max(np.abs(b))
+
+
+def test_c3_contraction_per_precision(gpu, c3):
+    c, o = c3
+    u = np.random.default_rng(30).standard_normal(c.numeq)
+    yo = o.apply_k_nodal(u)
+    for prec, bound in ((gpu.FP64, 1e-13), (gpu.FP32, 1e-5), (gpu.BF16, 3e-2)):
+        err = _rel(c.apply_K(u, prec), yo)
+        print(f"C3 K.u precision {prec}: max error / max |Ku| = {err:.3e}")
+        assert err < bound
+
+
+def test_c3_solves_against_oracle(gpu, c3):
+    c, o = c3
+    out = {}
+    for prec in (gpu.FP64, gpu.FP32):
+        u, st = c.solve(tol=TOL, maxit=40000, precision=prec, precond=gpu.PRECOND_MG)
+        r_or = o.true_residual_nodal(u)
+        cc, df, _ = c.compliance_sensitivity()
+        c_or = o.compliance(u)
+        df_or, _ = o.sensitivity(o.node_values(u))
+        print(f"C3 multigrid, fine level precision {prec}: {st.iters} iterations, "
+              f"{st.refine_steps} verifications, {st.seconds:.2f} s; true residual GPU "
+              f"{st.true_relres:.4e} oracle {r_or:.4e}")
+        assert st.status == 0 and st.true_relres <= TOL
+        assert r_or <= TOL
+        # the same u and KE0, but the SIMP young of the oracle (std::pow) and of
+        # the GPU (device pow) can differ in the last bit: on this field a
+        # relative 1e-16 change of K moves the residual by up to ~1e-9
+        assert abs(r_or - st.true_relres) < 0.15 * r_or
+        # compliance and sensitivities of this u, GPU vs oracle
+        assert abs(cc - c_or) < 1e-12 * abs(c_or)
+        assert _rel(df, df_or) < 1e-12
+        out[prec] = (u, c_or, df_or)
+    # FP64-oracle error per precision: the oracle's compliance and
+    # sensitivities of each solution against those of the FP64 solution
+    u64, c64, df64 = out[gpu.FP64]
+    for prec, (u, cp, dfp) in out.items():
+        eu = np.linalg.norm(u - u64) / np.linalg.norm(u64)
+        ec = abs(cp - c64) / abs(c64)
+        ed = _rel(dfp, df64)
+        print(f"C3 precision {prec}: |u - u64|/|u64| {eu:.3e}, |c - c64|/c64 {ec:.3e}, "
+              f"max|dc - dc64|/max|dc64| {ed:.3e}")
+        assert eu < 1e-7 and ec < 1e-8 and ed < 1e-6

@@ -56,6 +56,24 @@ def test_mg_solve_agrees_with_jacobi(gpu, dims):
     assert sm.status == 0 and sm.true_relres <= 1e-9
     assert np.max(np.abs(um - uj)) < 1e-6 * np.max(np.abs(uj))
     assert sm.iters < sj.iters
+    ms = c.mg_status()
+    assert ms.levels == len(c.mg_info()) and ms.factorizations >= 1 and ms.shifted == 0
+    if dims == (40, 20, 20):
+        # the blocked 10x5x5 coarsest grid (1188 DOFs > 1024) is solved densely
+        assert c.mg_info() == [(40, 20, 20), (20, 10, 10), (10, 5, 5)]
+        assert ms.dense == 1
+        # a new density: its inverse is factored on the side stream and swapped
+        # in after the first batches (restart), and the solve still matches
+        x2 = np.clip(x * (1.0 + 0.5 * np.cos(np.arange(c.nelx) * 0.01)), 1e-3, 1.0)
+        c.set_density(x2)
+        uj2, sj2 = c.solve(tol=1e-9)
+        um2, sm2 = c.solve(tol=1e-9, precond=gpu.PRECOND_MG)
+        ms2 = c.mg_status()
+        print(f"second density: Jacobi {sj2.iters}, multigrid {sm2.iters} iterations, "
+              f"factorizations {ms2.factorizations}")
+        assert sm2.status == 0 and sm2.true_relres <= 1e-9
+        assert ms2.factorizations == ms.factorizations + 1 and ms2.shifted == 0
+        assert np.max(np.abs(um2 - uj2)) < 1e-6 * np.max(np.abs(uj2))
 
 
 @pytest.mark.parametrize("dims", [(48, 16, 16), (60, 20, 20)])

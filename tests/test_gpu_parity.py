@@ -293,6 +293,12 @@ def test_config2_one_solve_sens_filter(gpu, oracle):
     assert abs(a - b) < 1e-11 * abs(a)
     ug, st = c.solve(tol=1e-8)
     assert st.status == 0 and st.true_relres < 1e-8
+    # the oracle recomputes the true residual of the GPU solution with its
+    # assembled K (analysis/linear.hpp:93-103)
+    r_or = o.true_residual(ug)
+    print(f"C2 true residual: GPU {st.true_relres:.4e}, oracle {r_or:.4e}")
+    assert r_or < 1e-8
+    assert abs(r_or - st.true_relres) < 1e-3 * r_or
     cc, df, dg = c.compliance_sensitivity()
     dfo, _ = o.sensitivity(o.node_values(ug))
     assert _rel(df, dfo) < 1e-12

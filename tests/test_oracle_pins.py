@@ -132,8 +132,9 @@ def test_face_load_unit_faces(oracle):
 # --------------------------------------------------------------------- KE0
 def _exact_h8_stiffness(nu):
     """Exact stiffness of the unit trilinear hexahedron from products of 1-D
-    integrals (independent of any quadrature)."""
-    nu = Fraction(nu).limit_denominator(10**6)
+    integrals (independent of any quadrature), at the exact value of the
+    double nu, each entry correctly rounded to FP64."""
+    nu = Fraction(nu)
     lam = nu / ((1 + nu) * (1 - 2 * nu))
     mu = Fraction(1, 2) / (1 + nu)
     corners = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
@@ -168,10 +169,14 @@ def _exact_h8_stiffness(nu):
 
 
 def test_ke0_equals_exact_integral(oracle):
-    pr = oracle.Problem(2, 2, 2, poisson=0.3)
-    K = pr.ke0()
-    Kx = _exact_h8_stiffness(0.3)
-    assert np.max(np.abs(K - Kx)) < 1e-15 * 10
+    """KE0 is the FP64 rounding of the exact integral, entry by entry (the
+    Gauss sums run in extended precision): equal exact entries are equal
+    doubles, so the operator does not depend on a summation order."""
+    for nu in (0.3, 0.25, 0.45):
+        pr = oracle.Problem(2, 2, 2, poisson=nu)
+        K = pr.ke0()
+        Kx = _exact_h8_stiffness(nu)
+        assert np.array_equal(K, Kx), np.max(np.abs(K - Kx))
 
 
 def test_ke0_invariants(oracle):
@@ -217,6 +222,42 @@ def test_csr_equals_elementwise(oracle):
     # symmetry of the assembled operator
     v = rng.standard_normal(pr.numeq)
     assert abs(v @ pr.spmv(u) - u @ pr.spmv(v)) < 1e-12 * abs(v @ y1)
+
+
+def test_nodal_apply_equals_csr_and_elementwise(oracle):
+    """The node-by-node K.u (used where the CSR does not fit) against the
+    assembled K and the element-by-element scatter, with nonzero prescribed
+    displacements, box voxels and the matrix-free density setter."""
+    rng = np.random.default_rng(11)
+    dims, L = (6, 5, 4), (3.0, 7.5, 2.0)
+    pr = oracle.Problem(*dims, *L)
+    pr.add_dirichlet([1, 1, 1, 0, 0, 0], [0.0] * 6, [[0, 0], [0, L[1]], [0, L[2]]])
+    pr.add_dirichlet([1, 0, 1, 0, 0, 0], [0.02, 0.0, -0.01, 0, 0, 0], [[L[0], L[0]], [0, L[1]], [0, L[2]]])
+    pr.make_dof()
+    pr.add_neumann([0.0, 1.0, 0.0], [[L[0], L[0]], [0, L[1]], [0, L[2]]])
+    x = rng.uniform(1e-3, 1.0, pr.nelx)
+    pr.set_density(x)
+    u = rng.standard_normal(pr.numeq)
+    dv = pr.dirichlet_values()
+    y_csr = pr.spmv(u)
+    y_el = pr.apply_k_elementwise(u, dv)
+    y_el0 = pr.apply_k_elementwise(u)
+    scale = np.max(np.abs(y_el))
+    assert np.max(np.abs(pr.apply_k_nodal(u, dv) - y_el)) <= 1e-13 * scale
+    assert np.max(np.abs(pr.apply_k_nodal(u) - y_csr)) <= 1e-13 * np.max(np.abs(y_csr))
+    assert np.max(np.abs(y_el0 - y_csr)) <= 1e-13 * np.max(np.abs(y_csr))
+    # prescribed displacements enter the nodal apply (the term K_fc u_c)
+    assert np.max(np.abs(y_el - y_el0)) > 1e-6 * scale
+    r_ref = pr.true_residual(u)
+    q = oracle.Problem(*dims, *L)
+    q.add_dirichlet([1, 1, 1, 0, 0, 0], [0.0] * 6, [[0, 0], [0, L[1]], [0, L[2]]])
+    q.add_dirichlet([1, 0, 1, 0, 0, 0], [0.02, 0.0, -0.01, 0, 0, 0], [[L[0], L[0]], [0, L[1]], [0, L[2]]])
+    q.make_dof()
+    q.add_neumann([0.0, 1.0, 0.0], [[L[0], L[0]], [0, L[1]], [0, L[2]]])
+    q.set_density_matrix_free(x)
+    assert q.nnz() == 0
+    assert np.array_equal(q.apply_k_nodal(u, dv), pr.apply_k_nodal(u, dv))
+    assert abs(q.true_residual_nodal(u) - r_ref) <= 1e-12 * r_ref
 
 
 def _dense_K(pr):
