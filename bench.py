@@ -294,8 +294,12 @@ def precision_leg(A, cfg, method, tol, maxit, ref):
     c, x0 = make_problem(A, cfg)
     c.set_density(c.density_filter(x0))
     log(f"[bench] one solve, {method}, maxit {maxit}")
-    u, st = c.solve(tol=tol, maxit=maxit, precision=prec, precond=pre, want_u=False,
-                    allow_cap=True)
+    try:
+        u, st = c.solve(tol=tol, maxit=maxit, precision=prec, precond=pre, want_u=False,
+                        allow_cap=True)
+    except A.IcaratError as e:
+        # e.g. a breakdown of the low-precision Jacobi-PCG on this contrast
+        return {"reached": False, "error": str(e)}
     comp, _, _ = c.compliance_sensitivity()
     out = {"reached": bool(st.status == 0 and st.true_relres <= tol), "iterations": int(st.iters),
            "verifications": int(st.refine_steps), "solve_s": st.seconds,
@@ -330,6 +334,9 @@ def main():
                          "(filter_helm.hpp)")
     ap.add_argument("--budget", type=int, default=None,
                     help="c5: admission budget in elements (0: all problems at once)")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="time only the dominant, vector and copy kernels on the first design "
+                         "(for profiler passes); prints their figures, not the contract line")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only; the multi-GPU (slab "
@@ -352,6 +359,21 @@ def main():
         return
     helm = a.filter == "helm"
     dims_s = "x".join(map(str, cfg["dims"]))
+    if a.roofline_only:
+        c, x0 = make_problem(A, cfg, helm)
+        c.set_density(c.density_filter(x0))
+        kid = DOMINANT[a.method][0]
+        res = {}
+        hooks = [("dominant", kid), ("vector", 11 if METHODS[a.method][1] == 2 else 1),
+                 ("copy", 7)]
+        if helm:
+            hooks.append(("helmholtz_filter", 9))
+        for name, k in hooks:
+            ms, fl, by = c.time_kernel(k, a.reps)
+            res[name] = {"timing_hook": k, "avg_ms": ms, "flops": fl, "bytes": by}
+            log(f"[bench] {name} kernel (hook {k}): {ms * 1e3:.2f} us")
+        print(json.dumps({"workload": dims_s, "method": a.method, "kernels": res}), flush=True)
+        return
     c, x0, x, hist, st, dt = run_gpu(A, cfg, a.method, a.steps, a.warmup, a.tol, helm)
     pcg_per_solve = float(hist[:, 2].mean())
     # the plain method (FP64 Jacobi-PCG) on the same workload and tolerance:

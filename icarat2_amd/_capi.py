@@ -47,14 +47,16 @@ class TomlInfo(C.Structure):
 class SimpOpts(C.Structure):
     _fields_ = [("n_iter", C.c_int32), ("optimizer", C.c_int32), ("use_projection", C.c_int32),
                 ("filter", C.c_int32), ("xmin", C.c_double), ("xmax", C.c_double),
-                ("solve", SolveOpts)]
+                ("solve", SolveOpts), ("conv_eps", C.c_double), ("conv_minstep", C.c_int32),
+                ("history_csv", C.c_char_p)]
 
 
 class SimpStats(C.Structure):
     _fields_ = [("seconds", C.c_double), ("solve_s", C.c_double), ("filter_s", C.c_double),
                 ("sens_s", C.c_double), ("update_s", C.c_double), ("pcg_iters", C.c_int64),
                 ("max_true_relres", C.c_double), ("train_s", C.c_double),
-                ("fallbacks", C.c_int64), ("inner_iters", C.c_int64)]
+                ("fallbacks", C.c_int64), ("inner_iters", C.c_int64), ("steps", C.c_int32),
+                ("converged", C.c_int32)]
 
 
 class LearnedOpts(C.Structure):
@@ -116,6 +118,8 @@ _SIGS = {
     "icarat_hip_create_from_toml": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p),
                                               C.c_void_p, C.c_char_p, C.c_int32]),
     "icarat_hip_write_vtk": (C.c_int, [C.c_void_p, C.c_char_p, dp]),
+    "icarat_hip_add_plot": (C.c_int, [C.c_char_p, C.c_double, C.c_double, C.c_int32]),
+    "icarat_hip_export_graph": (C.c_int, [C.c_char_p, dp, dp, C.c_int64]),
     "icarat_hip_gcmma_set_params": (C.c_int, [C.c_void_p] + [C.c_double] * 5),
     "icarat_hip_gcmma_reset": (C.c_int, [C.c_void_p]),
     "icarat_hip_gcmma_outer": (C.c_int, [C.c_void_p, dp, dp, C.c_double, dp, C.c_double, dp,
@@ -293,6 +297,22 @@ class UNet:
         self._chk(self.L.icarat_hip_unet_train_step(self.h, p, tp, lr, C.byref(loss)),
                   "unet_train_step")
         return loss.value
+
+
+def add_plot(filename, x, y, is_first):
+    """addPlot(x, y, filename, isFirst) of the reference's export_graph.hpp."""
+    rc = lib().icarat_hip_add_plot(os.fsencode(filename), float(x), float(y), int(is_first))
+    if rc:
+        raise IcaratError(rc, f"add_plot: cannot write {filename}")
+
+
+def export_graph(filename, yy, xx=None):
+    """exportGraph(yy, filename) or exportGraph(xx, yy, filename)."""
+    yy = _f64(yy)
+    xp = None if xx is None else _p(_f64(xx))
+    rc = lib().icarat_hip_export_graph(os.fsencode(filename), xp, _p(yy), yy.size)
+    if rc:
+        raise IcaratError(rc, f"export_graph: cannot write {filename}")
 
 
 def device_available():
@@ -619,17 +639,22 @@ class Context:
 
     def simp_run(self, x0, n_iter, optimizer=0, use_projection=False, xmin=1e-4, xmax=1.0,
                  tol=1e-8, maxit=10000, precision=FP64, precond=PRECOND_JACOBI, inner_tol=0.1,
-                 max_refine=200, warm=False, helmholtz=False):
+                 max_refine=200, warm=False, helmholtz=False, conv_eps=0.0, conv_minstep=0,
+                 history_csv=None):
+        """The design loop; with conv_eps > 0 it may end before n_iter
+        (isConvergence): hist then has st.steps rows."""
         o = SimpOpts(n_iter, optimizer, 1 if use_projection else 0, 1 if helmholtz else 0,
                      xmin, xmax,
                      SolveOpts(tol, maxit, precision, precond, inner_tol, max_refine,
-                               1 if warm else 0))
+                               1 if warm else 0), conv_eps, conv_minstep,
+                     None if history_csv is None else os.fsencode(history_csv))
         x = np.array(x0, dtype=np.float64)
         hist = np.zeros(4 * max(n_iter, 1))
         st = SimpStats()
         self._chk(self.L.icarat_hip_simp_run(self.h, C.byref(o), _p(x), _p(hist), C.byref(st)),
                   "simp_run")
-        return x, hist[:4 * n_iter].reshape(n_iter, 4), st
+        steps = st.steps if conv_eps > 0 else n_iter
+        return x, hist[:4 * steps].reshape(steps, 4), st
 
 
 def cantilever(nx, ny, nz, **kw):

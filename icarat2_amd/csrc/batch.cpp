@@ -180,7 +180,7 @@ void Batch::simp_run(const icarat_simp_opts &o, double *const *x, double *const 
       o.solve.tol, o.solve.maxit,
       [&](int i) {
         Context &c = *ctx_[i];
-        if (step[i] > o.n_iter) return false;
+        if (step[i] > o.n_iter || c.simp_.S.converged) return false;
         const int nc = c.problem().num_load_cases();
         if (lc[i] < 0) c.simp_pre(step[i]);  // a new design iteration
         ++lc[i];

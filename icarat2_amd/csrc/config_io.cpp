@@ -23,6 +23,16 @@
 #include "toml_lite.hpp"
 
 namespace icarat_hip {
+
+// addPlot (io/export_graph.hpp:56-70): "x y" with the stream's default
+// formatting; the file is created afresh on the first line
+void add_plot(const char *path, double x, double y, bool is_first) {
+  std::ofstream f(path, is_first ? std::ios::out : std::ios::app);
+  if (!f) throw Error(kInvalidArgument, std::string("cannot write ") + path);
+  f << x << " " << y << std::endl;
+  if (!f) throw Error(kInternal, std::string("write failed: ") + path);
+}
+
 // defined in capi.cpp: wraps a context into the C handle
 icarat_hip_ctx *wrap_context(Context *c);
 Context *unwrap_context(icarat_hip_ctx *h);
@@ -156,6 +166,32 @@ int icarat_hip_create_from_toml(const char *path, icarat_hip_ctx **out, icarat_t
   *out = icarat_hip::wrap_context(c);
   copy_err(err, err_len, "");
   return ICARAT_HIP_OK;
+}
+
+int icarat_hip_add_plot(const char *filename, double x, double y, int32_t is_first) {
+  if (!filename) return ICARAT_HIP_INVALID_ARGUMENT;
+  try {
+    icarat_hip::add_plot(filename, x, y, is_first == 1);
+  } catch (const icarat_hip::Error &e) {
+    return e.code;
+  } catch (const std::exception &) {
+    return ICARAT_HIP_INTERNAL;
+  }
+  return ICARAT_HIP_OK;
+}
+
+int icarat_hip_export_graph(const char *filename, const double *xx, const double *yy, int64_t n) {
+  if (!filename || !yy || n < 0) return ICARAT_HIP_INVALID_ARGUMENT;
+  // exportGraph(yy) / exportGraph(xx, yy) (io/export_graph.hpp:15-40)
+  std::ofstream f(filename, std::ios::out);
+  if (!f) return ICARAT_HIP_INVALID_ARGUMENT;
+  for (int64_t i = 0; i < n; ++i) {
+    if (xx)
+      f << xx[i] << " " << yy[i] << std::endl;
+    else
+      f << (int)i << " " << yy[i] << std::endl;
+  }
+  return f ? ICARAT_HIP_OK : ICARAT_HIP_INTERNAL;
 }
 
 int icarat_hip_write_vtk(icarat_hip_ctx *h, const char *path, const double *density) {

@@ -768,6 +768,7 @@ void Context::simp_begin(const icarat_simp_opts &o, const double *x) {
   if (o.filter == 0 && !filter_ready_) throw Error(kNotReady, "filter_setup has not been called");
   if (o.filter == 1 && !helm_) throw Error(kNotReady, "helm_setup has not been called");
   if (o.n_iter < 0) throw Error(kInvalidArgument, "n_iter must be >= 0");
+  if (!(o.conv_eps >= 0.0)) throw Error(kInvalidArgument, "conv_eps must be >= 0");
   if (o.optimizer != 0 && o.optimizer != 1 && o.optimizer != 2)
     throw Error(kInvalidArgument, "optimizer must be 0 (OC), 1 (MMA) or 2 (GCMMA)");
   sync();
@@ -852,6 +853,17 @@ void Context::simp_update(int step, double *hist) {
     hist[4 * (step - 1) + 2] = (double)simp_.iters;
     hist[4 * (step - 1) + 3] = simp_.relres;
   }
+  // the drivers' history line and convergence test, both after the compliance
+  // (opt_macro_oc.cpp:134-137, optimization/opt_base.hpp:28-44); the design
+  // update of the converged iteration still runs
+  if (o.history_csv) add_plot(o.history_csv, (double)step, simp_.c, step == 1);
+  if (o.conv_eps > 0.0) {
+    if (step > o.conv_minstep && std::fabs(simp_.c_old - simp_.c) / simp_.c < o.conv_eps)
+      simp_.S.converged = 1;
+    else
+      simp_.c_old = simp_.c;
+  }
+  simp_.S.steps = step;
   if (o.optimizer == 1) {
     const double gc = step == 1 ? 0.0 : simp_.vol / simp_.V0 - 1.0;
     mma_->update(d_e[E_DES], d_e[E_DFDS], gc, d_e[E_DGDS], o.xmin, o.xmax);
@@ -1046,6 +1058,7 @@ void Context::simp_run(const icarat_simp_opts &o, double *x, double *hist, icara
     }
     if (ncase > 1) use_load_case(prob_.load_case());
     simp_update(step, hist);
+    if (simp_.S.converged) break;
   }
   simp_end(x, st);
 }

@@ -21,6 +21,10 @@ class HelmFilter;
 class LearnedPrecond;
 class Multigrid;
 
+// addPlot of io/export_graph.hpp:56-70: the line "x y" appended to path, the
+// file created afresh when is_first (config_io.cpp)
+void add_plot(const char *path, double x, double y, bool is_first);
+
 // the fixed device scratch of the no-allocation entry point (noalloc.hip):
 // its per-device address is looked up by the device set-up of a context
 void noalloc_init_device(int device);
@@ -91,6 +95,7 @@ class Context {
     icarat_simp_opts o{};
     icarat_simp_stats S{};
     double t_all = 0.0, V0 = 0.0, vol = 0.0, beta = 1.0, c = 0.0, relres = 0.0;
+    double c_old = 0.0;  // the convergence test's previous compliance (Optimize::object_old)
     long long iters = 0;
   } simp_;
 

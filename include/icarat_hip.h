@@ -121,6 +121,14 @@ int icarat_hip_create_from_toml(const char *path, icarat_hip_ctx **out, icarat_t
  * "density" when density (nelx) is not NULL. */
 int icarat_hip_write_vtk(icarat_hip_ctx *ctx, const char *path, const double *density);
 
+/* The drivers' history files (io/export_graph.hpp): addPlot(x, y, filename,
+ * isFirst) appends the line "x y" (default stream formatting), creating the file
+ * afresh when is_first == 1 (export_graph.hpp:56-70); exportGraph(xx, yy,
+ * filename) writes n lines "xx[i] yy[i]", or "i yy[i]" when xx is NULL
+ * (export_graph.hpp:15-40).  Host only. */
+int icarat_hip_add_plot(const char *filename, double x, double y, int32_t is_first);
+int icarat_hip_export_graph(const char *filename, const double *xx, const double *yy, int64_t n);
+
 /* ---- boundary conditions (BCIcarat) -------------------------------------- */
 /* Dirichlet box: flag[6]/val[6]/range {x0,x1,y0,y1,z0,z1}, tolerance 1e-7
  * (io/boundary_condition_icarat.hpp:72-138, io/boundary_condition_base.hpp:19-33). */
@@ -258,6 +266,15 @@ typedef struct icarat_simp_opts {
                               loop of opt-mma-HelmholtzFilter/opt_mma_Helmholtz.cpp:84-150) */
   double xmin, xmax;       /* Optimize(nelx, 1, xmin, xmax) (optimization/opt_base.hpp:20-24) */
   icarat_solve_opts solve;
+  double conv_eps;         /* Optimize::isConvergence(conv_eps, conv_minstep) (opt_base.hpp:28-44,
+                              called after the compliance, opt_macro_oc.cpp:137): the loop ends
+                              with design iteration k (its update included) once
+                              |c_old - c_k| / c_k < conv_eps and k > conv_minstep, c_old being the
+                              compliance of the last iteration that did not pass; n_iter stays the
+                              cap.  0: always n_iter iterations (OC and MMA loops) */
+  int32_t conv_minstep;
+  const char *history_csv; /* addPlot(step, compliance, history_csv, step) every iteration
+                              (io/export_graph.hpp:56-70, opt_macro_oc.cpp:134); NULL: none */
 } icarat_simp_opts;
 
 typedef struct icarat_simp_stats {
@@ -268,6 +285,8 @@ typedef struct icarat_simp_stats {
   double train_s;          /* learned preconditioner: time spent training the network */
   int64_t fallbacks;       /* learned preconditioner: solves that fell back to Jacobi-PCG */
   int64_t inner_iters;     /* GCMMA: inner iterations, all design iterations */
+  int32_t steps;           /* design iterations run */
+  int32_t converged;       /* 1 if the loop ended on the conv_eps test */
 } icarat_simp_stats;
 
 /* The reference drivers' loop (sample/algorithm/opt-macro-oc/opt_macro_oc.cpp:91-178,

@@ -355,6 +355,31 @@ class GCMMASolver {
   double move_ = 0.15, albefa_ = 0.2, asyminit_ = 0.2, asymdec_ = 0.7, asyminc_ = 1.2;
 };
 
+// addPlot(xx, yy, filename, isFirst) and exportGraph(yy, filename) /
+// exportGraph(xx, yy, filename) of io/export_graph.hpp:15-83.
+inline void addPlot(double xx, double yy, const std::string &filename, int isFirst) {
+  if (icarat_hip_add_plot(filename.c_str(), xx, yy, isFirst) != ICARAT_HIP_OK) {
+    std::cerr << "Error : cannot write " << filename << std::endl;
+    std::exit(1);
+  }
+}
+inline void exportGraph(const std::vector<double> &yy, const std::string &filename) {
+  if (icarat_hip_export_graph(filename.c_str(), nullptr, yy.data(), (int64_t)yy.size()) !=
+      ICARAT_HIP_OK) {
+    std::cerr << "Error : cannot write " << filename << std::endl;
+    std::exit(1);
+  }
+}
+inline void exportGraph(const std::vector<double> &xx, const std::vector<double> &yy,
+                        const std::string &filename) {
+  if (xx.size() != yy.size() ||
+      icarat_hip_export_graph(filename.c_str(), xx.data(), yy.data(), (int64_t)yy.size()) !=
+          ICARAT_HIP_OK) {
+    std::cerr << "Error : cannot write " << filename << std::endl;
+    std::exit(1);
+  }
+}
+
 }  // namespace icarat_hip
 
 #endif  // ICARAT_HIP_HPP

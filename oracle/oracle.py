@@ -114,7 +114,8 @@ def lib():
 
 
 def ref_lib():
-    """oracle/_ref/libref_optim.so (reference MMASolver.cpp / OCSolver.hpp), or None."""
+    """oracle/_ref/libref_optim.so (the reference's MMASolver.cpp, GCMMASolver.cpp,
+    OCSolver.hpp, opt_base.hpp and export_graph.hpp), or None."""
     global _REF
     if _REF is None:
         path = os.path.join(HERE, "_ref", "libref_optim.so")
@@ -137,6 +138,9 @@ def ref_lib():
         L.ref_gcmma_inner.argtypes = [vp, dp, C.c_double, dp, dp, C.c_double, dp, dp, dp, dp, dp]
         L.ref_gcmma_concheck.restype = C.c_int
         L.ref_gcmma_concheck.argtypes = [vp, C.c_double, dp]
+        L.ref_add_plot.argtypes = [C.c_char_p, C.c_int, C.c_double, C.c_int]
+        L.ref_export_graph.argtypes = [C.c_char_p, C.c_void_p, dp, C.c_int]
+        L.ref_is_convergence.argtypes = [dp, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_int)]
         _REF = L
     return _REF
 

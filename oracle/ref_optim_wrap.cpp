@@ -1,6 +1,7 @@
 // ref_optim_wrap.cpp -- C entry points around the reference's own std-only
 // optimizers, compiled from the reference sources where they lie
-// (optimization/opt-solver/MMASolver.cpp, GCMMASolver.cpp, OCSolver.hpp)
+// (optimization/opt-solver/MMASolver.cpp, GCMMASolver.cpp, OCSolver.hpp,
+// and the std-only headers optimization/opt_base.hpp, io/export_graph.hpp)
 // into oracle/_ref/libref_optim.so by oracle/Makefile.
 //
 // TEST INFRASTRUCTURE ONLY: used by tests/ to pin the oracle's OC and MMA
@@ -11,9 +12,11 @@
 #include <iostream>
 #include <vector>
 
+#include "io/export_graph.hpp"
 #include "optimization/opt-solver/GCMMASolver.hpp"
 #include "optimization/opt-solver/MMASolver.hpp"
 #include "optimization/opt-solver/OCSolver.hpp"
+#include "optimization/opt_base.hpp"
 
 extern "C" {
 
@@ -70,5 +73,32 @@ void ref_oc_update(int n, double *x, double *dfdx, double g, double *dgdx,
   oc.UpdateVariables(xv, df, g, dg, lo, hi,
                      [&](std::vector<double> &xk) { return gk(xk.data(), n, user); });
   std::copy(xv.begin(), xv.end(), x);
+}
+
+// addPlot(step, value, filename, step) as the drivers call it (an int step)
+void ref_add_plot(const char *filename, int step, double value, int is_first) {
+  icarat::addPlot(step, value, std::string(filename), is_first);
+}
+// exportGraph(yy, filename) and exportGraph(xx, yy, filename)
+void ref_export_graph(const char *filename, const double *xx, const double *yy, int n) {
+  std::vector<double> y(yy, yy + n);
+  if (xx) {
+    std::vector<double> x(xx, xx + n);
+    icarat::exportGraph(x, y, std::string(filename));
+  } else {
+    icarat::exportGraph(y, std::string(filename));
+  }
+}
+// Optimize::isConvergence(eps, minstep) over a compliance history f[0..n),
+// optstep = k + 1 at entry k; out[k] = its return value (0: converged)
+void ref_is_convergence(const double *f, int n, double eps, double minstep, int *out) {
+  icarat::Optimize o(1, 1, 0.0, 1.0);
+  for (int k = 0; k < n; ++k) {
+    o.optstep = k + 1;
+    o.object_f = f[k];
+    std::streambuf *old = std::cout.rdbuf(nullptr);  // the banner it prints
+    out[k] = o.isConvergence(eps, minstep);
+    std::cout.rdbuf(old);
+  }
 }
 }
