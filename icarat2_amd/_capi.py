@@ -63,13 +63,16 @@ class LearnedOpts(C.Structure):
     _fields_ = [("train_iters", C.c_int32), ("train_steps", C.c_int32),
                 ("snapshot_every", C.c_int32), ("max_samples", C.c_int32),
                 ("batch_iters", C.c_int32), ("stall_iters", C.c_int32), ("lr", C.c_float),
-                ("gate", C.c_float), ("seed", C.c_uint64)]
+                ("gate", C.c_float), ("seed", C.c_uint64), ("base", C.c_int32),
+                ("correction_scale", C.c_float)]
 
 
 class LearnedInfo(C.Structure):
     _fields_ = [("samples", C.c_int32), ("active", C.c_int32), ("steps", C.c_int64),
                 ("loss", C.c_double), ("val_loss", C.c_double), ("gain", C.c_double),
-                ("n_params", C.c_int64), ("grid", C.c_int32 * 3), ("harvested", C.c_int32)]
+                ("n_params", C.c_int64), ("grid", C.c_int32 * 3), ("harvested", C.c_int32),
+                ("fallback_iter", C.c_int32), ("trial_iters", C.c_int32),
+                ("trial_base_iters", C.c_int32)]
 
 
 class MgOpts(C.Structure):

@@ -392,6 +392,8 @@ int icarat_hip_learned_default_opts(icarat_learned_opts *o) {
   o->lr = 1e-3f;
   o->gate = 0.8f;
   o->seed = 1;
+  o->base = 0;
+  o->correction_scale = 1.f;
   return ICARAT_HIP_OK;
 }
 
@@ -439,6 +441,9 @@ int icarat_hip_learned_info(const icarat_hip_ctx *hc, icarat_learned_info *info)
     d.grid[1] = lp.grid().Y;
     d.grid[2] = lp.grid().Z;
     d.harvested = lp.harvested();
+    d.fallback_iter = lp.fallback_iter();
+    d.trial_iters = lp.trial_iters();
+    d.trial_base_iters = lp.trial_base_iters();
     *info = d;
   });
 }

@@ -1031,7 +1031,8 @@ void Context::simp_run(const icarat_simp_opts &o, double *x, double *hist, icara
       const bool harvest = so.precond == ICARAT_HIP_PRECOND_LEARNED && learned_ &&
                            learned_->harvested() < learned_->opts().train_iters;
       if (harvest) {
-        so.precond = ICARAT_HIP_PRECOND_JACOBI;
+        // the base method's own solve (Jacobi-PCG or multigrid PCG) is harvested
+        so.precond = learned_->opts().base == 1 ? ICARAT_HIP_PRECOND_MG : ICARAT_HIP_PRECOND_JACOBI;
         learned_->begin_harvest();
         batch_hook_ = [this]() { learned_->snapshot(); };
         try {
@@ -1043,6 +1044,7 @@ void Context::simp_run(const icarat_simp_opts &o, double *x, double *hist, icara
         }
         batch_hook_ = nullptr;
         learned_->end_harvest(ss.status == ICARAT_HIP_OK);
+        learned_->note_base_iters(ss.iters);
         const int ti = std::max(1, learned_->opts().train_iters);
         const int h = learned_->harvested();  // 1..ti
         const int steps =
@@ -1051,7 +1053,10 @@ void Context::simp_run(const icarat_simp_opts &o, double *x, double *hist, icara
         learned_->train(steps);
         simp_.S.train_s += now_s() - tt;
       } else {
+        const bool trial = so.precond == ICARAT_HIP_PRECOND_LEARNED && learned_ &&
+                           learned_->trial_pending();
         solve(so, nullptr, &ss);
+        if (trial) learned_->trial_result(ss.iters);
       }
       simp_.S.solve_s += now_s() - t1;
       simp_case_post(lc, ss);
@@ -1082,7 +1087,7 @@ void Context::learned_harvest(const icarat_solve_opts &o, double *u_host, icarat
   LearnedPrecond &lp = learned();
   require_density();
   icarat_solve_opts so = o;
-  so.precond = ICARAT_HIP_PRECOND_JACOBI;
+  so.precond = lp.opts().base == 1 ? ICARAT_HIP_PRECOND_MG : ICARAT_HIP_PRECOND_JACOBI;
   lp.begin_harvest();
   batch_hook_ = [&lp]() { lp.snapshot(); };
   icarat_solve_stats ss{};

@@ -134,7 +134,8 @@ __global__ void k_density_feature(Lattice L, NetGrid g, const double *__restrict
 __global__ void k_prolong_add(Lattice L, NetGrid g, const float *__restrict__ out, int cs,
                               const double *__restrict__ scale, const double *__restrict__ r,
                               const double *__restrict__ dinv, const uint8_t *__restrict__ fmask,
-                              double *__restrict__ z, int use_net, double gain) {
+                              double *__restrict__ z, int use_net, double gain,
+                              double *__restrict__ corr_out) {
   const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (n >= L.numnp) return;
   const int x = (int)(n % L.NX), y = (int)((n / L.NX) % L.NY), zz = (int)(n / ((long long)L.NX * L.NY));
@@ -170,8 +171,54 @@ __global__ void k_prolong_add(Lattice L, NetGrid g, const float *__restrict__ ou
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     const long long q = k * L.Np + n;
-    z[q] = ((m >> k) & 1u) ? dinv[q] * r[q] + sig * corr[k] : 0.0;
+    const bool f = (m >> k) & 1u;
+    z[q] = f ? dinv[q] * r[q] + sig * corr[k] : 0.0;
+    if (corr_out) corr_out[q] = f ? sig * corr[k] : 0.0;
   }
+}
+
+// Descent guard of the corrected preconditioner z = b + c (b = dinv base):
+// partials of b.r and c.r
+__global__ __launch_bounds__(NT) void k_guard_dots(long long n, const double *__restrict__ base,
+                                                   const double *__restrict__ dinv,
+                                                   const double *__restrict__ c,
+                                                   const double *__restrict__ r, double *partials) {
+  __shared__ double sh[8];
+  double br = 0.0, cr = 0.0;
+  const long long stride = (long long)gridDim.x * NT;
+  for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += stride) {
+    br += dinv[i] * base[i] * r[i];
+    cr += c[i] * r[i];
+  }
+  double v[2] = {br, cr};
+  store_partials<NT, 2>(v, partials, sh);
+}
+
+// gamma in scale[3]: 1 when z.r = b.r + c.r >= b.r / 2, else the factor that
+// makes it b.r / 2 (0 when b.r <= 0), so z.r stays positive
+__global__ void k_guard_gamma(const double *__restrict__ partials, unsigned G, double *scale) {
+  __shared__ double sh[8];
+  double t[2];
+  sum_partials<NT, 2>(partials, G, t, sh);
+  if (threadIdx.x != 0) return;
+  const double br = t[0], cr = t[1];
+  double g = 1.0;
+  if (!(br > 0.0)) g = 0.0;
+  else if (br + cr < 0.5 * br) g = -0.5 * br / cr;
+  scale[3] = g;
+}
+
+// z = b + gamma c where the guard shrank the correction
+__global__ __launch_bounds__(NT) void k_guard_apply(long long n, const double *__restrict__ base,
+                                                    const double *__restrict__ dinv,
+                                                    const double *__restrict__ c,
+                                                    const double *__restrict__ scale,
+                                                    double *__restrict__ z) {
+  const double g = scale[3];
+  if (g == 1.0) return;
+  const long long stride = (long long)gridDim.x * NT;
+  for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += stride)
+    z[i] = dinv[i] * base[i] + g * c[i];
 }
 
 __global__ void k_scale_f32(long long n, float a, float *__restrict__ x) {
@@ -408,9 +455,21 @@ void launch_density_feature(const Lattice &L, const NetGrid &g, const double *yo
 
 void launch_prolong_add(const Lattice &L, const NetGrid &g, const float *out, int cs,
                         const double *scale, const double *r, const double *dinv,
-                        const uint8_t *fmask, double *z, int use_net, double gain, hipStream_t s) {
+                        const uint8_t *fmask, double *z, int use_net, double gain, hipStream_t s,
+                        double *corr_out) {
   hipLaunchKernelGGL(k_prolong_add, dim3(blocks_for(L.numnp, 256)), dim3(256), 0, s, L, g, out, cs,
-                     scale, r, dinv, fmask, z, use_net, gain);
+                     scale, r, dinv, fmask, z, use_net, gain, corr_out);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_descent_guard(long long n, const double *base, const double *dinv, const double *c,
+                          const double *r, double *z, double *partials, double *scale,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_guard_dots, dim3(VEC_BLOCKS), dim3(NT), 0, s, n, base, dinv, c, r, partials);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_guard_gamma, dim3(1), dim3(NT), 0, s, partials, (unsigned)VEC_BLOCKS, scale);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_guard_apply, dim3(VEC_BLOCKS), dim3(NT), 0, s, n, base, dinv, c, scale, z);
   HIPCHK(hipGetLastError());
 }
 

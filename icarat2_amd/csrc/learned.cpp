@@ -7,6 +7,7 @@
 #include "context.hpp"
 #include "learned_kernels.hpp"
 #include "mixed_kernels.hpp"
+#include "multigrid.hpp"
 
 namespace icarat_hip {
 
@@ -16,6 +17,10 @@ int round4(int v) { return (v + 3) / 4 * 4; }
 
 LearnedPrecond::LearnedPrecond(Context &ctx, const icarat_learned_opts &o) : ctx_(ctx), o_(o) {
   ctx_.ensure_device();
+  if (o_.base != 0 && o_.base != 1) throw Error(kInvalidArgument, "learned base must be 0 or 1");
+  if (!(o_.correction_scale > 0.f)) o_.correction_scale = 1.f;
+  mg_base_ = o_.base == 1;
+  if (mg_base_ && !ctx_.mg_) ctx_.mg_setup(mg_default_opts());
   const Lattice &L = ctx_.lattice();
   g0_ = NetGrid{round4((L.nx + 1) / 2), round4((L.ny + 1) / 2), round4((L.nz + 1) / 2)};
   ncell_ = g0_.nvox();
@@ -27,6 +32,7 @@ LearnedPrecond::LearnedPrecond(Context &ctx, const icarat_learned_opts &o) : ctx
   xin_.alloc(ncell_ * 8);
   scale_.alloc(4);
   z_.alloc(ctx_.n3_);
+  corr_.alloc(ctx_.n3_);
   ones_.alloc(ctx_.n3_);
   HIPCHK(hipMemsetAsync(z_.p, 0, z_.bytes(), s));
   HIPCHK(hipMemsetAsync(scale_.p, 0, scale_.bytes(), s));
@@ -59,9 +65,20 @@ void LearnedPrecond::begin_harvest() {
   launch_density_feature(ctx_.lattice(), g0_, ctx_.d_young, feat_, ctx_.stream());
 }
 
+const double *LearnedPrecond::base_vec(const double *r, const double **dinv) {
+  if (!mg_base_) {
+    *dinv = ctx_.d_dinv.p;
+    return r;
+  }
+  *dinv = ones_.p;
+  return ctx_.mg_->vcycle(r);
+}
+
 void LearnedPrecond::snapshot() {
   if (!harvesting_) return;
-  if (harvest_batches_++ % std::max(1, o_.snapshot_every / Context::kBatch) != 0) return;
+  // iterations per batch of the harvesting solve (Jacobi-PCG or multigrid PCG)
+  const int per = mg_base_ ? ctx_.mg_->batch_iters() : Context::kBatch;
+  if (harvest_batches_++ % std::max(1, o_.snapshot_every / per) != 0) return;
   // the snapshots of one solve never overwrite each other
   if ((int)pending_.size() >= cap_) return;
   hipStream_t s = ctx_.stream();
@@ -69,7 +86,11 @@ void LearnedPrecond::snapshot() {
   const int k = next_;
   next_ = (next_ + 1) % cap_;
   valid_[k] = 0;
-  launch_restrict_residual(L, g0_, ctx_.d_r, ctx_.d_dinv, s3_, ctx_.d_partials, scale_, s);
+  // B(r_k): D^-1 r_k, or the V-cycle the multigrid-PCG iteration just applied
+  // to r_k (its result buffer holds V(r) at a batch boundary)
+  const double *bz = mg_base_ ? ctx_.mg_->result() : ctx_.d_r.p;
+  launch_restrict_residual(L, g0_, bz, mg_base_ ? ones_.p : ctx_.d_dinv.p, s3_, ctx_.d_partials,
+                           scale_, s);
   launch_pack_input(g0_, feat_, s3_, scale_, sx_.p + (size_t)k * ncell_ * 8,
                     szr_.p + (size_t)k * ncell_ * 3, s);
   HIPCHK(hipMemcpyAsync(ssig_.p + k, scale_.p, sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -187,17 +208,24 @@ double LearnedPrecond::train(int steps) {
 void LearnedPrecond::precondition(const double *r) {
   hipStream_t s = ctx_.stream();
   const Lattice &L = ctx_.lattice();
+  const double *dinv = nullptr;
+  const double *b = base_vec(r, &dinv);
   if (!active_) {
-    launch_prolong_add(L, g0_, out_, 8, scale_, r, ctx_.d_dinv, ctx_.d_free, z_, 0, 0.0, s);
+    launch_prolong_add(L, g0_, out_, 8, scale_, b, dinv, ctx_.d_free, z_, 0, 0.0, s);
     return;
   }
-  launch_restrict_residual(L, g0_, r, ctx_.d_dinv, s3_, ctx_.d_partials, scale_, s);
+  launch_restrict_residual(L, g0_, b, dinv, s3_, ctx_.d_partials, scale_, s);
   launch_pack_input(g0_, feat_, s3_, scale_, xin_, nullptr, s);
   net_->forward(xin_, out_);
-  launch_prolong_add(L, g0_, out_, 8, scale_, r, ctx_.d_dinv, ctx_.d_free, z_, 1, gain_, s);
+  launch_prolong_add(L, g0_, out_, 8, scale_, b, dinv, ctx_.d_free, z_, 1,
+                     gain_ * (double)o_.correction_scale, s, corr_.p);
+  // z.r > 0 whatever the network returns: the correction is shrunk where it
+  // would cancel more than half of the base step's descent
+  launch_descent_guard(ctx_.n3_, b, dinv, corr_, r, z_, ctx_.d_partials, scale_, s);
 }
 
 void LearnedPrecond::apply(const double *r, double *z) {
+  if (mg_base_) ctx_.mg_->refresh_now();
   launch_density_feature(ctx_.lattice(), g0_, ctx_.d_young, feat_, ctx_.stream());
   precondition(r);
   HIPCHK(hipMemcpyAsync(z, z_.p, z_.bytes(), hipMemcpyDeviceToDevice, ctx_.stream()));
@@ -236,10 +264,10 @@ void LearnedPrecond::drop_graphs() {
 void LearnedPrecond::build_graph(int kmode) {
   const int m = kmode + 1;
   // the captured batch bakes in whether the network runs and its gain
-  if (graph_active_ != active_ || graph_gain_ != gain_) {
+  if (graph_active_ != active_ || graph_gain_ != gain_ * (double)o_.correction_scale) {
     drop_graphs();
     graph_active_ = active_;
-    graph_gain_ = gain_;
+    graph_gain_ = gain_ * (double)o_.correction_scale;
   }
   if (!ctx_.use_graphs_ || exec_[m][0]) return;
   hipStream_t s = ctx_.stream();
@@ -262,7 +290,20 @@ void LearnedPrecond::solve(const icarat_solve_opts &o, icarat_solve_stats &st) {
   Context &c = ctx_;
   hipStream_t s = c.stream();
   const Lattice &L = c.lattice();
-  const int kmode = o.precision == ICARAT_HIP_FP64 ? -1 : (o.precision == ICARAT_HIP_FP32 ? 0 : 1);
+  const int kmode = mg_base_ ? -1
+                             : (o.precision == ICARAT_HIP_FP64 ? -1
+                                                               : (o.precision == ICARAT_HIP_FP32 ? 0 : 1));
+  if (mg_base_) {
+    if (!active_) {
+      // without the network the preconditioner is the V-cycle: multigrid PCG
+      ctx_.mg_->solve(o, st);
+      return;
+    }
+    if (o.precision == ICARAT_HIP_BF16)
+      throw Error(kInvalidArgument, "the multigrid base runs an FP64 or an FP32 fine level");
+    ctx_.mg_->set_fine_fp32(o.precision == ICARAT_HIP_FP32);
+    ctx_.mg_->refresh_now();
+  }
   if (!active_) {
     // without the network M = D^-1 is fixed: plain Jacobi-PCG (whose beta
     // keeps the short recurrence exact) instead of the flexible variant
@@ -344,8 +385,24 @@ void LearnedPrecond::solve(const icarat_solve_opts &o, icarat_solve_stats &st) {
   st.refine_steps = repl;
   const bool converged = h->done && h->status == 0;
   if (!converged && (fail || h->status == 2 || h->status == 4) && iters < o.maxit) {
+    // why: 1 the residual stalled (or too many verifications), 2 a breakdown,
+    // 3 the preconditioned residual was not a descent direction (z.r <= 0)
+    st.fallback = fail ? 1 : (h->status == 4 ? 3 : 2);
+    fallback_iter_ = iters;
+    if (mg_base_) {
+      // multigrid PCG from the current iterate
+      icarat_solve_opts fo = o;
+      fo.maxit = o.maxit - iters;
+      icarat_solve_stats fs{};
+      ctx_.mg_->solve(fo, fs, true);
+      iters += fs.iters;
+      st.iters = iters;
+      st.refine_steps = repl + fs.refine_steps;
+      st.rec_relres = fs.rec_relres;
+      st.status = fs.status;
+      return;
+    }
     // Jacobi-PCG from the current iterate
-    st.fallback = 1;
     const PcgScalars r = c.pcg64(o.tol, o.maxit - iters, true, kmode, kmode < 0 ? 0.0 : 0.1);
     iters += r.iter;
     if (r.status == 2) throw Error(kNumerical, "PCG breakdown");

@@ -1,10 +1,12 @@
 // learned.hpp -- the learned preconditioner: a 3-D U-Net on a 2x coarsened
-// cell grid maps (stiffness feature, Jacobi-scaled residual) to the part of
-// the error that Jacobi misses; flexible PCG uses
-//     z = D^-1 r + sigma * P(net(feature, R(D^-1 r) / sigma)),
-// with a Jacobi-PCG fallback.  The network is trained online (Adam, on the
-// project's own forward/backward kernels) on residual/error pairs harvested
-// from the Jacobi-PCG iterates of the first design iterations.
+// cell grid maps (stiffness feature, base-preconditioned residual) to the part
+// of the error that the base preconditioner misses; flexible PCG uses
+//     z = B(r) + sigma * P(net(feature, R(B(r)) / sigma)),
+// B = D^-1 (Jacobi base, Jacobi-PCG fallback) or B = one multigrid V-cycle
+// (multigrid base, multigrid-PCG fallback).  The network is trained online
+// (Adam, on the project's own forward/backward kernels) on pairs harvested from
+// the base method's PCG iterates of the first design iterations: input
+// R(B(r_k)), target R(x* - x_k - B(r_k)) (both over sigma_k = RMS R(B(r_k))).
 #pragma once
 #include <memory>
 #include <vector>
@@ -42,6 +44,19 @@ class LearnedPrecond {
   double last_loss() const { return last_loss_; }
   double val_loss() const { return val_loss_; }
   double gain() const { return gain_; }
+  int fallback_iter() const { return fallback_iter_; }  // iteration of the last fallback
+  // The design loop's usefulness gate: the first solve with the network is a
+  // trial against the iterations of the last harvesting (base-method) solve;
+  // a network that needs more is switched off for the rest of the loop.
+  void note_base_iters(int iters) { base_iters_ = iters; }
+  bool trial_pending() const { return active_ && !trialed_ && base_iters_ > 0; }
+  void trial_result(int iters) {
+    trialed_ = true;
+    trial_iters_ = iters;
+    if (iters > base_iters_) active_ = false;
+  }
+  int trial_iters() const { return trial_iters_; }
+  int trial_base_iters() const { return trialed_ ? base_iters_ : -1; }
   bool active() const { return active_; }
   long long total_steps() const { return total_steps_; }
   // Flexible PCG with the network preconditioner on ctx.d_b -> ctx.d_x.
@@ -52,12 +67,17 @@ class LearnedPrecond {
  private:
   Context &ctx_;
   icarat_learned_opts o_;
+  bool mg_base_ = false;           // base preconditioner: the multigrid V-cycle
+  // B(r) on the device: D^-1 r is applied inside the restriction and the
+  // prolongation (dinv operand); the V-cycle result is passed with ones_
+  const double *base_vec(const double *r, const double **dinv);
   NetGrid g0_;
   std::unique_ptr<UNet> net_;
   long long ncell_;
   DevBuf<float> feat_, s3_, out_;
   DevBuf<__bf16> xin_;
   DevBuf<double> scale_, z_, ones_;
+  DevBuf<double> corr_;  // the network's correction alone (descent guard)
   // sample ring: valid_[k] once slot k holds a finished (input, target) pair
   int cap_ = 0, next_ = 0;
   std::vector<char> valid_;
@@ -77,6 +97,9 @@ class LearnedPrecond {
   bool graph_active_ = false;
   double graph_gain_ = 0.0;
   long long total_steps_ = 0;
+  int fallback_iter_ = -1;
+  int base_iters_ = 0, trial_iters_ = -1;
+  bool trialed_ = false;
   uint64_t rng_;
   // graphs of batches of flexible-PCG iterations [kmode + 1][parity]
   hipGraph_t graph_[3][2] = {};

@@ -19,10 +19,18 @@ void launch_pack_input(const NetGrid &g, const float *feat, const float *s3, con
                        __bf16 *xin, float *zr, hipStream_t s);
 void launch_density_feature(const Lattice &L, const NetGrid &g, const double *young, float *feat,
                             hipStream_t s);
-// z = dinv r + gain sigma P(out) (use_net = 0: z = dinv r)
+// z = dinv r + gain sigma P(out) (use_net = 0: z = dinv r); corr_out (may be
+// null) receives the correction gain sigma P(out) alone
 void launch_prolong_add(const Lattice &L, const NetGrid &g, const float *out, int cs,
                         const double *scale, const double *r, const double *dinv,
-                        const uint8_t *fmask, double *z, int use_net, double gain, hipStream_t s);
+                        const uint8_t *fmask, double *z, int use_net, double gain, hipStream_t s,
+                        double *corr_out = nullptr);
+// Descent guard of z = dinv base + c against the residual r: the correction is
+// shrunk to gamma c (gamma in scale[3]) so that z.r >= (dinv base).r / 2 > 0
+// (gamma = 1 leaves z as it is)
+void launch_descent_guard(long long n, const double *base, const double *dinv, const double *c,
+                          const double *r, double *z, double *partials, double *scale,
+                          hipStream_t s);
 void launch_scale_f32(long long n, float a, float *x, hipStream_t s);
 void launch_make_target(long long n, const float *rxs, const float *rxk, double inv_sigma,
                         const float *zr, float *t, hipStream_t s);

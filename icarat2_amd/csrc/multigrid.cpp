@@ -492,6 +492,20 @@ void Multigrid::build_graph() {
   }
 }
 
+const double *Multigrid::vcycle(const double *r) {
+  return cycle(0, r, true);
+}
+
+void Multigrid::refresh_now() {
+  hipStream_t s = f_.stream();
+  if (refreshed_ != f_.density_version_) {
+    swap_inverse(s);
+    refresh();
+    refreshed_ = f_.density_version_;
+  }
+  swap_inverse(s);  // a side-stream factorization goes in now
+}
+
 void Multigrid::enqueue_restart() {
   Context &c = f_;
   hipStream_t s = c.stream();
@@ -617,6 +631,8 @@ void Multigrid::solve(const icarat_solve_opts &o, icarat_solve_stats &st, bool w
           enqueue_batch(h->iter & 1);
         HIPCHK(hipMemcpyAsync(h, c.d_pcg.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
         c.sync();
+        // a harvesting caller snapshots (x, r, V(r)) at batch boundaries
+        if (c.batch_hook_ && !h->done) c.batch_hook_();
       }
       if (!h->done) throw Error(kInternal, "multigrid PCG did not reach its stop test");
     }

@@ -6,6 +6,7 @@
 // symmetric V-cycle as the preconditioner of (flexible) PCG with the FP64
 // verified stop.
 #pragma once
+#include <algorithm>
 #include <memory>
 #include <vector>
 
@@ -88,6 +89,18 @@ class Multigrid {
   bool fine_fp32() const { return fp32_ && fine_fp32_capable(); }
   // whether the coarsest level is solved exactly (dense inverse)
   bool dense() const { return dense_; }
+  // One V-cycle z = V(r) of the fine lattice vector r, enqueued on the fine
+  // context's stream; returns the device buffer holding z (the V-cycle's
+  // result buffer; the next V-cycle overwrites it).  The coarse operators must
+  // be current (refresh_now).
+  const double *vcycle(const double *r);
+  // the buffer that holds the last V-cycle's result (inside multigrid PCG: the
+  // V-cycle of the current residual at every batch boundary)
+  const double *result() const { return zres_; }
+  // coarse operators and the coarsest inverse of the current densities, in
+  // place before the call returns to the stream's order (no overlapped swap)
+  void refresh_now();
+  int batch_iters() const { return std::max(1, o_.batch_iters); }
   // whether level l smooths with the fused one-thread-per-node kernel
   bool direct(int l) const { return direct_[l] != 0; }
   // coarsest-level Cholesky factorizations so far, and those redone with the

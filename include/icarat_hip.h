@@ -82,7 +82,10 @@ typedef struct icarat_solve_stats {
   double rec_relres;      /* recursive (Krylov) relative residual at exit */
   double true_relres;     /* FP64 ||f - K u|| / ||f|| recomputed by the FP64 kernel */
   double seconds;         /* device wall time of the solve */
-  int32_t fallback;       /* 1 if the learned preconditioner fell back to Jacobi */
+  int32_t fallback;       /* learned preconditioner: 0 none, else it fell back to its base
+                             method (Jacobi-PCG or multigrid PCG) from the current iterate
+                             because the residual stalled (1), the iteration broke down (2)
+                             or z.r <= 0 (3) */
   int32_t status;         /* ICARAT_HIP_OK or ICARAT_HIP_NOT_CONVERGED */
 } icarat_solve_stats;
 
@@ -319,6 +322,12 @@ typedef struct icarat_learned_opts {
   float gate;              /* the network is used only while its mean relative squared error
                               on the stored samples is below gate (else z = D^-1 r) */
   uint64_t seed;           /* weight initialisation and sample order */
+  int32_t base;            /* the preconditioner the network corrects: 0 Jacobi,
+                              z = D^-1 r + s P(net(feature, R(D^-1 r) / s)); 1 the multigrid
+                              V-cycle of icarat_hip_mg_setup, z = V(r) + s P(net(feature,
+                              R(V(r)) / s)), trained on the error the V-cycle leaves, e - V(r),
+                              from multigrid-PCG iterates; the fallback is then multigrid PCG */
+  float correction_scale;  /* the network's correction is added times this (<= 0: 1) */
 } icarat_learned_opts;
 
 typedef struct icarat_learned_info {
@@ -331,14 +340,19 @@ typedef struct icarat_learned_info {
   int64_t n_params;        /* network parameters */
   int32_t grid[3];         /* cell grid X, Y, Z */
   int32_t harvested;       /* harvesting solves finished */
+  int32_t fallback_iter;   /* iteration at which the last solve fell back (-1: none yet) */
+  int32_t trial_iters;     /* simp_run: iterations of the first solve with the network (-1: none) */
+  int32_t trial_base_iters;/* iterations of the last harvesting solve it was held against; a
+                              trial that needs more switches the network off for the loop */
 } icarat_learned_info;
 
-/* Fills *o with the defaults (10, 400, 64, 160, 8, 400, 1e-3, 0.8, 1). */
+/* Fills *o with the defaults (10, 400, 64, 160, 8, 400, 1e-3, 0.8, 1, 0, 1). */
 int icarat_hip_learned_default_opts(icarat_learned_opts *o);
 /* (Re)creates the preconditioner of the context (o = NULL: defaults).  Needs make_dof. */
 int icarat_hip_learned_setup(icarat_hip_ctx *ctx, const icarat_learned_opts *o);
-/* Jacobi-PCG solve of the current density (as icarat_hip_solve with precond =
- * JACOBI) that stores training snapshots; u may be NULL. */
+/* The base method's solve of the current density (as icarat_hip_solve with
+ * precond = JACOBI, or MG for the multigrid base) that stores training
+ * snapshots; u may be NULL. */
 int icarat_hip_learned_harvest(icarat_hip_ctx *ctx, const icarat_solve_opts *opts, double *u,
                                icarat_solve_stats *stats);
 /* Adam steps on the stored samples, then the gate on all samples; *loss = mean
