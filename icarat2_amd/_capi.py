@@ -115,6 +115,7 @@ _SIGS = {
     "icarat_hip_compliance_sensitivity": (C.c_int, [C.c_void_p, C.c_double, dp, dp, dp]),
     "icarat_hip_filter_setup": (C.c_int, [C.c_void_p, C.c_double]),
     "icarat_hip_filter": (C.c_int, [C.c_void_p, dp, dp, C.c_int32]),
+    "icarat_hip_sensitivity_filter": (C.c_int, [C.c_void_p, dp, dp, dp]),
     "icarat_hip_helm_setup": (C.c_int, [C.c_void_p, C.c_double]),
     "icarat_hip_mises": (C.c_int, [C.c_void_p, dp, dp]),
     "icarat_hip_mises_noalloc": (C.c_int, [C.c_void_p, dp, dp]),
@@ -483,6 +484,13 @@ class Context:
         out = np.zeros(self.nelx)
         self._chk(self.L.icarat_hip_filter(self.h, _p(_f64(x)), _p(out), 1 if is_sens else 0),
                   "filter")
+        return out
+
+    def sensitivity_filter(self, x, dfds):
+        """Filter::sensitivityFilter(design_s, dfds) (optimization/filter.hpp:76-104)."""
+        out = np.zeros(self.nelx)
+        self._chk(self.L.icarat_hip_sensitivity_filter(self.h, _p(_f64(x)), _p(_f64(dfds)),
+                                                       _p(out)), "sensitivity_filter")
         return out
 
     def mises(self):

@@ -293,6 +293,12 @@ int icarat_hip_filter(icarat_hip_ctx *h, const double *in, double *out, int32_t 
   return guard(h, [&](icarat_hip::Context &c) { c.filter_host(in, out, is_sens ? 1 : 0); });
 }
 
+int icarat_hip_sensitivity_filter(icarat_hip_ctx *h, const double *x, const double *dfds,
+                                  double *out) {
+  if (!x || !dfds || !out) return fail(h, ICARAT_HIP_INVALID_ARGUMENT, "null argument");
+  return guard(h, [&](icarat_hip::Context &c) { c.sensitivity_filter_host(x, dfds, out); });
+}
+
 int icarat_hip_helm_setup(icarat_hip_ctx *h, double radius) {
   return guard(h, [&](icarat_hip::Context &c) { c.helm_setup(radius); });
 }

@@ -653,6 +653,18 @@ void Context::filter_device(const double *in, double *out, int is_sens) {
   launch_density_filter(L_, ftab_, in, d_e[E_AA], out, is_sens, stream_);
 }
 
+void Context::sensitivity_filter_host(const double *x, const double *dfds, double *out) {
+  if (!filter_ready_) throw Error(kNotReady, "filter_setup has not been called");
+  const long long n = L_.nelx;
+  if (d_sfilt_.n < (size_t)(3 * n)) d_sfilt_.alloc(3 * n);
+  h2d(d_sfilt_.p, x, n, stream_);
+  h2d(d_sfilt_.p + n, dfds, n, stream_);
+  launch_sensitivity_filter(L_, ftab_, d_sfilt_.p, d_sfilt_.p + n, d_sfilt_.p + 2 * n, vol_e_,
+                            stream_);
+  d2h(out, d_sfilt_.p + 2 * n, n, stream_);
+  sync();
+}
+
 void Context::filter_host(const double *in, double *out, int is_sens) {
   h2d(d_e[E_TMP].p, in, L_.nelx, stream_);
   filter_device(d_e[E_TMP], d_e[E_XF], is_sens);

@@ -73,6 +73,8 @@ class Context {
   void filter_setup(double radius);
   void filter_device(const double *in, double *out, int is_sens);  // device pointers
   void filter_host(const double *in, double *out, int is_sens);
+  // Filter::sensitivityFilter on host arrays (x, dfds -> out, nelx each)
+  void sensitivity_filter_host(const double *x, const double *dfds, double *out);
   void threshold_host(const double *in, double beta, double T, double *out, int deriv);
   // Helmholtz PDE filter (FilterHelmholtz): set-up and both modes on host vectors
   void helm_setup(double radius);
@@ -151,6 +153,7 @@ class Context {
   bool solved_ = false;
   bool fext_dirty_ = false;
   double vol_e_ = 1.0;
+  DevBuf<double> d_sfilt_;  // 3 x nelx: x, dfds, out of the sensitivity filter
   std::string last_error;
 
   // PCG iteration batch captured as a graph, per contraction mode (FP64 VALU,

@@ -42,7 +42,7 @@ __global__ __launch_bounds__(NT) void k_filter(Lattice L, int count, const int *
                                                const double *__restrict__ c_fw,
                                                const double *__restrict__ in,
                                                double *__restrict__ aa, double *__restrict__ out,
-                                               int is_sens) {
+                                               int is_sens, double vol) {
 #pragma clang fp contract(off)
   using T = Tile<R>;
   __shared__ double sv[T::HN];
@@ -72,7 +72,7 @@ __global__ __launch_bounds__(NT) void k_filter(Lattice L, int count, const int *
       }
       out[e] = b / a;
       aa[e] = a;
-    } else {
+    } else if (is_sens == 1) {
       double s = 0.0;
       for (int t = 0; t < count; ++t) {
         const int h = base + c_flin[t];
@@ -80,6 +80,18 @@ __global__ __launch_bounds__(NT) void k_filter(Lattice L, int count, const int *
         s += c_fw[t] * sv[h] / sa[h];
       }
       out[e] = s;
+    } else {
+      // sensitivity filter: sv = densities, sa = sensitivities
+      double a = 0.0, b = 0.0;
+      for (int t = 0; t < count; ++t) {
+        const int h = base + c_flin[t];
+        if (!sm[h]) continue;
+        const double w = c_fw[t];
+        a += w;
+        b += (w * sv[h] * sa[h]) / (sv[h] * vol);
+      }
+      const double xi = sv[base];
+      out[e] = ((xi * vol) / (xi * a)) * b;
     }
   }
 }
@@ -229,7 +241,15 @@ void filter_lds_offsets(const int *off, int count, int R, int *lin) {
 void launch_density_filter(const Lattice &L, const FilterTable &T, const double *in, double *aa,
                            double *out, int is_sens, hipStream_t s) {
   FILTER_DISPATCH(T.R, hipLaunchKernelGGL(k_filter<RR>, filter_grid(L), dim3(NT), 0, s, L,
-                                          T.count, T.lin, T.w, in, aa, out, is_sens));
+                                          T.count, T.lin, T.w, in, aa, out, is_sens, 0.0));
+  HIPCHK(hipGetLastError());
+}
+
+void launch_sensitivity_filter(const Lattice &L, const FilterTable &T, const double *x,
+                               const double *dfds, double *out, double vol_e, hipStream_t s) {
+  FILTER_DISPATCH(T.R, hipLaunchKernelGGL(k_filter<RR>, filter_grid(L), dim3(NT), 0, s, L,
+                                          T.count, T.lin, T.w, x, const_cast<double *>(dfds),
+                                          out, 2, vol_e));
   HIPCHK(hipGetLastError());
 }
 

@@ -153,6 +153,10 @@ void filter_lds_offsets(const int *off, int count, int R, int *lin);
 // is_sens == 1: out = sum w v / aa_j
 void launch_density_filter(const Lattice &L, const FilterTable &T, const double *in,
                            double *aa, double *out, int is_sens, hipStream_t s);
+// Sigmund's sensitivity filter: out_i = (x_i V)/(x_i sum_j w_ij) *
+// sum_j (w_ij x_j dfds_j)/(x_j V), neighbours in ascending element id
+void launch_sensitivity_filter(const Lattice &L, const FilterTable &T, const double *x,
+                               const double *dfds, double *out, double vol_e, hipStream_t s);
 void launch_threshold(long long n, const double *x, double beta, double T, double *out,
                       int deriv, hipStream_t s);
 void launch_mul(long long n, const double *a, double *b, hipStream_t s);

@@ -202,6 +202,12 @@ int icarat_hip_mises_noalloc(icarat_hip_ctx *ctx, double *mises, double *stress)
 int icarat_hip_filter_setup(icarat_hip_ctx *ctx, double radius);
 /* Filter::densityFilter(in, is_sens) (optimization/filter.hpp:106-155). */
 int icarat_hip_filter(icarat_hip_ctx *ctx, const double *in, double *out, int32_t is_sens);
+/* Filter::sensitivityFilter(design_s, dfds) (optimization/filter.hpp:76-104):
+ * Sigmund's sensitivity filter over the cone neighbours of icarat_hip_filter_setup,
+ * out_i = (x_i V)/(x_i sum_j w_ij) * sum_j (w_ij x_j dfds_j)/(x_j V); x, dfds, out
+ * hold nelx values (host memory).  x must be nonzero (the reference divides by it). */
+int icarat_hip_sensitivity_filter(icarat_hip_ctx *ctx, const double *x, const double *dfds,
+                                  double *out);
 /* Filter::threshold (deriv = 0) / thresholdDerivative (deriv = 1)
  * (optimization/filter.hpp:216-248). */
 int icarat_hip_threshold(icarat_hip_ctx *ctx, const double *in, double beta, double T,

@@ -1569,6 +1569,27 @@ void orc_filter_lists(void *f, long long *rowptr, int *ids, double *dis) {
 int orc_density_filter(void *f, const double *in, double *out, int isSens) {
   return density_filter(*(Filter *)f, in, out, isSens);
 }
+
+// Filter::sensitivityFilter(design_s, dfds) (reference optimization/filter.hpp:
+// 76-104): aa = sum_j w_ij, bb = sum_j (w_ij x_j dfds_j) / (x_j V_j), out_i =
+// (x_i V_i) / (x_i aa) * bb, neighbours in ascending element id, the reference's
+// operation order.
+void orc_sensitivity_filter(void *f, const double *x, const double *dfds, double *out) {
+  Filter &F = *(Filter *)f;
+  const double V = F.P->vol_e;
+#pragma omp parallel for schedule(static)
+  for (i64 i = 0; i < F.n; ++i) {
+    double aa = 0.0, bb = 0.0;
+    for (i64 t = F.rowptr[i]; t < F.rowptr[i + 1]; ++t) {
+      const int j = F.ids[t];
+      const double w = F.radius - F.dis[t];
+      aa += w;
+      bb += (w * x[j] * dfds[j]) / (x[j] * V);
+    }
+    const double cc = (x[i] * V) / (x[i] * aa);
+    out[i] = cc * bb;
+  }
+}
 void orc_threshold(long long n, const double *x, double beta, double T, double *out) {
   threshold(n, x, beta, T, out);
 }

@@ -97,6 +97,7 @@ def lib():
             "orc_helm_create": (vp, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                      C.c_double, C.c_double]),
             "orc_helm_destroy": (None, [vp]), "orc_helm_numnp": (C.c_longlong, [vp]),
+            "orc_sensitivity_filter": (None, [vp, dp, dp, dp]),
             "orc_helm_element": (None, [vp, C.c_longlong, dp, ip]),
             "orc_helm_apply": (None, [vp, dp, dp]),
             "orc_helm_filter": (C.c_int, [vp, dp, dp, C.c_int, C.c_double, C.c_int]),
@@ -347,6 +348,14 @@ class Filter:
         rc = self.L.orc_density_filter(self.h, P(x), P(out), 1 if is_sens else 0)
         if rc != 0:
             raise RuntimeError("coefficient aa is too small in the density filter")
+        return out
+
+    def sensitivity_filter(self, x, dfds):
+        """Filter::sensitivityFilter(design_s, dfds) (filter.hpp:76-104)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        d = np.ascontiguousarray(dfds, dtype=np.float64)
+        out = np.zeros(self.prob.nelx)
+        self.L.orc_sensitivity_filter(self.h, P(x), P(d), P(out))
         return out
 
     def oc_update(self, x, dfdx, g, dgdx, xmin=1e-4, xmax=1.0, use_projection=False, beta=1.0):

@@ -169,6 +169,29 @@ def test_filter_reference_kat_gpu(gpu):
     assert np.sum((c.threshold(sf, 1.0, deriv=True) - 1.0) ** 2) < 0.015
 
 
+def test_sensitivity_filter_gpu(gpu, oracle):
+    """Filter::sensitivityFilter: the KAT of test_filter.cpp:63-89, then bit
+    equality with the oracle on a 40x20x10 lattice with random densities."""
+    c = gpu.Context(2, 2, 1)
+    c.make_dof()
+    c.filter_setup(2.0)
+    answer = np.array([0.34587, 0.65423, 0.34587, 0.65423])
+    out = c.sensitivity_filter(np.full(4, 0.5), np.array([1e-4, 1.0, 1e-4, 1.0]))
+    assert np.sum((out - answer) ** 2) < 1e-10
+    dims = (40, 20, 10)
+    rng = np.random.default_rng(23)
+    for radius in (1.5, 2.5):
+        c = gpu.Context(*dims, 40.0, 20.0, 10.0)
+        c.make_dof()
+        c.filter_setup(radius)
+        o = oracle.Problem(*dims, 40.0, 20.0, 10.0)
+        o.make_dof()
+        F = oracle.Filter(o, radius)
+        x = rng.uniform(1e-3, 1.0, c.nelx)
+        dfds = -rng.uniform(0.0, 3.0, c.nelx)
+        assert np.array_equal(c.sensitivity_filter(x, dfds), F.sensitivity_filter(x, dfds))
+
+
 def test_threshold_matches_oracle(gpu, oracle):
     c = gpu.Context(20, 10, 5)
     c.make_dof()

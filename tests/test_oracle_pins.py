@@ -349,6 +349,41 @@ def test_filter_reference_kat(oracle):
     assert np.sum((d1 - 1.0) ** 2) < 0.015
 
 
+def test_sensitivity_filter_reference_kat(oracle):
+    """test_filter.cpp:63-89 (Filter::sensitivityFilter) restated on the 2x2x1
+    slab: s = 0.5 everywhere, dfds = [1e-4, 1, 1e-4, 1], radius 2."""
+    pr = oracle.Problem(2, 2, 1)
+    pr.make_dof()
+    F = oracle.Filter(pr, 2.0, brute=True)
+    s = np.full(4, 0.5)
+    dfds = np.array([1e-4, 1.0, 1e-4, 1.0])
+    answer = np.array([0.34587, 0.65423, 0.34587, 0.65423])
+    out = F.sensitivity_filter(s, dfds)
+    assert np.sum((out - answer) ** 2) < 1e-10
+    # closed form: with uniform densities the filter is the cone average of dfds
+    w = np.array([2.0, 1.0, 1.0, 2.0 - np.sqrt(2.0)])
+    exact0 = (w[0] * 1e-4 + w[1] * 1.0 + w[2] * 1e-4 + w[3] * 1.0) / w.sum()
+    assert abs(out[0] - exact0) < 1e-15
+
+
+@pytest.mark.parametrize("radius", [1.5, 2.5])
+def test_sensitivity_filter_cancels_densities(oracle, radius):
+    """filter.hpp:93-100 divides each term by x_j V and multiplies the sum by
+    x_i V / (x_i sum w): the densities cancel, so for any positive x the result
+    equals the cone average sum w dfds / sum w (the density filter of dfds) up
+    to rounding.  A term with x_i for x_j (or a missing V) fails this."""
+    pr = oracle.Problem(9, 6, 5, 18.0, 12.0, 10.0)
+    pr.make_dof()
+    F = oracle.Filter(pr, radius * 2.0)
+    rng = np.random.default_rng(5)
+    x = rng.uniform(1e-3, 1.0, pr.nelx)
+    dfds = -rng.uniform(0.0, 3.0, pr.nelx)
+    out = F.sensitivity_filter(x, dfds)
+    avg = F.density_filter(dfds, False)
+    assert np.max(np.abs(out - avg)) <= 1e-14 * np.max(np.abs(avg))
+    assert not np.array_equal(out, dfds)
+
+
 @pytest.mark.parametrize("dims,h", [((7, 5, 4), 1.0), ((6, 4, 5), 0.5)])
 @pytest.mark.parametrize("radius", [1.2, 1.5, 2.0, 2.5])
 def test_structured_neighbours_equal_bruteforce(oracle, dims, h, radius):
