@@ -88,6 +88,9 @@ void Context::ensure_device() {
   h_pcg.alloc(1);
   d_scal.alloc(16);
   h_scal.alloc(16);
+  static_assert(sizeof(MmaDualState) % sizeof(double) == 0, "MmaDualState size");
+  d_mmadual_.alloc(sizeof(MmaDualState) / sizeof(double));
+  h_mmadual_.alloc(sizeof(MmaDualState) / sizeof(double));
   d_oc.alloc(1);
   h_oc.alloc(1);
   sync();

@@ -11,6 +11,7 @@
 #include "../../include/icarat_hip.h"
 #include "hip_util.hpp"
 #include "kernels.hpp"
+#include "mma_kernels.hpp"
 #include "problem.hpp"
 
 namespace icarat_hip {
@@ -141,6 +142,9 @@ class Context {
   PinnedBuf<PcgScalars> h_pcg;
   DevBuf<double> d_scal;              // small reduction results
   PinnedBuf<double> h_scal;
+  // state of the MMA / GCMMA dual iteration (device) and its host copy
+  MmaDualState *mma_dual_state() { return (MmaDualState *)d_mmadual_.p; }
+  MmaDualState *mma_dual_state_host() { return (MmaDualState *)h_mmadual_.p; }
   DevBuf<double> d_e[8];              // element work vectors
   DevBuf<OcState> d_oc;
   PinnedBuf<OcState> h_oc;
@@ -153,6 +157,8 @@ class Context {
   bool solved_ = false;
   bool fext_dirty_ = false;
   double vol_e_ = 1.0;
+  DevBuf<double> d_mmadual_;
+  PinnedBuf<double> h_mmadual_;
   DevBuf<double> d_sfilt_;  // 3 x nelx: x, dfds, out of the sensitivity filter
   std::string last_error;
 

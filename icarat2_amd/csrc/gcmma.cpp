@@ -57,7 +57,7 @@ void DeviceGCMMA::gensub(const double *xval, double f0x, const double *df0dx, do
 void DeviceGCMMA::solve_and_approx(double *xmma, const double *xval) {
   hipStream_t s = ctx_.stream();
   HIPCHK(hipMemcpyAsync(xmma, xval, n_ * sizeof(double), hipMemcpyDeviceToDevice, s));
-  solve_dual_m1(ctx_, n_, A_, xmma, b_, a_, c_, nullptr);
+  solve_dual_m1(ctx_, n_, A_, xmma, b_, nullptr, a_, c_, nullptr);
   double *sc = ctx_.d_scal.p + 4;
   double *hs = ctx_.h_scal.p + 4;
   launch_gcmma_approx(n_, xmma, A_, ctx_.d_partials, sc, s);

@@ -107,6 +107,120 @@ __global__ __launch_bounds__(NT) void k_mma_dual(long long n, double lam, MmaArr
   store_partials<NT, 2>(v, partials, sh);
 }
 
+// k_mma_dual at the lambda of a device state; no work once the state is done
+__global__ __launch_bounds__(NT) void k_mma_dual_dev(long long n, const MmaDualState *st,
+                                                     MmaArrays A, double *__restrict__ x,
+                                                     double *partials) {
+  __shared__ double sh[8];
+  if (st->done) return;
+  const double lam = st->lam;
+  double sg = 0.0, shs = 0.0;
+  const long long stride = (long long)gridDim.x * NT;
+  for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += stride) {
+    const double lo = A.low[i], up = A.upp[i];
+    const double pl = A.p0[i] + A.p1[i] * lam;
+    const double ql = A.q0[i] + A.q1[i] * lam;
+    const double sp = sqrt(pl), sq = sqrt(ql);
+    const double xp = (sp * lo + sq * up) / (sp + sq);
+    double xi = xp < A.alpha[i] ? A.alpha[i] : xp;
+    xi = xi > A.beta[i] ? A.beta[i] : xi;
+    x[i] = xi;
+    const double ux = up - xi, xl = xi - lo;
+    sg += A.p1[i] / ux + A.q1[i] / xl;
+    const double pq = A.p1[i] / (ux * ux) - A.q1[i] / (xl * xl);
+    double df2 = -1.0 / (2.0 * pl / (ux * ux * ux) + 2.0 * ql / (xl * xl * xl));
+    if (xp < A.alpha[i] || xp > A.beta[i]) df2 = 0.0;
+    shs += pq * df2 * pq;
+  }
+  double v[2] = {sg, shs};
+  store_partials<NT, 2>(v, partials, sh);
+}
+
+__device__ __forceinline__ double dmax(double a, double b) { return a < b ? b : a; }
+
+__global__ void k_mma_dual_init(MmaDualState *st, double b0, const double *bsum, double a,
+                                double c, double epsimin) {
+#pragma clang fp contract(off)
+  MmaDualState S{};
+  S.b = bsum ? b0 + bsum[0] : b0;
+  S.a = a;
+  S.c = c;
+  S.epsimin = epsimin;
+  S.lam = c / 2.0;
+  S.mu = 1.0;
+  S.epsi = 1.0;
+  S.err = 1.0;
+  *st = S;
+}
+
+// SolveDIP's scalar part (reference MMASolver.cpp:88-154 with the Newton step
+// of :156-236, GCMMASolver.cpp:128-196), m = 1, in the host loop's order:
+//   pass(lam); while (epsi > epsimin && outer < 100) { ++outer; loop = 0;
+//     while (err > 0.9 epsi && loop < 100) { Newton step; pass(lam); err }
+//     epsi *= 0.1 }
+// Each launch consumes one pass: phase 0 = the first pass, 1 = a pass after a
+// Newton step (err is updated from it).
+__global__ __launch_bounds__(NT) void k_mma_newton(const double *partials, unsigned G,
+                                                   MmaDualState *st) {
+#pragma clang fp contract(off)
+  __shared__ double sh[8];
+  if (st->done) return;
+  double tot[2];
+  sum_partials<NT, 2>(partials, G, tot, sh);
+  if (threadIdx.x != 0) return;
+  MmaDualState S = *st;
+  const double Sg = tot[0], Sh = tot[1];
+  const double a = S.a, c = S.c, b = S.b;
+  if (S.phase == 1) {
+    const double y = dmax(0.0, S.lam - c);
+    const double z = dmax(0.0, 10.0 * (S.lam * a - 1.0));
+    const double r1 = -b - a * z - y + S.mu + Sg;
+    const double r2 = S.mu * S.lam - S.epsi;
+    S.err = dmax(fabs(r1), fabs(r2));
+  }
+  for (;;) {
+    if (S.phase == 1) {
+      if (S.err > 0.9 * S.epsi && S.loop < 100) {
+        ++S.loop;
+        ++S.steps;
+        double lam = dmax(S.lam, 0.0);
+        const double mu = S.mu, epsi = S.epsi;
+        const double y = dmax(0.0, lam - c);
+        const double z = dmax(0.0, 10.0 * (lam * a - 1.0));
+        double grad = -b - a * z - y + Sg;
+        grad = -1.0 * grad - epsi / lam;
+        double hess = Sh;
+        if (lam > c) hess += -1.0;
+        hess += -mu / lam;
+        if (lam * a > 0.0) hess += -10.0 * a * a;
+        double corr = 1e-4 * hess / 1.0;
+        if (-1.0 * corr < 1.0e-7) corr = -1.0e-7;
+        hess += corr;
+        const double s0 = grad / hess;
+        const double s1 = -mu + epsi / lam - s0 * mu / lam;
+        double th = 1.005;
+        th = dmax(th, -1.01 * s0 / lam);
+        th = dmax(th, -1.01 * s1 / mu);
+        th = 1.0 / th;
+        S.lam = lam + th * s0;
+        S.mu = mu + th * s1;
+        break;  // the next launch evaluates x(lambda)
+      }
+      S.epsi *= 0.1;
+      S.phase = 2;
+    }
+    if (S.epsi > S.epsimin && S.outer < 100) {
+      ++S.outer;
+      S.loop = 0;
+      S.phase = 1;
+      continue;
+    }
+    S.done = 1;
+    break;
+  }
+  *st = S;
+}
+
 // GCMMA subproblem (reference GCMMASolver.cpp:486-560 Asymp when asymp != 0,
 // then :593-640 GenSub): MMA's asymptotes and history shift, move limits
 // without a second clamp to the bounds, raa0 in the objective and raa1 in the
@@ -244,6 +358,20 @@ void launch_mma_dual(long long n, double lam, const MmaArrays &A, double *x, dou
                      out);
   HIPCHK(hipGetLastError());
   launch_finalize(kFinOut, 2, partials, BLOCKS, nullptr, nullptr, out, s);
+}
+
+void launch_mma_dual_init(MmaDualState *st, double b0, const double *bsum, double a, double c,
+                          double epsimin, hipStream_t s) {
+  hipLaunchKernelGGL(k_mma_dual_init, dim3(1), dim3(1), 0, s, st, b0, bsum, a, c, epsimin);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_mma_dual_step(long long n, const MmaArrays &A, double *x, double *partials,
+                          MmaDualState *st, hipStream_t s) {
+  hipLaunchKernelGGL(k_mma_dual_dev, dim3(BLOCKS), dim3(NT), 0, s, n, st, A, x, partials);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_mma_newton, dim3(1), dim3(NT), 0, s, partials, (unsigned)BLOCKS, st);
+  HIPCHK(hipGetLastError());
 }
 
 }  // namespace icarat_hip

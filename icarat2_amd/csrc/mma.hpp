@@ -1,5 +1,5 @@
 // mma.hpp -- MMA design update with one (volume) constraint on the device:
-// element passes on the GPU, the 1x1 dual Newton system on the host, all FP64.
+// element passes and the 1x1 dual Newton system on the GPU, all FP64.
 #pragma once
 #include "hip_util.hpp"
 #include "mma_kernels.hpp"
@@ -10,11 +10,12 @@ class Context;
 
 // The dual interior-point iteration of MMA / GCMMA for one constraint
 // (SolveDIP, reference MMASolver.cpp:88-154, GCMMASolver.cpp:128-196): the
-// element passes on the device (x(lambda) and the dual sums), the 1x1 Newton
-// system on the host.  b is the constraint constant of the subproblem; x
-// receives x(lambda*).  Returns the outer (epsilon) loops.
-int solve_dual_m1(Context &ctx, long long n, const MmaArrays &A, double *x, double b, double a,
-                  double c, int *newton_steps);
+// element passes (x(lambda) and the dual sums) and the 1x1 Newton system, both
+// on the device.  The constraint constant of the subproblem is b = b0, plus
+// bsum[0] (device) when bsum is given; x receives x(lambda*).  Returns the
+// outer (epsilon) loops.
+int solve_dual_m1(Context &ctx, long long n, const MmaArrays &A, double *x, double b0,
+                  const double *bsum, double a, double c, int *newton_steps);
 
 class DeviceMMA {
  public:

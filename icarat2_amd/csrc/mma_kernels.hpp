@@ -28,6 +28,22 @@ void launch_mma_gensub(long long n, const double *x, const double *dfdx, const d
 void launch_mma_dual(long long n, double lam, const MmaArrays &A, double *x, double *partials,
                      unsigned *counter, double *out, hipStream_t s);
 
+// State of the dual interior-point iteration for one constraint, kept in
+// device memory so that the Newton steps run without host round trips.
+struct MmaDualState {
+  double lam, mu, epsi, err, b, a, c, epsimin;
+  int outer, loop, steps, phase, done, pad;
+};
+// st = the starting point lambda = c/2, mu = 1, epsi = err = 1 with
+// b = b0 (+ bsum[0] when bsum is given: the subproblem's sum on the device)
+void launch_mma_dual_init(MmaDualState *st, double b0, const double *bsum, double a, double c,
+                          double epsimin, hipStream_t s);
+// One evaluation x(lambda), Sg, Sh at the state's lambda, then the scalar
+// Newton logic up to the next lambda that needs an evaluation (or done).
+// Both kernels return at once when the state is done.
+void launch_mma_dual_step(long long n, const MmaArrays &A, double *x, double *partials,
+                          MmaDualState *st, hipStream_t s);
+
 // GCMMA (reference optimization/opt-solver/GCMMASolver.cpp) for one constraint:
 // the subproblem at x (asymp != 0: move the asymptotes and shift the history
 // first, as OuterUpdate does); out[0] = sum p0/(U-x) + q0/(x-L), out[1] = the
