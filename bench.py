@@ -276,6 +276,8 @@ def run_batch(A, a):
         "pcg_iters_per_design_iter": iters,
         "graph_launches": b.info()[1],
     }
+    if not a.no_cases:
+        out["load_cases_together"] = cases_leg(A, specs, a.tol)
     if pre != 0 and not a.no_plain:
         (xp, hp, sp), dtp = timed(0)
         out["plain_fp64"] = {
@@ -284,6 +286,44 @@ def run_batch(A, a):
             "design_max_abs_diff": float(max(np.max(np.abs(x1 - x2)) for x1, x2 in zip(xs, xp)))}
         out["speedup_vs_plain_fp64"] = out["value"] / out["plain_fp64"]["value"]
     return out
+
+
+def cases_leg(A, specs, tol):
+    """One FP64 Jacobi-PCG solve of every load case of each multi-case problem
+    (initial design): all cases in one multi-column iteration
+    (icarat_hip_solve_cases) against one case after another; iteration counts
+    must agree."""
+    from icarat2_amd import workloads as W
+    rows, t_all, t_seq, t_auto = [], 0.0, 0.0, 0.0
+    for s in specs:
+        if len(s["loads"]) < 2:
+            continue
+        c = W.build_context(A, s)
+        c.set_density(c.density_filter(W.initial_design(s)))
+        c.solve_cases(tol=tol, maxit=100000, want_u=False)  # graphs built
+        t0 = time.perf_counter()
+        _, st = c.solve_cases(tol=tol, maxit=100000, want_u=False)
+        t1 = time.perf_counter()
+        its = []
+        for lc in range(c.load_case_count()):
+            c.select_load_case(lc)
+            _, s1 = c.solve(tol=tol, maxit=100000, want_u=False)
+            its.append(s1.iters)
+        t2 = time.perf_counter()
+        if [x.iters for x in st] != its:
+            raise RuntimeError("load cases together: iteration counts differ from single solves")
+        rows.append({"dims": "x".join(map(str, s["dims"])), "cases": len(its),
+                     "together_ms": (t1 - t0) * 1e3, "one_after_another_ms": (t2 - t1) * 1e3})
+        t_all += t1 - t0
+        t_seq += t2 - t1
+        # simp_run's automatic mode: together up to 600,000 elements
+        t_auto += (t1 - t0) if c.nelx <= 600000 else (t2 - t1)
+    log(f"[bench] load cases together {t_all:.3f} s, one after another {t_seq:.3f} s")
+    return {"method": "fp64 Jacobi-PCG, one solve of each multi-case problem's cases",
+            "together_s": t_all, "one_after_another_s": t_seq,
+            "speedup": t_seq / t_all if t_all > 0 else None,
+            "automatic_s": t_auto, "automatic_speedup": t_seq / t_auto if t_auto > 0 else None,
+            "problems": rows}
 
 
 def precision_leg(A, cfg, method, tol, maxit, ref):
@@ -334,6 +374,8 @@ def main():
                          "(filter_helm.hpp)")
     ap.add_argument("--budget", type=int, default=None,
                     help="c5: admission budget in elements (0: all problems at once)")
+    ap.add_argument("--no-cases", action="store_true",
+                    help="c5: skip the load-cases-together leg")
     ap.add_argument("--roofline-only", action="store_true",
                     help="time only the dominant, vector and copy kernels on the first design "
                          "(for profiler passes); prints their figures, not the contract line")

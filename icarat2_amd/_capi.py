@@ -48,7 +48,7 @@ class SimpOpts(C.Structure):
     _fields_ = [("n_iter", C.c_int32), ("optimizer", C.c_int32), ("use_projection", C.c_int32),
                 ("filter", C.c_int32), ("xmin", C.c_double), ("xmax", C.c_double),
                 ("solve", SolveOpts), ("conv_eps", C.c_double), ("conv_minstep", C.c_int32),
-                ("history_csv", C.c_char_p)]
+                ("history_csv", C.c_char_p), ("case_mode", C.c_int32)]
 
 
 class SimpStats(C.Structure):
@@ -115,6 +115,7 @@ _SIGS = {
     "icarat_hip_compliance_sensitivity": (C.c_int, [C.c_void_p, C.c_double, dp, dp, dp]),
     "icarat_hip_filter_setup": (C.c_int, [C.c_void_p, C.c_double]),
     "icarat_hip_filter": (C.c_int, [C.c_void_p, dp, dp, C.c_int32]),
+    "icarat_hip_solve_cases": (C.c_int, [C.c_void_p, C.c_void_p, dp, C.c_void_p]),
     "icarat_hip_sensitivity_filter": (C.c_int, [C.c_void_p, dp, dp, dp]),
     "icarat_hip_helm_setup": (C.c_int, [C.c_void_p, C.c_double]),
     "icarat_hip_mises": (C.c_int, [C.c_void_p, dp, dp]),
@@ -463,6 +464,19 @@ class Context:
             self._chk(rc, "solve")
         return u, st
 
+    def solve_cases(self, tol=1e-8, maxit=10000, precision=FP64, precond=PRECOND_JACOBI,
+                    want_u=True, allow_cap=False, warm=False):
+        """Every load case (icarat_hip_solve_cases): (ncase x numeq solutions or
+        None, list of per-case stats)."""
+        nc = self.load_case_count()
+        o = SolveOpts(tol, maxit, precision, precond, 0.1, 200, 1 if warm else 0)
+        st = (SolveStats * nc)()
+        u = np.zeros((nc, self.numeq)) if want_u else None
+        rc = self.L.icarat_hip_solve_cases(self.h, C.byref(o), None if u is None else _p(u), st)
+        if rc and not (allow_cap and rc == NOT_CONVERGED):
+            self._chk(rc, "solve_cases")
+        return u, list(st)
+
     def true_residual(self, u):
         r = C.c_double()
         self._chk(self.L.icarat_hip_true_residual(self.h, _p(_f64(u)), C.byref(r)), "true_residual")
@@ -651,14 +665,16 @@ class Context:
     def simp_run(self, x0, n_iter, optimizer=0, use_projection=False, xmin=1e-4, xmax=1.0,
                  tol=1e-8, maxit=10000, precision=FP64, precond=PRECOND_JACOBI, inner_tol=0.1,
                  max_refine=200, warm=False, helmholtz=False, conv_eps=0.0, conv_minstep=0,
-                 history_csv=None):
+                 history_csv=None, cases_together=None):
         """The design loop; with conv_eps > 0 it may end before n_iter
-        (isConvergence): hist then has st.steps rows."""
+        (isConvergence): hist then has st.steps rows.  cases_together: None
+        automatic, True together, False one load case after another."""
         o = SimpOpts(n_iter, optimizer, 1 if use_projection else 0, 1 if helmholtz else 0,
                      xmin, xmax,
                      SolveOpts(tol, maxit, precision, precond, inner_tol, max_refine,
                                1 if warm else 0), conv_eps, conv_minstep,
-                     None if history_csv is None else os.fsencode(history_csv))
+                     None if history_csv is None else os.fsencode(history_csv),
+                     0 if cases_together is None else (2 if cases_together else 1))
         x = np.array(x0, dtype=np.float64)
         hist = np.zeros(4 * max(n_iter, 1))
         st = SimpStats()

@@ -2,6 +2,7 @@
 // exception-to-status translation and the per-context error text.
 #include <cstring>
 #include <new>
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -212,6 +213,24 @@ int icarat_hip_apply_K(icarat_hip_ctx *h, const double *u, double *y, int32_t pr
 int icarat_hip_get_diag(icarat_hip_ctx *h, double *d) {
   if (!d) return fail(h, ICARAT_HIP_INVALID_ARGUMENT, "null argument");
   return guard(h, [&](icarat_hip::Context &c) { c.get_diag(d); });
+}
+
+int icarat_hip_solve_cases(icarat_hip_ctx *h, const icarat_solve_opts *o, double *u,
+                           icarat_solve_stats *st) {
+  if (!o) return fail(h, ICARAT_HIP_INVALID_ARGUMENT, "null options");
+  std::vector<icarat_solve_stats> s;
+  int rc = guard(h, [&](icarat_hip::Context &c) {
+    s.assign(c.problem().num_load_cases(), icarat_solve_stats{});
+    c.solve_cases(*o, u, s.data());
+  });
+  if (st) std::copy(s.begin(), s.end(), st);
+  if (rc == ICARAT_HIP_OK)
+    for (const auto &x : s)
+      if (x.status != ICARAT_HIP_OK) {
+        h->err = "iteration cap reached before the tolerance";
+        return x.status;
+      }
+  return rc;
 }
 
 int icarat_hip_solve(icarat_hip_ctx *h, const icarat_solve_opts *o, double *u,

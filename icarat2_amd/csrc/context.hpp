@@ -58,6 +58,15 @@ class Context {
   // FP64 / mixed solve of K u = fext; result kept on device (d_u, nodal with
   // Dirichlet values); optional numeq copy to host.
   void solve(const icarat_solve_opts &o, double *u_host, icarat_solve_stats *st);
+  // Every load case solved together (FP64 Jacobi-PCG): one multi-column
+  // iteration, each case with its own scalars and verified stop, the same
+  // arithmetic as a solve of that case alone.  u_host: numeq values per case
+  // (may be NULL); st[ncase] (may be NULL).  Afterwards d_u, d_x hold the
+  // active case's solution.
+  void solve_cases(const icarat_solve_opts &o, double *u_host, icarat_solve_stats *st);
+  // o can be solved by one multi-column iteration (FP64 Jacobi, z-marching K.u)
+  bool cases_together(const icarat_solve_opts &o) const;
+  static constexpr long long kCasesTogetherMaxElements = 600000;
   double true_residual_device();          // ||f - K d_u|| / ||f||
   double true_residual_host(const double *u);
   void compliance_sensitivity(double vnorm, double *c, double *dfds, double *dgds);
@@ -255,6 +264,24 @@ class Context {
   void require_density();
   void build_pcg_graph(int kmode);
   void launch_pcg_batch(int kmode, int par);   // as a graph, or enqueued directly
+  // load cases together: stacked vectors (case c at c * n3, p at c * 2 n3),
+  // per-case scalars, partials and the captured batch
+  struct Cases {
+    int nc = 0;
+    DevBuf<double> X, R, Q, P, B, pst, pup;
+    DevBuf<PcgScalars> S;
+    PinnedBuf<PcgScalars> hS;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+  } mc_;
+  void mc_alloc(int nc);
+  void mc_enqueue_batch();
+  void mc_drop_graph();
+  // the device part of solve_cases; per-case stats without true residuals
+  void mc_solve(const icarat_solve_opts &o, icarat_solve_stats *st);
+  // d_x, d_u <- case c of the stacked solution, d_f <- its load
+  void mc_take_case(int c);
+  void simp_cases_together(const icarat_solve_opts &so);
 };
 
 }  // namespace icarat_hip

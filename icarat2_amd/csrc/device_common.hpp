@@ -103,15 +103,22 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double *sh /* NV*NT/6
 // Every block stores its partial sums (plain stores, no fences); a separate
 // single-block finalize kernel sums them in block order after the kernel
 // boundary has made them visible.  Deterministic for a fixed grid.
+// (store_partials_at: the block's partials go to slot bid of G per quantity)
 template <int NT, int NV>
-__device__ __forceinline__ void store_partials(double (&v)[NV], double *partials, double *sh) {
-  const unsigned G = gridDim.x * gridDim.y * gridDim.z;
-  const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+__device__ __forceinline__ void store_partials_at(double (&v)[NV], double *partials, double *sh,
+                                                  unsigned G, unsigned bid) {
   block_sum<NT, NV>(v, sh);
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) partials[(size_t)k * G + bid] = v[k];
   }
+}
+
+template <int NT, int NV>
+__device__ __forceinline__ void store_partials(double (&v)[NV], double *partials, double *sh) {
+  const unsigned G = gridDim.x * gridDim.y * gridDim.z;
+  const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  store_partials_at<NT, NV>(v, partials, sh, G, bid);
 }
 
 // Fixed-order sum of G partials per quantity (call from one block of NT threads).

@@ -41,7 +41,8 @@ __device__ __forceinline__ bool first_thread() {
 // beta_j = rho_j / rho_{j-1} and the same stop decision; the first thread
 // records them.  Returns false when the iteration must not run.
 __device__ __forceinline__ bool fused_beta(const double *__restrict__ pu, unsigned Gu, int j,
-                                           PcgScalars *S, double *sh, double &beta) {
+                                           PcgScalars *S, double *sh, double &beta,
+                                           bool leader) {
   double tot[2];
   sum_partials<NT, 2>(pu, Gu, tot, sh);
   const double rho1 = tot[0], rr = tot[1];
@@ -54,7 +55,7 @@ __device__ __forceinline__ bool fused_beta(const double *__restrict__ pu, unsign
   else if (iter > S->maxit) status = 1;
   else if (rnorm < S->repl_norm) status = 3;
   beta = rho1 / rho0;
-  if (first_thread()) {
+  if (leader) {
     S->rho = rho1;
     S->rr = rr;
     S->iter = iter;
@@ -71,13 +72,14 @@ __device__ __forceinline__ bool fused_beta(const double *__restrict__ pu, unsign
 // Alpha step of a fused batch, at the start of the update kernel of in-batch
 // iteration j: every block reduces the stencil's p.q partials.
 __device__ __forceinline__ bool fused_alpha(const double *__restrict__ ps, unsigned Gs, int j,
-                                            PcgScalars *S, double *sh, double &alpha) {
+                                            PcgScalars *S, double *sh, double &alpha,
+                                            bool leader) {
   double tot[1];
   sum_partials<NT, 1>(ps, Gs, tot, sh);
   const double pq = tot[0];
   alpha = S->rho_slot[j & 1] / pq;
   const bool ok = pq > 0.0 && isfinite(alpha);
-  if (first_thread()) {
+  if (leader) {
     S->pq = pq;
     S->alpha = alpha;
     if (!ok) {
@@ -92,11 +94,12 @@ __device__ __forceinline__ bool fused_alpha(const double *__restrict__ ps, unsig
 // update's partials.  Returns false when the kernel must not run.
 template <bool PCG, bool FUSE>
 __device__ __forceinline__ bool pcg_prologue(PcgScalars *S, const double *pu, unsigned Gu, int j,
-                                             double *sh, double &beta) {
+                                             double *sh, double &beta,
+                                             bool leader = first_thread()) {
   beta = 0.0;
   if constexpr (PCG) {
     if (S->done) return false;
-    if constexpr (FUSE) return fused_beta(pu, Gu, j, S, sh, beta);
+    if constexpr (FUSE) return fused_beta(pu, Gu, j, S, sh, beta, leader);
     beta = S->beta;
   }
   return true;
@@ -338,26 +341,34 @@ __device__ __forceinline__ void layer_contract(const T *__restrict__ c_KE0, cons
 // y receives x' = x + omega dinv (b - K x) (out of place: neighbouring columns
 // and the next z-chunk still read x) and the block partials of x'.b and x'.q.
 // The epilogue computes in FP64 and y, b, dinv, q are FP64 whatever T is.
-template <bool PCG, int SYM, bool FUSE, bool JAC, class T = double, class TO = OutT<JAC, T>>
-__global__ __launch_bounds__(NT) void k_stencil64z(
-    Lattice L, int nchunks, const T *__restrict__ c_KE0, KeValsT<T> kv, const T *__restrict__ young,
-    const uint8_t *__restrict__ fmask, const T *__restrict__ src,
+// XCD-aware tile order: workgroups go to the 8 XCDs round-robin by id, so
+// each XCD is given a contiguous range of the nt work items (columns fastest):
+// one XCD marches neighbouring columns of one z-slab and shares their halos in
+// its L2.  xcd_order maps a workgroup id to its work item, xcd_slot back.
+__device__ __forceinline__ unsigned xcd_order(unsigned b, unsigned nt) {
+  const unsigned q = nt / 8, rem = nt % 8, xcd = b % 8, slot = b / 8;
+  return (xcd < rem ? xcd * (q + 1) : rem * (q + 1) + (xcd - rem) * q) + slot;
+}
+__device__ __forceinline__ unsigned xcd_slot(unsigned t, unsigned nt) {
+  const unsigned q = nt / 8, rem = nt % 8, big = rem * (q + 1);
+  if (t < big) return (t % (q + 1)) * 8 + t / (q + 1);
+  return ((t - big) % q) * 8 + rem + (t - big) / q;
+}
+
+// The z-marching K.u of tile T_ (column T_ % ncols, z-chunk T_ / ncols); the
+// block's partials go to slot pslot of Gp.
+template <bool PCG, int SYM, bool FUSE, bool JAC, class T, class TO>
+__device__ __forceinline__ void stencil64z_tile(
+    const Lattice &L, int nchunks, const T *__restrict__ c_KE0, const KeValsT<T> &kv,
+    const T *__restrict__ young, const uint8_t *__restrict__ fmask, const T *__restrict__ src,
     const double *__restrict__ dinv, const double *__restrict__ p_old, double *__restrict__ p_new,
     TO *__restrict__ y, PcgScalars *S, double *partials, const double *__restrict__ pu,
-    unsigned Gu, int j, JacEpilogue je) {
+    unsigned Gu, int j, const JacEpilogue &je, unsigned T_, unsigned Gp, unsigned pslot,
+    bool leader, T (*sp)[3][ZPN], T (*se)[ZLN], double *sh) {
   static_assert(!PCG || std::is_same<T, double>::value, "the PCG stencil is FP64");
-  __shared__ T sp[2][3][ZPN];
-  __shared__ T se[2][ZLN];
-  __shared__ double sh[8];
   double beta;
-  if (!pcg_prologue<PCG, FUSE>(S, pu, Gu, j, sh, beta)) return;
-  // XCD-aware tile order: workgroups go to the 8 XCDs round-robin by id, so
-  // each XCD is given a contiguous range of tiles (columns fastest): one XCD
-  // marches neighbouring columns of one z-slab and shares their halos in its L2.
+  if (!pcg_prologue<PCG, FUSE>(S, pu, Gu, j, sh, beta, leader)) return;
   const int nbx = (L.NX + ZX - 1) / ZX, nby = (L.NY + ZY - 1) / ZY, ncols = nbx * nby;
-  const unsigned nt = gridDim.x, b = blockIdx.x;
-  const unsigned q = nt / 8, rem = nt % 8, xcd = b % 8, slot = b / 8;
-  const unsigned T_ = (xcd < rem ? xcd * (q + 1) : rem * (q + 1) + (xcd - rem) * q) + slot;
   const int col = (int)(T_ % ncols), zc = (int)(T_ / ncols);
   const int x0 = (col % nbx) * ZX, y0 = (col / nbx) * ZY;
   // balanced z-chunks: chunk zc covers node planes [z0, z1)
@@ -430,12 +441,60 @@ __global__ __launch_bounds__(NT) void k_stencil64z(
   }
   if constexpr (PCG) {
     double v[1] = {pq};
-    store_partials<NT, 1>(v, partials, sh);
+    store_partials_at<NT, 1>(v, partials, sh, Gp, pslot);
   }
   if constexpr (JAC) {
     double v[2] = {zr, zq};
-    store_partials<NT, 2>(v, partials, sh);
+    store_partials_at<NT, 2>(v, partials, sh, Gp, pslot);
   }
+}
+
+template <bool PCG, int SYM, bool FUSE, bool JAC, class T = double, class TO = OutT<JAC, T>>
+__global__ __launch_bounds__(NT) void k_stencil64z(
+    Lattice L, int nchunks, const T *__restrict__ c_KE0, KeValsT<T> kv, const T *__restrict__ young,
+    const uint8_t *__restrict__ fmask, const T *__restrict__ src,
+    const double *__restrict__ dinv, const double *__restrict__ p_old, double *__restrict__ p_new,
+    TO *__restrict__ y, PcgScalars *S, double *partials, const double *__restrict__ pu,
+    unsigned Gu, int j, JacEpilogue je) {
+  __shared__ T sp[2][3][ZPN];
+  __shared__ T se[2][ZLN];
+  __shared__ double sh[8];
+  const unsigned nt = gridDim.x, b = blockIdx.x;
+  stencil64z_tile<PCG, SYM, FUSE, JAC, T, TO>(L, nchunks, c_KE0, kv, young, fmask, src, dinv,
+                                              p_old, p_new, y, S, partials, pu, Gu, j, je,
+                                              xcd_order(b, nt), nt, b, first_thread(), sp, se, sh);
+}
+
+// Several right-hand sides with one operator (load cases): nc cases of a
+// Jacobi-PCG batch step in one launch.  Case c's vectors start at c * cs
+// (p: two buffers per case, c * 2 cs, chosen by the parity of the case's
+// iteration count at the batch start, as the single-case batches do); its
+// scalars are S[c]; its partials are stored at [c * nt, (c+1) * nt) in the
+// slots of the single-case launch, so each case's sums equal those of its
+// own solve bit for bit.  Work items run case-fastest: the cases of one tile
+// land on the same XCD together and read the tile's young and mask through
+// its L2.
+template <int SYM, bool FUSE>
+__global__ __launch_bounds__(NT) void k_stencil64z_mc(
+    Lattice L, int nchunks, const double *__restrict__ c_KE0, KeVals kv,
+    const double *__restrict__ young, const uint8_t *__restrict__ fmask,
+    const double *__restrict__ r, const double *__restrict__ dinv, double *__restrict__ p,
+    double *__restrict__ q, PcgScalars *S, double *partials, const double *__restrict__ pu,
+    unsigned Gu, int j, int nc, long long cs) {
+  __shared__ double sp[2][3][ZPN];
+  __shared__ double se[2][ZLN];
+  __shared__ double sh[8];
+  const unsigned ntot = gridDim.x, nt = ntot / nc;
+  const unsigned w = xcd_order(blockIdx.x, ntot);
+  const int c = (int)(w % nc);
+  const unsigned T_ = w / nc;
+  PcgScalars *Sc = S + c;
+  const int par = (j + Sc->iter_base) & 1;
+  double *pc = p + 2 * cs * c;
+  stencil64z_tile<true, SYM, FUSE, false, double, double>(
+      L, nchunks, c_KE0, kv, young, fmask, r + cs * c, dinv, pc + cs * par, pc + cs * (par ^ 1),
+      q + cs * c, Sc, partials + (size_t)nt * c, pu + (size_t)2 * Gu * c, Gu, j, JacEpilogue{}, T_,
+      nt, xcd_slot(T_, nt), T_ == 0 && threadIdx.x == 0, sp, se, sh);
 }
 
 __global__ __launch_bounds__(NT) void k_pcg_update64(long long n, double *__restrict__ x,
@@ -474,7 +533,7 @@ __global__ __launch_bounds__(NT) void k_pcg_update64f(long long n, double *__res
   __shared__ double sh[8];
   if (S->done) return;
   double alpha;
-  if (!fused_alpha(ps, Gs, j, S, sh, alpha)) return;
+  if (!fused_alpha(ps, Gs, j, S, sh, alpha, first_thread())) return;
   double rz = 0.0, rr = 0.0;
   const long long stride = (long long)gridDim.x * NT;
   for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += stride) {
@@ -486,6 +545,39 @@ __global__ __launch_bounds__(NT) void k_pcg_update64f(long long n, double *__res
   }
   double v[2] = {rz, rr};
   store_partials<NT, 2>(v, partials, sh);
+}
+
+// k_pcg_update64f for nc load cases (blockIdx.y = case, the layout of
+// k_stencil64z_mc); each case's partials equal those of its own solve.
+__global__ __launch_bounds__(NT) void k_pcg_update64f_mc(long long n, double *__restrict__ x,
+                                                         double *__restrict__ r,
+                                                         const double *__restrict__ p,
+                                                         const double *__restrict__ q,
+                                                         const double *__restrict__ dinv,
+                                                         PcgScalars *S, double *partials,
+                                                         const double *__restrict__ ps,
+                                                         unsigned Gs, int j, long long cs) {
+  __shared__ double sh[8];
+  const int c = blockIdx.y;
+  PcgScalars *Sc = S + c;
+  if (Sc->done) return;
+  double alpha;
+  if (!fused_alpha(ps + (size_t)Gs * c, Gs, j, Sc, sh, alpha, blockIdx.x == 0 && threadIdx.x == 0))
+    return;
+  const int par = (j + Sc->iter_base + 1) & 1;
+  double *xc = x + cs * c, *rc = r + cs * c;
+  const double *pc = p + 2 * cs * c + cs * par, *qc = q + cs * c;
+  double rz = 0.0, rr = 0.0;
+  const long long stride = (long long)gridDim.x * NT;
+  for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += stride) {
+    xc[i] += alpha * pc[i];
+    const double ri = rc[i] - alpha * qc[i];
+    rc[i] = ri;
+    rz += (dinv[i] * ri) * ri;
+    rr += ri * ri;
+  }
+  double v[2] = {rz, rr};
+  store_partials_at<NT, 2>(v, partials + (size_t)2 * gridDim.x * c, sh, gridDim.x, blockIdx.x);
 }
 
 __global__ __launch_bounds__(NT) void k_pcg_init64(long long n, const double *__restrict__ b,
@@ -940,6 +1032,49 @@ unsigned launch_pcg_stencil64_fused(const Lattice &L, const KeCoef &ke, const do
                                          partials_st, nullptr, 0, 0, s);
   return launch_stencil64<true, true>(L, ke, young, freemask, r, dinv, p_old, p_new, q, S,
                                       partials_st, partials_up, Gu, j, s);
+}
+
+unsigned pcg_update_blocks() { return UPD_BLOCKS; }
+
+unsigned launch_pcg_stencil64_mc(const Lattice &L, const KeCoef &ke, const double *young,
+                                 const uint8_t *freemask, const double *r, const double *dinv,
+                                 double *p, double *q, PcgScalars *S, double *partials_st,
+                                 const double *partials_up, unsigned Gu, int j, int nc,
+                                 long long cs, hipStream_t s) {
+  if (ke.kernel != 0) throw Error(kInvalidArgument, "load cases together need the z-marching K.u");
+  if (nc < 1) throw Error(kInvalidArgument, "no load cases");
+  const int def = stencil_nchunks(L);
+  const int nch = ke.nchunks > 0 ? std::min(ke.nchunks, def) : def;
+  const unsigned nt = zgrid(L, nch).x;
+  const dim3 g(nt * (unsigned)nc);
+#define MC_LAUNCH(SYM_)                                                                       \
+  if (j == 0)                                                                                 \
+    hipLaunchKernelGGL((k_stencil64z_mc<SYM_, false>), g, dim3(NT), 0, s, L, nch, ke.ke,      \
+                       ke.vals, young, freemask, r, dinv, p, q, S, partials_st, nullptr, 0u,  \
+                       0, nc, cs);                                                            \
+  else                                                                                        \
+    hipLaunchKernelGGL((k_stencil64z_mc<SYM_, true>), g, dim3(NT), 0, s, L, nch, ke.ke,       \
+                       ke.vals, young, freemask, r, dinv, p, q, S, partials_st, partials_up,  \
+                       Gu, j, nc, cs);
+  switch (ke.sym) {
+    case 0: MC_LAUNCH(0); break;
+    case 1: MC_LAUNCH(1); break;
+    case 2: MC_LAUNCH(2); break;
+    default: throw Error(kInvalidArgument, "unknown KE0 symmetry class");
+  }
+#undef MC_LAUNCH
+  HIPCHK(hipGetLastError());
+  return nt;
+}
+
+unsigned launch_pcg_update64_mc(long long n, double *x, double *r, const double *p,
+                                const double *q, const double *dinv, PcgScalars *S,
+                                double *partials_up, const double *partials_st, unsigned Gs,
+                                int j, int nc, long long cs, hipStream_t s) {
+  hipLaunchKernelGGL(k_pcg_update64f_mc, dim3(UPD_BLOCKS, (unsigned)nc), dim3(NT), 0, s, n, x, r,
+                     p, q, dinv, S, partials_up, partials_st, Gs, j, cs);
+  HIPCHK(hipGetLastError());
+  return UPD_BLOCKS;
 }
 
 unsigned launch_pcg_update64_fused(long long n, double *x, double *r, const double *p,

@@ -100,6 +100,20 @@ unsigned launch_pcg_update64_fused(long long n, double *x, double *r, const doub
                                    const double *q, const double *dinv, PcgScalars *S,
                                    double *partials_up, const double *partials_st, unsigned Gs,
                                    int j, hipStream_t s);
+// Jacobi-PCG batch steps of nc load cases in one launch each (k_stencil64z_mc,
+// k_pcg_update64f_mc): case c's r, q, x at c*cs, its two p buffers at c*2*cs,
+// scalars S[c]; stencil partials nt per case, update partials 2*UPD per case.
+// Each case's arithmetic and sums are those of its own single-case batch.
+unsigned pcg_update_blocks();  // blocks of the fused update kernels (partials per quantity)
+unsigned launch_pcg_stencil64_mc(const Lattice &L, const KeCoef &ke, const double *young,
+                                 const uint8_t *freemask, const double *r, const double *dinv,
+                                 double *p, double *q, PcgScalars *S, double *partials_st,
+                                 const double *partials_up, unsigned Gu, int j, int nc,
+                                 long long cs, hipStream_t s);
+unsigned launch_pcg_update64_mc(long long n, double *x, double *r, const double *p,
+                                const double *q, const double *dinv, PcgScalars *S,
+                                double *partials_up, const double *partials_st, unsigned Gs,
+                                int j, int nc, long long cs, hipStream_t s);
 // Vector half: x += alpha p; r -= alpha q; rho' = r.dinv.r; r.r; beta; stop test.
 void launch_pcg_update64(long long n, double *x, double *r, const double *p, const double *q,
                          const double *dinv, PcgScalars *S, double *partials, unsigned *counter,
