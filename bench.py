@@ -537,6 +537,16 @@ def main():
                          "the per-solve set-up in the time, not in the bytes)",
                 "bytes": b_it, "s_per_iteration": t_it, "gbs": b_it / t_it / 1e9,
                 "frac": b_it / t_it / 1e9 / MI355X_PEAK["hbm_gbs"]}
+    # the same K.u (y = K x) on the VALU and on the matrix cores (FP64 MFMA
+    # element contraction); the solver runs the VALU kernel
+    ms_k8, fl_k8, _ = c.time_kernel(8, a.reps)
+    ms_k12, fl_k12, _ = c.time_kernel(12, a.reps)
+    out["k_apply_variants"] = {
+        "fp64_valu": {"kernel": "k_stencil64z<false, SYM, false, false>", "avg_ms": ms_k8,
+                      "tflops": fl_k8 / (ms_k8 * 1e-3) / 1e12},
+        "fp64_mfma": {"kernel": "k_apply64_mfma (v_mfma_f64_4x4x4_4b)", "avg_ms": ms_k12,
+                      "tflops": fl_k12 / (ms_k12 * 1e-3) / 1e12},
+        "used": "fp64_valu" if ms_k8 <= ms_k12 else "fp64_mfma (not wired into the solver)"}
     if helm:
         ms_h, fl_h, by_h = c.time_kernel(9, a.reps)
         out["helm_filter"] = {"kernel": "Helmholtz filter pass (element loads, 6 FP64 GEMM "

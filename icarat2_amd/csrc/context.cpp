@@ -250,6 +250,8 @@ void Context::apply_k(const double *u, double *y, int precision) {
   launch_numeq_to_lattice(L_, d_dof, numeq, d_numeq[0], nullptr, d_y, stream_);
   if (precision == ICARAT_HIP_FP64) {
     launch_apply_k64(L_, kc_, d_young, d_free, d_y, d_q, stream_);
+  } else if (precision == ICARAT_HIP_FP64_MFMA) {
+    launch_apply_k64_mfma(L_, kc_, d_young, d_free, d_y, d_q, stream_);
   } else if (precision == ICARAT_HIP_FP32 || precision == ICARAT_HIP_BF16) {
     launch_apply_k_mixed(L_, precision == ICARAT_HIP_BF16 ? 1 : 0, d_ke32, d_young32, d_free, d_y,
                          d_q, stream_);
@@ -1495,6 +1497,12 @@ void Context::time_kernel(int kernel, int reps, double *avg_ms, double *flops, d
     // y = K x as the multigrid smoother runs it on the fine lattice
     d2d(d_r.p, d_f.p, n3_, stream_);
     launch = [this]() { launch_apply_k64(L_, kc_, d_young, d_free, d_r, d_y, stream_); };
+    F = 2.0 * 24 * 24 * E;
+    B = 2.0 * 3 * N * 8 + E * 8 + N;            // x in, y out; young; mask
+  } else if (kernel == 12) {
+    // y = K x on the matrix cores (FP64 MFMA element contraction)
+    d2d(d_r.p, d_f.p, n3_, stream_);
+    launch = [this]() { launch_apply_k64_mfma(L_, kc_, d_young, d_free, d_r, d_y, stream_); };
     F = 2.0 * 24 * 24 * E;
     B = 2.0 * 3 * N * 8 + E * 8 + N;            // x in, y out; young; mask
   } else if (kernel == 10) {

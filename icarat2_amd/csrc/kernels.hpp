@@ -104,6 +104,10 @@ unsigned launch_pcg_update64_fused(long long n, double *x, double *r, const doub
 // k_pcg_update64f_mc): case c's r, q, x at c*cs, its two p buffers at c*2*cs,
 // scalars S[c]; stencil partials nt per case, update partials 2*UPD per case.
 // Each case's arithmetic and sums are those of its own single-case batch.
+// y = K x on the matrix cores (v_mfma_f64_4x4x4_4b element contraction, node
+// gather); the z-marching tile and z-chunks of launch_apply_k64
+void launch_apply_k64_mfma(const Lattice &L, const KeCoef &ke, const double *young,
+                           const uint8_t *freemask, const double *x, double *y, hipStream_t s);
 unsigned pcg_update_blocks();  // blocks of the fused update kernels (partials per quantity)
 unsigned launch_pcg_stencil64_mc(const Lattice &L, const KeCoef &ke, const double *young,
                                  const uint8_t *freemask, const double *r, const double *dinv,

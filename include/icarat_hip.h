@@ -38,7 +38,8 @@ enum {
 enum {
   ICARAT_HIP_FP64 = 0,         /* plain FP64 throughout */
   ICARAT_HIP_FP32 = 1,         /* FP32 MFMA contraction inside FP64 refinement */
-  ICARAT_HIP_BF16 = 2          /* BF16 MFMA contraction (FP32 accumulate) inside FP64 refinement */
+  ICARAT_HIP_BF16 = 2,         /* BF16 MFMA contraction (FP32 accumulate) inside FP64 refinement */
+  ICARAT_HIP_FP64_MFMA = 3     /* icarat_hip_apply_K only: FP64 MFMA element contraction */
 };
 
 /* Preconditioner of the (inner) Krylov solve. */
@@ -522,6 +523,8 @@ int icarat_hip_mg_status(const icarat_hip_ctx *ctx, icarat_mg_status *status);
  *             V-cycle's z, no D^-1; q = K p, p.q), 1152 flop/element
  *   kernel 11: FP64 vector update of the multigrid solve (alpha step reduced from the
  *             stencil's partials, x, r update, r.r, z0 = omega D^-1 r)
+ *   kernel 12: FP64 y = K x on the matrix cores (v_mfma_f64_4x4x4 element contraction,
+ *             ICARAT_HIP_FP64_MFMA of icarat_hip_apply_K), 1152 flop/element
  * Needs set_density (and filter_setup for kernel 2, helm_setup for kernel 9).  Overwrites the solver's
  * work vectors. */
 int icarat_hip_time_kernel(icarat_hip_ctx *ctx, int32_t kernel, int32_t reps, double *avg_ms,
