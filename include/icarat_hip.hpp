@@ -156,6 +156,14 @@ class Filter {
           dev_.handle(), "densityFilter");
     return out;
   }
+  // Filter::sensitivityFilter (optimization/filter.hpp:76-104)
+  std::vector<double> sensitivityFilter(const std::vector<double> &design_s,
+                                        const std::vector<double> &dfds) {
+    std::vector<double> out(design_s.size());
+    check(icarat_hip_sensitivity_filter(dev_.handle(), design_s.data(), dfds.data(), out.data()),
+          dev_.handle(), "sensitivityFilter");
+    return out;
+  }
   std::vector<double> threshold(const std::vector<double> &design_s, double beta, double T) {
     std::vector<double> out(design_s.size());
     check(icarat_hip_threshold(dev_.handle(), design_s.data(), beta, T, out.data(), 0),
