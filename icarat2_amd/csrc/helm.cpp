@@ -61,6 +61,8 @@ HelmFilter::HelmFilter(Context &c, double radius) : c_(c), r_(radius) {
     throw Error(kInternal, "rocblas_create_handle failed");
   hipStream_t s = c_.stream();
   chk(rocblas_set_stream(blas_, s), "rocblas_set_stream");
+  // no atomics in the library kernels: results repeat bit for bit
+  chk(rocblas_set_atomics_mode(blas_, rocblas_atomics_not_allowed), "rocblas_set_atomics_mode");
   DevBuf<double> dM, dE;
   DevBuf<int> info(1);
   for (int d = 0; d < 3; ++d) {

@@ -118,6 +118,11 @@ Multigrid::Multigrid(Context &fine, const icarat_mg_opts &o) : f_(fine), o_(o) {
           throw Error(kInternal, "rocblas_create_handle failed");
         if (rocblas_set_stream(blas_[k], k == 0 ? f_.stream() : side_) != rocblas_status_success)
           throw Error(kInternal, "rocblas_set_stream failed");
+        // library kernels that accumulate with atomics give run-to-run
+        // differences; the solve must repeat bit for bit
+        if (rocblas_set_atomics_mode(blas_[k], rocblas_atomics_not_allowed) !=
+            rocblas_status_success)
+          throw Error(kInternal, "rocblas_set_atomics_mode failed");
       }
     }
   }
