@@ -50,6 +50,7 @@ double jacobi_bound(const double *ke) {
 Multigrid::Multigrid(Context &fine, const icarat_mg_opts &o) : f_(fine), o_(o) {
   f_.ensure_device();
   if (const char *e = std::getenv("ICARAT_HIP_MG_FUSE")) fuse_ = std::string(e) != "0";
+  if (const char *e = std::getenv("ICARAT_HIP_MG_OVERLAP")) overlap_ = std::string(e) == "1";
   const GridSpec g0 = f_.problem().spec();
   GridSpec g = g0;
   const long long dense_max = std::min<long long>(std::max(0, o_.dense_dofs), MG_DENSE_MAX);
@@ -190,7 +191,7 @@ void Multigrid::refresh() {
     launch_diag_inv(cc.lattice(), cc.d_ke, cc.d_young, cc.d_free, cc.d_dinv, nullptr, s);
   }
   if (dense_) {
-    if (!have_inverse_) {
+    if (!have_inverse_ || !overlap_) {
       factor_checked(ainv_);
       have_inverse_ = true;
       swap_pending_ = false;

@@ -125,6 +125,14 @@ class Multigrid {
   // direct levels: the coarse correction and the first post-sweep in one
   // kernel (launch_mg_prolong_jacobi); ICARAT_HIP_MG_FUSE=0 keeps them apart
   bool fuse_ = true;
+  // A new density's coarsest inverse is factored synchronously before the
+  // solve.  ICARAT_HIP_MG_OVERLAP=1 factors it on the side stream while the
+  // first batches run with the previous inverse: about 5% faster at 192x64x64,
+  // but its results differ from run to run at rounding level (only while the
+  // factorization runs concurrently with the solve's kernels; input snapshots
+  // and guard zones around every device buffer ruled out input races and
+  // out-of-bounds writes), so it is off by default.
+  bool overlap_ = false;
   // Galerkin coarse operators (coarse_avg 5): explicit element matrices per
   // coarse level (kmat_[l], 576 per element), the per-element smoothing bounds
   // and their maximum per level; the damping is folded into the levels' dinv

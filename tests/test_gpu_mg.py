@@ -62,8 +62,8 @@ def test_mg_solve_agrees_with_jacobi(gpu, dims):
         # the blocked 10x5x5 coarsest grid (1188 DOFs > 1024) is solved densely
         assert c.mg_info() == [(40, 20, 20), (20, 10, 10), (10, 5, 5)]
         assert ms.dense == 1
-        # a new density: its inverse is factored on the side stream and swapped
-        # in after the first batches (restart), and the solve still matches
+        # a new density: its inverse is factored again before the solve (one
+        # factorization more), and the solve still matches
         x2 = np.clip(x * (1.0 + 0.5 * np.cos(np.arange(c.nelx) * 0.01)), 1e-3, 1.0)
         c.set_density(x2)
         uj2, sj2 = c.solve(tol=1e-9)
@@ -234,10 +234,10 @@ def test_fused_correction_sweep_matches_separate_kernels(gpu, monkeypatch, dims,
 
 def test_multigrid_loop_repeats_bit_for_bit(gpu):
     """Two fresh contexts run the same multigrid SIMP loop (192x64x64, seeded
-    field, 6 design iterations, each density's coarsest inverse factored on the
-    side stream): designs and histories equal bit for bit.  (rocBLAS handles
-    run with atomics disallowed; with them allowed the loop drifted from run to
-    run after a few iterations.)"""
+    field, 6 design iterations, each density's coarsest inverse refactored):
+    designs and histories equal bit for bit, also after other contexts have
+    run in the process.  (The overlapped refactorization, ICARAT_HIP_MG_OVERLAP=1,
+    does not meet this and is off by default.)"""
     dims = (192, 64, 64)
     x0, _ = gpu.gaussian_field(*dims, 2)
     out = []
