@@ -78,12 +78,14 @@ class LearnedInfo(C.Structure):
 class MgOpts(C.Structure):
     _fields_ = [("max_levels", C.c_int32), ("nu_pre", C.c_int32), ("nu_post", C.c_int32),
                 ("nu_coarse", C.c_int32), ("omega", C.c_double), ("batch_iters", C.c_int32),
-                ("dense_dofs", C.c_int32), ("direct_nodes", C.c_int32), ("coarse_avg", C.c_int32)]
+                ("dense_dofs", C.c_int32), ("direct_nodes", C.c_int32), ("coarse_avg", C.c_int32),
+                ("w_levels", C.c_int32), ("w_switch", C.c_double)]
 
 
 class MgStatus(C.Structure):
     _fields_ = [("levels", C.c_int32), ("dense", C.c_int32), ("direct_fine", C.c_int32),
-                ("fine_fp32", C.c_int32), ("factorizations", C.c_int64), ("shifted", C.c_int64)]
+                ("fine_fp32", C.c_int32), ("factorizations", C.c_int64), ("shifted", C.c_int64),
+                ("w_levels", C.c_int32)]
 
 
 fp = C.POINTER(C.c_float)

@@ -530,6 +530,7 @@ int icarat_hip_mg_status(const icarat_hip_ctx *hc, icarat_mg_status *st) {
     o.fine_fp32 = c.mg_->fine_fp32() ? 1 : 0;
     o.factorizations = c.mg_->factorizations();
     o.shifted = c.mg_->shifted();
+    o.w_levels = c.mg_->w_levels();
     *st = o;
   });
 }

@@ -127,6 +127,16 @@ Multigrid::Multigrid(Context &fine, const icarat_mg_opts &o) : f_(fine), o_(o) {
       }
     }
   }
+  // W-cycle depth: automatic (w_levels < 0) by the fine mesh size, as measured
+  // on the SIMP loops: none below 500,000 elements (the coarse levels' launches
+  // cost more than the iterations saved), the first coarse level up to
+  // 2,000,000, every coarse level above
+  if (o_.w_levels >= 0) {
+    wlev_ = o_.w_levels;
+  } else {
+    const long long ne = f_.lattice().nelx;
+    wlev_ = ne >= 2000000 ? std::max(0, levels() - 2) : (ne >= 500000 ? 1 : 0);
+  }
   z_.alloc(f_.n3_);
   y0_.alloc(f_.n3_);
   r0_.alloc(f_.n3_);
@@ -158,12 +168,13 @@ Multigrid::~Multigrid() {
 }
 
 void Multigrid::drop_graphs() {
-  for (int p = 0; p < 2; ++p) {
-    if (exec_[p]) (void)hipGraphExecDestroy(exec_[p]);
-    if (graph_[p]) (void)hipGraphDestroy(graph_[p]);
-    exec_[p] = nullptr;
-    graph_[p] = nullptr;
-  }
+  for (int m = 0; m < 2; ++m)
+    for (int p = 0; p < 2; ++p) {
+      if (exec_[m][p]) (void)hipGraphExecDestroy(exec_[m][p]);
+      if (graph_[m][p]) (void)hipGraphDestroy(graph_[m][p]);
+      exec_[m][p] = nullptr;
+      graph_[m][p] = nullptr;
+    }
 }
 
 Context &Multigrid::level(int l) { return l == 0 ? f_ : *lv_[l - 1]; }
@@ -273,6 +284,13 @@ void Multigrid::before_batch(hipStream_t s) {
     swap_inverse(s);
     // the Krylov directions so far used the previous preconditioner: restart
     // from z = M(r) with the new one (beta = 0)
+    enqueue_restart();
+  }
+  // W-cycle phase over: from here on the V-cycle, restarted from z = M_V(r)
+  const PcgScalars *h = f_.h_pcg.p;
+  if (wmode_ && !h->done && h->rr < o_.w_switch * o_.w_switch * tol_ * tol_ * h->bnorm2) {
+    wmode_ = false;
+    wcur_ = 0;
     enqueue_restart();
   }
   ++batches_;
@@ -400,6 +418,26 @@ double *Multigrid::cycle(int l, const double *b, bool emit, bool x0_ready, ZDot 
     }
   }
   const double *xc = cycle(l + 1, cc.d_b, emit);
+  if (l + 1 <= wcur_ && l + 1 < levels() - 1) {
+    // W-cycle on this coarse level: a second coarse cycle on the remaining
+    // coarse residual b_c - K_c xc, its correction added to xc (in d_u)
+    const Lattice &Lc = cc.lattice();
+    const long long nc = cc.n3_;
+    if (emit) {
+      launch_copy64(nc, xc, cc.d_u, s);
+      if (direct_[l + 1]) {
+        coarse_op(l + 1, MG_RESIDUAL, xc, cc.d_b, 0.0, cc.d_nv, s);
+      } else {
+        launch_apply_k64(Lc, cc.kc_, cc.d_young, cc.d_free, xc, cc.d_nv, s);
+        launch_axpby(nc, 1.0, cc.d_b, -1.0, cc.d_nv, s);
+      }
+      // the first sweep from zero (the restriction writes it for the first cycle)
+      if (starts_swept(l + 1)) launch_mg_jacobi(nc, omega_[l + 1], cc.d_dinv, cc.d_nv, nullptr, cc.d_x, s);
+    }
+    const double *e = cycle(l + 1, cc.d_nv, emit);
+    if (emit) launch_axpby(nc, 1.0, e, 1.0, cc.d_u, s);
+    xc = cc.d_u;
+  }
   const int post = std::max(0, o_.nu_post);
   // a direct level's first post-sweep reads x + P xc straight from the
   // correction (same arithmetic as the two kernels, one pass fewer)
@@ -475,31 +513,44 @@ void Multigrid::launch_batch(int par) {
   build_graph();
   before_batch(f_.stream());
   if (f_.use_graphs_)
-    HIPCHK(hipGraphLaunch(exec_[par & 1], f_.stream()));
+    HIPCHK(hipGraphLaunch(exec_for(par), f_.stream()));
   else
     enqueue_batch(par & 1);
 }
 
+// Captures the batches of the V-cycle and, with w_levels > 0, of the W-cycle
+// (the cycle shape is fixed in a graph; the solve picks the graph of its phase).
 void Multigrid::build_graph() {
-  if (!f_.use_graphs_ || exec_[0]) return;
+  if (!f_.use_graphs_ || exec_[0][0]) return;
   hipStream_t s = f_.stream();
-  for (int par = 0; par < 2; ++par) {
-    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    try {
-      enqueue_batch(par);
-    } catch (...) {
-      hipGraph_t g = nullptr;
-      (void)hipStreamEndCapture(s, &g);
-      if (g) (void)hipGraphDestroy(g);
-      throw;
+  const int wsave = wcur_;
+  for (int m = 0; m < (wlev_ > 0 ? 2 : 1); ++m) {
+    wcur_ = m ? wlev_ : 0;
+    for (int par = 0; par < 2; ++par) {
+      HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      try {
+        enqueue_batch(par);
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(s, &g);
+        if (g) (void)hipGraphDestroy(g);
+        wcur_ = wsave;
+        throw;
+      }
+      HIPCHK(hipStreamEndCapture(s, &graph_[m][par]));
+      HIPCHK(hipGraphInstantiate(&exec_[m][par], graph_[m][par], nullptr, nullptr, 0));
     }
-    HIPCHK(hipStreamEndCapture(s, &graph_[par]));
-    HIPCHK(hipGraphInstantiate(&exec_[par], graph_[par], nullptr, nullptr, 0));
   }
+  wcur_ = wsave;
 }
 
+// one V-cycle (the learned preconditioner's base), whatever phase a solve is in
 const double *Multigrid::vcycle(const double *r) {
-  return cycle(0, r, true);
+  const int wsave = wcur_;
+  wcur_ = 0;
+  const double *z = cycle(0, r, true);
+  wcur_ = wsave;
+  return z;
 }
 
 void Multigrid::refresh_now() {
@@ -530,6 +581,9 @@ void Multigrid::start(double tol, int maxit, hipStream_t s_ext, bool warm) {
     HIPCHK(hipStreamWaitEvent(s, ev_[0], 0));
   }
   batches_ = 0;
+  wmode_ = wlev_ > 0;
+  wcur_ = wmode_ ? wlev_ : 0;
+  tol_ = tol;
   best_ = INFINITY;
   since_best_ = 0;
   verifications_ = 0;
@@ -632,7 +686,7 @@ void Multigrid::solve(const icarat_solve_opts &o, icarat_solve_stats &st, bool w
       for (long long b = 0; b < max_batches && !h->done; ++b) {
         before_batch(s);
         if (c.use_graphs_)
-          HIPCHK(hipGraphLaunch(exec_[h->iter & 1], s));
+          HIPCHK(hipGraphLaunch(exec_for(h->iter), s));
         else
           enqueue_batch(h->iter & 1);
         HIPCHK(hipMemcpyAsync(h, c.d_pcg.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));

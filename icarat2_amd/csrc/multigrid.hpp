@@ -31,6 +31,9 @@ inline icarat_mg_opts mg_default_opts() {
   o.batch_iters = 4;
   o.dense_dofs = 1024;
   o.direct_nodes = 262144;
+  o.coarse_avg = 0;
+  o.w_levels = -1;
+  o.w_switch = 30.0;
   return o;
 }
 
@@ -101,6 +104,7 @@ class Multigrid {
   // place before the call returns to the stream's order (no overlapped swap)
   void refresh_now();
   int batch_iters() const { return std::max(1, o_.batch_iters); }
+  int w_levels() const { return wlev_; }
   // whether level l smooths with the fused one-thread-per-node kernel
   bool direct(int l) const { return direct_[l] != 0; }
   // coarsest-level Cholesky factorizations so far, and those redone with the
@@ -133,6 +137,12 @@ class Multigrid {
   // and guard zones around every device buffer ruled out input races and
   // out-of-bounds writes), so it is off by default.
   bool overlap_ = false;
+  // W-cycle: coarse levels 1..wcur_ are cycled twice per visit; wcur_ is
+  // o_.w_levels while the solve runs the W-cycle and 0 once it has switched
+  int wcur_ = 0;
+  int wlev_ = 0;        // the W-cycle depth of this hierarchy (o_.w_levels or automatic)
+  bool wmode_ = false;  // the running solve is in its W-cycle phase
+  double tol_ = 0.0;    // the running solve's tolerance (the switch point)
   // Galerkin coarse operators (coarse_avg 5): explicit element matrices per
   // coarse level (kmat_[l], 576 per element), the per-element smoothing bounds
   // and their maximum per level; the damping is folded into the levels' dinv
@@ -168,8 +178,10 @@ class Multigrid {
   void before_batch(hipStream_t s);  // the swap at its fixed batch
   ::_rocblas_handle *blas_[2] = {nullptr, nullptr};
   double *zres_ = nullptr;  // buffer that holds the fine V-cycle's result
-  hipGraph_t graph_[2] = {};
-  hipGraphExec_t exec_[2] = {};
+  // captured batches [W-cycle][parity]
+  hipGraph_t graph_[2][2] = {};
+  hipGraphExec_t exec_[2][2] = {};
+  hipGraphExec_t exec_for(int par) const { return exec_[wmode_ ? 1 : 0][par & 1]; }
   // Partials of z.r and z.q produced by the fine level's last sweep (count
   // stays 0 when that sweep cannot produce them).
   struct ZDot {

@@ -480,8 +480,16 @@ typedef struct icarat_mg_opts {
                               the mean density, 3 quadratic mean, 4 maximum) times the coarse
                               KE0; or 5: Galerkin, R A P element by element (explicit 24x24
                               matrices per coarse element, 4.6 KB each) */
+  int32_t w_levels;        /* coarse levels 1..w_levels are cycled twice per visit (a W-cycle
+                              at the top of the hierarchy; 0: V-cycle; < 0: automatic by the
+                              fine mesh: 0 below 500,000 elements, 1 up to 2,000,000, every
+                              coarse level above) */
+  double w_switch;         /* with w_levels > 0 the solve switches to the V-cycle (and
+                              restarts with beta = 0) once its recursive relative residual is
+                              below w_switch x tol: the W-cycle's coarse corrections lose
+                              accuracy near the FP64 floor of high-contrast fields */
 } icarat_mg_opts;
-/* Fills *o with the defaults (10, 1, 1, 6, 1.95, 4, 1024, 262144). */
+/* Fills *o with the defaults (10, 1, 1, 6, 1.95, 4, 1024, 262144, 0, -1, 30). */
 int icarat_hip_mg_default_opts(icarat_mg_opts *o);
 /* (Re)builds the hierarchy of the context (o = NULL: defaults).  Needs make_dof. */
 int icarat_hip_mg_setup(icarat_hip_ctx *ctx, const icarat_mg_opts *o);
@@ -496,6 +504,7 @@ typedef struct icarat_mg_status {
   int64_t factorizations;  /* Cholesky factorizations of the coarsest matrix so far */
   int64_t shifted;         /* of which broke down and were redone with the diagonal of the
                               free DOFs raised by a relative 1e-10 */
+  int32_t w_levels;        /* coarse levels cycled twice (W-cycle phase of a solve; 0: V) */
 } icarat_mg_status;
 /* State of the context's hierarchy (NOT_READY without one). */
 int icarat_hip_mg_status(const icarat_hip_ctx *ctx, icarat_mg_status *status);

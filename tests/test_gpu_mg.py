@@ -249,3 +249,54 @@ def test_multigrid_loop_repeats_bit_for_bit(gpu):
     (x1, h1, _), (x2, h2, _) = out
     assert np.array_equal(h1, h2)
     assert np.array_equal(x1, x2)
+
+
+def test_w_cycle_solves_and_switches(gpu):
+    """The W-cycle at the top of the hierarchy (coarse levels 1..w cycled twice,
+    then the V-cycle from w_switch x tol on): the verified residual, the Jacobi
+    solution to 1e-6 of max |u|, fewer iterations than the V-cycle; the
+    automatic depth by mesh size."""
+    dims = (96, 48, 48)
+    c = gpu.cantilever(*dims)
+    x, _ = gpu.gaussian_field(*dims, 6)
+    c.set_density(x)
+    uj, sj = c.solve(tol=1e-9)
+    its = {}
+    for w in (0, 1, 2):
+        c.mg_setup(w_levels=w)
+        um, sm = c.solve(tol=1e-9, precond=gpu.PRECOND_MG)
+        assert c.mg_status().w_levels == w
+        print(f"w_levels {w}: {sm.iters} iterations, relres {sm.true_relres:.2e}")
+        assert sm.status == 0 and sm.true_relres <= 1e-9
+        assert np.max(np.abs(um - uj)) < 1e-6 * np.max(np.abs(uj))
+        its[w] = sm.iters
+    assert its[1] < its[0] and its[2] <= its[1]
+    # automatic: none below 500,000 elements, one level up to 2,000,000
+    c.mg_setup()
+    c.solve(tol=1e-9, precond=gpu.PRECOND_MG)
+    assert c.mg_status().w_levels == 0
+    big = gpu.cantilever(192, 64, 64)
+    big.set_density(np.full(big.nelx, 0.5))
+    big.solve(tol=1e-6, precond=gpu.PRECOND_MG)
+    assert big.mg_status().w_levels == 1
+
+
+def test_w_cycle_batched_equals_standalone(gpu):
+    from icarat2_amd import workloads as W
+    specs = [W.batch_problem(s) for s in (108, 113)]
+    alone, ctxs = [], []
+    for s in specs:
+        c = W.build_context(gpu, s)
+        c.mg_setup(w_levels=2)
+        c.set_density(W.initial_design(s))
+        u, st = c.solve(tol=1e-9, precond=gpu.PRECOND_MG)
+        alone.append((c.fext() @ u, st.iters))
+        c2 = W.build_context(gpu, s)
+        c2.mg_setup(w_levels=2)
+        c2.set_density(W.initial_design(s))
+        ctxs.append(c2)
+    st = gpu.Batch(ctxs).solve(tol=1e-9, precond=gpu.PRECOND_MG)
+    for c, (c0, i0), s1 in zip(ctxs, alone, st):
+        c1, _, _ = c.compliance_sensitivity()
+        assert s1.iters == i0 and s1.true_relres <= 1e-9
+        assert abs(c1 - c0) <= 1e-12 * abs(c0)
