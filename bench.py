@@ -527,14 +527,19 @@ def main():
     }
     if mg_like:
         out["config"]["mg_levels"] = c.mg_info()
+        ms_ = c.mg_status()
+        out["config"]["mg_w_levels"] = int(ms_.w_levels)
+        out["mg_status"] = {"factorizations": int(ms_.factorizations), "shifted": int(ms_.shifted),
+                            "w_levels": int(ms_.w_levels), "w_fallbacks": int(ms_.w_fallbacks)}
         n_it = float(hist[:, 2].sum())
         if n_it > 0:
             t_it = st.solve_s / n_it
             b_it = mg_fine_bytes(float(c.numnp), float(c.nelx))
             out["iteration_roofline"] = {
                 "scope": "fine-level compulsory bytes of one multigrid PCG iteration (23 lattice "
-                         "vectors + young and mask per fine K.u; coarse levels, launch gaps and "
-                         "the per-solve set-up in the time, not in the bytes)",
+                         "vectors + young and mask per fine K.u; coarse levels -- twice per "
+                         "visit in the W-cycle phase --, launch gaps and the per-solve set-up "
+                         "in the time, not in the bytes)",
                 "bytes": b_it, "s_per_iteration": t_it, "gbs": b_it / t_it / 1e9,
                 "frac": b_it / t_it / 1e9 / MI355X_PEAK["hbm_gbs"]}
     # the same K.u (y = K x) on the VALU and on the matrix cores (FP64 MFMA

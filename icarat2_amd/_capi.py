@@ -85,7 +85,7 @@ class MgOpts(C.Structure):
 class MgStatus(C.Structure):
     _fields_ = [("levels", C.c_int32), ("dense", C.c_int32), ("direct_fine", C.c_int32),
                 ("fine_fp32", C.c_int32), ("factorizations", C.c_int64), ("shifted", C.c_int64),
-                ("w_levels", C.c_int32)]
+                ("w_levels", C.c_int32), ("w_fallbacks", C.c_int64)]
 
 
 fp = C.POINTER(C.c_float)

@@ -91,11 +91,12 @@ void Batch::engine(double tol, int maxit, const std::function<bool(int)> &start,
   HIPCHK(hipStreamSynchronize(ms_));
   for (;;) {
     // solves whose recursive residual passed: the FP64 true residual decides
-    std::vector<int> ver, fin;
+    std::vector<int> ver, fin, rec;
     for (int i = 0; i < n; ++i) {
       if (!run[i]) continue;
       const PcgScalars &h = *ctx_[i]->h_pcg.p;
       if (h.done && h.status == 0) ver.push_back(i);
+      else if (h.done && ctx_[i]->pcg_recover(ms_)) rec.push_back(i);  // W-cycle -> V-cycle
       else if (h.done) fin.push_back(i);  // iteration cap or breakdown
       else if (h.iter > maxit) {
         ctx_[i]->h_pcg.p->status = 1;
@@ -103,7 +104,7 @@ void Batch::engine(double tol, int maxit, const std::function<bool(int)> &start,
       }
     }
     for (int i : ver) ctx_[i]->pcg_verify(ms_);
-    if (!ver.empty()) HIPCHK(hipStreamSynchronize(ms_));
+    if (!ver.empty() || !rec.empty()) HIPCHK(hipStreamSynchronize(ms_));
     for (int i : ver)
       if (ctx_[i]->pcg_after_verify()) fin.push_back(i);
     // finished solves: hand over, then the problem's next solve (if any)

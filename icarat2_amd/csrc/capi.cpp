@@ -531,6 +531,7 @@ int icarat_hip_mg_status(const icarat_hip_ctx *hc, icarat_mg_status *st) {
     o.factorizations = c.mg_->factorizations();
     o.shifted = c.mg_->shifted();
     o.w_levels = c.mg_->w_levels();
+    o.w_fallbacks = c.mg_->w_fallbacks();
     *st = o;
   });
 }

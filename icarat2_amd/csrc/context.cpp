@@ -557,6 +557,10 @@ void Context::pcg_verify(hipStream_t s) {
   ++replacements_;
 }
 
+bool Context::pcg_recover(hipStream_t s) {
+  return batch_precond_ == ICARAT_HIP_PRECOND_MG && mg_ && mg_->recover_w(s);
+}
+
 bool Context::pcg_after_verify() {
   if (batch_precond_ == ICARAT_HIP_PRECOND_MG) return mg_->after_verify();
   return h_pcg.p->done != 0;

@@ -230,6 +230,9 @@ class Context {
   // there (multigrid: passed or stalled at its floor; Jacobi: passed)
   bool pcg_after_verify();
   void pcg_finish(icarat_solve_stats &S);
+  // a multigrid W-cycle phase that broke down resumes with the V-cycle
+  // (enqueued on s, scalars to h_pcg); false: the breakdown stands
+  bool pcg_recover(hipStream_t s);
   void enqueue_pcg_batch(int kmode, int par);  // kBatch Jacobi-PCG iterations on the stream
   // one batch of iterations of the interleaved solve (batch_precond_): enqueued
   // (for a caller's capture), or launched as this context's own graph

@@ -641,6 +641,34 @@ bool Multigrid::after_verify() {
   return true;
 }
 
+// A W-cycle phase whose preconditioned residual stopped being a descent
+// direction (z.r <= 0: rounding in the repeated coarse corrections of a
+// near-void field) goes on with the V-cycle from the current iterate
+// (restart, beta = 0) instead of ending the solve.
+bool Multigrid::recover_w(hipStream_t s_ext) {
+  PcgScalars *h = f_.h_pcg.p;
+  if (!(wmode_ && h->done && h->status == 4)) return false;
+  Context &c = f_;
+  hipStream_t s = c.stream();
+  if (s_ext != s) {
+    HIPCHK(hipEventRecord(ev_[0], s_ext));
+    HIPCHK(hipStreamWaitEvent(s, ev_[0], 0));
+  }
+  wmode_ = false;
+  wcur_ = 0;
+  ++w_fallbacks_;
+  h->done = 0;
+  h->status = 0;
+  HIPCHK(hipMemcpyAsync(c.d_pcg.p, h, sizeof(PcgScalars), hipMemcpyHostToDevice, s));
+  enqueue_restart();
+  HIPCHK(hipMemcpyAsync(h, c.d_pcg.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
+  if (s_ext != s) {
+    HIPCHK(hipEventRecord(ev_[1], s));
+    HIPCHK(hipStreamWaitEvent(s_ext, ev_[1], 0));
+  }
+  return true;
+}
+
 void Multigrid::check_factor() const {
   if (dense_ && *h_info_[0] != 0)
     throw Error(kNumerical, "coarsest multigrid matrix is not positive definite (Cholesky pivot " +
@@ -696,7 +724,13 @@ void Multigrid::solve(const icarat_solve_opts &o, icarat_solve_stats &st, bool w
       }
       if (!h->done) throw Error(kInternal, "multigrid PCG did not reach its stop test");
     }
-    if (h->status != 0) break;
+    if (h->status != 0) {
+      if (recover_w(s)) {
+        c.sync();
+        continue;
+      }
+      break;
+    }
     // verified stop
     verify(s);
     c.sync();

@@ -105,6 +105,11 @@ class Multigrid {
   void refresh_now();
   int batch_iters() const { return std::max(1, o_.batch_iters); }
   int w_levels() const { return wlev_; }
+  long long w_fallbacks() const { return w_fallbacks_; }
+  // after a batch: a W-cycle phase that broke down (z.r <= 0) continues with
+  // the V-cycle; true if the solve was resumed (scalars on the host again once
+  // s_ext has synchronized)
+  bool recover_w(hipStream_t s_ext);
   // whether level l smooths with the fused one-thread-per-node kernel
   bool direct(int l) const { return direct_[l] != 0; }
   // coarsest-level Cholesky factorizations so far, and those redone with the
@@ -141,6 +146,7 @@ class Multigrid {
   // o_.w_levels while the solve runs the W-cycle and 0 once it has switched
   int wcur_ = 0;
   int wlev_ = 0;        // the W-cycle depth of this hierarchy (o_.w_levels or automatic)
+  long long w_fallbacks_ = 0;  // W-cycle phases that ended in a breakdown
   bool wmode_ = false;  // the running solve is in its W-cycle phase
   double tol_ = 0.0;    // the running solve's tolerance (the switch point)
   // Galerkin coarse operators (coarse_avg 5): explicit element matrices per

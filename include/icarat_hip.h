@@ -505,6 +505,8 @@ typedef struct icarat_mg_status {
   int64_t shifted;         /* of which broke down and were redone with the diagonal of the
                               free DOFs raised by a relative 1e-10 */
   int32_t w_levels;        /* coarse levels cycled twice (W-cycle phase of a solve; 0: V) */
+  int64_t w_fallbacks;     /* W-cycle phases that broke down (z.r <= 0) and went on with the
+                              V-cycle */
 } icarat_mg_status;
 /* State of the context's hierarchy (NOT_READY without one). */
 int icarat_hip_mg_status(const icarat_hip_ctx *ctx, icarat_mg_status *status);
