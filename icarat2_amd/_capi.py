@@ -79,7 +79,7 @@ class MgOpts(C.Structure):
     _fields_ = [("max_levels", C.c_int32), ("nu_pre", C.c_int32), ("nu_post", C.c_int32),
                 ("nu_coarse", C.c_int32), ("omega", C.c_double), ("batch_iters", C.c_int32),
                 ("dense_dofs", C.c_int32), ("direct_nodes", C.c_int32), ("coarse_avg", C.c_int32),
-                ("w_levels", C.c_int32), ("w_switch", C.c_double)]
+                ("w_levels", C.c_int32), ("w_switch", C.c_double), ("compensated", C.c_int32)]
 
 
 class MgStatus(C.Structure):

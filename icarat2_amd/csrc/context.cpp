@@ -138,6 +138,15 @@ void Context::upload_dof() {
   const int numeq = prob_.numeq();
   d_free.alloc(L_.numnp);
   h2d(d_free.p, prob_.free_mask().data(), L_.numnp, stream_);
+  {
+    // a node column x = 0 without free DOFs (a clamped face) is left to the
+    // z-marching K.u's first tile column, whose tiles then start at x = 1
+    const auto &fm = prob_.free_mask();
+    bool none = L_.NX > 1;
+    for (long long k = 0; k < (long long)L_.NY * L_.NZ && none; ++k) none = fm[k * L_.NX] == 0;
+    kc_.xlo = none ? 1 : 0;
+    drop_graphs();
+  }
   d_dof.alloc(L_.numnp * 3);
   h2d(d_dof.p, prob_.dof().data(), L_.numnp * 3, stream_);
   d_dirval.alloc(L_.numnp * 3);
@@ -1387,9 +1396,9 @@ void Context::drop_graphs() {
 void Context::mg_setup(const icarat_mg_opts &o) {
   require_dof();
   if (o.max_levels < 1 || o.nu_pre < 0 || o.nu_post < 1 || o.nu_coarse < 1 || !(o.omega > 0.0) ||
-      !(o.omega < 2.0) || o.batch_iters < 1 || o.batch_iters > 64 || o.dense_dofs < 0 ||
+      !(o.omega < 2.0) || o.batch_iters < 1 || o.batch_iters > 64 || o.dense_dofs < -1 ||
       o.direct_nodes < 0 || o.coarse_avg < 0 || o.coarse_avg > 5 || o.w_levels < -1 ||
-      (o.w_levels != 0 && !(o.w_switch >= 1.0)))
+      (o.w_levels != 0 && !(o.w_switch >= 1.0)) || o.compensated < 0 || o.compensated > 1)
     throw Error(kInvalidArgument, "invalid multigrid options");
   sync();
   mg_.reset();
@@ -1418,6 +1427,7 @@ void Context::set_stencil(int kernel, int sym_max) {
   }
   k.kernel = kernel;
   k.ke32 = d_ke32.p;
+  k.xlo = kc_.xlo;
   sync();
   kc_ = k;
   drop_graphs();

@@ -254,10 +254,11 @@ __device__ __forceinline__ void fetch_plane(const Lattice &L, int x0, int y0, in
 }
 
 // Writes a fetched plane (and layer) into LDS; PCG: forms p = dinv r + beta
-// p_old and stores it to p_new for owned nodes.
+// p_old and stores it to p_new for owned nodes (own_x0: the halo column at
+// x = 0 is owned too, for tiles that start at x = 1).
 template <bool PCG, class T>
 __device__ __forceinline__ void commit_plane(const Lattice &L, int x0, int y0, int zp, bool own_z,
-                                             double beta, const PlaneFetch<PCG, T> &F,
+                                             bool own_x0, double beta, const PlaneFetch<PCG, T> &F,
                                              double *__restrict__ p_new, T (*sp)[ZPN], T *se) {
 #pragma unroll
   for (int k = 0; k < ZPER; ++k) {
@@ -273,7 +274,8 @@ __device__ __forceinline__ void commit_plane(const Lattice &L, int x0, int y0, i
     if constexpr (PCG) {
       const int hx = t % ZPX, hy = t / ZPX;
       const int gx = x0 - 1 + hx, gy = y0 - 1 + hy;
-      if (own_z && hx >= 1 && hx <= ZX && hy >= 1 && hy <= ZY && gx < L.NX && gy < L.NY) {
+      const bool own_col = (hx >= 1 && hx <= ZX) || (own_x0 && hx == 0);
+      if (own_z && own_col && hy >= 1 && hy <= ZY && gx < L.NX && gy < L.NY) {
         const long long n = gx + (long long)L.NX * (gy + (long long)L.NY * zp);
 #pragma unroll
         for (int c = 0; c < 3; ++c) p_new[c * L.Np + n] = v[c];
@@ -341,6 +343,18 @@ __device__ __forceinline__ void layer_contract(const T *__restrict__ c_KE0, cons
 // y receives x' = x + omega dinv (b - K x) (out of place: neighbouring columns
 // and the next z-chunk still read x) and the block partials of x'.b and x'.q.
 // The epilogue computes in FP64 and y, b, dinv, q are FP64 whatever T is.
+// The work split of a z-marching launch.  Tiles of ZX x ZY node columns start
+// at node column x = xlo: 1 when column x = 0 holds no free DOF (a clamped
+// face; its K.u is 0 and the first tile column writes it from its halo), else
+// 0.  The W = tiles x NZ tile-planes are dealt in equal consecutive ranges
+// (tile-major, z fastest) to the nb blocks, so one round of resident
+// workgroups covers the lattice however its tiles divide it; a block marches
+// one segment per tile its range touches.
+struct ZSplit {
+  int xlo, nbx, nby, nb;
+  long long W;
+};
+
 // XCD-aware tile order: workgroups go to the 8 XCDs round-robin by id, so
 // each XCD is given a contiguous range of the nt work items (columns fastest):
 // one XCD marches neighbouring columns of one z-slab and shares their halos in
@@ -355,43 +369,38 @@ __device__ __forceinline__ unsigned xcd_slot(unsigned t, unsigned nt) {
   return ((t - big) % q) * 8 + rem + (t - big) / q;
 }
 
-// The z-marching K.u of tile T_ (column T_ % ncols, z-chunk T_ / ncols); the
-// block's partials go to slot pslot of Gp.
-template <bool PCG, int SYM, bool FUSE, bool JAC, class T, class TO>
-__device__ __forceinline__ void stencil64z_tile(
-    const Lattice &L, int nchunks, const T *__restrict__ c_KE0, const KeValsT<T> &kv,
-    const T *__restrict__ young, const uint8_t *__restrict__ fmask, const T *__restrict__ src,
-    const double *__restrict__ dinv, const double *__restrict__ p_old, double *__restrict__ p_new,
-    TO *__restrict__ y, PcgScalars *S, double *partials, const double *__restrict__ pu,
-    unsigned Gu, int j, const JacEpilogue &je, unsigned T_, unsigned Gp, unsigned pslot,
-    bool leader, T (*sp)[3][ZPN], T (*se)[ZLN], double *sh) {
-  static_assert(!PCG || std::is_same<T, double>::value, "the PCG stencil is FP64");
-  double beta;
-  if (!pcg_prologue<PCG, FUSE>(S, pu, Gu, j, sh, beta, leader)) return;
-  const int nbx = (L.NX + ZX - 1) / ZX, nby = (L.NY + ZY - 1) / ZY, ncols = nbx * nby;
-  const int col = (int)(T_ % ncols), zc = (int)(T_ / ncols);
-  const int x0 = (col % nbx) * ZX, y0 = (col / nbx) * ZY;
-  // balanced z-chunks: chunk zc covers node planes [z0, z1)
-  const int z0 = (int)((long long)zc * L.NZ / nchunks);
-  const int z1 = (int)((long long)(zc + 1) * L.NZ / nchunks);
+// One segment of a block's z-marching K.u: node planes [z0, z1) of tile t
+// (element layers z0-1 .. z1-1: the layer below z0 completes plane z0).  The
+// block partials accumulate into pq (PCG) and zr, zq (JAC).
+template <bool PCG, int SYM, bool JAC, class T, class TO>
+__device__ __forceinline__ void stencil64z_seg(
+    const Lattice &L, const ZSplit &Z, int t, int z0, int z1, double beta,
+    const T *__restrict__ c_KE0, const KeValsT<T> &kv, const T *__restrict__ young,
+    const uint8_t *__restrict__ fmask, const T *__restrict__ src, const double *__restrict__ dinv,
+    const double *__restrict__ p_old, double *__restrict__ p_new, TO *__restrict__ y,
+    const JacEpilogue &je, T (*sp)[3][ZPN], T (*se)[ZLN], double &pq, double &zr, double &zq) {
+  const int x0 = Z.xlo + (t % Z.nbx) * ZX, y0 = (t / Z.nbx) * ZY;
   const int tx = threadIdx.x % ZX, ty = threadIdx.x / ZX;
   const int gx = x0 + tx, gy = y0 + ty;
   const bool col_in = gx < L.NX && gy < L.NY;
+  // tiles starting at x = 1: the first tile column also writes node column
+  // x = 0 (no free DOF, K.u = 0 there) from the halo it holds
+  const bool own_x0 = Z.xlo == 1 && x0 == 1;
+  const bool edge = own_x0 && tx == 0 && gy < L.NY;
   PlaneFetch<PCG, T> F;
   T Ulo[9][3], Uhi[9][3];
   // Slot s holds node plane zb+1 and element layer zb of the step that reads
   // it.  Plane z0-1 goes to registers through slot 1.
   fetch_plane<PCG, T>(L, x0, y0, z0 - 1, src, dinv, p_old, young, F);
-  commit_plane<PCG, T>(L, x0, y0, z0 - 1, false, beta, F, p_new, sp[1], se[1]);
+  commit_plane<PCG, T>(L, x0, y0, z0 - 1, false, own_x0, beta, F, p_new, sp[1], se[1]);
   __syncthreads();
   read_nbhd(sp[1], tx, ty, Ulo);
   __syncthreads();
   // plane z0 and layer z0-1 -> slot 0 for step 0
   fetch_plane<PCG, T>(L, x0, y0, z0, src, dinv, p_old, young, F);
-  commit_plane<PCG, T>(L, x0, y0, z0, z0 < z1, beta, F, p_new, sp[0], se[0]);
+  commit_plane<PCG, T>(L, x0, y0, z0, z0 < z1, own_x0, beta, F, p_new, sp[0], se[0]);
   __syncthreads();
   T acc[3] = {T(0), T(0), T(0)};
-  double pq = 0.0, zr = 0.0, zq = 0.0;
   const int nsteps = z1 - z0 + 1;  // element layers z0-1 .. z1-1
   for (int k = 0; k < nsteps; ++k) {
     const int zb = z0 - 1 + k;
@@ -430,14 +439,60 @@ __device__ __forceinline__ void stencil64z_tile(
       }
       if constexpr (PCG) pq += Ulo[4][0] * out[0] + Ulo[4][1] * out[1] + Ulo[4][2] * out[2];
     }
+    if (k >= 1 && edge) {
+      // node (0, gy, zb): every component constrained, K.u masked to 0
+      const long long n = (long long)L.NX * (gy + (long long)L.NY * zb);
+      if constexpr (JAC) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          const long long ic = c * L.Np + n;
+          const double bc = je.b[ic];
+          const double xn = (double)Ulo[3][c] + je.omega * dinv[ic] * (bc - (double)T(0));
+          y[ic] = xn;
+          zr += xn * bc;
+          zq += xn * je.q[ic];
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) y[c * L.Np + n] = (TO)T(0);
+      }
+    }
 #pragma unroll
     for (int c = 0; c < 3; ++c) acc[c] = top[c];
 #pragma unroll
     for (int o = 0; o < 9; ++o)
 #pragma unroll
       for (int c = 0; c < 3; ++c) Ulo[o][c] = Uhi[o][c];
-    if (more) commit_plane<PCG, T>(L, x0, y0, zb + 2, zb + 2 < z1, beta, F, p_new, sp[nxt], se[nxt]);
+    if (more)
+      commit_plane<PCG, T>(L, x0, y0, zb + 2, zb + 2 < z1, own_x0, beta, F, p_new, sp[nxt],
+                           se[nxt]);
     __syncthreads();
+  }
+}
+
+// A block's share of a z-marching K.u: work range bw of the split Z (its
+// consecutive tile-planes, one segment per tile they touch); the block's
+// partials go to slot pslot of Gp.
+template <bool PCG, int SYM, bool FUSE, bool JAC, class T, class TO>
+__device__ __forceinline__ void stencil64z_work(
+    const Lattice &L, const ZSplit &Z, unsigned bw, const T *__restrict__ c_KE0,
+    const KeValsT<T> &kv, const T *__restrict__ young, const uint8_t *__restrict__ fmask,
+    const T *__restrict__ src, const double *__restrict__ dinv, const double *__restrict__ p_old,
+    double *__restrict__ p_new, TO *__restrict__ y, PcgScalars *S, double *partials,
+    const double *__restrict__ pu, unsigned Gu, int j, const JacEpilogue &je, unsigned Gp,
+    unsigned pslot, bool leader, T (*sp)[3][ZPN], T (*se)[ZLN], double *sh) {
+  static_assert(!PCG || std::is_same<T, double>::value, "the PCG stencil is FP64");
+  double beta;
+  if (!pcg_prologue<PCG, FUSE>(S, pu, Gu, j, sh, beta, leader)) return;
+  double pq = 0.0, zr = 0.0, zq = 0.0;
+  const long long u1 = (long long)(bw + 1) * Z.W / Z.nb;
+  for (long long u = (long long)bw * Z.W / Z.nb; u < u1;) {
+    const int t = (int)(u / L.NZ);
+    const int z0 = (int)(u - (long long)t * L.NZ);
+    const int z1 = (int)std::min<long long>(L.NZ, u1 - (long long)t * L.NZ);
+    stencil64z_seg<PCG, SYM, JAC, T, TO>(L, Z, t, z0, z1, beta, c_KE0, kv, young, fmask, src, dinv,
+                                         p_old, p_new, y, je, sp, se, pq, zr, zq);
+    u = (long long)t * L.NZ + z1;
   }
   if constexpr (PCG) {
     double v[1] = {pq};
@@ -451,7 +506,7 @@ __device__ __forceinline__ void stencil64z_tile(
 
 template <bool PCG, int SYM, bool FUSE, bool JAC, class T = double, class TO = OutT<JAC, T>>
 __global__ __launch_bounds__(NT) void k_stencil64z(
-    Lattice L, int nchunks, const T *__restrict__ c_KE0, KeValsT<T> kv, const T *__restrict__ young,
+    Lattice L, ZSplit Z, const T *__restrict__ c_KE0, KeValsT<T> kv, const T *__restrict__ young,
     const uint8_t *__restrict__ fmask, const T *__restrict__ src,
     const double *__restrict__ dinv, const double *__restrict__ p_old, double *__restrict__ p_new,
     TO *__restrict__ y, PcgScalars *S, double *partials, const double *__restrict__ pu,
@@ -459,24 +514,24 @@ __global__ __launch_bounds__(NT) void k_stencil64z(
   __shared__ T sp[2][3][ZPN];
   __shared__ T se[2][ZLN];
   __shared__ double sh[8];
-  const unsigned nt = gridDim.x, b = blockIdx.x;
-  stencil64z_tile<PCG, SYM, FUSE, JAC, T, TO>(L, nchunks, c_KE0, kv, young, fmask, src, dinv,
-                                              p_old, p_new, y, S, partials, pu, Gu, j, je,
-                                              xcd_order(b, nt), nt, b, first_thread(), sp, se, sh);
+  const unsigned b = blockIdx.x;
+  stencil64z_work<PCG, SYM, FUSE, JAC, T, TO>(L, Z, xcd_order(b, Z.nb), c_KE0, kv, young, fmask,
+                                              src, dinv, p_old, p_new, y, S, partials, pu, Gu, j,
+                                              je, Z.nb, b, first_thread(), sp, se, sh);
 }
 
 // Several right-hand sides with one operator (load cases): nc cases of a
 // Jacobi-PCG batch step in one launch.  Case c's vectors start at c * cs
 // (p: two buffers per case, c * 2 cs, chosen by the parity of the case's
 // iteration count at the batch start, as the single-case batches do); its
-// scalars are S[c]; its partials are stored at [c * nt, (c+1) * nt) in the
-// slots of the single-case launch, so each case's sums equal those of its
-// own solve bit for bit.  Work items run case-fastest: the cases of one tile
-// land on the same XCD together and read the tile's young and mask through
-// its L2.
+// scalars are S[c]; its partials are stored at [c * nb, (c+1) * nb) in the
+// slots of the single-case launch (same split), so each case's sums equal
+// those of its own solve bit for bit.  Work items run case-fastest: the cases
+// of one work range land on the same XCD together and read its young and
+// mask through its L2.
 template <int SYM, bool FUSE>
 __global__ __launch_bounds__(NT) void k_stencil64z_mc(
-    Lattice L, int nchunks, const double *__restrict__ c_KE0, KeVals kv,
+    Lattice L, ZSplit Z, const double *__restrict__ c_KE0, KeVals kv,
     const double *__restrict__ young, const uint8_t *__restrict__ fmask,
     const double *__restrict__ r, const double *__restrict__ dinv, double *__restrict__ p,
     double *__restrict__ q, PcgScalars *S, double *partials, const double *__restrict__ pu,
@@ -484,16 +539,16 @@ __global__ __launch_bounds__(NT) void k_stencil64z_mc(
   __shared__ double sp[2][3][ZPN];
   __shared__ double se[2][ZLN];
   __shared__ double sh[8];
-  const unsigned ntot = gridDim.x, nt = ntot / nc;
+  const unsigned ntot = gridDim.x, nt = (unsigned)Z.nb;
   const unsigned w = xcd_order(blockIdx.x, ntot);
   const int c = (int)(w % nc);
   const unsigned T_ = w / nc;
   PcgScalars *Sc = S + c;
   const int par = (j + Sc->iter_base) & 1;
   double *pc = p + 2 * cs * c;
-  stencil64z_tile<true, SYM, FUSE, false, double, double>(
-      L, nchunks, c_KE0, kv, young, fmask, r + cs * c, dinv, pc + cs * par, pc + cs * (par ^ 1),
-      q + cs * c, Sc, partials + (size_t)nt * c, pu + (size_t)2 * Gu * c, Gu, j, JacEpilogue{}, T_,
+  stencil64z_work<true, SYM, FUSE, false, double, double>(
+      L, Z, T_, c_KE0, kv, young, fmask, r + cs * c, dinv, pc + cs * par, pc + cs * (par ^ 1),
+      q + cs * c, Sc, partials + (size_t)nt * c, pu + (size_t)2 * Gu * c, Gu, j, JacEpilogue{},
       nt, xcd_slot(T_, nt), T_ == 0 && threadIdx.x == 0, sp, se, sh);
 }
 
@@ -858,14 +913,25 @@ int stencil_nchunks(const Lattice &L) {
   return std::max(1, std::min(L.NZ / 2, resident_workgroups() / ncols));
 }
 
-static dim3 zgrid(const Lattice &L, int nchunks) {
-  const int ncols = ((L.NX + ZX - 1) / ZX) * ((L.NY + ZY - 1) / ZY);
-  return dim3((unsigned)(ncols * nchunks));
+// The split of a z-marching launch: one round of resident workgroups, each
+// with at least 2 tile-planes; the autotuner's nchunks caps the blocks at
+// tiles x nchunks.
+static ZSplit make_zsplit(const Lattice &L, const KeCoef &ke) {
+  ZSplit z;
+  z.xlo = ke.xlo == 1 && L.NX > 1 ? 1 : 0;
+  z.nbx = (L.NX - z.xlo + ZX - 1) / ZX;
+  z.nby = (L.NY + ZY - 1) / ZY;
+  const long long tiles = (long long)z.nbx * z.nby;
+  z.W = tiles * L.NZ;
+  long long nb = std::min<long long>(resident_workgroups(), z.W / 2);
+  if (ke.nchunks > 0) nb = std::min<long long>(nb, tiles * ke.nchunks);
+  z.nb = (int)std::max<long long>(1, nb);
+  return z;
 }
 
 long long stencil_blocks(const Lattice &L) {
-  const dim3 a = stencil_grid(L), b = zgrid(L, stencil_nchunks(L));
-  return std::max((long long)a.x * a.y * a.z, (long long)b.x * b.y * b.z);
+  const dim3 a = stencil_grid(L);
+  return std::max((long long)a.x * a.y * a.z, (long long)resident_workgroups());
 }
 
 KeCoef make_ke_coef_class(const double *ke, const double *ke_dev, int cls) {
@@ -901,13 +967,11 @@ static void launch_stencil64_t(const Lattice &L, const KeCoef &ke, const double 
                                double *partials, const double *pu, unsigned Gu, int j,
                                hipStream_t s, unsigned *nblocks, JacEpilogue je = JacEpilogue{}) {
   if (ke.kernel == 0) {
-    const int def = stencil_nchunks(L);
-    const int nch = ke.nchunks > 0 ? std::min(ke.nchunks, def) : def;
-    const dim3 g = zgrid(L, nch);
-    hipLaunchKernelGGL((k_stencil64z<PCG, SYM, FUSE, JAC>), g, dim3(NT), 0, s, L, nch, ke.ke,
-                       ke.vals, young, freemask, src, dinv, p_old, p_new, y, S, partials, pu, Gu,
-                       j, je);
-    *nblocks = g.x * g.y * g.z;
+    const ZSplit z = make_zsplit(L, ke);
+    hipLaunchKernelGGL((k_stencil64z<PCG, SYM, FUSE, JAC>), dim3((unsigned)z.nb), dim3(NT), 0, s, L,
+                       z, ke.ke, ke.vals, young, freemask, src, dinv, p_old, p_new, y, S, partials,
+                       pu, Gu, j, je);
+    *nblocks = (unsigned)z.nb;
   } else {
     if (JAC) throw Error(kInternal, "the fused smoother needs the z-marching kernel");
     const dim3 g = stencil_grid(L);
@@ -975,11 +1039,10 @@ static unsigned launch_k32(const Lattice &L, const KeCoef &ke, const float *youn
                            const uint8_t *freemask, const float *x, double *y,
                            const double *dinv, double *partials, JacEpilogue je, hipStream_t s) {
   if (ke.sym == 0 && !ke.ke32) throw Error(kInternal, "FP32 K.u without an FP32 KE0");
-  const int def = stencil_nchunks(L);
-  const int nch = ke.nchunks > 0 ? std::min(ke.nchunks, def) : def;
-  const dim3 g = zgrid(L, nch);
+  const ZSplit z = make_zsplit(L, ke);
+  const dim3 g((unsigned)z.nb);
   auto go = [&](auto kern) {
-    hipLaunchKernelGGL(kern, g, dim3(NT), 0, s, L, nch, ke.ke32, ke.vals32, young32, freemask, x,
+    hipLaunchKernelGGL(kern, g, dim3(NT), 0, s, L, z, ke.ke32, ke.vals32, young32, freemask, x,
                        dinv, nullptr, nullptr, y, nullptr, partials, nullptr, 0u, 0, je);
   };
   switch (ke.sym) {
@@ -1043,17 +1106,16 @@ unsigned launch_pcg_stencil64_mc(const Lattice &L, const KeCoef &ke, const doubl
                                  long long cs, hipStream_t s) {
   if (ke.kernel != 0) throw Error(kInvalidArgument, "load cases together need the z-marching K.u");
   if (nc < 1) throw Error(kInvalidArgument, "no load cases");
-  const int def = stencil_nchunks(L);
-  const int nch = ke.nchunks > 0 ? std::min(ke.nchunks, def) : def;
-  const unsigned nt = zgrid(L, nch).x;
+  const ZSplit z = make_zsplit(L, ke);
+  const unsigned nt = (unsigned)z.nb;
   const dim3 g(nt * (unsigned)nc);
 #define MC_LAUNCH(SYM_)                                                                       \
   if (j == 0)                                                                                 \
-    hipLaunchKernelGGL((k_stencil64z_mc<SYM_, false>), g, dim3(NT), 0, s, L, nch, ke.ke,      \
+    hipLaunchKernelGGL((k_stencil64z_mc<SYM_, false>), g, dim3(NT), 0, s, L, z, ke.ke,        \
                        ke.vals, young, freemask, r, dinv, p, q, S, partials_st, nullptr, 0u,  \
                        0, nc, cs);                                                            \
   else                                                                                        \
-    hipLaunchKernelGGL((k_stencil64z_mc<SYM_, true>), g, dim3(NT), 0, s, L, nch, ke.ke,       \
+    hipLaunchKernelGGL((k_stencil64z_mc<SYM_, true>), g, dim3(NT), 0, s, L, z, ke.ke,         \
                        ke.vals, young, freemask, r, dinv, p, q, S, partials_st, partials_up,  \
                        Gu, j, nc, cs);
   switch (ke.sym) {

@@ -318,11 +318,17 @@ __global__ void k_fpcg_check(const double *__restrict__ partials, unsigned G, Pc
 // partials in the same order, so all reach the same alpha = rho / p.q and the
 // same breakdown decision; x += alpha p; r -= alpha q; r.r partials; with z0:
 // z0 = omega dinv r as well (the first sweep of a V-cycle on r).
-template <class TZ>
+// COMP: x carries the low part xlo of a compensated sum (x + xlo is the exact
+// sum of the updates; x is its rounding).  Near the FP64 floor of a
+// high-contrast field the updates alpha p of the large displacements of
+// near-void regions fall below half an ulp of x and would be lost, while the
+// recursive residual counts them.
+template <class TZ, bool COMP>
 __global__ __launch_bounds__(NT) void k_fpcg_update_alpha(
     long long n, double *__restrict__ x, double *__restrict__ r, const double *__restrict__ p,
     const double *__restrict__ q, PcgScalars *S, const double *__restrict__ part_st, unsigned Gs,
-    double *part_up, const double *__restrict__ dinv, double omega, TZ *__restrict__ z0) {
+    double *part_up, const double *__restrict__ dinv, double omega, TZ *__restrict__ z0,
+    double *__restrict__ xlo) {
   __shared__ double sh[8];
   if (S->done) return;
   double tot[1];
@@ -342,7 +348,15 @@ __global__ __launch_bounds__(NT) void k_fpcg_update_alpha(
   double rr = 0.0;
   const long long stride = (long long)gridDim.x * NT;
   for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += stride) {
-    x[i] += alpha * p[i];
+    if constexpr (COMP) {
+      // two-sum of x and the update (with the carried low part)
+      const double a = x[i], b = fma(alpha, p[i], xlo[i]);
+      const double sum = a + b, bb = sum - a;
+      xlo[i] = (a - (sum - bb)) + (b - bb);
+      x[i] = sum;
+    } else {
+      x[i] += alpha * p[i];
+    }
     const double ri = r[i] - alpha * q[i];
     r[i] = ri;
     rr += ri * ri;
@@ -504,9 +518,13 @@ void launch_fpcg_update(long long n, double *x, double *r, const double *p, cons
 unsigned launch_fpcg_update_alpha(long long n, double *x, double *r, const double *p,
                                   const double *q, PcgScalars *S, const double *part_st,
                                   unsigned Gs, double *part_up, const double *dinv, double omega,
-                                  double *z0, hipStream_t s) {
-  hipLaunchKernelGGL(k_fpcg_update_alpha<double>, dim3(VEC_BLOCKS), dim3(NT), 0, s, n, x, r, p, q,
-                     S, part_st, Gs, part_up, dinv, omega, z0);
+                                  double *z0, hipStream_t s, double *xlo) {
+  if (xlo)
+    hipLaunchKernelGGL((k_fpcg_update_alpha<double, true>), dim3(VEC_BLOCKS), dim3(NT), 0, s, n, x,
+                       r, p, q, S, part_st, Gs, part_up, dinv, omega, z0, xlo);
+  else
+    hipLaunchKernelGGL((k_fpcg_update_alpha<double, false>), dim3(VEC_BLOCKS), dim3(NT), 0, s, n, x,
+                       r, p, q, S, part_st, Gs, part_up, dinv, omega, z0, nullptr);
   HIPCHK(hipGetLastError());
   return VEC_BLOCKS;
 }
@@ -514,9 +532,13 @@ unsigned launch_fpcg_update_alpha(long long n, double *x, double *r, const doubl
 unsigned launch_fpcg_update_alpha(long long n, double *x, double *r, const double *p,
                                   const double *q, PcgScalars *S, const double *part_st,
                                   unsigned Gs, double *part_up, const double *dinv, double omega,
-                                  float *z0, hipStream_t s) {
-  hipLaunchKernelGGL(k_fpcg_update_alpha<float>, dim3(VEC_BLOCKS), dim3(NT), 0, s, n, x, r, p, q,
-                     S, part_st, Gs, part_up, dinv, omega, z0);
+                                  float *z0, hipStream_t s, double *xlo) {
+  if (xlo)
+    hipLaunchKernelGGL((k_fpcg_update_alpha<float, true>), dim3(VEC_BLOCKS), dim3(NT), 0, s, n, x,
+                       r, p, q, S, part_st, Gs, part_up, dinv, omega, z0, xlo);
+  else
+    hipLaunchKernelGGL((k_fpcg_update_alpha<float, false>), dim3(VEC_BLOCKS), dim3(NT), 0, s, n, x,
+                       r, p, q, S, part_st, Gs, part_up, dinv, omega, z0, nullptr);
   HIPCHK(hipGetLastError());
   return VEC_BLOCKS;
 }

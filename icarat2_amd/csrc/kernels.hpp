@@ -25,6 +25,8 @@ struct KeCoef {
   int sym = 0;
   int kernel = 0;              // 0: z-marching column kernel, 1: 3-D tile kernel
   int nchunks = 0;             // z-marching: z-chunks per column (0: default, capped by it)
+  int xlo = 0;                 // 1: node column x = 0 holds no free DOF (the z-marching
+                               // tiles start at x = 1)
   KeVals vals{};
   // the same in FP32 for the FP32 z-marching K.u (multigrid fine level)
   const float *ke32 = nullptr; // device KE0 in FP32

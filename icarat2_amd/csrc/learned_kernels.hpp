@@ -44,17 +44,18 @@ void launch_fpcg_update(long long n, double *x, double *r, const double *p, cons
 // Flexible PCG with the scalar steps folded into neighbouring kernels (used by
 // the multigrid solve).  launch_fpcg_update_alpha: alpha = rho / p.q from the
 // stencil's partials (Gs blocks) in every block, x and r updated, r.r partials
-// to part_up, and with z0 != nullptr also z0 = omega dinv r.  Returns the
-// partials count.
+// to part_up, and with z0 != nullptr also z0 = omega dinv r.  With xlo !=
+// nullptr, x is updated as a compensated sum whose low part is xlo.  Returns
+// the partials count.
 unsigned launch_fpcg_update_alpha(long long n, double *x, double *r, const double *p,
                                   const double *q, PcgScalars *S, const double *part_st,
                                   unsigned Gs, double *part_up, const double *dinv, double omega,
-                                  double *z0, hipStream_t s);
+                                  double *z0, hipStream_t s, double *xlo = nullptr);
 // the same with z0 in FP32 (the FP32 fine level of the multigrid V-cycle)
 unsigned launch_fpcg_update_alpha(long long n, double *x, double *r, const double *p,
                                   const double *q, PcgScalars *S, const double *part_st,
                                   unsigned Gs, double *part_up, const double *dinv, double omega,
-                                  float *z0, hipStream_t s);
+                                  float *z0, hipStream_t s, double *xlo = nullptr);
 // z.r and z.q partials; returns their count
 unsigned launch_zdot_partials(long long n, const double *z, const double *r, const double *q,
                               double *part_z, hipStream_t s);

@@ -60,7 +60,7 @@ void launch_mg_prolong_jacobi(const Lattice &Lf, const Lattice &Lc, const KeCoef
 // Exact solve on a small lattice through the dense inverse of its K.  Dense
 // index d = c * numnp + n for lattice DOF c * Np + n; constrained DOFs carry
 // identity rows and columns.
-constexpr int MG_DENSE_MAX = 4096;  // largest 3 numnp solved densely
+constexpr int MG_DENSE_MAX = 6144;  // largest 3 numnp solved densely
 // A = K of the lattice (n = 3 numnp, n x n, symmetric)
 void launch_mg_dense_assemble(const Lattice &L, const double *ke, const double *young,
                               const uint8_t *fmask, double *A, hipStream_t s,

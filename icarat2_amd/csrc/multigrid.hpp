@@ -29,11 +29,12 @@ inline icarat_mg_opts mg_default_opts() {
   o.nu_coarse = 6;
   o.omega = 1.95;
   o.batch_iters = 4;
-  o.dense_dofs = 1024;
+  o.dense_dofs = -1;
   o.direct_nodes = 262144;
   o.coarse_avg = 0;
   o.w_levels = -1;
   o.w_switch = 30.0;
+  o.compensated = 1;
   return o;
 }
 
@@ -124,6 +125,7 @@ class Multigrid {
   std::vector<double> omega_;                  // Jacobi damping per level
   DevBuf<double> z_, y0_, r0_;
   DevBuf<double> xbest_;  // best verified iterate of a solve that stalls at its floor
+  DevBuf<double> xlo_;    // low part of the compensated iterate (unallocated: plain updates)
   static constexpr int kStallVerifications = 4, kMaxVerifications = 50;
   double best_ = 0.0;
   int since_best_ = 0, verifications_ = 0;

@@ -469,10 +469,11 @@ typedef struct icarat_mg_opts {
                               KE0) >= lambda_max(D^-1 K) for any densities; in (0, 2) */
   int32_t batch_iters;     /* PCG iterations per graph launch */
   int32_t dense_dofs;      /* coarsening stops at the first grid of at most this many lattice
-                              DOFs (3 x nodes, capped at 4096), which is solved exactly by
+                              DOFs (3 x nodes, capped at 6144), which is solved exactly by
                               its dense inverse; a coarsest grid that cannot be halved
                               further is solved exactly up to 1.5 x this many (0: never;
-                              the coarsest grid is smoothed) */
+                              the coarsest grid is smoothed; -1: automatic, 1/3000 of the
+                              fine lattice DOFs, at least 1024) */
   int32_t direct_nodes;    /* levels of at most this many nodes smooth with the fused
                               one-thread-per-node sweep; larger ones with the z-marching K.u */
   int32_t coarse_avg;      /* coarse operators: a coarse element's Young's modulus from its 8
@@ -488,8 +489,13 @@ typedef struct icarat_mg_opts {
                               restarts with beta = 0) once its recursive relative residual is
                               below w_switch x tol: the W-cycle's coarse corrections lose
                               accuracy near the FP64 floor of high-contrast fields */
+  int32_t compensated;     /* 1: the PCG iterate accumulates its updates as a compensated sum
+                              (x plus a low-part vector, x its rounding), so updates below
+                              half an ulp of x -- the large displacements of near-void
+                              regions near the tolerance -- are not lost while the recursive
+                              residual counts them; 0: x += alpha p */
 } icarat_mg_opts;
-/* Fills *o with the defaults (10, 1, 1, 6, 1.95, 4, 1024, 262144, 0, -1, 30). */
+/* Fills *o with the defaults (10, 1, 1, 6, 1.95, 4, -1, 262144, 0, -1, 30, 1). */
 int icarat_hip_mg_default_opts(icarat_mg_opts *o);
 /* (Re)builds the hierarchy of the context (o = NULL: defaults).  Needs make_dof. */
 int icarat_hip_mg_setup(icarat_hip_ctx *ctx, const icarat_mg_opts *o);
