@@ -115,12 +115,13 @@ def traffic_from_profiles(key, workload):
 def mg_fine_bytes(numnp, nelx):
     """Compulsory HBM bytes of one multigrid PCG iteration on the fine level
     (coarse levels excluded), V = one FP64 lattice vector (3 numnp doubles):
-    PCG stencil z, p_old in, p, q out (4V); update x, p, r, q, dinv in, x, r, z0
-    out (8V); residual K z0 (2V); restriction r, K z0 in (2V); prolongation x
-    in and out (2V); fused last sweep x, b, dinv, q in, z out (5V); young and
-    the free mask for each of the three fine K.u passes."""
+    PCG stencil z, p_old in, p, q out (4V); update x, its compensation part xlo,
+    p, r, q, dinv in, x, xlo, r, z0 out (10V); residual K z0 (2V); restriction
+    r, K z0 in (2V); prolongation x in and out (2V); fused last sweep x, b,
+    dinv, q in, z out (5V); young and the free mask for each of the three fine
+    K.u passes."""
     V = 3.0 * numnp * 8
-    return 23.0 * V + 3.0 * (8.0 * nelx + numnp)
+    return 25.0 * V + 3.0 * (8.0 * nelx + numnp)
 
 
 def log(msg):
@@ -520,7 +521,8 @@ def main():
                      "traffic": traffic_from_profiles(tkey, dims_s),
                      "kernel": kname, "avg_ms": ms, "algorithmic_flops": flops,
                      "compulsory_bytes": nbytes, "hbm_gbs_at_compulsory_bytes": nbytes / t_s / 1e9},
-        "vector_kernel": {"kernel": "k_fpcg_update_alpha" if mg_like else "k_pcg_update64f",
+        "vector_kernel": {"kernel": ("k_fpcg_update_alpha<compensated x>" if mg_like
+                                     else "k_pcg_update64f"),
                           "avg_ms": ms_v, "bytes": bytes_v, "gbs": bytes_v / (ms_v * 1e-3) / 1e9},
         "copy_kernel": {"kernel": "k_copy64", "avg_ms": ms_c, "bytes": bytes_c,
                         "gbs": bytes_c / (ms_c * 1e-3) / 1e9},
@@ -536,7 +538,7 @@ def main():
             t_it = st.solve_s / n_it
             b_it = mg_fine_bytes(float(c.numnp), float(c.nelx))
             out["iteration_roofline"] = {
-                "scope": "fine-level compulsory bytes of one multigrid PCG iteration (23 lattice "
+                "scope": "fine-level compulsory bytes of one multigrid PCG iteration (25 lattice "
                          "vectors + young and mask per fine K.u; coarse levels -- twice per "
                          "visit in the W-cycle phase --, launch gaps and the per-solve set-up "
                          "in the time, not in the bytes)",

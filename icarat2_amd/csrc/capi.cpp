@@ -527,7 +527,7 @@ int icarat_hip_mg_status(const icarat_hip_ctx *hc, icarat_mg_status *st) {
     o.levels = c.mg_->levels();
     o.dense = c.mg_->dense() ? 1 : 0;
     o.direct_fine = c.mg_->direct(0) ? 1 : 0;
-    o.fine_fp32 = c.mg_->fine_fp32() ? 1 : 0;
+    o.fine_fp32 = c.mg_->fine_fp32_on() ? 1 : 0;
     o.factorizations = c.mg_->factorizations();
     o.shifted = c.mg_->shifted();
     o.w_levels = c.mg_->w_levels();

@@ -1441,6 +1441,7 @@ void Context::time_kernel(int kernel, int reps, double *avg_ms, double *flops, d
   const double N = (double)L_.numnp, E = (double)L_.nelx;
   double F = 0.0, B = 0.0;
   std::function<void()> launch;
+  DevBuf<double> hook_xlo;
   if (kernel == 0 || (kernel >= 3 && kernel <= 6)) {
     // p = dinv r + beta p_old with r = f (non-zero) and beta = 0, so p.q > 0
     // and the stop flag never rises between launches
@@ -1538,8 +1539,11 @@ void Context::time_kernel(int kernel, int reps, double *avg_ms, double *flops, d
     B = 2.0 * 3 * N * 8 + 2.0 * 3 * N * 8 + E * 8 + N;  // z, p_old in; p, q out; young; mask
   } else if (kernel == 11) {
     // the multigrid solve's vector update: alpha = rho / p.q from stencil
-    // partials (ones; rho = 0 keeps the vectors fixed), x, r updated, r.r
-    // partials, z0 = omega dinv r for the V-cycle's first sweep
+    // partials (ones; rho = 0 keeps the vectors fixed), x, r updated (x as a
+    // compensated sum, as the solve runs it by default), r.r partials,
+    // z0 = omega dinv r for the V-cycle's first sweep
+    hook_xlo.alloc(n3_);
+    HIPCHK(hipMemsetAsync(hook_xlo.p, 0, hook_xlo.bytes(), stream_));
     PcgScalars s0{};
     s0.rho = 0.0;
     s0.tol = 0.0;
@@ -1549,12 +1553,12 @@ void Context::time_kernel(int kernel, int reps, double *avg_ms, double *flops, d
     const unsigned gs = (unsigned)std::min<long long>(stencil_blocks(L_), 4096);
     std::vector<double> ones(gs, 1.0);
     h2d(d_partials.p, ones.data(), gs, stream_);
-    launch = [this, gs]() {
+    launch = [this, gs, &hook_xlo]() {
       launch_fpcg_update_alpha(n3_, d_x, d_r, d_p[1], d_q, d_pcg, d_partials, gs, d_partials2,
-                               d_dinv, 1.0, d_y.p, stream_);
+                               d_dinv, 1.0, d_y.p, stream_, hook_xlo.p);
     };
-    F = 8.0 * 3 * N;
-    B = (5.0 + 3.0) * 3 * N * 8;                // x, p, r, q, dinv in; x, r, z0 out
+    F = 10.0 * 3 * N;
+    B = (6.0 + 4.0) * 3 * N * 8;                // x, xlo, p, r, q, dinv in; x, xlo, r, z0 out
   } else if (kernel == 9) {
     // one Helmholtz filter pass (density mode): element loads, the six 1-D
     // transforms (FP64 GEMMs) with the spectral scaling, element read-back

@@ -346,21 +346,50 @@ __global__ __launch_bounds__(NT) void k_fpcg_update_alpha(
   }
   if (!ok) return;
   double rr = 0.0;
-  const long long stride = (long long)gridDim.x * NT;
-  for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += stride) {
+  // one element: x (compensated with COMP), r, r.r, z0
+  auto elem = [&](double &xi, double &li, double pi, double &ri, double qi, double di, TZ *zi) {
     if constexpr (COMP) {
       // two-sum of x and the update (with the carried low part)
-      const double a = x[i], b = fma(alpha, p[i], xlo[i]);
+      const double a = xi, b = fma(alpha, pi, li);
       const double sum = a + b, bb = sum - a;
-      xlo[i] = (a - (sum - bb)) + (b - bb);
-      x[i] = sum;
+      li = (a - (sum - bb)) + (b - bb);
+      xi = sum;
     } else {
-      x[i] += alpha * p[i];
+      xi += alpha * pi;
     }
-    const double ri = r[i] - alpha * q[i];
-    r[i] = ri;
+    ri = ri - alpha * qi;
     rr += ri * ri;
-    if (z0) z0[i] = (TZ)(omega * dinv[i] * ri);
+    if (zi) *zi = (TZ)(omega * di * ri);
+  };
+  // pairs of elements with 16-byte accesses (the vectors are 16-byte aligned)
+  const long long half = n >> 1;
+  const long long stride = (long long)gridDim.x * NT;
+  for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < half; i += stride) {
+    double2 xv = reinterpret_cast<const double2 *>(x)[i];
+    double2 rv = reinterpret_cast<const double2 *>(r)[i];
+    const double2 pv = reinterpret_cast<const double2 *>(p)[i];
+    const double2 qv = reinterpret_cast<const double2 *>(q)[i];
+    double2 lv = make_double2(0.0, 0.0), dv = make_double2(0.0, 0.0);
+    if constexpr (COMP) lv = reinterpret_cast<const double2 *>(xlo)[i];
+    if (z0) dv = reinterpret_cast<const double2 *>(dinv)[i];
+    TZ zv[2];
+    elem(xv.x, lv.x, pv.x, rv.x, qv.x, dv.x, z0 ? &zv[0] : nullptr);
+    elem(xv.y, lv.y, pv.y, rv.y, qv.y, dv.y, z0 ? &zv[1] : nullptr);
+    reinterpret_cast<double2 *>(x)[i] = xv;
+    reinterpret_cast<double2 *>(r)[i] = rv;
+    if constexpr (COMP) reinterpret_cast<double2 *>(xlo)[i] = lv;
+    if (z0) {
+      z0[2 * i] = zv[0];
+      z0[2 * i + 1] = zv[1];
+    }
+  }
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+    const long long i = n - 1;
+    double li = COMP ? xlo[i] : 0.0;
+    TZ zi;
+    elem(x[i], li, p[i], r[i], q[i], z0 ? dinv[i] : 0.0, z0 ? &zi : nullptr);
+    if constexpr (COMP) xlo[i] = li;
+    if (z0) z0[i] = zi;
   }
   double v[1] = {rr};
   store_partials<NT, 1>(v, part_up, sh);

@@ -159,9 +159,11 @@ __global__ __launch_bounds__(256) void k_mg_restrict(Lattice Lf, Lattice Lc,
 }
 
 // xf += P xc (ADD) or xf = P xc (trilinear), masked by the fine free DOFs
+// (xc2 != nullptr: the coarse values are xc + xc2, the W-cycle's two visits)
 template <bool ADD, class TF = double>
 __global__ __launch_bounds__(256) void k_mg_prolong_add(Lattice Lf, Lattice Lc,
                                                         const double *__restrict__ xc,
+                                                        const double *__restrict__ xc2,
                                                         const uint8_t *__restrict__ fmask_f,
                                                         TF *__restrict__ xf) {
   __shared__ double sc[3][PCN];
@@ -191,7 +193,7 @@ __global__ __launch_bounds__(256) void k_mg_prolong_add(Lattice Lf, Lattice Lc,
     if (gx < Lc.NX && gy < Lc.NY && gz < Lc.NZ) {
       const long long q = gx + (long long)Lc.NX * (gy + (long long)Lc.NY * gz);
 #pragma unroll
-      for (int c = 0; c < 3; ++c) v[c] = xc[c * Lc.Np + q];
+      for (int c = 0; c < 3; ++c) v[c] = xc2 ? xc[c * Lc.Np + q] + xc2[c * Lc.Np + q] : xc[c * Lc.Np + q];
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) sc[c][t] = v[c];
@@ -301,6 +303,35 @@ __global__ __launch_bounds__(256) void k_mg_direct(Lattice L, const double *__re
   direct_out<MODE>(L.Np, n, fmask[n], y, xc, b, dinv, omega, out);
 }
 
+// The start of the W-cycle's second visit of a direct level, in one pass over
+// the first visit's result xc: u = xc (kept for the coarse correction),
+// r = b - K xc (masked) and, with x0, the second visit's first sweep from zero
+// x0 = omega dinv r.  xc may not alias u, r or x0.
+template <int SYM>
+__global__ __launch_bounds__(256) void k_mg_direct_wres(Lattice L, const double *__restrict__ c_KE0,
+                                                        KeVals kv, const double *__restrict__ young,
+                                                        const uint8_t *__restrict__ fmask,
+                                                        const double *__restrict__ xc,
+                                                        const double *__restrict__ b,
+                                                        const double *__restrict__ dinv,
+                                                        double omega, double *__restrict__ u,
+                                                        double *__restrict__ r,
+                                                        double *__restrict__ x0) {
+  const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (n >= L.numnp) return;
+  double y[3], xn[3];
+  direct_node<SYM>(L, c_KE0, kv, young, xc, n, y, xn);
+  const unsigned m = fmask[n];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const long long q = c * L.Np + n;
+    u[q] = xn[c];
+    const double v = ((m >> c) & 1u) ? b[q] - y[c] : 0.0;
+    r[q] = v;
+    if (x0) x0[q] = omega * dinv[q] * v;
+  }
+}
+
 // The coarse correction and the post-smoothing sweep of a direct level in one
 // pass: xp = x + P xc (masked by the fine free DOFs, as k_mg_prolong_add<true>
 // forms it), out = xp + omega dinv (b - K xp) (masked; xp itself is not
@@ -316,8 +347,9 @@ template <int SYM>
 __global__ __launch_bounds__(256) void k_mg_prolong_jacobi(
     Lattice Lf, Lattice Lc, const double *__restrict__ c_KE0, KeVals kv,
     const double *__restrict__ young, const uint8_t *__restrict__ fmask,
-    const double *__restrict__ xcoarse, const double *__restrict__ x, const double *__restrict__ b,
-    const double *__restrict__ dinv, double omega, double *__restrict__ out) {
+    const double *__restrict__ xcoarse, const double *__restrict__ xcoarse2,
+    const double *__restrict__ x, const double *__restrict__ b, const double *__restrict__ dinv,
+    double omega, double *__restrict__ out) {
   __shared__ double sc[3][QCN];
   __shared__ double su[3][QHN];
   const int fx0 = blockIdx.x * QTX, fy0 = blockIdx.y * QTY, fz0 = blockIdx.z * QTZ;
@@ -338,7 +370,10 @@ __global__ __launch_bounds__(256) void k_mg_prolong_jacobi(
     const bool in = t < QCN && gx >= 0 && gx < Lc.NX && gy >= 0 && gy < Lc.NY && gz >= 0 && gz < Lc.NZ;
     const long long q = in ? gx + (long long)Lc.NX * (gy + (long long)Lc.NY * gz) : 0;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) cv[r][c] = in ? xcoarse[c * Lc.Np + q] : 0.0;
+    for (int c = 0; c < 3; ++c)
+      cv[r][c] = in ? (xcoarse2 ? xcoarse[c * Lc.Np + q] + xcoarse2[c * Lc.Np + q]
+                                : xcoarse[c * Lc.Np + q])
+                    : 0.0;
   }
   double hv[NH][3];
   unsigned hm[NH];
@@ -486,20 +521,84 @@ __global__ void k_mg_dense_symmetrize(int n, double *__restrict__ A) {
 }
 
 // x = A b: one block per row, threads over the columns, block reduction
-__global__ __launch_bounds__(256) void k_mg_dense_solve(Lattice L, const double *__restrict__ A,
-                                                        const double *__restrict__ b,
-                                                        double *__restrict__ x) {
-  __shared__ double sh[4];
-  const int n = (int)(3 * L.numnp), np1 = (int)L.numnp;
-  const int row = blockIdx.x;
-  double s = 0.0;
-  for (int j = threadIdx.x; j < n; j += 256)
-    s += A[(long long)row * n + j] * b[(long long)(j / np1) * L.Np + j % np1];
+// x = A^-1 b on the coarsest lattice with the symmetric A^-1 (3 numnp square,
+// row and column j = c numnp + n for node n, component c) read once on and
+// below its diagonal: the lower triangle in DT x DT tiles, tile (I, J), I >= J,
+// giving the rows I the products A_IJ b_J and, for I > J, the rows J the
+// products A_IJ^T b_I (half the bytes of a full product).  Each tile stores its
+// two partial vectors in the slots (I, J) and (J, I) of a dense_tiles^2 x DT
+// partial array; k_mg_dense_sum adds the dense_tiles slots of every row in a
+// fixed order (deterministic, no atomics).
+constexpr int DT = 64;
+__device__ __forceinline__ double dense_rhs(const Lattice &L, const double *__restrict__ b, int j,
+                                            int n) {
+  const int np1 = (int)L.numnp;
+  return j < n ? b[(long long)(j / np1) * L.Np + j % np1] : 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_mg_dense_symv(Lattice L, const double *__restrict__ A,
+                                                       const double *__restrict__ b,
+                                                       double *__restrict__ part) {
+  __shared__ double tile[DT][DT + 1];
+  __shared__ double sbi[DT], sbj[DT];
+  const int n = (int)(3 * L.numnp), nt = (n + DT - 1) / DT;
+  // tile t -> (I, J), I >= J, t = I (I + 1) / 2 + J
+  const int t = blockIdx.x;
+  int I = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((I + 1) * (I + 2) / 2 <= t) ++I;
+  while (I * (I + 1) / 2 > t) --I;
+  const int J = t - I * (I + 1) / 2;
+  if (threadIdx.x < DT) {
+    sbi[threadIdx.x] = dense_rhs(L, b, I * DT + threadIdx.x, n);
+    sbj[threadIdx.x] = dense_rhs(L, b, J * DT + threadIdx.x, n);
+  }
+  // the tile through LDS: rows of 64 contiguous doubles per wave
+  {
+    const int c = threadIdx.x & 63, w = threadIdx.x >> 6, col = J * DT + c;
+    double a[DT / 4];
 #pragma unroll
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    for (int k = 0; k < DT / 4; ++k) {
+      const int row = I * DT + w + 4 * k;
+      a[k] = (row < n && col < n) ? A[(long long)row * n + col] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < DT / 4; ++k) tile[w + 4 * k][c] = a[k];
+  }
   __syncthreads();
-  if (threadIdx.x == 0) x[(long long)(row / np1) * L.Np + row % np1] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  // four threads per row (row sums) and per column (column sums)
+  const int e = threadIdx.x >> 2, q = threadIdx.x & 3;
+  double u = 0.0, v = 0.0;
+#pragma unroll
+  for (int k = 0; k < DT / 4; ++k) {
+    const int o = q * (DT / 4) + k;
+    u += tile[e][o] * sbj[o];
+    v += tile[o][e] * sbi[o];
+  }
+  u += __shfl_xor(u, 1);
+  u += __shfl_xor(u, 2);
+  v += __shfl_xor(v, 1);
+  v += __shfl_xor(v, 2);
+  if (q == 0) {
+    part[((long long)I * nt + J) * DT + e] = u;
+    if (I > J) part[((long long)J * nt + I) * DT + e] = v;
+  }
+}
+
+// x rows of block I: the dense_tiles partial slots summed (a quarter of the
+// slots per thread, then the four quarters in order)
+__global__ __launch_bounds__(256) void k_mg_dense_sum(Lattice L, const double *__restrict__ part,
+                                                      double *__restrict__ x) {
+  __shared__ double sh[4][DT];
+  const int n = (int)(3 * L.numnp), nt = (n + DT - 1) / DT, np1 = (int)L.numnp;
+  const int I = blockIdx.x, r = threadIdx.x & 63, q = threadIdx.x >> 6;
+  double s = 0.0;
+  for (int J = q; J < nt; J += 4) s += part[((long long)I * nt + J) * DT + r];
+  sh[q][r] = s;
+  __syncthreads();
+  if (q == 0) {
+    const int i = I * DT + r;
+    if (i < n) x[(long long)(i / np1) * L.Np + i % np1] = (sh[0][r] + sh[1][r]) + (sh[2][r] + sh[3][r]);
+  }
 }
 
 template <int SYM>
@@ -523,7 +622,8 @@ void launch_mg_prolong_add_f32(const Lattice &Lf, const Lattice &Lc, const doubl
     throw Error(kInvalidArgument, "prolongation: lattices are not 2x apart");
   const dim3 g((unsigned)(((Lf.NX + PTX - 1) / PTX) * ((Lf.NY + PTY - 1) / PTY) *
                           ((Lf.NZ + PTZ - 1) / PTZ)));
-  hipLaunchKernelGGL((k_mg_prolong_add<true, float>), g, dim3(256), 0, s, Lf, Lc, xc, fmask_f, xf);
+  hipLaunchKernelGGL((k_mg_prolong_add<true, float>), g, dim3(256), 0, s, Lf, Lc, xc, nullptr, fmask_f,
+                     xf);
   HIPCHK(hipGetLastError());
 }
 
@@ -551,15 +651,16 @@ void launch_mg_restrict(const Lattice &Lf, const Lattice &Lc, const double *rf, 
 }
 
 void launch_mg_prolong_add(const Lattice &Lf, const Lattice &Lc, const double *xc,
-                           const uint8_t *fmask_f, double *xf, hipStream_t s, bool add) {
+                           const uint8_t *fmask_f, double *xf, hipStream_t s, bool add,
+                           const double *xc2) {
   if (Lf.NX != 2 * Lc.NX - 1 || Lf.NY != 2 * Lc.NY - 1 || Lf.NZ != 2 * Lc.NZ - 1)
     throw Error(kInvalidArgument, "prolongation: lattices are not 2x apart");
   const dim3 g((unsigned)(((Lf.NX + PTX - 1) / PTX) * ((Lf.NY + PTY - 1) / PTY) *
                           ((Lf.NZ + PTZ - 1) / PTZ)));
   if (add)
-    hipLaunchKernelGGL(k_mg_prolong_add<true>, g, dim3(256), 0, s, Lf, Lc, xc, fmask_f, xf);
+    hipLaunchKernelGGL(k_mg_prolong_add<true>, g, dim3(256), 0, s, Lf, Lc, xc, xc2, fmask_f, xf);
   else
-    hipLaunchKernelGGL(k_mg_prolong_add<false>, g, dim3(256), 0, s, Lf, Lc, xc, fmask_f, xf);
+    hipLaunchKernelGGL(k_mg_prolong_add<false>, g, dim3(256), 0, s, Lf, Lc, xc, xc2, fmask_f, xf);
   HIPCHK(hipGetLastError());
 }
 
@@ -593,10 +694,18 @@ void launch_mg_dense_symmetrize(int n, double *A, hipStream_t s) {
   HIPCHK(hipGetLastError());
 }
 
+long long mg_dense_part_size(const Lattice &L) {
+  const long long nt = (3 * L.numnp + DT - 1) / DT;
+  return nt * nt * DT;
+}
+
 void launch_mg_dense_solve(const Lattice &L, const double *ainv, const double *b, double *x,
-                           hipStream_t s) {
+                           double *part, hipStream_t s) {
   if (3 * L.numnp > MG_DENSE_MAX) throw Error(kInvalidArgument, "lattice too large for the dense solve");
-  hipLaunchKernelGGL(k_mg_dense_solve, dim3((unsigned)(3 * L.numnp)), dim3(256), 0, s, L, ainv, b, x);
+  const long long n = 3 * L.numnp, nt = (n + DT - 1) / DT;
+  hipLaunchKernelGGL(k_mg_dense_symv, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(256), 0, s, L, ainv,
+                     b, part);
+  hipLaunchKernelGGL(k_mg_dense_sum, dim3((unsigned)nt), dim3(256), 0, s, L, part, x);
   HIPCHK(hipGetLastError());
 }
 
@@ -630,20 +739,38 @@ unsigned launch_mg_jacobi_dot(long long n, double omega, const double *dinv, con
 void launch_mg_prolong_jacobi(const Lattice &Lf, const Lattice &Lc, const KeCoef &kc,
                               const double *young, const uint8_t *fmask, const double *xcoarse,
                               const double *x, const double *b, const double *dinv, double omega,
-                              double *out, hipStream_t s) {
+                              double *out, hipStream_t s, const double *xcoarse2) {
   if (Lf.NX != 2 * Lc.NX - 1 || Lf.NY != 2 * Lc.NY - 1 || Lf.NZ != 2 * Lc.NZ - 1)
     throw Error(kInvalidArgument, "prolongation: lattices are not 2x apart");
   if (out == x) throw Error(kInternal, "prolong_jacobi: out may not alias x");
   const dim3 g((Lf.NX + QTX - 1) / QTX, (Lf.NY + QTY - 1) / QTY, (Lf.NZ + QTZ - 1) / QTZ), t(256);
   if (kc.sym == 2)
     hipLaunchKernelGGL(k_mg_prolong_jacobi<2>, g, t, 0, s, Lf, Lc, kc.ke, kc.vals, young, fmask,
-                       xcoarse, x, b, dinv, omega, out);
+                       xcoarse, xcoarse2, x, b, dinv, omega, out);
   else if (kc.sym == 1)
     hipLaunchKernelGGL(k_mg_prolong_jacobi<1>, g, t, 0, s, Lf, Lc, kc.ke, kc.vals, young, fmask,
-                       xcoarse, x, b, dinv, omega, out);
+                       xcoarse, xcoarse2, x, b, dinv, omega, out);
   else
     hipLaunchKernelGGL(k_mg_prolong_jacobi<0>, g, t, 0, s, Lf, Lc, kc.ke, kc.vals, young, fmask,
-                       xcoarse, x, b, dinv, omega, out);
+                       xcoarse, xcoarse2, x, b, dinv, omega, out);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_mg_direct_wres(const Lattice &L, const KeCoef &kc, const double *young,
+                           const uint8_t *fmask, const double *xc, const double *b,
+                           const double *dinv, double omega, double *u, double *r, double *x0,
+                           hipStream_t s) {
+  if (xc == u || xc == r || xc == x0) throw Error(kInternal, "direct_wres: xc aliases an output");
+  const dim3 g(blocks_for(L.numnp, 256)), t(256);
+  if (kc.sym == 2)
+    hipLaunchKernelGGL(k_mg_direct_wres<2>, g, t, 0, s, L, kc.ke, kc.vals, young, fmask, xc, b, dinv,
+                       omega, u, r, x0);
+  else if (kc.sym == 1)
+    hipLaunchKernelGGL(k_mg_direct_wres<1>, g, t, 0, s, L, kc.ke, kc.vals, young, fmask, xc, b, dinv,
+                       omega, u, r, x0);
+  else
+    hipLaunchKernelGGL(k_mg_direct_wres<0>, g, t, 0, s, L, kc.ke, kc.vals, young, fmask, xc, b, dinv,
+                       omega, u, r, x0);
   HIPCHK(hipGetLastError());
 }
 

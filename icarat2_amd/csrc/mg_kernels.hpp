@@ -17,7 +17,8 @@ void launch_mg_restrict(const Lattice &Lf, const Lattice &Lc, const double *rf, 
 // xf += P xc (trilinear interpolation), masked; add = false: xf = P xc at the
 // free DOFs (constrained DOFs keep their value)
 void launch_mg_prolong_add(const Lattice &Lf, const Lattice &Lc, const double *xc,
-                           const uint8_t *fmask_f, double *xf, hipStream_t s, bool add = true);
+                           const uint8_t *fmask_f, double *xf, hipStream_t s, bool add = true,
+                           const double *xc2 = nullptr);
 // FP32 fine level of the multigrid V-cycle: the prolongation added into an
 // FP32 xf (accumulated in FP64), and the first sweep from zero x = omega dinv b
 // written in FP32
@@ -48,14 +49,22 @@ enum MgDirectMode { MG_JACOBI = 1, MG_RESIDUAL = 2 };
 void launch_mg_direct(int mode, const Lattice &L, const KeCoef &kc, const double *young,
                       const uint8_t *fmask, const double *x, const double *b, const double *dinv,
                       double omega, double *out, hipStream_t s);
+// The start of the W-cycle's second visit of a direct level in one pass:
+// u = xc, r = b - K xc (masked) and, with x0 != nullptr, x0 = omega dinv r
+// (xc may not alias u, r or x0)
+void launch_mg_direct_wres(const Lattice &L, const KeCoef &kc, const double *young,
+                           const uint8_t *fmask, const double *xc, const double *b,
+                           const double *dinv, double omega, double *u, double *r, double *x0,
+                           hipStream_t s);
 
 // out = xp + omega dinv (b - K xp) with xp = x + P xcoarse at the fine free
 // DOFs (x elsewhere): launch_mg_prolong_add followed by an MG_JACOBI sweep, in
-// one pass and without storing xp (out may not alias x)
+// one pass and without storing xp (out may not alias x).  With xcoarse2 the
+// coarse values are xcoarse + xcoarse2 (the W-cycle's two coarse visits).
 void launch_mg_prolong_jacobi(const Lattice &Lf, const Lattice &Lc, const KeCoef &kc,
                               const double *young, const uint8_t *fmask, const double *xcoarse,
                               const double *x, const double *b, const double *dinv, double omega,
-                              double *out, hipStream_t s);
+                              double *out, hipStream_t s, const double *xcoarse2 = nullptr);
 
 // Exact solve on a small lattice through the dense inverse of its K.  Dense
 // index d = c * numnp + n for lattice DOF c * Np + n; constrained DOFs carry
@@ -70,8 +79,10 @@ void launch_mg_dense_shift(int n, const Lattice &L, const uint8_t *fmask, double
                            hipStream_t s);
 // copies the lower triangle (column-major) over the upper one
 void launch_mg_dense_symmetrize(int n, double *A, hipStream_t s);
-// x = A b on the lattice (A symmetric, full)
+// x = A b on the lattice (A symmetric, full; read on and below its diagonal),
+// with a workspace part of mg_dense_part_size(L) doubles
 void launch_mg_dense_solve(const Lattice &L, const double *A, const double *b, double *x,
-                           hipStream_t s);
+                           double *part, hipStream_t s);
+long long mg_dense_part_size(const Lattice &L);
 
 }  // namespace icarat_hip

@@ -115,6 +115,7 @@ Multigrid::Multigrid(Context &fine, const icarat_mg_opts &o) : f_(fine), o_(o) {
     if (dense_) {
       ainv_.alloc(nd * nd);
       anext_.alloc(nd * nd);
+      dpart_.alloc(mg_dense_part_size(level(levels() - 1).lattice()));
       HIPCHK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
       HIPCHK(hipEventCreateWithFlags(&ev_side_, hipEventDisableTiming));
       HIPCHK(hipEventCreateWithFlags(&ev_fact_, hipEventDisableTiming));
@@ -294,11 +295,13 @@ void Multigrid::before_batch(hipStream_t s) {
     // from z = M(r) with the new one (beta = 0)
     enqueue_restart();
   }
-  // W-cycle phase over: from here on the V-cycle, restarted from z = M_V(r)
+  // first phase over (the W-cycle, the FP32 fine level): from here on the
+  // FP64 V-cycle, restarted from z = M_V(r)
   const PcgScalars *h = f_.h_pcg.p;
   if (wmode_ && !h->done && h->rr < o_.w_switch * o_.w_switch * tol_ * tol_ * h->bnorm2) {
     wmode_ = false;
     wcur_ = 0;
+    f32cur_ = false;
     enqueue_restart();
   }
   ++batches_;
@@ -386,7 +389,7 @@ double *Multigrid::cycle(int l, const double *b, bool emit, bool x0_ready, ZDot 
     return y0_.p;
   }
   if (l == levels() - 1 && dense_) {
-    if (emit) launch_mg_dense_solve(L, ainv_, b, x, s);
+    if (emit) launch_mg_dense_solve(L, ainv_, b, x, dpart_, s);
     return x;
   }
   double *cur = x, *oth = y;
@@ -426,24 +429,33 @@ double *Multigrid::cycle(int l, const double *b, bool emit, bool x0_ready, ZDot 
     }
   }
   const double *xc = cycle(l + 1, cc.d_b, emit);
+  // the second coarse visit's correction (W-cycle), added by the prolongation
+  const double *xc2 = nullptr;
   if (l + 1 <= wcur_ && l + 1 < levels() - 1) {
     // W-cycle on this coarse level: a second coarse cycle on the remaining
-    // coarse residual b_c - K_c xc, its correction added to xc (in d_u)
+    // coarse residual b_c - K_c xc; the prolongation takes xc (kept in d_u)
+    // plus its correction
     const Lattice &Lc = cc.lattice();
     const long long nc = cc.n3_;
+    double *x0 = starts_swept(l + 1) ? cc.d_x.p : nullptr;
     if (emit) {
-      launch_copy64(nc, xc, cc.d_u, s);
-      if (direct_[l + 1]) {
-        coarse_op(l + 1, MG_RESIDUAL, xc, cc.d_b, 0.0, cc.d_nv, s);
+      if (direct_[l + 1] && !galerkin_ && xc != cc.d_x.p) {
+        // the copy, the residual and the first sweep from zero in one pass
+        launch_mg_direct_wres(Lc, cc.kc_, cc.d_young, cc.d_free, xc, cc.d_b, cc.d_dinv,
+                              omega_[l + 1], cc.d_u, cc.d_nv, x0, s);
       } else {
-        launch_apply_k64(Lc, cc.kc_, cc.d_young, cc.d_free, xc, cc.d_nv, s);
-        launch_axpby(nc, 1.0, cc.d_b, -1.0, cc.d_nv, s);
+        launch_copy64(nc, xc, cc.d_u, s);
+        if (direct_[l + 1]) {
+          coarse_op(l + 1, MG_RESIDUAL, xc, cc.d_b, 0.0, cc.d_nv, s);
+        } else {
+          launch_apply_k64(Lc, cc.kc_, cc.d_young, cc.d_free, xc, cc.d_nv, s);
+          launch_axpby(nc, 1.0, cc.d_b, -1.0, cc.d_nv, s);
+        }
+        // the first sweep from zero (the restriction writes it for the first cycle)
+        if (x0) launch_mg_jacobi(nc, omega_[l + 1], cc.d_dinv, cc.d_nv, nullptr, x0, s);
       }
-      // the first sweep from zero (the restriction writes it for the first cycle)
-      if (starts_swept(l + 1)) launch_mg_jacobi(nc, omega_[l + 1], cc.d_dinv, cc.d_nv, nullptr, cc.d_x, s);
     }
-    const double *e = cycle(l + 1, cc.d_nv, emit);
-    if (emit) launch_axpby(nc, 1.0, e, 1.0, cc.d_u, s);
+    xc2 = cycle(l + 1, cc.d_nv, emit);
     xc = cc.d_u;
   }
   const int post = std::max(0, o_.nu_post);
@@ -454,12 +466,12 @@ double *Multigrid::cycle(int l, const double *b, bool emit, bool x0_ready, ZDot 
   if (fused) {
     if (emit)
       launch_mg_prolong_jacobi(L, cc.lattice(), c.kc_, c.d_young, c.d_free, xc, cur, b, c.d_dinv, w,
-                               oth, s);
+                               oth, s, xc2);
     std::swap(cur, oth);
     first = 1;
   } else if (emit) {
     // x = 0 without pre-smoothing: the correction is the iterate
-    launch_mg_prolong_add(L, cc.lattice(), xc, c.d_free, cur, s, pre > 0);
+    launch_mg_prolong_add(L, cc.lattice(), xc, c.d_free, cur, s, pre > 0, xc2);
   }
   for (int i = first; i < post; ++i) {
     if (i == post - 1 && l == 0 && !direct_[l]) {
@@ -526,14 +538,17 @@ void Multigrid::launch_batch(int par) {
     enqueue_batch(par & 1);
 }
 
-// Captures the batches of the V-cycle and, with w_levels > 0, of the W-cycle
-// (the cycle shape is fixed in a graph; the solve picks the graph of its phase).
+// Captures the batches of the FP64 V-cycle and, with a first phase (w_levels
+// > 0 or the FP32 fine level), of that phase's cycle (the cycle shape is fixed
+// in a graph; the solve picks the graph of its phase).
 void Multigrid::build_graph() {
   if (!f_.use_graphs_ || exec_[0][0]) return;
   hipStream_t s = f_.stream();
   const int wsave = wcur_;
-  for (int m = 0; m < (wlev_ > 0 ? 2 : 1); ++m) {
+  const bool fsave = f32cur_;
+  for (int m = 0; m < (two_phase() ? 2 : 1); ++m) {
     wcur_ = m ? wlev_ : 0;
+    f32cur_ = m == 1 && fp32_;
     for (int par = 0; par < 2; ++par) {
       HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       try {
@@ -543,6 +558,7 @@ void Multigrid::build_graph() {
         (void)hipStreamEndCapture(s, &g);
         if (g) (void)hipGraphDestroy(g);
         wcur_ = wsave;
+        f32cur_ = fsave;
         throw;
       }
       HIPCHK(hipStreamEndCapture(s, &graph_[m][par]));
@@ -550,6 +566,7 @@ void Multigrid::build_graph() {
     }
   }
   wcur_ = wsave;
+  f32cur_ = fsave;
 }
 
 // one V-cycle (the learned preconditioner's base), whatever phase a solve is in
@@ -589,8 +606,9 @@ void Multigrid::start(double tol, int maxit, hipStream_t s_ext, bool warm) {
     HIPCHK(hipStreamWaitEvent(s, ev_[0], 0));
   }
   batches_ = 0;
-  wmode_ = wlev_ > 0;
+  wmode_ = two_phase();
   wcur_ = wmode_ ? wlev_ : 0;
+  f32cur_ = wmode_ && fp32_;
   tol_ = tol;
   best_ = INFINITY;
   since_best_ = 0;
@@ -629,11 +647,17 @@ void Multigrid::start(double tol, int maxit, hipStream_t s_ext, bool warm) {
 
 bool Multigrid::after_verify() {
   PcgScalars *h = f_.h_pcg.p;
-  if (h->done) return true;  // the verified residual passes
+  if (h->done) return true;  // the verified residual passes the recursive target
   hipStream_t s = f_.stream();
-  ++verifications_;
   const double bnorm = std::sqrt(h->bnorm2);
   const double tr = bnorm > 0.0 ? std::sqrt(h->rr) / bnorm : 0.0;
+  if (tr < tol_) {
+    // passes the solve's tolerance (the recursive target is tighter)
+    h->done = 1;
+    h->status = 0;
+    return true;
+  }
+  ++verifications_;
   if (tr < best_) {
     best_ = tr;
     since_best_ = 0;
@@ -642,7 +666,21 @@ bool Multigrid::after_verify() {
   } else {
     ++since_best_;
   }
-  if (since_best_ < kStallVerifications && verifications_ <= kMaxVerifications) return false;
+  if (since_best_ < kStallVerifications && verifications_ <= kMaxVerifications) {
+    // The next phase stops on a tighter recursive target t.  A phase that
+    // stopped at the target t0 verified at tr: the recursive and the true
+    // residual differ by about g = sqrt(tr^2 - t0^2) (rounding, in
+    // quadrature), so the next phase aims at sqrt(t^2 + g^2) = 0.9 tol, at
+    // least 0.3 tol.
+    const double t0 = h->tol;
+    const double g2 = std::max(0.0, tr * tr - t0 * t0);
+    const double t = std::min(t0, std::sqrt(std::max(0.81 * tol_ * tol_ - g2, 0.09 * tol_ * tol_)));
+    if (t < t0) {
+      h->tol = t;
+      HIPCHK(hipMemcpyAsync(f_.d_pcg.p, h, sizeof(PcgScalars), hipMemcpyHostToDevice, s));
+    }
+    return false;
+  }
   // stalled at the floor: the best verified iterate, reported as not converged
   HIPCHK(hipMemcpyAsync(f_.d_x.p, xbest_.p, f_.d_x.bytes(), hipMemcpyDeviceToDevice, s));
   if (xlo_.p) HIPCHK(hipMemsetAsync(xlo_.p, 0, xlo_.bytes(), s));
@@ -666,6 +704,7 @@ bool Multigrid::recover_w(hipStream_t s_ext) {
   }
   wmode_ = false;
   wcur_ = 0;
+  f32cur_ = false;
   ++w_fallbacks_;
   h->done = 0;
   h->status = 0;

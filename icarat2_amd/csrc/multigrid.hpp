@@ -67,11 +67,13 @@ class Multigrid {
   void build_graph();  // the two parity graphs (no-op if built or graphs are off)
   void verify(hipStream_t s);
   // After a verification's scalars reached the host copy: true when the solve
-  // ends -- the verified residual passed, or it stalled.  Near the FP64 floor
-  // of a solution (|u| far above |f|, where rounding u itself costs a fraction
-  // of the tolerance) the verified residual stops improving and every restart
-  // adds rounding to x; the best verified iterate is kept, and after
-  // kStallVerifications verifications without a new best (or
+  // ends -- the verified residual passed the tolerance, or it stalled.  Near
+  // the FP64 floor of a solution (|u| far above |f|, where rounding u itself
+  // costs a fraction of the tolerance) the recursive and the verified
+  // residual differ by a fixed amount: after a failed verification the next
+  // phase stops on a tighter recursive target (the verified residual is still
+  // judged against the tolerance).  The best verified iterate is kept, and
+  // after kStallVerifications verifications without a new best (or
   // kMaxVerifications in all) it is put back in x and the host copy marked
   // capped (status 1, reported as not converged).
   bool after_verify();
@@ -90,7 +92,9 @@ class Multigrid {
   // level uses the z-marching K.u; needs nu_pre = nu_post = 1.  Captured
   // graphs are rebuilt when the mode changes.
   void set_fine_fp32(bool on);
-  bool fine_fp32() const { return fp32_ && fine_fp32_capable(); }
+  // the FP32 fine level runs in the first phase of a solve only
+  bool fine_fp32() const { return fp32_ && f32cur_ && fine_fp32_capable(); }
+  bool fine_fp32_on() const { return fp32_ && fine_fp32_capable(); }
   // whether the coarsest level is solved exactly (dense inverse)
   bool dense() const { return dense_; }
   // One V-cycle z = V(r) of the fine lattice vector r, enqueued on the fine
@@ -149,7 +153,12 @@ class Multigrid {
   int wcur_ = 0;
   int wlev_ = 0;        // the W-cycle depth of this hierarchy (o_.w_levels or automatic)
   long long w_fallbacks_ = 0;  // W-cycle phases that ended in a breakdown
-  bool wmode_ = false;  // the running solve is in its W-cycle phase
+  // The running solve is in its first phase: the W-cycle (wlev_ > 0) and the
+  // FP32 fine level (fp32_), until the recursive residual falls below
+  // w_switch x tol; then the FP64 V-cycle
+  bool wmode_ = false;
+  bool f32cur_ = false;  // the cycle being run or captured has the FP32 fine level
+  bool two_phase() const { return wlev_ > 0 || (fp32_ && fine_fp32_capable()); }
   double tol_ = 0.0;    // the running solve's tolerance (the switch point)
   // Galerkin coarse operators (coarse_avg 5): explicit element matrices per
   // coarse level (kmat_[l], 576 per element), the per-element smoothing bounds
@@ -162,6 +171,7 @@ class Multigrid {
                  hipStream_t s);
   bool dense_ = false;
   DevBuf<double> ainv_;  // dense inverse of the coarsest K
+  DevBuf<double> dpart_;  // partial products of the dense solve's tiles
   // Factorization status per stream (index 0: the fine context's stream, 1:
   // the side stream), each with a pinned host copy, and one rocBLAS handle
   // (with its workspace) per stream, so the two factorizations share nothing.

@@ -485,10 +485,12 @@ typedef struct icarat_mg_opts {
                               at the top of the hierarchy; 0: V-cycle; < 0: automatic by the
                               fine mesh: 0 below 500,000 elements, 1 up to 2,000,000, every
                               coarse level above) */
-  double w_switch;         /* with w_levels > 0 the solve switches to the V-cycle (and
-                              restarts with beta = 0) once its recursive relative residual is
-                              below w_switch x tol: the W-cycle's coarse corrections lose
-                              accuracy near the FP64 floor of high-contrast fields */
+  double w_switch;         /* a solve's first phase -- the W-cycle (w_levels > 0) and the
+                              FP32 fine level (precision FP32) -- ends once its recursive
+                              relative residual is below w_switch x tol; it goes on with the
+                              FP64 V-cycle (restarted with beta = 0): the W-cycle's coarse
+                              corrections and the FP32 sweeps lose accuracy near the FP64
+                              floor of high-contrast fields */
   int32_t compensated;     /* 1: the PCG iterate accumulates its updates as a compensated sum
                               (x plus a low-part vector, x its rounding), so updates below
                               half an ulp of x -- the large displacements of near-void
@@ -506,7 +508,8 @@ typedef struct icarat_mg_status {
   int32_t levels;          /* levels of the hierarchy */
   int32_t dense;           /* 1 if the coarsest level is solved exactly by its dense inverse */
   int32_t direct_fine;     /* 1 if the fine level smooths with the one-thread-per-node kernel */
-  int32_t fine_fp32;       /* 1 if the last solve ran the FP32 fine level (precision FP32) */
+  int32_t fine_fp32;       /* 1 if the last solve ran the FP32 fine level (precision FP32; in
+                              its first phase) */
   int64_t factorizations;  /* Cholesky factorizations of the coarsest matrix so far */
   int64_t shifted;         /* of which broke down and were redone with the diagonal of the
                               free DOFs raised by a relative 1e-10 */
