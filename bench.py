@@ -10,12 +10,12 @@ equilibrium solve to the verified FP64 relative residual -> compliance and
 sensitivities -> sensitivity filter -> OC update.  Inputs are resident in HBM
 when the timed region starts.
 
-Tolerance: 1e-8 (the metric's) where FP64 reaches it; 3e-8 on the
+Tolerance: 1e-8 (the metric's) where FP64 reaches it; 2e-8 on the
 256x128x128 field, where 47% of the filtered densities lie below 0.01 and
-rounding the solution itself to FP64 costs about 1e-8 of relative residual,
-so the verified residual of an FP64 iterate stalls between 1.5e-8 and 4e-8
-and 3e-8 is the tightest tolerance the FP64 solve reaches reliably (the
-chosen value is in config.workload).
+the displacements of near-void regions reach 3.9e7: rounding the solution
+itself to FP64 leaves a relative residual of 1.4-1.9e-8 on the first design,
+so no FP64 iterate verifies at 1e-8, and 2e-8 is the tightest tolerance every
+solve of the loop reaches (the chosen value is in config.workload).
 
 The default method solves with PCG preconditioned by a geometric multigrid
 V-cycle (FP64, same verified stop).  Beside the value, on the same workload
@@ -46,7 +46,7 @@ CONFIGS = {
     "c5": dict(dims=None, rmin=None, field=None),
 }
 # tolerance of each workload: the metric's 1e-8 where FP64 reaches it
-CONFIG_TOL = {"c1": 1e-8, "c2": 1e-8, "c3": 3e-8, "c4": 1e-8, "c5": 1e-8}
+CONFIG_TOL = {"c1": 1e-8, "c2": 1e-8, "c3": 2e-8, "c4": 1e-8, "c5": 1e-8}
 METHODS = {
     # name: (precision, preconditioner)
     "fp64": (0, 0),

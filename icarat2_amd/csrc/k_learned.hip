@@ -361,23 +361,27 @@ __global__ __launch_bounds__(NT) void k_fpcg_update_alpha(
     rr += ri * ri;
     if (zi) *zi = (TZ)(omega * di * ri);
   };
-  // pairs of elements with 16-byte accesses (the vectors are 16-byte aligned)
+  // pairs of elements with 16-byte accesses (the vectors are 16-byte aligned);
+  // the streams no later kernel of the iteration reads (x and its low part,
+  // p, q, dinv) bypass the caches' retention (non-temporal), r and z0 do not
+  typedef double d2 __attribute__((ext_vector_type(2)));
   const long long half = n >> 1;
   const long long stride = (long long)gridDim.x * NT;
   for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < half; i += stride) {
-    double2 xv = reinterpret_cast<const double2 *>(x)[i];
-    double2 rv = reinterpret_cast<const double2 *>(r)[i];
-    const double2 pv = reinterpret_cast<const double2 *>(p)[i];
-    const double2 qv = reinterpret_cast<const double2 *>(q)[i];
-    double2 lv = make_double2(0.0, 0.0), dv = make_double2(0.0, 0.0);
-    if constexpr (COMP) lv = reinterpret_cast<const double2 *>(xlo)[i];
-    if (z0) dv = reinterpret_cast<const double2 *>(dinv)[i];
+    d2 xv = __builtin_nontemporal_load(reinterpret_cast<const d2 *>(x) + i);
+    d2 rv = reinterpret_cast<const d2 *>(r)[i];
+    const d2 pv = __builtin_nontemporal_load(reinterpret_cast<const d2 *>(p) + i);
+    const d2 qv = __builtin_nontemporal_load(reinterpret_cast<const d2 *>(q) + i);
+    d2 lv = {0.0, 0.0}, dv = {0.0, 0.0};
+    if constexpr (COMP) lv = __builtin_nontemporal_load(reinterpret_cast<const d2 *>(xlo) + i);
+    if (z0) dv = __builtin_nontemporal_load(reinterpret_cast<const d2 *>(dinv) + i);
     TZ zv[2];
-    elem(xv.x, lv.x, pv.x, rv.x, qv.x, dv.x, z0 ? &zv[0] : nullptr);
-    elem(xv.y, lv.y, pv.y, rv.y, qv.y, dv.y, z0 ? &zv[1] : nullptr);
-    reinterpret_cast<double2 *>(x)[i] = xv;
-    reinterpret_cast<double2 *>(r)[i] = rv;
-    if constexpr (COMP) reinterpret_cast<double2 *>(xlo)[i] = lv;
+    double x0 = xv.x, x1 = xv.y, l0 = lv.x, l1 = lv.y, r0 = rv.x, r1 = rv.y;
+    elem(x0, l0, pv.x, r0, qv.x, dv.x, z0 ? &zv[0] : nullptr);
+    elem(x1, l1, pv.y, r1, qv.y, dv.y, z0 ? &zv[1] : nullptr);
+    __builtin_nontemporal_store(d2{x0, x1}, reinterpret_cast<d2 *>(x) + i);
+    reinterpret_cast<d2 *>(r)[i] = d2{r0, r1};
+    if constexpr (COMP) __builtin_nontemporal_store(d2{l0, l1}, reinterpret_cast<d2 *>(xlo) + i);
     if (z0) {
       z0[2 * i] = zv[0];
       z0[2 * i + 1] = zv[1];

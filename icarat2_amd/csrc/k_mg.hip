@@ -541,9 +541,9 @@ __global__ __launch_bounds__(256) void k_mg_dense_symv(Lattice L, const double *
                                                        double *__restrict__ part) {
   __shared__ double tile[DT][DT + 1];
   __shared__ double sbi[DT], sbj[DT];
-  const int n = (int)(3 * L.numnp), nt = (n + DT - 1) / DT;
+  const int n = (int)(3 * L.numnp), nt = (n + DT - 1) / DT, ntiles = nt * (nt + 1) / 2;
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
   // tile t -> (I, J), I >= J, t = I (I + 1) / 2 + J
-  const int t = blockIdx.x;
   int I = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
   while ((I + 1) * (I + 2) / 2 <= t) ++I;
   while (I * (I + 1) / 2 > t) --I;
@@ -581,6 +581,8 @@ __global__ __launch_bounds__(256) void k_mg_dense_symv(Lattice L, const double *
   if (q == 0) {
     part[((long long)I * nt + J) * DT + e] = u;
     if (I > J) part[((long long)J * nt + I) * DT + e] = v;
+  }
+  __syncthreads();  // the tile and the b slices are refilled for the next tile
   }
 }
 
@@ -703,8 +705,10 @@ void launch_mg_dense_solve(const Lattice &L, const double *ainv, const double *b
                            double *part, hipStream_t s) {
   if (3 * L.numnp > MG_DENSE_MAX) throw Error(kInvalidArgument, "lattice too large for the dense solve");
   const long long n = 3 * L.numnp, nt = (n + DT - 1) / DT;
-  hipLaunchKernelGGL(k_mg_dense_symv, dim3((unsigned)(nt * (nt + 1) / 2)), dim3(256), 0, s, L, ainv,
-                     b, part);
+  // one round of resident blocks (4 per CU by LDS), tiles in a grid-stride loop
+  const long long ntiles = nt * (nt + 1) / 2;
+  hipLaunchKernelGGL(k_mg_dense_symv, dim3((unsigned)std::min<long long>(ntiles, 1024)), dim3(256), 0,
+                     s, L, ainv, b, part);
   hipLaunchKernelGGL(k_mg_dense_sum, dim3((unsigned)nt), dim3(256), 0, s, L, part, x);
   HIPCHK(hipGetLastError());
 }

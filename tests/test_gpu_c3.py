@@ -5,12 +5,11 @@ the FP64 multigrid solve and the solve with the FP32 fine level judged by the
 oracle's own recomputation of the true residual (analysis/linear.hpp:93-103),
 and the FP64-oracle error of u, compliance and sensitivities per precision.
 
-The tolerance is 3e-8: this field leaves 47% of the filtered densities below
-0.01, |u| reaches 3.9e7 with |f| = 1, and rounding u to FP64 alone costs about
-1e-8 of relative residual (a random half-ulp perturbation adds 1.8e-8 in
-quadrature), so the verified residual of an FP64 iterate stalls between 1.5e-8
-and 4e-8 depending on where the iteration stands; 3e-8 is the tightest
-tolerance the FP64 multigrid solve reaches reliably.
+The tolerance is 2e-8: this field leaves 47% of the filtered densities below
+0.01, |u| reaches 3.9e7 with |f| = 1, and rounding u to FP64 alone leaves a
+relative residual of 1.4-1.9e-8 on this design (a random half-ulp perturbation
+adds 1.8e-8 in quadrature), so no FP64 iterate verifies at 1e-8; 2e-8 is the
+tightest tolerance the multigrid solve reaches on every design of the loop.
 """
 import numpy as np
 import pytest
@@ -18,7 +17,7 @@ import pytest
 pytestmark = [pytest.mark.gpu, pytest.mark.slow]
 
 DIMS = (256, 128, 128)
-TOL = 3e-8
+TOL = 2e-8
 
 
 @pytest.fixture(scope="module")
@@ -89,9 +88,9 @@ def test_c3_compensated_update_reaches_tolerance(gpu, c3):
     solved exactly at this size): with x += alpha p in plain FP64 the updates
     of the near-void regions' large displacements (|u| up to 3.9e7) fall below
     half an ulp near the tolerance and are lost while the recursive residual
-    counts them, so the verified residual stalled near 8e-8 after 1,457
-    iterations; accumulated as a compensated sum they are kept and the solve
-    passes 3e-8."""
+    counts them, so the verified residual stalled near 8e-8 (1,457 iterations
+    at a 3e-8 tolerance); accumulated as a compensated sum they are kept and
+    the solve passes."""
     c, o = c3
     c.mg_setup()
     assert c.mg_info()[-1] == (16, 8, 8)
