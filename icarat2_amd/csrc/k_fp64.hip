@@ -207,8 +207,8 @@ constexpr int ZLX = ZX + 1, ZLY = ZY + 1, ZLN = ZLX * ZLY;
 constexpr int ZPER = (ZPN + NT - 1) / NT;  // plane nodes per thread
 constexpr int ZEPER = (ZLN + NT - 1) / NT; // layer elements per thread
 // output of the z-marching K.u: the node-value type, FP64 for the fused sweep
-template <bool JAC, class T>
-using OutT = typename std::conditional<JAC, double, T>::type;
+template <int EPI, class T>
+using OutT = typename std::conditional<EPI != 0, double, T>::type;
 
 // T is the type of the node values, LDS planes and contraction (double; float
 // for the multigrid smoother's FP32 fine level, PCG = false only).
@@ -338,11 +338,14 @@ __device__ __forceinline__ void layer_contract(const T *__restrict__ c_KE0, cons
   }
 }
 
-// JAC (with PCG = false): the multigrid smoother's last fine sweep fused into
-// the epilogue.  The node's own value x is the centre of the bottom plane, so
-// y receives x' = x + omega dinv (b - K x) (out of place: neighbouring columns
-// and the next z-chunk still read x) and the block partials of x'.b and x'.q.
-// The epilogue computes in FP64 and y, b, dinv, q are FP64 whatever T is.
+// Epilogues of the z-marching K.u (EPI, with PCG = false): 0 y = K x.  1 the
+// multigrid smoother's last fine sweep: the node's own value x is the centre
+// of the bottom plane, so y receives x' = x + omega dinv (b - K x) (out of
+// place: neighbouring columns and the next z-chunk still read x) and the block
+// partials of x'.b and x'.q.  2 the same sweep without partials (coarser
+// z-marching levels), or the start of a W-cycle's second visit (jac_epilogue).
+// The epilogues compute in FP64 and y, b, dinv, q are FP64 whatever T is.
+
 // The work split of a z-marching launch.  Tiles of ZX x ZY node columns start
 // at node column x = xlo: 1 when column x = 0 holds no free DOF (a clamped
 // face; its K.u is 0 and the first tile column writes it from its halo), else
@@ -369,10 +372,34 @@ __device__ __forceinline__ unsigned xcd_slot(unsigned t, unsigned nt) {
   return ((t - big) % q) * 8 + rem + (t - big) / q;
 }
 
+// The epilogues of EPI 2 at an output node n with centre value xc and masked
+// K x = kx: the smoother sweep y = xc + omega dinv (b - kx) (no partials); or,
+// with je.u, u = xc, y = b - kx and x0 = omega dinv y.  In FP64.  (EPI 1, the
+// fine level's last sweep with the y.b and y.q partials, is written out at
+// its call sites.)
+template <class T, class TO>
+__device__ __forceinline__ void jac_epilogue(const JacEpilogue &je, long long Np, long long n,
+                                             const T (&xc)[3], const T (&kx)[3],
+                                             const double *__restrict__ dinv, TO *__restrict__ y) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const long long ic = c * Np + n;
+    const double bc = je.b[ic];
+    if (je.u) {
+      const double r = bc - (double)kx[c];
+      je.u[ic] = (double)xc[c];
+      y[ic] = r;
+      if (je.x0) je.x0[ic] = je.omega * dinv[ic] * r;
+    } else {
+      y[ic] = (double)xc[c] + je.omega * dinv[ic] * (bc - (double)kx[c]);
+    }
+  }
+}
+
 // One segment of a block's z-marching K.u: node planes [z0, z1) of tile t
 // (element layers z0-1 .. z1-1: the layer below z0 completes plane z0).  The
-// block partials accumulate into pq (PCG) and zr, zq (JAC).
-template <bool PCG, int SYM, bool JAC, class T, class TO>
+// block partials accumulate into pq (PCG) and zr, zq (EPI 1).
+template <bool PCG, int SYM, int EPI, class T, class TO>
 __device__ __forceinline__ void stencil64z_seg(
     const Lattice &L, const ZSplit &Z, int t, int z0, int z1, double beta,
     const T *__restrict__ c_KE0, const KeValsT<T> &kv, const T *__restrict__ young,
@@ -423,7 +450,7 @@ __device__ __forceinline__ void stencil64z_seg(
       T out[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) out[c] = ((m >> c) & 1u) ? acc[c] + bot[c] : T(0);
-      if constexpr (JAC) {
+      if constexpr (EPI == 1) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const long long ic = c * L.Np + n;
@@ -433,6 +460,8 @@ __device__ __forceinline__ void stencil64z_seg(
           zr += xn * bc;
           zq += xn * je.q[ic];
         }
+      } else if constexpr (EPI == 2) {
+        jac_epilogue(je, L.Np, n, Ulo[4], out, dinv, y);
       } else {
 #pragma unroll
         for (int c = 0; c < 3; ++c) y[c * L.Np + n] = (TO)out[c];
@@ -442,7 +471,7 @@ __device__ __forceinline__ void stencil64z_seg(
     if (k >= 1 && edge) {
       // node (0, gy, zb): every component constrained, K.u masked to 0
       const long long n = (long long)L.NX * (gy + (long long)L.NY * zb);
-      if constexpr (JAC) {
+      if constexpr (EPI == 1) {
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const long long ic = c * L.Np + n;
@@ -452,6 +481,9 @@ __device__ __forceinline__ void stencil64z_seg(
           zr += xn * bc;
           zq += xn * je.q[ic];
         }
+      } else if constexpr (EPI == 2) {
+        const T zero[3] = {T(0), T(0), T(0)};
+        jac_epilogue(je, L.Np, n, Ulo[3], zero, dinv, y);
       } else {
 #pragma unroll
         for (int c = 0; c < 3; ++c) y[c * L.Np + n] = (TO)T(0);
@@ -473,7 +505,7 @@ __device__ __forceinline__ void stencil64z_seg(
 // A block's share of a z-marching K.u: work range bw of the split Z (its
 // consecutive tile-planes, one segment per tile they touch); the block's
 // partials go to slot pslot of Gp.
-template <bool PCG, int SYM, bool FUSE, bool JAC, class T, class TO>
+template <bool PCG, int SYM, bool FUSE, int EPI, class T, class TO>
 __device__ __forceinline__ void stencil64z_work(
     const Lattice &L, const ZSplit &Z, unsigned bw, const T *__restrict__ c_KE0,
     const KeValsT<T> &kv, const T *__restrict__ young, const uint8_t *__restrict__ fmask,
@@ -490,7 +522,7 @@ __device__ __forceinline__ void stencil64z_work(
     const int t = (int)(u / L.NZ);
     const int z0 = (int)(u - (long long)t * L.NZ);
     const int z1 = (int)std::min<long long>(L.NZ, u1 - (long long)t * L.NZ);
-    stencil64z_seg<PCG, SYM, JAC, T, TO>(L, Z, t, z0, z1, beta, c_KE0, kv, young, fmask, src, dinv,
+    stencil64z_seg<PCG, SYM, EPI, T, TO>(L, Z, t, z0, z1, beta, c_KE0, kv, young, fmask, src, dinv,
                                          p_old, p_new, y, je, sp, se, pq, zr, zq);
     u = (long long)t * L.NZ + z1;
   }
@@ -498,13 +530,13 @@ __device__ __forceinline__ void stencil64z_work(
     double v[1] = {pq};
     store_partials_at<NT, 1>(v, partials, sh, Gp, pslot);
   }
-  if constexpr (JAC) {
+  if constexpr (EPI == 1) {
     double v[2] = {zr, zq};
     store_partials_at<NT, 2>(v, partials, sh, Gp, pslot);
   }
 }
 
-template <bool PCG, int SYM, bool FUSE, bool JAC, class T = double, class TO = OutT<JAC, T>>
+template <bool PCG, int SYM, bool FUSE, int EPI, class T = double, class TO = OutT<EPI, T>>
 __global__ __launch_bounds__(NT) void k_stencil64z(
     Lattice L, ZSplit Z, const T *__restrict__ c_KE0, KeValsT<T> kv, const T *__restrict__ young,
     const uint8_t *__restrict__ fmask, const T *__restrict__ src,
@@ -515,7 +547,7 @@ __global__ __launch_bounds__(NT) void k_stencil64z(
   __shared__ T se[2][ZLN];
   __shared__ double sh[8];
   const unsigned b = blockIdx.x;
-  stencil64z_work<PCG, SYM, FUSE, JAC, T, TO>(L, Z, xcd_order(b, Z.nb), c_KE0, kv, young, fmask,
+  stencil64z_work<PCG, SYM, FUSE, EPI, T, TO>(L, Z, xcd_order(b, Z.nb), c_KE0, kv, young, fmask,
                                               src, dinv, p_old, p_new, y, S, partials, pu, Gu, j,
                                               je, Z.nb, b, first_thread(), sp, se, sh);
 }
@@ -915,7 +947,10 @@ int stencil_nchunks(const Lattice &L) {
 
 // The split of a z-marching launch: one round of resident workgroups, each
 // with at least 2 tile-planes; the autotuner's nchunks caps the blocks at
-// tiles x nchunks.
+// tiles x nchunks.  The block count is a multiple of the tile rows where it
+// can be, so every tile row is cut at the same planes: tiles of neighbouring
+// rows (their 34-node y-halo planes are most of what tiles share) march in
+// step through an XCD's L2.
 static ZSplit make_zsplit(const Lattice &L, const KeCoef &ke) {
   ZSplit z;
   z.xlo = ke.xlo == 1 && L.NX > 1 ? 1 : 0;
@@ -925,6 +960,7 @@ static ZSplit make_zsplit(const Lattice &L, const KeCoef &ke) {
   z.W = tiles * L.NZ;
   long long nb = std::min<long long>(resident_workgroups(), z.W / 2);
   if (ke.nchunks > 0) nb = std::min<long long>(nb, tiles * ke.nchunks);
+  if (nb >= z.nby) nb -= nb % z.nby;
   z.nb = (int)std::max<long long>(1, nb);
   return z;
 }
@@ -960,7 +996,7 @@ KeCoef make_ke_coef(const double *ke, const double *ke_dev) {
   return k;
 }
 
-template <bool PCG, int SYM, bool FUSE, bool JAC = false>
+template <bool PCG, int SYM, bool FUSE, int EPI = 0>
 static void launch_stencil64_t(const Lattice &L, const KeCoef &ke, const double *young,
                                const uint8_t *freemask, const double *src, const double *dinv,
                                const double *p_old, double *p_new, double *y, PcgScalars *S,
@@ -968,12 +1004,12 @@ static void launch_stencil64_t(const Lattice &L, const KeCoef &ke, const double 
                                hipStream_t s, unsigned *nblocks, JacEpilogue je = JacEpilogue{}) {
   if (ke.kernel == 0) {
     const ZSplit z = make_zsplit(L, ke);
-    hipLaunchKernelGGL((k_stencil64z<PCG, SYM, FUSE, JAC>), dim3((unsigned)z.nb), dim3(NT), 0, s, L,
+    hipLaunchKernelGGL((k_stencil64z<PCG, SYM, FUSE, EPI>), dim3((unsigned)z.nb), dim3(NT), 0, s, L,
                        z, ke.ke, ke.vals, young, freemask, src, dinv, p_old, p_new, y, S, partials,
                        pu, Gu, j, je);
     *nblocks = (unsigned)z.nb;
   } else {
-    if (JAC) throw Error(kInternal, "the fused smoother needs the z-marching kernel");
+    if (EPI) throw Error(kInternal, "the fused smoother needs the z-marching kernel");
     const dim3 g = stencil_grid(L);
     hipLaunchKernelGGL((k_stencil64<PCG, SYM, FUSE>), g, dim3(NT), 0, s, L, ke.ke, ke.vals, young,
                        freemask, src, dinv, p_old, p_new, y, S, partials, pu, Gu, j);
@@ -1013,28 +1049,62 @@ unsigned launch_apply_k64_jacobi(const Lattice &L, const KeCoef &ke, const doubl
                                  const double *b, const double *dinv, double omega,
                                  const double *q, double *partials, hipStream_t s) {
   if (ke.kernel != 0) throw Error(kInternal, "the fused smoother needs the z-marching kernel");
-  const JacEpilogue je{b, q, omega};
+  if ((q == nullptr) != (partials == nullptr))
+    throw Error(kInternal, "apply_k64_jacobi: q and partials go together");
+  JacEpilogue je;
+  je.b = b;
+  je.q = q;
+  je.omega = omega;
+  unsigned nb = 0;
+  // with the partials (the fine level's last sweep) or without (EPI 2)
+#define JAC_LAUNCH(SYM_)                                                                          \
+  if (partials)                                                                                   \
+    launch_stencil64_t<false, SYM_, false, 1>(L, ke, young, freemask, x, dinv, nullptr, nullptr,  \
+                                              xout, nullptr, partials, nullptr, 0, 0, s, &nb, je); \
+  else                                                                                            \
+    launch_stencil64_t<false, SYM_, false, 2>(L, ke, young, freemask, x, dinv, nullptr, nullptr,  \
+                                              xout, nullptr, nullptr, nullptr, 0, 0, s, &nb, je);
+  switch (ke.sym) {
+    case 0: JAC_LAUNCH(0); break;
+    case 1: JAC_LAUNCH(1); break;
+    case 2: JAC_LAUNCH(2); break;
+    default: throw Error(kInvalidArgument, "unknown KE0 symmetry class");
+  }
+#undef JAC_LAUNCH
+  return nb;
+}
+
+void launch_apply_k64_wres(const Lattice &L, const KeCoef &ke, const double *young,
+                           const uint8_t *freemask, const double *xc, const double *b,
+                           const double *dinv, double omega, double *u, double *r, double *x0,
+                           hipStream_t s) {
+  if (ke.kernel != 0) throw Error(kInternal, "the fused W-cycle residual needs the z-marching K.u");
+  if (!u || xc == u || xc == r || xc == x0) throw Error(kInternal, "apply_k64_wres: bad buffers");
+  JacEpilogue je;
+  je.b = b;
+  je.omega = omega;
+  je.u = u;
+  je.x0 = x0;
   unsigned nb = 0;
   switch (ke.sym) {
     case 0:
-      launch_stencil64_t<false, 0, false, true>(L, ke, young, freemask, x, dinv, nullptr, nullptr,
-                                                xout, nullptr, partials, nullptr, 0, 0, s, &nb, je);
+      launch_stencil64_t<false, 0, false, 2>(L, ke, young, freemask, xc, dinv, nullptr, nullptr, r,
+                                                nullptr, nullptr, nullptr, 0, 0, s, &nb, je);
       break;
     case 1:
-      launch_stencil64_t<false, 1, false, true>(L, ke, young, freemask, x, dinv, nullptr, nullptr,
-                                                xout, nullptr, partials, nullptr, 0, 0, s, &nb, je);
+      launch_stencil64_t<false, 1, false, 2>(L, ke, young, freemask, xc, dinv, nullptr, nullptr, r,
+                                                nullptr, nullptr, nullptr, 0, 0, s, &nb, je);
       break;
     case 2:
-      launch_stencil64_t<false, 2, false, true>(L, ke, young, freemask, x, dinv, nullptr, nullptr,
-                                                xout, nullptr, partials, nullptr, 0, 0, s, &nb, je);
+      launch_stencil64_t<false, 2, false, 2>(L, ke, young, freemask, xc, dinv, nullptr, nullptr, r,
+                                                nullptr, nullptr, nullptr, 0, 0, s, &nb, je);
       break;
     default:
       throw Error(kInvalidArgument, "unknown KE0 symmetry class");
   }
-  return nb;
 }
 
-template <bool JAC>
+template <int EPI>
 static unsigned launch_k32(const Lattice &L, const KeCoef &ke, const float *young32,
                            const uint8_t *freemask, const float *x, double *y,
                            const double *dinv, double *partials, JacEpilogue je, hipStream_t s) {
@@ -1046,9 +1116,9 @@ static unsigned launch_k32(const Lattice &L, const KeCoef &ke, const float *youn
                        dinv, nullptr, nullptr, y, nullptr, partials, nullptr, 0u, 0, je);
   };
   switch (ke.sym) {
-    case 0: go(k_stencil64z<false, 0, false, JAC, float, double>); break;
-    case 1: go(k_stencil64z<false, 1, false, JAC, float, double>); break;
-    case 2: go(k_stencil64z<false, 2, false, JAC, float, double>); break;
+    case 0: go(k_stencil64z<false, 0, false, EPI, float, double>); break;
+    case 1: go(k_stencil64z<false, 1, false, EPI, float, double>); break;
+    case 2: go(k_stencil64z<false, 2, false, EPI, float, double>); break;
     default: throw Error(kInvalidArgument, "unknown KE0 symmetry class");
   }
   HIPCHK(hipGetLastError());
@@ -1057,14 +1127,14 @@ static unsigned launch_k32(const Lattice &L, const KeCoef &ke, const float *youn
 
 void launch_apply_k32(const Lattice &L, const KeCoef &ke, const float *young32,
                       const uint8_t *freemask, const float *x, double *y, hipStream_t s) {
-  launch_k32<false>(L, ke, young32, freemask, x, y, nullptr, nullptr, JacEpilogue{}, s);
+  launch_k32<0>(L, ke, young32, freemask, x, y, nullptr, nullptr, JacEpilogue{}, s);
 }
 
 unsigned launch_apply_k32_jacobi(const Lattice &L, const KeCoef &ke, const float *young32,
                                  const uint8_t *freemask, const float *x, double *xout,
                                  const double *b, const double *dinv, double omega,
                                  const double *q, double *partials, hipStream_t s) {
-  return launch_k32<true>(L, ke, young32, freemask, x, xout, dinv, partials,
+  return launch_k32<1>(L, ke, young32, freemask, x, xout, dinv, partials,
                           JacEpilogue{b, q, omega}, s);
 }
 

@@ -63,20 +63,32 @@ unsigned launch_apply_k32_jacobi(const Lattice &L, const KeCoef &ke, const float
                                  const uint8_t *freemask, const float *x, double *xout,
                                  const double *b, const double *dinv, double omega,
                                  const double *q, double *partials, hipStream_t s);
-// Arguments of the fused smoother epilogue of the z-marching K.u.
+// Arguments of the fused epilogues of the z-marching K.u: the smoother sweep
+// (y = x + omega dinv (b - K x), with the partials of y.b and y.q when q is
+// given), or with u given the start of a W-cycle's second coarse visit
+// (u = x, y = b - K x masked, and with x0 also x0 = omega dinv y).
 struct JacEpilogue {
   const double *b = nullptr;
   const double *q = nullptr;
   double omega = 0.0;
+  double *u = nullptr;
+  double *x0 = nullptr;
 };
 // The multigrid smoother's sweep with K.x in one z-marching pass:
-// xout = x + omega dinv (b - K x) (x unchanged) and the block partials of
-// xout.b and xout.q (2 per block, the layout of store_partials); returns the
-// block count.  Needs the z-marching kernel shape.
+// xout = x + omega dinv (b - K x) (x unchanged) and, with q and partials given,
+// the block partials of xout.b and xout.q (2 per block, the layout of
+// store_partials); returns the block count.  Needs the z-marching kernel shape.
 unsigned launch_apply_k64_jacobi(const Lattice &L, const KeCoef &ke, const double *young,
                                  const uint8_t *freemask, const double *x, double *xout,
                                  const double *b, const double *dinv, double omega,
                                  const double *q, double *partials, hipStream_t s);
+// The start of a W-cycle's second visit of a z-marching level in one pass:
+// u = xc, r = b - K xc (masked) and, with x0, x0 = omega dinv r (xc may not
+// alias u, r or x0).  Needs the z-marching kernel shape.
+void launch_apply_k64_wres(const Lattice &L, const KeCoef &ke, const double *young,
+                           const uint8_t *freemask, const double *xc, const double *b,
+                           const double *dinv, double omega, double *u, double *r, double *x0,
+                           hipStream_t s);
 // One fused Jacobi-PCG iteration, stencil half: p_new = dinv*r + beta*p_old
 // (dinv == nullptr: p_new = r + beta*p_old, for a preconditioned r);
 // q = K p_new; p.q partials (returns their count); with finalize, alpha = rho /

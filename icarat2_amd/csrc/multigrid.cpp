@@ -397,6 +397,12 @@ double *Multigrid::cycle(int l, const double *b, bool emit, bool x0_ready, ZDot 
     if (direct_[l]) {
       if (emit) coarse_op(l, MG_JACOBI, cur, b, w, oth, s);
       std::swap(cur, oth);
+    } else if (c.kc_.kernel == 0) {
+      // z-marching K.u with the sweep in its epilogue, out of place
+      if (emit)
+        launch_apply_k64_jacobi(L, c.kc_, c.d_young, c.d_free, cur, oth, b, c.d_dinv, w, nullptr,
+                                nullptr, s);
+      std::swap(cur, oth);
     } else if (emit) {
       launch_apply_k64(L, c.kc_, c.d_young, c.d_free, cur, oth, s);
       launch_mg_jacobi(n, w, c.d_dinv, b, oth, cur, s);
@@ -442,6 +448,10 @@ double *Multigrid::cycle(int l, const double *b, bool emit, bool x0_ready, ZDot 
       if (direct_[l + 1] && !galerkin_ && xc != cc.d_x.p) {
         // the copy, the residual and the first sweep from zero in one pass
         launch_mg_direct_wres(Lc, cc.kc_, cc.d_young, cc.d_free, xc, cc.d_b, cc.d_dinv,
+                              omega_[l + 1], cc.d_u, cc.d_nv, x0, s);
+      } else if (!direct_[l + 1] && cc.kc_.kernel == 0 && xc != cc.d_x.p) {
+        // the same in the z-marching K.u's epilogue
+        launch_apply_k64_wres(Lc, cc.kc_, cc.d_young, cc.d_free, xc, cc.d_b, cc.d_dinv,
                               omega_[l + 1], cc.d_u, cc.d_nv, x0, s);
       } else {
         launch_copy64(nc, xc, cc.d_u, s);
