@@ -64,16 +64,16 @@ MI355X_PEAK = {"fp64_tflops": 78.6, "fp32_mfma_tflops": 157.3, "bf16_mfma_tflops
 # key of its PMC traffic line)
 DOMINANT = {
     "fp64": (0, "k_stencil64z<PCG, cube, fused beta> (FP64 VALU, KE0 orbit values in SGPRs)",
-             "fp64_tflops", "k_stencil64z<true,2,true>"),
+             "fp64_tflops", "k_stencil64z<true,2,true,0>"),
     "learned": (0, "k_stencil64z<PCG, cube, fused beta> (FP64 VALU, KE0 orbit values in SGPRs)",
-                "fp64_tflops", "k_stencil64z<true,2,true>"),
+                "fp64_tflops", "k_stencil64z<true,2,true,0>"),
     # the kernel with the largest time share of a multigrid PCG iteration
     "mg": (10, "k_stencil64z<PCG, cube> of the multigrid solve (p = z + beta p_old, q = K p; "
                "FP64 VALU, KE0 orbit values in SGPRs)", "fp64_tflops",
-           "k_stencil64z<true,2,false,false>"),
+           "k_stencil64z<true,2,false,0>"),
     "mg32": (10, "k_stencil64z<PCG, cube> of the multigrid solve (p = z + beta p_old, q = K p; "
                  "FP64 VALU, KE0 orbit values in SGPRs)", "fp64_tflops",
-             "k_stencil64z<true,2,false,false>"),
+             "k_stencil64z<true,2,false,0>"),
     "fp32": (5, "k_stencil_mfma<FP32> (v_mfma_f32_16x16x4_f32, FP64 vectors)", "fp32_mfma_tflops",
              "k_stencil_mfma<0>"),
     "bf16": (6, "k_stencil_mfma<BF16> (v_mfma_f32_16x16x32_bf16, FP64 vectors)", "bf16_mfma_tflops",
