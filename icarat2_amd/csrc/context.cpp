@@ -1398,7 +1398,7 @@ void Context::mg_setup(const icarat_mg_opts &o) {
   if (o.max_levels < 1 || o.nu_pre < 0 || o.nu_post < 1 || o.nu_coarse < 1 || !(o.omega > 0.0) ||
       !(o.omega < 2.0) || o.batch_iters < 1 || o.batch_iters > 64 || o.dense_dofs < -1 ||
       o.direct_nodes < 0 || o.coarse_avg < 0 || o.coarse_avg > 5 || o.w_levels < -1 ||
-      (o.w_levels != 0 && !(o.w_switch >= 1.0)) || o.compensated < 0 || o.compensated > 1)
+      !(o.w_switch >= 0.0) || o.compensated < 0 || o.compensated > 1)
     throw Error(kInvalidArgument, "invalid multigrid options");
   sync();
   mg_.reset();

@@ -296,9 +296,12 @@ void Multigrid::before_batch(hipStream_t s) {
     enqueue_restart();
   }
   // first phase over (the W-cycle, the FP32 fine level): from here on the
-  // FP64 V-cycle, restarted from z = M_V(r)
+  // FP64 V-cycle, restarted from z = M_V(r).  The FP32 fine level ends at
+  // kF32Switch x tol at the latest: its sweeps cannot resolve the last digits
+  // of a field at its FP64 floor.  w_switch 0: the W-cycle runs to the end.
   const PcgScalars *h = f_.h_pcg.p;
-  if (wmode_ && !h->done && h->rr < o_.w_switch * o_.w_switch * tol_ * tol_ * h->bnorm2) {
+  const double ws = f32cur_ ? std::max(o_.w_switch, kF32Switch) : o_.w_switch;
+  if (wmode_ && !h->done && h->rr < ws * ws * tol_ * tol_ * h->bnorm2) {
     wmode_ = false;
     wcur_ = 0;
     f32cur_ = false;

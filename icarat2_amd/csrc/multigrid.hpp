@@ -33,7 +33,7 @@ inline icarat_mg_opts mg_default_opts() {
   o.direct_nodes = 262144;
   o.coarse_avg = 0;
   o.w_levels = -1;
-  o.w_switch = 30.0;
+  o.w_switch = 0.0;
   o.compensated = 1;
   return o;
 }
@@ -131,6 +131,7 @@ class Multigrid {
   DevBuf<double> xbest_;  // best verified iterate of a solve that stalls at its floor
   DevBuf<double> xlo_;    // low part of the compensated iterate (unallocated: plain updates)
   static constexpr int kStallVerifications = 4, kMaxVerifications = 50;
+  static constexpr double kF32Switch = 30.0;  // FP32 fine level off below this x tol
   double best_ = 0.0;
   int since_best_ = 0, verifications_ = 0;
   bool fp32_ = false;

@@ -487,17 +487,18 @@ typedef struct icarat_mg_opts {
                               coarse level above) */
   double w_switch;         /* a solve's first phase -- the W-cycle (w_levels > 0) and the
                               FP32 fine level (precision FP32) -- ends once its recursive
-                              relative residual is below w_switch x tol; it goes on with the
-                              FP64 V-cycle (restarted with beta = 0): the W-cycle's coarse
-                              corrections and the FP32 sweeps lose accuracy near the FP64
-                              floor of high-contrast fields */
+                              relative residual is below w_switch x tol (0: the W-cycle runs
+                              to the end); it goes on with the FP64 V-cycle (restarted with
+                              beta = 0).  The FP32 fine level ends at 30 x tol at the latest:
+                              its sweeps cannot resolve the last digits of a field at its FP64
+                              floor */
   int32_t compensated;     /* 1: the PCG iterate accumulates its updates as a compensated sum
                               (x plus a low-part vector, x its rounding), so updates below
                               half an ulp of x -- the large displacements of near-void
                               regions near the tolerance -- are not lost while the recursive
                               residual counts them; 0: x += alpha p */
 } icarat_mg_opts;
-/* Fills *o with the defaults (10, 1, 1, 6, 1.95, 4, -1, 262144, 0, -1, 30, 1). */
+/* Fills *o with the defaults (10, 1, 1, 6, 1.95, 4, -1, 262144, 0, -1, 0, 1). */
 int icarat_hip_mg_default_opts(icarat_mg_opts *o);
 /* (Re)builds the hierarchy of the context (o = NULL: defaults).  Needs make_dof. */
 int icarat_hip_mg_setup(icarat_hip_ctx *ctx, const icarat_mg_opts *o);

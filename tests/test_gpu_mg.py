@@ -252,10 +252,10 @@ def test_multigrid_loop_repeats_bit_for_bit(gpu):
 
 
 def test_w_cycle_solves_and_switches(gpu):
-    """The W-cycle at the top of the hierarchy (coarse levels 1..w cycled twice,
-    then the V-cycle from w_switch x tol on): the verified residual, the Jacobi
-    solution to 1e-6 of max |u|, fewer iterations than the V-cycle; the
-    automatic depth by mesh size."""
+    """The W-cycle at the top of the hierarchy (coarse levels 1..w cycled twice;
+    to the end by default, or with w_switch the V-cycle from w_switch x tol on):
+    the verified residual, the Jacobi solution to 1e-6 of max |u|, fewer
+    iterations than the V-cycle; the automatic depth by mesh size."""
     dims = (96, 48, 48)
     c = gpu.cantilever(*dims)
     x, _ = gpu.gaussian_field(*dims, 6)
@@ -271,6 +271,12 @@ def test_w_cycle_solves_and_switches(gpu):
         assert np.max(np.abs(um - uj)) < 1e-6 * np.max(np.abs(uj))
         its[w] = sm.iters
     assert its[1] < its[0] and its[2] <= its[1]
+    # the switch to the V-cycle near the tolerance keeps the verified stop
+    c.mg_setup(w_levels=2, w_switch=30.0)
+    um, sm = c.solve(tol=1e-9, precond=gpu.PRECOND_MG)
+    print(f"w_levels 2, w_switch 30: {sm.iters} iterations, relres {sm.true_relres:.2e}")
+    assert sm.status == 0 and sm.true_relres <= 1e-9
+    assert np.max(np.abs(um - uj)) < 1e-6 * np.max(np.abs(uj))
     # automatic: none below 500,000 elements, one level up to 2,000,000
     c.mg_setup()
     c.solve(tol=1e-9, precond=gpu.PRECOND_MG)
