@@ -584,6 +584,56 @@ __global__ __launch_bounds__(NT) void k_stencil64z_mc(
       nt, xcd_slot(T_, nt), T_ == 0 && threadIdx.x == 0, sp, se, sh);
 }
 
+// The multigrid solve's PCG stencil for nc load cases in one launch: case c
+// forms p_new = z + beta_c p_old from its V-cycle result z (no D^-1), q = K p
+// and its p.q partials, each at c x cs; its scalars are S[c], its partials at
+// [c x nb, (c+1) x nb) in the slots of the single-case launch, so every case's
+// sums equal those of its own solve.  Work items run case-fastest.
+template <int SYM>
+__global__ __launch_bounds__(NT) void k_stencil64z_mcz(
+    Lattice L, ZSplit Z, const double *__restrict__ c_KE0, KeVals kv,
+    const double *__restrict__ young, const uint8_t *__restrict__ fmask,
+    const double *__restrict__ z, const double *__restrict__ p_old, double *__restrict__ p_new,
+    double *__restrict__ q, PcgScalars *S, double *partials, int nc, long long cs) {
+  __shared__ double sp[2][3][ZPN];
+  __shared__ double se[2][ZLN];
+  __shared__ double sh[8];
+  const unsigned ntot = gridDim.x, nt = (unsigned)Z.nb;
+  const unsigned w = xcd_order(blockIdx.x, ntot);
+  const int c = (int)(w % nc);
+  const unsigned T_ = w / nc;
+  stencil64z_work<true, SYM, false, 0, double, double>(
+      L, Z, T_, c_KE0, kv, young, fmask, z + cs * c, nullptr, p_old + cs * c, p_new + cs * c,
+      q + cs * c, S + c, partials + (size_t)nt * c, nullptr, 0u, 0, JacEpilogue{}, nt,
+      xcd_slot(T_, nt), T_ == 0 && threadIdx.x == 0, sp, se, sh);
+}
+
+// K.u (EPI 0) or the fine level's last sweep with its partials (EPI 1) for nc
+// load cases in one launch: case c's x, y, b, q at c x cs (D^-1 shared), its
+// two partials per block at [2 c nb, 2 (c+1) nb) in the layout of the
+// single-case launch.
+template <int SYM, int EPI>
+__global__ __launch_bounds__(NT) void k_stencil64z_mcv(
+    Lattice L, ZSplit Z, const double *__restrict__ c_KE0, KeVals kv,
+    const double *__restrict__ young, const uint8_t *__restrict__ fmask,
+    const double *__restrict__ x, const double *__restrict__ dinv, double *__restrict__ y,
+    double *partials, JacEpilogue je, int nc, long long cs) {
+  __shared__ double sp[2][3][ZPN];
+  __shared__ double se[2][ZLN];
+  __shared__ double sh[8];
+  const unsigned ntot = gridDim.x, nt = (unsigned)Z.nb;
+  const unsigned w = xcd_order(blockIdx.x, ntot);
+  const int c = (int)(w % nc);
+  const unsigned T_ = w / nc;
+  JacEpilogue jc = je;
+  if (jc.b) jc.b += cs * c;
+  if (jc.q) jc.q += cs * c;
+  stencil64z_work<false, SYM, false, EPI, double, double>(
+      L, Z, T_, c_KE0, kv, young, fmask, x + cs * c, dinv, nullptr, nullptr, y + cs * c, nullptr,
+      EPI == 1 ? partials + (size_t)2 * nt * c : nullptr, nullptr, 0u, 0, jc, nt, xcd_slot(T_, nt),
+      false, sp, se, sh);
+}
+
 __global__ __launch_bounds__(NT) void k_pcg_update64(long long n, double *__restrict__ x,
                                                      double *__restrict__ r,
                                                      const double *__restrict__ p,
@@ -1297,6 +1347,57 @@ void launch_lattice_to_numeq(const Lattice &L, const int *dof, int numeq, const 
 void launch_axpby(long long n, double a, const double *x, double b, double *y, hipStream_t s) {
   hipLaunchKernelGGL(k_axpby, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, a, x, b, y);
   HIPCHK(hipGetLastError());
+}
+
+// ---- load cases of the multigrid solve in one launch --------------------------
+unsigned launch_pcg_stencil64_mcz(const Lattice &L, const KeCoef &ke, const double *young,
+                                  const uint8_t *freemask, const double *z, const double *p_old,
+                                  double *p_new, double *q, PcgScalars *S, double *partials,
+                                  int nc, long long cs, hipStream_t s) {
+  if (ke.kernel != 0) throw Error(kInvalidArgument, "load cases together need the z-marching K.u");
+  if (nc < 1) throw Error(kInternal, "no load cases");
+  const ZSplit z_ = make_zsplit(L, ke);
+  const dim3 g((unsigned)z_.nb * (unsigned)nc);
+  switch (ke.sym) {
+    case 0: hipLaunchKernelGGL(k_stencil64z_mcz<0>, g, dim3(NT), 0, s, L, z_, ke.ke, ke.vals, young, freemask, z, p_old, p_new, q, S, partials, nc, cs); break;
+    case 1: hipLaunchKernelGGL(k_stencil64z_mcz<1>, g, dim3(NT), 0, s, L, z_, ke.ke, ke.vals, young, freemask, z, p_old, p_new, q, S, partials, nc, cs); break;
+    case 2: hipLaunchKernelGGL(k_stencil64z_mcz<2>, g, dim3(NT), 0, s, L, z_, ke.ke, ke.vals, young, freemask, z, p_old, p_new, q, S, partials, nc, cs); break;
+    default: throw Error(kInvalidArgument, "unknown KE0 symmetry class");
+  }
+  HIPCHK(hipGetLastError());
+  return (unsigned)z_.nb;
+}
+
+unsigned launch_apply_k64_mc(const Lattice &L, const KeCoef &ke, const double *young,
+                             const uint8_t *freemask, const double *x, double *y, const double *b,
+                             const double *dinv, double omega, const double *q, double *partials,
+                             int nc, long long cs, hipStream_t s) {
+  if (ke.kernel != 0) throw Error(kInvalidArgument, "load cases together need the z-marching K.u");
+  if (nc < 1) throw Error(kInternal, "no load cases");
+  const bool jac = b != nullptr;
+  if (jac && (!q || !partials || !dinv)) throw Error(kInternal, "apply_k64_mc: sweep needs q and partials");
+  JacEpilogue je;
+  je.b = b;
+  je.q = q;
+  je.omega = omega;
+  const ZSplit z_ = make_zsplit(L, ke);
+  const dim3 g((unsigned)z_.nb * (unsigned)nc);
+#define MCV_LAUNCH(SYM_)                                                                           \
+  if (jac)                                                                                         \
+    hipLaunchKernelGGL((k_stencil64z_mcv<SYM_, 1>), g, dim3(NT), 0, s, L, z_, ke.ke, ke.vals, young, \
+                       freemask, x, dinv, y, partials, je, nc, cs);                               \
+  else                                                                                             \
+    hipLaunchKernelGGL((k_stencil64z_mcv<SYM_, 0>), g, dim3(NT), 0, s, L, z_, ke.ke, ke.vals, young, \
+                       freemask, x, dinv, y, partials, je, nc, cs);
+  switch (ke.sym) {
+    case 0: MCV_LAUNCH(0); break;
+    case 1: MCV_LAUNCH(1); break;
+    case 2: MCV_LAUNCH(2); break;
+    default: throw Error(kInvalidArgument, "unknown KE0 symmetry class");
+  }
+#undef MCV_LAUNCH
+  HIPCHK(hipGetLastError());
+  return (unsigned)z_.nb;
 }
 
 }  // namespace icarat_hip

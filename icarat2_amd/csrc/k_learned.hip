@@ -328,8 +328,22 @@ __global__ __launch_bounds__(NT) void k_fpcg_update_alpha(
     long long n, double *__restrict__ x, double *__restrict__ r, const double *__restrict__ p,
     const double *__restrict__ q, PcgScalars *S, const double *__restrict__ part_st, unsigned Gs,
     double *part_up, const double *__restrict__ dinv, double omega, TZ *__restrict__ z0,
-    double *__restrict__ xlo) {
+    double *__restrict__ xlo, long long cs) {
   __shared__ double sh[8];
+  // load case blockIdx.y: vectors at case x cs (dinv shared), its scalars,
+  // the stencil's partials at case x Gs, ours at case x gridDim.x
+  {
+    const unsigned c = blockIdx.y;
+    S += c;
+    part_st += (size_t)c * Gs;
+    part_up += (size_t)c * gridDim.x;
+    x += c * cs;
+    r += c * cs;
+    p += c * cs;
+    q += c * cs;
+    if (z0) z0 += c * cs;
+    if (COMP) xlo += c * cs;
+  }
   if (S->done) return;
   double tot[1];
   sum_partials<NT, 1>(part_st, Gs, tot, sh);
@@ -396,7 +410,7 @@ __global__ __launch_bounds__(NT) void k_fpcg_update_alpha(
     if (z0) z0[i] = zi;
   }
   double v[1] = {rr};
-  store_partials<NT, 1>(v, part_up, sh);
+  store_partials_at<NT, 1>(v, part_up, sh, gridDim.x, blockIdx.x);
 }
 
 // z.r and z.q partials (no finalize)
@@ -421,6 +435,10 @@ __global__ __launch_bounds__(NT) void k_zdot_partials(long long n, const double 
 __global__ void k_fpcg_check_beta(const double *__restrict__ part_up, unsigned Gu,
                                   const double *__restrict__ part_z, unsigned Gz, PcgScalars *S) {
   __shared__ double sh[8];
+  // one block per load case (one update partial, two z partials per block)
+  part_up += (size_t)blockIdx.x * Gu;
+  part_z += (size_t)blockIdx.x * 2 * Gz;
+  S += blockIdx.x;
   if (S->done) return;
   double t1[1], t2[2];
   sum_partials<NT, 1>(part_up, Gu, t1, sh);
@@ -551,13 +569,15 @@ void launch_fpcg_update(long long n, double *x, double *r, const double *p, cons
 unsigned launch_fpcg_update_alpha(long long n, double *x, double *r, const double *p,
                                   const double *q, PcgScalars *S, const double *part_st,
                                   unsigned Gs, double *part_up, const double *dinv, double omega,
-                                  double *z0, hipStream_t s, double *xlo) {
+                                  double *z0, hipStream_t s, double *xlo, int nc, long long cs) {
+  if (nc < 1) throw Error(kInternal, "fpcg update: no load case");
+  const dim3 g(VEC_BLOCKS, (unsigned)nc);
   if (xlo)
-    hipLaunchKernelGGL((k_fpcg_update_alpha<double, true>), dim3(VEC_BLOCKS), dim3(NT), 0, s, n, x,
-                       r, p, q, S, part_st, Gs, part_up, dinv, omega, z0, xlo);
+    hipLaunchKernelGGL((k_fpcg_update_alpha<double, true>), g, dim3(NT), 0, s, n, x, r, p, q, S,
+                       part_st, Gs, part_up, dinv, omega, z0, xlo, cs);
   else
-    hipLaunchKernelGGL((k_fpcg_update_alpha<double, false>), dim3(VEC_BLOCKS), dim3(NT), 0, s, n, x,
-                       r, p, q, S, part_st, Gs, part_up, dinv, omega, z0, nullptr);
+    hipLaunchKernelGGL((k_fpcg_update_alpha<double, false>), g, dim3(NT), 0, s, n, x, r, p, q, S,
+                       part_st, Gs, part_up, dinv, omega, z0, nullptr, cs);
   HIPCHK(hipGetLastError());
   return VEC_BLOCKS;
 }
@@ -568,10 +588,10 @@ unsigned launch_fpcg_update_alpha(long long n, double *x, double *r, const doubl
                                   float *z0, hipStream_t s, double *xlo) {
   if (xlo)
     hipLaunchKernelGGL((k_fpcg_update_alpha<float, true>), dim3(VEC_BLOCKS), dim3(NT), 0, s, n, x,
-                       r, p, q, S, part_st, Gs, part_up, dinv, omega, z0, xlo);
+                       r, p, q, S, part_st, Gs, part_up, dinv, omega, z0, xlo, 0LL);
   else
     hipLaunchKernelGGL((k_fpcg_update_alpha<float, false>), dim3(VEC_BLOCKS), dim3(NT), 0, s, n, x,
-                       r, p, q, S, part_st, Gs, part_up, dinv, omega, z0, nullptr);
+                       r, p, q, S, part_st, Gs, part_up, dinv, omega, z0, nullptr, 0LL);
   HIPCHK(hipGetLastError());
   return VEC_BLOCKS;
 }
@@ -589,8 +609,10 @@ void launch_fpcg_restart_flag(PcgScalars *S, hipStream_t s) {
 }
 
 void launch_fpcg_check_beta(const double *part_up, unsigned Gu, const double *part_z, unsigned Gz,
-                            PcgScalars *S, hipStream_t s) {
-  hipLaunchKernelGGL(k_fpcg_check_beta, dim3(1), dim3(NT), 0, s, part_up, Gu, part_z, Gz, S);
+                            PcgScalars *S, hipStream_t s, int nc) {
+  if (nc < 1) throw Error(kInternal, "fpcg check: no load case");
+  hipLaunchKernelGGL(k_fpcg_check_beta, dim3((unsigned)nc), dim3(NT), 0, s, part_up, Gu, part_z, Gz,
+                     S);
   HIPCHK(hipGetLastError());
 }
 

@@ -100,8 +100,14 @@ __global__ __launch_bounds__(256) void k_mg_restrict(Lattice Lf, Lattice Lc,
                                                      const uint8_t *__restrict__ fmask_c,
                                                      double *__restrict__ bc,
                                                      const double *__restrict__ dinv_c,
-                                                     double omega, double *__restrict__ xc) {
+                                                     double omega, double *__restrict__ xc,
+                                                     long long sfs, long long scs) {
   __shared__ double sv[3][RFN];
+  // load case blockIdx.y: its vectors start at blockIdx.y x the level's stride
+  rf += blockIdx.y * sfs;
+  if (HAS_Y) yf += blockIdx.y * sfs;
+  bc += blockIdx.y * scs;
+  if (xc) xc += blockIdx.y * scs;
   int bx, by, bz;
   xcd_tile((Lc.NX + RTX - 1) / RTX, (Lc.NY + RTY - 1) / RTY, bx, by, bz);
   const int I0 = bx * RTX, J0 = by * RTY, K0 = bz * RTZ;
@@ -165,8 +171,12 @@ __global__ __launch_bounds__(256) void k_mg_prolong_add(Lattice Lf, Lattice Lc,
                                                         const double *__restrict__ xc,
                                                         const double *__restrict__ xc2,
                                                         const uint8_t *__restrict__ fmask_f,
-                                                        TF *__restrict__ xf) {
+                                                        TF *__restrict__ xf, long long sfs,
+                                                        long long scs) {
   __shared__ double sc[3][PCN];
+  xc += blockIdx.y * scs;
+  if (xc2) xc2 += blockIdx.y * scs;
+  xf += blockIdx.y * sfs;
   int bx, by, bz;
   xcd_tile((Lf.NX + PTX - 1) / PTX, (Lf.NY + PTY - 1) / PTY, bx, by, bz);
   const int fx0 = bx * PTX, fy0 = by * PTY, fz0 = bz * PTZ;
@@ -259,6 +269,11 @@ __global__ void k_mg_jacobi(long long n, double omega, const double *__restrict_
                             double *__restrict__ x) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i >= n) return;
+  // load case blockIdx.y (stride n; dinv is the level's, shared)
+  const long long o = blockIdx.y * n;
+  b += o;
+  x += o;
+  if (y) y += o;
   if (y)
     x[i] += omega * dinv[i] * (b[i] - y[i]);
   else
@@ -295,9 +310,12 @@ __global__ __launch_bounds__(256) void k_mg_direct(Lattice L, const double *__re
                                                    const double *__restrict__ x,
                                                    const double *__restrict__ b,
                                                    const double *__restrict__ dinv, double omega,
-                                                   double *__restrict__ out) {
+                                                   double *__restrict__ out, long long cs) {
   const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (n >= L.numnp) return;
+  x += blockIdx.y * cs;
+  b += blockIdx.y * cs;
+  out += blockIdx.y * cs;
   double y[3], xc[3];
   direct_node<SYM>(L, c_KE0, kv, young, x, n, y, xc);
   direct_out<MODE>(L.Np, n, fmask[n], y, xc, b, dinv, omega, out);
@@ -316,9 +334,14 @@ __global__ __launch_bounds__(256) void k_mg_direct_wres(Lattice L, const double 
                                                         const double *__restrict__ dinv,
                                                         double omega, double *__restrict__ u,
                                                         double *__restrict__ r,
-                                                        double *__restrict__ x0) {
+                                                        double *__restrict__ x0, long long cs) {
   const long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (n >= L.numnp) return;
+  xc += blockIdx.y * cs;
+  b += blockIdx.y * cs;
+  u += blockIdx.y * cs;
+  r += blockIdx.y * cs;
+  if (x0) x0 += blockIdx.y * cs;
   double y[3], xn[3];
   direct_node<SYM>(L, c_KE0, kv, young, xc, n, y, xn);
   const unsigned m = fmask[n];
@@ -349,10 +372,17 @@ __global__ __launch_bounds__(256) void k_mg_prolong_jacobi(
     const double *__restrict__ young, const uint8_t *__restrict__ fmask,
     const double *__restrict__ xcoarse, const double *__restrict__ xcoarse2,
     const double *__restrict__ x, const double *__restrict__ b, const double *__restrict__ dinv,
-    double omega, double *__restrict__ out) {
+    double omega, double *__restrict__ out, long long sfs, long long scs) {
   __shared__ double sc[3][QCN];
   __shared__ double su[3][QHN];
-  const int fx0 = blockIdx.x * QTX, fy0 = blockIdx.y * QTY, fz0 = blockIdx.z * QTZ;
+  // load case blockIdx.z / (z tiles): its vectors at case x the level's stride
+  const int nbz = (Lf.NZ + QTZ - 1) / QTZ, lc = blockIdx.z / nbz;
+  xcoarse += lc * scs;
+  if (xcoarse2) xcoarse2 += lc * scs;
+  x += lc * sfs;
+  b += lc * sfs;
+  out += lc * sfs;
+  const int fx0 = blockIdx.x * QTX, fy0 = blockIdx.y * QTY, fz0 = (blockIdx.z % nbz) * QTZ;
   // coarse node C0 + h covers fine halo nodes from 2 (C0 + h) - 1 to 2 (C0 + h) + 1
   const int cx0 = fx0 / 2 - 1, cy0 = fy0 / 2 - 1, cz0 = fz0 / 2 - 1;
   const int tx = threadIdx.x % QTX, ty = (threadIdx.x / QTX) % QTY, tz = threadIdx.x / (QTX * QTY);
@@ -538,60 +568,68 @@ __device__ __forceinline__ double dense_rhs(const Lattice &L, const double *__re
 
 __global__ __launch_bounds__(256) void k_mg_dense_symv(Lattice L, const double *__restrict__ A,
                                                        const double *__restrict__ b,
-                                                       double *__restrict__ part) {
+                                                       double *__restrict__ part, int nc,
+                                                       long long bs, long long ps) {
   __shared__ double tile[DT][DT + 1];
   __shared__ double sbi[DT], sbj[DT];
   const int n = (int)(3 * L.numnp), nt = (n + DT - 1) / DT, ntiles = nt * (nt + 1) / 2;
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-  // tile t -> (I, J), I >= J, t = I (I + 1) / 2 + J
-  int I = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-  while ((I + 1) * (I + 2) / 2 <= t) ++I;
-  while (I * (I + 1) / 2 > t) --I;
-  const int J = t - I * (I + 1) / 2;
-  if (threadIdx.x < DT) {
-    sbi[threadIdx.x] = dense_rhs(L, b, I * DT + threadIdx.x, n);
-    sbj[threadIdx.x] = dense_rhs(L, b, J * DT + threadIdx.x, n);
-  }
-  // the tile through LDS: rows of 64 contiguous doubles per wave
-  {
-    const int c = threadIdx.x & 63, w = threadIdx.x >> 6, col = J * DT + c;
-    double a[DT / 4];
+    // tile t -> (I, J), I >= J, t = I (I + 1) / 2 + J
+    int I = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((I + 1) * (I + 2) / 2 <= t) ++I;
+    while (I * (I + 1) / 2 > t) --I;
+    const int J = t - I * (I + 1) / 2;
+    // the tile through LDS once: rows of 64 contiguous doubles per wave
+    {
+      const int c = threadIdx.x & 63, w = threadIdx.x >> 6, col = J * DT + c;
+      double a[DT / 4];
 #pragma unroll
-    for (int k = 0; k < DT / 4; ++k) {
-      const int row = I * DT + w + 4 * k;
-      a[k] = (row < n && col < n) ? A[(long long)row * n + col] : 0.0;
+      for (int k = 0; k < DT / 4; ++k) {
+        const int row = I * DT + w + 4 * k;
+        a[k] = (row < n && col < n) ? A[(long long)row * n + col] : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < DT / 4; ++k) tile[w + 4 * k][c] = a[k];
     }
+    // then every right-hand side (load case lc at lc x bs, its partials at lc x ps)
+    for (int lc = 0; lc < nc; ++lc) {
+      if (threadIdx.x < DT) {
+        sbi[threadIdx.x] = dense_rhs(L, b + lc * bs, I * DT + threadIdx.x, n);
+        sbj[threadIdx.x] = dense_rhs(L, b + lc * bs, J * DT + threadIdx.x, n);
+      }
+      __syncthreads();
+      // four threads per row (row sums) and per column (column sums)
+      const int e = threadIdx.x >> 2, q = threadIdx.x & 3;
+      double u = 0.0, v = 0.0;
 #pragma unroll
-    for (int k = 0; k < DT / 4; ++k) tile[w + 4 * k][c] = a[k];
-  }
-  __syncthreads();
-  // four threads per row (row sums) and per column (column sums)
-  const int e = threadIdx.x >> 2, q = threadIdx.x & 3;
-  double u = 0.0, v = 0.0;
-#pragma unroll
-  for (int k = 0; k < DT / 4; ++k) {
-    const int o = q * (DT / 4) + k;
-    u += tile[e][o] * sbj[o];
-    v += tile[o][e] * sbi[o];
-  }
-  u += __shfl_xor(u, 1);
-  u += __shfl_xor(u, 2);
-  v += __shfl_xor(v, 1);
-  v += __shfl_xor(v, 2);
-  if (q == 0) {
-    part[((long long)I * nt + J) * DT + e] = u;
-    if (I > J) part[((long long)J * nt + I) * DT + e] = v;
-  }
-  __syncthreads();  // the tile and the b slices are refilled for the next tile
+      for (int k = 0; k < DT / 4; ++k) {
+        const int o = q * (DT / 4) + k;
+        u += tile[e][o] * sbj[o];
+        v += tile[o][e] * sbi[o];
+      }
+      u += __shfl_xor(u, 1);
+      u += __shfl_xor(u, 2);
+      v += __shfl_xor(v, 1);
+      v += __shfl_xor(v, 2);
+      if (q == 0) {
+        double *pc = part + lc * ps;
+        pc[((long long)I * nt + J) * DT + e] = u;
+        if (I > J) pc[((long long)J * nt + I) * DT + e] = v;
+      }
+      __syncthreads();  // the b slices are refilled for the next case or tile
+    }
   }
 }
 
 // x rows of block I: the dense_tiles partial slots summed (a quarter of the
-// slots per thread, then the four quarters in order)
+// slots per thread, then the four quarters in order); load case blockIdx.y
 __global__ __launch_bounds__(256) void k_mg_dense_sum(Lattice L, const double *__restrict__ part,
-                                                      double *__restrict__ x) {
+                                                      double *__restrict__ x, long long ps,
+                                                      long long xs) {
   __shared__ double sh[4][DT];
   const int n = (int)(3 * L.numnp), nt = (n + DT - 1) / DT, np1 = (int)L.numnp;
+  part += blockIdx.y * ps;
+  x += blockIdx.y * xs;
   const int I = blockIdx.x, r = threadIdx.x & 63, q = threadIdx.x >> 6;
   double s = 0.0;
   for (int J = q; J < nt; J += 4) s += part[((long long)I * nt + J) * DT + r];
@@ -606,14 +644,14 @@ __global__ __launch_bounds__(256) void k_mg_dense_sum(Lattice L, const double *_
 template <int SYM>
 void direct_dispatch(int mode, const Lattice &L, const KeCoef &kc, const double *young,
                      const uint8_t *fmask, const double *x, const double *b, const double *dinv,
-                     double omega, double *out, hipStream_t s) {
-  const dim3 g(blocks_for(L.numnp, 256)), t(256);
+                     double omega, double *out, hipStream_t s, int nc, long long cs) {
+  const dim3 g(blocks_for(L.numnp, 256), (unsigned)nc), t(256);
   if (mode == MG_JACOBI)
     hipLaunchKernelGGL((k_mg_direct<SYM, MG_JACOBI>), g, t, 0, s, L, kc.ke, kc.vals, young, fmask,
-                       x, b, dinv, omega, out);
+                       x, b, dinv, omega, out, cs);
   else
     hipLaunchKernelGGL((k_mg_direct<SYM, MG_RESIDUAL>), g, t, 0, s, L, kc.ke, kc.vals, young,
-                       fmask, x, b, dinv, omega, out);
+                       fmask, x, b, dinv, omega, out, cs);
 }
 
 }  // namespace
@@ -625,7 +663,7 @@ void launch_mg_prolong_add_f32(const Lattice &Lf, const Lattice &Lc, const doubl
   const dim3 g((unsigned)(((Lf.NX + PTX - 1) / PTX) * ((Lf.NY + PTY - 1) / PTY) *
                           ((Lf.NZ + PTZ - 1) / PTZ)));
   hipLaunchKernelGGL((k_mg_prolong_add<true, float>), g, dim3(256), 0, s, Lf, Lc, xc, nullptr, fmask_f,
-                     xf);
+                     xf, 0LL, 0LL);
   HIPCHK(hipGetLastError());
 }
 
@@ -638,31 +676,37 @@ void launch_mg_jacobi0_f32(long long n, double omega, const double *dinv, const 
 
 void launch_mg_restrict(const Lattice &Lf, const Lattice &Lc, const double *rf, const double *yf,
                         const uint8_t *fmask_c, double *bc, const double *dinv_c, double omega,
-                        double *xc, hipStream_t s) {
+                        double *xc, hipStream_t s, int nc, long long sf, long long sc) {
   if (Lf.NX != 2 * Lc.NX - 1 || Lf.NY != 2 * Lc.NY - 1 || Lf.NZ != 2 * Lc.NZ - 1)
     throw Error(kInvalidArgument, "restriction: lattices are not 2x apart");
+  if (nc < 1) throw Error(kInternal, "restriction: no load case");
   const dim3 g((unsigned)(((Lc.NX + RTX - 1) / RTX) * ((Lc.NY + RTY - 1) / RTY) *
-                          ((Lc.NZ + RTZ - 1) / RTZ)));
+                          ((Lc.NZ + RTZ - 1) / RTZ)),
+               (unsigned)nc);
   if (yf)
     hipLaunchKernelGGL(k_mg_restrict<true>, g, dim3(256), 0, s, Lf, Lc, rf, yf, fmask_c, bc, dinv_c,
-                       omega, xc);
+                       omega, xc, sf, sc);
   else
     hipLaunchKernelGGL(k_mg_restrict<false>, g, dim3(256), 0, s, Lf, Lc, rf, yf, fmask_c, bc, dinv_c,
-                       omega, xc);
+                       omega, xc, sf, sc);
   HIPCHK(hipGetLastError());
 }
 
 void launch_mg_prolong_add(const Lattice &Lf, const Lattice &Lc, const double *xc,
                            const uint8_t *fmask_f, double *xf, hipStream_t s, bool add,
-                           const double *xc2) {
+                           const double *xc2, int nc, long long sf, long long sc) {
   if (Lf.NX != 2 * Lc.NX - 1 || Lf.NY != 2 * Lc.NY - 1 || Lf.NZ != 2 * Lc.NZ - 1)
     throw Error(kInvalidArgument, "prolongation: lattices are not 2x apart");
+  if (nc < 1) throw Error(kInternal, "prolongation: no load case");
   const dim3 g((unsigned)(((Lf.NX + PTX - 1) / PTX) * ((Lf.NY + PTY - 1) / PTY) *
-                          ((Lf.NZ + PTZ - 1) / PTZ)));
+                          ((Lf.NZ + PTZ - 1) / PTZ)),
+               (unsigned)nc);
   if (add)
-    hipLaunchKernelGGL(k_mg_prolong_add<true>, g, dim3(256), 0, s, Lf, Lc, xc, xc2, fmask_f, xf);
+    hipLaunchKernelGGL(k_mg_prolong_add<true>, g, dim3(256), 0, s, Lf, Lc, xc, xc2, fmask_f, xf, sf,
+                       sc);
   else
-    hipLaunchKernelGGL(k_mg_prolong_add<false>, g, dim3(256), 0, s, Lf, Lc, xc, xc2, fmask_f, xf);
+    hipLaunchKernelGGL(k_mg_prolong_add<false>, g, dim3(256), 0, s, Lf, Lc, xc, xc2, fmask_f, xf, sf,
+                       sc);
   HIPCHK(hipGetLastError());
 }
 
@@ -702,14 +746,17 @@ long long mg_dense_part_size(const Lattice &L) {
 }
 
 void launch_mg_dense_solve(const Lattice &L, const double *ainv, const double *b, double *x,
-                           double *part, hipStream_t s) {
+                           double *part, hipStream_t s, int nc, long long cs) {
   if (3 * L.numnp > MG_DENSE_MAX) throw Error(kInvalidArgument, "lattice too large for the dense solve");
-  const long long n = 3 * L.numnp, nt = (n + DT - 1) / DT;
-  // one round of resident blocks (4 per CU by LDS), tiles in a grid-stride loop
+  if (nc < 1) throw Error(kInternal, "dense solve: no load case");
+  const long long n = 3 * L.numnp, nt = (n + DT - 1) / DT, ps = mg_dense_part_size(L);
+  // one round of resident blocks (4 per CU by LDS), tiles in a grid-stride loop;
+  // A^-1 read once for every load case
   const long long ntiles = nt * (nt + 1) / 2;
   hipLaunchKernelGGL(k_mg_dense_symv, dim3((unsigned)std::min<long long>(ntiles, 1024)), dim3(256), 0,
-                     s, L, ainv, b, part);
-  hipLaunchKernelGGL(k_mg_dense_sum, dim3((unsigned)nt), dim3(256), 0, s, L, part, x);
+                     s, L, ainv, b, part, nc, cs, ps);
+  hipLaunchKernelGGL(k_mg_dense_sum, dim3((unsigned)nt, (unsigned)nc), dim3(256), 0, s, L, part, x,
+                     ps, cs);
   HIPCHK(hipGetLastError());
 }
 
@@ -726,8 +773,10 @@ void launch_mg_young(const Lattice &Lf, const Lattice &Lc, const double *yf, dou
 }
 
 void launch_mg_jacobi(long long n, double omega, const double *dinv, const double *b,
-                      const double *y, double *x, hipStream_t s) {
-  hipLaunchKernelGGL(k_mg_jacobi, dim3(blocks_for(n, 256)), dim3(256), 0, s, n, omega, dinv, b, y, x);
+                      const double *y, double *x, hipStream_t s, int nc) {
+  if (nc < 1) throw Error(kInternal, "jacobi: no load case");
+  hipLaunchKernelGGL(k_mg_jacobi, dim3(blocks_for(n, 256), (unsigned)nc), dim3(256), 0, s, n, omega,
+                     dinv, b, y, x);
   HIPCHK(hipGetLastError());
 }
 
@@ -743,51 +792,57 @@ unsigned launch_mg_jacobi_dot(long long n, double omega, const double *dinv, con
 void launch_mg_prolong_jacobi(const Lattice &Lf, const Lattice &Lc, const KeCoef &kc,
                               const double *young, const uint8_t *fmask, const double *xcoarse,
                               const double *x, const double *b, const double *dinv, double omega,
-                              double *out, hipStream_t s, const double *xcoarse2) {
+                              double *out, hipStream_t s, const double *xcoarse2, int nc,
+                              long long sf, long long sc) {
   if (Lf.NX != 2 * Lc.NX - 1 || Lf.NY != 2 * Lc.NY - 1 || Lf.NZ != 2 * Lc.NZ - 1)
     throw Error(kInvalidArgument, "prolongation: lattices are not 2x apart");
   if (out == x) throw Error(kInternal, "prolong_jacobi: out may not alias x");
-  const dim3 g((Lf.NX + QTX - 1) / QTX, (Lf.NY + QTY - 1) / QTY, (Lf.NZ + QTZ - 1) / QTZ), t(256);
+  if (nc < 1) throw Error(kInternal, "prolong_jacobi: no load case");
+  const dim3 g((Lf.NX + QTX - 1) / QTX, (Lf.NY + QTY - 1) / QTY,
+               ((Lf.NZ + QTZ - 1) / QTZ) * (unsigned)nc),
+      t(256);
   if (kc.sym == 2)
     hipLaunchKernelGGL(k_mg_prolong_jacobi<2>, g, t, 0, s, Lf, Lc, kc.ke, kc.vals, young, fmask,
-                       xcoarse, xcoarse2, x, b, dinv, omega, out);
+                       xcoarse, xcoarse2, x, b, dinv, omega, out, sf, sc);
   else if (kc.sym == 1)
     hipLaunchKernelGGL(k_mg_prolong_jacobi<1>, g, t, 0, s, Lf, Lc, kc.ke, kc.vals, young, fmask,
-                       xcoarse, xcoarse2, x, b, dinv, omega, out);
+                       xcoarse, xcoarse2, x, b, dinv, omega, out, sf, sc);
   else
     hipLaunchKernelGGL(k_mg_prolong_jacobi<0>, g, t, 0, s, Lf, Lc, kc.ke, kc.vals, young, fmask,
-                       xcoarse, xcoarse2, x, b, dinv, omega, out);
+                       xcoarse, xcoarse2, x, b, dinv, omega, out, sf, sc);
   HIPCHK(hipGetLastError());
 }
 
 void launch_mg_direct_wres(const Lattice &L, const KeCoef &kc, const double *young,
                            const uint8_t *fmask, const double *xc, const double *b,
                            const double *dinv, double omega, double *u, double *r, double *x0,
-                           hipStream_t s) {
+                           hipStream_t s, int nc, long long cs) {
   if (xc == u || xc == r || xc == x0) throw Error(kInternal, "direct_wres: xc aliases an output");
-  const dim3 g(blocks_for(L.numnp, 256)), t(256);
+  if (nc < 1) throw Error(kInternal, "direct_wres: no load case");
+  const dim3 g(blocks_for(L.numnp, 256), (unsigned)nc), t(256);
   if (kc.sym == 2)
     hipLaunchKernelGGL(k_mg_direct_wres<2>, g, t, 0, s, L, kc.ke, kc.vals, young, fmask, xc, b, dinv,
-                       omega, u, r, x0);
+                       omega, u, r, x0, cs);
   else if (kc.sym == 1)
     hipLaunchKernelGGL(k_mg_direct_wres<1>, g, t, 0, s, L, kc.ke, kc.vals, young, fmask, xc, b, dinv,
-                       omega, u, r, x0);
+                       omega, u, r, x0, cs);
   else
     hipLaunchKernelGGL(k_mg_direct_wres<0>, g, t, 0, s, L, kc.ke, kc.vals, young, fmask, xc, b, dinv,
-                       omega, u, r, x0);
+                       omega, u, r, x0, cs);
   HIPCHK(hipGetLastError());
 }
 
 void launch_mg_direct(int mode, const Lattice &L, const KeCoef &kc, const double *young,
                       const uint8_t *fmask, const double *x, const double *b, const double *dinv,
-                      double omega, double *out, hipStream_t s) {
+                      double omega, double *out, hipStream_t s, int nc, long long cs) {
   if (mode != MG_JACOBI && mode != MG_RESIDUAL) throw Error(kInvalidArgument, "unknown direct mode");
+  if (nc < 1) throw Error(kInternal, "direct: no load case");
   if (kc.sym == 2)
-    direct_dispatch<2>(mode, L, kc, young, fmask, x, b, dinv, omega, out, s);
+    direct_dispatch<2>(mode, L, kc, young, fmask, x, b, dinv, omega, out, s, nc, cs);
   else if (kc.sym == 1)
-    direct_dispatch<1>(mode, L, kc, young, fmask, x, b, dinv, omega, out, s);
+    direct_dispatch<1>(mode, L, kc, young, fmask, x, b, dinv, omega, out, s, nc, cs);
   else
-    direct_dispatch<0>(mode, L, kc, young, fmask, x, b, dinv, omega, out, s);
+    direct_dispatch<0>(mode, L, kc, young, fmask, x, b, dinv, omega, out, s, nc, cs);
   HIPCHK(hipGetLastError());
 }
 

@@ -46,11 +46,13 @@ void launch_fpcg_update(long long n, double *x, double *r, const double *p, cons
 // stencil's partials (Gs blocks) in every block, x and r updated, r.r partials
 // to part_up, and with z0 != nullptr also z0 = omega dinv r.  With xlo !=
 // nullptr, x is updated as a compensated sum whose low part is xlo.  Returns
-// the partials count.
+// the partials count (per load case: nc cases stacked with stride cs, their
+// scalars S[c], the stencil's partials at c x Gs, ours at c x the count).
 unsigned launch_fpcg_update_alpha(long long n, double *x, double *r, const double *p,
                                   const double *q, PcgScalars *S, const double *part_st,
                                   unsigned Gs, double *part_up, const double *dinv, double omega,
-                                  double *z0, hipStream_t s, double *xlo = nullptr);
+                                  double *z0, hipStream_t s, double *xlo = nullptr, int nc = 1,
+                                  long long cs = 0);
 // the same with z0 in FP32 (the FP32 fine level of the multigrid V-cycle)
 unsigned launch_fpcg_update_alpha(long long n, double *x, double *r, const double *p,
                                   const double *q, PcgScalars *S, const double *part_st,
@@ -61,8 +63,9 @@ unsigned launch_zdot_partials(long long n, const double *z, const double *r, con
                               double *part_z, hipStream_t s);
 // the next beta step restarts (beta = 0) unless the solve is done
 void launch_fpcg_restart_flag(PcgScalars *S, hipStream_t s);
-// stop tests from the r.r partials, then rho' = z.r and beta (Polak-Ribiere)
+// stop tests from the r.r partials, then rho' = z.r and beta (Polak-Ribiere);
+// nc load cases: one block per case, its partials at c x Gu and 2 c x Gz, S[c]
 void launch_fpcg_check_beta(const double *part_up, unsigned Gu, const double *part_z, unsigned Gz,
-                            PcgScalars *S, hipStream_t s);
+                            PcgScalars *S, hipStream_t s, int nc = 1);
 
 }  // namespace icarat_hip
