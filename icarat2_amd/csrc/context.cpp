@@ -592,8 +592,9 @@ void Context::pcg_finish(icarat_solve_stats &S) {
 
 // ---- every load case in one multi-column Jacobi-PCG ---------------------------
 bool Context::cases_together(const icarat_solve_opts &o) const {
-  return o.precond == ICARAT_HIP_PRECOND_JACOBI && o.precision == ICARAT_HIP_FP64 &&
-         kc_.kernel == 0;
+  if (o.precision != ICARAT_HIP_FP64 || kc_.kernel != 0) return false;
+  if (o.precond == ICARAT_HIP_PRECOND_JACOBI) return true;
+  return o.precond == ICARAT_HIP_PRECOND_MG && mg_ && mg_->cases_capable(o);
 }
 
 void Context::mc_drop_graph() {
@@ -657,6 +658,10 @@ void Context::mc_solve(const icarat_solve_opts &o, icarat_solve_stats *st) {
     init.tol = o.tol;
     init.maxit = o.maxit;
     mc_.hS.p[c] = init;
+  }
+  if (o.precond == ICARAT_HIP_PRECOND_MG) {
+    mc_solve_mg(o, st);
+    return;
   }
   h2d(mc_.S.p, mc_.hS.p, nc, stream_);
   for (int c = 0; c < nc; ++c) {
@@ -736,15 +741,44 @@ void Context::mc_solve(const icarat_solve_opts &o, icarat_solve_stats *st) {
     S.refine_steps = repl[c];
     S.rec_relres = r.bnorm2 > 0 ? std::sqrt(r.rr) / std::sqrt(r.bnorm2) : 0.0;
     if (st) st[c] = S;
-    // the case's warm start
+  }
+  mc_save_warm();
+}
+
+void Context::mc_save_warm() {
+  // each case's warm start
+  for (int c = 0; c < mc_.nc; ++c) {
     if (c >= (int)d_xcases_.size()) {
       d_xcases_.resize(c + 1);
       xcase_valid_.resize(c + 1, 0);
     }
-    if (!d_xcases_[c].p) d_xcases_[c].alloc(n3);
-    d2d(d_xcases_[c].p, vec(mc_.X, c), n3, stream_);
+    if (!d_xcases_[c].p) d_xcases_[c].alloc(n3_);
+    d2d(d_xcases_[c].p, mc_.X.p + (size_t)c * n3_, n3_, stream_);
     xcase_valid_[c] = 1;
   }
+}
+
+// The multigrid solve of every case together (right-hand sides in mc_.B): a
+// case whose W-cycle broke down is solved on its own afterwards, as its own
+// solve would have gone on.
+void Context::mc_solve_mg(const icarat_solve_opts &o, icarat_solve_stats *st) {
+  const int nc = mc_.nc;
+  std::vector<char> guess(nc, 0), redo;
+  for (int c = 0; c < nc; ++c)
+    if (o.warm && c < (int)xcase_valid_.size() && xcase_valid_[c]) {
+      d2d(mc_.X.p + (size_t)c * n3_, d_xcases_[c].p, n3_, stream_);
+      guess[c] = 1;
+    }
+  std::vector<icarat_solve_stats> S(nc);
+  mg_->solve_cases(o, nc, mc_.B, mc_.X, guess.data(), S.data(), redo);
+  for (int c = 0; c < nc; ++c) {
+    if (!redo[c]) continue;
+    use_load_case(c);
+    solve(o, nullptr, &S[c]);
+    d2d(mc_.X.p + (size_t)c * n3_, d_x.p, n3_, stream_);
+  }
+  if (st) std::copy(S.begin(), S.end(), st);
+  mc_save_warm();
 }
 
 void Context::mc_take_case(int c) {
@@ -766,6 +800,7 @@ void Context::solve_cases(const icarat_solve_opts &o, double *u_host, icarat_sol
   const int numeq = prob_.numeq();
   const int active = prob_.load_case();
   std::vector<icarat_solve_stats> S(nc);
+  if (o.precond == ICARAT_HIP_PRECOND_MG && !mg_) mg_setup(mg_default_opts());
   if (!cases_together(o)) {
     // preconditioners and precisions without a multi-column iteration: one case after another
     for (int c = 0; c < nc; ++c) {
@@ -1272,6 +1307,7 @@ void Context::simp_run(const icarat_simp_opts &o, double *x, double *hist, icara
     // every load case: solve, compliance f.u, sensitivity; both summed over the
     // cases (reference opt_multi_load.cpp:118-146)
     const int ncase = prob_.num_load_cases();
+    if (ncase > 1 && o.solve.precond == ICARAT_HIP_PRECOND_MG && !mg_) mg_setup(mg_default_opts());
     // automatic (case_mode 0): together up to kCasesTogetherMaxElements elements,
     // where one launch over all cases beat one launch per case on the batched
     // workload's problems (1.1-2x up to 128x64x64; 0.9x at 96^3 and 160x80x80)

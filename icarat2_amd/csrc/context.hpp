@@ -282,6 +282,8 @@ class Context {
   void mc_drop_graph();
   // the device part of solve_cases; per-case stats without true residuals
   void mc_solve(const icarat_solve_opts &o, icarat_solve_stats *st);
+  void mc_solve_mg(const icarat_solve_opts &o, icarat_solve_stats *st);
+  void mc_save_warm();  // each case's solution as its warm start
   // d_x, d_u <- case c of the stacked solution, d_f <- its load
   void mc_take_case(int c);
   void simp_cases_together(const icarat_solve_opts &so);

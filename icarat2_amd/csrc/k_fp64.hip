@@ -608,10 +608,11 @@ __global__ __launch_bounds__(NT) void k_stencil64z_mcz(
       xcd_slot(T_, nt), T_ == 0 && threadIdx.x == 0, sp, se, sh);
 }
 
-// K.u (EPI 0) or the fine level's last sweep with its partials (EPI 1) for nc
-// load cases in one launch: case c's x, y, b, q at c x cs (D^-1 shared), its
-// two partials per block at [2 c nb, 2 (c+1) nb) in the layout of the
-// single-case launch.
+// K.u (EPI 0), the fine level's last sweep with its partials (EPI 1) or an
+// EPI 2 epilogue (a coarser level's sweep, or the W-cycle's second-visit
+// start) for nc load cases in one launch: case c's x, y, b, q, u, x0 at c x cs
+// (D^-1 shared), its two partials per block (EPI 1) at [2 c nb, 2 (c+1) nb) in
+// the layout of the single-case launch.
 template <int SYM, int EPI>
 __global__ __launch_bounds__(NT) void k_stencil64z_mcv(
     Lattice L, ZSplit Z, const double *__restrict__ c_KE0, KeVals kv,
@@ -628,6 +629,8 @@ __global__ __launch_bounds__(NT) void k_stencil64z_mcv(
   JacEpilogue jc = je;
   if (jc.b) jc.b += cs * c;
   if (jc.q) jc.q += cs * c;
+  if (jc.u) jc.u += cs * c;
+  if (jc.x0) jc.x0 += cs * c;
   stencil64z_work<false, SYM, false, EPI, double, double>(
       L, Z, T_, c_KE0, kv, young, fmask, x + cs * c, dinv, nullptr, nullptr, y + cs * c, nullptr,
       EPI == 1 ? partials + (size_t)2 * nt * c : nullptr, nullptr, 0u, 0, jc, nt, xcd_slot(T_, nt),
@@ -1371,24 +1374,32 @@ unsigned launch_pcg_stencil64_mcz(const Lattice &L, const KeCoef &ke, const doub
 unsigned launch_apply_k64_mc(const Lattice &L, const KeCoef &ke, const double *young,
                              const uint8_t *freemask, const double *x, double *y, const double *b,
                              const double *dinv, double omega, const double *q, double *partials,
-                             int nc, long long cs, hipStream_t s) {
+                             int nc, long long cs, hipStream_t s, double *u, double *x0) {
   if (ke.kernel != 0) throw Error(kInvalidArgument, "load cases together need the z-marching K.u");
   if (nc < 1) throw Error(kInternal, "no load cases");
-  const bool jac = b != nullptr;
-  if (jac && (!q || !partials || !dinv)) throw Error(kInternal, "apply_k64_mc: sweep needs q and partials");
+  const int epi = b == nullptr ? 0 : partials ? 1 : 2;
+  if (epi == 1 && (!q || !dinv)) throw Error(kInternal, "apply_k64_mc: the last sweep needs q");
+  if (epi == 2 && !dinv) throw Error(kInternal, "apply_k64_mc: the sweep needs dinv");
+  if ((u || x0) && (epi != 2 || !u || x == u || x == y || x == x0))
+    throw Error(kInternal, "apply_k64_mc: bad W-cycle residual buffers");
   JacEpilogue je;
   je.b = b;
-  je.q = q;
+  je.q = epi == 1 ? q : nullptr;
   je.omega = omega;
+  je.u = u;
+  je.x0 = x0;
   const ZSplit z_ = make_zsplit(L, ke);
   const dim3 g((unsigned)z_.nb * (unsigned)nc);
 #define MCV_LAUNCH(SYM_)                                                                           \
-  if (jac)                                                                                         \
+  if (epi == 1)                                                                                    \
     hipLaunchKernelGGL((k_stencil64z_mcv<SYM_, 1>), g, dim3(NT), 0, s, L, z_, ke.ke, ke.vals, young, \
                        freemask, x, dinv, y, partials, je, nc, cs);                               \
+  else if (epi == 2)                                                                               \
+    hipLaunchKernelGGL((k_stencil64z_mcv<SYM_, 2>), g, dim3(NT), 0, s, L, z_, ke.ke, ke.vals, young, \
+                       freemask, x, dinv, y, nullptr, je, nc, cs);                                \
   else                                                                                             \
     hipLaunchKernelGGL((k_stencil64z_mcv<SYM_, 0>), g, dim3(NT), 0, s, L, z_, ke.ke, ke.vals, young, \
-                       freemask, x, dinv, y, partials, je, nc, cs);
+                       freemask, x, dinv, y, nullptr, je, nc, cs);
   switch (ke.sym) {
     case 0: MCV_LAUNCH(0); break;
     case 1: MCV_LAUNCH(1); break;

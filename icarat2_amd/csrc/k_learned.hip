@@ -413,12 +413,20 @@ __global__ __launch_bounds__(NT) void k_fpcg_update_alpha(
   store_partials_at<NT, 1>(v, part_up, sh, gridDim.x, blockIdx.x);
 }
 
-// z.r and z.q partials (no finalize)
+// z.r and z.q partials (no finalize); load case blockIdx.y: vectors at case x
+// cs, its two partials per block at case x 2 gridDim.x
 __global__ __launch_bounds__(NT) void k_zdot_partials(long long n, const double *__restrict__ z,
                                                       const double *__restrict__ r,
                                                       const double *__restrict__ q,
-                                                      double *part_z) {
+                                                      double *part_z, long long cs) {
   __shared__ double sh[8];
+  {
+    const unsigned c = blockIdx.y;
+    z += c * cs;
+    r += c * cs;
+    q += c * cs;
+    part_z += (size_t)c * 2 * gridDim.x;
+  }
   double a = 0.0, b = 0.0;
   const long long stride = (long long)gridDim.x * NT;
   for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += stride) {
@@ -426,7 +434,7 @@ __global__ __launch_bounds__(NT) void k_zdot_partials(long long n, const double 
     b += z[i] * q[i];
   }
   double v[2] = {a, b};
-  store_partials<NT, 2>(v, part_z, sh);
+  store_partials_at<NT, 2>(v, part_z, sh, gridDim.x, blockIdx.x);
 }
 
 // One finalize per FPCG iteration: the stop tests of k_fpcg_check from the
@@ -597,8 +605,10 @@ unsigned launch_fpcg_update_alpha(long long n, double *x, double *r, const doubl
 }
 
 unsigned launch_zdot_partials(long long n, const double *z, const double *r, const double *q,
-                              double *part_z, hipStream_t s) {
-  hipLaunchKernelGGL(k_zdot_partials, dim3(VEC_BLOCKS), dim3(NT), 0, s, n, z, r, q, part_z);
+                              double *part_z, hipStream_t s, int nc, long long cs) {
+  if (nc < 1) throw Error(kInternal, "zdot partials: no load case");
+  hipLaunchKernelGGL(k_zdot_partials, dim3(VEC_BLOCKS, (unsigned)nc), dim3(NT), 0, s, n, z, r, q,
+                     part_z, cs);
   HIPCHK(hipGetLastError());
   return VEC_BLOCKS;
 }

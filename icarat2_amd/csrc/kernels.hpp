@@ -84,10 +84,13 @@ unsigned launch_apply_k64_jacobi(const Lattice &L, const KeCoef &ke, const doubl
                                  const double *q, double *partials, hipStream_t s);
 // Load cases of the multigrid solve in one launch (case c's vectors at c x cs,
 // its scalars S[c], its partials -- one or two per block -- at c x (one or two)
-// x the returned count, the layout of the single-case launch).  launch_pcg_stencil64_mcz: p_new = z + beta_c p_old,
-// q = K p_new and the p.q partials.  launch_apply_k64_mc: y = K x (b ==
-// nullptr), or the smoother sweep y = x + omega dinv (b - K x) with the y.b
-// and y.q partials.  Both need the z-marching kernel shape.
+// x the returned count, the layout of the single-case launch).
+// launch_pcg_stencil64_mcz: p_new = z + beta_c p_old, q = K p_new and the p.q
+// partials.  launch_apply_k64_mc: y = K x (b == nullptr); with partials, the
+// smoother sweep y = x + omega dinv (b - K x) with the y.b and y.q partials;
+// without, the sweep alone, or with u the W-cycle residual of
+// launch_apply_k64_wres (u = x, y = b - K x, x0 = omega dinv y).  Both need
+// the z-marching kernel shape.
 unsigned launch_pcg_stencil64_mcz(const Lattice &L, const KeCoef &ke, const double *young,
                                   const uint8_t *freemask, const double *z, const double *p_old,
                                   double *p_new, double *q, PcgScalars *S, double *partials,
@@ -95,7 +98,8 @@ unsigned launch_pcg_stencil64_mcz(const Lattice &L, const KeCoef &ke, const doub
 unsigned launch_apply_k64_mc(const Lattice &L, const KeCoef &ke, const double *young,
                              const uint8_t *freemask, const double *x, double *y, const double *b,
                              const double *dinv, double omega, const double *q, double *partials,
-                             int nc, long long cs, hipStream_t s);
+                             int nc, long long cs, hipStream_t s, double *u = nullptr,
+                             double *x0 = nullptr);
 // The start of a W-cycle's second visit of a z-marching level in one pass:
 // u = xc, r = b - K xc (masked) and, with x0, x0 = omega dinv r (xc may not
 // alias u, r or x0).  Needs the z-marching kernel shape.

@@ -58,9 +58,10 @@ unsigned launch_fpcg_update_alpha(long long n, double *x, double *r, const doubl
                                   const double *q, PcgScalars *S, const double *part_st,
                                   unsigned Gs, double *part_up, const double *dinv, double omega,
                                   float *z0, hipStream_t s, double *xlo = nullptr);
-// z.r and z.q partials; returns their count
+// z.r and z.q partials; returns their count per quantity (nc load cases:
+// vectors stacked with stride cs, case c's partials at 2 c x the count)
 unsigned launch_zdot_partials(long long n, const double *z, const double *r, const double *q,
-                              double *part_z, hipStream_t s);
+                              double *part_z, hipStream_t s, int nc = 1, long long cs = 0);
 // the next beta step restarts (beta = 0) unless the solve is done
 void launch_fpcg_restart_flag(PcgScalars *S, hipStream_t s);
 // stop tests from the r.r partials, then rho' = z.r and beta (Polak-Ribiere);

@@ -184,6 +184,7 @@ void Multigrid::drop_graphs() {
       exec_[m][p] = nullptr;
       graph_[m][p] = nullptr;
     }
+  mc_drop_graphs();
 }
 
 Context &Multigrid::level(int l) { return l == 0 ? f_ : *lv_[l - 1]; }

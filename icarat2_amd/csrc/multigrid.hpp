@@ -7,6 +7,7 @@
 // verified stop.
 #pragma once
 #include <algorithm>
+#include <map>
 #include <memory>
 #include <vector>
 
@@ -122,6 +123,26 @@ class Multigrid {
   long long factorizations() const { return n_fact_; }
   long long shifted() const { return n_shifted_; }
 
+  // ---- the load cases of one problem in one solve ----
+  // Whether solve_cases can run this hierarchy's cases together: FP64, the
+  // z-marching K.u on the fine level and on every level not smoothed by the
+  // direct kernel, no Galerkin operators, an even batch, no overlapped
+  // factorization, and one cycle shape for the whole solve (the W-cycle, if
+  // any, runs to the end: w_switch 0).
+  bool cases_capable(const icarat_solve_opts &o) const;
+  // K u_c = b_c for the nc right-hand sides stacked at B (case c at c x the
+  // fine lattice's vector length) into X; guess[c] != 0: X holds case c's
+  // starting guess (else it starts from 0).  Every case runs the arithmetic
+  // of its own solve -- flexible PCG with the V- or W-cycle, the verified FP64
+  // stop, the endgame target and the stall guard -- while each kernel of a
+  // batch is launched once for the running cases (the operator's Young's
+  // moduli and masks read once per launch), so each case's iterate equals
+  // that of its own solve bit for bit.  A case whose W-cycle broke down (z.r
+  // <= 0, which its own solve would resume with the V-cycle) is left to the
+  // caller: redo[c] = 1, solve it on its own.
+  void solve_cases(const icarat_solve_opts &o, int nc, const double *B, double *X,
+                   const char *guess, icarat_solve_stats *st, std::vector<char> &redo);
+
  private:
   Context &f_;
   icarat_mg_opts o_;
@@ -214,6 +235,30 @@ class Multigrid {
   // only follows the buffers.
   double *cycle(int l, const double *b, bool emit, bool x0_ready = false, ZDot *zd = nullptr);
   void enqueue_restart();                      // restart flag, z = M(r), beta = 0
+  // Load cases together: per level the stacked cycle vectors (case c at c x
+  // the level's vector length; level 0: x, y, r only), the fine PCG vectors
+  // and scalars per case, the partials of nc cases, and the captured batches
+  // per range of running cases.
+  struct Cases {
+    int nc = 0;
+    double *X = nullptr;  // the iterates the captured batches were built with
+    std::vector<DevBuf<double>> b, x, y, r, u, nv;
+    DevBuf<double> R, P[2], Q, xlo, xbest, yt;
+    DevBuf<PcgScalars> S;
+    PinnedBuf<PcgScalars> hS;
+    DevBuf<double> pst, pup, pz, dpart;
+    double *zres = nullptr;  // base of the fine cycle's result
+    std::map<int, hipGraph_t> graph;  // key: first case x 256 + case count
+    std::map<int, hipGraphExec_t> exec;
+  };
+  Cases mc_;
+  void mc_alloc(int nc);
+  void mc_drop_graphs();
+  // the cycle of cycle() for cases [c0, c0 + ncs); b and the result are the
+  // bases of level l's stacked vectors
+  double *cycle_mc(int l, const double *b, bool x0_ready, ZDot *zd, int c0, int ncs);
+  void mc_enqueue_batch(int lo, int cnt);  // one batch of cases [lo, lo + cnt)
+  void mc_restart(int lo, int cnt);        // restart flags, z = M(r), beta = 0
   // whether level l's V-cycle starts with the sweep x = omega D^-1 b (a
   // smoothed coarsest level, or any other level with pre-smoothing)
   bool starts_swept(int l) const;
