@@ -4,6 +4,8 @@
 #include <algorithm>
 #include <chrono>
 
+#include "multigrid.hpp"
+
 namespace icarat_hip {
 
 namespace {
@@ -43,22 +45,30 @@ void Batch::launch_running(const std::vector<char> &run) {
     ev_.push_back(e);
   }
   // graphs are captured before any stream is made to wait
-  for (int i = 0; i < n; ++i)
-    if (run[i]) ctx_[i]->build_solver_graph();
+  for (int i = 0; i < n; ++i) {
+    if (run[i] == kSingle) ctx_[i]->build_solver_graph();
+    if (run[i] == kCases) ctx_[i]->cases_build_graph();
+  }
   HIPCHK(hipEventRecord(fork_, ms_));
   for (int i = 0; i < n; ++i) {
     if (!run[i]) continue;
     Context &c = *ctx_[i];
     HIPCHK(hipStreamWaitEvent(c.stream(), fork_, 0));
-    c.launch_solver_batch(c.h_pcg.p->iter & 1);
-    HIPCHK(hipMemcpyAsync(c.h_pcg.p, c.d_pcg.p, sizeof(PcgScalars), hipMemcpyDeviceToHost, c.stream()));
+    if (run[i] == kCases) {
+      c.cases_launch();  // with the cases' scalars to the host
+    } else {
+      c.launch_solver_batch(c.h_pcg.p->iter & 1);
+      HIPCHK(hipMemcpyAsync(c.h_pcg.p, c.d_pcg.p, sizeof(PcgScalars), hipMemcpyDeviceToHost,
+                            c.stream()));
+    }
     HIPCHK(hipEventRecord(ev_[i], c.stream()));
     HIPCHK(hipStreamWaitEvent(ms_, ev_[i], 0));
   }
 }
 
-void Batch::engine(double tol, int maxit, const std::function<bool(int)> &start,
-                   const std::function<void(int, const icarat_solve_stats &)> &done) {
+void Batch::engine(double tol, int maxit, const std::function<int(int)> &start,
+                   const std::function<void(int, const icarat_solve_stats &)> &done,
+                   const std::function<void(int)> &done_cases) {
   const int n = size();
   std::vector<char> run(n, 0), admitted(n, 0), over(n, 0);
   // admission order: largest problem first
@@ -70,7 +80,8 @@ void Batch::engine(double tol, int maxit, const std::function<bool(int)> &start,
   long long load = 0;
   auto begin = [&](int i) {
     ctx_[i]->sync();
-    run[i] = start(i) ? 1 : 0;
+    run[i] = (char)start(i);
+    if (run[i] == kCases && !done_cases) throw Error(kInternal, "batch engine: no cases handler");
     if (!run[i] && admitted[i] && !over[i]) {
       over[i] = 1;
       load -= ctx_[i]->lattice().nelx;
@@ -94,6 +105,11 @@ void Batch::engine(double tol, int maxit, const std::function<bool(int)> &start,
     std::vector<int> ver, fin, rec;
     for (int i = 0; i < n; ++i) {
       if (!run[i]) continue;
+      if (run[i] == kCases) {
+        // the cases' own verifications (on the problem's stream)
+        if (ctx_[i]->cases_poll()) fin.push_back(i);
+        continue;
+      }
       const PcgScalars &h = *ctx_[i]->h_pcg.p;
       if (h.done && h.status == 0) ver.push_back(i);
       else if (h.done && ctx_[i]->pcg_recover(ms_)) rec.push_back(i);  // W-cycle -> V-cycle
@@ -109,10 +125,15 @@ void Batch::engine(double tol, int maxit, const std::function<bool(int)> &start,
       if (ctx_[i]->pcg_after_verify()) fin.push_back(i);
     // finished solves: hand over, then the problem's next solve (if any)
     for (int i : fin) {
-      icarat_solve_stats st{};
-      ctx_[i]->pcg_finish(st);
-      run[i] = 0;
-      done(i, st);
+      if (run[i] == kCases) {
+        run[i] = 0;
+        done_cases(i);
+      } else {
+        icarat_solve_stats st{};
+        ctx_[i]->pcg_finish(st);
+        run[i] = 0;
+        done(i, st);
+      }
       begin(i);
     }
     admit();  // problems that finished their loop free budget
@@ -124,8 +145,10 @@ void Batch::engine(double tol, int maxit, const std::function<bool(int)> &start,
     // a solve that has just started or verified may already be done (zero
     // right-hand side, or the true residual passed): handle it before launching
     bool pending = false;
-    for (int i = 0; i < n; ++i)
-      if (run[i] && ctx_[i]->h_pcg.p->done) pending = true;
+    for (int i = 0; i < n; ++i) {
+      if (run[i] == kSingle && ctx_[i]->h_pcg.p->done) pending = true;
+      if (run[i] == kCases && !ctx_[i]->cases_ready()) pending = true;  // not polled yet
+    }
     if (pending) continue;
     launch_running(run);
     ++launches_;
@@ -148,12 +171,12 @@ void Batch::solve(const icarat_solve_opts &o, icarat_solve_stats *st) {
   std::vector<double> t0(n, 0.0);
   engine(
       o.tol, o.maxit,
-      [&](int i) {
-        if (started[i]) return false;
+      [&](int i) -> int {
+        if (started[i]) return 0;
         started[i] = 1;
         t0[i] = now_s();
         ctx_[i]->pcg_start(o.tol, o.maxit, ms_, o.warm != 0);
-        return true;
+        return kSingle;
       },
       [&](int i, const icarat_solve_stats &s) {
         ctx_[i]->copy_out_solution(nullptr);
@@ -177,18 +200,35 @@ void Batch::simp_run(const icarat_simp_opts &o, double *const *x, double *const 
   // per problem: design step and load case of the solve in flight
   std::vector<int> step(n, 1), lc(n, -1);
   std::vector<double> t_solve(n, 0.0);
+  // multigrid problems with several load cases solve them together, as the
+  // single-problem loop's automatic choice does (same results either way)
+  auto together = [&](Context &c) {
+    if (c.problem().num_load_cases() < 2 || o.solve.precond != ICARAT_HIP_PRECOND_MG ||
+        o.case_mode == 1)
+      return false;
+    if (!c.mg_) c.mg_setup(mg_default_opts());
+    return c.cases_together(o.solve);
+  };
   engine(
       o.solve.tol, o.solve.maxit,
-      [&](int i) {
+      [&](int i) -> int {
         Context &c = *ctx_[i];
-        if (step[i] > o.n_iter || c.simp_.S.converged) return false;
+        if (step[i] > o.n_iter || c.simp_.S.converged) return 0;
         const int nc = c.problem().num_load_cases();
-        if (lc[i] < 0) c.simp_pre(step[i]);  // a new design iteration
+        if (lc[i] < 0) {
+          c.simp_pre(step[i]);  // a new design iteration
+          if (together(c)) {
+            lc[i] = nc - 1;
+            t_solve[i] = now_s();
+            c.cases_start(o.solve);
+            return kCases;
+          }
+        }
         ++lc[i];
         if (nc > 1) c.use_load_case(lc[i]);
         t_solve[i] = now_s();
         c.pcg_start(o.solve.tol, o.solve.maxit, ms_, o.solve.warm != 0);
-        return true;
+        return kSingle;
       },
       [&](int i, const icarat_solve_stats &s) {
         Context &c = *ctx_[i];
@@ -198,6 +238,17 @@ void Batch::simp_run(const icarat_simp_opts &o, double *const *x, double *const 
         const int nc = c.problem().num_load_cases();
         if (lc[i] + 1 < nc) return;  // the next case of this design iteration
         if (nc > 1) c.use_load_case(c.problem().load_case());
+        c.simp_update(step[i], hist ? hist[i] : nullptr);
+        ++step[i];
+        lc[i] = -1;
+      },
+      [&](int i) {
+        Context &c = *ctx_[i];
+        std::vector<icarat_solve_stats> S(c.problem().num_load_cases());
+        c.cases_finish(o.solve, S.data());
+        c.simp_.S.solve_s += now_s() - t_solve[i];
+        c.simp_cases_post(S.data());
+        c.use_load_case(c.problem().load_case());
         c.simp_update(step[i], hist ? hist[i] : nullptr);
         ++step[i];
         lc[i] = -1;

@@ -58,13 +58,19 @@ class Batch {
   // than one after another, against 1.2x with a 3M-element budget and 1.1x
   // with 1.5M
   long long budget_ = 0;
+  // run[i]: 0 idle, kSingle a single solve, kCases the multigrid solve of
+  // all of problem i's load cases
+  static constexpr char kSingle = 1, kCases = 2;
   void launch_running(const std::vector<char> &run);
-  // The solve engine: start(i) begins problem i's next solve (pcg_start on the
-  // batch stream) and returns false when problem i has no more solves; done(i,
-  // st) receives a finished solve.  Iterates every running solve in shared
-  // graph launches until no problem starts another solve.
-  void engine(double tol, int maxit, const std::function<bool(int)> &start,
-              const std::function<void(int, const icarat_solve_stats &)> &done);
+  // The solve engine: start(i) begins problem i's next solve and returns its
+  // kind -- kSingle (pcg_start on the batch stream), kCases (cases_start), or
+  // 0 when problem i has no more solves; done(i, st) receives a finished
+  // single solve, done_cases(i) a finished cases' solve (cases_finish is
+  // left to it).  Iterates every running solve in shared launches until no
+  // problem starts another solve.
+  void engine(double tol, int maxit, const std::function<int(int)> &start,
+              const std::function<void(int, const icarat_solve_stats &)> &done,
+              const std::function<void(int)> &done_cases = {});
   void apply_config(int cfg);
   void set_precond(const icarat_solve_opts &o);
 };

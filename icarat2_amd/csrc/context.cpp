@@ -641,27 +641,38 @@ void Context::mc_enqueue_batch() {
                     nullptr, stream_);
 }
 
-void Context::mc_solve(const icarat_solve_opts &o, icarat_solve_stats *st) {
+void Context::mc_prepare() {
   const int nc = prob_.num_load_cases();
   mc_alloc(nc);
-  const long long n3 = n3_;
-  auto vec = [&](DevBuf<double> &b, int c) { return b.p + (size_t)c * n3; };
   // right-hand sides: each case's fext minus the load of the prescribed displacements
   const bool dir = prob_.has_nonzero_dirichlet();
   if (dir) launch_apply_k64(L_, kc_, d_young, d_free, d_dirlat, d_y, stream_);
   for (int c = 0; c < nc; ++c) {
-    d2d(vec(mc_.B, c), d_fcases_[c].p, n3, stream_);
-    if (dir) launch_axpby(n3, -1.0, d_y, 1.0, vec(mc_.B, c), stream_);
+    double *b = mc_.B.p + (size_t)c * n3_;
+    d2d(b, d_fcases_[c].p, n3_, stream_);
+    if (dir) launch_axpby(n3_, -1.0, d_y, 1.0, b, stream_);
   }
+}
+
+void Context::mc_solve(const icarat_solve_opts &o, icarat_solve_stats *st) {
+  if (o.precond == ICARAT_HIP_PRECOND_MG) {
+    cases_start(o);
+    while (!cases_poll()) {
+      cases_launch();
+      sync();
+    }
+    cases_finish(o, st);
+    return;
+  }
+  mc_prepare();
+  const int nc = mc_.nc;
+  const long long n3 = n3_;
+  auto vec = [&](DevBuf<double> &b, int c) { return b.p + (size_t)c * n3; };
   for (int c = 0; c < nc; ++c) {
     PcgScalars init{};
     init.tol = o.tol;
     init.maxit = o.maxit;
     mc_.hS.p[c] = init;
-  }
-  if (o.precond == ICARAT_HIP_PRECOND_MG) {
-    mc_solve_mg(o, st);
-    return;
   }
   h2d(mc_.S.p, mc_.hS.p, nc, stream_);
   for (int c = 0; c < nc; ++c) {
@@ -758,19 +769,32 @@ void Context::mc_save_warm() {
   }
 }
 
-// The multigrid solve of every case together (right-hand sides in mc_.B): a
-// case whose W-cycle broke down is solved on its own afterwards, as its own
-// solve would have gone on.
-void Context::mc_solve_mg(const icarat_solve_opts &o, icarat_solve_stats *st) {
+// The multigrid solve of every case together, in pieces (the batched engine
+// interleaves them with other problems' launches): a case whose W-cycle broke
+// down is solved on its own at the end, as its own solve would have gone on.
+void Context::cases_start(const icarat_solve_opts &o) {
+  if (!mg_) mg_setup(mg_default_opts());
+  mc_prepare();
   const int nc = mc_.nc;
-  std::vector<char> guess(nc, 0), redo;
+  std::vector<char> guess(nc, 0);
   for (int c = 0; c < nc; ++c)
     if (o.warm && c < (int)xcase_valid_.size() && xcase_valid_[c]) {
       d2d(mc_.X.p + (size_t)c * n3_, d_xcases_[c].p, n3_, stream_);
       guess[c] = 1;
     }
+  mg_->mc_begin(o, nc, mc_.B, mc_.X, guess.data());
+}
+
+bool Context::cases_poll() { return mg_->mc_poll(); }
+bool Context::cases_ready() const { return mg_ && mg_->mc_ready(); }
+void Context::cases_build_graph() { mg_->mc_build_graph(); }
+void Context::cases_launch() { mg_->mc_launch(); }
+
+void Context::cases_finish(const icarat_solve_opts &o, icarat_solve_stats *st) {
+  const int nc = mc_.nc;
   std::vector<icarat_solve_stats> S(nc);
-  mg_->solve_cases(o, nc, mc_.B, mc_.X, guess.data(), S.data(), redo);
+  std::vector<char> redo;
+  mg_->mc_end(S.data(), redo);
   for (int c = 0; c < nc; ++c) {
     if (!redo[c]) continue;
     use_load_case(c);
@@ -834,9 +858,16 @@ void Context::solve_cases(const icarat_solve_opts &o, double *u_host, icarat_sol
 void Context::simp_cases_together(const icarat_solve_opts &so) {
   const int nc = prob_.num_load_cases();
   std::vector<icarat_solve_stats> S(nc);
+  const double t0 = now_s();
+  mc_solve(so, S.data());
+  simp_.S.solve_s += now_s() - t0;
+  simp_cases_post(S.data());
+}
+
+void Context::simp_cases_post(icarat_solve_stats *S) {
+  const int nc = prob_.num_load_cases();
   for (int lc = 0; lc < nc; ++lc) {
     const double t1 = now_s();
-    if (lc == 0) mc_solve(so, S.data());
     mc_take_case(lc);
     S[lc].true_relres = true_residual_device();
     solved_ = true;
@@ -1308,11 +1339,9 @@ void Context::simp_run(const icarat_simp_opts &o, double *x, double *hist, icara
     // cases (reference opt_multi_load.cpp:118-146)
     const int ncase = prob_.num_load_cases();
     if (ncase > 1 && o.solve.precond == ICARAT_HIP_PRECOND_MG && !mg_) mg_setup(mg_default_opts());
-    // automatic (case_mode 0): together up to kCasesTogetherMaxElements elements,
-    // where one launch over all cases beat one launch per case on the batched
-    // workload's problems (1.1-2x up to 128x64x64; 0.9x at 96^3 and 160x80x80)
+    // automatic (case_mode 0): cases_auto
     const bool together = ncase > 1 && cases_together(o.solve) &&
-                          (o.case_mode == 2 || (o.case_mode == 0 && L_.nelx <= kCasesTogetherMaxElements));
+                          (o.case_mode == 2 || (o.case_mode == 0 && cases_auto(o.solve)));
     if (together) {
       // one multi-column iteration over the cases (the same arithmetic per case)
       simp_cases_together(o.solve);

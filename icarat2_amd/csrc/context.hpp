@@ -58,15 +58,39 @@ class Context {
   // FP64 / mixed solve of K u = fext; result kept on device (d_u, nodal with
   // Dirichlet values); optional numeq copy to host.
   void solve(const icarat_solve_opts &o, double *u_host, icarat_solve_stats *st);
-  // Every load case solved together (FP64 Jacobi-PCG): one multi-column
-  // iteration, each case with its own scalars and verified stop, the same
-  // arithmetic as a solve of that case alone.  u_host: numeq values per case
-  // (may be NULL); st[ncase] (may be NULL).  Afterwards d_u, d_x hold the
-  // active case's solution.
+  // Every load case solved together (FP64 Jacobi-PCG or multigrid PCG): one
+  // multi-column iteration, each case with its own scalars and verified stop,
+  // the same arithmetic as a solve of that case alone.  u_host: numeq values
+  // per case (may be NULL); st[ncase] (may be NULL).  Afterwards d_u, d_x hold
+  // the active case's solution.
   void solve_cases(const icarat_solve_opts &o, double *u_host, icarat_solve_stats *st);
-  // o can be solved by one multi-column iteration (FP64 Jacobi, z-marching K.u)
+  // o can be solved by one multi-column iteration (FP64 Jacobi or multigrid,
+  // z-marching K.u; multigrid: Multigrid::cases_capable)
   bool cases_together(const icarat_solve_opts &o) const;
+  // The design loop's automatic choice (case_mode 0) runs the cases together
+  // with Jacobi-PCG up to this many elements (one launch over all cases beat
+  // one launch per case on the batched workload's problems, 1.1-2x up to
+  // 128x64x64, 0.9x at 96^3 and 160x80x80) and with multigrid PCG at every
+  // size (1.3-2.0x from 32^3 to 160x80x80).
   static constexpr long long kCasesTogetherMaxElements = 600000;
+  bool cases_auto(const icarat_solve_opts &o) const {
+    return o.precond == ICARAT_HIP_PRECOND_MG || L_.nelx <= kCasesTogetherMaxElements;
+  }
+  // The multigrid solve of every case in pieces (Multigrid::mc_begin and
+  // the following on this context's stream), for the batched engine:
+  // cases_start (the stream synchronized on return); after each launch has
+  // completed, cases_poll (true once every case is finished); else
+  // cases_build_graph and cases_launch; then cases_finish (per-case stats, a
+  // case left over by the W-cycle solved on its own, warm starts saved).
+  void cases_start(const icarat_solve_opts &o);
+  bool cases_poll();
+  bool cases_ready() const;  // polled since the start, a batch to launch
+  void cases_build_graph();
+  void cases_launch();
+  void cases_finish(const icarat_solve_opts &o, icarat_solve_stats *st);
+  // after the cases' solve: each case's true residual and its compliance and
+  // sensitivity terms, in case order
+  void simp_cases_post(icarat_solve_stats *S);
   double true_residual_device();          // ||f - K d_u|| / ||f||
   double true_residual_host(const double *u);
   void compliance_sensitivity(double vnorm, double *c, double *dfds, double *dgds);
@@ -282,7 +306,7 @@ class Context {
   void mc_drop_graph();
   // the device part of solve_cases; per-case stats without true residuals
   void mc_solve(const icarat_solve_opts &o, icarat_solve_stats *st);
-  void mc_solve_mg(const icarat_solve_opts &o, icarat_solve_stats *st);
+  void mc_prepare();    // stacked buffers for the cases, right-hand sides in mc_.B
   void mc_save_warm();  // each case's solution as its warm start
   // d_x, d_u <- case c of the stacked solution, d_f <- its load
   void mc_take_case(int c);

@@ -142,6 +142,21 @@ class Multigrid {
   // caller: redo[c] = 1, solve it on its own.
   void solve_cases(const icarat_solve_opts &o, int nc, const double *B, double *X,
                    const char *guess, icarat_solve_stats *st, std::vector<char> &redo);
+  // solve_cases in pieces, for a caller that interleaves it with other work
+  // (every launch on the fine context's stream): mc_begin sets the solve up
+  // (returns with the stream synchronized); mc_poll, after a launch's scalars
+  // reached the host (the stream synchronized), verifies the cases whose
+  // batch ended and returns true once every case is finished; else
+  // mc_build_graph captures the next batch (if graphs are on and it is new)
+  // and mc_launch enqueues it with the scalars' copy to the host; mc_end ends
+  // the solve (stats and redo as solve_cases).
+  void mc_begin(const icarat_solve_opts &o, int nc, const double *B, double *X,
+                const char *guess);
+  bool mc_poll();
+  bool mc_ready() const { return mc_.cnt > 0; }  // mc_poll has chosen the next batch
+  void mc_build_graph();
+  void mc_launch();
+  void mc_end(icarat_solve_stats *st, std::vector<char> &redo);
 
  private:
   Context &f_;
@@ -242,6 +257,7 @@ class Multigrid {
   struct Cases {
     int nc = 0;
     double *X = nullptr;  // the iterates the captured batches were built with
+    const double *B = nullptr;
     std::vector<DevBuf<double>> b, x, y, r, u, nv;
     DevBuf<double> R, P[2], Q, xlo, xbest, yt;
     DevBuf<PcgScalars> S;
@@ -250,6 +266,14 @@ class Multigrid {
     double *zres = nullptr;  // base of the fine cycle's result
     std::map<int, hipGraph_t> graph;  // key: first case x 256 + case count
     std::map<int, hipGraphExec_t> exec;
+    // the running solve: per case 0 running, 1 finished, 2 left to the caller;
+    // verifications, the stall guard's counters and best residual
+    std::vector<int> state, repl, since, verifs;
+    std::vector<double> best;
+    std::vector<char> redo;
+    long long batches = 0, max_batches = 0;
+    int lo = 0, cnt = 0;  // the range of the next batch
+    int wsave = 0;        // wcur_ outside the solve
   };
   Cases mc_;
   void mc_alloc(int nc);

@@ -239,8 +239,8 @@ void Multigrid::mc_restart(int lo, int cnt) {
     launch_zdot_beta(n, z + k * n, m.R.p + k * n, m.Q.p + k * n, m.S.p + k, c.d_partials, s);
 }
 
-void Multigrid::solve_cases(const icarat_solve_opts &o, int nc, const double *B, double *X,
-                            const char *guess, icarat_solve_stats *st, std::vector<char> &redo) {
+void Multigrid::mc_begin(const icarat_solve_opts &o, int nc, const double *B, double *X,
+                         const char *guess) {
   if (!cases_capable(o)) throw Error(kInternal, "multigrid cases: unsupported configuration");
   if (nc < 1 || nc >= kRangeKey) throw Error(kInvalidArgument, "load case count out of range");
   Context &c = f_;
@@ -253,7 +253,8 @@ void Multigrid::solve_cases(const icarat_solve_opts &o, int nc, const double *B,
     mc_drop_graphs();
     m.X = X;
   }
-  const int wsave = wcur_;
+  m.B = B;
+  m.wsave = wcur_;
   wcur_ = wlev_;  // the first phase's cycle for the whole solve (w_switch 0)
   tol_ = o.tol;
   const long long n = c.n3_;
@@ -284,12 +285,25 @@ void Multigrid::solve_cases(const icarat_solve_opts &o, int nc, const double *B,
   HIPCHK(hipMemcpyAsync(m.hS.p, m.S.p, sizeof(PcgScalars) * nc, hipMemcpyDeviceToHost, s));
   c.sync();
   // 0 running, 1 finished, 2 left to the caller
-  std::vector<int> state(nc, 0), repl(nc, 0), since(nc, 0), verifs(nc, 0);
-  std::vector<double> best(nc, INFINITY);
-  redo.assign(nc, 0);
-  const long long max_batches =
-      ((long long)o.maxit / batch_iters() + 3) * (kMaxVerifications + 2) + 16;
-  long long batches = 0;
+  m.state.assign(nc, 0);
+  m.repl.assign(nc, 0);
+  m.since.assign(nc, 0);
+  m.verifs.assign(nc, 0);
+  m.best.assign(nc, INFINITY);
+  m.redo.assign(nc, 0);
+  m.max_batches = ((long long)o.maxit / batch_iters() + 3) * (kMaxVerifications + 2) + 16;
+  m.batches = 0;
+  m.lo = 0;
+  m.cnt = 0;
+}
+
+bool Multigrid::mc_poll() {
+  Cases &m = mc_;
+  Context &c = f_;
+  hipStream_t s = c.stream();
+  const int nc = m.nc;
+  const long long n = c.n3_;
+  const Lattice &L = c.lattice();
   auto put = [&](int k) {
     HIPCHK(hipMemcpyAsync(m.S.p + k, m.hS.p + k, sizeof(PcgScalars), hipMemcpyHostToDevice, s));
   };
@@ -297,32 +311,32 @@ void Multigrid::solve_cases(const icarat_solve_opts &o, int nc, const double *B,
     // cases whose batch ended: the FP64 true residual decides (verified stop)
     std::vector<int> ver;
     for (int k = 0; k < nc; ++k) {
-      if (state[k] != 0) continue;
+      if (m.state[k] != 0) continue;
       PcgScalars &h = m.hS.p[k];
       if (!h.done) continue;
       if (h.status == 4 && wcur_ > 0) {
         // the W-cycle broke down: its own solve goes on with the V-cycle
-        state[k] = 2;
-        redo[k] = 1;
+        m.state[k] = 2;
+        m.redo[k] = 1;
         continue;
       }
       if (h.status != 0) {
-        state[k] = 1;  // iteration cap or breakdown
+        m.state[k] = 1;  // iteration cap or breakdown
         continue;
       }
-      double *xk = X + k * n, *yk = m.yt.p + k * n;
+      double *xk = m.X + k * n, *yk = m.yt.p + k * n;
       launch_apply_k64(L, c.kc_, c.d_young, c.d_free, xk, yk, s);
-      launch_replace64(n, B + k * n, yk, m.R.p + k * n, c.d_dinv, m.S.p + k, c.d_partials, s);
+      launch_replace64(n, m.B + k * n, yk, m.R.p + k * n, c.d_dinv, m.S.p + k, c.d_partials, s);
       mc_restart(k, 1);
       HIPCHK(hipMemcpyAsync(m.hS.p + k, m.S.p + k, sizeof(PcgScalars), hipMemcpyDeviceToHost, s));
       ver.push_back(k);
     }
     if (!ver.empty()) c.sync();
     for (int k : ver) {
-      ++repl[k];
+      ++m.repl[k];
       PcgScalars &h = m.hS.p[k];
       if (h.done) {
-        state[k] = 1;  // the verified residual passes the recursive target
+        m.state[k] = 1;  // the verified residual passes the recursive target
         continue;
       }
       const double bnorm = std::sqrt(h.bnorm2);
@@ -332,19 +346,19 @@ void Multigrid::solve_cases(const icarat_solve_opts &o, int nc, const double *B,
         h.done = 1;
         h.status = 0;
         put(k);
-        state[k] = 1;
+        m.state[k] = 1;
         continue;
       }
-      ++verifs[k];
-      if (tr < best[k]) {
-        best[k] = tr;
-        since[k] = 0;
+      ++m.verifs[k];
+      if (tr < m.best[k]) {
+        m.best[k] = tr;
+        m.since[k] = 0;
         if (!m.xbest.p) m.xbest.alloc((size_t)nc * n);
-        launch_copy64(n, X + k * n, m.xbest.p + k * n, s);
+        launch_copy64(n, m.X + k * n, m.xbest.p + k * n, s);
       } else {
-        ++since[k];
+        ++m.since[k];
       }
-      if (since[k] < kStallVerifications && verifs[k] <= kMaxVerifications) {
+      if (m.since[k] < kStallVerifications && m.verifs[k] <= kMaxVerifications) {
         // the endgame target of after_verify()
         const double t0 = h.tol;
         const double g2 = std::max(0.0, tr * tr - t0 * t0);
@@ -357,73 +371,106 @@ void Multigrid::solve_cases(const icarat_solve_opts &o, int nc, const double *B,
         continue;
       }
       // stalled at the floor: the best verified iterate, not converged
-      launch_copy64(n, m.xbest.p + k * n, X + k * n, s);
+      launch_copy64(n, m.xbest.p + k * n, m.X + k * n, s);
       if (m.xlo.p) HIPCHK(hipMemsetAsync(m.xlo.p + k * n, 0, sizeof(double) * n, s));
       h.done = 1;
       h.status = 1;
       put(k);
-      state[k] = 1;
+      m.state[k] = 1;
     }
     int lo = nc, hi = -1;
     for (int k = 0; k < nc; ++k)
-      if (state[k] == 0) {
+      if (m.state[k] == 0) {
         lo = std::min(lo, k);
         hi = std::max(hi, k);
       }
-    if (hi < 0) break;
+    if (hi < 0) return true;
     // a case that has just verified may be done already: handle it first
     bool pending = false;
     for (int k = lo; k <= hi; ++k)
-      if (state[k] == 0 && m.hS.p[k].done) pending = true;
+      if (m.state[k] == 0 && m.hS.p[k].done) pending = true;
     if (pending) continue;
-    if (++batches > max_batches) throw Error(kInternal, "multigrid PCG did not reach its stop test");
-    // one batch over the running range (cases in it that are done are skipped
-    // by the PCG kernels; their cycle work is discarded)
-    const int cnt = hi - lo + 1;
-    if (c.use_graphs_) {
-      const int key = lo * kRangeKey + cnt;
-      auto it = m.exec.find(key);
-      if (it == m.exec.end()) {
-        hipGraph_t g = nullptr;
-        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        try {
-          mc_enqueue_batch(lo, cnt);
-        } catch (...) {
-          hipGraph_t gg = nullptr;
-          (void)hipStreamEndCapture(s, &gg);
-          if (gg) (void)hipGraphDestroy(gg);
-          wcur_ = wsave;
-          throw;
-        }
-        HIPCHK(hipStreamEndCapture(s, &g));
-        hipGraphExec_t e = nullptr;
-        HIPCHK(hipGraphInstantiate(&e, g, nullptr, nullptr, 0));
-        m.graph[key] = g;
-        it = m.exec.emplace(key, e).first;
-      }
-      HIPCHK(hipGraphLaunch(it->second, s));
-    } else {
-      mc_enqueue_batch(lo, cnt);
-    }
-    HIPCHK(hipMemcpyAsync(m.hS.p + lo, m.S.p + lo, sizeof(PcgScalars) * cnt, hipMemcpyDeviceToHost,
-                          s));
-    c.sync();
+    // the next batch runs over the range of running cases (cases in it that
+    // are done are skipped by the PCG kernels; their cycle work is discarded)
+    m.lo = lo;
+    m.cnt = hi - lo + 1;
+    return false;
   }
-  wcur_ = wsave;
+}
+
+void Multigrid::mc_build_graph() {
+  Cases &m = mc_;
+  Context &c = f_;
+  if (!c.use_graphs_ || m.cnt < 1) return;
+  const int key = m.lo * kRangeKey + m.cnt;
+  if (m.exec.count(key)) return;
+  hipStream_t s = c.stream();
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  try {
+    mc_enqueue_batch(m.lo, m.cnt);
+  } catch (...) {
+    hipGraph_t gg = nullptr;
+    (void)hipStreamEndCapture(s, &gg);
+    if (gg) (void)hipGraphDestroy(gg);
+    throw;
+  }
+  HIPCHK(hipStreamEndCapture(s, &g));
+  hipGraphExec_t e = nullptr;
+  HIPCHK(hipGraphInstantiate(&e, g, nullptr, nullptr, 0));
+  m.graph[key] = g;
+  m.exec[key] = e;
+}
+
+void Multigrid::mc_launch() {
+  Cases &m = mc_;
+  Context &c = f_;
+  hipStream_t s = c.stream();
+  if (m.cnt < 1) throw Error(kInternal, "multigrid cases: no running case to launch");
+  if (++m.batches > m.max_batches) throw Error(kInternal, "multigrid PCG did not reach its stop test");
+  if (c.use_graphs_) {
+    mc_build_graph();
+    HIPCHK(hipGraphLaunch(m.exec.at(m.lo * kRangeKey + m.cnt), s));
+  } else {
+    mc_enqueue_batch(m.lo, m.cnt);
+  }
+  HIPCHK(hipMemcpyAsync(m.hS.p + m.lo, m.S.p + m.lo, sizeof(PcgScalars) * m.cnt,
+                        hipMemcpyDeviceToHost, s));
+}
+
+void Multigrid::mc_end(icarat_solve_stats *st, std::vector<char> &redo) {
+  Cases &m = mc_;
+  wcur_ = m.wsave;
   finish();
-  for (int k = 0; k < nc; ++k) {
-    if (redo[k]) continue;
+  redo = m.redo;
+  for (int k = 0; k < m.nc; ++k) {
+    if (m.redo[k]) continue;
     const PcgScalars &h = m.hS.p[k];
     if (h.done && (h.status == 2 || h.status == 4))
       throw Error(kNumerical, "multigrid PCG breakdown (the V-cycle is not positive definite here)");
     icarat_solve_stats S{};
     S.iters = h.iter;
-    S.refine_steps = repl[k];
+    S.refine_steps = m.repl[k];
     const double bnorm = std::sqrt(h.bnorm2);
     S.rec_relres = bnorm > 0.0 ? std::sqrt(h.rr) / bnorm : 0.0;
     S.status = (h.done && h.status == 0) || bnorm == 0.0 ? ICARAT_HIP_OK : ICARAT_HIP_NOT_CONVERGED;
     if (st) st[k] = S;
   }
+}
+
+void Multigrid::solve_cases(const icarat_solve_opts &o, int nc, const double *B, double *X,
+                            const char *guess, icarat_solve_stats *st, std::vector<char> &redo) {
+  mc_begin(o, nc, B, X, guess);
+  try {
+    while (!mc_poll()) {
+      mc_launch();
+      f_.sync();
+    }
+  } catch (...) {
+    wcur_ = mc_.wsave;
+    throw;
+  }
+  mc_end(st, redo);
 }
 
 }  // namespace icarat_hip

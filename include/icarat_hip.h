@@ -178,10 +178,12 @@ int icarat_hip_solve(icarat_hip_ctx *ctx, const icarat_solve_opts *opts, double 
                      icarat_solve_stats *stats);
 /* Every load case of the context solved (the loop of
  * sample/software/opt-multi-load/opt_multi_load.cpp:118-146, which solves each
- * load case in turn).  With FP64 Jacobi-PCG (and the z-marching K.u) all cases
- * run as one multi-column iteration: each launch applies K to every case's
- * search direction, each case keeps its own scalars and verified stop, and
- * each case's arithmetic is that of a solve of the case alone, so the results
+ * load case in turn).  With FP64 Jacobi-PCG or FP64 multigrid PCG (and the
+ * z-marching K.u; for multigrid also no Galerkin coarse operators, an even
+ * batch_iters, w_switch 0) all cases run as one multi-column iteration: each
+ * launch applies K (or a cycle's transfer, sweep or dense solve) to every
+ * case, each case keeps its own scalars, verified stop and endgame, and each
+ * case's arithmetic is that of a solve of the case alone, so the results
  * equal icarat_hip_solve per case bit for bit.  Other preconditioners and
  * precisions solve the cases one after another.  u: ncase x numeq (may be
  * NULL); stats: ncase entries (may be NULL).  Afterwards the context holds the
@@ -298,10 +300,12 @@ typedef struct icarat_simp_opts {
   const char *history_csv; /* addPlot(step, compliance, history_csv, step) every iteration
                               (io/export_graph.hpp:56-70, opt_macro_oc.cpp:134); NULL: none */
   int32_t case_mode;       /* several load cases: 0 = automatic (solved together as in
-                              icarat_hip_solve_cases where the solve options allow it and the mesh
-                              has at most 600,000 elements, above which one launch per case was
-                              faster), 1 = one after another, 2 = together where the options allow
-                              it.  The results are the same in every mode */
+                              icarat_hip_solve_cases where the solve options allow it: multigrid
+                              PCG at every size, Jacobi-PCG up to 600,000 elements, above which
+                              one launch per case was faster), 1 = one after another, 2 =
+                              together where the options allow it.  The results are the same in
+                              every mode; icarat_hip_batch_simp_run follows the same choice for
+                              multigrid PCG and solves Jacobi-PCG cases one after another */
 } icarat_simp_opts;
 
 typedef struct icarat_simp_stats {
