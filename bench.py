@@ -278,7 +278,7 @@ def run_batch(A, a):
         "graph_launches": b.info()[1],
     }
     if not a.no_cases:
-        out["load_cases_together"] = cases_leg(A, specs, a.tol)
+        out["load_cases_together"] = cases_leg(A, specs, a.tol, pre)
     if pre != 0 and not a.no_plain:
         (xp, hp, sp), dtp = timed(0)
         out["plain_fp64"] = {
@@ -289,26 +289,28 @@ def run_batch(A, a):
     return out
 
 
-def cases_leg(A, specs, tol):
-    """One FP64 Jacobi-PCG solve of every load case of each multi-case problem
-    (initial design): all cases in one multi-column iteration
-    (icarat_hip_solve_cases) against one case after another; iteration counts
-    must agree."""
+def cases_leg(A, specs, tol, precond):
+    """One FP64 solve (Jacobi-PCG or multigrid PCG, the bench's method) of
+    every load case of each multi-case problem (initial design): all cases in
+    one multi-column iteration (icarat_hip_solve_cases) against one case
+    after another; iteration counts must agree."""
     from icarat2_amd import workloads as W
+    mg = precond == A.PRECOND_MG
     rows, t_all, t_seq, t_auto = [], 0.0, 0.0, 0.0
     for s in specs:
         if len(s["loads"]) < 2:
             continue
         c = W.build_context(A, s)
         c.set_density(c.density_filter(W.initial_design(s)))
-        c.solve_cases(tol=tol, maxit=100000, want_u=False)  # graphs built
+        kw = dict(tol=tol, maxit=100000, want_u=False, precond=precond)
+        c.solve_cases(**kw)  # graphs built
         t0 = time.perf_counter()
-        _, st = c.solve_cases(tol=tol, maxit=100000, want_u=False)
+        _, st = c.solve_cases(**kw)
         t1 = time.perf_counter()
         its = []
         for lc in range(c.load_case_count()):
             c.select_load_case(lc)
-            _, s1 = c.solve(tol=tol, maxit=100000, want_u=False)
+            _, s1 = c.solve(**kw)
             its.append(s1.iters)
         t2 = time.perf_counter()
         if [x.iters for x in st] != its:
@@ -317,10 +319,12 @@ def cases_leg(A, specs, tol):
                      "together_ms": (t1 - t0) * 1e3, "one_after_another_ms": (t2 - t1) * 1e3})
         t_all += t1 - t0
         t_seq += t2 - t1
-        # simp_run's automatic mode: together up to 600,000 elements
-        t_auto += (t1 - t0) if c.nelx <= 600000 else (t2 - t1)
+        # simp_run's automatic mode: multigrid together at every size,
+        # Jacobi-PCG together up to 600,000 elements
+        t_auto += (t1 - t0) if (mg or c.nelx <= 600000) else (t2 - t1)
     log(f"[bench] load cases together {t_all:.3f} s, one after another {t_seq:.3f} s")
-    return {"method": "fp64 Jacobi-PCG, one solve of each multi-case problem's cases",
+    return {"method": ("fp64 multigrid PCG" if mg else "fp64 Jacobi-PCG")
+                      + ", one solve of each multi-case problem's cases",
             "together_s": t_all, "one_after_another_s": t_seq,
             "speedup": t_seq / t_all if t_all > 0 else None,
             "automatic_s": t_auto, "automatic_speedup": t_seq / t_auto if t_auto > 0 else None,
