@@ -179,6 +179,14 @@ _SIGS = {
     "icarat_hip_batch_simp_run": (C.c_int, [C.c_void_p, C.POINTER(SimpOpts), C.POINTER(dp),
                                             C.POINTER(dp), C.POINTER(SimpStats)]),
     "icarat_hip_batch_autotune": (C.c_int, [C.c_void_p, C.c_int32, i32p, dp]),
+    "icarat_hip_mmasolver_create": (C.c_int, [C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                              C.c_double, C.POINTER(C.c_void_p)]),
+    "icarat_hip_mmasolver_destroy": (C.c_int, [C.c_void_p]),
+    "icarat_hip_mmasolver_last_error": (C.c_char_p, [C.c_void_p]),
+    "icarat_hip_mmasolver_set_params": (C.c_int, [C.c_void_p] + [C.c_double] * 5),
+    "icarat_hip_mmasolver_reset": (C.c_int, [C.c_void_p]),
+    "icarat_hip_mmasolver_update": (C.c_int, [C.c_void_p, dp, dp, dp, dp, dp, dp, i32p]),
+    "icarat_hip_mmasolver_asymptotes": (C.c_int, [C.c_void_p, dp, dp]),
     "icarat_hip_mg_status": (C.c_int, [C.c_void_p, C.POINTER(MgStatus)]),
     "icarat_hip_time_kernel": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, dp, dp, dp]),
     "icarat_hip_gaussian_field": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_double,
@@ -697,6 +705,60 @@ def cantilever(nx, ny, nz, **kw):
         v = -1.0 / (2 * ny) if j in (0, ny) else -1.0 / ny
         c.add_neumann([0.0, 0.0, v], [[nx, nx], [j, j], [0, 0]])
     return c
+
+
+class MMASolver:
+    """MMASolver(n, m, a, c, d) on the device (icarat_hip_mmasolver_*): any n,
+    1 <= m <= 8 constraints, host arrays as the reference's Update."""
+
+    def __init__(self, n, m, a=0.0, c=100000.0, d=0.0):
+        self.L = lib()
+        self.n, self.m = int(n), int(m)
+        h = C.c_void_p()
+        rc = self.L.icarat_hip_mmasolver_create(self.n, self.m, a, c, d, C.byref(h))
+        if rc:
+            raise IcaratError(rc, "mmasolver_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.icarat_hip_mmasolver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc, what):
+        if rc:
+            msg = self.L.icarat_hip_mmasolver_last_error(self.h)
+            raise IcaratError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def set_params(self, move=0.15, albefa=0.2, asyminit=0.2, asymdec=0.7, asyminc=1.2):
+        self._chk(self.L.icarat_hip_mmasolver_set_params(self.h, move, albefa, asyminit, asymdec,
+                                                         asyminc), "mmasolver_set_params")
+
+    def reset(self):
+        self._chk(self.L.icarat_hip_mmasolver_reset(self.h), "mmasolver_reset")
+
+    def update(self, x, dfdx, gx, dgdx, xmin, xmax):
+        """x (n) updated in place; dgdx (n x m, element-major, or flat n m);
+        returns the interior-point outer loops."""
+        n, m = self.n, self.m
+        arr = [_f64(v).reshape(-1) for v in (dfdx, gx, dgdx, xmin, xmax)]
+        if x.dtype != np.float64 or not x.flags.c_contiguous or x.size != n:
+            raise ValueError("x must be a contiguous float64 array of n values")
+        if [a.size for a in arr] != [n, m, n * m, n, n]:
+            raise ValueError("array sizes do not match n and m")
+        outer = C.c_int32()
+        self._chk(self.L.icarat_hip_mmasolver_update(self.h, _p(x), *[_p(a) for a in arr],
+                                                     C.byref(outer)), "mmasolver_update")
+        return outer.value
+
+    def asymptotes(self):
+        low, upp = np.zeros(self.n), np.zeros(self.n)
+        self._chk(self.L.icarat_hip_mmasolver_asymptotes(self.h, _p(low), _p(upp)),
+                  "mmasolver_asymptotes")
+        return low, upp
 
 
 class Batch:

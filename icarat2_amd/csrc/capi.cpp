@@ -12,7 +12,7 @@
 #include "gcmma.hpp"
 #include "learned.hpp"
 #include "mma.hpp"
-#include "multigrid.hpp"
+#include "mmam.hpp"
 #include "multigrid.hpp"
 #include "unet.hpp"
 
@@ -23,6 +23,11 @@ struct icarat_hip_ctx {
 
 struct icarat_hip_batch {
   icarat_hip::Batch *b = nullptr;
+  std::string err;
+};
+
+struct icarat_hip_mmasolver {
+  icarat_hip::DeviceMMAm *m = nullptr;
   std::string err;
 };
 
@@ -71,6 +76,25 @@ int guard(icarat_hip_ctx *h, F &&f) {
 int fail(icarat_hip_ctx *h, int code, const char *msg) {
   if (h) h->err = msg;
   return code;
+}
+
+template <class F>
+int mma_guard(icarat_hip_mmasolver *h, F &&f) {
+  if (!h || !h->m) return ICARAT_HIP_INVALID_ARGUMENT;
+  try {
+    f(*h->m);
+    h->err.clear();
+    return ICARAT_HIP_OK;
+  } catch (const icarat_hip::Error &e) {
+    h->err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    h->err = "host allocation failed";
+    return ICARAT_HIP_OUT_OF_MEMORY;
+  } catch (const std::exception &e) {
+    h->err = e.what();
+    return ICARAT_HIP_INTERNAL;
+  }
 }
 
 }  // namespace
@@ -783,6 +807,65 @@ int icarat_hip_batch_autotune(icarat_hip_batch *h, int32_t reps, int32_t *config
     const int c = b.autotune(reps, ms);
     if (config) *config = c;
   });
+}
+
+int icarat_hip_mmasolver_create(int32_t n, int32_t m, double a, double c, double d,
+                                icarat_hip_mmasolver **out) {
+  if (!out) return ICARAT_HIP_INVALID_ARGUMENT;
+  *out = nullptr;
+  if (n < 1 || m < 1 || m > icarat_hip::kMmaMaxM) return ICARAT_HIP_INVALID_ARGUMENT;
+  icarat_hip_mmasolver *h = new (std::nothrow) icarat_hip_mmasolver();
+  if (!h) return ICARAT_HIP_OUT_OF_MEMORY;
+  try {
+    h->m = new icarat_hip::DeviceMMAm(n, m, a, c, d);
+  } catch (const icarat_hip::Error &e) {
+    delete h;
+    return e.code;
+  } catch (...) {
+    delete h;
+    return ICARAT_HIP_INTERNAL;
+  }
+  *out = h;
+  return ICARAT_HIP_OK;
+}
+
+int icarat_hip_mmasolver_destroy(icarat_hip_mmasolver *h) {
+  if (!h) return ICARAT_HIP_INVALID_ARGUMENT;
+  delete h->m;
+  delete h;
+  return ICARAT_HIP_OK;
+}
+
+const char *icarat_hip_mmasolver_last_error(const icarat_hip_mmasolver *h) {
+  return h ? h->err.c_str() : "null MMA solver";
+}
+
+int icarat_hip_mmasolver_set_params(icarat_hip_mmasolver *h, double move, double albefa,
+                                    double asyminit, double asymdec, double asyminc) {
+  return mma_guard(h, [&](icarat_hip::DeviceMMAm &m) {
+    m.set_params(move, albefa, asyminit, asymdec, asyminc);
+  });
+}
+
+int icarat_hip_mmasolver_reset(icarat_hip_mmasolver *h) {
+  return mma_guard(h, [&](icarat_hip::DeviceMMAm &m) { m.reset(); });
+}
+
+int icarat_hip_mmasolver_update(icarat_hip_mmasolver *h, double *x, const double *dfdx,
+                                const double *gx, const double *dgdx, const double *xmin,
+                                const double *xmax, int32_t *outer) {
+  if (!x || !dfdx || !gx || !dgdx || !xmin || !xmax) {
+    if (h) h->err = "null argument";
+    return ICARAT_HIP_INVALID_ARGUMENT;
+  }
+  return mma_guard(h, [&](icarat_hip::DeviceMMAm &m) {
+    const int o = m.update(x, dfdx, gx, dgdx, xmin, xmax);
+    if (outer) *outer = o;
+  });
+}
+
+int icarat_hip_mmasolver_asymptotes(icarat_hip_mmasolver *h, double *low, double *upp) {
+  return mma_guard(h, [&](icarat_hip::DeviceMMAm &m) { m.asymptotes(low, upp); });
 }
 
 int icarat_hip_gaussian_field(int32_t nx, int32_t ny, int32_t nz, double h, uint64_t seed,

@@ -257,6 +257,30 @@ int icarat_hip_mma_reset(icarat_hip_ctx *ctx);
 int icarat_hip_mma_update(icarat_hip_ctx *ctx, double *x, const double *dfdx, double g,
                           const double *dgdx, double xmin, double xmax, int32_t *outer);
 
+/* MMASolver(n, m, a, c, d) (optimization/opt-solver/MMASolver.hpp:62-116,
+ * MMASolver.cpp) for any n and 1 <= m <= 8 constraints, on the device with its
+ * own stream; not tied to a context.  The subproblem (GenSub), x(lambda) and the
+ * dual sums run as element passes, the dual interior-point iteration's m x m
+ * Newton system (Factorize / Solve: LU without pivoting, as the reference), line
+ * search and epsilon loop in one device block.  set_params = SetMoveLimit /
+ * SetOscillation / SetAsymptotes (MMASolver.hpp:68-74); reset = Reset; update =
+ * Update(xval, dfdx, gx, dgdx, xmin, xmax) (MMASolver.cpp:66-82) with host arrays:
+ * x (n, updated in place), dfdx (n), gx (m), dgdx (n m, dgdx[i m + j]), xmin and
+ * xmax (n); *outer = interior-point outer loops (may be NULL).  asymptotes:
+ * the last update's low and upp (n each, either may be NULL). */
+typedef struct icarat_hip_mmasolver icarat_hip_mmasolver;
+int icarat_hip_mmasolver_create(int32_t n, int32_t m, double a, double c, double d,
+                                icarat_hip_mmasolver **out);
+int icarat_hip_mmasolver_destroy(icarat_hip_mmasolver *h);
+const char *icarat_hip_mmasolver_last_error(const icarat_hip_mmasolver *h);
+int icarat_hip_mmasolver_set_params(icarat_hip_mmasolver *h, double move, double albefa,
+                                    double asyminit, double asymdec, double asyminc);
+int icarat_hip_mmasolver_reset(icarat_hip_mmasolver *h);
+int icarat_hip_mmasolver_update(icarat_hip_mmasolver *h, double *x, const double *dfdx,
+                                const double *gx, const double *dgdx, const double *xmin,
+                                const double *xmax, int32_t *outer);
+int icarat_hip_mmasolver_asymptotes(icarat_hip_mmasolver *h, double *low, double *upp);
+
 /* GCMMASolver(n = nelx, m = 1, a = 0, c = 1000, d = 0) (optimization/opt-solver/
  * GCMMASolver.hpp:62-85) on the device, for the single volume constraint fx:
  * SetMoveLimit / SetOscillation / SetAsymptotes, and Reset (which here also

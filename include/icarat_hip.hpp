@@ -247,17 +247,22 @@ class OC {
   long long n_;
 };
 
-// MMASolver(n, m = 1) on the device
+// MMASolver(n, m, a, c, d) on the device: any n, 1 <= m <= 8 constraints
+// (icarat_hip_mmasolver_*; not tied to a Device)
 class MMASolver {
  public:
-  MMASolver(Device &dev, int n, int m, double a = 0.0, double c = 100000.0, double d = 0.0)
-      : dev_(dev), n_(n) {
-    if (m != 1 || a != 0.0 || c != 100000.0 || d != 0.0 || n != dev.nelx()) {
-      std::cerr << "Error : MMASolver on the device supports n = nelx, m = 1, a = 0, c = 1e5, d = 0"
-                << std::endl;
+  MMASolver(int n, int m, double a = 0.0, double c = 100000.0, double d = 0.0) {
+    if (icarat_hip_mmasolver_create(n, m, a, c, d, &h_) != ICARAT_HIP_OK) {
+      std::cerr << "Error : MMASolver on the device supports n >= 1 and 1 <= m <= 8" << std::endl;
       std::exit(1);
     }
-    check(icarat_hip_mma_reset(dev_.handle()), dev_.handle(), "MMASolver");
+  }
+  MMASolver(Device &, int n, int m, double a = 0.0, double c = 100000.0, double d = 0.0)
+      : MMASolver(n, m, a, c, d) {}
+  MMASolver(const MMASolver &) = delete;
+  MMASolver &operator=(const MMASolver &) = delete;
+  ~MMASolver() {
+    if (h_) icarat_hip_mmasolver_destroy(h_);
   }
   void SetAsymptotes(double init, double decrease, double increase) {
     asyminit_ = init;
@@ -276,22 +281,23 @@ class MMASolver {
   void ConstraintModification(bool) {}
   void Update(double *xval, const double *dfdx, const double *gx, const double *dgdx,
               const double *xmin, const double *xmax) {
-    if (!uniform(xmin, n_) || !uniform(xmax, n_)) {
-      std::cerr << "Error : MMASolver::Update needs uniform bounds" << std::endl;
-      std::exit(1);
-    }
-    check(icarat_hip_mma_update(dev_.handle(), xval, dfdx, gx[0], dgdx, xmin[0], xmax[0], nullptr),
-          dev_.handle(), "MMASolver::Update");
+    done(icarat_hip_mmasolver_update(h_, xval, dfdx, gx, dgdx, xmin, xmax, nullptr),
+         "MMASolver::Update");
   }
-  void Reset() { check(icarat_hip_mma_reset(dev_.handle()), dev_.handle(), "MMASolver::Reset"); }
+  void Reset() { done(icarat_hip_mmasolver_reset(h_), "MMASolver::Reset"); }
 
  private:
   void push() {
-    check(icarat_hip_mma_set_params(dev_.handle(), move_, albefa_, asyminit_, asymdec_, asyminc_),
-          dev_.handle(), "MMASolver parameters");
+    done(icarat_hip_mmasolver_set_params(h_, move_, albefa_, asyminit_, asymdec_, asyminc_),
+         "MMASolver parameters");
   }
-  Device &dev_;
-  long long n_;
+  void done(int rc, const char *what) {
+    if (rc == ICARAT_HIP_OK) return;
+    std::cerr << "Error : " << what << ": " << icarat_hip_mmasolver_last_error(h_) << " (status "
+              << rc << ")" << std::endl;
+    std::exit(1);
+  }
+  icarat_hip_mmasolver *h_ = nullptr;
   double move_ = 0.15, albefa_ = 0.2, asyminit_ = 0.2, asymdec_ = 0.7, asyminc_ = 1.2;
 };
 

@@ -474,6 +474,12 @@ class MMA:
         args = [np.ascontiguousarray(a, dtype=np.float64) for a in (dfdx, gx, dgdx, xmin, xmax)]
         f(self.h, P(x), *[P(a) for a in args])
 
+    def state(self):
+        """the asymptotes (low, upp) of the last update (oracle only)"""
+        low, upp = np.zeros(self.n), np.zeros(self.n)
+        self.L.orc_mma_state(self.h, P(low), P(upp))
+        return low, upp
+
     def __del__(self):
         if getattr(self, "h", None):
             (self.L.ref_mma_destroy if self.ref else self.L.orc_mma_destroy)(self.h)

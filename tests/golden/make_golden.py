@@ -7,6 +7,9 @@ inputs and outputs only:
                 design after each of 10 updates
   mma_m1.npz    a 500-variable, one-constraint MMA run: per update the inputs
                 (dfdx, g, dgdx) and the design after the update
+  mma_m3.npz    a 400-variable, three-constraint MMA run with per-variable
+                bounds and non-default parameters: per update the inputs
+                (dfdx, gx, dgdx) and the design and asymptotes after it
   oc_filter.npz OC on an 8x6x5 mesh whose volume callback is the cone filter
                 (r = 1.5) summed, as the SIMP drivers pass it: inputs and the
                 updated design
@@ -69,6 +72,30 @@ def main():
         xs.append(x.copy())
     np.savez(os.path.join(HERE, "mma_m1.npz"), x0=x0, dfdx=np.array(dfs), g=np.array(gs),
              dgdx=np.array(dgs), x=np.array(xs))
+    # MMA, n = 400, m = 3: weighted-volume constraints, per-variable bounds
+    rng = np.random.default_rng(8)
+    n, m = 400, 3
+    x0 = rng.uniform(0.2, 0.8, n)
+    xmin = rng.uniform(0.0, 0.1, n)
+    xmax = rng.uniform(0.9, 1.0, n)
+    w = rng.uniform(0.5, 1.5, (m, n)) / n
+    cap = np.array([0.45, 0.5, 0.4])
+    x = x0.copy()
+    mm = O.MMA(n, m, reference=True)
+    mm.set(0.3, 0.15, 0.3, 0.65, 1.1)
+    dfs, gs, dgs, xs = [], [], [], []
+    for _ in range(8):
+        df = -rng.uniform(0.1, 3.0, n) * (1.0 + x)
+        gx = w @ x - cap
+        dg = np.ascontiguousarray(w.T).reshape(-1)  # dgdx[i m + j]
+        mm.update(x, df, gx, dg, xmin, xmax)
+        dfs.append(df)
+        gs.append(gx)
+        dgs.append(dg)
+        xs.append(x.copy())
+    np.savez(os.path.join(HERE, "mma_m3.npz"), x0=x0, xmin=xmin, xmax=xmax, dfdx=np.array(dfs),
+             gx=np.array(gs), dgdx=np.array(dgs), x=np.array(xs),
+             params=np.array([0.3, 0.15, 0.3, 0.65, 1.1]))
     # OC with the filtered-volume callback
     rng = np.random.default_rng(7)
     pr = O.Problem(8, 6, 5)

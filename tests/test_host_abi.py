@@ -113,6 +113,11 @@ def test_errors_are_status_codes(capi):
     assert L.icarat_hip_create(C.byref(d), C.byref(h)) == capi.INVALID_ARGUMENT
     d = capi.GridDesc(2, 2, 2, 2.0, 2.0, 2.0, 1.0, 1e-9, 0.3, 3.0)  # young2 < young1
     assert L.icarat_hip_create(C.byref(d), C.byref(h)) == capi.INVALID_ARGUMENT
+    # MMA with m constraints: 1 <= m <= 8, n >= 1 (checked before any device call)
+    for n, m in ((10, 9), (10, 0), (0, 2)):
+        assert L.icarat_hip_mmasolver_create(n, m, 0.0, 1e5, 0.0, C.byref(h)) == capi.INVALID_ARGUMENT
+    assert L.icarat_hip_mmasolver_update(None, None, None, None, None, None, None, None) == \
+        capi.INVALID_ARGUMENT
 
 
 def test_device_calls_without_gpu_fail_loudly(capi):
