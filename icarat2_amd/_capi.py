@@ -187,6 +187,18 @@ _SIGS = {
     "icarat_hip_mmasolver_reset": (C.c_int, [C.c_void_p]),
     "icarat_hip_mmasolver_update": (C.c_int, [C.c_void_p, dp, dp, dp, dp, dp, dp, i32p]),
     "icarat_hip_mmasolver_asymptotes": (C.c_int, [C.c_void_p, dp, dp]),
+    "icarat_hip_gcmmasolver_create": (C.c_int, [C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                                C.c_double, C.POINTER(C.c_void_p)]),
+    "icarat_hip_gcmmasolver_destroy": (C.c_int, [C.c_void_p]),
+    "icarat_hip_gcmmasolver_last_error": (C.c_char_p, [C.c_void_p]),
+    "icarat_hip_gcmmasolver_set_params": (C.c_int, [C.c_void_p] + [C.c_double] * 5),
+    "icarat_hip_gcmmasolver_reset": (C.c_int, [C.c_void_p]),
+    "icarat_hip_gcmmasolver_outer": (C.c_int, [C.c_void_p, dp, dp, C.c_double, dp, dp, dp, dp,
+                                               dp]),
+    "icarat_hip_gcmmasolver_inner": (C.c_int, [C.c_void_p, dp, C.c_double, dp, dp, C.c_double,
+                                               dp, dp, dp, dp, dp]),
+    "icarat_hip_gcmmasolver_concheck": (C.c_int, [C.c_void_p, C.c_double, dp, i32p]),
+    "icarat_hip_gcmmasolver_state": (C.c_int, [C.c_void_p, dp]),
     "icarat_hip_mg_status": (C.c_int, [C.c_void_p, C.POINTER(MgStatus)]),
     "icarat_hip_time_kernel": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, dp, dp, dp]),
     "icarat_hip_gaussian_field": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_double,
@@ -759,6 +771,87 @@ class MMASolver:
         self._chk(self.L.icarat_hip_mmasolver_asymptotes(self.h, _p(low), _p(upp)),
                   "mmasolver_asymptotes")
         return low, upp
+
+
+class GCMMASolver:
+    """GCMMASolver(n, m, a, c, d) on the device (icarat_hip_gcmmasolver_*): any n,
+    1 <= m <= 8 constraints, host arrays as the reference's OuterUpdate /
+    InnerUpdate / ConCheck."""
+
+    def __init__(self, n, m, a=0.0, c=1000.0, d=0.0):
+        self.L = lib()
+        self.n, self.m = int(n), int(m)
+        h = C.c_void_p()
+        rc = self.L.icarat_hip_gcmmasolver_create(self.n, self.m, a, c, d, C.byref(h))
+        if rc:
+            raise IcaratError(rc, "gcmmasolver_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.icarat_hip_gcmmasolver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _chk(self, rc, what):
+        if rc:
+            msg = self.L.icarat_hip_gcmmasolver_last_error(self.h)
+            raise IcaratError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def _arrays(self, fx, dfdx, rest):
+        n, m = self.n, self.m
+        a = [_f64(v).reshape(-1) for v in rest]
+        if any(v.size != n for v in a):
+            raise ValueError("n-arrays of the wrong size")
+        fx, dfdx = _f64(fx).reshape(-1), _f64(dfdx).reshape(-1)
+        if fx.size != m or dfdx.size != n * m:
+            raise ValueError("constraint arrays of the wrong size")
+        return fx, dfdx, a
+
+    def set_params(self, move=0.15, albefa=0.2, asyminit=0.2, asymdec=0.7, asyminc=1.2):
+        self._chk(self.L.icarat_hip_gcmmasolver_set_params(self.h, move, albefa, asyminit,
+                                                           asymdec, asyminc),
+                  "gcmmasolver_set_params")
+
+    def reset(self):
+        self._chk(self.L.icarat_hip_gcmmasolver_reset(self.h), "gcmmasolver_reset")
+
+    def outer(self, xval, f0x, df0dx, fx, dfdx, xmin, xmax):
+        """OuterUpdate: returns xmma"""
+        fx, dfdx, (xv, df, lo, hi) = self._arrays(fx, dfdx, (xval, df0dx, xmin, xmax))
+        xm = np.zeros(self.n)
+        self._chk(self.L.icarat_hip_gcmmasolver_outer(self.h, _p(xm), _p(xv), float(f0x), _p(df),
+                                                      _p(fx), _p(dfdx), _p(lo), _p(hi)),
+                  "gcmmasolver_outer")
+        return xm
+
+    def inner(self, xmma, f0xnew, fxnew, xval, f0x, df0dx, fx, dfdx, xmin, xmax):
+        """InnerUpdate from xmma: returns the new xmma"""
+        fx, dfdx, (xv, df, lo, hi) = self._arrays(fx, dfdx, (xval, df0dx, xmin, xmax))
+        fxn = _f64(fxnew).reshape(-1)
+        xm = np.array(xmma, dtype=np.float64)
+        if xm.size != self.n or fxn.size != self.m:
+            raise ValueError("xmma or fxnew of the wrong size")
+        self._chk(self.L.icarat_hip_gcmmasolver_inner(self.h, _p(xm), float(f0xnew), _p(fxn),
+                                                      _p(xv), float(f0x), _p(df), _p(fx),
+                                                      _p(dfdx), _p(lo), _p(hi)),
+                  "gcmmasolver_inner")
+        return xm
+
+    def concheck(self, f0xnew, fxnew):
+        ok = C.c_int32()
+        self._chk(self.L.icarat_hip_gcmmasolver_concheck(self.h, float(f0xnew),
+                                                         _p(_f64(fxnew).reshape(-1)),
+                                                         C.byref(ok)), "gcmmasolver_concheck")
+        return bool(ok.value)
+
+    def state(self):
+        """{raa0, raa[m], f0app, fapp[m]}"""
+        out = np.zeros(2 + 2 * self.m)
+        self._chk(self.L.icarat_hip_gcmmasolver_state(self.h, _p(out)), "gcmmasolver_state")
+        return out
 
 
 class Batch:

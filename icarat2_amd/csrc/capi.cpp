@@ -27,7 +27,12 @@ struct icarat_hip_batch {
 };
 
 struct icarat_hip_mmasolver {
-  icarat_hip::DeviceMMAm *m = nullptr;
+  icarat_hip::DeviceMMAm *obj = nullptr;
+  std::string err;
+};
+
+struct icarat_hip_gcmmasolver {
+  icarat_hip::DeviceGCMMAm *obj = nullptr;
   std::string err;
 };
 
@@ -78,11 +83,12 @@ int fail(icarat_hip_ctx *h, int code, const char *msg) {
   return code;
 }
 
-template <class F>
-int mma_guard(icarat_hip_mmasolver *h, F &&f) {
-  if (!h || !h->m) return ICARAT_HIP_INVALID_ARGUMENT;
+// a solver object's handle (obj, err): exception-to-status translation
+template <class H, class F>
+int obj_guard(H *h, F &&f) {
+  if (!h || !h->obj) return ICARAT_HIP_INVALID_ARGUMENT;
   try {
-    f(*h->m);
+    f(*h->obj);
     h->err.clear();
     return ICARAT_HIP_OK;
   } catch (const icarat_hip::Error &e) {
@@ -817,7 +823,7 @@ int icarat_hip_mmasolver_create(int32_t n, int32_t m, double a, double c, double
   icarat_hip_mmasolver *h = new (std::nothrow) icarat_hip_mmasolver();
   if (!h) return ICARAT_HIP_OUT_OF_MEMORY;
   try {
-    h->m = new icarat_hip::DeviceMMAm(n, m, a, c, d);
+    h->obj = new icarat_hip::DeviceMMAm(n, m, a, c, d);
   } catch (const icarat_hip::Error &e) {
     delete h;
     return e.code;
@@ -831,7 +837,7 @@ int icarat_hip_mmasolver_create(int32_t n, int32_t m, double a, double c, double
 
 int icarat_hip_mmasolver_destroy(icarat_hip_mmasolver *h) {
   if (!h) return ICARAT_HIP_INVALID_ARGUMENT;
-  delete h->m;
+  delete h->obj;
   delete h;
   return ICARAT_HIP_OK;
 }
@@ -842,13 +848,13 @@ const char *icarat_hip_mmasolver_last_error(const icarat_hip_mmasolver *h) {
 
 int icarat_hip_mmasolver_set_params(icarat_hip_mmasolver *h, double move, double albefa,
                                     double asyminit, double asymdec, double asyminc) {
-  return mma_guard(h, [&](icarat_hip::DeviceMMAm &m) {
+  return obj_guard(h, [&](icarat_hip::DeviceMMAm &m) {
     m.set_params(move, albefa, asyminit, asymdec, asyminc);
   });
 }
 
 int icarat_hip_mmasolver_reset(icarat_hip_mmasolver *h) {
-  return mma_guard(h, [&](icarat_hip::DeviceMMAm &m) { m.reset(); });
+  return obj_guard(h, [&](icarat_hip::DeviceMMAm &m) { m.reset(); });
 }
 
 int icarat_hip_mmasolver_update(icarat_hip_mmasolver *h, double *x, const double *dfdx,
@@ -858,14 +864,98 @@ int icarat_hip_mmasolver_update(icarat_hip_mmasolver *h, double *x, const double
     if (h) h->err = "null argument";
     return ICARAT_HIP_INVALID_ARGUMENT;
   }
-  return mma_guard(h, [&](icarat_hip::DeviceMMAm &m) {
+  return obj_guard(h, [&](icarat_hip::DeviceMMAm &m) {
     const int o = m.update(x, dfdx, gx, dgdx, xmin, xmax);
     if (outer) *outer = o;
   });
 }
 
 int icarat_hip_mmasolver_asymptotes(icarat_hip_mmasolver *h, double *low, double *upp) {
-  return mma_guard(h, [&](icarat_hip::DeviceMMAm &m) { m.asymptotes(low, upp); });
+  return obj_guard(h, [&](icarat_hip::DeviceMMAm &m) { m.asymptotes(low, upp); });
+}
+
+int icarat_hip_gcmmasolver_create(int32_t n, int32_t m, double a, double c, double d,
+                                  icarat_hip_gcmmasolver **out) {
+  if (!out) return ICARAT_HIP_INVALID_ARGUMENT;
+  *out = nullptr;
+  if (n < 1 || m < 1 || m > icarat_hip::kMmaMaxM) return ICARAT_HIP_INVALID_ARGUMENT;
+  icarat_hip_gcmmasolver *h = new (std::nothrow) icarat_hip_gcmmasolver();
+  if (!h) return ICARAT_HIP_OUT_OF_MEMORY;
+  try {
+    h->obj = new icarat_hip::DeviceGCMMAm(n, m, a, c, d);
+  } catch (const icarat_hip::Error &e) {
+    delete h;
+    return e.code;
+  } catch (...) {
+    delete h;
+    return ICARAT_HIP_INTERNAL;
+  }
+  *out = h;
+  return ICARAT_HIP_OK;
+}
+
+int icarat_hip_gcmmasolver_destroy(icarat_hip_gcmmasolver *h) {
+  if (!h) return ICARAT_HIP_INVALID_ARGUMENT;
+  delete h->obj;
+  delete h;
+  return ICARAT_HIP_OK;
+}
+
+const char *icarat_hip_gcmmasolver_last_error(const icarat_hip_gcmmasolver *h) {
+  return h ? h->err.c_str() : "null GCMMA solver";
+}
+
+int icarat_hip_gcmmasolver_set_params(icarat_hip_gcmmasolver *h, double move, double albefa,
+                                      double asyminit, double asymdec, double asyminc) {
+  return obj_guard(h, [&](icarat_hip::DeviceGCMMAm &g) {
+    g.set_params(move, albefa, asyminit, asymdec, asyminc);
+  });
+}
+
+int icarat_hip_gcmmasolver_reset(icarat_hip_gcmmasolver *h) {
+  return obj_guard(h, [&](icarat_hip::DeviceGCMMAm &g) { g.reset(); });
+}
+
+int icarat_hip_gcmmasolver_outer(icarat_hip_gcmmasolver *h, double *xmma, const double *xval,
+                                 double f0x, const double *df0dx, const double *fx,
+                                 const double *dfdx, const double *xmin, const double *xmax) {
+  if (!xmma || !xval || !df0dx || !fx || !dfdx || !xmin || !xmax) {
+    if (h) h->err = "null argument";
+    return ICARAT_HIP_INVALID_ARGUMENT;
+  }
+  return obj_guard(h, [&](icarat_hip::DeviceGCMMAm &g) {
+    g.outer(xmma, xval, f0x, df0dx, fx, dfdx, xmin, xmax);
+  });
+}
+
+int icarat_hip_gcmmasolver_inner(icarat_hip_gcmmasolver *h, double *xmma, double f0xnew,
+                                 const double *fxnew, const double *xval, double f0x,
+                                 const double *df0dx, const double *fx, const double *dfdx,
+                                 const double *xmin, const double *xmax) {
+  if (!xmma || !fxnew || !xval || !df0dx || !fx || !dfdx || !xmin || !xmax) {
+    if (h) h->err = "null argument";
+    return ICARAT_HIP_INVALID_ARGUMENT;
+  }
+  return obj_guard(h, [&](icarat_hip::DeviceGCMMAm &g) {
+    g.inner(xmma, f0xnew, fxnew, xval, f0x, df0dx, fx, dfdx, xmin, xmax);
+  });
+}
+
+int icarat_hip_gcmmasolver_concheck(icarat_hip_gcmmasolver *h, double f0xnew, const double *fxnew,
+                                    int32_t *ok) {
+  if (!fxnew || !ok) {
+    if (h) h->err = "null argument";
+    return ICARAT_HIP_INVALID_ARGUMENT;
+  }
+  return obj_guard(h, [&](icarat_hip::DeviceGCMMAm &g) { *ok = g.concheck(f0xnew, fxnew) ? 1 : 0; });
+}
+
+int icarat_hip_gcmmasolver_state(icarat_hip_gcmmasolver *h, double *out) {
+  if (!out) {
+    if (h) h->err = "null argument";
+    return ICARAT_HIP_INVALID_ARGUMENT;
+  }
+  return obj_guard(h, [&](icarat_hip::DeviceGCMMAm &g) { g.state(out); });
 }
 
 int icarat_hip_gaussian_field(int32_t nx, int32_t ny, int32_t nz, double h, uint64_t seed,

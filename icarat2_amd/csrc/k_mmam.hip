@@ -273,6 +273,136 @@ __global__ __launch_bounds__(NT) void k_mmam_newton(const double *partials, unsi
   *st = S;
 }
 
+// GCMMA (reference GCMMASolver.cpp): the subproblem at xval -- with asymp,
+// Asymp's asymptotes first (:432-508) and the history shift of OuterUpdate
+// (:66-90) -- as GenSub (:546-596): move limits without a second clamp to the
+// bounds, raa0 in the objective and raa[j] in the constraints; partials: the
+// objective's sum p0/(U-x) + q0/(x-L) and the m constraint sums.
+constexpr int NG = 1 + MM;
+__global__ __launch_bounds__(NT) void k_gcmmam_sub(MmamParams P, int asymp,
+                                                   const double *__restrict__ x,
+                                                   const double *__restrict__ df0dx,
+                                                   const double *__restrict__ dfdx, double raa0,
+                                                   GcmmaRaa raa, MmamArrays A, double *partials) {
+  __shared__ double sh[NG * (NT / 64)];
+  const long long n = P.n;
+  const int m = P.m;
+  double v[NG];
+#pragma unroll
+  for (int k = 0; k < NG; ++k) v[k] = 0.0;
+  const long long stride = (long long)gridDim.x * NT;
+  for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += stride) {
+    const double xi = x[i];
+    const double xmn = A.xmin[i], xmx = A.xmax[i];
+    double lo, up;
+    if (asymp) {
+      const double xo1 = A.xold1[i], xo2 = A.xold2[i];
+      if (P.iter < 3) {
+        lo = xi - P.asyminit * (xmx - xmn);
+        up = xi + P.asyminit * (xmx - xmn);
+      } else {
+        const double osc = (xi - xo1) * (xo1 - xo2);
+        const double gam = osc < 0.0 ? P.asymdec : (osc > 0.0 ? P.asyminc : 1.0);
+        lo = xi - gam * (xo1 - A.low[i]);
+        up = xi + gam * (A.upp[i] - xo1);
+        const double w = fmax(P.xmamieps, xmx - xmn);
+        lo = fmin(fmax(lo, xi - 100.0 * w), xi - 1e-5 * w);
+        up = fmin(fmax(up, xi + 1e-5 * w), xi + 100.0 * w);
+        const double xl = xmn - 1.0e-6, xh = xmx + 1.0e-6;
+        if (xi < xl) {
+          lo = xi - (xh - xi) / 0.9;
+          up = xi + (xh - xi) / 0.9;
+        }
+        if (xi > xh) {
+          lo = xi - (xi - xl) / 0.9;
+          up = xi + (xi - xl) / 0.9;
+        }
+      }
+      A.low[i] = lo;
+      A.upp[i] = up;
+      A.xold2[i] = xo1;
+      A.xold1[i] = xi;
+    } else {
+      lo = A.low[i];
+      up = A.upp[i];
+    }
+    A.alpha[i] = fmax(fmax(xmn, lo + P.albefa * (xi - lo)), xi - P.move * (xmx - xmn));
+    A.beta[i] = fmin(fmin(xmx, up - P.albefa * (up - xi)), xi + P.move * (xmx - xmn));
+    const double uxinv = 1.0 / (up - xi), xlinv = 1.0 / (xi - lo);
+    const double ux2 = (up - xi) * (up - xi), xl2 = (xi - lo) * (xi - lo);
+    const double winv = 1.0 / fmax(P.xmamieps, xmx - xmn);
+    const double df = df0dx[i];
+    const double r0 = 0.001 * fabs(df) + raa0 * winv;
+    const double p0 = ux2 * (fmax(0.0, df) + r0), q0 = xl2 * (fmax(0.0, -df) + r0);
+    A.p0[i] = p0;
+    A.q0[i] = q0;
+    v[0] += p0 * uxinv + q0 * xlinv;
+#pragma unroll
+    for (int j = 0; j < MM; ++j) {
+      if (j < m) {
+        const double dg = dfdx[i * m + j];
+        const double r1 = 0.001 * fabs(dg) + raa.r[j] * winv;
+        const double pij = ux2 * (fmax(0.0, dg) + r1), qij = xl2 * (fmax(0.0, -dg) + r1);
+        A.P[j * n + i] = pij;
+        A.Q[j * n + i] = qij;
+        v[1 + j] += pij * uxinv + qij * xlinv;
+      }
+    }
+  }
+  store_partials<NT, NG>(v, partials, sh);
+}
+
+// ComputeApprox's sums at xmma (GCMMASolver.cpp:598-620): the objective's and
+// the m constraints' p/(U-x) + q/(x-L)
+__global__ __launch_bounds__(NT) void k_gcmmam_approx(MmamParams P, const double *__restrict__ xm,
+                                                      MmamArrays A, double *partials) {
+  __shared__ double sh[NG * (NT / 64)];
+  const long long n = P.n;
+  const int m = P.m;
+  double v[NG];
+#pragma unroll
+  for (int k = 0; k < NG; ++k) v[k] = 0.0;
+  const long long stride = (long long)gridDim.x * NT;
+  for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += stride) {
+    const double uxinv = 1.0 / (A.upp[i] - xm[i]), xlinv = 1.0 / (xm[i] - A.low[i]);
+    v[0] += A.p0[i] * uxinv + A.q0[i] * xlinv;
+#pragma unroll
+    for (int j = 0; j < MM; ++j)
+      if (j < m) v[1 + j] += A.P[j * n + i] * uxinv + A.Q[j * n + i] * xlinv;
+  }
+  store_partials<NT, NG>(v, partials, sh);
+}
+
+// RaaUpdate's raacof (GCMMASolver.cpp:510-526): sum ((xm-xv)/(U-xm))
+// ((xm-xv)/(xm-L)) (U-L)/xmami
+__global__ __launch_bounds__(NT) void k_gcmmam_raacof(MmamParams P, const double *__restrict__ xm,
+                                                      const double *__restrict__ xv,
+                                                      MmamArrays A, double *partials) {
+  __shared__ double sh[NT / 64];
+  const long long n = P.n;
+  double acc = 0.0;
+  const long long stride = (long long)gridDim.x * NT;
+  for (long long i = blockIdx.x * (long long)NT + threadIdx.x; i < n; i += stride) {
+    const double xmami = fmax(P.xmamieps, A.xmax[i] - A.xmin[i]);
+    const double lo = A.low[i], up = A.upp[i];
+    const double xxux = (xm[i] - xv[i]) / (up - xm[i]);
+    const double xxxl = (xm[i] - xv[i]) / (xm[i] - lo);
+    acc += (xxux * xxxl) * ((up - lo) / xmami);
+  }
+  double v[1] = {acc};
+  store_partials<NT, 1>(v, partials, sh);
+}
+
+// the blocks' partials summed in a fixed order into out[0 .. nv)
+template <int NVS>
+__global__ void k_mmam_sum(const double *partials, unsigned G, int nv, double *out) {
+  __shared__ double sh[NVS * (NT / 64)];
+  double tot[NVS];
+  sum_partials<NT, NVS>(partials, G, tot, sh);
+  if (threadIdx.x == 0)
+    for (int k = 0; k < nv; ++k) out[k] = tot[k];
+}
+
 }  // namespace
 
 unsigned mmam_blocks(long long n) {
@@ -311,5 +441,36 @@ void launch_mmam_step(const MmamParams &P, const MmamArrays &A, double *x, doubl
   hipLaunchKernelGGL(k_mmam_newton, dim3(1), dim3(NT), 0, s, partials, G, st);
   HIPCHK(hipGetLastError());
 }
+
+}  // namespace icarat_hip
+
+namespace icarat_hip {
+
+void launch_gcmmam_sub(const MmamParams &P, int asymp, const double *x, const double *df0dx,
+                       const double *dfdx, double raa0, const GcmmaRaa &raa, const MmamArrays &A,
+                       double *partials, double *out, hipStream_t s) {
+  const unsigned G = mmam_blocks(P.n);
+  hipLaunchKernelGGL(k_gcmmam_sub, dim3(G), dim3(NT), 0, s, P, asymp, x, df0dx, dfdx, raa0, raa,
+                     A, partials);
+  hipLaunchKernelGGL(k_mmam_sum<NG>, dim3(1), dim3(NT), 0, s, partials, G, 1 + P.m, out);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_gcmmam_approx(const MmamParams &P, const double *xm, const MmamArrays &A,
+                          double *partials, double *out, hipStream_t s) {
+  const unsigned G = mmam_blocks(P.n);
+  hipLaunchKernelGGL(k_gcmmam_approx, dim3(G), dim3(NT), 0, s, P, xm, A, partials);
+  hipLaunchKernelGGL(k_mmam_sum<NG>, dim3(1), dim3(NT), 0, s, partials, G, 1 + P.m, out);
+  HIPCHK(hipGetLastError());
+}
+
+void launch_gcmmam_raacof(const MmamParams &P, const double *xm, const double *xv,
+                          const MmamArrays &A, double *partials, double *out, hipStream_t s) {
+  const unsigned G = mmam_blocks(P.n);
+  hipLaunchKernelGGL(k_gcmmam_raacof, dim3(G), dim3(NT), 0, s, P, xm, xv, A, partials);
+  hipLaunchKernelGGL(k_mmam_sum<1>, dim3(1), dim3(NT), 0, s, partials, G, 1, out);
+  HIPCHK(hipGetLastError());
+}
+
 
 }  // namespace icarat_hip

@@ -1,6 +1,7 @@
-// mmam.hpp -- Svanberg's MMA with m constraints (1 <= m <= kMmaMaxM) on the
-// device: MMASolver(n, m, a, c, d) of the reference
-// (optimization/opt-solver/MMASolver.hpp:62-116, MMASolver.cpp), FP64.  The
+// mmam.hpp -- Svanberg's MMA and GCMMA with m constraints (1 <= m <=
+// kMmaMaxM) on the device: MMASolver(n, m, a, c, d) and GCMMASolver(n, m, a,
+// c, d) of the reference (optimization/opt-solver/MMASolver.hpp:62-116,
+// MMASolver.cpp, GCMMASolver.hpp:62-85, GCMMASolver.cpp), FP64.  The
 // subproblem, x(lambda) and the dual sums are element-parallel passes; the
 // m x m Newton system of the dual interior-point iteration, its line search
 // and the epsilon loop run in one device block, so an update needs no host
@@ -52,6 +53,22 @@ void launch_mmam_init(const MmamParams &P, const double *gx_host, double a, doub
 void launch_mmam_step(const MmamParams &P, const MmamArrays &A, double *x, double *partials,
                       MmamState *st, hipStream_t s);
 
+// GCMMA (reference GCMMASolver.cpp) passes: the subproblem at x (asymp != 0:
+// Asymp's asymptotes and the history shift first) with out[0] = the
+// objective's sum p0/(U-x) + q0/(x-L) and out[1 + j] the constraints' sums;
+// the approximation sums at xm (same layout); raacof at xm, xv (out[0]).
+// out is device memory.
+struct GcmmaRaa {
+  double r[kMmaMaxM];
+};
+void launch_gcmmam_sub(const MmamParams &P, int asymp, const double *x, const double *df0dx,
+                       const double *dfdx, double raa0, const GcmmaRaa &raa, const MmamArrays &A,
+                       double *partials, double *out, hipStream_t s);
+void launch_gcmmam_approx(const MmamParams &P, const double *xm, const MmamArrays &A,
+                          double *partials, double *out, hipStream_t s);
+void launch_gcmmam_raacof(const MmamParams &P, const double *xm, const double *xv,
+                          const MmamArrays &A, double *partials, double *out, hipStream_t s);
+
 // MMASolver(n, m, a, c, d) with its own stream; host pointers in update.
 class DeviceMMAm {
  public:
@@ -80,6 +97,48 @@ class DeviceMMAm {
   DevBuf<double> low_, upp_, alpha_, beta_, p0_, q0_, pij_, qij_, xold1_, xold2_, xmin_, xmax_;
   DevBuf<MmamState> st_;
   PinnedBuf<MmamState> hst_;
+  MmamArrays A_;
+};
+
+// GCMMASolver(n, m, a, c, d) (reference GCMMASolver.hpp:62-85) with its own
+// stream; host pointers.  The scalars (raa0, raa, the approximation
+// constants, ConCheck) stay on the host; the element passes and the dual
+// iteration run on the device.
+class DeviceGCMMAm {
+ public:
+  DeviceGCMMAm(long long n, int m, double a, double c, double d);
+  ~DeviceGCMMAm();
+  DeviceGCMMAm(const DeviceGCMMAm &) = delete;
+  DeviceGCMMAm &operator=(const DeviceGCMMAm &) = delete;
+  void set_params(double move, double albefa, double asyminit, double asymdec, double asyminc);
+  void reset() { P_.iter = 0; }  // Reset (GCMMASolver.hpp:83): the outer counter only
+  // OuterUpdate / InnerUpdate (GCMMASolver.cpp:66-113): xmma (n) receives the
+  // new design (InnerUpdate reads it first); fx, fxnew (m); dfdx (n m,
+  // dfdx[i m + j]); xval, df0dx, xmin, xmax (n)
+  void outer(double *xmma, const double *xval, double f0x, const double *df0dx, const double *fx,
+             const double *dfdx, const double *xmin, const double *xmax);
+  void inner(double *xmma, double f0xnew, const double *fxnew, const double *xval, double f0x,
+             const double *df0dx, const double *fx, const double *dfdx, const double *xmin,
+             const double *xmax);
+  bool concheck(double f0xnew, const double *fxnew) const;  // ConCheck (:115-133)
+  // {raa0, raa[m], f0app, fapp[m]}
+  void state(double *out) const;
+
+ private:
+  void load(const double *xval, const double *df0dx, const double *dfdx, const double *xmin,
+            const double *xmax);
+  void subproblem(int asymp, double f0x, const double *fx);
+  void solve_approx(double *xmma);
+  hipStream_t s_ = nullptr;
+  MmamParams P_;
+  double a_, c_, d_, epsimin_;
+  double raa0_ = 1.0, r0_ = 0.0, f0app_ = 0.0;
+  GcmmaRaa raa_, r_, fapp_;
+  DevBuf<double> xv_, xm_, df_, dg_, part_, sums_;
+  DevBuf<double> low_, upp_, alpha_, beta_, p0_, q0_, pij_, qij_, xold1_, xold2_, xmin_, xmax_;
+  DevBuf<MmamState> st_;
+  PinnedBuf<MmamState> hst_;
+  PinnedBuf<double> hsum_;
   MmamArrays A_;
 };
 

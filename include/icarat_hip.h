@@ -281,6 +281,36 @@ int icarat_hip_mmasolver_update(icarat_hip_mmasolver *h, double *x, const double
                                 const double *xmax, int32_t *outer);
 int icarat_hip_mmasolver_asymptotes(icarat_hip_mmasolver *h, double *low, double *upp);
 
+/* GCMMASolver(n, m, a, c, d) (optimization/opt-solver/GCMMASolver.hpp:62-85,
+ * GCMMASolver.cpp) for any n and 1 <= m <= 8 constraints, on the device with its
+ * own stream; not tied to a context.  The subproblem, the approximation sums,
+ * raacof and the dual iteration (as icarat_hip_mmasolver) run on the device; raa0,
+ * raa, the approximation constants and ConCheck on the host.  set_params =
+ * SetMoveLimit / SetOscillation / SetAsymptotes; reset = Reset (the outer counter);
+ * outer = OuterUpdate(xmma, xval, f0x, df0dx, fx, dfdx, xmin, xmax); inner =
+ * InnerUpdate(xmma, f0xnew, fxnew, xval, f0x, df0dx, fx, dfdx, xmin, xmax) with xmma
+ * in/out; concheck = ConCheck(f0xnew, fxnew) into *ok; host arrays: xmma, xval,
+ * df0dx, xmin, xmax (n), fx, fxnew (m), dfdx (n m, dfdx[i m + j]).  state: {raa0,
+ * raa[m], f0app, fapp[m]} (2 + 2 m values). */
+typedef struct icarat_hip_gcmmasolver icarat_hip_gcmmasolver;
+int icarat_hip_gcmmasolver_create(int32_t n, int32_t m, double a, double c, double d,
+                                  icarat_hip_gcmmasolver **out);
+int icarat_hip_gcmmasolver_destroy(icarat_hip_gcmmasolver *h);
+const char *icarat_hip_gcmmasolver_last_error(const icarat_hip_gcmmasolver *h);
+int icarat_hip_gcmmasolver_set_params(icarat_hip_gcmmasolver *h, double move, double albefa,
+                                      double asyminit, double asymdec, double asyminc);
+int icarat_hip_gcmmasolver_reset(icarat_hip_gcmmasolver *h);
+int icarat_hip_gcmmasolver_outer(icarat_hip_gcmmasolver *h, double *xmma, const double *xval,
+                                 double f0x, const double *df0dx, const double *fx,
+                                 const double *dfdx, const double *xmin, const double *xmax);
+int icarat_hip_gcmmasolver_inner(icarat_hip_gcmmasolver *h, double *xmma, double f0xnew,
+                                 const double *fxnew, const double *xval, double f0x,
+                                 const double *df0dx, const double *fx, const double *dfdx,
+                                 const double *xmin, const double *xmax);
+int icarat_hip_gcmmasolver_concheck(icarat_hip_gcmmasolver *h, double f0xnew, const double *fxnew,
+                                    int32_t *ok);
+int icarat_hip_gcmmasolver_state(icarat_hip_gcmmasolver *h, double *out);
+
 /* GCMMASolver(n = nelx, m = 1, a = 0, c = 1000, d = 0) (optimization/opt-solver/
  * GCMMASolver.hpp:62-85) on the device, for the single volume constraint fx:
  * SetMoveLimit / SetOscillation / SetAsymptotes, and Reset (which here also

@@ -301,18 +301,24 @@ class MMASolver {
   double move_ = 0.15, albefa_ = 0.2, asyminit_ = 0.2, asymdec_ = 0.7, asyminc_ = 1.2;
 };
 
-// GCMMASolver(n, m = 1) on the device (optimization/opt-solver/GCMMASolver.hpp:62-85)
+// GCMMASolver(n, m, a, c, d) on the device: any n, 1 <= m <= 8 constraints
+// (optimization/opt-solver/GCMMASolver.hpp:62-85; icarat_hip_gcmmasolver_*, not
+// tied to a Device)
 class GCMMASolver {
  public:
-  GCMMASolver(Device &dev, int n, int m, double a = 0.0, double c = 1000.0, double d = 0.0)
-      : dev_(dev), n_(n) {
-    if (m != 1 || a != 0.0 || c != 1000.0 || d != 0.0 || n != dev.nelx()) {
-      std::cerr << "Error : GCMMASolver on the device supports n = nelx, m = 1, a = 0, c = 1000, "
-                   "d = 0"
+  GCMMASolver(int n, int m, double a = 0.0, double c = 1000.0, double d = 0.0) {
+    if (icarat_hip_gcmmasolver_create(n, m, a, c, d, &h_) != ICARAT_HIP_OK) {
+      std::cerr << "Error : GCMMASolver on the device supports n >= 1 and 1 <= m <= 8"
                 << std::endl;
       std::exit(1);
     }
-    check(icarat_hip_gcmma_reset(dev_.handle()), dev_.handle(), "GCMMASolver");
+  }
+  GCMMASolver(Device &, int n, int m, double a = 0.0, double c = 1000.0, double d = 0.0)
+      : GCMMASolver(n, m, a, c, d) {}
+  GCMMASolver(const GCMMASolver &) = delete;
+  GCMMASolver &operator=(const GCMMASolver &) = delete;
+  ~GCMMASolver() {
+    if (h_) icarat_hip_gcmmasolver_destroy(h_);
   }
   void SetAsymptotes(double init, double decrease, double increase) {
     asyminit_ = init;
@@ -331,41 +337,35 @@ class GCMMASolver {
   void OuterUpdate(double *xmma, const double *xval, double f0x, const double *df0dx,
                    const double *fx, const double *dfdx, const double *xmin,
                    const double *xmax) {
-    bounds(xmin, xmax);
-    check(icarat_hip_gcmma_outer(dev_.handle(), xmma, xval, f0x, df0dx, fx[0], dfdx, xmin[0],
-                                 xmax[0]),
-          dev_.handle(), "GCMMASolver::OuterUpdate");
+    done(icarat_hip_gcmmasolver_outer(h_, xmma, xval, f0x, df0dx, fx, dfdx, xmin, xmax),
+         "GCMMASolver::OuterUpdate");
   }
   void InnerUpdate(double *xmma, double f0xnew, const double *fxnew, const double *xval,
                    double f0x, const double *df0dx, const double *fx, const double *dfdx,
                    const double *xmin, const double *xmax) {
-    bounds(xmin, xmax);
-    check(icarat_hip_gcmma_inner(dev_.handle(), xmma, f0xnew, fxnew[0], xval, f0x, df0dx, fx[0],
-                                 dfdx, xmin[0], xmax[0]),
-          dev_.handle(), "GCMMASolver::InnerUpdate");
+    done(icarat_hip_gcmmasolver_inner(h_, xmma, f0xnew, fxnew, xval, f0x, df0dx, fx, dfdx, xmin,
+                                      xmax),
+         "GCMMASolver::InnerUpdate");
   }
   bool ConCheck(double f0xnew, const double *fxnew) {
     int32_t ok = 0;
-    check(icarat_hip_gcmma_concheck(dev_.handle(), f0xnew, fxnew[0], &ok), dev_.handle(),
-          "GCMMASolver::ConCheck");
+    done(icarat_hip_gcmmasolver_concheck(h_, f0xnew, fxnew, &ok), "GCMMASolver::ConCheck");
     return ok != 0;
   }
-  void Reset() { check(icarat_hip_gcmma_reset(dev_.handle()), dev_.handle(), "GCMMASolver::Reset"); }
+  void Reset() { done(icarat_hip_gcmmasolver_reset(h_), "GCMMASolver::Reset"); }
 
  private:
-  void bounds(const double *xmin, const double *xmax) {
-    if (!uniform(xmin, n_) || !uniform(xmax, n_)) {
-      std::cerr << "Error : GCMMASolver needs uniform bounds" << std::endl;
-      std::exit(1);
-    }
-  }
   void push() {
-    check(icarat_hip_gcmma_set_params(dev_.handle(), move_, albefa_, asyminit_, asymdec_,
-                                      asyminc_),
-          dev_.handle(), "GCMMASolver parameters");
+    done(icarat_hip_gcmmasolver_set_params(h_, move_, albefa_, asyminit_, asymdec_, asyminc_),
+         "GCMMASolver parameters");
   }
-  Device &dev_;
-  long long n_;
+  void done(int rc, const char *what) {
+    if (rc == ICARAT_HIP_OK) return;
+    std::cerr << "Error : " << what << ": " << icarat_hip_gcmmasolver_last_error(h_)
+              << " (status " << rc << ")" << std::endl;
+    std::exit(1);
+  }
+  icarat_hip_gcmmasolver *h_ = nullptr;
   double move_ = 0.15, albefa_ = 0.2, asyminit_ = 0.2, asymdec_ = 0.7, asyminc_ = 1.2;
 };
 

@@ -9,7 +9,14 @@ inputs and outputs only:
                 (dfdx, g, dgdx) and the design after the update
   mma_m3.npz    a 400-variable, three-constraint MMA run with per-variable
                 bounds and non-default parameters: per update the inputs
-                (dfdx, gx, dgdx) and the design and asymptotes after it
+                (dfdx, gx, dgdx) and the design after it
+  gcmma_toy.npz the reference's GCMMA toy problem (test_opt_solver.cpp:75-150,
+                its loop as written): every xmma returned, the inner count
+                per outer iteration, the final design
+  gcmma_m3.npz  a 300-variable, three-constraint GCMMA loop (convex
+                separable objective, weighted-volume constraints,
+                per-variable bounds): every xmma returned and the inner
+                count per outer iteration
   oc_filter.npz OC on an 8x6x5 mesh whose volume callback is the cone filter
                 (r = 1.5) summed, as the SIMP drivers pass it: inputs and the
                 updated design
@@ -96,6 +103,71 @@ def main():
     np.savez(os.path.join(HERE, "mma_m3.npz"), x0=x0, xmin=xmin, xmax=xmax, dfdx=np.array(dfs),
              gx=np.array(gs), dgdx=np.array(dgs), x=np.array(xs),
              params=np.array([0.3, 0.15, 0.3, 0.65, 1.1]))
+    # GCMMA toy (test_opt_solver.cpp:75-150 as written: objectivenew is
+    # evaluated before the outer update, on the zero vector)
+    def toy_fg(v):
+        return toy(v)[0], toy(v)[2]
+    g = O.GCMMA(3, 2, reference=True)
+    g.set(0.5, 0.1, 0.5, 0.7, 1.2)
+    s = np.array([4.0, 3.0, 2.0])
+    xs_all, inners = [], []
+    for _ in range(10):
+        f, df, gx, dg = toy(s)
+        snew = np.zeros(3)
+        fnew = float(np.sum(snew ** 2))
+        snew = g.outer(s, f, df, gx, dg, np.zeros(3), np.full(3, 5.0))
+        xs_all.append(snew.copy())
+        _, gnew = toy_fg(snew)
+        ok = g.concheck(fnew, gnew)
+        k = 0
+        while not ok and k < 15:
+            snew = g.inner(snew, fnew, gnew, s, f, df, gx, dg, np.zeros(3), np.full(3, 5.0))
+            xs_all.append(snew.copy())
+            fnew, gnew = toy_fg(snew)
+            ok = g.concheck(fnew, gnew)
+            k += 1
+        inners.append(k)
+        s = snew.copy()
+    np.savez(os.path.join(HERE, "gcmma_toy.npz"), x0=np.array([4.0, 3.0, 2.0]),
+             xmma=np.array(xs_all), inners=np.array(inners), x=s,
+             params=np.array([0.5, 0.1, 0.5, 0.7, 1.2]))
+    # GCMMA, n = 300, m = 3
+    rng = np.random.default_rng(9)
+    n, m = 300, 3
+    cw = rng.uniform(0.5, 2.0, n) / n
+    w = rng.uniform(0.5, 1.5, (m, n)) / n
+    cap = np.array([0.4, 0.45, 0.35])
+    xmin = rng.uniform(0.01, 0.05, n)
+    xmax = rng.uniform(0.95, 1.0, n)
+
+    def fg(v):
+        return float(np.sum(cw / v)), w @ v - cap
+
+    x = rng.uniform(0.3, 0.7, n)
+    x0 = x.copy()
+    g = O.GCMMA(n, m, reference=True)
+    g.set(0.2, 0.2, 0.3, 0.7, 1.2)
+    xs_all, inners = [], []
+    dg = np.ascontiguousarray(w.T).reshape(-1)
+    for _ in range(6):
+        f, gx = fg(x)
+        df = -cw / x ** 2
+        xm = g.outer(x, f, df, gx, dg, xmin, xmax)
+        xs_all.append(xm.copy())
+        fnew, gnew = fg(xm)
+        ok = g.concheck(fnew, gnew)
+        k = 0
+        while not ok and k < 5:
+            xm = g.inner(xm, fnew, gnew, x, f, df, gx, dg, xmin, xmax)
+            xs_all.append(xm.copy())
+            fnew, gnew = fg(xm)
+            ok = g.concheck(fnew, gnew)
+            k += 1
+        inners.append(k)
+        x = xm.copy()
+    np.savez(os.path.join(HERE, "gcmma_m3.npz"), x0=x0, cw=cw, w=w, cap=cap, xmin=xmin, xmax=xmax,
+             xmma=np.array(xs_all), inners=np.array(inners),
+             params=np.array([0.2, 0.2, 0.3, 0.7, 1.2]))
     # OC with the filtered-volume callback
     rng = np.random.default_rng(7)
     pr = O.Problem(8, 6, 5)

@@ -118,6 +118,10 @@ def test_errors_are_status_codes(capi):
         assert L.icarat_hip_mmasolver_create(n, m, 0.0, 1e5, 0.0, C.byref(h)) == capi.INVALID_ARGUMENT
     assert L.icarat_hip_mmasolver_update(None, None, None, None, None, None, None, None) == \
         capi.INVALID_ARGUMENT
+    for n, m in ((10, 9), (10, 0), (0, 2)):
+        assert L.icarat_hip_gcmmasolver_create(n, m, 0.0, 1e3, 0.0, C.byref(h)) == \
+            capi.INVALID_ARGUMENT
+    assert L.icarat_hip_gcmmasolver_state(None, None) == capi.INVALID_ARGUMENT
 
 
 def test_device_calls_without_gpu_fail_loudly(capi):
