@@ -542,81 +542,125 @@ __global__ void k_mg_dense_shift(Lattice L, const uint8_t *__restrict__ fmask, d
   if ((fmask[r % L.numnp] >> (r / L.numnp)) & 1u) A[r * n + r] *= 1.0 + shift;
 }
 
-// column-major: A(i, j) = A[i + j n]; the lower triangle (i > j) is valid
-__global__ void k_mg_dense_symmetrize(int n, double *__restrict__ A) {
-  const long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (q >= (long long)n * n) return;
-  const int i = (int)(q % n), j = (int)(q / n);
-  if (i < j) A[q] = A[j + (long long)i * n];
-}
-
-// x = A b: one block per row, threads over the columns, block reduction
 // x = A^-1 b on the coarsest lattice with the symmetric A^-1 (3 numnp square,
 // row and column j = c numnp + n for node n, component c) read once on and
-// below its diagonal: the lower triangle in DT x DT tiles, tile (I, J), I >= J,
-// giving the rows I the products A_IJ b_J and, for I > J, the rows J the
-// products A_IJ^T b_I (half the bytes of a full product).  Each tile stores its
+// below its diagonal, half the bytes of a full product.  The factorization
+// leaves A^-1 column-major in its lower triangle; k_mg_dense_pack copies that
+// triangle into DT x DT tiles (I, J), I >= J, stored one after another (tile
+// t = I (I + 1) / 2 + J) and column-major inside, so that one tile is one
+// contiguous 32 KB read.  Tile (I, J) gives the rows I the products A_IJ b_J
+// and, for I > J, the rows J the products A_IJ^T b_I.  Each tile stores its
 // two partial vectors in the slots (I, J) and (J, I) of a dense_tiles^2 x DT
 // partial array; k_mg_dense_sum adds the dense_tiles slots of every row in a
 // fixed order (deterministic, no atomics).
 constexpr int DT = 64;
+__host__ __device__ __forceinline__ long long dense_tiles(long long n) { return (n + DT - 1) / DT; }
+__device__ __forceinline__ void dense_tile_ij(int t, int &I, int &J) {
+  I = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((I + 1) * (I + 2) / 2 <= t) ++I;
+  while (I * (I + 1) / 2 > t) --I;
+  J = t - I * (I + 1) / 2;
+}
+
+// packed tile t, element (r, c) at t DT^2 + c DT + r = A(I DT + r, J DT + c),
+// taken from the lower triangle (the upper half of a diagonal tile from its
+// mirror), zero outside the n x n matrix
+__global__ void k_mg_dense_pack(int n, const double *__restrict__ A, double *__restrict__ P,
+                                long long total) {
+  const long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (q >= total) return;
+  int I, J;
+  dense_tile_ij((int)(q / (DT * DT)), I, J);
+  const int r = (int)(q % DT), c = (int)((q / DT) % DT);
+  const int i = I * DT + r, j = J * DT + c;
+  double v = 0.0;
+  if (i < n && j < n) v = i >= j ? A[i + (long long)j * n] : A[j + (long long)i * n];
+  P[q] = v;
+}
+
 __device__ __forceinline__ double dense_rhs(const Lattice &L, const double *__restrict__ b, int j,
                                             int n) {
   const int np1 = (int)L.numnp;
   return j < n ? b[(long long)(j / np1) * L.Np + j % np1] : 0.0;
 }
 
-__global__ __launch_bounds__(256) void k_mg_dense_symv(Lattice L, const double *__restrict__ A,
-                                                       const double *__restrict__ b,
-                                                       double *__restrict__ part, int nc,
-                                                       long long bs, long long ps) {
-  __shared__ double tile[DT][DT + 1];
-  __shared__ double sbi[DT], sbj[DT];
-  const int n = (int)(3 * L.numnp), nt = (n + DT - 1) / DT, ntiles = nt * (nt + 1) / 2;
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    // tile t -> (I, J), I >= J, t = I (I + 1) / 2 + J
-    int I = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+// One tile per block of DW waves, in registers: wave w, lane e holds the
+// tile's columns w + DW k (k < DT / DW) of row e.  Row sums (A_IJ b_J)
+// accumulate per lane over k and then over the waves through LDS; column sums
+// (A_IJ^T b_I) are the 64-lane sums of the products a_k b_I[e], reduced by a
+// butterfly that halves the values per lane at each exchange (32, 16, 8 lanes
+// apart) and adds the rest (4, 2, 1), leaving column k = e / 8 in lane e.  No
+// LDS holds the tile, so many blocks fit a CU and the reads of later tiles
+// overlap the arithmetic of earlier ones.
+constexpr int DW = 8, DK = DT / DW;
+__global__ __launch_bounds__(64 * DW, 8) void k_mg_dense_symv(Lattice L, const double *__restrict__ P,
+                                                           const double *__restrict__ b,
+                                                           double *__restrict__ part, int nc,
+                                                           long long bs, long long ps) {
+  __shared__ double sbi[DT], sbj[DT], red[DW][DT];
+  const int n = (int)(3 * L.numnp), nt = (int)dense_tiles(n), ntiles = nt * (nt + 1) / 2;
+  const int e = threadIdx.x & 63, w = threadIdx.x >> 6, t = blockIdx.x;
+  if (t >= ntiles) return;
+  {
+    // tile t -> (I, J) by a scalar search (t is uniform across the block)
+    int I = 0;
     while ((I + 1) * (I + 2) / 2 <= t) ++I;
-    while (I * (I + 1) / 2 > t) --I;
     const int J = t - I * (I + 1) / 2;
-    // the tile through LDS once: rows of 64 contiguous doubles per wave
-    {
-      const int c = threadIdx.x & 63, w = threadIdx.x >> 6, col = J * DT + c;
-      double a[DT / 4];
+    const double *__restrict__ tp = P + (long long)t * DT * DT;
+    double a[DK];
 #pragma unroll
-      for (int k = 0; k < DT / 4; ++k) {
-        const int row = I * DT + w + 4 * k;
-        a[k] = (row < n && col < n) ? A[(long long)row * n + col] : 0.0;
-      }
-#pragma unroll
-      for (int k = 0; k < DT / 4; ++k) tile[w + 4 * k][c] = a[k];
-    }
-    // then every right-hand side (load case lc at lc x bs, its partials at lc x ps)
+    for (int k = 0; k < DK; ++k) a[k] = tp[(w + DW * k) * DT + e];
+    // every right-hand side (load case lc at lc x bs, its partials at lc x ps)
     for (int lc = 0; lc < nc; ++lc) {
-      if (threadIdx.x < DT) {
-        sbi[threadIdx.x] = dense_rhs(L, b + lc * bs, I * DT + threadIdx.x, n);
-        sbj[threadIdx.x] = dense_rhs(L, b + lc * bs, J * DT + threadIdx.x, n);
-      }
+      if (threadIdx.x < DT) sbi[threadIdx.x] = dense_rhs(L, b + lc * bs, I * DT + threadIdx.x, n);
+      else if (threadIdx.x < 2 * DT)
+        sbj[threadIdx.x - DT] = dense_rhs(L, b + lc * bs, J * DT + threadIdx.x - DT, n);
       __syncthreads();
-      // four threads per row (row sums) and per column (column sums)
-      const int e = threadIdx.x >> 2, q = threadIdx.x & 3;
-      double u = 0.0, v = 0.0;
+      double u = 0.0, v[DK];
+      const double bi = sbi[e];
 #pragma unroll
-      for (int k = 0; k < DT / 4; ++k) {
-        const int o = q * (DT / 4) + k;
-        u += tile[e][o] * sbj[o];
-        v += tile[o][e] * sbi[o];
+      for (int k = 0; k < DK; ++k) {
+        u += a[k] * sbj[w + DW * k];
+        v[k] = a[k] * bi;
       }
-      u += __shfl_xor(u, 1);
-      u += __shfl_xor(u, 2);
-      v += __shfl_xor(v, 1);
-      v += __shfl_xor(v, 2);
-      if (q == 0) {
-        double *pc = part + lc * ps;
-        pc[((long long)I * nt + J) * DT + e] = u;
-        if (I > J) pc[((long long)J * nt + I) * DT + e] = v;
+      red[w][e] = u;
+      // butterfly: after the exchange at distance d the lane keeps the half of
+      // its values selected by its own bit d
+      // (the halves are read into scalars first: a select between two
+      // elements of one array becomes an array index, compiled to a compare
+      // and select chain over the whole array)
+#pragma unroll
+      for (int st = 0; st < 3; ++st) {
+        const int h = DK >> (st + 1), d = 32 >> st;
+        const bool up = (e & d) != 0;
+        double lo[DK / 2], hi[DK / 2];
+#pragma unroll
+        for (int k = 0; k < h; ++k) {
+          lo[k] = v[k];
+          hi[k] = v[k + h];
+        }
+#pragma unroll
+        for (int k = 0; k < h; ++k) {
+          const double send = up ? lo[k] : hi[k];
+          const double keep = up ? hi[k] : lo[k];
+          v[k] = keep + __shfl_xor(send, d);
+        }
       }
-      __syncthreads();  // the b slices are refilled for the next case or tile
+      v[0] += __shfl_xor(v[0], 4);
+      v[0] += __shfl_xor(v[0], 2);
+      v[0] += __shfl_xor(v[0], 1);
+      __syncthreads();
+      double *pc = part + lc * ps;
+      if (w == 0) {
+        double r = 0.0;
+#pragma unroll
+        for (int q = 0; q < DW; ++q) r += red[q][e];
+        pc[((long long)I * nt + J) * DT + e] = r;
+      }
+      // lane e holds column k = e / 8 (bits 5, 4, 3 of e chose the halves
+      // kept at distances 32, 16, 8: k = 4 b5 + 2 b4 + b3)
+      if (I > J && (e & 7) == 0) pc[((long long)J * nt + I) * DT + w + DW * (e >> 3)] = v[0];
+      __syncthreads();  // sbi, sbj and red are refilled for the next case or tile
     }
   }
 }
@@ -734,14 +778,20 @@ void launch_mg_dense_shift(int n, const Lattice &L, const uint8_t *fmask, double
   HIPCHK(hipGetLastError());
 }
 
-void launch_mg_dense_symmetrize(int n, double *A, hipStream_t s) {
-  hipLaunchKernelGGL(k_mg_dense_symmetrize, dim3(blocks_for((long long)n * n, 256)), dim3(256), 0, s,
-                     n, A);
+long long mg_dense_store_size(long long n) {
+  const long long nt = dense_tiles(n);
+  return n * n + nt * (nt + 1) / 2 * DT * DT;
+}
+
+void launch_mg_dense_pack(int n, double *A, hipStream_t s) {
+  const long long nt = dense_tiles(n), total = nt * (nt + 1) / 2 * DT * DT;
+  hipLaunchKernelGGL(k_mg_dense_pack, dim3(blocks_for(total, 256)), dim3(256), 0, s, n, A,
+                     A + (long long)n * n, total);
   HIPCHK(hipGetLastError());
 }
 
 long long mg_dense_part_size(const Lattice &L) {
-  const long long nt = (3 * L.numnp + DT - 1) / DT;
+  const long long nt = dense_tiles(3 * L.numnp);
   return nt * nt * DT;
 }
 
@@ -749,12 +799,11 @@ void launch_mg_dense_solve(const Lattice &L, const double *ainv, const double *b
                            double *part, hipStream_t s, int nc, long long cs) {
   if (3 * L.numnp > MG_DENSE_MAX) throw Error(kInvalidArgument, "lattice too large for the dense solve");
   if (nc < 1) throw Error(kInternal, "dense solve: no load case");
-  const long long n = 3 * L.numnp, nt = (n + DT - 1) / DT, ps = mg_dense_part_size(L);
-  // one round of resident blocks (4 per CU by LDS), tiles in a grid-stride loop;
-  // A^-1 read once for every load case
+  const long long n = 3 * L.numnp, nt = dense_tiles(n), ps = mg_dense_part_size(L);
+  // one block per packed tile (A^-1 read once for every load case)
   const long long ntiles = nt * (nt + 1) / 2;
-  hipLaunchKernelGGL(k_mg_dense_symv, dim3((unsigned)std::min<long long>(ntiles, 1024)), dim3(256), 0,
-                     s, L, ainv, b, part, nc, cs, ps);
+  hipLaunchKernelGGL(k_mg_dense_symv, dim3((unsigned)ntiles), dim3(64 * DW), 0, s, L, ainv + n * n, b,
+                     part, nc, cs, ps);
   hipLaunchKernelGGL(k_mg_dense_sum, dim3((unsigned)nt, (unsigned)nc), dim3(256), 0, s, L, part, x,
                      ps, cs);
   HIPCHK(hipGetLastError());

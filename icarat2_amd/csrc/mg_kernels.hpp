@@ -81,10 +81,13 @@ void launch_mg_dense_assemble(const Lattice &L, const double *ke, const double *
 // the diagonal of the free DOFs of the assembled dense A (n = 3 numnp) times (1 + shift)
 void launch_mg_dense_shift(int n, const Lattice &L, const uint8_t *fmask, double shift, double *A,
                            hipStream_t s);
-// copies the lower triangle (column-major) over the upper one
-void launch_mg_dense_symmetrize(int n, double *A, hipStream_t s);
-// x = A b on the lattice (A symmetric, full; read on and below its diagonal),
-// with a workspace part of mg_dense_part_size(L) doubles per load case.
+// A dense store holds the n x n matrix (column-major) and after it the packed
+// tiles of its lower triangle: mg_dense_store_size(n) doubles.
+long long mg_dense_store_size(long long n);
+// packs the lower triangle of the store's matrix into its tiles
+void launch_mg_dense_pack(int n, double *A, hipStream_t s);
+// x = A b on the lattice (A symmetric: the store, read through its packed
+// tiles), with a workspace part of mg_dense_part_size(L) doubles per load case.
 // Launchers with nc > 1 run nc load cases in one launch: case c's vectors
 // start at c x the level's stride (sf fine, sc coarse, cs both).
 void launch_mg_dense_solve(const Lattice &L, const double *A, const double *b, double *x,

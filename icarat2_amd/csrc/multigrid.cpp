@@ -113,8 +113,8 @@ Multigrid::Multigrid(Context &fine, const icarat_mg_opts &o) : f_(fine), o_(o) {
     dense_ = nd <= dense_max ||
              (blocked && nd <= std::min<long long>(MG_DENSE_MAX, dense_max + dense_max / 2));
     if (dense_) {
-      ainv_.alloc(nd * nd);
-      anext_.alloc(nd * nd);
+      ainv_.alloc(mg_dense_store_size(nd));
+      anext_.alloc(mg_dense_store_size(nd));
       dpart_.alloc(mg_dense_part_size(level(levels() - 1).lattice()));
       HIPCHK(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
       HIPCHK(hipEventCreateWithFlags(&ev_side_, hipEventDisableTiming));
@@ -250,7 +250,7 @@ void Multigrid::factor_into(double *dst, int side, double shift) {
   chk(rocsolver_dpotrf(hb, rocblas_fill_lower, nd, dst, nd, info), "rocsolver_dpotrf");
   HIPCHK(hipMemcpyAsync(h_info_[side], info, sizeof(int), hipMemcpyDeviceToHost, s));
   chk(rocsolver_dpotri(hb, rocblas_fill_lower, nd, dst, nd, info), "rocsolver_dpotri");
-  launch_mg_dense_symmetrize(nd, dst, s);
+  launch_mg_dense_pack(nd, dst, s);
   ++n_fact_;
 }
 
