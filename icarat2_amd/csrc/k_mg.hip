@@ -584,84 +584,62 @@ __device__ __forceinline__ double dense_rhs(const Lattice &L, const double *__re
   return j < n ? b[(long long)(j / np1) * L.Np + j % np1] : 0.0;
 }
 
-// One tile per block of DW waves, in registers: wave w, lane e holds the
-// tile's columns w + DW k (k < DT / DW) of row e.  Row sums (A_IJ b_J)
-// accumulate per lane over k and then over the waves through LDS; column sums
-// (A_IJ^T b_I) are the 64-lane sums of the products a_k b_I[e], reduced by a
-// butterfly that halves the values per lane at each exchange (32, 16, 8 lanes
-// apart) and adds the rest (4, 2, 1), leaving column k = e / 8 in lane e.  No
-// LDS holds the tile, so many blocks fit a CU and the reads of later tiles
-// overlap the arithmetic of earlier ones.
-constexpr int DW = 8, DK = DT / DW;
-__global__ __launch_bounds__(64 * DW, 8) void k_mg_dense_symv(Lattice L, const double *__restrict__ P,
-                                                           const double *__restrict__ b,
-                                                           double *__restrict__ part, int nc,
-                                                           long long bs, long long ps) {
-  __shared__ double sbi[DT], sbj[DT], red[DW][DT];
+// One packed tile per block, staged through LDS: each wave reads whole 512-byte
+// tile columns (the tile is one contiguous 32 KB read) and the block then forms
+// the row sums A_IJ b_J and, for I > J, the column sums A_IJ^T b_I with four
+// threads per row (per column), each adding 16 consecutive products, the four
+// partials added in a fixed tree.  One tile per block (rather than a
+// grid-stride loop over tiles) lets the dispatcher start a new tile's reads as
+// soon as any block on a CU finishes.
+__global__ __launch_bounds__(256) void k_mg_dense_symv(Lattice L, const double *__restrict__ P,
+                                                       const double *__restrict__ b,
+                                                       double *__restrict__ part, int nc,
+                                                       long long bs, long long ps) {
+  __shared__ double tile[DT][DT + 1];
+  __shared__ double sbi[DT], sbj[DT];
   const int n = (int)(3 * L.numnp), nt = (int)dense_tiles(n), ntiles = nt * (nt + 1) / 2;
-  const int e = threadIdx.x & 63, w = threadIdx.x >> 6, t = blockIdx.x;
+  const int t = blockIdx.x;
   if (t >= ntiles) return;
+  // tile t -> (I, J) by a scalar search (t is uniform across the block)
+  int I = 0;
+  while ((I + 1) * (I + 2) / 2 <= t) ++I;
+  const int J = t - I * (I + 1) / 2;
   {
-    // tile t -> (I, J) by a scalar search (t is uniform across the block)
-    int I = 0;
-    while ((I + 1) * (I + 2) / 2 <= t) ++I;
-    const int J = t - I * (I + 1) / 2;
+    // packed element (r, c) at c DT + r: wave w reads columns w + 4 k
     const double *__restrict__ tp = P + (long long)t * DT * DT;
-    double a[DK];
+    const int r = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double a[DT / 4];
 #pragma unroll
-    for (int k = 0; k < DK; ++k) a[k] = tp[(w + DW * k) * DT + e];
-    // every right-hand side (load case lc at lc x bs, its partials at lc x ps)
-    for (int lc = 0; lc < nc; ++lc) {
-      if (threadIdx.x < DT) sbi[threadIdx.x] = dense_rhs(L, b + lc * bs, I * DT + threadIdx.x, n);
-      else if (threadIdx.x < 2 * DT)
-        sbj[threadIdx.x - DT] = dense_rhs(L, b + lc * bs, J * DT + threadIdx.x - DT, n);
-      __syncthreads();
-      double u = 0.0, v[DK];
-      const double bi = sbi[e];
+    for (int k = 0; k < DT / 4; ++k) a[k] = tp[(w + 4 * k) * DT + r];
 #pragma unroll
-      for (int k = 0; k < DK; ++k) {
-        u += a[k] * sbj[w + DW * k];
-        v[k] = a[k] * bi;
-      }
-      red[w][e] = u;
-      // butterfly: after the exchange at distance d the lane keeps the half of
-      // its values selected by its own bit d
-      // (the halves are read into scalars first: a select between two
-      // elements of one array becomes an array index, compiled to a compare
-      // and select chain over the whole array)
-#pragma unroll
-      for (int st = 0; st < 3; ++st) {
-        const int h = DK >> (st + 1), d = 32 >> st;
-        const bool up = (e & d) != 0;
-        double lo[DK / 2], hi[DK / 2];
-#pragma unroll
-        for (int k = 0; k < h; ++k) {
-          lo[k] = v[k];
-          hi[k] = v[k + h];
-        }
-#pragma unroll
-        for (int k = 0; k < h; ++k) {
-          const double send = up ? lo[k] : hi[k];
-          const double keep = up ? hi[k] : lo[k];
-          v[k] = keep + __shfl_xor(send, d);
-        }
-      }
-      v[0] += __shfl_xor(v[0], 4);
-      v[0] += __shfl_xor(v[0], 2);
-      v[0] += __shfl_xor(v[0], 1);
-      __syncthreads();
-      double *pc = part + lc * ps;
-      if (w == 0) {
-        double r = 0.0;
-#pragma unroll
-        for (int q = 0; q < DW; ++q) r += red[q][e];
-        pc[((long long)I * nt + J) * DT + e] = r;
-      }
-      // lane e holds column k = e / 8 (bits 5, 4, 3 of e chose the halves
-      // kept at distances 32, 16, 8: k = 4 b5 + 2 b4 + b3)
-      if (I > J && (e & 7) == 0) pc[((long long)J * nt + I) * DT + w + DW * (e >> 3)] = v[0];
-      __syncthreads();  // sbi, sbj and red are refilled for the next case or tile
+    for (int k = 0; k < DT / 4; ++k) tile[r][w + 4 * k] = a[k];
+  }
+  // then every right-hand side (load case lc at lc x bs, its partials at lc x ps)
+  for (int lc = 0; lc < nc; ++lc) {
+    if (threadIdx.x < DT) {
+      sbi[threadIdx.x] = dense_rhs(L, b + lc * bs, I * DT + threadIdx.x, n);
+      sbj[threadIdx.x] = dense_rhs(L, b + lc * bs, J * DT + threadIdx.x, n);
     }
+    __syncthreads();
+    // four threads per row (row sums) and per column (column sums)
+    const int e = threadIdx.x >> 2, q = threadIdx.x & 3;
+    double u = 0.0, v = 0.0;
+#pragma unroll
+    for (int k = 0; k < DT / 4; ++k) {
+      const int o = q * (DT / 4) + k;
+      u += tile[e][o] * sbj[o];
+      v += tile[o][e] * sbi[o];
+    }
+    u += __shfl_xor(u, 1);
+    u += __shfl_xor(u, 2);
+    v += __shfl_xor(v, 1);
+    v += __shfl_xor(v, 2);
+    if (q == 0) {
+      double *pc = part + lc * ps;
+      pc[((long long)I * nt + J) * DT + e] = u;
+      if (I > J) pc[((long long)J * nt + I) * DT + e] = v;
+    }
+    __syncthreads();  // the b slices are refilled for the next case
   }
 }
 
@@ -802,7 +780,7 @@ void launch_mg_dense_solve(const Lattice &L, const double *ainv, const double *b
   const long long n = 3 * L.numnp, nt = dense_tiles(n), ps = mg_dense_part_size(L);
   // one block per packed tile (A^-1 read once for every load case)
   const long long ntiles = nt * (nt + 1) / 2;
-  hipLaunchKernelGGL(k_mg_dense_symv, dim3((unsigned)ntiles), dim3(64 * DW), 0, s, L, ainv + n * n, b,
+  hipLaunchKernelGGL(k_mg_dense_symv, dim3((unsigned)ntiles), dim3(256), 0, s, L, ainv + n * n, b,
                      part, nc, cs, ps);
   hipLaunchKernelGGL(k_mg_dense_sum, dim3((unsigned)nt, (unsigned)nc), dim3(256), 0, s, L, part, x,
                      ps, cs);
