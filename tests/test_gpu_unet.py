@@ -1,8 +1,12 @@
 """The learned preconditioner's network on the GPU (BF16 MFMA convolutions,
 FP32 accumulation, BF16 activations) against the FP64 numpy network
 (oracle/unet.py, pinned by finite differences).  Tolerances, normwise:
-  forward output ............... 3e-2 of ||out|| (BF16 inputs, weights, activations)
-  parameter gradient per tensor  1e-1 of ||g||, cosine >= 0.99
+  forward output ............... 1e-2 of ||out|| (BF16 inputs, weights, activations;
+                                 measured 2.8e-3)
+  parameter gradient per tensor  7e-2 of ||g||, cosine >= 0.998 (measured at most
+                                 4.8e-2 and 0.9989: the backward pass rounds the
+                                 stored activations and each layer's output
+                                 gradient to BF16, about 2e-3 per rounding)
   first Adam step .............. >= 95 % of the gradient mass moves with the oracle's sign
 """
 import numpy as np
@@ -61,7 +65,7 @@ def test_forward_matches_oracle(gpu, oracle):
     ref, _ = U.forward(P, x8)
     e = _nrel(out, ref)
     print(f"forward normwise error {e:.2e}")
-    assert e < 3e-2
+    assert e < 1e-2
 
 
 def test_gradients_match_oracle(gpu, oracle):
@@ -78,7 +82,7 @@ def test_gradients_match_oracle(gpu, oracle):
             e = _nrel(Gg[k], G[k])
             cos = float(np.sum(Gg[k] * G[k]) / (np.linalg.norm(Gg[k]) * np.linalg.norm(G[k])))
             print(f"{k}: normwise {e:.2e} cosine {cos:.5f}")
-            assert e < 1e-1 and cos > 0.99, k
+            assert e < 7e-2 and cos > 0.998, k
 
 
 def test_adam_step_matches_oracle(gpu, oracle):
