@@ -360,8 +360,8 @@ def precision_leg(A, cfg, method, tol, maxit, ref):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default="c3", choices=sorted(CONFIGS))
     ap.add_argument("--method", default="mg", choices=sorted(METHODS))
     ap.add_argument("--no-plain", action="store_true",
