@@ -584,14 +584,16 @@ __device__ __forceinline__ double dense_rhs(const Lattice &L, const double *__re
   return j < n ? b[(long long)(j / np1) * L.Np + j % np1] : 0.0;
 }
 
-// One packed tile per block, staged through LDS: each wave reads whole 512-byte
-// tile columns (the tile is one contiguous 32 KB read) and the block then forms
-// the row sums A_IJ b_J and, for I > J, the column sums A_IJ^T b_I with four
-// threads per row (per column), each adding 16 consecutive products, the four
-// partials added in a fixed tree.  One tile per block (rather than a
-// grid-stride loop over tiles) lets the dispatcher start a new tile's reads as
-// soon as any block on a CU finishes.
-__global__ __launch_bounds__(256) void k_mg_dense_symv(Lattice L, const double *__restrict__ P,
+// One packed tile per block of 512 threads, staged through LDS: each wave
+// reads whole 512-byte tile columns (the tile is one contiguous 32 KB read).
+// The first 256 threads then form the row sums A_IJ b_J and, for I > J, the
+// other 256 the column sums A_IJ^T b_I, four threads per row (per column),
+// each adding 16 consecutive products, the four partials added in a fixed
+// tree.  One tile per block (rather than a grid-stride loop over tiles) lets
+// the dispatcher start a new tile's reads as soon as any block on a CU
+// finishes; the tile's LDS limits a CU to 4 blocks, and 512 threads keep twice
+// the reads in flight that 256 would.
+__global__ __launch_bounds__(512) void k_mg_dense_symv(Lattice L, const double *__restrict__ P,
                                                        const double *__restrict__ b,
                                                        double *__restrict__ part, int nc,
                                                        long long bs, long long ps) {
@@ -605,15 +607,17 @@ __global__ __launch_bounds__(256) void k_mg_dense_symv(Lattice L, const double *
   while ((I + 1) * (I + 2) / 2 <= t) ++I;
   const int J = t - I * (I + 1) / 2;
   {
-    // packed element (r, c) at c DT + r: wave w reads columns w + 4 k
+    // packed element (r, c) at c DT + r: wave w reads columns w + 8 k
     const double *__restrict__ tp = P + (long long)t * DT * DT;
     const int r = threadIdx.x & 63, w = threadIdx.x >> 6;
-    double a[DT / 4];
+    double a[DT / 8];
 #pragma unroll
-    for (int k = 0; k < DT / 4; ++k) a[k] = tp[(w + 4 * k) * DT + r];
+    for (int k = 0; k < DT / 8; ++k) a[k] = tp[(w + 8 * k) * DT + r];
 #pragma unroll
-    for (int k = 0; k < DT / 4; ++k) tile[r][w + 4 * k] = a[k];
+    for (int k = 0; k < DT / 8; ++k) tile[r][w + 8 * k] = a[k];
   }
+  const bool cols = threadIdx.x >= 256;  // column sums (uniform per wave)
+  const int e = (threadIdx.x & 255) >> 2, q = threadIdx.x & 3;
   // then every right-hand side (load case lc at lc x bs, its partials at lc x ps)
   for (int lc = 0; lc < nc; ++lc) {
     if (threadIdx.x < DT) {
@@ -621,23 +625,26 @@ __global__ __launch_bounds__(256) void k_mg_dense_symv(Lattice L, const double *
       sbj[threadIdx.x] = dense_rhs(L, b + lc * bs, J * DT + threadIdx.x, n);
     }
     __syncthreads();
-    // four threads per row (row sums) and per column (column sums)
-    const int e = threadIdx.x >> 2, q = threadIdx.x & 3;
-    double u = 0.0, v = 0.0;
+    double u = 0.0;
+    if (!cols) {
 #pragma unroll
-    for (int k = 0; k < DT / 4; ++k) {
-      const int o = q * (DT / 4) + k;
-      u += tile[e][o] * sbj[o];
-      v += tile[o][e] * sbi[o];
+      for (int k = 0; k < DT / 4; ++k) {
+        const int o = q * (DT / 4) + k;
+        u += tile[e][o] * sbj[o];
+      }
+    } else if (I > J) {
+#pragma unroll
+      for (int k = 0; k < DT / 4; ++k) {
+        const int o = q * (DT / 4) + k;
+        u += tile[o][e] * sbi[o];
+      }
     }
     u += __shfl_xor(u, 1);
     u += __shfl_xor(u, 2);
-    v += __shfl_xor(v, 1);
-    v += __shfl_xor(v, 2);
     if (q == 0) {
       double *pc = part + lc * ps;
-      pc[((long long)I * nt + J) * DT + e] = u;
-      if (I > J) pc[((long long)J * nt + I) * DT + e] = v;
+      if (!cols) pc[((long long)I * nt + J) * DT + e] = u;
+      else if (I > J) pc[((long long)J * nt + I) * DT + e] = u;
     }
     __syncthreads();  // the b slices are refilled for the next case
   }
@@ -780,7 +787,7 @@ void launch_mg_dense_solve(const Lattice &L, const double *ainv, const double *b
   const long long n = 3 * L.numnp, nt = dense_tiles(n), ps = mg_dense_part_size(L);
   // one block per packed tile (A^-1 read once for every load case)
   const long long ntiles = nt * (nt + 1) / 2;
-  hipLaunchKernelGGL(k_mg_dense_symv, dim3((unsigned)ntiles), dim3(256), 0, s, L, ainv + n * n, b,
+  hipLaunchKernelGGL(k_mg_dense_symv, dim3((unsigned)ntiles), dim3(512), 0, s, L, ainv + n * n, b,
                      part, nc, cs, ps);
   hipLaunchKernelGGL(k_mg_dense_sum, dim3((unsigned)nt, (unsigned)nc), dim3(256), 0, s, L, part, x,
                      ps, cs);
